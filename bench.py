@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Throughput of the electron kinetic step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
+
+A step is one kinetic update of the whole particle population: the fused
+collision + push + deposit + compaction kernel, the deposit reduction and
+the device field step (moment update, fluid ions, Poisson).  The state is
+synthetic (Maxwellian, uniform density) and resident in HBM before timing.
+Prints one JSON line.  The roofline object is for the dominant kernel
+(k_push), timed with HIP events on its own stream inside the library; the
+CPU baseline is the oracle port timed on a bounded sample on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(cfg, seconds: float = 10.0):
+    """Oracle (FP64-capable C++ port of the reference step, OpenMP) on a
+    bounded sample of the same workload: push + collisions + deposit +
+    compaction of up to 1M particles, repeated for about `seconds`."""
+    import numpy as np
+
+    from boltzmann_amd import physics as ph
+    from oracle import oracle as O
+
+    n = min(cfg.n_particles, 1_000_000)
+    sub = cfg.with_(n_particles=n)
+    st = ph.initial_state(sub)
+    table = ph.build_table(cfg.xsec, cfg.gas_density_m3, cfg.np_dtype) \
+        if cfg.xs_mode == 1 else None
+    cores = max(1, min(os.cpu_count() or 1, 16))
+    setup = O.Setup(dim=cfg.dim, dt=cfg.dt, xs_mode=cfg.xs_mode,
+                    n_cells=cfg.n_cells, length_cm=cfg.length_cm, bnf=cfg.bnf,
+                    gas_density_m3=cfg.gas_density_m3,
+                    nu_max=table.nu_max if table is not None else 0.0,
+                    e_field0=cfg.e_field0_vcm() if cfg.dim == 0 else 0.0, nthreads=cores)
+    tab = None if table is None else (table.energy, table.sig, table.slope)
+    E = np.zeros(cfg.n_cells, dtype=cfg.np_dtype) if cfg.dim == 1 else None
+    # uniforms generated outside the timed region (the GPU draws them in-kernel)
+    R = O.collision_uniforms(cfg.seed, 0, n, cfg.np_dtype)
+    done = 0
+    psteps = 0
+    t0 = time.perf_counter()
+    cur = st
+    while True:
+        cur, _, _ = O.push_step(setup, cur, E, R[:, :cur[0].size].copy() if cur[0].size < n
+                                else R, table=tab, capacity=2 * n)
+        psteps += cur[0].size
+        done += 1
+        if time.perf_counter() - t0 > seconds or done >= 1000:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": psteps / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} particles x {done} steps of the oracle push+MCC+deposit+compaction "
+                      f"({'FP64' if cfg.precision == 8 else 'FP32'}, OpenMP {cores} threads)"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--shape", type=int, default=-1, help="push tile shape (-1: autotune)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the particle "
+              "decomposition across GPUs is a design note, not a measured path.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from boltzmann_amd import _lib, configs
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+
+    cfg = configs.get(args.workload)
+    eng = KineticEngine(cfg)
+    if args.shape >= 0:
+        _lib.check(eng.lib.bz_set_push_shape(eng.ctx, args.shape))
+    eng.load_state(ph.initial_state(cfg))
+    eng.initialize_fields()
+    for _ in range(args.warmup):
+        eng.step()
+    torch.cuda.synchronize()
+    c0 = eng.counters()
+    _lib.check(eng.lib.bz_timing_enable(eng.ctx, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    import ctypes as C
+    ms = (C.c_double * 3)()
+    nl = (C.c_int64 * 3)()
+    _lib.check(eng.lib.bz_timing_get(eng.ctx, ms, nl))
+    c1 = eng.counters()
+    psteps = c1["particle_steps"] - c0["particle_steps"]
+    elapsed = t1 - t0
+    value = psteps / elapsed
+    word = 8 if cfg.precision == _lib.BZ_F64 else 4
+    bytes_per_pstep = 2 * 5 * word  # read + write x, vx, vy, vz, w
+    push_launches = max(int(nl[0]), 1)
+    avg_push_s = ms[0] / 1e3 / push_launches
+    per_launch = psteps / args.steps
+    achieved = bytes_per_pstep * per_launch / avg_push_s / 1e9
+    n_end = eng.count()
+    out = {
+        "metric": "particle-steps/s on 1 MI355X",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64" if cfg.precision == _lib.BZ_F64 else "fp32",
+        "data": "synthetic (Maxwellian electrons, uniform density, tabulated argon-like sigma)",
+        "config": {"workload": cfg.name, "particles_start": cfg.n_particles,
+                   "particles_end": n_end, "cells": cfg.n_cells if cfg.dim == 1 else 0,
+                   "dt_s": cfg.dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_push", "kernel_avg_us": avg_push_s * 1e6,
+                     "bytes_per_particle_step": bytes_per_pstep},
+        "counters": {k: c1[k] - c0[k] for k in ("births", "absorbed", "ionization", "overflow")},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,64 @@
+"""The five workloads of the electron kinetic step.
+
+c0  0-D argon-like swarm, 1 bin, E/N = 100 Td, N_gas = 3.2e22 m^-3, 100k
+    electrons, FP64, weight 1, Maxwellian 1 eV, dt = 1e-12 s, 5000 steps.
+c1  1-D bounded discharge, 3 bins, gap 2.5 cm, 256 cells, 1M electrons,
+    FP32 push / FP64 sums, 200 V at 13.56 MHz, fluid ions, uniform
+    1e15 m^-3 at 2 eV, resampling when a segment reaches 2x its target
+    (4000 per cell), 20 RF periods x 400 steps.
+c2  256 batched 0-D swarms, E/N log-spaced 1..1000 Td, counts log-uniform
+    in [1e4, 5e5].
+c3  learned closure on the c1 discharge with 1024 cells and 4M electrons.
+c4  1-D discharge, 5 bins, 4096 cells, 32M electrons, FP32, 300 V,
+    resampling every step towards 8000 per cell, 10 periods x 800 steps.
+
+Gas density of the 1-D discharges is the reference's 0.1 Torr argon,
+3.22e15 cm^-3 (boltzmann.py:47-49); the 0-D swarms use 3.2e22 m^-3.
+"""
+from __future__ import annotations
+
+from . import _lib
+from .physics import RunConfig, bin_fractions, bin_thresholds
+
+RF = 13.56e6
+
+
+def c0() -> RunConfig:
+    return RunConfig(name="c0_0d_swarm_100Td_100k_fp64", dim=0, n_particles=100_000,
+                     precision=_lib.BZ_F64, nb=1, dt=1e-12, steps=5000,
+                     gas_density_m3=3.2e22, e_over_n_td=100.0, te_ev=1.0,
+                     bin_fractions=bin_fractions(1), bin_thresholds=bin_thresholds(1),
+                     fluid_on=False, poisson_on=False, capacity_factor=1.5)
+
+
+def c1() -> RunConfig:
+    return RunConfig(name="c1_1d_3bin_256cells_1M", dim=1, n_particles=1_000_000,
+                     precision=_lib.BZ_F32, nb=3, n_cells=256, length_cm=2.5,
+                     dt=1.0 / (RF * 400), steps=20 * 400, gas_density_m3=3.22e21,
+                     te_ev=2.0, density_cm3=1e9, rf_voltage=200.0, rf_freq=RF,
+                     target_ppc=4000, resample_every=1, trigger_at=2.0, merge_at=2.0,
+                     bin_fractions=bin_fractions(3), bin_thresholds=bin_thresholds(3))
+
+
+def c3() -> RunConfig:
+    return c1().with_(name="c3_1d_closure_1024cells_4M", n_particles=4_000_000, n_cells=1024)
+
+
+def c4() -> RunConfig:
+    return RunConfig(name="c4_1d_5bin_4096cells_32M", dim=1, n_particles=32_000_000,
+                     precision=_lib.BZ_F32, nb=5, n_cells=4096, length_cm=2.5,
+                     dt=1.0 / (RF * 800), steps=10 * 800, gas_density_m3=3.22e21,
+                     te_ev=2.0, density_cm3=1e9, rf_voltage=300.0, rf_freq=RF,
+                     target_ppc=8000, resample_every=1, trigger_at=1.0, merge_at=1.5,
+                     bin_fractions=bin_fractions(5), bin_thresholds=bin_thresholds(5),
+                     capacity_factor=1.5)
+
+
+WORKLOADS = {"c0": c0, "c1": c1, "c3": c3, "c4": c4}
+
+
+def get(name: str) -> RunConfig:
+    try:
+        return WORKLOADS[name]()
+    except KeyError:
+        raise KeyError(f"unknown workload {name!r}; choose from {sorted(WORKLOADS)}") from None

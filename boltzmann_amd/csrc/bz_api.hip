@@ -1,0 +1,358 @@
+// C ABI of libbz.so: context, workspace, clock and the stage entry points.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <new>
+
+#include "bz_launch.h"
+
+namespace bz {
+
+static thread_local char g_err[512] = "";
+
+int fail(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return -1;
+}
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail("%s: %s", what, hipGetErrorString(e));
+  return 0;
+}
+
+void timing_begin(const LaunchEnv& env, int kind) {
+  Timing* t = env.timing;
+  if (!t || !t->on || t->used[kind] >= t->cap) return;
+  hipEventRecord(t->start[kind][t->used[kind]], env.stream);
+}
+
+void timing_end(const LaunchEnv& env, int kind) {
+  Timing* t = env.timing;
+  if (!t || !t->on || t->used[kind] >= t->cap) return;
+  hipEventRecord(t->stop[kind][t->used[kind]], env.stream);
+  t->used[kind] += 1;
+  t->launches[kind] += 1;
+}
+
+}  // namespace bz
+
+using namespace bz;
+
+struct bz_ctx {
+  int device;
+  LaunchEnv env;
+  Clock* host_clock;  // pinned mirror
+};
+
+#define BZ_HIP(call)                                                      \
+  do {                                                                    \
+    const hipError_t e_ = (call);                                         \
+    if (e_ != hipSuccess) return fail("%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define BZ_CTX(ctx)                                        \
+  do {                                                     \
+    if (!(ctx)) return fail("null context");               \
+    BZ_HIP(hipSetDevice((ctx)->device));                   \
+  } while (0)
+
+extern "C" {
+
+const char* bz_last_error(void) { return g_err; }
+int bz_version(void) { return 1; }
+
+void bz_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  const U4 r = philox(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+  out[0] = r.a;
+  out[1] = r.b;
+  out[2] = r.c;
+  out[3] = r.d;
+}
+
+int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, int32_t max_nodes,
+                  int32_t max_keys) {
+  if (!out) return fail("null output pointer");
+  *out = nullptr;
+  if (precision != BZ_F32 && precision != BZ_F64) return fail("precision must be 4 or 8");
+  if (capacity < 1 || capacity > 0x7FFFFFFFll)
+    return fail("capacity must be in [1, 2^31-1] (32-bit scan values)");
+  if (max_nodes < 1) return fail("max_nodes must be >= 1");
+  if (max_keys < 1) max_keys = 1;
+  BZ_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  BZ_HIP(hipGetDeviceProperties(&prop, device));
+  bz_ctx* c = new (std::nothrow) bz_ctx();
+  if (!c) return fail("out of host memory");
+  memset(c, 0, sizeof(*c));
+  c->device = device;
+  LaunchEnv& e = c->env;
+  e.stream = nullptr;
+  e.precision = precision;
+  e.num_cu = prop.multiProcessorCount;
+  e.capacity = capacity;
+  e.max_nodes = max_nodes;
+  e.max_keys = max_keys;
+  e.push_shape = 0;
+  // smallest push tile is 256 particles
+  e.status_words = capacity / 256 + 128;
+  const int64_t per_block = 3ll * (max_nodes < 1024 ? max_nodes : 1024);
+  e.partials_doubles = (int64_t)e.num_cu * 8 * per_block;
+  if (e.partials_doubles < 3ll * max_nodes * e.num_cu * 2)
+    e.partials_doubles = 3ll * max_nodes * e.num_cu * 2;
+  e.sort_ws_bytes = (int64_t)sort_workspace_bytes(capacity, max_keys);
+  hipError_t st = hipMalloc((void**)&e.clock, sizeof(Clock));
+  if (st == hipSuccess) st = hipMalloc((void**)&e.status, e.status_words * sizeof(uint64_t));
+  if (st == hipSuccess) st = hipMalloc((void**)&e.partials, e.partials_doubles * sizeof(double));
+  if (st == hipSuccess) st = hipMalloc(&e.sort_ws, e.sort_ws_bytes);
+  if (st == hipSuccess) st = hipHostMalloc((void**)&c->host_clock, sizeof(Clock));
+  if (st == hipSuccess) st = hipMemset(e.clock, 0, sizeof(Clock));
+  if (st == hipSuccess) st = hipMemset(e.status, 0, e.status_words * sizeof(uint64_t));
+  if (st == hipSuccess) st = hipDeviceSynchronize();
+  if (st != hipSuccess) {
+    const int rc = fail("bz_ctx_create: %s", hipGetErrorString(st));
+    bz_ctx_destroy(c);
+    return rc;
+  }
+  memset(c->host_clock, 0, sizeof(Clock));
+  c->host_clock->epoch = 1;
+  if (hipMemcpy(e.clock, c->host_clock, sizeof(Clock), hipMemcpyHostToDevice) != hipSuccess) {
+    const int rc = fail("bz_ctx_create: clock upload failed");
+    bz_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int bz_ctx_destroy(bz_ctx_t c) {
+  if (!c) return 0;
+  hipSetDevice(c->device);
+  hipDeviceSynchronize();
+  if (c->env.clock) hipFree(c->env.clock);
+  if (c->env.status) hipFree(c->env.status);
+  if (c->env.partials) hipFree(c->env.partials);
+  if (c->env.sort_ws) hipFree(c->env.sort_ws);
+  if (c->host_clock) hipHostFree(c->host_clock);
+  if (Timing* t = c->env.timing) {
+    for (int k = 0; k < kTimedKinds; ++k) {
+      for (int i = 0; i < t->cap; ++i) {
+        hipEventDestroy(t->start[k][i]);
+        hipEventDestroy(t->stop[k][i]);
+      }
+      delete[] t->start[k];
+      delete[] t->stop[k];
+    }
+    delete t;
+  }
+  delete c;
+  return 0;
+}
+
+int bz_set_stream(bz_ctx_t c, void* stream) {
+  if (!c) return fail("null context");
+  c->env.stream = (hipStream_t)stream;
+  return 0;
+}
+
+int bz_set_push_shape(bz_ctx_t c, int shape) {
+  if (!c) return fail("null context");
+  if (shape < 0 || shape > 3) return fail("push shape must be 0..3");
+  c->env.push_shape = shape;
+  return 0;
+}
+
+int bz_timing_enable(bz_ctx_t c, int on) {
+  BZ_CTX(c);
+  if (!c->env.timing) {
+    Timing* t = new (std::nothrow) Timing();
+    if (!t) return fail("out of host memory");
+    memset(t, 0, sizeof(*t));
+    t->cap = 8192;
+    for (int k = 0; k < kTimedKinds; ++k) {
+      t->start[k] = new hipEvent_t[t->cap];
+      t->stop[k] = new hipEvent_t[t->cap];
+      for (int i = 0; i < t->cap; ++i) {
+        BZ_HIP(hipEventCreate(&t->start[k][i]));
+        BZ_HIP(hipEventCreate(&t->stop[k][i]));
+      }
+    }
+    c->env.timing = t;
+  }
+  BZ_HIP(hipStreamSynchronize(c->env.stream));
+  Timing* t = c->env.timing;
+  if (on) {
+    for (int k = 0; k < kTimedKinds; ++k) {
+      t->used[k] = 0;
+      t->launches[k] = 0;
+    }
+  }
+  t->on = on ? 1 : 0;
+  return 0;
+}
+
+int bz_timing_get(bz_ctx_t c, double* ms3, int64_t* launches3) {
+  BZ_CTX(c);
+  BZ_HIP(hipStreamSynchronize(c->env.stream));
+  Timing* t = c->env.timing;
+  for (int k = 0; k < kTimedKinds; ++k) {
+    double s = 0.0;
+    if (t) {
+      for (int i = 0; i < t->used[k]; ++i) {
+        float ms = 0.f;
+        BZ_HIP(hipEventElapsedTime(&ms, t->start[k][i], t->stop[k][i]));
+        s += ms;
+      }
+    }
+    if (ms3) ms3[k] = s;
+    if (launches3) launches3[k] = t ? t->used[k] : 0;
+  }
+  return 0;
+}
+
+int bz_synchronize(bz_ctx_t c) {
+  BZ_CTX(c);
+  BZ_HIP(hipStreamSynchronize(c->env.stream));
+  return 0;
+}
+
+int bz_clock_set(bz_ctx_t c, int64_t n, uint64_t step, double t) {
+  BZ_CTX(c);
+  if (n < 0 || n > c->env.capacity) return fail("count %lld outside [0, capacity]", (long long)n);
+  BZ_HIP(hipStreamSynchronize(c->env.stream));
+  BZ_HIP(hipMemcpy(c->host_clock, c->env.clock, sizeof(Clock), hipMemcpyDeviceToHost));
+  c->host_clock->n = n;
+  c->host_clock->n_next = n;
+  c->host_clock->step = step;
+  c->host_clock->t = t;
+  c->host_clock->ticket = 0;
+  BZ_HIP(hipMemcpy(c->env.clock, c->host_clock, sizeof(Clock), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int bz_clock_get(bz_ctx_t c, int64_t* n, uint64_t* step, double* t, uint64_t* counters) {
+  BZ_CTX(c);
+  BZ_HIP(hipMemcpyAsync(c->host_clock, c->env.clock, sizeof(Clock), hipMemcpyDeviceToHost,
+                        c->env.stream));
+  BZ_HIP(hipStreamSynchronize(c->env.stream));
+  if (n) *n = c->host_clock->n;
+  if (step) *step = c->host_clock->step;
+  if (t) *t = c->host_clock->t;
+  if (counters)
+    for (int k = 0; k < BZ_NCOUNTERS; ++k) counters[k] = c->host_clock->cnt[k];
+  return 0;
+}
+
+int bz_counters_reset(bz_ctx_t c) {
+  BZ_CTX(c);
+  BZ_HIP(hipStreamSynchronize(c->env.stream));
+  BZ_HIP(hipMemcpy(c->host_clock, c->env.clock, sizeof(Clock), hipMemcpyDeviceToHost));
+  memset(c->host_clock->cnt, 0, sizeof(c->host_clock->cnt));
+  BZ_HIP(hipMemcpy(c->env.clock, c->host_clock, sizeof(Clock), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // extern "C"
+
+__global__ void k_clock_advance(Clock* clk, double dt) {
+  clk->step += 1;
+  clk->t += dt;
+}
+
+extern "C" {
+
+int bz_clock_advance(bz_ctx_t c, double dt) {
+  BZ_CTX(c);
+  hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(1), 0, c->env.stream, c->env.clock, dt);
+  return check_launch("k_clock_advance");
+}
+
+static int check_params(const bz_ctx* c, const bz_params_t* p) {
+  if (!p) return fail("null params");
+  if (p->dim != 0 && p->dim != 1) return fail("dim must be 0 or 1");
+  if (p->dim == 1) {
+    if (p->n_cells < 1 || p->n_nodes < 1) return fail("n_cells / n_nodes must be >= 1");
+    if (p->n_nodes > c->env.max_nodes)
+      return fail("n_nodes %d exceeds context max_nodes %d", p->n_nodes, c->env.max_nodes);
+    if (p->bnf < 1) return fail("bnf must be >= 1");
+    if (!(p->dx > 0.0) || !(p->L > 0.0)) return fail("dx and L must be > 0");
+  }
+  if (p->xs_mode != BZ_XS_FIT && p->xs_mode != BZ_XS_TABLE) return fail("bad xs_mode");
+  if (p->rng_mode != BZ_RNG_PHILOX && p->rng_mode != BZ_RNG_STREAM) return fail("bad rng_mode");
+  return 0;
+}
+
+int bz_push_mcc(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const void* E,
+                const bz_params_t* p, const bz_xsec_t* xs, const void* R, const void* gas_v,
+                double* moments) {
+  BZ_CTX(c);
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!in || !out || !moments) return fail("null state or moments pointer");
+  if (in->x == out->x || in->w == out->w) return fail("in and out state must not alias");
+  if (p->dim == 1 && !E) return fail("1-D push needs the field array E");
+  if (p->xs_mode == BZ_XS_TABLE) {
+    if (!xs || !xs->energy || !xs->sig || !xs->slope || !xs->hint || xs->n < 2 ||
+        xs->n_hint < 1)
+      return fail("table mode needs a cross-section table");
+    if (xs->hint_bits < 1 || xs->hint_bits > 20) return fail("hint_bits must be in [1,20]");
+  }
+  if (p->rng_mode == BZ_RNG_STREAM) {
+    if (!R) return fail("stream mode needs the uniform array R");
+    if (p->r_stride < 1) return fail("stream mode needs r_stride >= 1");
+  }
+  return launch_push(c->env, in, out, E, p, xs, R, gas_v, moments, c->env.push_shape);
+}
+
+int bz_deposit(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p, double* moments) {
+  BZ_CTX(c);
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!s || !moments) return fail("null state or moments pointer");
+  return launch_deposit(c->env, s, p, moments);
+}
+
+int bz_field_step(bz_ctx_t c, const bz_field_params_t* fp, const double* moments, double* ni,
+                  double* ns, double* Te, double* g2, double* V, double* E, void* E_t,
+                  const double* chol, int32_t advance_time) {
+  BZ_CTX(c);
+  if (!fp) return fail("null field params");
+  if (fp->n < 3) return fail("field needs N >= 3 cells");
+  if (fp->n + 1 > c->env.max_nodes) return fail("N+1 exceeds context max_nodes");
+  if (!moments || !ni || !ns || !Te || !g2 || !V || !E || !E_t || !chol)
+    return fail("null field array");
+  return launch_field(c->env, fp, moments, ni, ns, Te, g2, V, E, E_t, chol, advance_time);
+}
+
+int bz_bin_sort(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
+                const double* thresholds_host, int32_t nb, int32_t* counts, int32_t* offsets,
+                uint32_t* perm) {
+  BZ_CTX(c);
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (nb < 1 || nb > 8) return fail("nb must be in [1,8]");
+  if (nb > 1 && !thresholds_host) return fail("nb > 1 needs thresholds");
+  const int64_t n_keys = (int64_t)(p->dim == 1 ? p->n_nodes : 1) * nb;
+  if (n_keys > c->env.max_keys) return fail("n_keys %lld exceeds context max_keys", (long long)n_keys);
+  if (n_keys > 65536) return fail("n_keys must be <= 65536 (two 8-bit radix passes)");
+  if (!s || !counts || !offsets || !perm) return fail("null bin/sort pointer");
+  return launch_bin_sort(c->env, s, p, thresholds_host, nb, counts, offsets, perm);
+}
+
+int bz_resample(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const bz_params_t* p,
+                const int32_t* counts, const int32_t* offsets, const uint32_t* perm,
+                const int32_t* targets, int32_t n_keys, const bz_resample_policy_t* pol) {
+  BZ_CTX(c);
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!in || !out || !counts || !offsets || !perm || !targets || !pol)
+    return fail("null resample pointer");
+  if (in->x == out->x) return fail("in and out state must not alias");
+  if (n_keys < 1 || n_keys > c->env.max_keys) return fail("bad n_keys");
+  return launch_resample(c->env, in, out, p, counts, offsets, perm, targets, n_keys, pol);
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// Host-side launch environment shared by the libbz translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "../../include/bz.h"
+#include "bz_device.h"
+
+namespace bz {
+
+struct LaunchEnv {
+  hipStream_t stream;
+  int precision;
+  int num_cu;
+  int64_t capacity;
+  Clock* clock;
+  uint64_t* status;          // tile status words
+  int64_t status_words;
+  double* partials;          // per-block partial histograms
+  int64_t partials_doubles;
+  void* sort_ws;             // sort / resample workspace
+  int64_t sort_ws_bytes;
+  int32_t max_nodes;
+  int32_t max_keys;
+  int push_shape;            // tile shape picked by the autotuner
+  // optional kernel timing: event pairs per timed kernel class
+  struct Timing* timing;
+};
+
+constexpr int kTimedKinds = 3;  // 0 push, 1 sort, 2 resample
+struct Timing {
+  int on;
+  int cap;
+  int used[kTimedKinds];
+  int64_t launches[kTimedKinds];
+  hipEvent_t* start[kTimedKinds];
+  hipEvent_t* stop[kTimedKinds];
+};
+// record start/stop around a launch when timing is on
+void timing_begin(const LaunchEnv& env, int kind);
+void timing_end(const LaunchEnv& env, int kind);
+
+int fail(const char* fmt, ...);
+int check_launch(const char* what);
+
+int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                const void* E, const bz_params_t* p, const bz_xsec_t* xs, const void* R,
+                const void* G, double* moments, int shape);
+int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                   double* moments);
+int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double* moments,
+                 double* ni, double* ns, double* Te, double* g2, double* V, double* E,
+                 void* E_t, const double* chol, int advance);
+int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                    const double* thr, int nb, int32_t* counts, int32_t* offsets,
+                    uint32_t* perm);
+int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                    const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
+                    const uint32_t* perm, const int32_t* targets, int32_t n_keys,
+                    const bz_resample_policy_t* pol);
+size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys);
+
+}  // namespace bz

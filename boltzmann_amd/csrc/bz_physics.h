@@ -1,0 +1,309 @@
+// Per-particle physics of the electron step (device side).
+//
+// Mirrors kernels/electron_kernel_numba.py:39-287 of the reference for the
+// fit model and adds the tabulated three-channel null-collision model.  All
+// arithmetic on the decision path is + - * / sqrt and compares, evaluated in
+// the order the reference writes it, compiled with -ffp-contract=off.
+#pragma once
+
+#include "bz_device.h"
+
+namespace bz {
+
+template <typename T>
+struct P5 {
+  T x, vx, vy, vz, w;
+};
+
+// Argon fits of the reference (electron_kernel_numba.py:67-171):
+// log10(sigma) = (a*L + b)*L + c with L = log10(E/eV), piecewise in E.
+__constant__ double kElBreak[11] = {0.000780396, 0.006098667, 0.0476269, 0.1015574,
+                                    0.1943527,   0.2995781,   1.097156,  2.339524,
+                                    4.988691,    10.63765,    22.68322};
+__constant__ double kElCoef[12][3] = {
+    {-0.02704763, -0.23720051, -19.67900951}, {-0.08847237, -0.6084786, -20.24111992},
+    {-0.37608274, -1.8167778, -21.51414308},  {-1.4874467, -4.82420619, -23.55745478},
+    {-0.9870356, -4.2206026, -23.44715988},   {14.28063581, 18.92275458, -14.70556113},
+    {-2.12069169, 0.7555105, -19.71943671},   {-0.3585636, 0.79246666, -19.71260274},
+    {1.25262128, -0.40002029, -19.48909003},  {-2.28332905, 4.89566076, -21.46789173},
+    {-1.47508661, 2.82263476, -20.1928456},   {-0.11090525, -0.77506081, -17.81752307}};
+__constant__ double kIonBreak[7] = {15.77742, 15.81964, 15.96196, 16.44162,
+                                    18.05837, 23.50771, 41.87498};
+__constant__ double kIonCoef[8][3] = {
+    {-2149239.99337822, 5150451.29014456, -3085665.64874862},
+    {-227229.83440255, 545171.14839945, -327016.69508374},
+    {-20173.68563579, 48599.13254342, -29290.96100603},
+    {-1838.90888364, 4491.58838596, -2763.82866252},
+    {-186.98923927, 477.04784782, -324.75732578},
+    {-24.48843901, 69.43965607, -69.145292411},
+    {-4.33101714, 14.47124519, -31.66224458},
+    {-0.68345667, 2.6053539, -21.99823511}};
+
+__device__ __forceinline__ double dlog10(double v) { return log10(v); }
+__device__ __forceinline__ float dlog10(float v) { return log10f(v); }
+__device__ __forceinline__ double dpow10(double v) { return pow(10.0, v); }
+__device__ __forceinline__ float dpow10(float v) { return powf(10.0f, v); }
+__device__ __forceinline__ double dexp(double v) { return exp(v); }
+__device__ __forceinline__ float dexp(float v) { return expf(v); }
+__device__ __forceinline__ double dlog(double v) { return log(v); }
+__device__ __forceinline__ float dlog(float v) { return logf(v); }
+__device__ __forceinline__ double dsqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ float dsqrt(float v) { return sqrtf(v); }
+__device__ __forceinline__ void dsincos(double v, double* s, double* c) { sincos(v, s, c); }
+__device__ __forceinline__ void dsincos(float v, float* s, float* c) { sincosf(v, s, c); }
+
+template <typename T>
+__device__ __forceinline__ T fit_poly(const double (*coef)[3], int seg, T L) {
+  const T a = (T)coef[seg][0], b = (T)coef[seg][1], c = (T)coef[seg][2];
+  return (a * L + b) * L + c;
+}
+
+// Fit model collision thresholds (electron_kernel_numba.py:115-195).
+template <typename T>
+__device__ __forceinline__ void fit_thresholds(T e, T vmag, const bz_params_t& p, T& r0, T& r1,
+                                               T& r2) {
+  const T L = dlog10(e);
+  int seg = 0;
+#pragma unroll
+  for (int k = 0; k < 11; ++k) seg += (e >= (T)kElBreak[k]) ? 1 : 0;
+  T sig0 = dpow10(fit_poly<T>(kElCoef, seg, L));
+  const T eion = (T)p.e_ion;
+  T sig2;
+  if (e < eion) {
+    sig2 = (T)0;
+  } else if (e == eion) {
+    sig2 = (T)1;  // no interval matches: 10**0 (electron_kernel_numba.py:164-174)
+  } else {
+    int s2 = 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) s2 += (e >= (T)kIonBreak[k]) ? 1 : 0;
+    sig2 = dpow10(fit_poly<T>(kIonCoef, s2, L));
+  }
+  const T nn = (T)p.nn;
+  sig0 *= nn;
+  sig2 *= nn;
+  const T sig_tot = sig0 + sig2;
+  const T P = (T)1 - dexp(-(T)p.dt * vmag * sig_tot);
+  const T pcst = P * ((T)1 / sig_tot);
+  r0 = sig0 * pcst;
+  r1 = r0;  // no excitation channel in the fit model
+  r2 = r0 + sig2 * pcst;
+}
+
+// Tabulated model: sigma*N piecewise linear in E, clamped to the table,
+// zero below each threshold; null-collision thresholds nu_k * P_null/nu_max.
+template <typename T>
+__device__ __forceinline__ int xs_bucket(T e, int bits);
+template <>
+__device__ __forceinline__ int xs_bucket<double>(double e, int bits) {
+  return (int)((uint64_t)__double_as_longlong(e) >> (52 - bits));
+}
+template <>
+__device__ __forceinline__ int xs_bucket<float>(float e, int bits) {
+  return (int)(__float_as_uint(e) >> (23 - bits));
+}
+
+template <typename T>
+__device__ __forceinline__ void table_thresholds(T e, T vmag, const bz_params_t& p,
+                                                 const bz_xsec_t& xs, T& r0, T& r1, T& r2,
+                                                 bool& over) {
+  const T* E = (const T*)xs.energy;
+  const T* S = (const T*)xs.sig;
+  const T* D = (const T*)xs.slope;
+  const int n = xs.n;
+  const T e0 = E[0], e1 = E[n - 1];
+  T ec = e < e0 ? e0 : e;
+  ec = ec > e1 ? e1 : ec;
+  int b = xs_bucket<T>(ec, xs.hint_bits) - xs.hint_base;
+  b = b < 0 ? 0 : (b >= xs.n_hint ? xs.n_hint - 1 : b);
+  int j = xs.hint[b];
+  for (int it = 0; it < 8; ++it) {
+    if (j + 1 <= n - 2 && E[j + 1] <= ec) ++j;
+    else break;
+  }
+  const T d = ec - E[j];
+  T s0 = S[j] + D[j] * d;
+  T s1 = S[n + j] + D[n + j] * d;
+  T s2 = S[2 * n + j] + D[2 * n + j] * d;
+  if (e < (T)p.e_exc) s1 = (T)0;
+  if (e < (T)p.e_ion) s2 = (T)0;
+  const T pc = (T)p.pc;
+  r0 = (s0 * vmag) * pc;
+  r1 = r0 + (s1 * vmag) * pc;
+  r2 = r1 + (s2 * vmag) * pc;
+  over = r2 > (T)p.p_null;
+}
+
+template <typename T>
+__device__ __forceinline__ void iso_dir(T vnew, T ra, T rb, T& vx, T& vy, T& vz) {
+  const T cchi = (T)2 * ra - (T)1;
+  T s2 = (T)1 - cchi * cchi;
+  const T schi = dsqrt(s2 > (T)0 ? s2 : (T)0);
+  const T phi = (T)6.283185307179586 * rb;
+  T sphi, cphi;
+  dsincos(phi, &sphi, &cphi);
+  vx = vnew * schi * cphi;
+  vy = vnew * schi * sphi;
+  vz = vnew * cchi;
+}
+
+template <typename T>
+__device__ __forceinline__ void gas_velocity(uint64_t seed, uint32_t i, uint64_t step, T sigma,
+                                             T& gx, T& gy, T& gz);
+template <>
+__device__ __forceinline__ void gas_velocity<double>(uint64_t seed, uint32_t i, uint64_t step,
+                                                     double sigma, double& gx, double& gy,
+                                                     double& gz) {
+  const U4 a = draw(seed, kPurposeGas, i, 0, step);
+  const U4 b = draw(seed, kPurposeGas, i, 1, step);
+  const double g0 = u53(a.a, a.b), g1 = u53(a.c, a.d), g2 = u53(b.a, b.b), g3 = u53(b.c, b.d);
+  const double r1 = sqrt(-2.0 * log(1.0 - g0)), r2 = sqrt(-2.0 * log(1.0 - g2));
+  double s, c;
+  sincos(6.283185307179586 * g1, &s, &c);
+  gx = sigma * (r1 * c);
+  gy = sigma * (r1 * s);
+  gz = sigma * (r2 * cos(6.283185307179586 * g3));
+}
+template <>
+__device__ __forceinline__ void gas_velocity<float>(uint64_t seed, uint32_t i, uint64_t step,
+                                                    float sigma, float& gx, float& gy,
+                                                    float& gz) {
+  const U4 a = draw(seed, kPurposeGas, i, 0, step);
+  const float g0 = u24(a.a), g1 = u24(a.b), g2 = u24(a.c), g3 = u24(a.d);
+  const float r1 = sqrtf(-2.0f * logf(1.0f - g0)), r2 = sqrtf(-2.0f * logf(1.0f - g2));
+  float s, c;
+  sincosf(6.2831853f * g1, &s, &c);
+  gx = sigma * (r1 * c);
+  gy = sigma * (r1 * s);
+  gz = sigma * (r2 * cosf(6.2831853f * g3));
+}
+
+enum : int { kChanNone = 0, kChanElastic = 1, kChanExc = 2, kChanIon = 3 };
+
+template <typename T>
+struct StepOut {
+  P5<T> p;      // primary after collision, push and walls
+  P5<T> s;      // secondary (valid when born)
+  T e_post;     // primary energy after push, eV
+  T e_sec;      // secondary energy, eV
+  int alive;
+  int born;
+  int chan;
+  int over;
+};
+
+// One particle through collision + push + walls
+// (electron_kernel_numba.py:39-287; dead particles are dropped, not pushed).
+template <typename T, int XS, int DIM, int RNG>
+__device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, uint64_t step,
+                                                    const bz_params_t& p, const bz_xsec_t& xs,
+                                                    const T* __restrict__ E, const T* R,
+                                                    const T* G) {
+  StepOut<T> o;
+  o.alive = 0;
+  o.born = 0;
+  o.chan = kChanNone;
+  o.over = 0;
+  o.e_post = (T)0;
+  o.e_sec = (T)0;
+  o.s = in;
+  if (!(in.w > (T)0)) {
+    o.p = in;
+    return o;
+  }
+  T gx = (T)0, gy = (T)0, gz = (T)0;
+  if (RNG == BZ_RNG_STREAM) {
+    if (G) {
+      const int64_t st = p.r_stride;
+      gx = G[i];
+      gy = G[st + i];
+      gz = G[2 * st + i];
+    }
+  } else if (p.gas_sigma > 0.0) {
+    gas_velocity<T>(p.seed, i, step, (T)p.gas_sigma, gx, gy, gz);
+  }
+  T vx = in.vx, vy = in.vy, vz = in.vz;
+  const T ux = vx - gx, uy = vy - gy, uz = vz - gz;
+  const T s = ux * ux + uy * uy + uz * uz;
+  const T vmag = dsqrt(s);
+  const T e = (T)p.kev * s;
+
+  Uniforms<T> U;
+  if (RNG == BZ_RNG_STREAM) {
+    const int64_t st = p.r_stride;
+    U.u[0] = R[i];
+    U.u[1] = R[st + i];
+    U.u[2] = R[2 * st + i];
+    U.u[3] = R[3 * st + i];
+    U.u[4] = R[5 * st + i];
+  } else {
+    U.first(p.seed, i, step);
+  }
+
+  T r0, r1, r2;
+  bool over = false;
+  if (XS == BZ_XS_FIT) fit_thresholds<T>(e, vmag, p, r0, r1, r2);
+  else table_thresholds<T>(e, vmag, p, xs, r0, r1, r2, over);
+  o.over = over ? 1 : 0;
+
+  const T R0 = U.u[0];
+  T tx = (T)0, ty = (T)0, tz = (T)0;
+  if (R0 < r0) {
+    o.chan = kChanElastic;
+    if (RNG != BZ_RNG_STREAM) U.second(p.seed, i, step);
+    iso_dir<T>(vmag * (T)p.fel, U.u[1], U.u[2], vx, vy, vz);
+  } else if (XS == BZ_XS_TABLE && R0 < r1) {
+    o.chan = kChanExc;
+    if (RNG != BZ_RNG_STREAM) U.second(p.seed, i, step);
+    const T enew = e - (T)p.e_exc;
+    iso_dir<T>(dsqrt(((T)2 * enew) / (T)p.jm), U.u[1], U.u[2], vx, vy, vz);
+  } else if ((XS == BZ_XS_TABLE && R0 < r2) || (XS == BZ_XS_FIT && r0 < R0 && R0 < r2)) {
+    o.chan = kChanIon;
+    if (RNG != BZ_RNG_STREAM) {
+      U.second(p.seed, i, step);
+      U.third(p.seed, i, step);
+    }
+    const T eej = fabs((T)0.5 * (e - (T)p.e_ion));
+    const T vnew = dsqrt(((T)2 * eej) / (T)p.jm);
+    iso_dir<T>(vnew, U.u[1], U.u[2], vx, vy, vz);
+    iso_dir<T>(vnew, U.u[3], U.u[4], tx, ty, tz);
+    o.born = 1;
+    o.s.x = in.x;
+    o.s.vx = tx;
+    o.s.vy = ty;
+    o.s.vz = tz;
+    o.s.w = in.w;
+    o.e_sec = (T)p.kev * (tx * tx + ty * ty + tz * tz);
+  }
+
+  T x = in.x;
+  T ef;
+  if (DIM == 1) {
+    x = x + (T)p.xfac * vx;
+    int c = (int)((double)in.x / p.dx);
+    c = c < 0 ? 0 : (c >= p.n_cells ? p.n_cells - 1 : c);
+    ef = E[c];
+  } else {
+    ef = E ? E[0] : (T)p.e_field0;
+  }
+  vx = vx - (T)p.afac * ((T)100 * ef);
+  if (DIM == 1) o.alive = ((T)0 < x && x < (T)p.L) ? 1 : 0;
+  else o.alive = 1;
+  o.p.x = x;
+  o.p.vx = vx;
+  o.p.vy = vy;
+  o.p.vz = vz;
+  o.p.w = in.w;
+  o.e_post = (T)p.kev * (vx * vx + vy * vy + vz * vz);
+  return o;
+}
+
+// Node of the deposit: fine bin int((x+dx/2)/(L+dx) * M), then fine -> node
+// by integer division by bnf (kernels-new.cu:63 and :8-50).
+__device__ __forceinline__ int dep_node(double x, const bz_params_t& p) {
+  const int b = (int)(((x + p.half_dx) / p.L_dx) * p.dep_M);
+  int k = b / p.bnf;
+  return k < 0 ? 0 : (k >= p.n_nodes ? p.n_nodes - 1 : k);
+}
+
+}  // namespace bz

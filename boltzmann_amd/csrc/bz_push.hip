@@ -1,0 +1,402 @@
+// Fused electron step for gfx950: MCC + push + walls + node deposit +
+// compaction with births, in one persistent pass over the SoA state.
+//
+// Data flow per tile of BLOCK*ITEMS particles (index-contiguous):
+//   HBM read  x, vx, vy, vz, w            (5 words / particle)
+//   registers collision, push, walls, secondary
+//   LDS       per-block FP64 node histograms (ne, ni_src, nrg)
+//   scan      per-particle output count (alive + born) -> block scan ->
+//             decoupled look-back across tiles -> output slot
+//   HBM write survivors, each secondary right after its parent
+// Node histograms are flushed once per block to a partial buffer and summed
+// over blocks in a fixed order by k_dep_reduce, so the moments do not depend
+// on scheduling across blocks.
+#include "bz_physics.h"
+
+namespace bz {
+
+struct PushArgs {
+  bz_params_t p;
+  bz_xsec_t xs;
+  const void* in[5];
+  void* out[5];
+  const void* E;
+  const void* R;
+  const void* G;
+  Clock* clk;
+  uint64_t* status;
+  double* partials;  // [gridDim][3][n_nodes]
+  int64_t capacity;
+};
+
+template <typename T>
+__device__ __forceinline__ void store5(void* const* out, int64_t pos, const P5<T>& q) {
+  ((T*)out[0])[pos] = q.x;
+  ((T*)out[1])[pos] = q.vx;
+  ((T*)out[2])[pos] = q.vy;
+  ((T*)out[3])[pos] = q.vz;
+  ((T*)out[4])[pos] = q.w;
+}
+
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
+  static_assert(ITEMS >= 1 && ITEMS <= 4, "packed scan holds 4 fields of 16 bits");
+  constexpr int NW = BLOCK / kWave;
+  constexpr int TILE = BLOCK * ITEMS;
+  extern __shared__ double s_dep[];  // [3][n_nodes] (DIM 1) or [3] (DIM 0)
+  __shared__ uint64_t s_wsum[NW];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_prefix;
+  __shared__ unsigned long long s_cnt[8];
+
+  const bz_params_t& p = a.p;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  const int nn = (DIM == 1) ? p.n_nodes : 1;
+  for (int j = tid; j < 3 * nn; j += BLOCK) s_dep[j] = 0.0;
+  if (tid < 8) s_cnt[tid] = 0ull;
+
+  const int64_t n = a.clk->n;
+  const uint64_t step = a.clk->step;
+  const uint32_t epoch = a.clk->epoch;
+  const uint32_t ntiles = (uint32_t)((n + TILE - 1) / TILE);
+  if (blockIdx.x == 0 && tid == 0) {
+    if (n == 0) a.clk->n_next = 0;
+    atomicAdd(&a.clk->cnt[BZ_CNT_PSTEPS], (unsigned long long)n);
+  }
+
+  const T* __restrict__ ix = (const T*)a.in[0];
+  const T* __restrict__ ivx = (const T*)a.in[1];
+  const T* __restrict__ ivy = (const T*)a.in[2];
+  const T* __restrict__ ivz = (const T*)a.in[3];
+  const T* __restrict__ iw = (const T*)a.in[4];
+  const T* __restrict__ E = (const T*)a.E;
+  const T* R = (const T*)a.R;
+  const T* G = (const T*)a.G;
+
+  uint32_t c_el = 0, c_ex = 0, c_io = 0, c_abs = 0, c_null = 0, c_over = 0, c_drop = 0;
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;  // DIM 0 deposit in registers
+  __syncthreads();
+
+  while (true) {
+    if (tid == 0) s_tile = atomicAdd(&a.clk->ticket, 1u);
+    __syncthreads();
+    const uint32_t t = s_tile;
+    if (t >= ntiles) break;
+    const int64_t base = (int64_t)t * TILE;
+
+    StepOut<T> o[ITEMS];
+    uint64_t packed = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int64_t i = base + (int64_t)k * BLOCK + tid;
+      P5<T> q;
+      if (i < n) {
+        q.x = ix[i];
+        q.vx = ivx[i];
+        q.vy = ivy[i];
+        q.vz = ivz[i];
+        q.w = iw[i];
+      } else {
+        q.x = q.vx = q.vy = q.vz = q.w = (T)0;
+      }
+      o[k] = step_particle<T, XS, DIM, RNG>(q, (uint32_t)i, step, p, a.xs, E, R, G);
+      if (i < n && q.w > (T)0) {
+        c_el += o[k].chan == kChanElastic;
+        c_ex += o[k].chan == kChanExc;
+        c_io += o[k].chan == kChanIon;
+        c_null += o[k].chan == kChanNone;
+        c_abs += !o[k].alive;
+        c_over += o[k].over;
+      }
+      const uint32_t cnt = (uint32_t)(o[k].alive + o[k].born);
+      packed |= (uint64_t)cnt << (16 * k);
+      // deposit: survivors at their new position, secondaries where born
+      if (o[k].alive) {
+        const double w = (double)o[k].p.w;
+        const double ni = (o[k].chan == kChanIon) ? w : 0.0;
+        const double en = w * (double)o[k].e_post;
+        if (DIM == 1) {
+          const int nd = dep_node((double)o[k].p.x, p);
+          atomicAdd(&s_dep[nd], w);
+          if (ni != 0.0) atomicAdd(&s_dep[nn + nd], ni);
+          atomicAdd(&s_dep[2 * nn + nd], en);
+        } else {
+          acc0 += w;
+          acc1 += ni;
+          acc2 += en;
+        }
+      }
+      if (o[k].born) {
+        const double w = (double)o[k].s.w;
+        const double en = w * (double)o[k].e_sec;
+        if (DIM == 1) {
+          const int nd = dep_node((double)o[k].s.x, p);
+          atomicAdd(&s_dep[nd], w);
+          atomicAdd(&s_dep[2 * nn + nd], en);
+        } else {
+          acc0 += w;
+          acc2 += en;
+        }
+      }
+    }
+
+    // block exclusive scan of the packed counts, in index order k-major
+    const uint64_t incl = wave_incl_scan(packed);
+    if (lane == kWave - 1) s_wsum[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+      uint64_t v = lane < NW ? s_wsum[lane] : 0ull;
+      v = wave_incl_scan(v);
+      if (lane < NW) s_wsum[lane] = v;
+    }
+    __syncthreads();
+    const uint64_t excl = (wid ? s_wsum[wid - 1] : 0ull) + incl - packed;
+    const uint64_t tot = s_wsum[NW - 1];
+    uint32_t field_base[ITEMS];
+    uint32_t agg = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      field_base[k] = agg;
+      agg += (uint32_t)((tot >> (16 * k)) & 0xFFFFu);
+    }
+    if (wid == 0) {
+      const uint32_t pre = lookback(a.status, t, agg, epoch);
+      if (lane == 0) {
+        s_prefix = pre;
+        if (t == ntiles - 1) a.clk->n_next = (int64_t)(pre + agg);
+      }
+    }
+    __syncthreads();
+    const int64_t tile_pre = s_prefix;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int64_t pos =
+          tile_pre + field_base[k] + (int64_t)((excl >> (16 * k)) & 0xFFFFu);
+      if (o[k].alive) {
+        if (pos < a.capacity) store5<T>(a.out, pos, o[k].p);
+        else ++c_drop;
+      }
+      if (o[k].born) {
+        const int64_t ps = pos + o[k].alive;
+        if (ps < a.capacity) store5<T>(a.out, ps, o[k].s);
+        else ++c_drop;
+      }
+    }
+    // s_tile / s_wsum / s_prefix are rewritten only after the next barrier
+  }
+
+  // counters: wave reduce, one atomic per wave
+  {
+    const uint32_t v[7] = {c_io, c_abs, c_el, c_ex, c_io, c_null, c_drop};
+    const int slot[7] = {BZ_CNT_BIRTHS, BZ_CNT_ABSORBED, BZ_CNT_ELASTIC, BZ_CNT_EXCITATION,
+                         BZ_CNT_IONIZATION, BZ_CNT_NULL, BZ_CNT_OVERFLOW};
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const uint32_t s = wave_sum(v[q]);
+      if (lane == 0 && s) atomicAdd(&a.clk->cnt[slot[q]], (unsigned long long)s);
+    }
+    const uint32_t so = wave_sum(c_over);
+    if (lane == 0 && so) atomicAdd(&a.clk->cnt[BZ_CNT_NUMAX], (unsigned long long)so);
+  }
+  if (DIM == 0) {
+    acc0 = wave_sum(acc0);
+    acc1 = wave_sum(acc1);
+    acc2 = wave_sum(acc2);
+    if (lane == 0) {
+      atomicAdd(&s_dep[0], acc0);
+      atomicAdd(&s_dep[1], acc1);
+      atomicAdd(&s_dep[2], acc2);
+    }
+  }
+  __syncthreads();
+  double* part = a.partials + (size_t)blockIdx.x * 3 * nn;
+  for (int j = tid; j < 3 * nn; j += BLOCK) part[j] = s_dep[j];
+}
+
+// Sum the per-block partial histograms in block order and commit the clock.
+__global__ void k_dep_reduce(const double* __restrict__ partials, int nblocks, int m,
+                             double* __restrict__ out, Clock* clk, int commit) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * m + j];
+    out[j] = s;
+  }
+  if (commit && j == 0) {
+    clk->n = clk->n_next;
+    clk->ticket = 0;
+    clk->epoch = clk->epoch + 1;
+  }
+}
+
+// Deposit of a state without pushing it (initial moments).
+template <typename T, int DIM, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_deposit(bz_params_t p, const T* __restrict__ x,
+                                                  const T* __restrict__ vx,
+                                                  const T* __restrict__ vy,
+                                                  const T* __restrict__ vz,
+                                                  const T* __restrict__ w, const Clock* clk,
+                                                  double* partials) {
+  extern __shared__ double s_dep[];
+  const int nn = DIM == 1 ? p.n_nodes : 1;
+  for (int j = threadIdx.x; j < 3 * nn; j += BLOCK) s_dep[j] = 0.0;
+  __syncthreads();
+  const int64_t n = clk->n;
+  double a0 = 0.0, a2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * BLOCK) {
+    const T wi = w[i];
+    if (!(wi > (T)0)) continue;
+    const T e = (T)p.kev * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    const double wd = (double)wi;
+    if (DIM == 1) {
+      const int nd = dep_node((double)x[i], p);
+      atomicAdd(&s_dep[nd], wd);
+      atomicAdd(&s_dep[2 * nn + nd], wd * (double)e);
+    } else {
+      a0 += wd;
+      a2 += wd * (double)e;
+    }
+  }
+  if (DIM == 0) {
+    a0 = wave_sum(a0);
+    a2 = wave_sum(a2);
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&s_dep[0], a0);
+      atomicAdd(&s_dep[2], a2);
+    }
+  }
+  __syncthreads();
+  double* part = partials + (size_t)blockIdx.x * 3 * nn;
+  for (int j = threadIdx.x; j < 3 * nn; j += BLOCK) part[j] = s_dep[j];
+}
+
+}  // namespace bz
+
+// ---------------------------------------------------------------------------
+// Launch helpers (called from bz_api.hip)
+// ---------------------------------------------------------------------------
+#include "bz_launch.h"
+
+namespace bz {
+
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
+static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
+  auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS>;
+  const size_t smem = (size_t)3 * (DIM == 1 ? a.p.n_nodes : 1) * sizeof(double);
+  if (smem > 64 * 1024) {
+    if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)smem) != hipSuccess)
+      return fail("hipFuncSetAttribute(dynamic LDS) failed");
+  }
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, smem) != hipSuccess ||
+      occ < 1)
+    return fail("push kernel does not fit on a CU (LDS %zu B)", smem);
+  int grid = env.num_cu * (occ > 8 ? 8 : occ);
+  const int64_t cap_parts = env.partials_doubles / (3 * (DIM == 1 ? a.p.n_nodes : 1));
+  if (grid > cap_parts) grid = (int)cap_parts;
+  if (grid < 1) return fail("deposit partial buffer too small");
+  timing_begin(env, 0);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
+  timing_end(env, 0);
+  *grid_out = grid;
+  return check_launch("k_push");
+}
+
+template <typename T, int XS, int DIM, int RNG>
+static int launch_push_shape(const LaunchEnv& env, PushArgs& a, int shape, int* grid) {
+  switch (shape) {
+    case 0: return launch_push_t<T, XS, DIM, RNG, 256, 4>(env, a, grid);
+    case 1: return launch_push_t<T, XS, DIM, RNG, 512, 2>(env, a, grid);
+    case 2: return launch_push_t<T, XS, DIM, RNG, 256, 2>(env, a, grid);
+    case 3: return launch_push_t<T, XS, DIM, RNG, 1024, 1>(env, a, grid);
+    default: return fail("unknown push shape %d", shape);
+  }
+}
+
+template <typename T>
+static int launch_push_prec(const LaunchEnv& env, PushArgs& a, int shape, int* grid) {
+  const bz_params_t& p = a.p;
+  const bool stream = p.rng_mode == BZ_RNG_STREAM;
+  if (p.xs_mode == BZ_XS_TABLE) {
+    if (p.dim == 1)
+      return stream ? launch_push_shape<T, BZ_XS_TABLE, 1, BZ_RNG_STREAM>(env, a, 0, grid)
+                    : launch_push_shape<T, BZ_XS_TABLE, 1, BZ_RNG_PHILOX>(env, a, shape, grid);
+    return stream ? launch_push_shape<T, BZ_XS_TABLE, 0, BZ_RNG_STREAM>(env, a, 0, grid)
+                  : launch_push_shape<T, BZ_XS_TABLE, 0, BZ_RNG_PHILOX>(env, a, shape, grid);
+  }
+  if (p.dim == 1)
+    return stream ? launch_push_shape<T, BZ_XS_FIT, 1, BZ_RNG_STREAM>(env, a, 0, grid)
+                  : launch_push_shape<T, BZ_XS_FIT, 1, BZ_RNG_PHILOX>(env, a, 0, grid);
+  return stream ? launch_push_shape<T, BZ_XS_FIT, 0, BZ_RNG_STREAM>(env, a, 0, grid)
+                : launch_push_shape<T, BZ_XS_FIT, 0, BZ_RNG_PHILOX>(env, a, 0, grid);
+}
+
+int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                const void* E, const bz_params_t* p, const bz_xsec_t* xs, const void* R,
+                const void* G, double* moments, int shape) {
+  PushArgs a;
+  a.p = *p;
+  if (xs) a.xs = *xs;
+  else memset(&a.xs, 0, sizeof(a.xs));
+  const void* iv[5] = {in->x, in->vx, in->vy, in->vz, in->w};
+  void* ov[5] = {out->x, out->vx, out->vy, out->vz, out->w};
+  for (int k = 0; k < 5; ++k) {
+    a.in[k] = iv[k];
+    a.out[k] = ov[k];
+  }
+  a.E = E;
+  a.R = R;
+  a.G = G;
+  a.clk = env.clock;
+  a.status = env.status;
+  a.partials = env.partials;
+  a.capacity = env.capacity;
+  int grid = 0;
+  const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &grid)
+                                         : launch_push_prec<float>(env, a, shape, &grid);
+  if (rc) return rc;
+  const int m = 3 * (p->dim == 1 ? p->n_nodes : 1);
+  hipLaunchKernelGGL(k_dep_reduce, dim3((m + 255) / 256), dim3(256), 0, env.stream,
+                     (const double*)env.partials, grid, m, moments, env.clock, 1);
+  return check_launch("k_dep_reduce");
+}
+
+template <typename T, int DIM>
+static int launch_deposit_t(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                            double* moments) {
+  constexpr int BLOCK = 256;
+  auto kern = k_deposit<T, DIM, BLOCK>;
+  const int nn = DIM == 1 ? p->n_nodes : 1;
+  const size_t smem = (size_t)3 * nn * sizeof(double);
+  if (smem > 64 * 1024) {
+    if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)smem) != hipSuccess)
+      return fail("hipFuncSetAttribute(dynamic LDS) failed");
+  }
+  int grid = env.num_cu * 2;
+  const int64_t cap_parts = env.partials_doubles / (3 * nn);
+  if (grid > cap_parts) grid = (int)cap_parts;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, *p, (const T*)s->x,
+                     (const T*)s->vx, (const T*)s->vy, (const T*)s->vz, (const T*)s->w,
+                     (const Clock*)env.clock, env.partials);
+  int rc = check_launch("k_deposit");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dep_reduce, dim3((3 * nn + 255) / 256), dim3(256), 0, env.stream,
+                     (const double*)env.partials, grid, 3 * nn, moments, env.clock, 0);
+  return check_launch("k_dep_reduce");
+}
+
+int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                   double* moments) {
+  if (env.precision == BZ_F64)
+    return p->dim == 1 ? launch_deposit_t<double, 1>(env, s, p, moments)
+                       : launch_deposit_t<double, 0>(env, s, p, moments);
+  return p->dim == 1 ? launch_deposit_t<float, 1>(env, s, p, moments)
+                     : launch_deposit_t<float, 0>(env, s, p, moments);
+}
+
+}  // namespace bz

@@ -1,0 +1,235 @@
+/*
+ * bz.h -- C ABI of libbz.so, the MI355X (gfx950) electron kinetic step.
+ *
+ * Every entry point takes plain device pointers and sizes, returns 0 on
+ * success and a negative code on failure (text via bz_last_error()), and
+ * enqueues its work on the stream held by the context.  No entry point
+ * allocates device memory except bz_ctx_create / bz_ctx_destroy.
+ *
+ * Particle state is structure-of-arrays: x (cm), vx, vy, vz (m/s) and the
+ * weight w, each a contiguous array of the context precision (float for
+ * BZ_F32, double for BZ_F64).  The reference keeps AoS rows
+ * data[:,0:6] = (x,y,z,vx,vy,vz) and tosum[:,0:4] = (w, ion flag, 0, energy)
+ * (1bin/boltzmann.py:37-44); y and z are never read by any stage and never
+ * written to disk there (kernels/data_write_1D.py:71-75), so they are not
+ * stored, and the energy is recomputed from the velocity where it is needed.
+ *
+ * The live particle count, the step counter (which keys the random stream)
+ * and the simulated time live in device memory inside the context (the
+ * "clock"), so a whole step can be enqueued or captured in a hipGraph with
+ * no host synchronisation.  bz_clock_set / bz_clock_get move them to and
+ * from the host.
+ */
+#ifndef BZ_H_
+#define BZ_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bz_ctx* bz_ctx_t;
+
+enum { BZ_F32 = 4, BZ_F64 = 8 };
+/* collision model */
+enum {
+  BZ_XS_FIT = 0,   /* reference argon fits, direct P = 1-exp(-nu dt)
+                      (kernels/electron_kernel_numba.py:66-195) */
+  BZ_XS_TABLE = 1  /* tabulated sigma(E), 3 channels, null collision */
+};
+/* random stream */
+enum {
+  BZ_RNG_PHILOX = 0, /* counter-based Philox4x32-10 keyed by (seed, step, index) */
+  BZ_RNG_STREAM = 1  /* caller-supplied uniforms R[7][stride], gas_v[3][stride]
+                        (the reference's cp_R_vec / gas_v arrays) */
+};
+
+/* counters accumulated by the kernels (uint64 each) */
+enum {
+  BZ_CNT_BIRTHS = 0,
+  BZ_CNT_ABSORBED = 1,
+  BZ_CNT_ELASTIC = 2,
+  BZ_CNT_EXCITATION = 3,
+  BZ_CNT_IONIZATION = 4,
+  BZ_CNT_NULL = 5,
+  BZ_CNT_OVERFLOW = 6,  /* particles dropped because capacity was reached */
+  BZ_CNT_NUMAX = 7,     /* null-collision bound exceeded (table mode) */
+  BZ_CNT_SPLIT = 8,     /* resampling: segments split */
+  BZ_CNT_MERGE = 9,     /* resampling: segments merged */
+  BZ_CNT_RESAMPLE = 10, /* resampling passes that fired */
+  BZ_CNT_PSTEPS = 11,   /* particle-steps pushed (sum of input counts) */
+  BZ_NCOUNTERS = 16
+};
+
+typedef struct {
+  void* x;
+  void* vx;
+  void* vy;
+  void* vz;
+  void* w;
+} bz_state_t;
+
+/* Per-run constants of the push / collision step.  Derived constants are
+ * computed once on the host so that the device and the CPU check use the
+ * same values. */
+typedef struct {
+  double dt;        /* s */
+  double xfac;      /* 100*dt: cm moved per (m/s)          (electron_kernel_numba.py:258) */
+  double afac;      /* dt*q_e/m_e                           (electron_kernel_numba.py:264) */
+  double kev;       /* 0.5*m_e*j_ev: eV per (m/s)^2         (electron_kernel_numba.py:63) */
+  double jm;        /* j_ev*m_e                             (electron_kernel_numba.py:226) */
+  double fel;       /* sqrt(1 - 2 m_e/m_n), elastic speed factor (electron_kernel_numba.py:206) */
+  double L;         /* domain length, cm */
+  double dx;        /* cell size, cm */
+  double half_dx;   /* 0.5*dx */
+  double L_dx;      /* L+dx                                 (post_processing_1D.py:44) */
+  double dep_M;     /* bnf*n_nodes, fine bins of the deposit (boltzmann.py:67) */
+  double nn;        /* fit mode: gas density multiplying sigma, m^-3 */
+  double e_ion;     /* ionization threshold, eV */
+  double e_exc;     /* excitation threshold, eV (table mode) */
+  double p_null;    /* table mode: 1-exp(-nu_max dt) */
+  double pc;        /* table mode: p_null/nu_max */
+  double nu_max;    /* table mode: null-collision frequency bound, 1/s */
+  double gas_sigma; /* std of each gas-velocity component, m/s (0: cold gas) */
+  double e_field0;  /* 0-D mode: uniform E, V/cm (used when E pointer is NULL) */
+  uint64_t seed;
+  int32_t dim;      /* 0: 0-D (no advection, no walls), 1: 1-D bounded */
+  int32_t xs_mode;  /* BZ_XS_* */
+  int32_t rng_mode; /* BZ_RNG_* */
+  int32_t n_cells;  /* N: length of E */
+  int32_t n_nodes;  /* N+1: deposit nodes */
+  int32_t bnf;      /* fine bins per node of the deposit */
+  int64_t r_stride; /* BZ_RNG_STREAM: column stride of R and gas_v */
+} bz_params_t;
+
+/* Tabulated sigma(E)*N_gas, three channels (0 elastic, 1 excitation,
+ * 2 ionization), piecewise linear on n nodes.  Device pointers in the
+ * context precision.  hint[] maps the top bits of the IEEE energy to the
+ * lowest candidate segment (built by the host). */
+typedef struct {
+  const void* energy;   /* [n] eV, increasing */
+  const void* sig;      /* [3][n] sigma*N, 1/m */
+  const void* slope;    /* [3][n] d(sigma*N)/dE per segment */
+  const int32_t* hint;  /* [n_hint] */
+  int32_t n;
+  int32_t n_hint;
+  int32_t hint_base;    /* bucket number of hint[0] */
+  int32_t hint_bits;    /* mantissa bits used by the bucket */
+} bz_xsec_t;
+
+/* Field / fluid constants (FP64), 1-D mode. */
+typedef struct {
+  double dx;        /* cm */
+  double dt;        /* s, fluid step (= kinetic step) */
+  double eps;       /* permittivity in e/(V cm) (boltzmann.py:56) */
+  double v0;        /* RF amplitude, V */
+  double freq;      /* Hz */
+  double mu_i, D_i, D_s; /* ion mobility / diffusion, metastable diffusion (boltzmann.py:50-55) */
+  int32_t n;        /* N cells (N+1 nodes, N-1 interior unknowns) */
+  int32_t fluid_on; /* 0: heavies frozen (0-D test problems) */
+  int32_t poisson_on;
+  int32_t pad;
+} bz_field_params_t;
+
+/* Resampling policy (ES-WPNC merge / split). */
+typedef struct {
+  double split_below;   /* split when 0 < count < split_below*target (0.5) */
+  double merge_at;      /* merge when count >= merge_at*target (3.0) */
+  double merge_keep;    /* merged segment keeps int(merge_keep*target) (0.6) */
+  double trigger_at;    /* pass fires when some segment count >= trigger_at*target */
+  double min_wt;        /* split only particles with w > min_wt (0.1) */
+  double split_to;      /* split draws int(split_to*target) - count clones (2) */
+  int32_t ref_trigger;  /* 1: reference trigger (last bin's max only, quirk) */
+  int32_t max_ppc;      /* >0: merge uses at most max_ppc particles (reference clip) */
+  int32_t ref_rescale;  /* 1: reference merge rescale sqrt(T/T_hat) from total energy;
+                           0: central moments (conserves energy with drift) */
+  int32_t pad;
+} bz_resample_policy_t;
+
+const char* bz_last_error(void);
+int bz_version(void);
+
+/* Context: one device, one stream, workspace sized for `capacity` particles
+ * per state buffer and up to `max_nodes` deposit nodes / `max_keys` sort keys. */
+int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity,
+                  int32_t max_nodes, int32_t max_keys);
+int bz_ctx_destroy(bz_ctx_t ctx);
+int bz_set_stream(bz_ctx_t ctx, void* hip_stream);
+/* Tile shape of the fused push (0..3), chosen by the autotuner; no
+ * counterpart in the reference (its launch shape is fixed at 1024 x 64,
+ * 1bin/boltzmann.py:139-140). */
+int bz_set_push_shape(bz_ctx_t ctx, int shape);
+int bz_synchronize(bz_ctx_t ctx);
+
+/* Device clock: live count, step, time; counters (BZ_NCOUNTERS uint64). */
+int bz_clock_set(bz_ctx_t ctx, int64_t n, uint64_t step, double t);
+int bz_clock_get(bz_ctx_t ctx, int64_t* n, uint64_t* step, double* t, uint64_t* counters);
+int bz_counters_reset(bz_ctx_t ctx);
+/* step += 1, t += dt, on the stream */
+int bz_clock_advance(bz_ctx_t ctx, double dt);
+
+/* Kernel timing with HIP events recorded on the context stream around each
+ * launch of the fused push kernel (k_push) and of the resampling kernels.
+ * bz_timing_enable(ctx, 1) clears and starts, 0 stops; bz_timing_get
+ * synchronises and returns the summed milliseconds and launch counts of
+ * [0] push, [1] sort, [2] resample.  No counterpart in the reference. */
+int bz_timing_enable(bz_ctx_t ctx, int on);
+int bz_timing_get(bz_ctx_t ctx, double* ms3, int64_t* launches3);
+
+/* Fused electron step: MCC collision (elastic / excitation / ionization with
+ * a secondary born beside its parent), leapfrog push in E(x), absorbing walls,
+ * node deposit of (sum w, sum w*ionized, sum w*energy) and stream compaction.
+ *   in  -> out: survivors in input order; each secondary directly follows its
+ *               parent.  Count read from / written to the clock.
+ *   E          : [n_cells] field, V/cm, context precision (NULL in 0-D: e_field0)
+ *   moments    : [3][n_nodes] FP64 output (overwritten)
+ * Replaces electron_kernel (kernels/electron_kernel_numba.py:291-303) plus the
+ * deposit and compaction of post_processing (kernels/post_processing_1D.py:40-68,
+ * kernels/reduction/cuda_reduction/kernels-new.cu:56-132). */
+int bz_push_mcc(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
+                const void* E, const bz_params_t* p, const bz_xsec_t* xs,
+                const void* R, const void* gas_v, double* moments);
+
+/* Deposit only (no push): node moments of the current state, same layout
+ * as bz_push_mcc's output.  Replaces the initial post_processing call
+ * (1bin/boltzmann.py:363-371). */
+int bz_deposit(bz_ctx_t ctx, const bz_state_t* s, const bz_params_t* p, double* moments);
+
+/* Field step, FP64, on device: ni += ni_src; Te = 2/3 nrg/ne; g2 += ni_src;
+ * explicit drift-diffusion of ions / metastables; t += dt; tridiagonal
+ * Poisson solve at t; E (FP64) and E_t (context precision) written.
+ * Replaces boltzmann.py:643-647 (host moment update),
+ * kernels/heavies_kernel_1D.py:5-64 and kernels/EF_kernel_1D.py:8-44.
+ * chol: [2][N-1] inverse diagonal and lower band of the Cholesky factor. */
+int bz_field_step(bz_ctx_t ctx, const bz_field_params_t* fp, const double* moments,
+                  double* ni, double* ns, double* Te, double* g2, double* V,
+                  double* E, void* E_t, const double* chol, int32_t advance_time);
+
+/* Energy / cell binning and stable counting sort by key = cell*nb + ebin.
+ * thresholds: [nb-1] eV, ascending (bin b holds thr[b-1] <= e < thr[b]).
+ * counts[n_keys], offsets[n_keys+1], perm[n] (sorted position -> index).
+ * Replaces bin_particles (1bin/kernels/bin_particles.py:7-31,
+ * 3bin/kernels/bin_particles.py:39-78, 5bin/kernels/bin_particles.py:86-140). */
+int bz_bin_sort(bz_ctx_t ctx, const bz_state_t* s, const bz_params_t* p,
+                const double* thresholds_host, int32_t nb,
+                int32_t* counts, int32_t* offsets, uint32_t* perm);
+
+/* ES-WPNC resampling of the state sorted by bz_bin_sort: per (cell, bin)
+ * segment split or merge according to `pol`, written in key order to out.
+ * targets: [n_keys] int32 device array.  Count taken from / written to the
+ * clock.  Replaces _check_for_resampling + _do_resampling
+ * (1bin/boltzmann.py:799-916, 3bin/boltzmann.py:813-1113,
+ * 5bin/boltzmann.py:843-1334). */
+int bz_resample(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
+                const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
+                const uint32_t* perm, const int32_t* targets, int32_t n_keys,
+                const bz_resample_policy_t* pol);
+
+/* Host helper: Philox4x32-10 block (for tests and stream generation). */
+void bz_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BZ_H_ */

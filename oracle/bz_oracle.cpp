@@ -1,0 +1,516 @@
+// CPU oracle of the electron kinetic step -- TEST INFRASTRUCTURE ONLY.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load this library, and only as the checker (or as the timed CPU baseline).
+// It is a restatement, in plain sequential C++, of the reference algorithm:
+//
+//   collision + push + walls  1bin/kernels/electron_kernel_numba.py:39-287
+//   deposit                   1bin/kernels/post_processing_1D.py:44-56,
+//                             1bin/kernels/reduction/cuda_reduction/kernels-new.cu:56-132
+//   compaction                1bin/kernels/post_processing_1D.py:26-38
+//   host moment update        1bin/boltzmann.py:643-647
+//   heavies                   1bin/kernels/heavies_kernel_1D.py:5-64
+//   Poisson                   1bin/kernels/EF_kernel_1D.py:8-44
+//   binning                   1bin/kernels/bin_particles.py:7-31 (3bin :39-78, 5bin :86-140)
+//   resampling                1bin/boltzmann.py:799-916 (3bin :813-1113, 5bin :843-1334)
+//
+// plus the definitions this build adds on top of the reference: the Philox
+// stream, the tabulated three-channel null-collision model, the canonical
+// particle order (secondary right after its parent; segments in key order)
+// and the order-independent fixed-point selection of the resampler.  It is
+// written independently of the device code and compiled with
+// -ffp-contract=off so its arithmetic is plain IEEE in source order.
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10, Salmon et al. SC'11: round function
+//   (hi0,lo0) = M0*c0, (hi1,lo1) = M1*c2,
+//   c' = (hi1^c1^k0, lo1, hi0^c3^k1, lo0); keys bumped by the Weyl constants.
+// ---------------------------------------------------------------------------
+void orc_philox(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr_in[0], ctr_in[1], ctr_in[2], ctr_in[3]};
+  uint32_t k[2] = {key_in[0], key_in[1]};
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n[4] = {(uint32_t)(p1 >> 32) ^ c[1] ^ k[0], (uint32_t)p1,
+                           (uint32_t)(p0 >> 32) ^ c[3] ^ k[1], (uint32_t)p0};
+    memcpy(c, n, sizeof(c));
+    k[0] += 0x9E3779B9u;
+    k[1] += 0xBB67AE85u;
+  }
+  memcpy(out, c, sizeof(c));
+}
+
+static void orc_draw(uint64_t seed, uint32_t purpose, uint32_t index, uint32_t block,
+                     uint64_t step, uint32_t out[4]) {
+  const uint32_t ctr[4] = {index, block, (uint32_t)step,
+                           ((uint32_t)(step >> 32) << 8) | purpose};
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  orc_philox(ctr, key, out);
+}
+
+static double orc_u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
+}
+static float orc_u24(uint32_t a) { return (float)(a >> 8) / 16777216.0f; }
+
+// The collision uniforms of step `step` for particles 0..n-1, laid out as the
+// reference's R matrix transposed: R[c*stride + i], columns 0,1,2,3,5 used.
+void orc_collision_uniforms(uint64_t seed, uint64_t step, int64_t n, int prec, void* R,
+                            int64_t stride) {
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t b0[4], b1[4], b2[4];
+    orc_draw(seed, 0, (uint32_t)i, 0, step, b0);
+    orc_draw(seed, 0, (uint32_t)i, 1, step, b1);
+    if (prec == 8) {
+      orc_draw(seed, 0, (uint32_t)i, 2, step, b2);
+      double* r = (double*)R;
+      r[0 * stride + i] = orc_u53(b0[0], b0[1]);
+      r[1 * stride + i] = orc_u53(b0[2], b0[3]);
+      r[2 * stride + i] = orc_u53(b1[0], b1[1]);
+      r[3 * stride + i] = orc_u53(b1[2], b1[3]);
+      r[4 * stride + i] = 0.0;
+      r[5 * stride + i] = orc_u53(b2[0], b2[1]);
+      r[6 * stride + i] = 0.0;
+    } else {
+      float* r = (float*)R;
+      r[0 * stride + i] = orc_u24(b0[0]);
+      r[1 * stride + i] = orc_u24(b0[1]);
+      r[2 * stride + i] = orc_u24(b0[2]);
+      r[3 * stride + i] = orc_u24(b0[3]);
+      r[4 * stride + i] = 0.0f;
+      r[5 * stride + i] = orc_u24(b1[0]);
+      r[6 * stride + i] = 0.0f;
+    }
+  }
+}
+
+// Resampling draws: uniform m of segment `key` at step `step` (FP64).
+double orc_resample_uniform(uint64_t seed, uint64_t step, uint32_t key, uint32_t m) {
+  uint32_t b[4];
+  orc_draw(seed, 2, m >> 1, key, step, b);
+  return (m & 1) ? orc_u53(b[2], b[3]) : orc_u53(b[0], b[1]);
+}
+
+// ---------------------------------------------------------------------------
+// Parameters (filled by oracle/oracle.py from the physical inputs)
+// ---------------------------------------------------------------------------
+typedef struct {
+  double dt, xfac, afac, kev, jm, fel, L, dx, half_dx, L_dx, dep_M, nn, e_ion, e_exc, pc, e_field0;
+  int32_t dim, xs_mode, n_cells, n_nodes, bnf, nthreads;
+} orc_params;
+
+typedef struct {
+  const void* energy;
+  const void* sig;
+  const void* slope;
+  int32_t n;
+  int32_t pad;
+} orc_table;
+
+}  // extern "C"
+
+namespace {
+
+// Reference argon fits (electron_kernel_numba.py:67-171).
+const double el_i[11] = {0.000780396, 0.006098667, 0.0476269, 0.1015574, 0.1943527, 0.2995781,
+                         1.097156,    2.339524,    4.988691,  10.63765,  22.68322};
+const double el_a[12] = {-0.02704763, -0.08847237, -0.37608274, -1.4874467,
+                         -0.9870356,  14.28063581, -2.12069169, -0.3585636,
+                         1.25262128,  -2.28332905, -1.47508661, -0.11090525};
+const double el_b[12] = {-0.23720051, -0.6084786,  -1.8167778,  -4.82420619,
+                         -4.2206026,  18.92275458, 0.7555105,   0.79246666,
+                         -0.40002029, 4.89566076,  2.82263476,  -0.77506081};
+const double el_c[12] = {-19.67900951, -20.24111992, -21.51414308, -23.55745478,
+                         -23.44715988, -14.70556113, -19.71943671, -19.71260274,
+                         -19.48909003, -21.46789173, -20.1928456,  -17.81752307};
+const double io_i[7] = {15.77742, 15.81964, 15.96196, 16.44162, 18.05837, 23.50771, 41.87498};
+const double io_a[8] = {-2149239.99337822, -227229.83440255, -20173.68563579, -1838.90888364,
+                        -186.98923927,     -24.48843901,     -4.33101714,     -0.68345667};
+const double io_b[8] = {5150451.29014456, 545171.14839945, 48599.13254342, 4491.58838596,
+                        477.04784782,     69.43965607,     14.47124519,    2.6053539};
+const double io_c[8] = {-3085665.64874862, -327016.69508374, -29290.96100603, -2763.82866252,
+                        -324.75732578,     -69.145292411,    -31.66224458,    -21.99823511};
+
+inline double m_log10(double v) { return log10(v); }
+inline float m_log10(float v) { return log10f(v); }
+inline double m_p10(double v) { return pow(10.0, v); }
+inline float m_p10(float v) { return powf(10.0f, v); }
+inline double m_exp(double v) { return exp(v); }
+inline float m_exp(float v) { return expf(v); }
+inline double m_sqrt(double v) { return sqrt(v); }
+inline float m_sqrt(float v) { return sqrtf(v); }
+inline double m_sin(double v) { return sin(v); }
+inline float m_sin(float v) { return sinf(v); }
+inline double m_cos(double v) { return cos(v); }
+inline float m_cos(float v) { return cosf(v); }
+
+template <typename T>
+T poly(double a, double b, double c, T L) {
+  return ((T)a * L + (T)b) * L + (T)c;
+}
+
+// The reference sums indicator * polynomial over the intervals.
+template <typename T>
+void fit_cross_sections(T e, const orc_params& P, T* s_el, T* s_ion) {
+  const T L = m_log10(e);
+  T lg0 = (T)0;
+  for (int k = 0; k < 12; ++k) {
+    const bool lo = (k == 0) || ((T)el_i[k - 1] <= e);
+    const bool hi = (k == 11) || (e < (T)el_i[k]);
+    if (lo && hi) lg0 = lg0 + poly<T>(el_a[k], el_b[k], el_c[k], L);
+  }
+  T lg2 = (T)0;
+  for (int k = 0; k < 8; ++k) {
+    const bool lo = (k == 0) ? ((T)P.e_ion < e) : ((T)io_i[k - 1] <= e);
+    const bool hi = (k == 7) || (e < (T)io_i[k]);
+    if (lo && hi) lg2 = lg2 + poly<T>(io_a[k], io_b[k], io_c[k], L);
+  }
+  T s0 = m_p10(lg0);
+  T s2 = m_p10(lg2);
+  if (e < (T)P.e_ion) s2 = (T)0;
+  *s_el = s0 * (T)P.nn;
+  *s_ion = s2 * (T)P.nn;
+}
+
+template <typename T>
+void table_cross_sections(T e, const orc_params& P, const orc_table& tab, T s[3]) {
+  const T* E = (const T*)tab.energy;
+  const T* S = (const T*)tab.sig;
+  const T* D = (const T*)tab.slope;
+  const int n = tab.n;
+  T ec = e;
+  if (ec < E[0]) ec = E[0];
+  if (ec > E[n - 1]) ec = E[n - 1];
+  // largest j in [0, n-2] with E[j] <= ec
+  int lo = 0, hi = n - 2;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (E[mid] <= ec) lo = mid;
+    else hi = mid - 1;
+  }
+  const T d = ec - E[lo];
+  for (int c = 0; c < 3; ++c) s[c] = S[c * n + lo] + D[c * n + lo] * d;
+  if (e < (T)P.e_exc) s[1] = (T)0;
+  if (e < (T)P.e_ion) s[2] = (T)0;
+}
+
+template <typename T>
+void scatter(T speed, T ra, T rb, T* vx, T* vy, T* vz) {
+  const T cos_chi = (T)2 * ra - (T)1;
+  T q = (T)1 - cos_chi * cos_chi;
+  if (q < (T)0) q = (T)0;
+  const T sin_chi = m_sqrt(q);
+  const T phi = (T)(2.0 * 3.141592653589793) * rb;
+  *vx = speed * sin_chi * m_cos(phi);
+  *vy = speed * sin_chi * m_sin(phi);
+  *vz = speed * cos_chi;
+}
+
+template <typename T>
+struct Rec {
+  T x, vx, vy, vz, w;
+};
+
+int node_of(double x, const orc_params& P) {
+  if (P.dim == 0) return 0;
+  const double tx = (x + P.half_dx) / P.L_dx;
+  const int fine = (int)(tx * P.dep_M);
+  int k = fine / P.bnf;
+  if (k < 0) k = 0;
+  if (k >= P.n_nodes) k = P.n_nodes - 1;
+  return k;
+}
+
+// One particle; returns number of output records (0, 1 or 2) in out[].
+template <typename T>
+int one_particle(int64_t i, const orc_params& P, const T* x, const T* vx, const T* vy,
+                 const T* vz, const T* w, const T* E, const T* R, int64_t rs, const T* G,
+                 const orc_table* tab, Rec<T> out[2], T e_out[2], int* chan, int* ionized) {
+  *chan = 0;
+  *ionized = 0;
+  const T wi = w[i];
+  if (!(wi > (T)0)) return 0;
+  T gx = 0, gy = 0, gz = 0;
+  if (G) {
+    gx = G[i];
+    gy = G[rs + i];
+    gz = G[2 * rs + i];
+  }
+  T ux = vx[i] - gx, uy = vy[i] - gy, uz = vz[i] - gz;
+  T e = ux * ux + uy * uy + uz * uz;
+  const T v_mag = m_sqrt(e);
+  e = (T)P.kev * e;
+  const T R0 = R[i], R1 = R[rs + i], R2 = R[2 * rs + i], R3 = R[3 * rs + i],
+          R5 = R[5 * rs + i];
+  int c = 0;
+  if (P.xs_mode == 0) {
+    T s0, s2;
+    fit_cross_sections<T>(e, P, &s0, &s2);
+    const T tot = s0 + s2;
+    const T Pc = (T)1 - m_exp(-(T)P.dt * v_mag * tot);
+    const T pcst = Pc * ((T)1 / tot);
+    const T g0 = s0 * pcst;
+    const T g2 = g0 + s2 * pcst;
+    if (R0 < g0) c = 1;
+    else if (g0 < R0 && R0 < g2) c = 3;
+  } else {
+    T s[3];
+    table_cross_sections<T>(e, P, *tab, s);
+    const T pc = (T)P.pc;
+    const T g0 = (s[0] * v_mag) * pc;
+    const T g1 = g0 + (s[1] * v_mag) * pc;
+    const T g2 = g1 + (s[2] * v_mag) * pc;
+    if (R0 < g0) c = 1;
+    else if (R0 < g1) c = 2;
+    else if (R0 < g2) c = 3;
+  }
+  *chan = c;
+  T nvx = vx[i], nvy = vy[i], nvz = vz[i];
+  int born = 0;
+  Rec<T> sec = {0, 0, 0, 0, 0};
+  T e_sec = 0;
+  if (c == 1) {
+    scatter<T>(v_mag * (T)P.fel, R1, R2, &nvx, &nvy, &nvz);
+  } else if (c == 2) {
+    const T e_new = e - (T)P.e_exc;
+    scatter<T>(m_sqrt(((T)2 * e_new) / (T)P.jm), R1, R2, &nvx, &nvy, &nvz);
+  } else if (c == 3) {
+    T e_ej = (T)0.5 * (e - (T)P.e_ion);
+    if (e_ej < (T)0) e_ej = -e_ej;
+    const T sp = m_sqrt(((T)2 * e_ej) / (T)P.jm);
+    scatter<T>(sp, R1, R2, &nvx, &nvy, &nvz);
+    T tx, ty, tz;
+    scatter<T>(sp, R3, R5, &tx, &ty, &tz);
+    born = 1;
+    sec.x = x[i];
+    sec.vx = tx;
+    sec.vy = ty;
+    sec.vz = tz;
+    sec.w = wi;
+    e_sec = (T)P.kev * (tx * tx + ty * ty + tz * tz);
+  }
+  T nx = x[i];
+  T ef;
+  if (P.dim == 1) {
+    nx = nx + (T)P.xfac * nvx;
+    int cell = (int)((double)x[i] / P.dx);
+    if (cell < 0) cell = 0;
+    if (cell >= P.n_cells) cell = P.n_cells - 1;
+    ef = E[cell];
+  } else {
+    ef = E ? E[0] : (T)P.e_field0;
+  }
+  nvx = nvx - (T)P.afac * ((T)100 * ef);
+  const bool alive = (P.dim == 0) || ((T)0 < nx && nx < (T)P.L);
+  int k = 0;
+  if (alive) {
+    out[k] = Rec<T>{nx, nvx, nvy, nvz, wi};
+    e_out[k] = (T)P.kev * (nvx * nvx + nvy * nvy + nvz * nvz);
+    ++k;
+    *ionized = (c == 3);
+  }
+  if (born) {
+    out[k] = sec;
+    e_out[k] = e_sec;
+    ++k;
+  }
+  return k;
+}
+
+template <typename T>
+int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const T* vy,
+                  const T* vz, const T* w, const T* E, const T* R, int64_t rs, const T* G,
+                  const orc_table* tab, T* ox, T* ovx, T* ovy, T* ovz, T* ow, int64_t cap,
+                  double* mom, uint64_t* cnt) {
+  const int nn = P->dim == 1 ? P->n_nodes : 1;
+  int nt = P->nthreads > 0 ? P->nthreads : 1;
+#ifndef _OPENMP
+  nt = 1;
+#endif
+  // fixed chunking: chunk k handles [k*n/nt, (k+1)*n/nt)
+  std::vector<int64_t> chunk_cnt(nt + 1, 0);
+  std::vector<std::vector<double>> part(nt, std::vector<double>(3 * nn, 0.0));
+  std::vector<std::vector<uint64_t>> ccnt(nt, std::vector<uint64_t>(8, 0));
+  std::vector<std::vector<Rec<T>>> buf(nt);
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+  for (int k = 0; k < nt; ++k) {
+    const int64_t lo = n * k / nt, hi = n * (k + 1) / nt;
+    std::vector<Rec<T>>& b = buf[k];
+    b.reserve((size_t)(hi - lo) + 16);
+    double* pm = part[k].data();
+    uint64_t* cc = ccnt[k].data();
+    for (int64_t i = lo; i < hi; ++i) {
+      Rec<T> o[2];
+      T eo[2];
+      int chan, ion;
+      const int m = one_particle<T>(i, *P, x, vx, vy, vz, w, E, R, rs, G, tab, o, eo, &chan, &ion);
+      if (w[i] > (T)0) {
+        cc[chan == 0 ? 5 : (chan == 1 ? 2 : (chan == 2 ? 3 : 4))] += 1;
+        if (chan == 3) cc[0] += 1;
+        const bool alive_primary = (m == 2) || (m == 1 && chan != 3);
+        if (!alive_primary) cc[1] += 1;
+      }
+      for (int q = 0; q < m; ++q) {
+        const int nd = node_of((double)o[q].x, *P);
+        const double wd = (double)o[q].w;
+        pm[nd] += wd;
+        if (q == 0 && ion) pm[nn + nd] += wd;
+        pm[2 * nn + nd] += wd * (double)eo[q];
+        b.push_back(o[q]);
+      }
+    }
+    chunk_cnt[k + 1] = (int64_t)b.size();
+  }
+  for (int k = 0; k < nt; ++k) chunk_cnt[k + 1] += chunk_cnt[k];
+  const int64_t total = chunk_cnt[nt];
+  uint64_t dropped = 0;
+#pragma omp parallel for num_threads(nt) schedule(static, 1) reduction(+ : dropped)
+  for (int k = 0; k < nt; ++k) {
+    int64_t pos = chunk_cnt[k];
+    for (const Rec<T>& r : buf[k]) {
+      if (pos < cap) {
+        ox[pos] = r.x;
+        ovx[pos] = r.vx;
+        ovy[pos] = r.vy;
+        ovz[pos] = r.vz;
+        ow[pos] = r.w;
+      } else {
+        ++dropped;
+      }
+      ++pos;
+    }
+  }
+  for (int j = 0; j < 3 * nn; ++j) {
+    double s = 0.0;
+    for (int k = 0; k < nt; ++k) s += part[k][j];
+    mom[j] = s;
+  }
+  for (int q = 0; q < 8; ++q) {
+    uint64_t s = 0;
+    for (int k = 0; k < nt; ++k) s += ccnt[k][q];
+    cnt[q] = s;
+  }
+  cnt[6] = dropped;
+  return total < cap ? total : cap;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t orc_push_step_f64(const orc_params* P, int64_t n, const double* x, const double* vx,
+                          const double* vy, const double* vz, const double* w, const double* E,
+                          const double* R, int64_t rs, const double* G, const orc_table* tab,
+                          double* ox, double* ovx, double* ovy, double* ovz, double* ow,
+                          int64_t cap, double* mom, uint64_t* cnt) {
+  return push_step<double>(P, n, x, vx, vy, vz, w, E, R, rs, G, tab, ox, ovx, ovy, ovz, ow, cap,
+                           mom, cnt);
+}
+
+int64_t orc_push_step_f32(const orc_params* P, int64_t n, const float* x, const float* vx,
+                          const float* vy, const float* vz, const float* w, const float* E,
+                          const float* R, int64_t rs, const float* G, const orc_table* tab,
+                          float* ox, float* ovx, float* ovy, float* ovz, float* ow, int64_t cap,
+                          double* mom, uint64_t* cnt) {
+  return push_step<float>(P, n, x, vx, vy, vz, w, E, R, rs, G, tab, ox, ovx, ovy, ovz, ow, cap,
+                          mom, cnt);
+}
+
+// Deposit of a state (no push): ne, ni_src (=0), nrg per node.
+void orc_deposit_f64(const orc_params* P, int64_t n, const double* x, const double* vx,
+                     const double* vy, const double* vz, const double* w, double* mom) {
+  const int nn = P->dim == 1 ? P->n_nodes : 1;
+  for (int j = 0; j < 3 * nn; ++j) mom[j] = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!(w[i] > 0.0)) continue;
+    const double e = P->kev * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    const int nd = node_of(x[i], *P);
+    mom[nd] += w[i];
+    mom[2 * nn + nd] += w[i] * e;
+  }
+}
+
+void orc_deposit_f32(const orc_params* P, int64_t n, const float* x, const float* vx,
+                     const float* vy, const float* vz, const float* w, double* mom) {
+  const int nn = P->dim == 1 ? P->n_nodes : 1;
+  for (int j = 0; j < 3 * nn; ++j) mom[j] = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!(w[i] > 0.0f)) continue;
+    const float e = (float)P->kev * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    const int nd = node_of((double)x[i], *P);
+    mom[nd] += (double)w[i];
+    mom[2 * nn + nd] += (double)w[i] * (double)e;
+  }
+}
+
+// Grid step: moment update, heavies, Poisson (FP64).
+//   chol = [diag(N-1), lower(N-1)] of the banded Cholesky factor (upper form,
+//   as returned by scipy.linalg.cholesky_banded for boltzmann.py:744-755).
+void orc_field_step(int N, double dx, double dt, double eps, double v0, double freq, double mu_i,
+                    double D_i, double D_s, int fluid_on, int poisson_on, double t,
+                    const double* mom, double* ni, double* ns, double* Te, double* g2, double* V,
+                    double* E, const double* chol) {
+  const int nn = N + 1;
+  const double* ne = mom;
+  const double* nsrc = mom + nn;
+  const double* nrg = mom + 2 * nn;
+  for (int i = 0; i < nn; ++i) {
+    ni[i] = ni[i] + nsrc[i];
+    double te = (2.0 / 3.0) * nrg[i] / ne[i];
+    Te[i] = (te != te) ? 0.0 : te;
+    g2[i] = g2[i] + nsrc[i];
+  }
+  if (fluid_on) {
+    std::vector<double> Ji(N), Js(N);
+    auto bc = [&]() {
+      ni[0] = ni[1] * ((D_i / dx - 0.5 * mu_i * std::max(E[0], 0.0)) /
+                       (D_i / dx + 0.5 * mu_i * std::max(E[0], 0.0)));
+      ni[N] = ni[N - 1] * ((D_i / dx + 0.5 * mu_i * std::min(E[N - 1], 0.0)) /
+                           (D_i / dx - 0.5 * mu_i * std::min(E[N - 1], 0.0)));
+      ns[0] = 0;
+      ns[N] = 0;
+    };
+    bc();
+    for (int i = 0; i < N; ++i) {
+      Ji[i] = mu_i * (0.5 * (ni[i] + ni[i + 1])) * E[i] - D_i * (ni[i + 1] - ni[i]) / dx;
+      Js[i] = -D_s * (ns[i + 1] - ns[i]) / dx;
+    }
+    for (int i = 1; i < N; ++i) {
+      ni[i] = ni[i] - dt * (Ji[i] - Ji[i - 1]) / dx;
+      ns[i] = ns[i] - dt * (Js[i] - Js[i - 1]) / dx;
+    }
+    bc();
+  }
+  if (poisson_on) {
+    const double* diag = chol;
+    const double* low = chol + (N - 1);
+    std::vector<double> rhs(N - 1), ty(N - 1);
+    for (int i = 0; i < N - 1; ++i) rhs[i] = (1.0 / eps) * (ni[i + 1] - ne[i + 1]);
+    rhs[0] = rhs[0] + (1.0 / (dx * dx)) * v0 * sin(2 * 3.141592653589793 * freq * t);
+    ty[0] = rhs[0] / diag[0];
+    for (int i = 1; i < N - 1; ++i) ty[i] = (1.0 / diag[i]) * (rhs[i] - low[i] * ty[i - 1]);
+    V[N - 2] = ty[N - 2] / diag[N - 2];
+    for (int i = 1; i < N - 1; ++i)
+      V[N - 2 - i] = (1.0 / diag[N - 2 - i]) * (ty[N - 2 - i] - low[N - 2 - i + 1] * V[N - 2 - i + 1]);
+    for (int i = 0; i < N - 2; ++i) E[i + 1] = -(V[i + 1] - V[i]) / dx;
+    E[0] = -(V[0] - v0 * sin(2 * 3.141592653589793 * freq * t)) / dx;
+    E[N - 1] = -(0 - V[N - 2]) / dx;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+"""numpy front end of the CPU oracle (test infrastructure only).
+
+The constants the oracle needs are derived here from the physical inputs,
+written out from the reference's formulas (electron_kernel_numba.py:14-17,
+:63, :206, :226, :258, :264; post_processing_1D.py:44; boltzmann.py:66-71),
+not taken from the product's set-up code, so a mistake there shows up as a
+parity failure.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import build as _build
+
+_lib = None
+
+
+class OrcParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "dt", "xfac", "afac", "kev", "jm", "fel", "L", "dx", "half_dx", "L_dx", "dep_M", "nn",
+        "e_ion", "e_exc", "pc", "e_field0")] + [
+        (n, C.c_int32) for n in ("dim", "xs_mode", "n_cells", "n_nodes", "bnf", "nthreads")]
+
+
+class OrcTable(C.Structure):
+    _fields_ = [("energy", C.c_void_p), ("sig", C.c_void_p), ("slope", C.c_void_p),
+                ("n", C.c_int32), ("pad", C.c_int32)]
+
+
+def load():
+    global _lib
+    if _lib is None:
+        path = _build.ensure_built()
+        _lib = C.CDLL(str(path))
+        _lib.orc_philox.restype = None
+        _lib.orc_resample_uniform.restype = C.c_double
+        _lib.orc_resample_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
+        _lib.orc_collision_uniforms.restype = None
+        _lib.orc_collision_uniforms.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_int,
+                                                C.c_void_p, C.c_int64]
+        for nm in ("orc_push_step_f64", "orc_push_step_f32"):
+            getattr(_lib, nm).restype = C.c_int64
+        _lib.orc_field_step.restype = None
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def philox(ctr, key):
+    lib = load()
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.orc_philox(c, k, o)
+    return tuple(o)
+
+
+# Physical constants as the reference writes them.
+Q_E = 1.602 * 10 ** (-19)
+M_E = 9.10938 * 10 ** (-31)
+M_N = 6.6 * 10 ** (-26)
+J_EV = 6.241509 * (10 ** 18)
+
+
+@dataclass
+class Setup:
+    """Physical inputs of one oracle run."""
+    dim: int
+    dt: float
+    xs_mode: int = 1           # 0 reference fits, 1 table
+    n_cells: int = 1
+    length_cm: float = 2.54
+    bnf: int = 4
+    gas_density_m3: float = 3.22e21
+    mass_ratio: float = 1.36e-5
+    e_ion: float = 15.76
+    e_exc: float = 11.55
+    nu_max: float = 0.0
+    e_field0: float = 0.0
+    nthreads: int = 1
+
+    def params(self) -> OrcParams:
+        p = OrcParams()
+        p.dt = self.dt
+        p.xfac = 100 * self.dt
+        p.afac = self.dt * (Q_E / M_E)
+        p.kev = 0.5 * M_E * J_EV
+        p.jm = J_EV * M_E
+        p.fel = math.sqrt(1 - (2 * M_E / M_N)) if self.xs_mode == 0 else \
+            math.sqrt(1 - 2 * self.mass_ratio)
+        n = self.n_cells if self.dim == 1 else 1
+        p.L = self.length_cm
+        p.dx = self.length_cm / n
+        p.half_dx = 0.5 * p.dx
+        p.L_dx = self.length_cm + p.dx
+        p.dep_M = float(self.bnf * (n + 1))
+        p.nn = self.gas_density_m3
+        p.e_ion = self.e_ion
+        p.e_exc = self.e_exc
+        if self.nu_max > 0:
+            p_null = 1.0 - math.exp(-self.nu_max * self.dt)
+            p.pc = p_null / self.nu_max
+        p.e_field0 = self.e_field0
+        p.dim = self.dim
+        p.xs_mode = self.xs_mode
+        p.n_cells = n
+        p.n_nodes = n + 1 if self.dim == 1 else 1
+        p.bnf = self.bnf
+        p.nthreads = self.nthreads
+        return p
+
+
+def collision_uniforms(seed: int, step: int, n: int, dtype) -> np.ndarray:
+    """[7, n] uniforms of the Philox stream (columns 0,1,2,3,5 used)."""
+    lib = load()
+    R = np.zeros((7, max(n, 1)), dtype=dtype)
+    lib.orc_collision_uniforms(C.c_uint64(seed), C.c_uint64(step), n,
+                               8 if dtype == np.float64 else 4, _ptr(R), R.shape[1])
+    return R
+
+
+def resample_uniform(seed: int, step: int, key: int, m: int) -> float:
+    return load().orc_resample_uniform(seed, step, key, m)
+
+
+def push_step(setup: Setup, state, E, R, G=None, table=None, capacity=None):
+    """One collision + push + wall + deposit + compaction step.
+
+    state: (x, vx, vy, vz, w) arrays of one dtype; R: [7, stride] uniforms;
+    G: optional [3, stride] gas velocities; table: (energy, sig[3,n], slope[3,n]).
+    Returns (new_state, moments[3, n_nodes], counters[8])."""
+    lib = load()
+    x, vx, vy, vz, w = [np.ascontiguousarray(a) for a in state]
+    dt = x.dtype
+    n = x.size
+    cap = capacity if capacity is not None else 2 * n + 16
+    out = [np.zeros(cap, dtype=dt) for _ in range(5)]
+    p = setup.params()
+    nn = p.n_nodes
+    mom = np.zeros(3 * nn)
+    cnt = np.zeros(8, dtype=np.uint64)
+    tab = None
+    keep = []
+    if table is not None:
+        e, s, sl = [np.ascontiguousarray(a, dtype=dt) for a in table]
+        keep += [e, s, sl]
+        tab = OrcTable(energy=_ptr(e), sig=_ptr(s), slope=_ptr(sl), n=e.size)
+    Ea = None if E is None else np.ascontiguousarray(E, dtype=dt)
+    Ra = np.ascontiguousarray(R, dtype=dt)
+    Ga = None if G is None else np.ascontiguousarray(G, dtype=dt)
+    rs = Ra.shape[1]
+    if Ga is not None and Ga.shape[1] != rs:
+        raise ValueError("R and G must share the column stride")
+    fn = lib.orc_push_step_f64 if dt == np.float64 else lib.orc_push_step_f32
+    m = fn(C.byref(p), C.c_int64(n), *[_ptr(a) for a in (x, vx, vy, vz, w)], _ptr(Ea), _ptr(Ra),
+           C.c_int64(rs), _ptr(Ga), C.byref(tab) if tab is not None else None,
+           *[_ptr(a) for a in out], C.c_int64(cap), _ptr(mom), _ptr(cnt))
+    return tuple(a[:m] for a in out), mom.reshape(3, nn), cnt
+
+
+def deposit(setup: Setup, state):
+    lib = load()
+    x, vx, vy, vz, w = [np.ascontiguousarray(a) for a in state]
+    p = setup.params()
+    mom = np.zeros(3 * p.n_nodes)
+    fn = lib.orc_deposit_f64 if x.dtype == np.float64 else lib.orc_deposit_f32
+    fn(C.byref(p), C.c_int64(x.size), *[_ptr(a) for a in (x, vx, vy, vz, w)], _ptr(mom))
+    return mom.reshape(3, p.n_nodes)
+
+
+def field_step(N, dx, dt, eps, v0, freq, mu_i, D_i, D_s, fluid_on, poisson_on, t, mom, ni, ns,
+               Te, g2, V, E, chol):
+    """In-place FP64 grid step (arrays are modified)."""
+    lib = load()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (mom, ni, ns, Te, g2, V, E, chol)]
+    lib.orc_field_step(C.c_int(N), *[C.c_double(v) for v in (dx, dt, eps, v0, freq, mu_i, D_i,
+                                                               D_s)],
+                       C.c_int(fluid_on), C.c_int(poisson_on), C.c_double(t),
+                       *[_ptr(a) for a in arrs])
+    return arrs[1:7]

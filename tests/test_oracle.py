@@ -1,0 +1,327 @@
+"""Pins of the CPU oracle against closed forms, invariants, published
+known-answer vectors and brute force (the reference ships no tests or
+fixtures for this path)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+# Random123 known-answer vectors for philox4x32_10 (kat_vectors file of the
+# Random123 distribution, Salmon et al. SC'11).
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", PHILOX_KAT)
+def test_philox_kat(ctr, key, expect):
+    assert O.philox(ctr, key) == expect
+
+
+def _u53(a, b):
+    return ((a >> 5) * 67108864 + (b >> 6)) / 2.0 ** 53
+
+
+def test_collision_uniform_layout():
+    seed, step, n = 0x1234_5678_9ABC, 7, 5
+    R64 = O.collision_uniforms(seed, step, n, np.float64)
+    R32 = O.collision_uniforms(seed, step, n, np.float32)
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    for i in range(n):
+        b = [O.philox((i, blk, step, 0), key) for blk in range(3)]
+        want64 = [_u53(b[0][0], b[0][1]), _u53(b[0][2], b[0][3]), _u53(b[1][0], b[1][1]),
+                  _u53(b[1][2], b[1][3]), _u53(b[2][0], b[2][1])]
+        assert list(R64[[0, 1, 2, 3, 5], i]) == want64
+        want32 = [(b[0][k] >> 8) / 2.0 ** 24 for k in range(4)] + [(b[1][0] >> 8) / 2.0 ** 24]
+        assert list(R32[[0, 1, 2, 3, 5], i].astype(np.float64)) == want32
+    assert np.all(R64 >= 0) and np.all(R64 < 1)
+
+
+def test_uniform_moments():
+    R = O.collision_uniforms(99, 3, 200_000, np.float64)[0]
+    assert abs(R.mean() - 0.5) < 4 * math.sqrt(1 / 12 / R.size)
+    assert abs(R.var() - 1 / 12) < 5e-3 / 12
+
+
+# ---------------------------------------------------------------------------
+def _state(n, dtype, dim, L=2.5, e_ev=5.0, seed=1):
+    rng = np.random.default_rng(seed)
+    sv = math.sqrt(e_ev * 1.602e-19 / 9.10938e-31 / 3)
+    v = rng.normal(0, sv, size=(3, n))
+    x = rng.uniform(0.01 * L, 0.99 * L, n) if dim == 1 else np.full(n, 0.5 * L)
+    w = rng.uniform(0.5, 2.0, n)
+    return tuple(np.ascontiguousarray(a, dtype=dtype) for a in (x, v[0], v[1], v[2], w))
+
+
+def _table(dtype, gas=3.2e22):
+    from boltzmann_amd import physics as ph
+    t = ph.build_table(ph.XSecModel(), gas, dtype)
+    return t, (t.energy, t.sig, t.slope)
+
+
+KEV = 0.5 * O.M_E * O.J_EV
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_free_push_closed_form(dtype):
+    """R0 = 1-eps: no collision; x' = x + 100 dt v, vx' = vx - dt q/m 100 E[int(x/dx)]."""
+    n, N, L, dt = 2000, 64, 2.5, 2e-11
+    t, tab = _table(dtype)
+    s = O.Setup(dim=1, dt=dt, n_cells=N, length_cm=L, nu_max=t.nu_max, gas_density_m3=3.2e22)
+    st = _state(n, dtype, 1, L)
+    E = np.linspace(-50, 80, N).astype(dtype)
+    R = np.full((7, n), 0.9999, dtype=dtype)
+    out, mom, cnt = O.push_step(s, st, E, R, table=tab)
+    x, vx, vy, vz, w = [a.astype(np.float64) for a in st]
+    cell = np.clip((x / (L / N)).astype(int), 0, N - 1)
+    xn = x + 100 * dt * vx
+    vxn = vx - dt * (O.Q_E / O.M_E) * (100 * E.astype(np.float64)[cell])
+    keep = (xn > 0) & (xn < L)
+    assert out[0].size == keep.sum()
+    assert cnt[1] == (~keep).sum() and cnt[5] == n and cnt[0] == 0
+    rt = 1e-12 if dtype == np.float64 else 2e-6
+    np.testing.assert_allclose(out[0], xn[keep], rtol=rt)
+    kick = np.abs(dt * (O.Q_E / O.M_E) * (100 * E.astype(np.float64)[cell]))[keep]
+    assert np.all(np.abs(out[1] - vxn[keep]) <= rt * (np.abs(vx[keep]) + kick) + 1e-9)
+    np.testing.assert_array_equal(out[2], st[2][keep])
+    np.testing.assert_array_equal(out[4], st[4][keep])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_elastic_scatter_geometry(dtype):
+    """R0 = 0 forces the first channel: speed scales by sqrt(1-2 m_e/M),
+    cos(chi) = 2 R1 - 1 and phi = 2 pi R2 (electron_kernel_numba.py:197-209)."""
+    n = 3000
+    t, tab = _table(dtype)
+    s = O.Setup(dim=0, dt=1e-12, nu_max=t.nu_max, gas_density_m3=3.2e22, e_field0=0.0)
+    st = _state(n, dtype, 0)
+    rng = np.random.default_rng(5)
+    R = rng.uniform(0, 1, size=(7, n)).astype(dtype)
+    R[0] = 0.0
+    out, mom, cnt = O.push_step(s, st, None, R, table=tab)
+    assert cnt[2] == n and out[0].size == n
+    v0 = np.sqrt(sum(a.astype(np.float64) ** 2 for a in st[1:4]))
+    v1 = np.sqrt(sum(a.astype(np.float64) ** 2 for a in out[1:4]))
+    rt = 1e-13 if dtype == np.float64 else 1e-6
+    np.testing.assert_allclose(v1, v0 * math.sqrt(1 - 2 * 1.36e-5), rtol=rt)
+    np.testing.assert_allclose(out[3] / v1, 2 * R[1].astype(np.float64) - 1, atol=1e-6)
+    phi = np.mod(np.arctan2(out[2].astype(np.float64), out[1].astype(np.float64)), 2 * np.pi)
+    d = np.abs(phi - 2 * np.pi * R[2].astype(np.float64))
+    d = np.minimum(d, 2 * np.pi - d)
+    sin_chi = np.sqrt(np.maximum(0, 1 - (2 * R[1].astype(np.float64) - 1) ** 2))
+    ok = sin_chi > 1e-3
+    assert np.all(d[ok] < 1e-4)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_channel_energy_balance(dtype):
+    """At 40 eV with random R: every output has the energy of exactly one
+    channel; ionization pairs sit together, share (e - 15.76)/2 each, and
+    the secondary keeps the parent's pre-push x and weight."""
+    n = 20000
+    t, tab = _table(dtype, gas=3.2e24)
+    s = O.Setup(dim=1, dt=1e-12, n_cells=16, length_cm=2.5, nu_max=t.nu_max,
+                gas_density_m3=3.2e24)
+    rng = np.random.default_rng(11)
+    sp = math.sqrt(40.0 / KEV)
+    dirs = rng.normal(size=(3, n))
+    dirs /= np.linalg.norm(dirs, axis=0)
+    st = (rng.uniform(0.5, 2.0, n), *(sp * dirs), rng.uniform(0.5, 2, n))
+    st = tuple(np.ascontiguousarray(a, dtype=dtype) for a in st)
+    E = np.zeros(16, dtype=dtype)
+    R = rng.uniform(0, 1, size=(7, n)).astype(dtype)
+    out, mom, cnt = O.push_step(s, st, E, R, table=tab)
+    e0 = KEV * np.sum(np.stack(st[1:4]).astype(np.float64) ** 2, axis=0)
+    e1 = KEV * np.sum(np.stack(out[1:4]).astype(np.float64) ** 2, axis=0)
+    rt = 1e-9 if dtype == np.float64 else 1e-5
+    # walk outputs in canonical order
+    j = 0
+    counts = dict(el=0, ex=0, io=0, none=0)
+    for i in range(n):
+        e = e0[i]
+        f = e1[j]
+        if abs(f - e * (1 - 2 * 1.36e-5)) <= rt * e and abs(f - e) > rt * e:
+            counts["el"] += 1
+            j += 1
+        elif abs(f - e) <= rt * e:
+            counts["none"] += 1
+            j += 1
+        elif abs(f - (e - 11.55)) <= rt * e:
+            counts["ex"] += 1
+            j += 1
+        elif abs(f - 0.5 * (e - 15.76)) <= rt * e:
+            counts["io"] += 1
+            assert abs(e1[j + 1] - 0.5 * (e - 15.76)) <= rt * e
+            assert out[0][j + 1] == st[0][i] and out[4][j + 1] == st[4][i]
+            j += 2
+        else:
+            raise AssertionError(f"particle {i}: energy {f} matches no channel (from {e})")
+    assert j == out[0].size
+    assert counts["el"] == cnt[2] and counts["ex"] == cnt[3] and counts["io"] == cnt[4]
+    assert cnt[0] == cnt[4]
+    # channel frequencies follow P_null * nu_c / nu_max (binomial bounds)
+    sig = np.array([np.interp(40.0, t.energy.astype(np.float64), t.sig[c].astype(np.float64))
+                    for c in range(3)])
+    nu = sig * sp
+    pn = 1 - math.exp(-t.nu_max * 1e-12)
+    for c, key in enumerate(("el", "ex", "io")):
+        p = pn * nu[c] / t.nu_max
+        assert abs(counts[key] - n * p) < 5 * math.sqrt(n * p * (1 - p)) + 1
+
+
+def test_fit_cross_sections_threshold():
+    """Reference fit model: the elastic decision flips exactly at
+    R0 = P sigma_el / sigma_t computed from electron_kernel_numba.py:115-195."""
+    n = 400
+    rng = np.random.default_rng(3)
+    e = np.exp(rng.uniform(np.log(0.5), np.log(14.0), n))  # below ionization
+    sp = np.sqrt(e / KEV)
+    st = (np.full(n, 1.0), sp, np.zeros(n), np.zeros(n), np.ones(n))
+    nn, dt = 3.22e21, 1e-11
+    # elastic fit, as the reference writes it
+    a = [-0.02704763, -0.08847237, -0.37608274, -1.4874467, -0.9870356, 14.28063581,
+         -2.12069169, -0.3585636, 1.25262128, -2.28332905, -1.47508661, -0.11090525]
+    b = [-0.23720051, -0.6084786, -1.8167778, -4.82420619, -4.2206026, 18.92275458,
+         0.7555105, 0.79246666, -0.40002029, 4.89566076, 2.82263476, -0.77506081]
+    c = [-19.67900951, -20.24111992, -21.51414308, -23.55745478, -23.44715988, -14.70556113,
+         -19.71943671, -19.71260274, -19.48909003, -21.46789173, -20.1928456, -17.81752307]
+    brk = [0.000780396, 0.006098667, 0.0476269, 0.1015574, 0.1943527, 0.2995781, 1.097156,
+           2.339524, 4.988691, 10.63765, 22.68322]
+    ee = KEV * sp * sp
+    seg = np.searchsorted(brk, ee, side="right")
+    L = np.log10(ee)
+    lg = (np.take(a, seg) * L + np.take(b, seg)) * L + np.take(c, seg)
+    sig = 10.0 ** lg * nn
+    P = 1 - np.exp(-dt * np.sqrt(sp * sp) * sig)
+    thr = sig * (P * (1.0 / sig))
+    s = O.Setup(dim=0, dt=dt, xs_mode=0, gas_density_m3=nn)
+    for factor, expect_el in ((1 - 1e-9, True), (1 + 1e-9, False)):
+        R = np.zeros((7, n))
+        R[0] = thr * factor
+        R[1:] = 0.3
+        out, mom, cnt = O.push_step(s, st, None, R)
+        assert cnt[2] == (n if expect_el else 0)
+
+
+def test_deposit_brute_force():
+    """Node deposit = histogram over int(((x+dx/2)/(L+dx)) * bnf*(N+1)) // bnf."""
+    n, N, L, bnf = 5000, 37, 2.54, 4
+    st = _state(n, np.float64, 1, L, seed=8)
+    s = O.Setup(dim=1, dt=1e-12, n_cells=N, length_cm=L, bnf=bnf)
+    mom = O.deposit(s, st)
+    x, vx, vy, vz, w = st
+    dx = L / N
+    fine = (((x + 0.5 * dx) / (L + dx)) * (bnf * (N + 1))).astype(int)
+    node = fine // bnf
+    e = KEV * (vx * vx + vy * vy + vz * vz)
+    ne = np.bincount(node, weights=w, minlength=N + 1)
+    nrg = np.bincount(node, weights=w * e, minlength=N + 1)
+    np.testing.assert_allclose(mom[0], ne, rtol=1e-13)
+    np.testing.assert_allclose(mom[2], nrg, rtol=1e-13)
+    assert np.all(mom[1] == 0)
+    # nearest-node rule agrees with int((x + dx/2)/dx) away from rounding ties
+    near = ((x + 0.5 * dx) / dx).astype(int)
+    assert np.mean(near == node) > 0.999
+
+
+def test_threaded_push_matches_sequential():
+    n = 50_000
+    t, tab = _table(np.float64)
+    st = _state(n, np.float64, 1, 2.5, seed=21)
+    E = np.linspace(-100, 100, 32)
+    R = O.collision_uniforms(5, 0, n, np.float64)
+    s1 = O.Setup(dim=1, dt=5e-11, n_cells=32, length_cm=2.5, nu_max=t.nu_max,
+                 gas_density_m3=3.2e22, nthreads=1)
+    s4 = O.Setup(**{**s1.__dict__, "nthreads": 4})
+    a, ma, ca = O.push_step(s1, st, E, R, table=tab)
+    b, mb, cb = O.push_step(s4, st, E, R, table=tab)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+    np.testing.assert_allclose(ma, mb, rtol=1e-13)
+    np.testing.assert_array_equal(ca, cb)
+
+
+# ---------------------------------------------------------------------------
+def test_poisson_dense_and_manufactured():
+    """Oracle Poisson vs a dense solve of (1/dx^2) tridiag(-1,2,-1) V = rhs,
+    and the closed-form parabola of a uniform charge between grounded ends."""
+    from scipy.linalg import cholesky_banded
+    N, L = 50, 2.54
+    dx = L / N
+    Ac = np.zeros((2, N - 1))
+    Ac[0, 1:] = -1.0 / dx ** 2
+    Ac[1, :] = 2.0 / dx ** 2
+    c = cholesky_banded(Ac)
+    chol = np.stack([c[1], np.concatenate([[0.0], c[0, 1:]])])
+    rng = np.random.default_rng(2)
+    ne = rng.uniform(0.5e9, 1.5e9, N + 1)
+    ni = rng.uniform(0.5e9, 1.5e9, N + 1)
+    mom = np.stack([ne, np.zeros(N + 1), np.zeros(N + 1)]).ravel()
+    eps, v0, f, t = 5.5263e5, 100.0, 13.56e6, 1.3e-8
+    out = O.field_step(N, dx, 1e-11, eps, v0, f, 1, 1, 1, 0, 1, t, mom, ni.copy(),
+                       np.zeros(N + 1), np.zeros(N + 1), np.zeros(N + 1), np.zeros(N - 1),
+                       np.zeros(N), chol)
+    V, E = out[4], out[5]
+    A = (np.diag(2 * np.ones(N - 1)) - np.diag(np.ones(N - 2), 1) - np.diag(np.ones(N - 2), -1)) / dx ** 2
+    rhs = (ni[1:N] - ne[1:N]) / eps
+    vl = v0 * math.sin(2 * math.pi * f * t)
+    rhs[0] += vl / dx ** 2
+    np.testing.assert_allclose(V, np.linalg.solve(A, rhs), rtol=1e-9, atol=1e-9 * np.abs(V).max())
+    Vf = np.concatenate([[vl], V, [0.0]])
+    np.testing.assert_allclose(E, -(Vf[1:] - Vf[:-1]) / dx, rtol=1e-12, atol=1e-9)
+    # uniform charge rho, grounded ends: V(x) = rho/(2 eps) x (L - x)
+    rho = 1e8
+    ni2 = np.full(N + 1, rho)
+    mom2 = np.zeros(3 * (N + 1))
+    out2 = O.field_step(N, dx, 1e-11, eps, 0.0, f, 1, 1, 1, 0, 1, 0.0, mom2, ni2, np.zeros(N + 1),
+                        np.zeros(N + 1), np.zeros(N + 1), np.zeros(N - 1), np.zeros(N), chol)
+    xs = dx * np.arange(1, N)
+    np.testing.assert_allclose(out2[4], rho / (2 * eps) * xs * (L - xs), rtol=1e-9)
+
+
+def test_fluid_mass_balance():
+    """Explicit drift-diffusion: interior mass changes only by the edge fluxes."""
+    from boltzmann_amd import physics as ph
+    N, L = 40, 2.5
+    dx = L / N
+    rng = np.random.default_rng(4)
+    ni = rng.uniform(0.9e9, 1.1e9, N + 1)
+    E = rng.uniform(-30, 30, N)
+    chol = ph.poisson_cholesky(N, dx)
+    mu, Di, Ds, dt = 14.4, 0.64, 0.75, 1e-9
+    ni_b = ni.copy()
+    ni_b[0] = ni_b[1] * ((Di / dx - 0.5 * mu * max(E[0], 0)) / (Di / dx + 0.5 * mu * max(E[0], 0)))
+    ni_b[N] = ni_b[N - 1] * ((Di / dx + 0.5 * mu * min(E[N - 1], 0)) / (Di / dx - 0.5 * mu * min(E[N - 1], 0)))
+    J = mu * 0.5 * (ni_b[:-1] + ni_b[1:]) * E - Di * (ni_b[1:] - ni_b[:-1]) / dx
+    out = O.field_step(N, dx, dt, 5.5263e5, 0.0, 1.0, mu, Di, Ds, 1, 0, 0.0, np.zeros(3 * (N + 1)),
+                       ni.copy(), np.zeros(N + 1), np.zeros(N + 1), np.zeros(N + 1),
+                       np.zeros(N - 1), E.copy(), chol)
+    ni2 = out[0]
+    d_mass = np.sum(ni2[1:N] - ni_b[1:N]) * dx
+    mass = np.sum(np.abs(ni_b)) * dx
+    assert abs(d_mass - (-dt * (J[N - 1] - J[0]))) <= 1e-12 * mass
+    # and the interior update is the conservative difference of those fluxes
+    np.testing.assert_allclose(ni2[1:N], ni_b[1:N] - dt * (J[1:] - J[:-1]) / dx, rtol=1e-12)
+
+
+def test_moment_update():
+    N = 8
+    ne = np.arange(1, N + 2, dtype=float)
+    nsrc = np.linspace(0, 1, N + 1)
+    nrg = 3 * ne
+    nrg[0] = 0
+    ne[0] = 0
+    mom = np.concatenate([ne, nsrc, nrg])
+    ni = np.ones(N + 1)
+    g2 = np.zeros(N + 1)
+    out = O.field_step(N, 0.1, 1e-12, 1, 0, 1, 1, 1, 1, 0, 0, 0.0, mom, ni, np.zeros(N + 1),
+                       np.zeros(N + 1), g2, np.zeros(N - 1), np.zeros(N), np.zeros(2 * (N - 1)))
+    np.testing.assert_allclose(out[0], 1 + nsrc)
+    assert out[2][0] == 0.0  # nan_to_num
+    np.testing.assert_allclose(out[2][1:], 2.0)
+    np.testing.assert_allclose(out[3], nsrc)

@@ -1,0 +1,89 @@
+"""Host set-up of the product against independent derivations."""
+import math
+
+import numpy as np
+import pytest
+
+from boltzmann_amd import _lib
+from boltzmann_amd import physics as ph
+from oracle import oracle as O
+
+
+@pytest.mark.parametrize("xs_mode", [0, 1])
+@pytest.mark.parametrize("dim", [0, 1])
+def test_push_constants_match_oracle(xs_mode, dim):
+    cfg = ph.RunConfig(name="t", dim=dim, n_particles=10, n_cells=64, length_cm=2.5,
+                       dt=3e-11, xs_mode=xs_mode, gas_density_m3=3.2e22)
+    tab = ph.build_table(cfg.xsec, cfg.gas_density_m3, np.float64) if xs_mode else None
+    p = ph.push_params(cfg, table=tab)
+    s = O.Setup(dim=dim, dt=cfg.dt, xs_mode=xs_mode, n_cells=64, length_cm=2.5,
+                gas_density_m3=3.2e22, nu_max=tab.nu_max if tab else 0.0,
+                e_field0=cfg.e_field0_vcm() if dim == 0 else 0.0).params()
+    for f in ("dt", "xfac", "afac", "kev", "jm", "fel", "L", "dx", "half_dx", "L_dx", "dep_M",
+              "nn", "e_ion", "e_exc", "e_field0"):
+        assert getattr(p, f) == getattr(s, f), f
+    if xs_mode:
+        assert p.pc == pytest.approx(s.pc, rel=1e-15)
+    assert (p.dim, p.n_cells, p.n_nodes, p.bnf) == (s.dim, s.n_cells, s.n_nodes, s.bnf)
+
+
+def test_e_over_n_field():
+    cfg = ph.RunConfig(name="c0", dim=0, n_particles=1, gas_density_m3=3.2e22, e_over_n_td=100)
+    assert cfg.e_field0_vcm() == pytest.approx(32.0, rel=1e-12)  # 3200 V/m
+
+
+def test_poisson_cholesky_matches_scipy():
+    from scipy.linalg import cholesky_banded
+    for N, dx in ((5, 0.1), (256, 2.5 / 256), (1199, 2.54 / 1199)):
+        Ac = np.zeros((2, N - 1))
+        Ac[0, 1:] = -1.0 / dx ** 2
+        Ac[1, :] = 2.0 / dx ** 2
+        c = cholesky_banded(Ac)
+        mine = ph.poisson_cholesky(N, dx)
+        np.testing.assert_allclose(mine[0], c[1], rtol=1e-14)
+        np.testing.assert_allclose(mine[1][1:], c[0, 1:], rtol=1e-14)
+
+
+def test_cross_section_closed_forms():
+    m = ph.XSecModel()
+    e = np.array([0.5, 11.0, 11.55, 20.0, 30.0, 60.0, 15.76, 31.52, 1000.0])
+    s = m.sigma(e)
+    assert np.all(s[0] == 1e-19)
+    assert s[1][0] == 0 and s[1][1] == 0 and s[1][2] == 0
+    assert s[1][3] == pytest.approx(1e-20 * (20 - 11.55) / (30 - 11.55))
+    assert s[1][4] == pytest.approx(1e-20)
+    assert s[1][5] == pytest.approx(0.5e-20)
+    assert s[2][6] == 0
+    assert s[2][7] == pytest.approx(3e-20 * 0.5 * math.log(2) / 2)
+    assert np.all(np.diff(m.energies()) > 0) and m.energies().size == 1000
+    assert m.energies()[0] == pytest.approx(0.01) and m.energies()[-1] == pytest.approx(1000)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_table_hints_locate_segments(dtype):
+    """hint bucket + bounded upward scan == binary search, for random energies."""
+    t = ph.build_table(ph.XSecModel(), 3.2e22, dtype)
+    rng = np.random.default_rng(1)
+    e = np.exp(rng.uniform(np.log(0.01), np.log(1000), 200_000)).astype(dtype)
+    e = np.clip(e, t.energy[0], t.energy[-1])
+    b = np.clip(ph._bucket(e, t.hint_bits) - t.hint_base, 0, t.hint.size - 1)
+    j = t.hint[b].astype(np.int64)
+    n = t.energy.size
+    for _ in range(8):
+        step = (j + 1 <= n - 2) & (t.energy[np.minimum(j + 1, n - 1)] <= e)
+        j = j + step
+    want = np.clip(np.searchsorted(t.energy, e, side="right") - 1, 0, n - 2)
+    np.testing.assert_array_equal(j, want)
+
+
+def test_nu_max_bounds_collision_frequency():
+    t = ph.build_table(ph.XSecModel(), 3.2e22, np.float64)
+    m = ph.XSecModel()
+    e = np.exp(np.linspace(np.log(0.01), np.log(1000), 400_000))
+    sig = np.stack([np.interp(e, t.energy, t.sig[c]) for c in range(3)])
+    sig[1][e < m.e_exc] = 0
+    sig[2][e < m.e_ion] = 0
+    v = np.sqrt(2 * e / (ph.J_EV * ph.M_E))
+    nu = (sig * v).sum(axis=0)
+    assert nu.max() <= t.nu_max
+    assert nu.max() > 0.9 * t.nu_max
