@@ -49,7 +49,7 @@ def c4() -> RunConfig:
                      precision=_lib.BZ_F32, nb=5, n_cells=4096, length_cm=2.5,
                      dt=1.0 / (RF * 800), steps=10 * 800, gas_density_m3=3.22e21,
                      te_ev=2.0, density_cm3=1e9, rf_voltage=300.0, rf_freq=RF,
-                     target_ppc=8000, resample_every=1, trigger_at=1.0, merge_at=1.5,
+                     target_ppc=8000, resample_every=1, trigger_at=0.0, merge_at=1.5,
                      bin_fractions=bin_fractions(5), bin_thresholds=bin_thresholds(5),
                      capacity_factor=1.5)
 

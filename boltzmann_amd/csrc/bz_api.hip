@@ -337,7 +337,7 @@ int bz_bin_sort(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
   if (nb > 1 && !thresholds_host) return fail("nb > 1 needs thresholds");
   const int64_t n_keys = (int64_t)(p->dim == 1 ? p->n_nodes : 1) * nb;
   if (n_keys > c->env.max_keys) return fail("n_keys %lld exceeds context max_keys", (long long)n_keys);
-  if (n_keys > 65536) return fail("n_keys must be <= 65536 (two 8-bit radix passes)");
+  if (n_keys > 39999) return fail("n_keys must be <= 39999 (LDS key histogram)");
   if (!s || !counts || !offsets || !perm) return fail("null bin/sort pointer");
   return launch_bin_sort(c->env, s, p, thresholds_host, nb, counts, offsets, perm);
 }

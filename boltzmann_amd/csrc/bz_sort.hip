@@ -1,22 +1,856 @@
-// Energy/cell binning, stable radix sort and ES-WPNC resampling (gfx950).
+// Energy/cell binning, stable LSD radix sort by (cell, energy bin), and
+// ES-WPNC merge / split resampling, for gfx950.
+//
+// Binning (bin_particles.py of the 1/3/5-bin strategies) builds per-cell,
+// per-bin particle lists with atomic slots, so the list order depends on
+// scheduling.  Here every particle gets key = cell*nb + bin and the state is
+// ordered by a stable sort, so a segment lists its particles in ascending
+// index order on every run.
+//
+// Resampling (boltzmann.py _do_resampling) selects particles by searching a
+// cumulative weight sum.  The sum is formed in 64-bit fixed point with one
+// power-of-two scale per segment, so the prefix sums are exact integers and
+// the selection does not depend on the summation order.
 #include "bz_launch.h"
 
 namespace bz {
 
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;  // 4096
+constexpr int kScanChunk = 4096;
+
+struct SortWs {
+  uint32_t* keys_a;
+  uint32_t* keys_b;
+  uint32_t* vals_b;
+  uint32_t* hist;       // [256][tiles]
+  uint32_t* chunks;     // chunk sums of the hist scan
+  int64_t* cdf;         // resample: fixed-point prefix per sorted position
+  int32_t* copies;      // resample: split copies per sorted position
+  int32_t* plan;        // resample: [kind, new_count, new_off, hw_count] per key
+  uint32_t* sel;        // resample: selected sorted position per new particle
+  int64_t max_tiles;
+  int64_t max_chunks;
+};
+
+static int64_t tiles_for(int64_t cap) { return cap / kSortTile + 2; }
+
+static SortWs carve(void* base, int64_t cap, int32_t max_keys) {
+  SortWs w;
+  char* p = (char*)base;
+  auto take = [&](size_t bytes) {
+    char* r = p;
+    p += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  w.max_tiles = tiles_for(cap);
+  w.max_chunks = (256 * w.max_tiles) / kScanChunk + 2;
+  w.keys_a = (uint32_t*)take(4 * cap);
+  w.keys_b = (uint32_t*)take(4 * cap);
+  w.vals_b = (uint32_t*)take(4 * cap);
+  w.hist = (uint32_t*)take(4 * 256 * w.max_tiles);
+  w.chunks = (uint32_t*)take(4 * (w.max_chunks + 1));
+  w.cdf = (int64_t*)take(8 * cap);
+  w.copies = (int32_t*)take(4 * cap);
+  w.plan = (int32_t*)take(16 * (size_t)(max_keys + 1));
+  w.sel = (uint32_t*)take(4 * cap);
+  return w;
+}
+
 size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys) {
-  (void)max_keys;
-  return (size_t)(capacity > 0 ? capacity : 1) * 4;
+  const int64_t cap = capacity > 0 ? capacity : 1;
+  const int64_t tiles = tiles_for(cap);
+  const int64_t chunks = (256 * tiles) / kScanChunk + 2;
+  size_t b = 0;
+  auto add = [&](size_t bytes) { b += (bytes + 255) & ~(size_t)255; };
+  add(4 * cap);
+  add(4 * cap);
+  add(4 * cap);
+  add(4 * 256 * tiles);
+  add(4 * (chunks + 1));
+  add(8 * cap);
+  add(4 * cap);
+  add(16 * (size_t)(max_keys + 1));
+  add(4 * cap);
+  return b + 1024;
 }
 
-int launch_bin_sort(const LaunchEnv&, const bz_state_t*, const bz_params_t*, const double*, int,
-                    int32_t*, int32_t*, uint32_t*) {
-  return fail("bz_bin_sort: not built yet");
+// ---------------------------------------------------------------------------
+// keys + per-key counts
+// ---------------------------------------------------------------------------
+struct KeyArgs {
+  const void* x;
+  const void* vx;
+  const void* vy;
+  const void* vz;
+  const void* w;
+  double dx;
+  double kev;
+  double thr[8];
+  int nb;
+  int n_nodes;
+  int dim;
+  int n_keys;  // live keys; dead particles get key n_keys
+};
+
+template <typename T>
+__global__ __launch_bounds__(kSortBlock) void k_keys(KeyArgs a, const Clock* clk, uint32_t* keys,
+                                                     int32_t* counts) {
+  extern __shared__ int32_t s_cnt[];  // n_keys + 1
+  for (int j = threadIdx.x; j <= a.n_keys; j += kSortBlock) s_cnt[j] = 0;
+  __syncthreads();
+  const int64_t n = clk->n;
+  const T* x = (const T*)a.x;
+  const T* vx = (const T*)a.vx;
+  const T* vy = (const T*)a.vy;
+  const T* vz = (const T*)a.vz;
+  const T* w = (const T*)a.w;
+  for (int64_t i = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kSortBlock) {
+    uint32_t key;
+    if (!(w[i] > (T)0)) {
+      key = (uint32_t)a.n_keys;
+    } else {
+      int cell = 0;
+      if (a.dim == 1) {
+        cell = (int)(((double)x[i] + 0.5 * a.dx) / a.dx);
+        cell = cell < 0 ? 0 : (cell >= a.n_nodes ? a.n_nodes - 1 : cell);
+      }
+      const T e = (T)a.kev * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+      int b = 0;
+      for (int k = 0; k < a.nb - 1; ++k) b += (e >= (T)a.thr[k]) ? 1 : 0;
+      key = (uint32_t)(cell * a.nb + b);
+    }
+    keys[i] = key;
+    atomicAdd(&s_cnt[key], 1);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j <= a.n_keys; j += kSortBlock)
+    if (s_cnt[j]) atomicAdd(&counts[j], s_cnt[j]);
 }
 
-int launch_resample(const LaunchEnv&, const bz_state_t*, const bz_state_t*, const bz_params_t*,
-                    const int32_t*, const int32_t*, const uint32_t*, const int32_t*, int32_t,
-                    const bz_resample_policy_t*) {
-  return fail("bz_resample: not built yet");
+__global__ void k_zero_i32(int32_t* p, int64_t m) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
+       j += (int64_t)gridDim.x * blockDim.x)
+    p[j] = 0;
+}
+
+// exclusive scan of counts[0..m) -> offsets[0..m], single block
+__global__ __launch_bounds__(1024) void k_scan_small(const int32_t* in, int32_t* out, int m) {
+  __shared__ int32_t s_w[16];
+  __shared__ int32_t s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < m; base += 1024) {
+    const int i = base + tid;
+    const int v = i < m ? in[i] : 0;
+    const int inc = wave_incl_scan(v);
+    if (lane == 63) s_w[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+      int t = lane < 16 ? s_w[lane] : 0;
+      t = wave_incl_scan(t);
+      if (lane < 16) s_w[lane] = t;
+    }
+    __syncthreads();
+    const int excl = s_carry + (wid ? s_w[wid - 1] : 0) + inc - v;
+    if (i < m) out[i] = excl;
+    __syncthreads();
+    if (tid == 0) s_carry += s_w[15];
+    __syncthreads();
+  }
+  if (tid == 0) out[m] = s_carry;
+}
+
+// ---------------------------------------------------------------------------
+// radix pass: upsweep histogram, scan, stable downsweep scatter
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kSortBlock) void k_upsweep(const uint32_t* keys, const Clock* clk,
+                                                        int shift, uint32_t* hist, int64_t tiles) {
+  __shared__ uint32_t s_h[256];
+  s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t n = clk->n;
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + r * kSortBlock + threadIdx.x;
+    if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
+}
+
+// generic 3-phase exclusive scan of uint32 data[0..m)
+__global__ __launch_bounds__(1024) void k_chunk_reduce(const uint32_t* data, int64_t m,
+                                                       uint32_t* chunks) {
+  __shared__ uint32_t s_w[16];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+  uint32_t s = 0;
+  for (int k = threadIdx.x; k < kScanChunk; k += 1024) {
+    const int64_t i = base + k;
+    if (i < m) s += data[i];
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 16; ++w) t += s_w[w];
+    chunks[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* chunks, int64_t nch) {
+  // sequential carry over 1024-wide block scans
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < nch; base += 1024) {
+    const int64_t i = base + tid;
+    const uint32_t v = i < nch ? chunks[i] : 0u;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) s_w[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+      uint32_t t = lane < 16 ? s_w[lane] : 0u;
+      t = wave_incl_scan(t);
+      if (lane < 16) s_w[lane] = t;
+    }
+    __syncthreads();
+    const uint32_t excl = s_carry + (wid ? s_w[wid - 1] : 0u) + inc - v;
+    if (i < nch) chunks[i] = excl;
+    __syncthreads();
+    if (tid == 0) s_carry += s_w[15];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_chunk_apply(uint32_t* data, int64_t m,
+                                                      const uint32_t* chunks) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+  if (tid == 0) s_carry = chunks[blockIdx.x];
+  __syncthreads();
+  for (int k = 0; k < kScanChunk; k += 1024) {
+    const int64_t i = base + k + tid;
+    const uint32_t v = i < m ? data[i] : 0u;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) s_w[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+      uint32_t t = lane < 16 ? s_w[lane] : 0u;
+      t = wave_incl_scan(t);
+      if (lane < 16) s_w[lane] = t;
+    }
+    __syncthreads();
+    const uint32_t excl = s_carry + (wid ? s_w[wid - 1] : 0u) + inc - v;
+    if (i < m) data[i] = excl;
+    __syncthreads();
+    if (tid == 0) s_carry += s_w[15];
+    __syncthreads();
+  }
+}
+
+// Stable scatter of one tile: items are visited in index order (rounds of
+// 256, one per thread); equal digits keep their order through a wave-level
+// match (8 ballots) and per-wave counts combined in wave order.
+__global__ __launch_bounds__(kSortBlock) void k_downsweep(const uint32_t* keys_in,
+                                                          const uint32_t* vals_in,
+                                                          uint32_t* keys_out, uint32_t* vals_out,
+                                                          const Clock* clk, int shift,
+                                                          const uint32_t* hist, int64_t tiles,
+                                                          int identity_vals) {
+  constexpr int NW = kSortBlock / kWave;
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_wc[NW][256];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t n = clk->n;
+  const int64_t t = blockIdx.x;
+  s_run[tid] = 0;
+  s_base[tid] = hist[(int64_t)tid * tiles + t];
+  for (int w = 0; w < NW; ++w) s_wc[w][tid] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = t * kSortTile + r * kSortBlock + tid;
+    const bool valid = i < n;
+    const uint32_t key = valid ? keys_in[i] : 0u;
+    const uint32_t val = valid ? (identity_vals ? (uint32_t)i : vals_in[i]) : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bb = __ballot((d >> b) & 1u);
+      m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(m & lt);
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    const bool leader = valid && rank == 0;
+    if (leader) s_wc[wid][d] = cnt;
+    __syncthreads();
+    if (valid) {
+      uint32_t off = s_base[d] + s_run[d] + rank;
+      for (int w = 0; w < wid; ++w) off += s_wc[w][d];
+      keys_out[off] = key;
+      vals_out[off] = val;
+    }
+    __syncthreads();
+    {
+      uint32_t s = 0;
+      for (int w = 0; w < NW; ++w) {
+        s += s_wc[w][tid];
+        s_wc[w][tid] = 0;
+      }
+      s_run[tid] += s;
+    }
+    __syncthreads();
+  }
+}
+
+int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                    const double* thr, int nb, int32_t* counts, int32_t* offsets,
+                    uint32_t* perm) {
+  const int n_nodes = p->dim == 1 ? p->n_nodes : 1;
+  const int n_keys = n_nodes * nb;
+  if (n_keys + 1 > 40000) return fail("bin_sort: at most 39999 keys (LDS histogram)");
+  const int64_t cap = env.capacity;
+  SortWs ws = carve(env.sort_ws, cap, env.max_keys);
+  KeyArgs ka;
+  ka.x = s->x;
+  ka.vx = s->vx;
+  ka.vy = s->vy;
+  ka.vz = s->vz;
+  ka.w = s->w;
+  ka.dx = p->dx;
+  ka.kev = p->kev;
+  for (int k = 0; k < 8; ++k) ka.thr[k] = (thr && k < nb - 1) ? thr[k] : 0.0;
+  ka.nb = nb;
+  ka.n_nodes = n_nodes;
+  ka.dim = p->dim;
+  ka.n_keys = n_keys;
+  timing_begin(env, 1);
+  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, counts,
+                     (int64_t)n_keys + 1);
+  const size_t smem = (size_t)(n_keys + 1) * sizeof(int32_t);
+  auto kk = env.precision == BZ_F64 ? k_keys<double> : k_keys<float>;
+  if (smem > 64 * 1024) {
+    if (hipFuncSetAttribute((const void*)kk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)smem) != hipSuccess)
+      return fail("hipFuncSetAttribute(keys LDS) failed");
+  }
+  hipLaunchKernelGGL(kk, dim3(env.num_cu * 2), dim3(kSortBlock), smem, env.stream, ka,
+                     (const Clock*)env.clock, ws.keys_a, counts);
+  int rc = check_launch("k_keys");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, env.stream, (const int32_t*)counts,
+                     offsets, n_keys + 1);
+  // passes: 8 bits each over key range [0, n_keys]
+  const int passes = (n_keys + 1) <= 256 ? 1 : 2;
+  const int64_t tiles = ws.max_tiles;
+  const int64_t m = 256 * tiles;
+  const int64_t nch = (m + kScanChunk - 1) / kScanChunk;
+  uint32_t* kin = ws.keys_a;
+  const uint32_t* vin = nullptr;
+  for (int ps = 0; ps < passes; ++ps) {
+    const int shift = 8 * ps;
+    const bool last = ps == passes - 1;
+    uint32_t* kout = last ? (passes == 1 ? ws.keys_b : ws.keys_b) : ws.keys_b;
+    uint32_t* vout = last ? perm : ws.vals_b;
+    if (passes == 2 && ps == 1) {
+      kout = ws.keys_a;
+    }
+    hipLaunchKernelGGL(k_zero_i32, dim3(256), dim3(256), 0, env.stream, (int32_t*)ws.hist, m);
+    hipLaunchKernelGGL(k_upsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
+                       (const uint32_t*)kin, (const Clock*)env.clock, shift, ws.hist, tiles);
+    hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)nch), dim3(1024), 0, env.stream,
+                       (const uint32_t*)ws.hist, m, ws.chunks);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, env.stream, ws.chunks, nch);
+    hipLaunchKernelGGL(k_chunk_apply, dim3((unsigned)nch), dim3(1024), 0, env.stream, ws.hist, m,
+                       (const uint32_t*)ws.chunks);
+    hipLaunchKernelGGL(k_downsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
+                       (const uint32_t*)kin, vin, kout, vout, (const Clock*)env.clock, shift,
+                       (const uint32_t*)ws.hist, tiles, ps == 0 ? 1 : 0);
+    rc = check_launch("radix pass");
+    if (rc) return rc;
+    kin = kout;
+    vin = vout;
+  }
+  timing_end(env, 1);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// ES-WPNC resampling
+// ---------------------------------------------------------------------------
+enum : int { kCopy = 0, kSplit = 1, kMerge = 2 };
+
+struct ResArgs {
+  const void* in[5];
+  void* out[5];
+  bz_resample_policy_t pol;
+  double kev;
+  uint64_t seed;
+  int n_keys;
+  int prec;
+};
+
+// sorted keys of the final pass (keys_a for 2 passes, keys_b for 1)
+__device__ __forceinline__ int key_at(const uint32_t* skeys, int64_t s) { return (int)skeys[s]; }
+
+__device__ __forceinline__ int ceil_log2_i64(int64_t c) {
+  int b = 0;
+  while (((int64_t)1 << b) < c) ++b;
+  return b;
+}
+
+template <typename T>
+__device__ __forceinline__ double wt(const ResArgs& a, uint32_t idx) {
+  return (double)((const T*)a.in[4])[idx];
+}
+
+// Phase A: heavy-particle counts of split candidates (one block per key).
+template <typename T>
+__global__ __launch_bounds__(256) void k_res_hw(ResArgs a, const int32_t* counts,
+                                                const int32_t* offsets, const uint32_t* perm,
+                                                const int32_t* targets, int32_t* plan) {
+  const int k = blockIdx.x;
+  const int c = counts[k];
+  const int t = targets[k];
+  __shared__ int s_sum;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  if (c > 0 && t > 0 && (double)c < a.pol.split_below * (double)t) {
+    int h = 0;
+    for (int j = threadIdx.x; j < c; j += 256)
+      h += wt<T>(a, perm[offsets[k] + j]) > a.pol.min_wt ? 1 : 0;
+    atomicAdd(&s_sum, h);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) plan[4 * k + 3] = s_sum;
+}
+
+// Phase B (single block): kind and new count per key, trigger, offsets.
+__global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* counts,
+                                                   const int32_t* targets, int32_t* plan,
+                                                   int nb, Clock* clk) {
+  __shared__ int s_fire;
+  __shared__ int s_w[16];
+  __shared__ int s_carry;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const bz_resample_policy_t& P = a.pol;
+  if (tid == 0) {
+    s_fire = (!P.ref_trigger && P.trigger_at <= 0.0) ? 1 : 0;  // <= 0: every check fires
+    s_carry = 0;
+  }
+  __syncthreads();
+  for (int k = tid; k < a.n_keys; k += 1024) {
+    const int c = counts[k], t = targets[k];
+    if (t <= 0 || c <= 0) continue;
+    if (P.ref_trigger) {
+      // only the last bin's maximum decides (3bin/boltzmann.py:1110-1113)
+      if ((k % nb) == nb - 1 && (double)c >= 3.0 * (double)t) s_fire = 1;
+    } else if ((double)c >= P.trigger_at * (double)t) {
+      s_fire = 1;
+    }
+  }
+  __syncthreads();
+  const int fire = s_fire;
+  int n_split = 0, n_merge = 0;
+  for (int base = 0; base < a.n_keys; base += 1024) {
+    const int k = base + tid;
+    int kind = kCopy, nc = 0;
+    if (k < a.n_keys) {
+      const int c = counts[k], t = targets[k];
+      nc = c;
+      if (fire && t > 0) {
+        const int keep = (int)(P.merge_keep * (double)t);
+        if (c > 0 && (double)c < P.split_below * (double)t) {
+          const int draws = (int)(P.split_to * (double)t) - c;
+          if (plan[4 * k + 3] > 0 && draws > 0) {
+            kind = kSplit;
+            nc = c + draws;
+            ++n_split;
+          }
+        } else if ((double)c >= P.merge_at * (double)t && keep >= 1) {
+          const int ce = (P.max_ppc > 0 && c > P.max_ppc) ? P.max_ppc : c;
+          if (keep < ce) {
+            kind = kMerge;
+            nc = keep + (c - ce);
+            ++n_merge;
+          }
+        }
+      }
+      plan[4 * k + 0] = kind;
+      plan[4 * k + 1] = nc;
+    }
+    const int inc = wave_incl_scan(nc);
+    if (lane == 63) s_w[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+      int v = lane < 16 ? s_w[lane] : 0;
+      v = wave_incl_scan(v);
+      if (lane < 16) s_w[lane] = v;
+    }
+    __syncthreads();
+    if (k < a.n_keys) plan[4 * k + 2] = s_carry + (wid ? s_w[wid - 1] : 0) + inc - nc;
+    __syncthreads();
+    if (tid == 0) s_carry += s_w[15];
+    __syncthreads();
+  }
+  n_split = wave_sum(n_split);
+  n_merge = wave_sum(n_merge);
+  if (lane == 0) {
+    if (n_split) atomicAdd(&clk->cnt[BZ_CNT_SPLIT], (unsigned long long)n_split);
+    if (n_merge) atomicAdd(&clk->cnt[BZ_CNT_MERGE], (unsigned long long)n_merge);
+  }
+  if (tid == 0) {
+    clk->n_next = s_carry;
+    if (fire) atomicAdd(&clk->cnt[BZ_CNT_RESAMPLE], 1ull);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void copy_particle(const ResArgs& a, uint32_t src, int64_t dst) {
+#pragma unroll
+  for (int f = 0; f < 5; ++f) ((T*)a.out[f])[dst] = ((const T*)a.in[f])[src];
+}
+
+// Copy segments (and the unmerged tail of clipped merges), per sorted position.
+template <typename T>
+__global__ __launch_bounds__(256) void k_res_copy(ResArgs a, const uint32_t* skeys,
+                                                  const int32_t* offsets, const uint32_t* perm,
+                                                  const int32_t* plan, const int32_t* counts,
+                                                  const Clock* clk, int64_t cap) {
+  const int64_t n_live = offsets[a.n_keys];
+  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < n_live;
+       s += (int64_t)gridDim.x * 256) {
+    const int k = key_at(skeys, s);
+    const int kind = plan[4 * k];
+    const int64_t j = s - offsets[k];
+    int64_t dst = -1;
+    if (kind == kCopy) {
+      dst = plan[4 * k + 2] + j;
+    } else if (kind == kMerge) {
+      const int c = counts[k];
+      const int ce = (a.pol.max_ppc > 0 && c > a.pol.max_ppc) ? a.pol.max_ppc : c;
+      const int keep = plan[4 * k + 1] - (c - ce);
+      if (j >= ce) dst = plan[4 * k + 2] + keep + (j - ce);
+    }
+    if (dst >= 0 && dst < cap) copy_particle<T>(a, perm[s], dst);
+  }
+}
+
+// block-wide helpers for the per-segment kernels (256 threads)
+__device__ __forceinline__ double block_sum_d(double v, double* s_red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) s_red[wid] = v;
+  __syncthreads();
+  const double r = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ double block_max_d(double v, double* s_red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) s_red[wid] = v;
+  __syncthreads();
+  const double r = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+  __syncthreads();
+  return r;
+}
+
+// Inclusive fixed-point prefix of q over sorted positions [s0, s0+c) into cdf.
+// Returns the total.  q(j) is supplied by the functor.
+template <typename Q>
+__device__ int64_t block_fixed_scan(int64_t s0, int c, int64_t* cdf, Q q, int64_t* s_w64) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int64_t carry = 0;
+  for (int base = 0; base < c; base += 256) {
+    const int j = base + threadIdx.x;
+    const int64_t v = j < c ? q(j) : 0;
+    const int64_t inc = wave_incl_scan(v);
+    __syncthreads();
+    if (lane == 63) s_w64[wid] = inc;
+    __syncthreads();
+    int64_t before = carry;
+    for (int w = 0; w < wid; ++w) before += s_w64[w];
+    if (j < c) cdf[s0 + j] = before + inc;
+    carry += s_w64[0] + s_w64[1] + s_w64[2] + s_w64[3];
+  }
+  __syncthreads();
+  return carry;
+}
+
+// first j in [0, c) with cdf[s0+j] >= thr (left) or > thr (right)
+__device__ __forceinline__ int search_cdf(const int64_t* cdf, int64_t s0, int c, int64_t thr,
+                                          bool right) {
+  int lo = 0, hi = c;  // answer in [lo, hi]
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int64_t v = cdf[s0 + mid];
+    const bool go_left = right ? (v > thr) : (v >= thr);
+    if (go_left) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo < c ? lo : c - 1;
+}
+
+__device__ __forceinline__ double res_uniform(uint64_t seed, uint64_t step, uint32_t key,
+                                              uint32_t m) {
+  const U4 r = draw(seed, kPurposeResample, m >> 1, key, step);
+  return (m & 1) ? u53(r.c, r.d) : u53(r.a, r.b);
+}
+
+// Split: weighted draws with replacement among w > min_wt; each drawn
+// particle's weight is divided by (copies+1) and its copies follow it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* counts,
+                                                   const int32_t* offsets, const uint32_t* perm,
+                                                   const int32_t* targets, const int32_t* plan,
+                                                   SortWs ws, const Clock* clk, int64_t cap) {
+  const int k = blockIdx.x;
+  if (plan[4 * k] != kSplit) return;
+  __shared__ double s_red[4];
+  __shared__ int64_t s_w64[4];
+  __shared__ int s_wi[4];
+  const int c = counts[k];
+  const int64_t s0 = offsets[k];
+  const uint64_t step = clk->step;
+  const double minw = a.pol.min_wt;
+  // fixed-point scale from the largest heavy weight
+  double wmax = 0.0;
+  for (int j = threadIdx.x; j < c; j += 256) {
+    const double w = wt<T>(a, perm[s0 + j]);
+    if (w > minw) wmax = fmax(wmax, w);
+    ws.copies[s0 + j] = 0;
+  }
+  wmax = block_max_d(wmax, s_red);
+  int ew;
+  frexp(wmax, &ew);
+  const int S = 61 - ew - ceil_log2_i64(c);
+  auto q = [&](int j) -> int64_t {
+    const double w = wt<T>(a, perm[s0 + j]);
+    return w > minw ? (int64_t)ldexp(w, S) : 0;
+  };
+  const int64_t Wq = block_fixed_scan(s0, c, ws.cdf, q, s_w64);
+  const int draws = plan[4 * k + 1] - c;
+  for (int m = threadIdx.x; m < draws; m += 256) {
+    const double u = res_uniform(a.seed, step, (uint32_t)k, (uint32_t)m);
+    const int64_t thr = (int64_t)(u * (double)Wq);
+    const int j = search_cdf(ws.cdf, s0, c, thr, true);
+    atomicAdd(&ws.copies[s0 + j], 1);
+  }
+  __syncthreads();
+  // output: each particle followed by its copies
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int carry = 0;
+  const int64_t out0 = plan[4 * k + 2];
+  for (int base = 0; base < c; base += 256) {
+    const int j = base + threadIdx.x;
+    const int reps = j < c ? 1 + ws.copies[s0 + j] : 0;
+    const int inc = wave_incl_scan(reps);
+    __syncthreads();
+    if (lane == 63) s_wi[wid] = inc;
+    __syncthreads();
+    int before = carry;
+    for (int w = 0; w < wid; ++w) before += s_wi[w];
+    if (j < c) {
+      const uint32_t src = perm[s0 + j];
+      int64_t dst = out0 + before + inc - reps;
+      const T* in_w = (const T*)a.in[4];
+      T w = in_w[src];
+      if (reps > 1) w = w / (T)reps;
+      for (int r = 0; r < reps; ++r, ++dst) {
+        if (dst >= cap) break;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) ((T*)a.out[f])[dst] = ((const T*)a.in[f])[src];
+        ((T*)a.out[4])[dst] = w;
+      }
+    }
+    carry += s_wi[0] + s_wi[1] + s_wi[2] + s_wi[3];
+  }
+}
+
+// Merge: keep K = int(merge_keep*target) particles chosen by stratified
+// sampling of the weight CDF, each with weight W/K; velocities rescaled about
+// the mean so that weight, momentum and energy are conserved.
+template <typename T>
+__global__ __launch_bounds__(256) void k_res_merge(ResArgs a, const int32_t* counts,
+                                                   const int32_t* offsets, const uint32_t* perm,
+                                                   const int32_t* targets, const int32_t* plan,
+                                                   SortWs ws, const Clock* clk, int64_t cap) {
+  const int k = blockIdx.x;
+  if (plan[4 * k] != kMerge) return;
+  __shared__ double s_red[4];
+  __shared__ int64_t s_w64[4];
+  const int c = counts[k];
+  const int ce = (a.pol.max_ppc > 0 && c > a.pol.max_ppc) ? a.pol.max_ppc : c;
+  const int K = plan[4 * k + 1] - (c - ce);
+  const int64_t s0 = offsets[k];
+  const int64_t out0 = plan[4 * k + 2];
+  const uint64_t step = clk->step;
+  const T* X = (const T*)a.in[0];
+  const T* VX = (const T*)a.in[1];
+  const T* VY = (const T*)a.in[2];
+  const T* VZ = (const T*)a.in[3];
+  const T* Wt = (const T*)a.in[4];
+  // FP64 moments of the old set
+  double w0 = 0, px = 0, py = 0, pz = 0, en = 0, wmax = 0;
+  for (int j = threadIdx.x; j < ce; j += 256) {
+    const uint32_t i = perm[s0 + j];
+    const double w = (double)Wt[i];
+    const double vx = VX[i], vy = VY[i], vz = VZ[i];
+    w0 += w;
+    px += w * vx;
+    py += w * vy;
+    pz += w * vz;
+    en += w * (vx * vx + vy * vy + vz * vz);
+    wmax = fmax(wmax, w);
+  }
+  const double W = block_sum_d(w0, s_red);
+  const double Px = block_sum_d(px, s_red), Py = block_sum_d(py, s_red),
+               Pz = block_sum_d(pz, s_red);
+  const double En = block_sum_d(en, s_red);
+  wmax = block_max_d(wmax, s_red);
+  const double mx = Px / W, my = Py / W, mz = Pz / W;
+  double c2 = 0;
+  for (int j = threadIdx.x; j < ce; j += 256) {
+    const uint32_t i = perm[s0 + j];
+    const double w = (double)Wt[i];
+    const double dx_ = VX[i] - mx, dy_ = VY[i] - my, dz_ = VZ[i] - mz;
+    c2 += w * (dx_ * dx_ + dy_ * dy_ + dz_ * dz_);
+  }
+  const double C_old = block_sum_d(c2, s_red);
+  // exact fixed-point CDF of the weights
+  int ew;
+  frexp(wmax, &ew);
+  const int S = 61 - ew - ceil_log2_i64(ce);
+  auto q = [&](int j) -> int64_t { return (int64_t)ldexp((double)Wt[perm[s0 + j]], S); };
+  const int64_t Wq = block_fixed_scan(s0, ce, ws.cdf, q, s_w64);
+  const double wbar = W / (double)K;
+  // stratified selection: r_k = (1/K)(U_k + k), searchsorted left
+  double sx = 0, sy = 0, sz = 0, se = 0;
+  for (int m = threadIdx.x; m < K; m += 256) {
+    const double u = res_uniform(a.seed, step, (uint32_t)k, (uint32_t)m);
+    const double r = (1.0 / (double)K) * (u + (double)m);
+    const int64_t thr = (int64_t)(r * (double)Wq);
+    const int j = search_cdf(ws.cdf, s0, ce, thr, false);
+    if (out0 + m >= cap) continue;  // overflow: counted at commit
+    ws.sel[out0 + m] = (uint32_t)(s0 + j);
+    const uint32_t i = perm[s0 + j];
+    const double vx = VX[i], vy = VY[i], vz = VZ[i];
+    sx += vx;
+    sy += vy;
+    sz += vz;
+    se += vx * vx + vy * vy + vz * vz;
+  }
+  const double Sx = block_sum_d(sx, s_red), Sy = block_sum_d(sy, s_red),
+               Sz = block_sum_d(sz, s_red), Se = block_sum_d(se, s_red);
+  double hx, hy, hz, f;
+  if (a.pol.ref_rescale) {
+    // boltzmann.py:835-869: hats normalised by the original weight, factor from total energy
+    hx = wbar * Sx / W;
+    hy = wbar * Sy / W;
+    hz = wbar * Sz / W;
+    const double T_cell = (2.0 / 3.0) * (a.kev * En) / W;
+    const double T_hat = (2.0 / 3.0) * (wbar * (a.kev * Se)) / W;
+    f = T_hat > 0 ? sqrt(T_cell / T_hat) : 1.0;
+  } else {
+    hx = Sx / (double)K;
+    hy = Sy / (double)K;
+    hz = Sz / (double)K;
+    double cn = 0;
+    for (int m = threadIdx.x; m < K; m += 256) {
+      if (out0 + m >= cap) break;
+      const uint32_t i = perm[ws.sel[out0 + m]];
+      const double dx_ = VX[i] - hx, dy_ = VY[i] - hy, dz_ = VZ[i] - hz;
+      cn += dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+    }
+    const double C_new = wbar * block_sum_d(cn, s_red);
+    f = C_new > 0 ? sqrt(C_old / C_new) : 1.0;
+  }
+  for (int m = threadIdx.x; m < K; m += 256) {
+    const int64_t dst = out0 + m;
+    if (dst >= cap) break;
+    const uint32_t i = perm[ws.sel[dst]];
+    ((T*)a.out[0])[dst] = X[i];
+    ((T*)a.out[1])[dst] = (T)(mx + ((double)VX[i] - hx) * f);
+    ((T*)a.out[2])[dst] = (T)(my + ((double)VY[i] - hy) * f);
+    ((T*)a.out[3])[dst] = (T)(mz + ((double)VZ[i] - hz) * f);
+    ((T*)a.out[4])[dst] = (T)wbar;
+  }
+}
+
+__global__ void k_res_commit(Clock* clk, int64_t cap) {
+  int64_t n = clk->n_next;
+  if (n > cap) {
+    atomicAdd(&clk->cnt[BZ_CNT_OVERFLOW], (unsigned long long)(n - cap));
+    n = cap;
+  }
+  clk->n = n;
+  clk->n_next = n;
+}
+
+template <typename T>
+static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                             const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
+                             const uint32_t* perm, const int32_t* targets, int32_t n_keys,
+                             const bz_resample_policy_t* pol) {
+  SortWs ws = carve(env.sort_ws, env.capacity, env.max_keys);
+  const int nb = n_keys / (p->dim == 1 ? p->n_nodes : 1);
+  if (nb < 1 || nb * (p->dim == 1 ? p->n_nodes : 1) != n_keys)
+    return fail("resample: n_keys must be n_nodes * nb");
+  // the final pass wrote the sorted keys to keys_a (two passes) or keys_b (one)
+  const uint32_t* skeys = (n_keys + 1) <= 256 ? ws.keys_b : ws.keys_a;
+  ResArgs a;
+  const void* iv[5] = {in->x, in->vx, in->vy, in->vz, in->w};
+  void* ov[5] = {out->x, out->vx, out->vy, out->vz, out->w};
+  for (int f = 0; f < 5; ++f) {
+    a.in[f] = iv[f];
+    a.out[f] = ov[f];
+  }
+  a.pol = *pol;
+  a.kev = p->kev;
+  a.seed = p->seed;
+  a.n_keys = n_keys;
+  a.prec = env.precision;
+  const int64_t cap = env.capacity;
+  timing_begin(env, 2);
+  hipLaunchKernelGGL(k_res_hw<T>, dim3(n_keys), dim3(256), 0, env.stream, a, counts, offsets,
+                     perm, targets, ws.plan);
+  hipLaunchKernelGGL(k_res_plan, dim3(1), dim3(1024), 0, env.stream, a, counts, targets, ws.plan,
+                     nb, env.clock);
+  hipLaunchKernelGGL(k_res_copy<T>, dim3(env.num_cu * 4), dim3(256), 0, env.stream, a, skeys,
+                     offsets, perm, (const int32_t*)ws.plan, counts, (const Clock*)env.clock,
+                     cap);
+  hipLaunchKernelGGL(k_res_split<T>, dim3(n_keys), dim3(256), 0, env.stream, a, counts, offsets,
+                     perm, targets, (const int32_t*)ws.plan, ws, (const Clock*)env.clock, cap);
+  hipLaunchKernelGGL(k_res_merge<T>, dim3(n_keys), dim3(256), 0, env.stream, a, counts, offsets,
+                     perm, targets, (const int32_t*)ws.plan, ws, (const Clock*)env.clock, cap);
+  hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, cap);
+  timing_end(env, 2);
+  return check_launch("resample");
+}
+
+int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                    const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
+                    const uint32_t* perm, const int32_t* targets, int32_t n_keys,
+                    const bz_resample_policy_t* pol) {
+  if (env.precision == BZ_F64)
+    return launch_resample_t<double>(env, in, out, p, counts, offsets, perm, targets, n_keys,
+                                     pol);
+  return launch_resample_t<float>(env, in, out, p, counts, offsets, perm, targets, n_keys, pol);
 }
 
 }  // namespace bz

@@ -89,6 +89,22 @@ class KineticEngine:
             self.V = torch.zeros(max(N - 1, 1), **f64)
             self.E = torch.zeros(N, **f64)
             self.chol = torch.from_numpy(ph.poisson_cholesky(N, cfg.dx).ravel()).to(self.dev)
+        # binning / resampling
+        self.nb = max(cfg.nb, 1)
+        self.n_keys = nn * self.nb
+        i32 = dict(dtype=torch.int32, device=self.dev)
+        self.counts = torch.zeros(self.n_keys + 1, **i32)
+        self.offsets = torch.zeros(self.n_keys + 2, **i32)
+        self.perm = torch.zeros(self.capacity, **i32)
+        fr = cfg.bin_fractions if len(cfg.bin_fractions) == self.nb else ph.bin_fractions(self.nb)
+        tg = np.array([int(f * cfg.target_ppc) for f in fr] * nn, dtype=np.int32)
+        self.targets = torch.from_numpy(tg).to(self.dev)
+        self.targets_host = tg
+        thr = cfg.bin_thresholds if len(cfg.bin_thresholds) == self.nb - 1 else \
+            ph.bin_thresholds(self.nb)
+        self.thr = (C.c_double * 8)(*thr)
+        self.policy = _lib.ResamplePolicy(cfg.split_below, cfg.merge_at, cfg.merge_keep,
+                                          cfg.trigger_at, cfg.min_wt, cfg.split_to, 0, 0, 0, 0)
 
     # ------------------------------------------------------------------ state
     @property
@@ -153,13 +169,52 @@ class KineticEngine:
             _tptr(self.Te), _tptr(self.g2), _tptr(self.V), _tptr(self.E), _tptr(self.E_t),
             _tptr(self.chol), 1 if advance else 0), "bz_field_step")
 
+    def bin_sort(self):
+        """Keys (cell, energy bin), per-key counts/offsets and the stable permutation."""
+        _lib.check(self.lib.bz_bin_sort(self.ctx, C.byref(self.state.struct),
+                                        C.byref(self.params), self.thr, self.nb,
+                                        _tptr(self.counts), _tptr(self.offsets),
+                                        _tptr(self.perm)), "bz_bin_sort")
+
+    def trigger(self, counts: np.ndarray) -> bool:
+        """Resample trigger on host counts (reference _check_for_resampling)."""
+        c = counts[:self.n_keys].astype(np.int64)
+        t = self.targets_host.astype(np.int64)
+        ok = (t > 0) & (c > 0)
+        if not self.policy.ref_trigger and self.policy.trigger_at <= 0:
+            return True
+        if self.policy.ref_trigger:
+            last = (np.arange(self.n_keys) % self.nb) == self.nb - 1
+            return bool(np.any(ok & last & (c >= 3.0 * t)))
+        return bool(np.any(ok & (c >= self.policy.trigger_at * t)))
+
+    def resample(self, force: bool = False) -> bool:
+        """Bin, sort and (when triggered) split / merge into the other buffer."""
+        self.bin_sort()
+        if not force and not self.trigger(self.counts.cpu().numpy()):
+            return False
+        src, dst = self.bufs[self.cur], self.bufs[1 - self.cur]
+        _lib.check(self.lib.bz_resample(self.ctx, C.byref(src.struct), C.byref(dst.struct),
+                                        C.byref(self.params), _tptr(self.counts),
+                                        _tptr(self.offsets), _tptr(self.perm),
+                                        _tptr(self.targets), self.n_keys,
+                                        C.byref(self.policy)), "bz_resample")
+        self.cur = 1 - self.cur
+        return True
+
     def initialize_fields(self):
         """Initial deposit and Poisson solve at t (boltzmann.py:363-399)."""
         self.deposit()
         if self.cfg.dim == 1:
             self.field_step(advance=False)
 
-    def step(self):
+    def step(self, index: Optional[int] = None):
+        """One kinetic step in the reference's order (boltzmann.py:570-678):
+        resampling check every `resample_every` steps, push + collisions +
+        deposit + compaction, then the grid step."""
+        k = self.cfg.resample_every
+        if k and self.cfg.target_ppc > 0 and index is not None and index % k == 0:
+            self.resample()
         self.push()
         self.field_step(advance=True)
 

@@ -84,7 +84,7 @@ class RunConfig:
     rf_freq: float = 13.56e6
     target_ppc: int = 0           # resampling target per cell (0: none)
     resample_every: int = 0       # 0: never
-    trigger_at: float = 3.0
+    trigger_at: float = 3.0      # <= 0: every check fires
     merge_at: float = 3.0
     split_below: float = 0.5
     merge_keep: float = 0.6

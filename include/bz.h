@@ -206,9 +206,12 @@ int bz_field_step(bz_ctx_t ctx, const bz_field_params_t* fp, const double* momen
                   double* ni, double* ns, double* Te, double* g2, double* V,
                   double* E, void* E_t, const double* chol, int32_t advance_time);
 
-/* Energy / cell binning and stable counting sort by key = cell*nb + ebin.
+/* Energy / cell binning and stable radix sort by key = cell*nb + ebin,
+ * cell = int((x + dx/2)/dx) (1bin/boltzmann.py:658), n_keys = n_nodes*nb <= 39999.
  * thresholds: [nb-1] eV, ascending (bin b holds thr[b-1] <= e < thr[b]).
- * counts[n_keys], offsets[n_keys+1], perm[n] (sorted position -> index).
+ * Particles with w <= 0 get the extra key n_keys and sort last.
+ * counts[n_keys+1], offsets[n_keys+2] (offsets[n_keys] = live particles),
+ * perm[n] (sorted position -> index); equal keys keep ascending index order.
  * Replaces bin_particles (1bin/kernels/bin_particles.py:7-31,
  * 3bin/kernels/bin_particles.py:39-78, 5bin/kernels/bin_particles.py:86-140). */
 int bz_bin_sort(bz_ctx_t ctx, const bz_state_t* s, const bz_params_t* p,

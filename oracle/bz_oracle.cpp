@@ -458,6 +458,267 @@ void orc_deposit_f32(const orc_params* P, int64_t n, const float* x, const float
   }
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Binning and ES-WPNC resampling
+// ---------------------------------------------------------------------------
+extern "C" {
+typedef struct {
+  double split_below, merge_at, merge_keep, trigger_at, min_wt, split_to;
+  int32_t ref_trigger, max_ppc, ref_rescale, pad;
+} orc_policy;
+}
+
+namespace {
+
+template <typename T>
+int bin_key(const orc_params& P, int nb, const double* thr, T x, T vx, T vy, T vz, T w,
+            int n_keys) {
+  if (!(w > (T)0)) return n_keys;
+  int cell = 0;
+  if (P.dim == 1) {
+    // curr_xbins = int((x + dx/2)/dx)  (boltzmann.py:658)
+    cell = (int)(((double)x + 0.5 * P.dx) / P.dx);
+    if (cell < 0) cell = 0;
+    if (cell >= P.n_nodes) cell = P.n_nodes - 1;
+  }
+  const T e = (T)P.kev * (vx * vx + vy * vy + vz * vz);
+  // bin_particles.py: LE < thr[0] <= ... ; bins ascend with energy
+  int b = 0;
+  for (int k = 0; k < nb - 1; ++k)
+    if (e >= (T)thr[k]) b = k + 1;
+  return cell * nb + b;
+}
+
+int ceil_log2(int64_t c) {
+  int b = 0;
+  while (((int64_t)1 << b) < c) ++b;
+  return b;
+}
+
+// first j with cdf[j] >= thr (left) or > thr (right), clamped to c-1
+int search(const std::vector<int64_t>& cdf, int64_t thr, bool right) {
+  auto it = right ? std::upper_bound(cdf.begin(), cdf.end(), thr)
+                  : std::lower_bound(cdf.begin(), cdf.end(), thr);
+  int j = (int)(it - cdf.begin());
+  return j < (int)cdf.size() ? j : (int)cdf.size() - 1;
+}
+
+template <typename T>
+void bin_sort(const orc_params& P, int nb, const double* thr, int64_t n, const T* x, const T* vx,
+              const T* vy, const T* vz, const T* w, int32_t* counts, int32_t* offsets,
+              uint32_t* perm) {
+  const int nn = P.dim == 1 ? P.n_nodes : 1;
+  const int n_keys = nn * nb;
+  std::vector<int> key(n);
+  for (int k = 0; k <= n_keys; ++k) counts[k] = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    key[i] = bin_key<T>(P, nb, thr, x[i], vx[i], vy[i], vz[i], w[i], n_keys);
+    counts[key[i]] += 1;
+  }
+  offsets[0] = 0;
+  for (int k = 0; k <= n_keys; ++k) offsets[k + 1] = offsets[k] + counts[k];
+  std::vector<int32_t> fill(offsets, offsets + n_keys + 1);
+  for (int64_t i = 0; i < n; ++i) perm[fill[key[i]]++] = (uint32_t)i;  // stable
+}
+
+template <typename T>
+int64_t resample(const orc_params& P, const orc_policy& pol, int nb, const double* thr,
+                 const int32_t* targets, uint64_t seed, uint64_t step, int64_t n, const T* x,
+                 const T* vx, const T* vy, const T* vz, const T* w, T* ox, T* ovx, T* ovy,
+                 T* ovz, T* ow, int64_t cap, uint64_t* cnt) {
+  const int nn = P.dim == 1 ? P.n_nodes : 1;
+  const int n_keys = nn * nb;
+  std::vector<int32_t> counts(n_keys + 1), offsets(n_keys + 2);
+  std::vector<uint32_t> perm(n > 0 ? n : 1);
+  bin_sort<T>(P, nb, thr, n, x, vx, vy, vz, w, counts.data(), offsets.data(), perm.data());
+  bool fire = !pol.ref_trigger && pol.trigger_at <= 0.0;  // <= 0: every check fires
+  for (int k = 0; k < n_keys; ++k) {
+    const int c = counts[k], t = targets[k];
+    if (t <= 0 || c <= 0) continue;
+    if (pol.ref_trigger) {
+      if (k % nb == nb - 1 && (double)c >= 3.0 * t) fire = true;
+    } else if ((double)c >= pol.trigger_at * t) {
+      fire = true;
+    }
+  }
+  int64_t o = 0;
+  auto emit = [&](T a, T b, T c_, T d, T e) {
+    if (o < cap) {
+      ox[o] = a;
+      ovx[o] = b;
+      ovy[o] = c_;
+      ovz[o] = d;
+      ow[o] = e;
+    } else {
+      cnt[6] += 1;
+    }
+    ++o;
+  };
+  for (int k = 0; k < n_keys; ++k) {
+    const int c = counts[k], t = targets[k];
+    const uint32_t* ids = perm.data() + offsets[k];
+    int kind = 0;
+    int draws = 0, keep = 0, ce = c;
+    if (fire && t > 0) {
+      keep = (int)(pol.merge_keep * (double)t);
+      if (c > 0 && (double)c < pol.split_below * t) {
+        draws = (int)(pol.split_to * (double)t) - c;
+        int hw = 0;
+        for (int j = 0; j < c; ++j) hw += (double)w[ids[j]] > pol.min_wt;
+        if (hw > 0 && draws > 0) kind = 1;
+      } else if ((double)c >= pol.merge_at * t && keep >= 1) {
+        ce = (pol.max_ppc > 0 && c > pol.max_ppc) ? pol.max_ppc : c;
+        if (keep < ce) kind = 2;
+      }
+    }
+    if (kind == 0) {
+      for (int j = 0; j < c; ++j) {
+        const uint32_t i = ids[j];
+        emit(x[i], vx[i], vy[i], vz[i], w[i]);
+      }
+    } else if (kind == 1) {
+      // split (boltzmann.py:806-823): draws with replacement, p ~ w over w > min_wt
+      cnt[8] += 1;
+      double wmax = 0;
+      for (int j = 0; j < c; ++j)
+        if ((double)w[ids[j]] > pol.min_wt) wmax = std::max(wmax, (double)w[ids[j]]);
+      int ew;
+      frexp(wmax, &ew);
+      const int S = 61 - ew - ceil_log2(c);
+      std::vector<int64_t> cdf(c);
+      int64_t acc = 0;
+      for (int j = 0; j < c; ++j) {
+        const double wj = (double)w[ids[j]];
+        acc += wj > pol.min_wt ? (int64_t)ldexp(wj, S) : 0;
+        cdf[j] = acc;
+      }
+      std::vector<int> copies(c, 0);
+      for (int m = 0; m < draws; ++m) {
+        const double u = orc_resample_uniform(seed, step, (uint32_t)k, (uint32_t)m);
+        const int64_t th = (int64_t)(u * (double)acc);
+        copies[search(cdf, th, true)] += 1;
+      }
+      for (int j = 0; j < c; ++j) {
+        const uint32_t i = ids[j];
+        const int reps = 1 + copies[j];
+        T wn = w[i];
+        if (reps > 1) wn = wn / (T)reps;
+        for (int r = 0; r < reps; ++r) emit(x[i], vx[i], vy[i], vz[i], wn);
+      }
+    } else {
+      // merge (boltzmann.py:825-877)
+      cnt[9] += 1;
+      double W = 0, Px = 0, Py = 0, Pz = 0, En = 0, wmax = 0;
+      for (int j = 0; j < ce; ++j) {
+        const uint32_t i = ids[j];
+        const double wi = w[i], a = vx[i], b = vy[i], cc = vz[i];
+        W += wi;
+        Px += wi * a;
+        Py += wi * b;
+        Pz += wi * cc;
+        En += wi * (a * a + b * b + cc * cc);
+        wmax = std::max(wmax, wi);
+      }
+      const double mx = Px / W, my = Py / W, mz = Pz / W;
+      double Cold = 0;
+      for (int j = 0; j < ce; ++j) {
+        const uint32_t i = ids[j];
+        const double a = vx[i] - mx, b = vy[i] - my, cc = vz[i] - mz;
+        Cold += (double)w[i] * (a * a + b * b + cc * cc);
+      }
+      int ew;
+      frexp(wmax, &ew);
+      const int S = 61 - ew - ceil_log2(ce);
+      std::vector<int64_t> cdf(ce);
+      int64_t acc = 0;
+      for (int j = 0; j < ce; ++j) {
+        acc += (int64_t)ldexp((double)w[ids[j]], S);
+        cdf[j] = acc;
+      }
+      const int K = keep;
+      const double wbar = W / (double)K;
+      std::vector<uint32_t> sel(K);
+      double Sx = 0, Sy = 0, Sz = 0, Se = 0;
+      for (int m = 0; m < K; ++m) {
+        const double u = orc_resample_uniform(seed, step, (uint32_t)k, (uint32_t)m);
+        const double r = (1.0 / (double)K) * (u + (double)m);
+        const int64_t th = (int64_t)(r * (double)acc);
+        sel[m] = ids[search(cdf, th, false)];
+        const double a = vx[sel[m]], b = vy[sel[m]], cc = vz[sel[m]];
+        Sx += a;
+        Sy += b;
+        Sz += cc;
+        Se += a * a + b * b + cc * cc;
+      }
+      double hx, hy, hz, f;
+      if (pol.ref_rescale) {
+        hx = wbar * Sx / W;
+        hy = wbar * Sy / W;
+        hz = wbar * Sz / W;
+        const double Tc = (2.0 / 3.0) * (P.kev * En) / W;
+        const double Th = (2.0 / 3.0) * (wbar * (P.kev * Se)) / W;
+        f = Th > 0 ? sqrt(Tc / Th) : 1.0;
+      } else {
+        hx = Sx / K;
+        hy = Sy / K;
+        hz = Sz / K;
+        double Cn = 0;
+        for (int m = 0; m < K; ++m) {
+          const double a = vx[sel[m]] - hx, b = vy[sel[m]] - hy, cc = vz[sel[m]] - hz;
+          Cn += a * a + b * b + cc * cc;
+        }
+        Cn *= wbar;
+        f = Cn > 0 ? sqrt(Cold / Cn) : 1.0;
+      }
+      for (int m = 0; m < K; ++m) {
+        const uint32_t i = sel[m];
+        emit(x[i], (T)(mx + ((double)vx[i] - hx) * f), (T)(my + ((double)vy[i] - hy) * f),
+             (T)(mz + ((double)vz[i] - hz) * f), (T)wbar);
+      }
+      for (int j = ce; j < c; ++j) {
+        const uint32_t i = ids[j];
+        emit(x[i], vx[i], vy[i], vz[i], w[i]);
+      }
+    }
+  }
+  if (fire) cnt[10] += 1;
+  return o < cap ? o : cap;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_bin_sort_f64(const orc_params* P, int nb, const double* thr, int64_t n, const double* x,
+                      const double* vx, const double* vy, const double* vz, const double* w,
+                      int32_t* counts, int32_t* offsets, uint32_t* perm) {
+  bin_sort<double>(*P, nb, thr, n, x, vx, vy, vz, w, counts, offsets, perm);
+}
+void orc_bin_sort_f32(const orc_params* P, int nb, const double* thr, int64_t n, const float* x,
+                      const float* vx, const float* vy, const float* vz, const float* w,
+                      int32_t* counts, int32_t* offsets, uint32_t* perm) {
+  bin_sort<float>(*P, nb, thr, n, x, vx, vy, vz, w, counts, offsets, perm);
+}
+
+int64_t orc_resample_f64(const orc_params* P, const orc_policy* pol, int nb, const double* thr,
+                         const int32_t* targets, uint64_t seed, uint64_t step, int64_t n,
+                         const double* x, const double* vx, const double* vy, const double* vz,
+                         const double* w, double* ox, double* ovx, double* ovy, double* ovz,
+                         double* ow, int64_t cap, uint64_t* cnt) {
+  return resample<double>(*P, *pol, nb, thr, targets, seed, step, n, x, vx, vy, vz, w, ox, ovx,
+                          ovy, ovz, ow, cap, cnt);
+}
+int64_t orc_resample_f32(const orc_params* P, const orc_policy* pol, int nb, const double* thr,
+                         const int32_t* targets, uint64_t seed, uint64_t step, int64_t n,
+                         const float* x, const float* vx, const float* vy, const float* vz,
+                         const float* w, float* ox, float* ovx, float* ovy, float* ovz, float* ow,
+                         int64_t cap, uint64_t* cnt) {
+  return resample<float>(*P, *pol, nb, thr, targets, seed, step, n, x, vx, vy, vz, w, ox, ovx,
+                         ovy, ovz, ow, cap, cnt);
+}
+
 // Grid step: moment update, heavies, Poisson (FP64).
 //   chol = [diag(N-1), lower(N-1)] of the banded Cholesky factor (upper form,
 //   as returned by scipy.linalg.cholesky_banded for boltzmann.py:744-755).

@@ -185,3 +185,53 @@ def field_step(N, dx, dt, eps, v0, freq, mu_i, D_i, D_s, fluid_on, poisson_on, t
                        C.c_int(fluid_on), C.c_int(poisson_on), C.c_double(t),
                        *[_ptr(a) for a in arrs])
     return arrs[1:7]
+
+
+class OrcPolicy(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("split_below", "merge_at", "merge_keep", "trigger_at",
+                                          "min_wt", "split_to")] + [
+        (n, C.c_int32) for n in ("ref_trigger", "max_ppc", "ref_rescale", "pad")]
+
+
+def policy(split_below=0.5, merge_at=3.0, merge_keep=0.6, trigger_at=3.0, min_wt=0.1,
+           split_to=2.0, ref_trigger=0, max_ppc=0, ref_rescale=0) -> OrcPolicy:
+    return OrcPolicy(split_below, merge_at, merge_keep, trigger_at, min_wt, split_to,
+                     ref_trigger, max_ppc, ref_rescale, 0)
+
+
+def bin_sort(setup: Setup, nb: int, thresholds, state):
+    """Keys, counts[n_keys+1], offsets[n_keys+2] and the stable permutation."""
+    lib = load()
+    x, vx, vy, vz, w = [np.ascontiguousarray(a) for a in state]
+    p = setup.params()
+    n_keys = p.n_nodes * nb
+    thr = np.ascontiguousarray(np.asarray(thresholds, dtype=np.float64).reshape(-1))
+    thr = np.concatenate([thr, np.zeros(1)])
+    counts = np.zeros(n_keys + 1, dtype=np.int32)
+    offsets = np.zeros(n_keys + 2, dtype=np.int32)
+    perm = np.zeros(max(x.size, 1), dtype=np.uint32)
+    fn = lib.orc_bin_sort_f64 if x.dtype == np.float64 else lib.orc_bin_sort_f32
+    fn(C.byref(p), C.c_int(nb), _ptr(thr), C.c_int64(x.size),
+       *[_ptr(a) for a in (x, vx, vy, vz, w)], _ptr(counts), _ptr(offsets), _ptr(perm))
+    return counts, offsets, perm[:x.size]
+
+
+def resample(setup: Setup, pol: OrcPolicy, nb: int, thresholds, targets, seed: int, step: int,
+             state, capacity=None):
+    """Binning + sort + ES-WPNC split/merge; output in key order.
+    Returns (new_state, counters[16])."""
+    lib = load()
+    x, vx, vy, vz, w = [np.ascontiguousarray(a) for a in state]
+    dt = x.dtype
+    p = setup.params()
+    thr = np.concatenate([np.asarray(thresholds, dtype=np.float64).reshape(-1), np.zeros(1)])
+    tg = np.ascontiguousarray(np.asarray(targets, dtype=np.int32))
+    cap = capacity if capacity is not None else 4 * x.size + 1024
+    out = [np.zeros(cap, dtype=dt) for _ in range(5)]
+    cnt = np.zeros(16, dtype=np.uint64)
+    fn = lib.orc_resample_f64 if dt == np.float64 else lib.orc_resample_f32
+    fn.restype = C.c_int64
+    m = fn(C.byref(p), C.byref(pol), C.c_int(nb), _ptr(thr), _ptr(tg), C.c_uint64(seed),
+           C.c_uint64(step), C.c_int64(x.size), *[_ptr(a) for a in (x, vx, vy, vz, w)],
+           *[_ptr(a) for a in out], C.c_int64(cap), _ptr(cnt))
+    return tuple(a[:m] for a in out), cnt
