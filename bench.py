@@ -72,6 +72,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--shape", type=int, default=-1, help="push tile shape (-1: autotune)")
+    ap.add_argument("--sched", type=int, default=0,
+                    help="push tile schedule: 0 static + cooperative launch, 1 ticket counter")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the particle "
@@ -87,19 +89,28 @@ def main(argv=None) -> int:
 
     cfg = configs.get(args.workload)
     eng = KineticEngine(cfg)
-    if args.shape >= 0:
-        _lib.check(eng.lib.bz_set_push_shape(eng.ctx, args.shape))
+    _lib.check(eng.lib.bz_set_push_schedule(eng.ctx, args.sched))
     eng.load_state(ph.initial_state(cfg))
     eng.initialize_fields()
+    index = 0
+    tuned = None
+    if args.shape >= 0:
+        _lib.check(eng.lib.bz_set_push_shape(eng.ctx, args.shape))
+    else:
+        from boltzmann_amd.autotune import tune_push
+        tuned = tune_push(eng, steps=10)
+        index += tuned["steps_used"]
     for _ in range(args.warmup):
-        eng.step()
+        eng.step(index)
+        index += 1
     torch.cuda.synchronize()
     c0 = eng.counters()
     _lib.check(eng.lib.bz_timing_enable(eng.ctx, 1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        eng.step()
+        eng.step(index)
+        index += 1
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     import ctypes as C
@@ -132,12 +143,16 @@ def main(argv=None) -> int:
         "data": "synthetic (Maxwellian electrons, uniform density, tabulated argon-like sigma)",
         "config": {"workload": cfg.name, "particles_start": cfg.n_particles,
                    "particles_end": n_end, "cells": cfg.n_cells if cfg.dim == 1 else 0,
-                   "dt_s": cfg.dt, "parallelism": "single GPU"},
+                   "dt_s": cfg.dt, "parallelism": "single GPU", "push_schedule": args.sched},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_push", "kernel_avg_us": avg_push_s * 1e6,
                      "bytes_per_particle_step": bytes_per_pstep},
-        "counters": {k: c1[k] - c0[k] for k in ("births", "absorbed", "ionization", "overflow")},
+        "counters": {k: c1[k] - c0[k] for k in ("births", "absorbed", "ionization", "overflow",
+                                                "resample", "split", "merge")},
+        "kernel_ms": {"push": ms[0], "sort": ms[1], "resample": ms[2],
+                      "launches": [int(nl[0]), int(nl[1]), int(nl[2])]},
+        "autotune": tuned,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)

@@ -36,7 +36,7 @@ def c1() -> RunConfig:
                      precision=_lib.BZ_F32, nb=3, n_cells=256, length_cm=2.5,
                      dt=1.0 / (RF * 400), steps=20 * 400, gas_density_m3=3.22e21,
                      te_ev=2.0, density_cm3=1e9, rf_voltage=200.0, rf_freq=RF,
-                     target_ppc=4000, resample_every=1, trigger_at=2.0, merge_at=2.0,
+                     target_ppc=4000, resample_every=10, trigger_at=2.0, merge_at=2.0,
                      bin_fractions=bin_fractions(3), bin_thresholds=bin_thresholds(3))
 
 

@@ -165,6 +165,13 @@ int bz_set_push_shape(bz_ctx_t c, int shape) {
   return 0;
 }
 
+int bz_set_push_schedule(bz_ctx_t c, int sched) {
+  if (!c) return fail("null context");
+  if (sched != 0 && sched != 1) return fail("push schedule must be 0 or 1");
+  c->env.push_sched = sched;
+  return 0;
+}
+
 int bz_timing_enable(bz_ctx_t c, int on) {
   BZ_CTX(c);
   if (!c->env.timing) {

@@ -141,40 +141,71 @@ __device__ __forceinline__ void st_rel(uint64_t* p, uint64_t v) {
 constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagPre = 2ull << 62;
 
-// Decoupled look-back (single-pass chained scan).  Called by all 64 lanes of
-// one wave; returns the exclusive prefix of tile t and publishes its
-// inclusive prefix.  A tile's predecessors were claimed (by ticket) before
-// it, by blocks that are running, and they publish their aggregates without
-// waiting on anyone, so every spin below ends.
-__device__ __forceinline__ uint32_t lookback(uint64_t* status, uint32_t t, uint32_t agg,
-                                             uint32_t epoch) {
-  const int lane = lane_id();
+// Block-wide decoupled look-back (single-pass chained scan): every thread
+// inspects LB predecessor status words per round (window BLOCK*LB), so a
+// tile whose predecessors are all in flight at once sums their aggregates in
+// one or two rounds instead of walking back one 64-tile window per memory
+// round trip.  All threads of the block call it; returns the exclusive
+// prefix and publishes the inclusive one.  s_min / s_red are LDS scratch
+// (1 and BLOCK/64 words).  Tiles are processed by co-resident blocks in
+// increasing order per block and publish their aggregates before looking
+// back, so every wait below ends.
+template <int BLOCK, int LB>
+__device__ __forceinline__ uint32_t lookback_block(uint64_t* status, uint32_t t, uint32_t agg,
+                                                   uint32_t epoch, uint32_t* s_min,
+                                                   uint32_t* s_red) {
+  constexpr int NW = BLOCK / kWave;
+  constexpr int WIN = BLOCK * LB;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wid = tid / kWave;
   const uint64_t ep = (uint64_t)(epoch & 0x3FFFFFFFu) << 32;
   if (t == 0) {
-    if (lane == 0) st_rel(&status[0], kFlagPre | ep | agg);
+    if (tid == 0) st_rel(&status[0], kFlagPre | ep | agg);
     return 0;
   }
-  if (lane == 0) st_rel(&status[t], kFlagAgg | ep | agg);
+  if (tid == 0) st_rel(&status[t], kFlagAgg | ep | agg);
   uint32_t excl = 0;
   int64_t j = (int64_t)t - 1;
   while (true) {
-    const int64_t idx = j - lane;
-    uint64_t s = kFlagPre;  // virtual tile before 0: prefix 0
-    if (idx >= 0) {
-      do {
-        s = ld_acq(&status[idx]);
-      } while ((s >> 62) == 0 || (s & (0x3FFFFFFFull << 32)) != ep);
+    uint64_t s[LB];
+    uint32_t my_first = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < LB; ++k) {
+      const int off = tid + k * BLOCK;  // distance back from j
+      const int64_t idx = j - off;
+      s[k] = kFlagPre;  // virtual tile before 0: prefix 0
+      if (idx >= 0) {
+        s[k] = ld_acq(&status[idx]);
+        while ((s[k] >> 62) == 0 || (s[k] & (0x3FFFFFFFull << 32)) != ep) {
+          __builtin_amdgcn_s_sleep(2);  // back off instead of hammering the line
+          s[k] = ld_acq(&status[idx]);
+        }
+      }
+      if ((s[k] >> 62) == 2 && (uint32_t)off < my_first) my_first = (uint32_t)off;
     }
-    const uint64_t pre = __ballot((s >> 62) == 2);
-    if (pre) {
-      const int first = __ffsll((unsigned long long)pre) - 1;
-      excl += wave_sum((uint32_t)(lane <= first ? (uint32_t)s : 0u));
-      break;
+    if (tid == 0) *s_min = 0xFFFFFFFFu;
+    __syncthreads();
+    if (my_first != 0xFFFFFFFFu) atomicMin(s_min, my_first);
+    __syncthreads();
+    const uint32_t first = *s_min;  // nearest predecessor holding a prefix
+    uint32_t part = 0;
+#pragma unroll
+    for (int k = 0; k < LB; ++k) {
+      const uint32_t off = (uint32_t)(tid + k * BLOCK);
+      if (off <= first) part += (uint32_t)s[k];
     }
-    excl += wave_sum((uint32_t)s);
-    j -= kWave;
+    part = wave_sum(part);
+    if (lane == 0) s_red[wid] = part;
+    __syncthreads();
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) tot += s_red[w];
+    excl += tot;
+    __syncthreads();
+    if (first != 0xFFFFFFFFu) break;
+    j -= WIN;
   }
-  if (lane == 0) st_rel(&status[t], kFlagPre | ep | (uint32_t)(agg + excl));
+  if (tid == 0) st_rel(&status[t], kFlagPre | ep | (uint32_t)(agg + excl));
   return excl;
 }
 

@@ -18,7 +18,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
                                                        const double* __restrict__ chol,
                                                        double* Ji, double* Js, Clock* clk,
                                                        int advance) {
-  extern __shared__ double s_f[];  // rhs[N-1], tempy[N-1]
+  extern __shared__ double s_f[];  // rhs, tempy, 1/diag, lower [N-1 each], scan maps
   const int N = fp.n;
   const int nn = N + 1;
   const int tid = threadIdx.x;
@@ -77,17 +77,107 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
   if (fp.poisson_on) {
     const double dx = fp.dx;
     const double vl = fp.v0 * sin(2.0 * 3.141592653589793 * fp.freq * t);
-    for (int i = tid; i < N - 1; i += kFieldBlock) rhs[i] = (1.0 / fp.eps) * (ni[i + 1] - ne[i + 1]);
-    __syncthreads();
-    if (tid == 0) {
-      rhs[0] = rhs[0] + (1.0 / (dx * dx)) * fp.v0 * sin(2.0 * 3.141592653589793 * fp.freq * t);
-      ty[0] = rhs[0] / diag[0];
-      for (int i = 1; i < N - 1; ++i) ty[i] = (1.0 / diag[i]) * (rhs[i] - low[i] * ty[i - 1]);
-      V[N - 2] = ty[N - 2] / diag[N - 2];
-      for (int i = 1; i < N - 1; ++i)
-        V[N - 2 - i] = (1.0 / diag[N - 2 - i]) * (ty[N - 2 - i] - low[N - 1 - i] * V[N - 1 - i]);
+    const int M = N - 1;
+    double* inv = s_f + 2 * M;  // 1/diag
+    double* lw = s_f + 3 * M;   // lower band
+    double* sa = s_f + 4 * M;   // per-thread scan maps [kFieldBlock]
+    double* sb = sa + kFieldBlock;
+    for (int i = tid; i < M; i += kFieldBlock) {
+      rhs[i] = (1.0 / fp.eps) * (ni[i + 1] - ne[i + 1]);
+      inv[i] = 1.0 / diag[i];
+      lw[i] = low[i];
     }
     __syncthreads();
+    if (tid == 0)
+      rhs[0] = rhs[0] + (1.0 / (dx * dx)) * fp.v0 * sin(2.0 * 3.141592653589793 * fp.freq * t);
+    __syncthreads();
+    // Both triangular solves are first-order linear recurrences
+    // y_i = A_i y_{i-1} + B_i; each thread composes the maps of its chunk,
+    // a block scan composes across chunks, then each chunk is re-run from
+    // its incoming value with the reference's per-row expression.
+    const int chunk = (M + kFieldBlock - 1) / kFieldBlock;
+    const int lo = tid * chunk;
+    const int hi = lo + chunk < M ? lo + chunk : M;
+    // forward: U^T y = rhs (EF_kernel_1D.py:31-33)
+    {
+      double A = 1.0, B = 0.0;
+      for (int i = lo; i < hi; ++i) {
+        double ai, bi;
+        if (i == 0) {
+          ai = 0.0;
+          bi = rhs[0] / diag[0];
+        } else {
+          ai = -(inv[i] * lw[i]);
+          bi = inv[i] * rhs[i];
+        }
+        B = ai * B + bi;
+        A = ai * A;
+      }
+      sa[tid] = A;
+      sb[tid] = B;
+      __syncthreads();
+      for (int d = 1; d < kFieldBlock; d <<= 1) {
+        double pa = 1.0, pb = 0.0;
+        if (tid >= d) {
+          pa = sa[tid - d];
+          pb = sb[tid - d];
+        }
+        __syncthreads();
+        if (tid >= d) {
+          const double ca = sa[tid], cb = sb[tid];
+          sa[tid] = ca * pa;
+          sb[tid] = ca * pb + cb;
+        }
+        __syncthreads();
+      }
+      double y = tid > 0 ? sb[tid - 1] : 0.0;
+      for (int i = lo; i < hi; ++i) {
+        y = (i == 0) ? rhs[0] / diag[0] : inv[i] * (rhs[i] - lw[i] * y);
+        ty[i] = y;
+      }
+      __syncthreads();
+    }
+    // backward: U V = y (EF_kernel_1D.py:35-37), index r = M-1-i
+    {
+      double A = 1.0, B = 0.0;
+      for (int r = lo; r < hi; ++r) {
+        const int i = M - 1 - r;
+        double ai, bi;
+        if (r == 0) {
+          ai = 0.0;
+          bi = ty[i] / diag[i];
+        } else {
+          ai = -(inv[i] * lw[i + 1]);
+          bi = inv[i] * ty[i];
+        }
+        B = ai * B + bi;
+        A = ai * A;
+      }
+      sa[tid] = A;
+      sb[tid] = B;
+      __syncthreads();
+      for (int d = 1; d < kFieldBlock; d <<= 1) {
+        double pa = 1.0, pb = 0.0;
+        if (tid >= d) {
+          pa = sa[tid - d];
+          pb = sb[tid - d];
+        }
+        __syncthreads();
+        if (tid >= d) {
+          const double ca = sa[tid], cb = sb[tid];
+          sa[tid] = ca * pa;
+          sb[tid] = ca * pb + cb;
+        }
+        __syncthreads();
+      }
+      double v = tid > 0 ? sb[tid - 1] : 0.0;
+      for (int r = lo; r < hi; ++r) {
+        const int i = M - 1 - r;
+        v = (r == 0) ? ty[i] / diag[i] : inv[i] * (ty[i] - lw[i + 1] * v);
+        V[i] = v;
+      }
+      __syncthreads();
+    }
     for (int i = tid; i < N; i += kFieldBlock) {
       double e;
       if (i == 0) e = -(V[0] - vl) / dx;
@@ -111,8 +201,9 @@ int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double
                  double* ni, double* ns, double* Te, double* g2, double* V, double* E,
                  void* E_t, const double* chol, int advance) {
   const int N = fp->n;
-  const size_t smem = (size_t)2 * (N - 1) * sizeof(double);
-  if (smem > 160 * 1024) return fail("field solve limited to N <= 10240 cells");
+  // rhs, y, 1/diag, lower band (N-1 each) + scan maps (2 per thread)
+  const size_t smem = ((size_t)4 * (N - 1) + 2 * kFieldBlock) * sizeof(double);
+  if (smem > 160 * 1024) return fail("field solve limited to N <= 4865 cells (LDS)");
   if (smem > 64 * 1024) {
     if (hipFuncSetAttribute((const void*)k_field, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)smem) != hipSuccess)

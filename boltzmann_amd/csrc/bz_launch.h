@@ -26,6 +26,7 @@ struct LaunchEnv {
   int32_t max_nodes;
   int32_t max_keys;
   int push_shape;            // tile shape picked by the autotuner
+  int push_sched;            // 0 static tiles + cooperative launch, 1 ticket counter
   // optional kernel timing: event pairs per timed kernel class
   struct Timing* timing;
 };

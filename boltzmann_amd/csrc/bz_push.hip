@@ -25,8 +25,9 @@ struct PushArgs {
   const void* G;
   Clock* clk;
   uint64_t* status;
-  double* partials;  // [gridDim][3][n_nodes]
+  double* partials;  // [gridDim][3][n_nodes] doubles, then [gridDim][8] u32 counters
   int64_t capacity;
+  int sched;         // 0 static tile order (cooperative launch), 1 ticket counter
 };
 
 template <typename T>
@@ -38,15 +39,35 @@ __device__ __forceinline__ void store5(void* const* out, int64_t pos, const P5<T
   ((T*)out[4])[pos] = q.w;
 }
 
+template <typename T>
+__device__ __forceinline__ void load5(const T* const* in, int64_t i, int64_t n, P5<T>& q) {
+  if (i < n) {
+    q.x = in[0][i];
+    q.vx = in[1][i];
+    q.vy = in[2][i];
+    q.vz = in[3][i];
+    q.w = in[4][i];
+  } else {
+    q.x = q.vx = q.vy = q.vz = q.w = (T)0;
+  }
+}
+
+// LDS bytes of the staged cross-section table (energy, 3 sigma, 3 slopes, hints)
+__host__ __device__ inline size_t table_lds_bytes(int n, int n_hint, int word) {
+  return ((size_t)7 * n * word + (size_t)n_hint * 4 + 15) & ~(size_t)15;
+}
+
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   static_assert(ITEMS >= 1 && ITEMS <= 4, "packed scan holds 4 fields of 16 bits");
   constexpr int NW = BLOCK / kWave;
   constexpr int TILE = BLOCK * ITEMS;
-  extern __shared__ double s_dep[];  // [3][n_nodes] (DIM 1) or [3] (DIM 0)
+  constexpr int LB = 1024 / BLOCK > 0 ? 1024 / BLOCK : 1;  // look-back window 1024 tiles
+  extern __shared__ double s_dep[];  // [3][n_nodes] (DIM 1) or [3] (DIM 0), then the table
   __shared__ uint64_t s_wsum[NW];
-  __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_prefix;
+  __shared__ uint32_t s_lbmin;
+  __shared__ uint32_t s_lbred[NW];
+  __shared__ uint32_t s_next;
   __shared__ unsigned long long s_cnt[8];
 
   const bz_params_t& p = a.p;
@@ -56,6 +77,26 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   const int nn = (DIM == 1) ? p.n_nodes : 1;
   for (int j = tid; j < 3 * nn; j += BLOCK) s_dep[j] = 0.0;
   if (tid < 8) s_cnt[tid] = 0ull;
+  // stage the cross-section table in LDS (read-only for the whole launch)
+  bz_xsec_t xs = a.xs;
+  if (XS == BZ_XS_TABLE) {
+    const int n = a.xs.n;
+    T* s_tab = (T*)(s_dep + 3 * nn);
+    int32_t* s_hint = (int32_t*)(s_tab + 7 * n);
+    const T* ge = (const T*)a.xs.energy;
+    const T* gs = (const T*)a.xs.sig;
+    const T* gd = (const T*)a.xs.slope;
+    for (int j = tid; j < n; j += BLOCK) s_tab[j] = ge[j];
+    for (int j = tid; j < 3 * n; j += BLOCK) {
+      s_tab[n + j] = gs[j];
+      s_tab[4 * n + j] = gd[j];
+    }
+    for (int j = tid; j < a.xs.n_hint; j += BLOCK) s_hint[j] = a.xs.hint[j];
+    xs.energy = s_tab;
+    xs.sig = s_tab + n;
+    xs.slope = s_tab + 4 * n;
+    xs.hint = s_hint;
+  }
 
   const int64_t n = a.clk->n;
   const uint64_t step = a.clk->step;
@@ -66,43 +107,39 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     atomicAdd(&a.clk->cnt[BZ_CNT_PSTEPS], (unsigned long long)n);
   }
 
-  const T* __restrict__ ix = (const T*)a.in[0];
-  const T* __restrict__ ivx = (const T*)a.in[1];
-  const T* __restrict__ ivy = (const T*)a.in[2];
-  const T* __restrict__ ivz = (const T*)a.in[3];
-  const T* __restrict__ iw = (const T*)a.in[4];
+  const T* in[5] = {(const T*)a.in[0], (const T*)a.in[1], (const T*)a.in[2], (const T*)a.in[3],
+                    (const T*)a.in[4]};
   const T* __restrict__ E = (const T*)a.E;
   const T* R = (const T*)a.R;
   const T* G = (const T*)a.G;
 
   uint32_t c_el = 0, c_ex = 0, c_io = 0, c_abs = 0, c_null = 0, c_over = 0, c_drop = 0;
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;  // DIM 0 deposit in registers
-  __syncthreads();
 
-  while (true) {
-    if (tid == 0) s_tile = atomicAdd(&a.clk->ticket, 1u);
-    __syncthreads();
-    const uint32_t t = s_tile;
-    if (t >= ntiles) break;
+  // Tile schedule.  sched 0: static order b, b+G, ... (cooperative launch, so
+  // every block is resident).  sched 1: tiles claimed from a ticket counter by
+  // running blocks (any residency).  Either way a tile's look-back only waits
+  // on tiles owned by blocks that are running, so every wait ends.
+  if (tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : blockIdx.x;
+  __syncthreads();
+  uint32_t t = s_next;
+  __syncthreads();  // s_next is rewritten at the top of the loop
+  P5<T> cur[ITEMS];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) load5<T>(in, (int64_t)t * TILE + k * BLOCK + tid, n, cur[k]);
+
+  while (t < ntiles) {
     const int64_t base = (int64_t)t * TILE;
+    // claim the next tile now; it is read after the scan barriers below
+    if (tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : t + gridDim.x;
 
     StepOut<T> o[ITEMS];
     uint64_t packed = 0;
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const int64_t i = base + (int64_t)k * BLOCK + tid;
-      P5<T> q;
-      if (i < n) {
-        q.x = ix[i];
-        q.vx = ivx[i];
-        q.vy = ivy[i];
-        q.vz = ivz[i];
-        q.w = iw[i];
-      } else {
-        q.x = q.vx = q.vy = q.vz = q.w = (T)0;
-      }
-      o[k] = step_particle<T, XS, DIM, RNG>(q, (uint32_t)i, step, p, a.xs, E, R, G);
-      if (i < n && q.w > (T)0) {
+      o[k] = step_particle<T, XS, DIM, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G);
+      if (i < n && cur[k].w > (T)0) {
         c_el += o[k].chan == kChanElastic;
         c_ex += o[k].chan == kChanExc;
         c_io += o[k].chan == kChanIon;
@@ -154,6 +191,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     __syncthreads();
     const uint64_t excl = (wid ? s_wsum[wid - 1] : 0ull) + incl - packed;
     const uint64_t tot = s_wsum[NW - 1];
+    const uint32_t t_next = s_next;
     uint32_t field_base[ITEMS];
     uint32_t agg = 0;
 #pragma unroll
@@ -161,15 +199,13 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       field_base[k] = agg;
       agg += (uint32_t)((tot >> (16 * k)) & 0xFFFFu);
     }
-    if (wid == 0) {
-      const uint32_t pre = lookback(a.status, t, agg, epoch);
-      if (lane == 0) {
-        s_prefix = pre;
-        if (t == ntiles - 1) a.clk->n_next = (int64_t)(pre + agg);
-      }
-    }
-    __syncthreads();
-    const int64_t tile_pre = s_prefix;
+    // prefetch the next tile so its HBM latency overlaps the look-back
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+      load5<T>(in, (int64_t)t_next * TILE + k * BLOCK + tid, n, cur[k]);
+    const uint32_t pre = lookback_block<BLOCK, LB>(a.status, t, agg, epoch, &s_lbmin, s_lbred);
+    if (tid == 0 && t == ntiles - 1) a.clk->n_next = (int64_t)(pre + agg);
+    const int64_t tile_pre = pre;
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const int64_t pos =
@@ -184,50 +220,78 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
         else ++c_drop;
       }
     }
-    // s_tile / s_wsum / s_prefix are rewritten only after the next barrier
+    t = t_next;
+    // s_wsum, s_next and the look-back scratch are rewritten only after the
+    // look-back's barriers, which every thread has passed
   }
 
-  // counters: wave reduce, one atomic per wave
+  // counters: wave sums -> LDS -> one row per block, summed by k_dep_reduce
   {
-    const uint32_t v[7] = {c_io, c_abs, c_el, c_ex, c_io, c_null, c_drop};
-    const int slot[7] = {BZ_CNT_BIRTHS, BZ_CNT_ABSORBED, BZ_CNT_ELASTIC, BZ_CNT_EXCITATION,
-                         BZ_CNT_IONIZATION, BZ_CNT_NULL, BZ_CNT_OVERFLOW};
+    const uint32_t v[8] = {c_io, c_abs, c_el, c_ex, c_io, c_null, c_drop, c_over};
 #pragma unroll
-    for (int q = 0; q < 7; ++q) {
+    for (int q = 0; q < 8; ++q) {
       const uint32_t s = wave_sum(v[q]);
-      if (lane == 0 && s) atomicAdd(&a.clk->cnt[slot[q]], (unsigned long long)s);
+      if (lane == 0 && s) atomicAdd(&s_cnt[q], (unsigned long long)s);
     }
-    const uint32_t so = wave_sum(c_over);
-    if (lane == 0 && so) atomicAdd(&a.clk->cnt[BZ_CNT_NUMAX], (unsigned long long)so);
   }
   if (DIM == 0) {
+    // per-wave sums combined in wave order (no LDS atomics: reproducible)
+    __shared__ double s_w0[3][NW];
     acc0 = wave_sum(acc0);
     acc1 = wave_sum(acc1);
     acc2 = wave_sum(acc2);
     if (lane == 0) {
-      atomicAdd(&s_dep[0], acc0);
-      atomicAdd(&s_dep[1], acc1);
-      atomicAdd(&s_dep[2], acc2);
+      s_w0[0][wid] = acc0;
+      s_w0[1][wid] = acc1;
+      s_w0[2][wid] = acc2;
+    }
+    __syncthreads();
+    if (tid < 3) {
+      double s = 0.0;
+      for (int w = 0; w < NW; ++w) s += s_w0[tid][w];
+      s_dep[tid] = s;
     }
   }
   __syncthreads();
   double* part = a.partials + (size_t)blockIdx.x * 3 * nn;
   for (int j = tid; j < 3 * nn; j += BLOCK) part[j] = s_dep[j];
+  uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * 3 * nn);
+  if (tid < 8) bcnt[(size_t)blockIdx.x * 8 + tid] = (uint32_t)s_cnt[tid];
 }
 
 // Sum the per-block partial histograms in block order and commit the clock.
-__global__ void k_dep_reduce(const double* __restrict__ partials, int nblocks, int m,
-                             double* __restrict__ out, Clock* clk, int commit) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per output: lane l sums blocks l, l+64, ... in order, then a fixed
+// butterfly combines the lanes, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_dep_reduce(const double* __restrict__ partials,
+                                                    int nblocks, int m,
+                                                    double* __restrict__ out, Clock* clk,
+                                                    int commit, int64_t cap) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j < m) {
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * m + j];
-    out[j] = s;
+    for (int b = lane; b < nblocks; b += 64) s += partials[(size_t)b * m + j];
+    s = wave_sum(s);
+    if (lane == 0) out[j] = s;
   }
-  if (commit && j == 0) {
-    clk->n = clk->n_next;
-    clk->ticket = 0;
-    clk->epoch = clk->epoch + 1;
+  if (commit && blockIdx.x == 0 && threadIdx.x < 64) {
+    // per-block counter rows written by the push (after its partials)
+    const uint32_t* bcnt = (const uint32_t*)(partials + (size_t)nblocks * m);
+    const int slot[8] = {BZ_CNT_BIRTHS, BZ_CNT_ABSORBED, BZ_CNT_ELASTIC, BZ_CNT_EXCITATION,
+                         BZ_CNT_IONIZATION, BZ_CNT_NULL, BZ_CNT_OVERFLOW, BZ_CNT_NUMAX};
+    for (int q = 0; q < 8; ++q) {
+      unsigned long long s = 0;
+      for (int b = lane; b < nblocks; b += 64) s += bcnt[(size_t)b * 8 + q];
+      s = wave_sum(s);
+      if (lane == 0 && s) clk->cnt[slot[q]] += s;
+    }
+    if (lane == 0) {
+      // outputs past the capacity were dropped (and counted) by the push
+      const int64_t nx = clk->n_next;
+      clk->n = nx < cap ? nx : cap;
+      clk->epoch = clk->epoch + 1;
+      clk->ticket = 0;
+    }
   }
 }
 
@@ -285,7 +349,9 @@ namespace bz {
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
   auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS>;
-  const size_t smem = (size_t)3 * (DIM == 1 ? a.p.n_nodes : 1) * sizeof(double);
+  size_t smem = (size_t)3 * (DIM == 1 ? a.p.n_nodes : 1) * sizeof(double);
+  if (XS == BZ_XS_TABLE) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
+  if (smem > 160 * 1024) return fail("push: deposit + table need %zu B of LDS (> 160 KiB)", smem);
   if (smem > 64 * 1024) {
     if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)smem) != hipSuccess)
@@ -295,13 +361,23 @@ static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, smem) != hipSuccess ||
       occ < 1)
     return fail("push kernel does not fit on a CU (LDS %zu B)", smem);
+  // every block resident: occupancy-sized grid, launched cooperatively
   int grid = env.num_cu * (occ > 8 ? 8 : occ);
-  const int64_t cap_parts = env.partials_doubles / (3 * (DIM == 1 ? a.p.n_nodes : 1));
+  // partial histograms [grid][3 nn] doubles followed by [grid][8] u32 counters
+  const int64_t cap_parts = env.partials_doubles / (3 * (DIM == 1 ? a.p.n_nodes : 1) + 4);
   if (grid > cap_parts) grid = (int)cap_parts;
   if (grid < 1) return fail("deposit partial buffer too small");
+  a.sched = env.push_sched;
+  void* args[] = {&a};
   timing_begin(env, 0);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
+  hipError_t e = hipSuccess;
+  if (a.sched == 0)
+    e = hipLaunchCooperativeKernel((const void*)kern, dim3(grid), dim3(BLOCK), args,
+                                   (unsigned)smem, env.stream);
+  else
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
   timing_end(env, 0);
+  if (e != hipSuccess) return fail("k_push cooperative launch: %s", hipGetErrorString(e));
   *grid_out = grid;
   return check_launch("k_push");
 }
@@ -360,8 +436,9 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
                                          : launch_push_prec<float>(env, a, shape, &grid);
   if (rc) return rc;
   const int m = 3 * (p->dim == 1 ? p->n_nodes : 1);
-  hipLaunchKernelGGL(k_dep_reduce, dim3((m + 255) / 256), dim3(256), 0, env.stream,
-                     (const double*)env.partials, grid, m, moments, env.clock, 1);
+  hipLaunchKernelGGL(k_dep_reduce, dim3((m + 3) / 4), dim3(256), 0, env.stream,
+                     (const double*)env.partials, grid, m, moments, env.clock, 1,
+                     (int64_t)env.capacity);
   return check_launch("k_dep_reduce");
 }
 
@@ -385,8 +462,9 @@ static int launch_deposit_t(const LaunchEnv& env, const bz_state_t* s, const bz_
                      (const Clock*)env.clock, env.partials);
   int rc = check_launch("k_deposit");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_dep_reduce, dim3((3 * nn + 255) / 256), dim3(256), 0, env.stream,
-                     (const double*)env.partials, grid, 3 * nn, moments, env.clock, 0);
+  hipLaunchKernelGGL(k_dep_reduce, dim3((3 * nn + 3) / 4), dim3(256), 0, env.stream,
+                     (const double*)env.partials, grid, 3 * nn, moments, env.clock, 0,
+                     (int64_t)env.capacity);
   return check_launch("k_dep_reduce");
 }
 

@@ -614,6 +614,11 @@ __device__ __forceinline__ double res_uniform(uint64_t seed, uint64_t step, uint
 
 // Split: weighted draws with replacement among w > min_wt; each drawn
 // particle's weight is divided by (copies+1) and its copies follow it.
+// Segments of up to kSplitLds particles keep the CDF, copy counts and output
+// offsets in LDS; larger ones use the workspace.  Clones are written by all
+// threads: output slot o belongs to the particle whose offset range holds o.
+constexpr int kSplitLds = 3968;  // 3968 * 16 B stays under the 64 KiB default LDS limit
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* counts,
                                                    const int32_t* offsets, const uint32_t* perm,
@@ -621,6 +626,7 @@ __global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* cou
                                                    SortWs ws, const Clock* clk, int64_t cap) {
   const int k = blockIdx.x;
   if (plan[4 * k] != kSplit) return;
+  extern __shared__ int64_t s_split[];  // cdf[kSplitLds], copies/offs as int32 after
   __shared__ double s_red[4];
   __shared__ int64_t s_w64[4];
   __shared__ int s_wi[4];
@@ -628,57 +634,69 @@ __global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* cou
   const int64_t s0 = offsets[k];
   const uint64_t step = clk->step;
   const double minw = a.pol.min_wt;
+  const bool lds = c <= kSplitLds;
+  int64_t* cdf = lds ? s_split : ws.cdf + s0;
+  int32_t* copies = lds ? (int32_t*)(s_split + kSplitLds) : ws.copies + s0;
+  int32_t* offs = lds ? (int32_t*)(s_split + kSplitLds) + kSplitLds : (int32_t*)ws.sel + s0;
+  const T* Wt = (const T*)a.in[4];
   // fixed-point scale from the largest heavy weight
   double wmax = 0.0;
   for (int j = threadIdx.x; j < c; j += 256) {
-    const double w = wt<T>(a, perm[s0 + j]);
+    const double w = (double)Wt[perm[s0 + j]];
     if (w > minw) wmax = fmax(wmax, w);
-    ws.copies[s0 + j] = 0;
+    copies[j] = 0;
   }
   wmax = block_max_d(wmax, s_red);
   int ew;
   frexp(wmax, &ew);
   const int S = 61 - ew - ceil_log2_i64(c);
   auto q = [&](int j) -> int64_t {
-    const double w = wt<T>(a, perm[s0 + j]);
+    const double w = (double)Wt[perm[s0 + j]];
     return w > minw ? (int64_t)ldexp(w, S) : 0;
   };
-  const int64_t Wq = block_fixed_scan(s0, c, ws.cdf, q, s_w64);
+  const int64_t Wq = block_fixed_scan(0, c, cdf, q, s_w64);
   const int draws = plan[4 * k + 1] - c;
   for (int m = threadIdx.x; m < draws; m += 256) {
     const double u = res_uniform(a.seed, step, (uint32_t)k, (uint32_t)m);
     const int64_t thr = (int64_t)(u * (double)Wq);
-    const int j = search_cdf(ws.cdf, s0, c, thr, true);
-    atomicAdd(&ws.copies[s0 + j], 1);
+    atomicAdd(&copies[search_cdf(cdf, 0, c, thr, true)], 1);
   }
   __syncthreads();
-  // output: each particle followed by its copies
+  // exclusive offsets of (1 + copies) in segment order
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int carry = 0;
-  const int64_t out0 = plan[4 * k + 2];
   for (int base = 0; base < c; base += 256) {
     const int j = base + threadIdx.x;
-    const int reps = j < c ? 1 + ws.copies[s0 + j] : 0;
+    const int reps = j < c ? 1 + copies[j] : 0;
     const int inc = wave_incl_scan(reps);
     __syncthreads();
     if (lane == 63) s_wi[wid] = inc;
     __syncthreads();
     int before = carry;
     for (int w = 0; w < wid; ++w) before += s_wi[w];
-    if (j < c) {
-      const uint32_t src = perm[s0 + j];
-      int64_t dst = out0 + before + inc - reps;
-      const T* in_w = (const T*)a.in[4];
-      T w = in_w[src];
-      if (reps > 1) w = w / (T)reps;
-      for (int r = 0; r < reps; ++r, ++dst) {
-        if (dst >= cap) break;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) ((T*)a.out[f])[dst] = ((const T*)a.in[f])[src];
-        ((T*)a.out[4])[dst] = w;
-      }
-    }
+    if (j < c) offs[j] = before + inc - reps;
     carry += s_wi[0] + s_wi[1] + s_wi[2] + s_wi[3];
+  }
+  __syncthreads();
+  const int total = carry;  // = c + draws
+  const int64_t out0 = plan[4 * k + 2];
+  for (int o = threadIdx.x; o < total; o += 256) {
+    const int64_t dst = out0 + o;
+    if (dst >= cap) break;
+    // largest j with offs[j] <= o
+    int lo = 0, hi = c - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (offs[mid] <= o) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t src = perm[s0 + lo];
+    const int reps = 1 + copies[lo];
+    T w = Wt[src];
+    if (reps > 1) w = w / (T)reps;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) ((T*)a.out[f])[dst] = ((const T*)a.in[f])[src];
+    ((T*)a.out[4])[dst] = w;
   }
 }
 
@@ -834,7 +852,8 @@ static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const b
   hipLaunchKernelGGL(k_res_copy<T>, dim3(env.num_cu * 4), dim3(256), 0, env.stream, a, skeys,
                      offsets, perm, (const int32_t*)ws.plan, counts, (const Clock*)env.clock,
                      cap);
-  hipLaunchKernelGGL(k_res_split<T>, dim3(n_keys), dim3(256), 0, env.stream, a, counts, offsets,
+  hipLaunchKernelGGL(k_res_split<T>, dim3(n_keys), dim3(256), (size_t)kSplitLds * 16,
+                     env.stream, a, counts, offsets,
                      perm, targets, (const int32_t*)ws.plan, ws, (const Clock*)env.clock, cap);
   hipLaunchKernelGGL(k_res_merge<T>, dim3(n_keys), dim3(256), 0, env.stream, a, counts, offsets,
                      perm, targets, (const int32_t*)ws.plan, ws, (const Clock*)env.clock, cap);

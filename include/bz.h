@@ -160,6 +160,10 @@ int bz_set_stream(bz_ctx_t ctx, void* hip_stream);
  * counterpart in the reference (its launch shape is fixed at 1024 x 64,
  * 1bin/boltzmann.py:139-140). */
 int bz_set_push_shape(bz_ctx_t ctx, int shape);
+/* Tile schedule of the fused push: 0 static tile order under a cooperative
+ * launch (default), 1 tiles claimed from a ticket counter by running blocks.
+ * No counterpart in the reference. */
+int bz_set_push_schedule(bz_ctx_t ctx, int sched);
 int bz_synchronize(bz_ctx_t ctx);
 
 /* Device clock: live count, step, time; counters (BZ_NCOUNTERS uint64). */

@@ -117,16 +117,28 @@ def test_resample_parity(gpu, name, prec, nb, drift):
 def test_step_with_resampling_runs(gpu):
     """A few full steps of a small c1-like discharge with resampling each step."""
     from boltzmann_amd import configs
-    cfg = configs.c1().with_(n_particles=60_000, n_cells=32, target_ppc=1000, trigger_at=0.0)
+    cfg = configs.c1().with_(n_particles=60_000, n_cells=32, target_ppc=1000, trigger_at=0.0,
+                             resample_every=1)
     from boltzmann_amd import physics as ph
     from boltzmann_amd.engine import KineticEngine
     eng = KineticEngine(cfg)
-    eng.load_state(ph.initial_state(cfg))
+    st = ph.initial_state(cfg)
+    eng.load_state(st)
     eng.initialize_fields()
-    for i in range(5):
+    w0 = float(st[4].astype(np.float64).sum())
+    assert eng.resample()
+    w1 = float(eng.fetch_state()[4].astype(np.float64).sum())
+    assert abs(w1 - w0) <= 1e-5 * w0  # FP32 weights: storage rounding only
+    s = O.Setup(dim=1, dt=cfg.dt, n_cells=cfg.n_cells, length_cm=cfg.length_cm)
+    pol = O.policy(split_below=cfg.split_below, merge_at=cfg.merge_at, merge_keep=cfg.merge_keep,
+                   trigger_at=cfg.trigger_at, min_wt=cfg.min_wt, split_to=cfg.split_to)
+    want, _ = O.resample(s, pol, cfg.nb, cfg.bin_thresholds, eng.targets_host, cfg.seed, 0, st,
+                         capacity=eng.capacity)
+    assert eng.count() == want[0].size
+    for i in range(1, 5):
         eng.step(i)
     n = eng.count()
-    assert 0.5 * cfg.n_particles < n < 2 * cfg.n_particles
+    assert 0 < n < 2 * cfg.n_particles
     c = eng.counters()
     assert c["resample"] == 5 and c["overflow"] == 0
     eng.close()
