@@ -52,7 +52,8 @@ class FieldParams(C.Structure):
     _fields_ = [("dx", C.c_double), ("dt", C.c_double), ("eps", C.c_double),
                 ("v0", C.c_double), ("freq", C.c_double), ("mu_i", C.c_double),
                 ("D_i", C.c_double), ("D_s", C.c_double), ("n", C.c_int32),
-                ("fluid_on", C.c_int32), ("poisson_on", C.c_int32), ("pad", C.c_int32)]
+                ("fluid_on", C.c_int32), ("poisson_on", C.c_int32),
+                ("skip_moments", C.c_int32)]
 
 
 class ResamplePolicy(C.Structure):
@@ -84,6 +85,8 @@ SIGNATURES = {
     "bz_push_mcc": (C.c_int, [_P, C.POINTER(State), C.POINTER(State), _P, C.POINTER(Params),
                               C.POINTER(XSec), _P, _P, _P]),
     "bz_deposit": (C.c_int, [_P, C.POINTER(State), C.POINTER(Params), _P]),
+    "bz_reduction": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, C.c_int64]),
+    "bz_fill_uniforms": (C.c_int, [_P, _P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64]),
     "bz_field_step": (C.c_int, [_P, C.POINTER(FieldParams), _P, _P, _P, _P, _P, _P, _P, _P,
                                 _P, C.c_int32]),
     "bz_bin_sort": (C.c_int, [_P, C.POINTER(State), C.POINTER(Params), C.POINTER(C.c_double),

@@ -1,0 +1,178 @@
+"""BoltzmannDSMC: the reference's driver surface on the MI355X engine.
+
+Constructor and run() keep the reference's arguments (1bin/boltzmann.py:23,
+:565) plus the binning strategy (1, 3 or 5 energy bins), which the reference
+selects by directory.  Physics settings are the reference's (boltzmann.py:46-78):
+0.1 Torr argon, L = 2.54 cm, 100 V at 13.56 MHz, argon fit cross sections,
+300 K gas (with the reference's electron-mass variance), the Liu initial
+condition (boltzmann.py:242-333) and the ES-WPNC policy of _do_resampling /
+_check_for_resampling.  The step order is run()'s: resampling check every
+resample_freq steps, push + collisions + deposit + compaction, host moment
+update, heavies, Poisson, cycle averages and output every in_spc steps.
+Grid quantities stay on the device; host copies are made at output.
+"""
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import physics as ph
+from .engine import KineticEngine
+
+P_TORR = 0.1
+L_CM = 2.54
+V0 = 100.0
+FREQ = 13.56e6
+KTM = 1.5 * 1.17255 * 10 ** 11  # variance per component at 1 eV (boltzmann.py:263)
+
+
+def reference_config(Nc: int, gridpts: int, bnf: int, in_dt: float, nb: int = 1,
+                     seed: int = 12345) -> ph.RunConfig:
+    N = gridpts - 1
+    fr = ph.bin_fractions(nb)
+    # per-bin targets int(frac * Nc / N) (1bin/boltzmann.py:147, 3bin :147-149, 5bin :147-151)
+    tt = [int(f * Nc / N) for f in fr]
+    return ph.RunConfig(
+        name=f"reference_{nb}bin", dim=1, n_particles=sum(tt) * (N + 1),
+        precision=_lib.BZ_F64, xs_mode=_lib.BZ_XS_FIT, nb=nb, n_cells=N, length_cm=L_CM,
+        bnf=bnf, dt=(1.0 / FREQ) / in_dt, gas_density_m3=1e6 * 3.22e16 * P_TORR,
+        rf_voltage=V0, rf_freq=FREQ, target_ppc=sum(tt), resample_every=1, trigger_at=3.0,
+        merge_at=3.0, split_below=0.5, merge_keep=0.6, split_to=2.0, min_wt=1e-1,
+        bin_fractions=fr, bin_thresholds=ph.bin_thresholds(nb), gas_temp_k=300.0, seed=seed,
+        capacity_factor=4.0)
+
+
+def liu_initial_state(cfg: ph.RunConfig, seed: int):
+    """Per node: target particles at the node (+-dx/4 at the ends), Maxwellian
+    at 0.5 eV, weight n(x)/ppc with n(x) = P (1e7 + 1e9 (1-x/L)^2 (x/L)^2)."""
+    N = cfg.n_cells
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(0, cfg.length_cm, N + 1)
+    ne = P_TORR * (1e7 + 1e9 * (1 - xs / cfg.length_cm) ** 2 * (xs / cfg.length_cm) ** 2)
+    dx = cfg.dx
+    # particles per node = sum of the per-bin targets (boltzmann.py:278)
+    ppc = max(cfg.target_ppc, 1)
+    x, v, w = [], [], []
+    for ii in range(N + 1):
+        pos = xs[ii] + (dx / 4 if ii == 0 else (-dx / 4 if ii == N else 0.0))
+        x.append(np.full(ppc, pos))
+        v.append(rng.normal(0.0, math.sqrt(0.5 * KTM), size=(3, ppc)))
+        w.append(np.full(ppc, ne[ii] / ppc))
+    x = np.concatenate(x)
+    v = np.concatenate(v, axis=1)
+    w = np.concatenate(w)
+    return (x, v[0], v[1], v[2], w), ne
+
+
+class BoltzmannDSMC:
+    def __init__(self, num_steps, name, restart_cyclenum, Nc, gridpts, bnf, num_nodes, pyk,
+                 resample_freq, in_dt, in_spc, nbins: int = 1, ref_rescale: bool = False,
+                 seed: int = 12345, device: int = 0):
+        if restart_cyclenum and restart_cyclenum > 0:
+            # the reference exits here as well (IC flag and restart together)
+            raise ValueError("restart is not combined with the prescribed initial condition")
+        self.pyk = bool(pyk)
+        self.cfg = reference_config(int(Nc), int(gridpts), int(bnf), float(in_dt), int(nbins),
+                                    seed)
+        self.cfg = self.cfg.with_(resample_every=int(resample_freq))
+        self.N = self.cfg.n_cells
+        self.dx = self.cfg.dx
+        self.L = self.cfg.length_cm
+        self.dt_big = self.cfg.dt
+        self.steps_per_cycle = int(in_spc)
+        self.num_nodes = int(num_nodes)
+        self.curr_t = 0.0
+        state, ne0 = liu_initial_state(self.cfg, seed)
+        self.eng = KineticEngine(self.cfg, device=device)
+        tt = np.array([int(f * int(Nc) / self.N) for f in self.cfg.bin_fractions],
+                      dtype=np.int32)
+        self.eng.targets_host = np.tile(tt, self.N + 1)
+        self.eng.targets.copy_(torch.from_numpy(self.eng.targets_host))
+        pol = self.eng.policy
+        pol.ref_trigger = 1
+        pol.max_ppc = int(10.0 * max(self.eng.targets_host.max(), 1))
+        pol.ref_rescale = 1 if ref_rescale else 0
+        self.eng.ni.copy_(torch.from_numpy(ne0))
+        self.eng.load_state(state)
+        self.eng.initialize_fields()
+        # the reference adds the (zero) initial ionization source and sets Te
+        self.Nc_list = [self.eng.count()]
+        nn = self.N + 1
+        f64 = dict(dtype=torch.float64, device=self.eng.dev)
+        self._ca = {k: torch.zeros(nn if k not in ("E", "V") else (self.N if k == "E" else
+                                                                     self.N - 1), **f64)
+                    for k in ("ne", "ni", "ns", "Te", "E", "V")}
+
+    # reference-named views (host copies)
+    @property
+    def ne_ar(self):
+        return self.eng.moments[:self.N + 1].cpu().numpy()
+
+    @property
+    def ni_ar(self):
+        return self.eng.ni.cpu().numpy()
+
+    @property
+    def Te_ar(self):
+        return self.eng.Te.cpu().numpy()
+
+    @property
+    def E_ar(self):
+        return self.eng.E.cpu().numpy()
+
+    def run(self, num_steps, name, restart_cyclenum=0, verbose: bool = True):
+        e = self.eng
+        nn = self.N + 1
+        cyclestep = 0
+        if verbose:
+            print("INIT NC = ", self.Nc_list[0], flush=True)
+        for bigstep in range(int(num_steps)):
+            if verbose and bigstep % 100 == 0:
+                print("Step ", bigstep, ". Nc = ", e.count(), flush=True)
+            e.step(bigstep)
+            self.curr_t += self.dt_big
+            cyclestep += 1
+            if (bigstep + 1) % self.steps_per_cycle != 0:
+                self._ca["ne"] += e.moments[:nn]
+                self._ca["ni"] += e.ni
+                self._ca["ns"] += e.ns
+                self._ca["Te"] += e.Te
+                self._ca["E"] += e.E
+                self._ca["V"] += e.V
+            else:
+                cyclestep = 0
+                for v in self._ca.values():
+                    v /= self.steps_per_cycle
+                self.data_write(bigstep, name, restart_cyclenum)
+                for v in self._ca.values():
+                    v.zero_()
+                e.g2.zero_()
+        self.Nc_list = [e.count()]
+        return self
+
+    def data_write(self, bigstep, name, restart_cyclenum=0, rank: int = 0):
+        """Raw float64 files in the reference's naming (kernels/data_write_1D.py:3-75)."""
+        e = self.eng
+        k = str(int((bigstep + 1) / self.steps_per_cycle) + restart_cyclenum)
+        cyc = "_cycle" + k + ".dat"
+        cyc_rk = "_cycle" + k + "_mpi" + str(rank) + ".dat"
+        nn = self.N + 1
+        grids = {"ne": e.moments[:nn], "ni": e.ni, "Te": e.Te, "E": e.E, "ionprod_total": e.g2}
+        if rank == 0:
+            for key, arr in grids.items():
+                arr.cpu().numpy().astype(np.float64).tofile(name + key + cyc)
+            for key in ("ne", "ni", "Te", "E"):
+                self._ca[key].cpu().numpy().tofile(name + key + "_CA" + cyc)
+        st = e.fetch_state()
+        for key, arr in (("e_x0", st[0]), ("e_v0", st[1]), ("e_v1", st[2]), ("e_v2", st[3]),
+                         ("e_wt", st[4])):
+            np.asarray(arr, dtype=np.float64).tofile(name + key + cyc_rk)
+        print("Writing data for cycle ", (bigstep + 1) / self.steps_per_cycle, flush=True)
+        sys.stdout.flush()
+
+
+__all__ = ["BoltzmannDSMC", "reference_config", "liu_initial_state"]

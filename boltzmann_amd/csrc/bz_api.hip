@@ -363,3 +363,25 @@ int bz_resample(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const b
 }
 
 }  // extern "C"
+
+extern "C" int bz_reduction(bz_ctx_t c, int32_t r, int32_t M, int32_t RM, double* dw,
+                            const double* dv, const double* dx, int64_t ptcl) {
+  BZ_CTX(c);
+  if (r < 1 || M < 1 || RM < 1 || RM > M) return fail("reduction: need r, M >= 1, 1 <= RM <= M");
+  if (ptcl < 0) return fail("reduction: negative particle count");
+  if (!dw || (ptcl > 0 && (!dv || !dx))) return fail("reduction: null pointer");
+  return launch_reduction(c->env, r, M, RM, dw, dv, dx, ptcl);
+}
+
+namespace bz {
+int launch_fill_uniforms(const LaunchEnv& env, void* R, int64_t stride, int64_t n, uint64_t seed,
+                         uint64_t step);
+}
+
+extern "C" int bz_fill_uniforms(bz_ctx_t c, void* R, int64_t stride, int64_t n, uint64_t seed,
+                                uint64_t step) {
+  BZ_CTX(c);
+  if (n < 0 || stride < n) return fail("fill_uniforms: need 0 <= n <= stride");
+  if (n > 0 && !R) return fail("fill_uniforms: null array");
+  return launch_fill_uniforms(c->env, R, stride, n, seed, step);
+}

@@ -31,12 +31,14 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
   const double* low = chol + (N - 1);
 
   // host moment update (boltzmann.py:643-647)
-  for (int i = tid; i < nn; i += kFieldBlock) {
-    ni[i] = ni[i] + nsrc[i];
-    double te = ((2.0 / 3.0) * nrg[i]) / ne[i];
-    if (te != te) te = 0.0;  // nan_to_num
-    Te[i] = te;
-    g2[i] = g2[i] + nsrc[i];
+  if (!fp.skip_moments) {
+    for (int i = tid; i < nn; i += kFieldBlock) {
+      ni[i] = ni[i] + nsrc[i];
+      double te = ((2.0 / 3.0) * nrg[i]) / ne[i];
+      if (te != te) te = 0.0;  // nan_to_num
+      Te[i] = te;
+      g2[i] = g2[i] + nsrc[i];
+    }
   }
   __syncthreads();
   double t = clk->t;

@@ -478,3 +478,106 @@ int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t*
 }
 
 }  // namespace bz
+
+// ---------------------------------------------------------------------------
+// Reference reduction interface: r value columns histogrammed into coarse
+// rows (kernels/reduction/cuda_reduction/kernels-new.cu:56-132).  Fine bin
+// b = int(x*M), coarse row b/(M/RM); per-block LDS rows, summed over blocks
+// in a fixed order into rows 0..RM-1 of dw; the other rows are zeroed.
+// ---------------------------------------------------------------------------
+namespace bz {
+
+__global__ __launch_bounds__(256) void k_reduction(int r, int M, int RM, int bnf,
+                                                   const double* __restrict__ dv,
+                                                   const double* __restrict__ dx, int64_t ptcl,
+                                                   double* partials) {
+  extern __shared__ double s_h[];  // [RM][r]
+  const int m = RM * r;
+  for (int j = threadIdx.x; j < m; j += 256) s_h[j] = 0.0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < ptcl;
+       i += (int64_t)gridDim.x * 256) {
+    const int b = (int)(dx[i] * (double)M);
+    int k = b / bnf;
+    k = k < 0 ? 0 : (k >= RM ? RM - 1 : k);
+    for (int j = 0; j < r; ++j) {
+      const double v = dv[(int64_t)r * i + j];
+      if (v != 0.0) atomicAdd(&s_h[r * k + j], v);
+    }
+  }
+  __syncthreads();
+  double* part = partials + (size_t)blockIdx.x * m;
+  for (int j = threadIdx.x; j < m; j += 256) part[j] = s_h[j];
+}
+
+__global__ void k_zero_rows(double* dw, int64_t from, int64_t to) {
+  for (int64_t j = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < to;
+       j += (int64_t)gridDim.x * blockDim.x)
+    dw[j] = 0.0;
+}
+
+int launch_reduction(const LaunchEnv& env, int r, int M, int RM, double* dw, const double* dv,
+                     const double* dx, int64_t ptcl) {
+  const int bnf = (M + RM - 1) / RM;  // ceil(M/RM), the redcheck block size
+  const int m = RM * r;
+  const size_t smem = (size_t)m * sizeof(double);
+  if (smem > 160 * 1024) return fail("reduction: RM*r = %d doubles exceed LDS", m);
+  if (smem > 64 * 1024) {
+    if (hipFuncSetAttribute((const void*)k_reduction, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)smem) != hipSuccess)
+      return fail("hipFuncSetAttribute(reduction LDS) failed");
+  }
+  int grid = env.num_cu * 2;
+  const int64_t cap_parts = env.partials_doubles / m;
+  if (grid > cap_parts) grid = (int)cap_parts;
+  if (grid < 1) return fail("reduction: partial buffer too small");
+  hipLaunchKernelGGL(k_reduction, dim3(grid), dim3(256), smem, env.stream, r, M, RM, bnf, dv, dx,
+                     ptcl, env.partials);
+  int rc = check_launch("k_reduction");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dep_reduce, dim3((m + 3) / 4), dim3(256), 0, env.stream,
+                     (const double*)env.partials, grid, m, dw, env.clock, 0, (int64_t)0);
+  rc = check_launch("k_dep_reduce");
+  if (rc) return rc;
+  if ((int64_t)M * r > m)
+    hipLaunchKernelGGL(k_zero_rows, dim3(64), dim3(256), 0, env.stream, dw, (int64_t)m,
+                       (int64_t)M * r);
+  return check_launch("k_zero_rows");
+}
+
+}  // namespace bz
+
+// Collision uniforms of the Philox stream written out as R[c*stride + i]
+// (columns 0,1,2,3,5; 4 and 6 zero): the numbers the in-kernel stream draws.
+namespace bz {
+
+template <typename T>
+__global__ void k_fill_uniforms(T* R, int64_t stride, int64_t n, uint64_t seed, uint64_t step) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    Uniforms<T> U;
+    U.first(seed, (uint32_t)i, step);
+    U.second(seed, (uint32_t)i, step);
+    U.third(seed, (uint32_t)i, step);
+    R[i] = U.u[0];
+    R[stride + i] = U.u[1];
+    R[2 * stride + i] = U.u[2];
+    R[3 * stride + i] = U.u[3];
+    R[4 * stride + i] = (T)0;
+    R[5 * stride + i] = U.u[4];
+    R[6 * stride + i] = (T)0;
+  }
+}
+
+int launch_fill_uniforms(const LaunchEnv& env, void* R, int64_t stride, int64_t n, uint64_t seed,
+                         uint64_t step) {
+  if (env.precision == BZ_F64)
+    hipLaunchKernelGGL(k_fill_uniforms<double>, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
+                       (double*)R, stride, n, seed, step);
+  else
+    hipLaunchKernelGGL(k_fill_uniforms<float>, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
+                       (float*)R, stride, n, seed, step);
+  return check_launch("k_fill_uniforms");
+}
+
+}  // namespace bz

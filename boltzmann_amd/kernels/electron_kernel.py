@@ -1,0 +1,119 @@
+"""electron_kernel with the reference's signature (kernels/electron_kernel_numba.py:291-303).
+
+The reference refreshes cp_R_vec with uniforms, runs collisions + push on the
+AoS rows in place, appends each ionization's secondary at Nc + slot and
+leaves the number of births in d_curr_count[0].  Here the same stage runs in
+the fused HIP kernel (reference argon fits, direct collision probability):
+cp_R_vec / d_R_vec receive the uniforms used (Philox stream), the AoS rows
+[0, n_out) receive the compacted survivors with each secondary right after
+its parent, rows [n_out, Nc + births) are zeroed, and d_curr_count[0] is the
+number of births, so the reference's post-processing compaction is a no-op.
+The node moments of the pushed state are kept for post_processing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import torch
+
+from .. import _lib
+from .. import physics as ph
+from . import _session as S
+
+L_REF = 2.54  # electron_kernel_numba.py:13
+_deposit_bins = {"bnf": 4}
+
+
+def set_deposit_bins(bnf: int) -> None:
+    """Fine bins per node used for the moments post_processing returns (-bnf flag)."""
+    _deposit_bins["bnf"] = int(bnf)
+
+
+def _params(N: int, nn: float, dt: float, stride: int, seed: int) -> _lib.Params:
+    p = _lib.Params()
+    p.dt = dt
+    p.xfac = 100 * dt
+    p.afac = dt * (ph.Q_E / ph.M_E)
+    p.kev = 0.5 * ph.M_E * ph.J_EV
+    p.jm = ph.J_EV * ph.M_E
+    p.fel = math.sqrt(1 - (2 * ph.M_E / ph.M_N_AR))
+    p.L = L_REF
+    p.dx = L_REF / N
+    p.half_dx = 0.5 * p.dx
+    p.L_dx = L_REF + p.dx
+    p.dep_M = float(_deposit_bins["bnf"] * (N + 1))
+    p.nn = nn
+    p.e_ion = ph.E_ION
+    p.e_exc = ph.E_EXC
+    p.seed = seed
+    p.dim = 1
+    p.xs_mode = _lib.BZ_XS_FIT
+    p.rng_mode = _lib.BZ_RNG_STREAM
+    p.n_cells = N
+    p.n_nodes = N + 1
+    p.bnf = _deposit_bins["bnf"]
+    p.r_stride = stride
+    return p
+
+
+def electron_kernel(Nc, d_data_ar, d_tosum_ar, d_collct_ar, d_E_ar, cp_R_vec, d_R_vec,
+                    d_currxbins, d_forE_xbins, nn, dt_ratio, dt, d_curr_count, d_g0_ar, d_g2_ar,
+                    d_gas_v_ar, num_blocks=None, threads_per_block=None, seed: int = 12345):
+    if not isinstance(d_data_ar, torch.Tensor) or not d_data_ar.is_cuda:
+        raise TypeError("d_data_ar must be a CUDA (HIP) torch tensor of shape (Nmax, 6)")
+    Nc = int(Nc)
+    cap = int(d_data_ar.shape[0])
+    N = int(d_E_ar.shape[0])
+    dev = d_data_ar.device.index or 0
+    s = S.session(dev, _lib.BZ_F64, cap, N + 1)
+    a, b = s.soa
+    births_total = 0
+    n = Nc
+    for _ in range(max(int(dt_ratio), 1)):
+        a[0][:n].copy_(d_data_ar[:n, 0])
+        for k, col in ((1, 3), (2, 4), (3, 5)):
+            a[k][:n].copy_(d_data_ar[:n, col])
+        a[4][:n].copy_(d_tosum_ar[:n, 0])
+        _lib.check(s.lib.bz_fill_uniforms(s.ctx, S.ptr(s.R), s.capacity, n, seed, s.step))
+        if cp_R_vec is not None and n:
+            cp_R_vec[:n, :].copy_(s.R[:, :n].t())
+        if d_R_vec is not None and d_R_vec is not cp_R_vec and n:
+            d_R_vec[:n, :].copy_(s.R[:, :n].t())
+        G = None
+        if d_gas_v_ar is not None:
+            s.G[:, :n].copy_(d_gas_v_ar[:n, :3].t())
+            G = s.G
+        E = S.as_device(d_E_ar, s.dev)
+        p = _params(N, float(nn), float(dt) / max(int(dt_ratio), 1), s.capacity, seed)
+        s.clock_set(n)
+        before = C.c_uint64 * _lib.BZ_NCOUNTERS
+        c0 = before()
+        _lib.check(s.lib.bz_clock_get(s.ctx, None, None, None, c0))
+        _lib.check(s.lib.bz_push_mcc(s.ctx, C.byref(s.structs[0]), C.byref(s.structs[1]),
+                                     S.ptr(E), C.byref(p), None, S.ptr(s.R), S.ptr(G),
+                                     S.ptr(s.moments)), "bz_push_mcc")
+        c1 = before()
+        nout = C.c_int64()
+        _lib.check(s.lib.bz_clock_get(s.ctx, C.byref(nout), None, None, c1))
+        births = int(c1[_lib.COUNTER_NAMES.index("births")] - c0[_lib.COUNTER_NAMES.index("births")])
+        m = nout.value
+        # write back the compacted state in the reference layout
+        d_data_ar[:m, 0].copy_(b[0][:m])
+        d_data_ar[:m, 1:3].zero_()
+        for k, col in ((1, 3), (2, 4), (3, 5)):
+            d_data_ar[:m, col].copy_(b[k][:m])
+        d_tosum_ar[:m, 0].copy_(b[4][:m])
+        d_tosum_ar[:m, 1:3].zero_()
+        v2 = b[1][:m] * b[1][:m] + b[2][:m] * b[2][:m] + b[3][:m] * b[3][:m]
+        d_tosum_ar[:m, 3].copy_(p.kev * v2)
+        end = min(n + births, cap)
+        if end > m:
+            d_data_ar[m:end].zero_()
+            d_tosum_ar[m:end].zero_()
+        births_total += births
+        s.step += 1
+        n = m
+    d_curr_count[0] = births_total
+    s.cached[(d_data_ar.data_ptr(), Nc + births_total)] = (s.moments.clone(), n)
+    return None

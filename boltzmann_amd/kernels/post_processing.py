@@ -1,0 +1,87 @@
+"""post_processing with the reference's signature (kernels/post_processing_1D.py:40-68).
+
+Returns (ne_ar, ni_src, ns_src, nrg_ar, counter_g0..g3, Nnew).  After
+electron_kernel the node moments were already summed by the fused HIP
+kernel and the rows compacted, so they are returned as computed there; for a
+state that was not pushed (the set-up deposit, boltzmann.py:363-371) the
+deposit runs in bz_deposit and the zero-weight rows are compacted out.
+The collision-counter reduction (post_processing_1D.py:56) accumulates an
+array the reference never resets or writes out; its counters come back as
+zeros.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from .. import _lib
+from . import _session as S
+from .electron_kernel import _deposit_bins, _params
+
+
+def post_processing(data_out, data_out_np, data_out2, data_out_np2, data_out3, data_out_np3,
+                    big_data_ar, big_tosum_ar, big_collct_ar, need_to_reshuffle, Na, Nc, nni,
+                    ne_ar, ni_src, ns_src, nrg_ar, counter_g0_ar, counter_g1_ar, counter_g2_ar,
+                    counter_g3_ar, big_N, RN, L, dx, temp_x):
+    Nc, nni, RN = int(Nc), int(nni), int(RN)
+    cap = int(big_data_ar.shape[0])
+    dev = big_data_ar.device.index or 0
+    N = RN - 1
+    s = S.session(dev, _lib.BZ_F64, cap, RN)
+    key = (big_data_ar.data_ptr(), Nc + nni)
+    if key in s.cached:
+        mom, nnew = s.cached.pop(key)
+    else:
+        bnf = int(big_N) // RN
+        if bnf != _deposit_bins["bnf"]:
+            _deposit_bins["bnf"] = bnf
+        m = Nc + nni
+        alive = torch.nonzero(big_tosum_ar[:m, 0] > 0).flatten() if need_to_reshuffle else \
+            torch.arange(m, device=big_data_ar.device)
+        nnew = int(alive.numel())
+        if need_to_reshuffle:
+            big_data_ar[:nnew] = big_data_ar[alive]
+            big_tosum_ar[:nnew] = big_tosum_ar[alive]
+            if m > nnew:
+                big_data_ar[nnew:m].zero_()
+                big_tosum_ar[nnew:m].zero_()
+        a = s.soa[0]
+        a[0][:nnew].copy_(big_data_ar[:nnew, 0])
+        for k, col in ((1, 3), (2, 4), (3, 5)):
+            a[k][:nnew].copy_(big_data_ar[:nnew, col])
+        a[4][:nnew].copy_(big_tosum_ar[:nnew, 0])
+        p = _params(N, 0.0, 1e-12, s.capacity, 0)
+        p.L = float(L)
+        p.dx = float(dx)
+        p.half_dx = 0.5 * p.dx
+        p.L_dx = p.L + p.dx
+        p.dep_M = float(big_N)
+        p.bnf = bnf
+        s.clock_set(nnew)
+        _lib.check(s.lib.bz_deposit(s.ctx, C.byref(s.structs[0]), C.byref(p), S.ptr(s.moments)),
+                   "bz_deposit")
+        mom = s.moments.clone()
+        v2 = big_data_ar[:nnew, 3] ** 2 + big_data_ar[:nnew, 4] ** 2 + big_data_ar[:nnew, 5] ** 2
+        big_tosum_ar[:nnew, 3] = (0.5 * 9.10938 * 10 ** (-31) * 6.241509 * (10 ** 18)) * v2
+    mom = mom[:3 * RN].view(3, RN)
+    if data_out is not None:
+        data_out.zero_()
+        data_out[:RN, 0] = mom[0]
+        data_out[:RN, 1] = mom[1]
+        data_out[:RN, 3] = mom[2]
+    host = mom.cpu().numpy()
+    if data_out_np is not None:
+        data_out_np[:RN, 0] = host[0]
+        data_out_np[:RN, 1] = host[1]
+        data_out_np[:RN, 2] = 0.0
+        data_out_np[:RN, 3] = host[2]
+    ne_ar[0:RN] = host[0]
+    ni_src = np.array(host[1])
+    ns_src = np.zeros(RN)
+    nrg_ar[0:RN] = host[2]
+    for c in (counter_g0_ar, counter_g1_ar, counter_g2_ar, counter_g3_ar):
+        c[0:RN] = 0.0
+    return (ne_ar, ni_src, ns_src, nrg_ar, counter_g0_ar, counter_g1_ar, counter_g2_ar,
+            counter_g3_ar, int(nnew))

@@ -129,7 +129,7 @@ typedef struct {
   int32_t n;        /* N cells (N+1 nodes, N-1 interior unknowns) */
   int32_t fluid_on; /* 0: heavies frozen (0-D test problems) */
   int32_t poisson_on;
-  int32_t pad;
+  int32_t skip_moments; /* 1: leave ni / Te / g2 untouched (stand-alone solves) */
 } bz_field_params_t;
 
 /* Resampling policy (ES-WPNC merge / split). */
@@ -195,6 +195,13 @@ int bz_push_mcc(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
                 const void* E, const bz_params_t* p, const bz_xsec_t* xs,
                 const void* R, const void* gas_v, double* moments);
 
+/* The collision uniforms the Philox stream draws for particles 0..n-1 at
+ * `step`, written as R[c*stride + i] (columns 0,1,2,3,5 used, 4 and 6 zero)
+ * in the context precision.  The reference refreshes cp_R_vec with fresh
+ * uniforms before each push (kernels/electron_kernel_numba.py:300). */
+int bz_fill_uniforms(bz_ctx_t ctx, void* R, int64_t stride, int64_t n, uint64_t seed,
+                     uint64_t step);
+
 /* Deposit only (no push): node moments of the current state, same layout
  * as bz_push_mcc's output.  Replaces the initial post_processing call
  * (1bin/boltzmann.py:363-371). */
@@ -209,6 +216,17 @@ int bz_deposit(bz_ctx_t ctx, const bz_state_t* s, const bz_params_t* p, double* 
 int bz_field_step(bz_ctx_t ctx, const bz_field_params_t* fp, const double* moments,
                   double* ni, double* ns, double* Te, double* g2, double* V,
                   double* E, void* E_t, const double* chol, int32_t advance_time);
+
+/* Histogram of r value columns into coarse rows, the reference's native
+ * reduction with the same arguments: for i < ptcl, fine bin b = int(x[i]*M),
+ * coarse row k = b / (M/RM); dw[r*k + j] += v[r*i + j].  dw is [M][r] FP64;
+ * rows 0..RM-1 receive the coarse sums (all other rows are zeroed), summed in
+ * a fixed order.  Replaces cuda_red.calls (kernels/reduction/cuda_reduction/
+ * kernels-new.cu:76-132, reda :56-73, redcheck :8-50) behind
+ * reduction.reduction(r, p, M, RM, dw, dv, dx, ptcl)
+ * (kernels/reduction/cuda_reduction/__init__.py:11-15). */
+int bz_reduction(bz_ctx_t ctx, int32_t r, int32_t M, int32_t RM, double* dw, const double* dv,
+                 const double* dx, int64_t ptcl);
 
 /* Energy / cell binning and stable radix sort by key = cell*nb + ebin,
  * cell = int((x + dx/2)/dx) (1bin/boltzmann.py:658), n_keys = n_nodes*nb <= 39999.

@@ -1,0 +1,176 @@
+"""The reference-surface stage functions (boltzmann_amd.kernels) and the
+BoltzmannDSMC driver, called the way the reference's run() calls them."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KEV = 0.5 * O.M_E * O.J_EV
+
+
+def test_reduction_brute_force(gpu):
+    import torch
+    from boltzmann_amd.kernels import reduction as red
+    rng = np.random.default_rng(0)
+    RM, bnf, r, ptcl = 301, 4, 4, 200_000
+    M = bnf * RM
+    dx = rng.uniform(0, 1, ptcl)
+    dv = rng.normal(size=(ptcl, r))
+    dv[rng.uniform(size=ptcl) < 0.1] = 0.0
+    dev = torch.device("cuda", 0)
+    dw = torch.full((M, r), 7.0, dtype=torch.float64, device=dev)
+    red.reduction(r, 0, M, RM, dw, torch.from_numpy(dv).to(dev), torch.from_numpy(dx).to(dev),
+                  ptcl)
+    k = (dx * M).astype(int) // bnf
+    want = np.stack([np.bincount(k, weights=dv[:, j], minlength=RM) for j in range(r)], axis=1)
+    got = dw.cpu().numpy()
+    np.testing.assert_allclose(got[:RM], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    assert np.all(got[RM:] == 0.0)
+
+
+def test_electron_kernel_and_post_processing(gpu):
+    """electron_kernel (reference fits, FP64) + post_processing against the
+    oracle fed the same uniforms and gas velocities."""
+    import torch
+    from boltzmann_amd import kernels as K
+    rng = np.random.default_rng(1)
+    N, n, cap = 199, 40_000, 120_000
+    dev = torch.device("cuda", 0)
+    e = np.exp(rng.uniform(np.log(0.1), np.log(60.0), n))
+    d = rng.normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    v = np.sqrt(e / KEV) * d
+    data = np.zeros((cap, 6))
+    tosum = np.zeros((cap, 4))
+    data[:n, 0] = rng.uniform(0.001, 2.539, n)
+    data[:n, 3:6] = v.T
+    tosum[:n, 0] = rng.uniform(1e3, 2e3, n)
+    gas = rng.normal(0, 6.7e4, size=(cap, 3))
+    E = rng.uniform(-60, 60, N)
+    d_data = torch.from_numpy(data).to(dev)
+    d_tosum = torch.from_numpy(tosum).to(dev)
+    d_R = torch.zeros((cap, 7), dtype=torch.float64, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    nn_gas = 1e6 * 3.22e16 * 0.1
+    dt = (1 / 13.56e6) / 1e5
+    K.set_deposit_bins(4)
+    K.electron_kernel(n, d_data, d_tosum, None, torch.from_numpy(E).to(dev), d_R, d_R, None,
+                      None, nn_gas, 1, dt, cnt, None, None, torch.from_numpy(gas).to(dev), 1024,
+                      64)
+    nni = int(cnt[0])
+    RN = N + 1
+    ne, nrg = np.zeros(RN), np.zeros(RN)
+    cg = [np.zeros(RN) for _ in range(4)]
+    res = K.post_processing(None, None, None, None, None, None, d_data, d_tosum, None, True, n,
+                            n, nni, ne, np.zeros(RN), np.zeros(RN), nrg, *cg, 4 * RN, RN, 2.54,
+                            2.54 / N, None)
+    nnew = res[-1]
+    R = d_R[:n].cpu().numpy().T.copy()
+    G = gas[:n].T.copy()
+    s = O.Setup(dim=1, dt=dt, xs_mode=0, n_cells=N, length_cm=2.54, bnf=4, gas_density_m3=nn_gas)
+    st = (data[:n, 0], data[:n, 3], data[:n, 4], data[:n, 5], tosum[:n, 0])
+    want, wmom, wcnt = O.push_step(s, st, E, R, G=G, capacity=cap)
+    assert nni == int(wcnt[0]) and nnew == want[0].size
+    got = d_data[:nnew].cpu().numpy()
+    gw = d_tosum[:nnew].cpu().numpy()
+    vs = np.sqrt(want[1] ** 2 + want[2] ** 2 + want[3] ** 2) + 1
+    np.testing.assert_allclose(got[:, 0], want[0], rtol=1e-12)
+    for k, col in ((1, 3), (2, 4), (3, 5)):
+        assert np.all(np.abs(got[:, col] - want[k]) <= 1e-12 * vs)
+    np.testing.assert_array_equal(gw[:, 0], want[4])
+    np.testing.assert_allclose(res[0], wmom[0], rtol=1e-12, atol=1e-12 * wmom[0].max())
+    np.testing.assert_allclose(res[1], wmom[1], rtol=1e-12, atol=1e-12 * wmom[0].max())
+    np.testing.assert_allclose(res[3], wmom[2], rtol=1e-12, atol=1e-12 * wmom[2].max())
+    # uniforms handed back are the Philox stream
+    np.testing.assert_array_equal(R[[0, 1, 2, 3, 5]],
+                                  O.collision_uniforms(12345, 0, n, np.float64)[[0, 1, 2, 3, 5]])
+
+
+def test_bin_particles_3bin(gpu):
+    import torch
+    from boltzmann_amd import kernels as K
+    from boltzmann_amd import physics as ph
+    rng = np.random.default_rng(2)
+    N, n, cap = 50, 30_000, 40_000
+    dev = torch.device("cuda", 0)
+    tosum = np.zeros((cap, 4))
+    tosum[:n, 0] = rng.uniform(0.5, 2.0, n)
+    tosum[::31, 0] = 0.0
+    tosum[:n, 3] = rng.exponential(8.0, n)
+    cells = np.zeros(cap, dtype=np.int64)
+    cells[:n] = rng.integers(0, N + 1, n)
+    maxp = [400, 200, 300]  # HE, ME, LE
+    lists = [torch.zeros((m, N + 1), dtype=torch.int64, device=dev) for m in maxp]
+    inds = [torch.zeros(N + 1, dtype=torch.int64, device=dev) for _ in maxp]
+    K.bin_particles(torch.from_numpy(tosum).to(dev), *lists, *inds,
+                    torch.from_numpy(cells).to(dev), n, N, *maxp, 64)
+    # brute force in the reference's bin order
+    e = tosum[:n, 3].astype(np.float64)
+    b = np.where(e < 10.0, 2, np.where(e < 15.76, 1, 0))  # index into (HE, ME, LE)
+    alive = tosum[:n, 0] > 0
+    for q in range(3):
+        for c in range(N + 1):
+            ids = np.flatnonzero(alive & (b == q) & (cells[:n] == c))
+            assert int(inds[q][c]) == ids.size
+            m = min(ids.size, maxp[q])
+            np.testing.assert_array_equal(lists[q][:m, c].cpu().numpy(), ids[:m])
+    _ = ph
+
+
+def test_ef_and_heavies_kernels(gpu):
+    from scipy.linalg import cholesky_banded
+    from boltzmann_amd import kernels as K
+    N, L = 120, 2.54
+    dx = L / N
+    Ac = np.zeros((2, N - 1))
+    Ac[0, 1:] = -1.0 / dx ** 2
+    Ac[1, :] = 2.0 / dx ** 2
+    c = cholesky_banded(Ac)
+    diag = c[1].copy()
+    low = np.concatenate([[0.0], c[0, 1:]])
+    rng = np.random.default_rng(3)
+    ne = rng.uniform(0.5e8, 1e8, N + 1)
+    ni = rng.uniform(0.5e8, 1e8, N + 1)
+    E = np.zeros(N)
+    V = np.zeros(N - 1)
+    t = 2.2e-8
+    Vo, Eo = K.EF_kernel(E, ne, ni, t, V, np.zeros(N - 1), diag, low, np.zeros(N + 1), N, dx,
+                         5.5263e5, 100.0, 13.56e6)
+    mom = np.concatenate([ne, np.zeros(N + 1), np.zeros(N + 1)])
+    want = O.field_step(N, dx, 1e-11, 5.5263e5, 100.0, 13.56e6, 1, 1, 1, 0, 1, t, mom, ni.copy(),
+                        np.zeros(N + 1), np.zeros(N + 1), np.zeros(N + 1), np.zeros(N - 1),
+                        np.zeros(N), np.stack([diag, low]))
+    np.testing.assert_allclose(Vo, want[4], rtol=1e-10, atol=1e-10 * np.abs(want[4]).max())
+    np.testing.assert_allclose(Eo, want[5], rtol=1e-10, atol=1e-10 * np.abs(want[5]).max())
+    E2 = rng.uniform(-20, 20, N)
+    ni_h, ns_h, flag = K.heavies_kernel_fluid(0, E2, ni.copy(), np.zeros(N + 1), np.zeros(N),
+                                              np.zeros(N + 1), np.zeros(N + 1), np.zeros(N), dx,
+                                              1e-10, 144.0, 6.4, 7.5, N)
+    want = O.field_step(N, dx, 1e-10, 5.5263e5, 0.0, 1.0, 144.0, 6.4, 7.5, 1, 0, 0.0,
+                        np.zeros(3 * (N + 1)), ni.copy(), np.zeros(N + 1), np.zeros(N + 1),
+                        np.zeros(N + 1), np.zeros(N - 1), E2.copy(), np.zeros(2 * (N - 1)))
+    np.testing.assert_allclose(ni_h, want[0], rtol=1e-12)
+    assert flag == 0
+
+
+@pytest.mark.parametrize("nbins", [1, 3, 5])
+def test_boltzmann_driver_runs(gpu, nbins, tmp_path):
+    from boltzmann_amd.boltzmann import BoltzmannDSMC
+    name = str(tmp_path / "run_")
+    b = BoltzmannDSMC(30, name, 0, 60_000, 101, 4, 1, False, 10, 1e4, 15, nbins=nbins)
+    n0 = b.Nc_list[0]
+    b.run(30, name, 0, verbose=False)
+    n1 = b.Nc_list[0]
+    assert n0 > 0 and n1 > 0
+    c = b.eng.counters()
+    assert c["resample"] >= 0 and c["overflow"] == 0
+    for key, size in (("ne", 101), ("ni", 101), ("Te", 101), ("E", 100), ("ionprod_total", 101)):
+        arr = np.fromfile(name + key + "_cycle2.dat")
+        assert arr.size == size
+    x = np.fromfile(name + "e_x0_cycle2_mpi0.dat")
+    assert x.size > 0 and np.all((x > 0) & (x < 2.54))
+    assert math.isfinite(float(np.fromfile(name + "Te_CA_cycle1.dat").sum()))
