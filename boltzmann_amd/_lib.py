@@ -95,6 +95,16 @@ SIGNATURES = {
                               _P, _P, _P, _P, C.c_int32, C.POINTER(ResamplePolicy)]),
     "bz_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint32)]),
+    "bz_swarm_step": (C.c_int, [_P, C.POINTER(State), C.POINTER(State), C.POINTER(Params),
+                                C.POINTER(XSec), C.c_int32, _P, _P, _P, _P, C.c_int32]),
+    "bz_swarm_control": (C.c_int, [_P, C.POINTER(State), C.POINTER(State), C.POINTER(Params),
+                                   C.c_int32, _P, _P, _P, _P, C.POINTER(ResamplePolicy)]),
+    "bz_swarm_eedf": (C.c_int, [_P, C.POINTER(State), C.c_int32, _P, C.c_int32, C.c_double,
+                                C.c_double, C.c_double, _P]),
+    "bz_graph_begin": (C.c_int, [_P]),
+    "bz_graph_end": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "bz_graph_launch": (C.c_int, [_P, C.c_int32]),
+    "bz_graph_destroy": (C.c_int, [_P, C.c_int32]),
 }
 
 _lib = None

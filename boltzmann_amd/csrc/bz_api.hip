@@ -46,7 +46,10 @@ struct bz_ctx {
   int device;
   LaunchEnv env;
   Clock* host_clock;  // pinned mirror
+  void* graphs;       // captured graphs (GraphSlots), created on first capture
 };
+
+static void graph_release(bz_ctx* c);
 
 #define BZ_HIP(call)                                                      \
   do {                                                                    \
@@ -137,6 +140,7 @@ int bz_ctx_destroy(bz_ctx_t c) {
   if (c->env.partials) hipFree(c->env.partials);
   if (c->env.sort_ws) hipFree(c->env.sort_ws);
   if (c->host_clock) hipHostFree(c->host_clock);
+  graph_release(c);
   if (Timing* t = c->env.timing) {
     for (int k = 0; k < kTimedKinds; ++k) {
       for (int i = 0; i < t->cap; ++i) {
@@ -385,3 +389,154 @@ extern "C" int bz_fill_uniforms(bz_ctx_t c, void* R, int64_t stride, int64_t n, 
   if (n > 0 && !R) return fail("fill_uniforms: null array");
   return launch_fill_uniforms(c->env, R, stride, n, seed, step);
 }
+
+// ---------------------------------------------------------------------------
+// Batched swarms
+// ---------------------------------------------------------------------------
+namespace bz {
+int launch_swarm_step(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                      const bz_params_t* p, const bz_xsec_t* xs, int n_inst, int64_t* off,
+                      const void* inst_E, double* mom, double* acc, int accumulate);
+int launch_swarm_control(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                         const bz_params_t* p, int n_inst, int64_t* off, const int32_t* targets,
+                         int32_t* counts, int32_t* offsets, const bz_resample_policy_t* pol);
+int launch_swarm_eedf(const LaunchEnv& env, const bz_state_t* s, int n_inst, const int64_t* off,
+                      int n_bins, double e_lo, double e_hi, double kev, double* out);
+}  // namespace bz
+
+extern "C" {
+
+int bz_swarm_step(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const bz_params_t* p,
+                  const bz_xsec_t* xs, int32_t n_inst, int64_t* inst_off, const void* inst_E,
+                  double* inst_moments, double* inst_acc, int32_t accumulate) {
+  BZ_CTX(c);
+  if (!p || p->dim != 0) return fail("swarm step: params must be 0-D");
+  if (p->xs_mode != BZ_XS_TABLE || p->rng_mode != BZ_RNG_PHILOX)
+    return fail("swarm step: tabulated model and Philox stream only");
+  if (n_inst < 1 || n_inst > 1024) return fail("swarm step: 1 <= n_inst <= 1024");
+  if (!in || !out || !xs || !inst_off || !inst_E || !inst_moments)
+    return fail("swarm step: null pointer");
+  if (in->vx == out->vx) return fail("in and out state must not alias");
+  return launch_swarm_step(c->env, in, out, p, xs, n_inst, inst_off, inst_E, inst_moments,
+                           inst_acc, accumulate);
+}
+
+int bz_swarm_control(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out,
+                     const bz_params_t* p, int32_t n_inst, int64_t* inst_off,
+                     const int32_t* targets, int32_t* counts_ws, int32_t* offsets_ws,
+                     const bz_resample_policy_t* pol) {
+  BZ_CTX(c);
+  if (n_inst < 1 || n_inst > c->env.max_keys) return fail("swarm control: bad n_inst");
+  if (!in || !out || !p || !inst_off || !targets || !counts_ws || !offsets_ws || !pol)
+    return fail("swarm control: null pointer");
+  if (in->vx == out->vx) return fail("in and out state must not alias");
+  return launch_swarm_control(c->env, in, out, p, n_inst, inst_off, targets, counts_ws,
+                              offsets_ws, pol);
+}
+
+int bz_swarm_eedf(bz_ctx_t c, const bz_state_t* s, int32_t n_inst, const int64_t* inst_off,
+                  int32_t n_bins, double e_lo, double e_hi, double kev, double* out) {
+  BZ_CTX(c);
+  if (n_inst < 1 || n_bins < 1 || n_bins > 8192) return fail("swarm eedf: bad sizes");
+  if (!(e_lo > 0.0) || !(e_hi > e_lo)) return fail("swarm eedf: need 0 < e_lo < e_hi");
+  if (!s || !inst_off || !out) return fail("swarm eedf: null pointer");
+  return launch_swarm_eedf(c->env, s, n_inst, inst_off, n_bins, e_lo, e_hi, kev, out);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// hipGraph capture of enqueued stages (device clock makes replays valid)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kMaxGraphs = 64;
+struct GraphSlots {
+  hipGraphExec_t exec[kMaxGraphs];
+  int used[kMaxGraphs];
+  int saved_sched;
+  int saved_timing;
+  int capturing;
+};
+GraphSlots* slots_of(bz_ctx* c) {
+  if (!c->graphs) {
+    GraphSlots* s = new (std::nothrow) GraphSlots();
+    if (!s) return nullptr;
+    memset(s, 0, sizeof(GraphSlots));
+    c->graphs = s;
+  }
+  return (GraphSlots*)c->graphs;
+}
+}  // namespace
+
+static void graph_release(bz_ctx* c) {
+  GraphSlots* g = (GraphSlots*)c->graphs;
+  if (!g) return;
+  for (int k = 0; k < kMaxGraphs; ++k)
+    if (g->used[k]) hipGraphExecDestroy(g->exec[k]);
+  delete g;
+  c->graphs = nullptr;
+}
+
+extern "C" {
+
+int bz_graph_begin(bz_ctx_t c) {
+  BZ_CTX(c);
+  if (!c->env.stream) return fail("graph capture needs a non-default stream (bz_set_stream)");
+  GraphSlots* g = slots_of(c);
+  if (!g) return fail("too many contexts with graphs on this thread");
+  if (g->capturing) return fail("capture already in progress");
+  g->saved_sched = c->env.push_sched;
+  c->env.push_sched = 1;  // cooperative launches are not captured: ticket schedule
+  g->saved_timing = c->env.timing ? c->env.timing->on : 0;
+  if (c->env.timing) c->env.timing->on = 0;
+  BZ_HIP(hipStreamBeginCapture(c->env.stream, hipStreamCaptureModeRelaxed));
+  g->capturing = 1;
+  return 0;
+}
+
+int bz_graph_end(bz_ctx_t c, int32_t* id) {
+  BZ_CTX(c);
+  GraphSlots* g = slots_of(c);
+  if (!g || !g->capturing) return fail("no capture in progress");
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->env.stream, &graph);
+  g->capturing = 0;
+  c->env.push_sched = g->saved_sched;
+  if (c->env.timing) c->env.timing->on = g->saved_timing;
+  if (e != hipSuccess) return fail("hipStreamEndCapture: %s", hipGetErrorString(e));
+  int slot = -1;
+  for (int k = 0; k < kMaxGraphs; ++k)
+    if (!g->used[k]) {
+      slot = k;
+      break;
+    }
+  if (slot < 0) {
+    hipGraphDestroy(graph);
+    return fail("no free graph slot");
+  }
+  const hipError_t e2 = hipGraphInstantiate(&g->exec[slot], graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e2 != hipSuccess) return fail("hipGraphInstantiate: %s", hipGetErrorString(e2));
+  g->used[slot] = 1;
+  if (id) *id = slot;
+  return 0;
+}
+
+int bz_graph_launch(bz_ctx_t c, int32_t id) {
+  BZ_CTX(c);
+  GraphSlots* g = slots_of(c);
+  if (!g || id < 0 || id >= kMaxGraphs || !g->used[id]) return fail("bad graph id");
+  BZ_HIP(hipGraphLaunch(g->exec[id], c->env.stream));
+  return 0;
+}
+
+int bz_graph_destroy(bz_ctx_t c, int32_t id) {
+  BZ_CTX(c);
+  GraphSlots* g = slots_of(c);
+  if (!g || id < 0 || id >= kMaxGraphs || !g->used[id]) return fail("bad graph id");
+  hipGraphExecDestroy(g->exec[id]);
+  g->used[id] = 0;
+  return 0;
+}
+
+}  // extern "C"

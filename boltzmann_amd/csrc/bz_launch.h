@@ -67,3 +67,18 @@ int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t
 size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys);
 
 }  // namespace bz
+
+namespace bz {
+int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                      const bz_params_t* p, const bz_xsec_t* xs, int n_inst,
+                      const int64_t* inst_off, const void* inst_E, double* inst_moments,
+                      int shape);
+// resampling kernels on a state already grouped by key (no sort): counts,
+// offsets (int32, n_keys+1 / n_keys+2) and keys-per-position built by the caller
+int launch_resample_grouped(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                            const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
+                            const uint32_t* perm, const uint32_t* skeys, const int32_t* targets,
+                            int32_t n_keys, const bz_resample_policy_t* pol, int32_t** plan_out);
+void* sort_scratch_keys(const LaunchEnv& env);   // keys-per-position buffer
+void* sort_scratch_perm(const LaunchEnv& env);   // position -> index buffer
+}  // namespace bz

@@ -198,7 +198,7 @@ template <typename T, int XS, int DIM, int RNG>
 __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, uint64_t step,
                                                     const bz_params_t& p, const bz_xsec_t& xs,
                                                     const T* __restrict__ E, const T* R,
-                                                    const T* G) {
+                                                    const T* G, T e_local) {
   StepOut<T> o;
   o.alive = 0;
   o.born = 0;
@@ -284,7 +284,7 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
     c = c < 0 ? 0 : (c >= p.n_cells ? p.n_cells - 1 : c);
     ef = E[c];
   } else {
-    ef = E ? E[0] : (T)p.e_field0;
+    ef = e_local;  // 0-D: uniform field of the swarm (or of its instance)
   }
   vx = vx - (T)p.afac * ((T)100 * ef);
   if (DIM == 1) o.alive = ((T)0 < x && x < (T)p.L) ? 1 : 0;

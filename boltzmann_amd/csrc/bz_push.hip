@@ -28,21 +28,39 @@ struct PushArgs {
   double* partials;  // [gridDim][3][n_nodes] doubles, then [gridDim][8] u32 counters
   int64_t capacity;
   int sched;         // 0 static tile order (cooperative launch), 1 ticket counter
+  // batched 0-D swarms (DIM 2): instance k owns particles [off[k], off[k+1])
+  const int64_t* inst_off;
+  const void* inst_E;  // [n_inst] field of each instance, V/cm
+  int n_inst;
 };
 
-template <typename T>
+// per-instance moments of a batched step (FP64, per block then summed)
+enum : int {
+  kImW = 0,     // sum w
+  kImWE = 1,    // sum w * energy
+  kImWVX = 2,   // sum w * vx
+  kImEl = 3,    // weight of elastic events
+  kImExc = 4,   // weight of excitation events
+  kImIon = 5,   // weight of ionization events
+  kImBirth = 6, // number of births
+  kImWE2 = 7,   // sum w * energy^2
+  kImN = 8
+};
+
+// 0-D swarms carry no position: x is neither read nor written (WITHX false)
+template <typename T, bool WITHX>
 __device__ __forceinline__ void store5(void* const* out, int64_t pos, const P5<T>& q) {
-  ((T*)out[0])[pos] = q.x;
+  if (WITHX) ((T*)out[0])[pos] = q.x;
   ((T*)out[1])[pos] = q.vx;
   ((T*)out[2])[pos] = q.vy;
   ((T*)out[3])[pos] = q.vz;
   ((T*)out[4])[pos] = q.w;
 }
 
-template <typename T>
+template <typename T, bool WITHX>
 __device__ __forceinline__ void load5(const T* const* in, int64_t i, int64_t n, P5<T>& q) {
   if (i < n) {
-    q.x = in[0][i];
+    q.x = WITHX ? in[0][i] : (T)0;
     q.vx = in[1][i];
     q.vy = in[2][i];
     q.vz = in[3][i];
@@ -63,7 +81,10 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   constexpr int NW = BLOCK / kWave;
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int LB = 1024 / BLOCK > 0 ? 1024 / BLOCK : 1;  // look-back window 1024 tiles
-  extern __shared__ double s_dep[];  // [3][n_nodes] (DIM 1) or [3] (DIM 0), then the table
+  constexpr bool WITHX = DIM == 1;
+  // dynamic LDS: DIM 1 node histograms [3][n_nodes]; DIM 0 [3]; DIM 2 instance
+  // moments [n_inst][kImN], offsets [n_inst+1] and fields [n_inst]; then the table
+  extern __shared__ double s_dep[];
   __shared__ uint64_t s_wsum[NW];
   __shared__ uint32_t s_lbmin;
   __shared__ uint32_t s_lbred[NW];
@@ -75,13 +96,23 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   const int lane = tid & (kWave - 1);
   const int wid = tid / kWave;
   const int nn = (DIM == 1) ? p.n_nodes : 1;
-  for (int j = tid; j < 3 * nn; j += BLOCK) s_dep[j] = 0.0;
+  const int ni_ = (DIM == 2) ? a.n_inst : 0;
+  const int dep_words = (DIM == 2) ? ni_ * kImN + (ni_ + 1) + ni_ : 3 * nn;
+  for (int j = tid; j < 3 * nn && DIM != 2; j += BLOCK) s_dep[j] = 0.0;
+  double* s_im = s_dep;
+  int64_t* s_off = (int64_t*)(s_dep + ni_ * kImN);
+  T* s_ie = (T*)(s_off + ni_ + 1);
+  if (DIM == 2) {
+    for (int j = tid; j < ni_ * kImN; j += BLOCK) s_im[j] = 0.0;
+    for (int j = tid; j <= ni_; j += BLOCK) s_off[j] = a.inst_off[j];
+    for (int j = tid; j < ni_; j += BLOCK) s_ie[j] = ((const T*)a.inst_E)[j];
+  }
   if (tid < 8) s_cnt[tid] = 0ull;
   // stage the cross-section table in LDS (read-only for the whole launch)
   bz_xsec_t xs = a.xs;
   if (XS == BZ_XS_TABLE) {
     const int n = a.xs.n;
-    T* s_tab = (T*)(s_dep + 3 * nn);
+    T* s_tab = (T*)(s_dep + dep_words);
     int32_t* s_hint = (int32_t*)(s_tab + 7 * n);
     const T* ge = (const T*)a.xs.energy;
     const T* gs = (const T*)a.xs.sig;
@@ -112,6 +143,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   const T* __restrict__ E = (const T*)a.E;
   const T* R = (const T*)a.R;
   const T* G = (const T*)a.G;
+  const T e_uniform = (DIM == 0) ? (E ? E[0] : (T)p.e_field0) : (T)0;
 
   uint32_t c_el = 0, c_ex = 0, c_io = 0, c_abs = 0, c_null = 0, c_over = 0, c_drop = 0;
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;  // DIM 0 deposit in registers
@@ -126,7 +158,8 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   __syncthreads();  // s_next is rewritten at the top of the loop
   P5<T> cur[ITEMS];
 #pragma unroll
-  for (int k = 0; k < ITEMS; ++k) load5<T>(in, (int64_t)t * TILE + k * BLOCK + tid, n, cur[k]);
+  for (int k = 0; k < ITEMS; ++k)
+    load5<T, WITHX>(in, (int64_t)t * TILE + k * BLOCK + tid, n, cur[k]);
 
   while (t < ntiles) {
     const int64_t base = (int64_t)t * TILE;
@@ -138,7 +171,20 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const int64_t i = base + (int64_t)k * BLOCK + tid;
-      o[k] = step_particle<T, XS, DIM, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G);
+      int inst = 0;
+      T e_local = e_uniform;
+      if (DIM == 2) {
+        // instance of particle i: largest k with off[k] <= i
+        int lo = 0, hi = ni_ - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_off[mid] <= i) lo = mid;
+          else hi = mid - 1;
+        }
+        inst = lo;
+        e_local = s_ie[inst];
+      }
+      o[k] = step_particle<T, XS, DIM, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, e_local);
       if (i < n && cur[k].w > (T)0) {
         c_el += o[k].chan == kChanElastic;
         c_ex += o[k].chan == kChanExc;
@@ -149,6 +195,39 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       }
       const uint32_t cnt = (uint32_t)(o[k].alive + o[k].born);
       packed |= (uint64_t)cnt << (16 * k);
+      if (DIM == 2) {
+        // per-instance moments of this step; a wave whose lanes share one
+        // instance (the common case, instances are contiguous) sums first
+        double mv[kImN];
+        const double w = o[k].alive ? (double)o[k].p.w : 0.0;
+        const double ep = (double)o[k].e_post;
+        const double ws = o[k].born ? (double)o[k].s.w : 0.0;
+        const double es = (double)o[k].e_sec;
+        const double wc = (double)cur[k].w;
+        mv[kImW] = w + ws;
+        mv[kImWE] = w * ep + ws * es;
+        mv[kImWVX] = w * (double)o[k].p.vx + ws * (double)o[k].s.vx;
+        mv[kImEl] = o[k].chan == kChanElastic ? wc : 0.0;
+        mv[kImExc] = o[k].chan == kChanExc ? wc : 0.0;
+        mv[kImIon] = o[k].chan == kChanIon ? wc : 0.0;
+        mv[kImBirth] = (double)o[k].born;
+        mv[kImWE2] = w * ep * ep + ws * es * es;
+        const bool valid = i < n;
+        const int inst0 = __shfl(inst, 0, kWave);
+        const bool uniform = __all(!valid || inst == inst0);
+        if (uniform) {
+#pragma unroll
+          for (int q = 0; q < kImN; ++q) {
+            const double sq = wave_sum(valid ? mv[q] : 0.0);
+            if (lane == 0 && sq != 0.0) atomicAdd(&s_im[inst0 * kImN + q], sq);
+          }
+        } else if (valid) {
+#pragma unroll
+          for (int q = 0; q < kImN; ++q)
+            if (mv[q] != 0.0) atomicAdd(&s_im[inst * kImN + q], mv[q]);
+        }
+        continue;
+      }
       // deposit: survivors at their new position, secondaries where born
       if (o[k].alive) {
         const double w = (double)o[k].p.w;
@@ -202,7 +281,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     // prefetch the next tile so its HBM latency overlaps the look-back
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k)
-      load5<T>(in, (int64_t)t_next * TILE + k * BLOCK + tid, n, cur[k]);
+      load5<T, WITHX>(in, (int64_t)t_next * TILE + k * BLOCK + tid, n, cur[k]);
     const uint32_t pre = lookback_block<BLOCK, LB>(a.status, t, agg, epoch, &s_lbmin, s_lbred);
     if (tid == 0 && t == ntiles - 1) a.clk->n_next = (int64_t)(pre + agg);
     const int64_t tile_pre = pre;
@@ -211,12 +290,12 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       const int64_t pos =
           tile_pre + field_base[k] + (int64_t)((excl >> (16 * k)) & 0xFFFFu);
       if (o[k].alive) {
-        if (pos < a.capacity) store5<T>(a.out, pos, o[k].p);
+        if (pos < a.capacity) store5<T, WITHX>(a.out, pos, o[k].p);
         else ++c_drop;
       }
       if (o[k].born) {
         const int64_t ps = pos + o[k].alive;
-        if (ps < a.capacity) store5<T>(a.out, ps, o[k].s);
+        if (ps < a.capacity) store5<T, WITHX>(a.out, ps, o[k].s);
         else ++c_drop;
       }
     }
@@ -253,9 +332,10 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     }
   }
   __syncthreads();
-  double* part = a.partials + (size_t)blockIdx.x * 3 * nn;
-  for (int j = tid; j < 3 * nn; j += BLOCK) part[j] = s_dep[j];
-  uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * 3 * nn);
+  const int m = (DIM == 2) ? ni_ * kImN : 3 * nn;
+  double* part = a.partials + (size_t)blockIdx.x * m;
+  for (int j = tid; j < m; j += BLOCK) part[j] = s_dep[j];
+  uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * m);
   if (tid < 8) bcnt[(size_t)blockIdx.x * 8 + tid] = (uint32_t)s_cnt[tid];
 }
 
@@ -349,7 +429,11 @@ namespace bz {
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
   auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS>;
-  size_t smem = (size_t)3 * (DIM == 1 ? a.p.n_nodes : 1) * sizeof(double);
+  // same layout as the kernel's dep_words (the field slots are a double each)
+  const int dep_doubles = DIM == 2 ? a.n_inst * kImN + (a.n_inst + 1) + a.n_inst
+                                   : 3 * (DIM == 1 ? a.p.n_nodes : 1);
+  const int part_doubles = DIM == 2 ? a.n_inst * kImN : 3 * (DIM == 1 ? a.p.n_nodes : 1);
+  size_t smem = (size_t)dep_doubles * sizeof(double);
   if (XS == BZ_XS_TABLE) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
   if (smem > 160 * 1024) return fail("push: deposit + table need %zu B of LDS (> 160 KiB)", smem);
   if (smem > 64 * 1024) {
@@ -363,8 +447,8 @@ static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
     return fail("push kernel does not fit on a CU (LDS %zu B)", smem);
   // every block resident: occupancy-sized grid, launched cooperatively
   int grid = env.num_cu * (occ > 8 ? 8 : occ);
-  // partial histograms [grid][3 nn] doubles followed by [grid][8] u32 counters
-  const int64_t cap_parts = env.partials_doubles / (3 * (DIM == 1 ? a.p.n_nodes : 1) + 4);
+  // partial histograms [grid][part] doubles followed by [grid][8] u32 counters
+  const int64_t cap_parts = env.partials_doubles / (part_doubles + 4);
   if (grid > cap_parts) grid = (int)cap_parts;
   if (grid < 1) return fail("deposit partial buffer too small");
   a.sched = env.push_sched;
@@ -397,6 +481,11 @@ template <typename T>
 static int launch_push_prec(const LaunchEnv& env, PushArgs& a, int shape, int* grid) {
   const bz_params_t& p = a.p;
   const bool stream = p.rng_mode == BZ_RNG_STREAM;
+  if (a.n_inst > 0) {
+    if (p.xs_mode != BZ_XS_TABLE || stream)
+      return fail("batched swarms use the tabulated model and the Philox stream");
+    return launch_push_shape<T, BZ_XS_TABLE, 2, BZ_RNG_PHILOX>(env, a, shape, grid);
+  }
   if (p.xs_mode == BZ_XS_TABLE) {
     if (p.dim == 1)
       return stream ? launch_push_shape<T, BZ_XS_TABLE, 1, BZ_RNG_STREAM>(env, a, 0, grid)
@@ -431,6 +520,9 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
   a.status = env.status;
   a.partials = env.partials;
   a.capacity = env.capacity;
+  a.inst_off = nullptr;
+  a.inst_E = nullptr;
+  a.n_inst = 0;
   int grid = 0;
   const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &grid)
                                          : launch_push_prec<float>(env, a, shape, &grid);
@@ -578,6 +670,48 @@ int launch_fill_uniforms(const LaunchEnv& env, void* R, int64_t stride, int64_t 
     hipLaunchKernelGGL(k_fill_uniforms<float>, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
                        (float*)R, stride, n, seed, step);
   return check_launch("k_fill_uniforms");
+}
+
+}  // namespace bz
+
+// ---------------------------------------------------------------------------
+// Batched 0-D swarms: the fused step over all instances in one launch, then
+// the per-instance moments summed over blocks (k_dep_reduce commits the clock).
+// ---------------------------------------------------------------------------
+namespace bz {
+
+int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                      const bz_params_t* p, const bz_xsec_t* xs, int n_inst,
+                      const int64_t* inst_off, const void* inst_E, double* inst_moments,
+                      int shape) {
+  PushArgs a;
+  a.p = *p;
+  a.xs = *xs;
+  const void* iv[5] = {in->x, in->vx, in->vy, in->vz, in->w};
+  void* ov[5] = {out->x, out->vx, out->vy, out->vz, out->w};
+  for (int k = 0; k < 5; ++k) {
+    a.in[k] = iv[k];
+    a.out[k] = ov[k];
+  }
+  a.E = nullptr;
+  a.R = nullptr;
+  a.G = nullptr;
+  a.clk = env.clock;
+  a.status = env.status;
+  a.partials = env.partials;
+  a.capacity = env.capacity;
+  a.inst_off = inst_off;
+  a.inst_E = inst_E;
+  a.n_inst = n_inst;
+  int grid = 0;
+  const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &grid)
+                                         : launch_push_prec<float>(env, a, shape, &grid);
+  if (rc) return rc;
+  const int m = n_inst * kImN;
+  hipLaunchKernelGGL(k_dep_reduce, dim3((m + 3) / 4), dim3(256), 0, env.stream,
+                     (const double*)env.partials, grid, m, inst_moments, env.clock, 1,
+                     (int64_t)env.capacity);
+  return check_launch("k_dep_reduce");
 }
 
 }  // namespace bz

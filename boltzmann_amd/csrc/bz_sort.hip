@@ -824,13 +824,16 @@ template <typename T>
 static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
                              const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
                              const uint32_t* perm, const int32_t* targets, int32_t n_keys,
-                             const bz_resample_policy_t* pol) {
+                             const bz_resample_policy_t* pol, const uint32_t* skeys_in,
+                             int32_t** plan_out) {
   SortWs ws = carve(env.sort_ws, env.capacity, env.max_keys);
-  const int nb = n_keys / (p->dim == 1 ? p->n_nodes : 1);
-  if (nb < 1 || nb * (p->dim == 1 ? p->n_nodes : 1) != n_keys)
+  // keys per cell: nb bins per node (1-D), or one key per instance (grouped)
+  const int nb = skeys_in ? 1 : n_keys / (p->dim == 1 ? p->n_nodes : 1);
+  if (!skeys_in && (nb < 1 || nb * (p->dim == 1 ? p->n_nodes : 1) != n_keys))
     return fail("resample: n_keys must be n_nodes * nb");
   // the final pass wrote the sorted keys to keys_a (two passes) or keys_b (one)
-  const uint32_t* skeys = (n_keys + 1) <= 256 ? ws.keys_b : ws.keys_a;
+  const uint32_t* skeys = skeys_in ? skeys_in : ((n_keys + 1) <= 256 ? ws.keys_b : ws.keys_a);
+  if (plan_out) *plan_out = ws.plan;
   ResArgs a;
   const void* iv[5] = {in->x, in->vx, in->vy, in->vz, in->w};
   void* ov[5] = {out->x, out->vx, out->vy, out->vz, out->w};
@@ -868,8 +871,28 @@ int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t
                     const bz_resample_policy_t* pol) {
   if (env.precision == BZ_F64)
     return launch_resample_t<double>(env, in, out, p, counts, offsets, perm, targets, n_keys,
-                                     pol);
-  return launch_resample_t<float>(env, in, out, p, counts, offsets, perm, targets, n_keys, pol);
+                                     pol, nullptr, nullptr);
+  return launch_resample_t<float>(env, in, out, p, counts, offsets, perm, targets, n_keys, pol,
+                                  nullptr, nullptr);
+}
+
+int launch_resample_grouped(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
+                            const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
+                            const uint32_t* perm, const uint32_t* skeys, const int32_t* targets,
+                            int32_t n_keys, const bz_resample_policy_t* pol, int32_t** plan_out) {
+  if (env.precision == BZ_F64)
+    return launch_resample_t<double>(env, in, out, p, counts, offsets, perm, targets, n_keys,
+                                     pol, skeys, plan_out);
+  return launch_resample_t<float>(env, in, out, p, counts, offsets, perm, targets, n_keys, pol,
+                                  skeys, plan_out);
+}
+
+void* sort_scratch_keys(const LaunchEnv& env) {
+  return carve(env.sort_ws, env.capacity, env.max_keys).keys_b;
+}
+
+void* sort_scratch_perm(const LaunchEnv& env) {
+  return carve(env.sort_ws, env.capacity, env.max_keys).vals_b;
 }
 
 }  // namespace bz

@@ -251,6 +251,40 @@ int bz_resample(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
                 const uint32_t* perm, const int32_t* targets, int32_t n_keys,
                 const bz_resample_policy_t* pol);
 
+/* ---- Batched 0-D swarms (no counterpart in the reference: its 0-D test
+ * problems are single runs made by hand-editing the driver, README.md:7-8) ----
+ * All instances live in one SoA state; instance k owns [inst_off[k],
+ * inst_off[k+1]) (int64 device array, n_inst+1 entries, updated in place).
+ * x is not used.  inst_E: [n_inst] field (V/cm) in the context precision.
+ * inst_moments: [n_inst][8] FP64 of this step: sum w, sum w e, sum w vx,
+ * weights of elastic / excitation / ionization events, births, sum w e^2.
+ * inst_acc (optional): running sums of inst_moments when accumulate != 0. */
+int bz_swarm_step(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
+                  const bz_params_t* p, const bz_xsec_t* xs, int32_t n_inst, int64_t* inst_off,
+                  const void* inst_E, double* inst_moments, double* inst_acc,
+                  int32_t accumulate);
+/* Population control of every instance towards targets[k] with the ES-WPNC
+ * merge / split of bz_resample (instances are already grouped: no sort).
+ * counts_ws [n_inst+1] and offsets_ws [n_inst+2] are int32 device scratch. */
+int bz_swarm_control(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
+                     const bz_params_t* p, int32_t n_inst, int64_t* inst_off,
+                     const int32_t* targets, int32_t* counts_ws, int32_t* offsets_ws,
+                     const bz_resample_policy_t* pol);
+/* EEDF per instance: sum of weights in n_bins log-spaced energy bins on
+ * [e_lo, e_hi] eV (out [n_inst][n_bins] FP64). */
+int bz_swarm_eedf(bz_ctx_t ctx, const bz_state_t* s, int32_t n_inst, const int64_t* inst_off,
+                  int32_t n_bins, double e_lo, double e_hi, double kev, double* out);
+
+/* ---- hipGraph capture of enqueued stages (no counterpart in the reference).
+ * Everything between begin and end on the context stream (which must be a
+ * created stream) becomes one graph; counts, step and time are read from the
+ * device clock, so replays advance the run.  Pushes inside a capture use the
+ * ticket schedule. */
+int bz_graph_begin(bz_ctx_t ctx);
+int bz_graph_end(bz_ctx_t ctx, int32_t* id);
+int bz_graph_launch(bz_ctx_t ctx, int32_t id);
+int bz_graph_destroy(bz_ctx_t ctx, int32_t id);
+
 /* Host helper: Philox4x32-10 block (for tests and stream generation). */
 void bz_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

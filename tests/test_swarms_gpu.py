@@ -1,0 +1,136 @@
+"""Batched 0-D swarms: parity of one launch over many instances with the
+oracle run instance by instance, population control conservation, graph
+replay, diagnostics against closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EN = (5.0, 50.0, 200.0, 500.0, 800.0, 1000.0)
+COUNTS = (300, 1200, 50, 2000, 700, 900)
+
+
+def _batch(prec, **kw):
+    from boltzmann_amd import _lib
+    from boltzmann_amd.swarms import SwarmBatch, SwarmConfig
+    cfg = SwarmConfig(name="t", en_td=kw.pop("en", EN), counts=kw.pop("counts", COUNTS),
+                      precision=prec, te_ev=kw.pop("te", 8.0), dt=kw.pop("dt", 5e-12), **kw)
+    b = SwarmBatch(cfg)
+    b.load_initial()
+    _ = _lib
+    return b
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_batched_step_matches_oracle_per_instance(gpu, prec):
+    b = _batch(prec)
+    dtype = np.float64 if prec == 8 else np.float32
+    st = b.state()
+    off = b.offsets()
+    tab = (b.table.energy, b.table.sig, b.table.slope)
+    for step in range(3):
+        b.step(accumulate=False)
+        got = b.state()
+        goff = b.offsets()
+        mom = b.moments.cpu().numpy().reshape(b.n_inst, 8)
+        R = O.collision_uniforms(b.cfg.seed, step, st[0].size, dtype)
+        outs = []
+        for k in range(b.n_inst):
+            sl = slice(off[k], off[k + 1])
+            s = O.Setup(dim=0, dt=b.cfg.dt, xs_mode=1, gas_density_m3=b.cfg.gas_density_m3,
+                        nu_max=b.table.nu_max, e_field0=float(b.e_vcm[k]))
+            part = tuple(np.ascontiguousarray(a[sl]) for a in st)
+            o, wm, wc = O.push_step(s, part, None, np.ascontiguousarray(R[:, sl]), table=tab,
+                                    capacity=4 * part[0].size + 16)
+            outs.append(o)
+            assert goff[k + 1] - goff[k] == o[0].size, (step, k)
+            assert mom[k, 6] == wc[0]
+            rt = 1e-12 if prec == 8 else 1e-6
+            assert abs(mom[k, 0] - wm[0, 0]) <= rt * wm[0, 0]
+            assert abs(mom[k, 1] - wm[2, 0]) <= rt * wm[2, 0]
+        want = tuple(np.concatenate([o[j] for o in outs]) for j in range(5))
+        rt = 1e-12 if prec == 8 else 2e-6
+        vs = np.sqrt(sum(want[j].astype(np.float64) ** 2 for j in (1, 2, 3))) + 1
+        for j in (1, 2, 3):
+            assert np.all(np.abs(got[j] - want[j]) <= rt * vs), (step, j)
+        np.testing.assert_array_equal(got[4], want[4])
+        st, off = got, goff
+    b.close()
+
+
+def test_population_control_conserves(gpu):
+    b = _batch(8, en=(1000.0, 900.0, 10.0), counts=(400, 300, 200), te=30.0, dt=2e-11,
+               capacity_factor=40.0)
+    for _ in range(120):
+        b.step()
+    st = b.state()
+    off = b.offsets()
+    c0 = np.diff(off)
+    assert c0[0] > 1.5 * 400  # grew by ionization
+    b.control()
+    st2 = b.state()
+    off2 = b.offsets()
+    c1 = np.diff(off2)
+    for k, tgt in enumerate((400, 300, 200)):
+        # merged back to the target when at or above 1.5x, untouched otherwise
+        assert c1[k] == (tgt if c0[k] >= 1.5 * tgt else c0[k]), (k, c0[k], c1[k])
+    for k in range(3):
+        a = [x[off[k]:off[k + 1]] for x in st]
+        z = [x[off2[k]:off2[k + 1]] for x in st2]
+        W0, W1 = a[4].sum(), z[4].sum()
+        assert abs(W1 - W0) <= 1e-12 * W0
+        E0 = np.sum(a[4] * (a[1] ** 2 + a[2] ** 2 + a[3] ** 2))
+        E1 = np.sum(z[4] * (z[1] ** 2 + z[2] ** 2 + z[3] ** 2))
+        assert abs(E1 - E0) <= 1e-12 * E0
+        P0, P1 = np.sum(a[4] * a[1]), np.sum(z[4] * z[1])
+        assert abs(P1 - P0) <= 1e-12 * math.sqrt(W0 * E0)
+    b.close()
+
+
+def test_graph_replay_matches_eager(gpu):
+    b1 = _batch(4)
+    b2 = _batch(4)
+    for _ in range(8):
+        b1.step()
+    b1.control()
+    b1.control()
+    gid = b2.capture(8, accumulate=False, with_control=True)
+    b2.replay(gid)
+    s1, s2 = b1.state(), b2.state()
+    for u, v in zip(s1, s2):
+        np.testing.assert_array_equal(u, v)
+    np.testing.assert_array_equal(b1.offsets(), b2.offsets())
+    # replays keep advancing the run (step counter lives on the device)
+    b2.replay(gid, times=2)
+    for _ in range(2):
+        for _ in range(8):
+            b1.step()
+        b1.control()
+        b1.control()
+    for u, v in zip(b1.state(), b2.state()):
+        np.testing.assert_array_equal(u, v)
+    b1.close()
+    b2.close()
+
+
+def test_zero_field_elastic_rate_closed_form(gpu):
+    """E/N -> 0, Maxwellian at T: k_el = sigma <v> = sigma sqrt(8 e T/(pi m)).
+    200 steps x 2e5 electrons give ~8.6e4 elastic events (0.34% statistical
+    error); dividing by dt instead of the null-collision pc would bias the
+    estimate by nu_max dt / 2 = 3%."""
+    b = _batch(8, en=(1e-9,), counts=(200_000,), te=1.0, dt=1e-12)
+    b.reset_accumulators()
+    for _ in range(200):
+        b.step(accumulate=True)
+    r = b.results()
+    sigma = b.cfg.xsec.sigma_el
+    k = sigma * math.sqrt(8 * 1.602e-19 * 1.0 / (math.pi * 9.10938e-31))
+    assert abs(r["k_elastic"][0] / k - 1) < 0.015
+    assert abs(r["mean_energy"][0] / 1.5 - 1) < 0.01
+    edges, f = b.eedf(200, 0.01, 1000.0)
+    assert abs(np.sum(f[0] * np.diff(edges)) - 1) < 1e-12
+    b.close()
