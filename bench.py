@@ -1,20 +1,27 @@
 #!/usr/bin/env python3
 """Throughput of the electron kinetic step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0|c2|c3|c4]
 
-A step is one kinetic update of the whole particle population: the fused
-collision + push + deposit + compaction kernel, the deposit reduction and
-the device field step (moment update, fluid ions, Poisson).  The state is
+A step is one kinetic update of the whole particle population.  For the
+discharges and the single swarm (c0, c1, c3, c4) that is the fused
+collision + push + deposit + compaction kernel, the deposit reduction, the
+device field step (moment update, fluid ions, Poisson) and, on the steps the
+workload asks for it, binning + merge/split resampling.  For the batched
+swarms (c2) it is one launch over all 256 instances, replayed from a
+captured hipGraph with population control every 50 steps.  The state is
 synthetic (Maxwellian, uniform density) and resident in HBM before timing.
+
 Prints one JSON line.  The roofline object is for the dominant kernel
-(k_push), timed with HIP events on its own stream inside the library; the
-CPU baseline is the oracle port timed on a bounded sample on the host.
+(k_push), timed with HIP events on the library's stream; the CPU baseline is
+the oracle port timed on a bounded sample of the same workload on the host.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -22,65 +29,101 @@ import time
 HBM_PEAK_GBS = 8000.0
 
 
+def _cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def _time_oracle(setup, state, E, n, table, seed, seconds, capacity):
+    import numpy as np
+
+    from oracle import oracle as O
+    R = O.collision_uniforms(seed, 0, n, state[0].dtype)
+    done = 0
+    psteps = 0
+    cur = state
+    t0 = time.perf_counter()
+    while True:
+        m = cur[0].size
+        cur, _, _ = O.push_step(setup, cur, E, R if m >= n else np.ascontiguousarray(R[:, :m]),
+                                table=table, capacity=capacity)
+        psteps += m
+        done += 1
+        if time.perf_counter() - t0 > seconds or done >= 1000:
+            break
+    return psteps / (time.perf_counter() - t0), done
+
+
 def cpu_baseline(cfg, seconds: float = 10.0):
-    """Oracle (FP64-capable C++ port of the reference step, OpenMP) on a
-    bounded sample of the same workload: push + collisions + deposit +
-    compaction of up to 1M particles, repeated for about `seconds`."""
+    """Oracle (C++ restatement of the reference step, OpenMP over particles)
+    on a bounded sample of the same workload: push + collisions + deposit +
+    compaction of up to 1M particles, repeated for about `seconds`.  The
+    uniforms are drawn outside the timed region (the GPU draws them
+    in-kernel)."""
     import numpy as np
 
     from boltzmann_amd import physics as ph
     from oracle import oracle as O
 
+    from boltzmann_amd import _lib
+
     n = min(cfg.n_particles, 1_000_000)
-    sub = cfg.with_(n_particles=n)
+    # FP64, as the reference computes every stage
+    sub = cfg.with_(n_particles=n, precision=_lib.BZ_F64)
     st = ph.initial_state(sub)
-    table = ph.build_table(cfg.xsec, cfg.gas_density_m3, cfg.np_dtype) \
+    table = ph.build_table(cfg.xsec, cfg.gas_density_m3, sub.np_dtype) \
         if cfg.xs_mode == 1 else None
-    cores = max(1, min(os.cpu_count() or 1, 16))
+    cores = _cores()
     setup = O.Setup(dim=cfg.dim, dt=cfg.dt, xs_mode=cfg.xs_mode,
                     n_cells=cfg.n_cells, length_cm=cfg.length_cm, bnf=cfg.bnf,
                     gas_density_m3=cfg.gas_density_m3,
                     nu_max=table.nu_max if table is not None else 0.0,
                     e_field0=cfg.e_field0_vcm() if cfg.dim == 0 else 0.0, nthreads=cores)
     tab = None if table is None else (table.energy, table.sig, table.slope)
-    E = np.zeros(cfg.n_cells, dtype=cfg.np_dtype) if cfg.dim == 1 else None
-    # uniforms generated outside the timed region (the GPU draws them in-kernel)
-    R = O.collision_uniforms(cfg.seed, 0, n, cfg.np_dtype)
-    done = 0
-    psteps = 0
-    t0 = time.perf_counter()
-    cur = st
-    while True:
-        cur, _, _ = O.push_step(setup, cur, E, R[:, :cur[0].size].copy() if cur[0].size < n
-                                else R, table=tab, capacity=2 * n)
-        psteps += cur[0].size
-        done += 1
-        if time.perf_counter() - t0 > seconds or done >= 1000:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": psteps / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+    E = np.zeros(cfg.n_cells, dtype=np.float64) if cfg.dim == 1 else None
+    rate, done = _time_oracle(setup, st, E, n, tab, cfg.seed, seconds, 2 * n)
+    return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"{n} particles x {done} steps of the oracle push+MCC+deposit+compaction "
-                      f"({'FP64' if cfg.precision == 8 else 'FP32'}, OpenMP {cores} threads)"}
+                      f"(FP64 like the reference, OpenMP {cores} threads)"}
 
 
-def main(argv=None) -> int:
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c1")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--shape", type=int, default=-1, help="push tile shape (-1: autotune)")
-    ap.add_argument("--sched", type=int, default=0,
-                    help="push tile schedule: 0 static + cooperative launch, 1 ticket counter")
-    args = ap.parse_args(argv)
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the particle "
-              "decomposition across GPUs is a design note, not a measured path.",
-              file=sys.stderr)
-        return 2
+def cpu_baseline_swarms(scfg, seconds: float = 10.0):
+    """Oracle on one 1M-electron 0-D swarm at the median reduced field of
+    the batch (the per-instance work of the batched launch)."""
+    import numpy as np
 
+    from boltzmann_amd import physics as ph
+    from oracle import oracle as O
+
+    n = 1_000_000
+    en = float(np.median(scfg.fields_td()))
+    table = ph.build_table(scfg.xsec, scfg.gas_density_m3, np.float64)
+    rng = np.random.default_rng(scfg.seed)
+    sv = math.sqrt(scfg.te_ev * ph.Q_E / ph.M_E)
+    dt = np.float64
+    st = (np.zeros(n, dt), *(rng.normal(0, sv, n).astype(dt) for _ in range(3)), np.ones(n, dt))
+    cores = _cores()
+    setup = O.Setup(dim=0, dt=scfg.dt, xs_mode=1, gas_density_m3=scfg.gas_density_m3,
+                    nu_max=table.nu_max, e_field0=en * ph.TD * scfg.gas_density_m3 / 100.0,
+                    nthreads=cores)
+    rate, done = _time_oracle(setup, st, None, n, (table.energy, table.sig, table.slope),
+                              scfg.seed, seconds, 2 * n)
+    return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"one 0-D swarm of {n} particles at {en:.1f} Td x {done} steps of the "
+                      f"oracle push+MCC (FP64 like the reference, OpenMP {cores} threads)"}
+
+
+def _roofline(bytes_per_pstep, psteps_per_launch, avg_s, kernel):
+    achieved = bytes_per_pstep * psteps_per_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+            "kernel_avg_us": avg_s * 1e6, "bytes_per_particle_step": bytes_per_pstep}
+
+
+def bench_engine(args):
     import torch
 
     from boltzmann_amd import _lib, configs
@@ -113,24 +156,19 @@ def main(argv=None) -> int:
         index += 1
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    import ctypes as C
     ms = (C.c_double * 3)()
     nl = (C.c_int64 * 3)()
     _lib.check(eng.lib.bz_timing_get(eng.ctx, ms, nl))
     c1 = eng.counters()
     psteps = c1["particle_steps"] - c0["particle_steps"]
     elapsed = t1 - t0
-    value = psteps / elapsed
     word = 8 if cfg.precision == _lib.BZ_F64 else 4
-    bytes_per_pstep = 2 * 5 * word  # read + write x, vx, vy, vz, w
-    push_launches = max(int(nl[0]), 1)
-    avg_push_s = ms[0] / 1e3 / push_launches
-    per_launch = psteps / args.steps
-    achieved = bytes_per_pstep * per_launch / avg_push_s / 1e9
-    n_end = eng.count()
+    # read + write of the carried components (0-D runs carry no position)
+    bytes_per_pstep = 2 * (5 if cfg.dim == 1 else 4) * word
+    avg_push_s = ms[0] / 1e3 / max(int(nl[0]), 1)
     out = {
         "metric": "particle-steps/s on 1 MI355X",
-        "value": value,
+        "value": psteps / elapsed,
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -142,12 +180,9 @@ def main(argv=None) -> int:
         "dtype": "fp64" if cfg.precision == _lib.BZ_F64 else "fp32",
         "data": "synthetic (Maxwellian electrons, uniform density, tabulated argon-like sigma)",
         "config": {"workload": cfg.name, "particles_start": cfg.n_particles,
-                   "particles_end": n_end, "cells": cfg.n_cells if cfg.dim == 1 else 0,
+                   "particles_end": eng.count(), "cells": cfg.n_cells if cfg.dim == 1 else 0,
                    "dt_s": cfg.dt, "parallelism": "single GPU", "push_schedule": args.sched},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_push", "kernel_avg_us": avg_push_s * 1e6,
-                     "bytes_per_particle_step": bytes_per_pstep},
+        "roofline": _roofline(bytes_per_pstep, psteps / args.steps, avg_push_s, "k_push"),
         "counters": {k: c1[k] - c0[k] for k in ("births", "absorbed", "ionization", "overflow",
                                                 "resample", "split", "merge")},
         "kernel_ms": {"push": ms[0], "sort": ms[1], "resample": ms[2],
@@ -156,8 +191,110 @@ def main(argv=None) -> int:
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
     eng.close()
+    return out
+
+
+def bench_swarms(args):
+    """256 batched 0-D swarms: graph replay of 100 steps + 2 controls."""
+    import torch
+
+    from boltzmann_amd import _lib
+    from boltzmann_amd.autotune import tune_swarms
+    from boltzmann_amd.swarms import SwarmBatch, c2
+
+    scfg = c2()
+    b = SwarmBatch(scfg)
+    b.load_initial()
+    tuned = None
+    if args.shape >= 0:
+        _lib.check(b.lib.bz_set_push_shape(b.ctx, args.shape))
+    else:
+        tuned = tune_swarms(b, steps=10)
+    per_graph = 2 * scfg.control_every
+    cur_cap = b.cur
+    gid = b.capture(per_graph, accumulate=True, control_every=scfg.control_every)
+    reps_w = max(1, args.warmup // per_graph)
+    reps = max(1, args.steps // per_graph)
+    b.replay(gid, reps_w)
+    b.synchronize()
+    # kernel time of the push, from eager steps with the library's events
+    _lib.check(b.lib.bz_timing_enable(b.ctx, 1))
+    n_before = b.count()
+    for _ in range(20):
+        b.step(accumulate=True)
+    ms = (C.c_double * 3)()
+    nl = (C.c_int64 * 3)()
+    b.synchronize()
+    _lib.check(b.lib.bz_timing_get(b.ctx, ms, nl))
+    _lib.check(b.lib.bz_timing_enable(b.ctx, 0))
+    n_mid = b.count()
+    if b.cur != cur_cap:  # restore the buffer parity the graph was captured with
+        b.step(accumulate=True)
+    b.synchronize()
+    c0 = b.counters()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.replay(gid, reps)
+    b.synchronize()
+    t1 = time.perf_counter()
+    c1 = b.counters()
+    psteps = c1["particle_steps"] - c0["particle_steps"]
+    steps = reps * per_graph
+    elapsed = t1 - t0
+    word = 8 if scfg.precision == _lib.BZ_F64 else 4
+    bytes_per_pstep = 2 * 4 * word  # vx, vy, vz, w read + write
+    avg_push_s = ms[0] / 1e3 / max(int(nl[0]), 1)
+    res = b.results()
+    out = {
+        "metric": "particle-steps/s on 1 MI355X",
+        "value": psteps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": reps_w * per_graph,
+        "ms_per_step": elapsed / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64" if scfg.precision == _lib.BZ_F64 else "fp32",
+        "data": "synthetic (Maxwellian 1 eV swarms, tabulated argon-like sigma)",
+        "config": {"workload": scfg.name, "instances": b.n_inst,
+                   "particles_start": b.total0, "particles_end": b.count(),
+                   "en_td": [scfg.en_td_lo, scfg.en_td_hi], "dt_s": scfg.dt,
+                   "graph": f"{per_graph} steps + control every {scfg.control_every}",
+                   "parallelism": "single GPU"},
+        "roofline": _roofline(bytes_per_pstep, 0.5 * (n_before + n_mid), avg_push_s, "k_push"),
+        "counters": {k: c1[k] - c0[k] for k in ("births", "ionization", "split", "merge")},
+        "sample_results": {"en_td": [float(res["en_td"][i]) for i in (0, 128, 255)],
+                           "mean_energy_ev": [float(res["mean_energy"][i]) for i in (0, 128, 255)]},
+        "autotune": tuned,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_swarms(scfg, args.cpu_seconds)
+    b.close()
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--shape", type=int, default=-1, help="push tile shape (-1: autotune)")
+    ap.add_argument("--sched", type=int, default=0,
+                    help="push tile schedule: 0 static + cooperative launch, 1 ticket counter")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the particle "
+              "decomposition across GPUs is a design note, not a measured path.",
+              file=sys.stderr)
+        return 2
+    out = bench_swarms(args) if args.workload == "c2" else bench_engine(args)
+    print(json.dumps(out), flush=True)
     return 0
 
 

@@ -53,7 +53,8 @@ class FieldParams(C.Structure):
                 ("v0", C.c_double), ("freq", C.c_double), ("mu_i", C.c_double),
                 ("D_i", C.c_double), ("D_s", C.c_double), ("n", C.c_int32),
                 ("fluid_on", C.c_int32), ("poisson_on", C.c_int32),
-                ("skip_moments", C.c_int32)]
+                ("skip_moments", C.c_int32), ("ion_scheme", C.c_int32),
+                ("max_substeps", C.c_int32)]
 
 
 class ResamplePolicy(C.Structure):
@@ -63,6 +64,18 @@ class ResamplePolicy(C.Structure):
                 ("ref_trigger", C.c_int32), ("max_ppc", C.c_int32),
                 ("ref_rescale", C.c_int32), ("pad", C.c_int32)]
 
+
+class Mlp(C.Structure):
+    _fields_ = [("w", C.c_void_p), ("in_shift", C.c_double * 3), ("in_scale", C.c_double * 3),
+                ("out_shift", C.c_double * 3), ("out_scale", C.c_double * 3)]
+
+
+class Adam(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("bias1", C.c_float), ("bias2", C.c_float)]
+
+
+BZ_MLP_NPARAM = 8771
 
 # name -> (restype, argtypes); every symbol include/bz.h declares
 _P = C.c_void_p
@@ -101,6 +114,11 @@ SIGNATURES = {
                                    C.c_int32, _P, _P, _P, _P, C.POINTER(ResamplePolicy)]),
     "bz_swarm_eedf": (C.c_int, [_P, C.POINTER(State), C.c_int32, _P, C.c_int32, C.c_double,
                                 C.c_double, C.c_double, _P]),
+    "bz_mlp_forward": (C.c_int, [_P, C.POINTER(Mlp), _P, _P, C.c_int64]),
+    "bz_mlp_workspace_floats": (C.c_int64, [C.c_int64]),
+    "bz_mlp_grad": (C.c_int, [_P, C.POINTER(Mlp), _P, _P, C.c_int64, _P, _P, _P]),
+    "bz_mlp_adam": (C.c_int, [_P, C.POINTER(Adam), _P, _P, _P, _P, _P]),
+    "bz_mlp_round": (C.c_int, [_P, _P, _P]),
     "bz_graph_begin": (C.c_int, [_P]),
     "bz_graph_end": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "bz_graph_launch": (C.c_int, [_P, C.c_int32]),

@@ -12,7 +12,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libbz.so"
-SOURCES = ["bz_api.hip", "bz_push.hip", "bz_field.hip", "bz_sort.hip", "bz_swarm.hip"]
+SOURCES = ["bz_api.hip", "bz_push.hip", "bz_field.hip", "bz_sort.hip", "bz_swarm.hip",
+           "bz_mlp.hip"]
 ARCH = os.environ.get("BZ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -37,7 +37,8 @@ def c1() -> RunConfig:
                      dt=1.0 / (RF * 400), steps=20 * 400, gas_density_m3=3.22e21,
                      te_ev=2.0, density_cm3=1e9, rf_voltage=200.0, rf_freq=RF,
                      target_ppc=4000, resample_every=10, trigger_at=2.0, merge_at=2.0,
-                     bin_fractions=bin_fractions(3), bin_thresholds=bin_thresholds(3))
+                     bin_fractions=bin_fractions(3), bin_thresholds=bin_thresholds(3),
+                     ion_scheme=1)
 
 
 def c3() -> RunConfig:
@@ -51,7 +52,7 @@ def c4() -> RunConfig:
                      te_ev=2.0, density_cm3=1e9, rf_voltage=300.0, rf_freq=RF,
                      target_ppc=8000, resample_every=1, trigger_at=0.0, merge_at=1.5,
                      bin_fractions=bin_fractions(5), bin_thresholds=bin_thresholds(5),
-                     capacity_factor=1.5)
+                     capacity_factor=1.5, ion_scheme=1)
 
 
 WORKLOADS = {"c0": c0, "c1": c1, "c3": c3, "c4": c4}

@@ -102,10 +102,7 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   e.push_shape = 0;
   // smallest push tile is 256 particles
   e.status_words = capacity / 256 + 128;
-  const int64_t per_block = 3ll * (max_nodes < 1024 ? max_nodes : 1024);
-  e.partials_doubles = (int64_t)e.num_cu * 8 * per_block;
-  if (e.partials_doubles < 3ll * max_nodes * e.num_cu * 2)
-    e.partials_doubles = 3ll * max_nodes * e.num_cu * 2;
+  e.partials_doubles = push_partials_doubles(e.num_cu, max_nodes, max_keys);
   e.sort_ws_bytes = (int64_t)sort_workspace_bytes(capacity, max_keys);
   hipError_t st = hipMalloc((void**)&e.clock, sizeof(Clock));
   if (st == hipSuccess) st = hipMalloc((void**)&e.status, e.status_words * sizeof(uint64_t));
@@ -432,6 +429,43 @@ int bz_swarm_control(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out,
   if (in->vx == out->vx) return fail("in and out state must not alias");
   return launch_swarm_control(c->env, in, out, p, n_inst, inst_off, targets, counts_ws,
                               offsets_ws, pol);
+}
+
+int bz_mlp_forward(bz_ctx_t c, const bz_mlp_t* net, const float* x, float* y, int64_t n) {
+  BZ_CTX(c);
+  if (!net || !net->w) return fail("mlp forward: null network");
+  if (n < 0) return fail("mlp forward: negative size");
+  if (n > 0 && (!x || !y)) return fail("mlp forward: null pointer");
+  for (int i = 0; i < 3; ++i)
+    if (!(net->out_scale[i] != 0.0)) return fail("mlp: out_scale must be non-zero");
+  return launch_mlp_forward(c->env, net, x, y, n);
+}
+
+int64_t bz_mlp_workspace_floats(int64_t n) { return n > 0 ? mlp_workspace_floats(n) : 0; }
+
+int bz_mlp_grad(bz_ctx_t c, const bz_mlp_t* net, const float* x, const float* t, int64_t n,
+                float* ws, float* grad, float* loss) {
+  BZ_CTX(c);
+  if (!net || !net->w) return fail("mlp grad: null network");
+  if (n < 1) return fail("mlp grad: empty batch");
+  if (!x || !t || !ws || !grad) return fail("mlp grad: null pointer");
+  for (int i = 0; i < 3; ++i)
+    if (!(net->out_scale[i] != 0.0)) return fail("mlp: out_scale must be non-zero");
+  return launch_mlp_grad(c->env, net, x, t, n, ws, grad, loss);
+}
+
+int bz_mlp_adam(bz_ctx_t c, const bz_adam_t* opt, float* master, float* m, float* v,
+                const float* grad, uint16_t* w) {
+  BZ_CTX(c);
+  if (!opt || !master || !m || !v || !grad || !w) return fail("mlp adam: null pointer");
+  if (!(opt->bias1 > 0.f) || !(opt->bias2 > 0.f)) return fail("mlp adam: bias terms must be > 0");
+  return launch_mlp_adam(c->env, opt, master, m, v, grad, w);
+}
+
+int bz_mlp_round(bz_ctx_t c, const float* master, uint16_t* w) {
+  BZ_CTX(c);
+  if (!master || !w) return fail("mlp round: null pointer");
+  return launch_mlp_round(c->env, master, w);
 }
 
 int bz_swarm_eedf(bz_ctx_t c, const bz_state_t* s, int32_t n_inst, const int64_t* inst_off,

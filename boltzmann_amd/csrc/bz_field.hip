@@ -10,6 +10,91 @@ namespace bz {
 
 constexpr int kFieldBlock = 1024;
 
+// Bernoulli function B(x) = x / (e^x - 1), B(0) = 1.
+__device__ __forceinline__ double bernoulli(double x) {
+  if (fabs(x) < 1e-4) return 1.0 - 0.5 * x + x * x * (1.0 / 12.0);
+  return x / expm1(x);
+}
+
+// Ion / metastable transport with Scharfetter-Gummel fluxes
+//   G_{i+1/2} = a_i n_i - b_i n_{i+1},  a_i = (D/dx) B(-Pe_i),  b_i = (D/dx) B(Pe_i),
+//   Pe_i = mu E_i dx / D,
+// explicit in time, sub-cycled so that every sub-step keeps the update a
+// convex combination (h (a_i + b_{i-1}) / dx <= 1): positive and stable for
+// any field.  Wall nodes follow the reference's rule (heavies_kernel_1D.py:26-35)
+// restated for these fluxes: the wall face carries no inflow, i.e. its flux
+// is zero when E pushes ions into the domain (n_0 = n_1 exp(-Pe_0)) and pure
+// drift outflow otherwise (n_0 = n_1).  Metastables diffuse with the
+// reference's central flux and zero wall density, on the same sub-steps.
+// Uses LDS: a, b, Gi, Gs [N each] (the Poisson scratch, not yet in use).
+__device__ void sg_heavies(const bz_field_params_t& fp, int N, const double* E, double* ni,
+                           double* ns, double* lds) {
+  __shared__ double s_rate;
+  __shared__ int s_m;
+  const int tid = threadIdx.x;
+  const double dx = fp.dx, mu = fp.mu_i, Di = fp.D_i, Ds = fp.D_s;
+  double* a = lds;
+  double* b = lds + N;
+  double* Gi = lds + 2 * N;
+  double* Gs = lds + 3 * N;
+  const double dd = Di / dx;
+  for (int i = tid; i < N; i += kFieldBlock) {
+    const double pe = mu * E[i] * dx / Di;
+    a[i] = dd * bernoulli(-pe);
+    b[i] = dd * bernoulli(pe);
+  }
+  if (tid == 0) s_rate = 0.0;
+  __syncthreads();
+  // largest diagonal rate over interior nodes (max is order independent)
+  double r = 0.0;
+  for (int i = 1 + tid; i < N; i += kFieldBlock) r = fmax(r, (a[i] + b[i - 1]) / dx);
+  r = fmax(r, 2.0 * Ds / (dx * dx));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
+  if ((tid & 63) == 0) atomicMax((unsigned long long*)&s_rate, (unsigned long long)__double_as_longlong(r));
+  __syncthreads();
+  if (tid == 0) {
+    // non-negative doubles order like their bit patterns
+    const double rate = s_rate;
+    int m = (int)ceil(fp.dt * rate);
+    if (m < 1) m = 1;
+    if (fp.max_substeps > 0 && m > fp.max_substeps) m = fp.max_substeps;
+    s_m = m;
+  }
+  __syncthreads();
+  const int m = s_m;
+  const double h = fp.dt / m;
+  const double pe0 = mu * fmax(E[0], 0.0) * dx / Di;
+  const double peN = mu * fmin(E[N - 1], 0.0) * dx / Di;
+  const double f0 = exp(-pe0), fN = exp(peN);
+  for (int sub = 0; sub < m; ++sub) {
+    if (tid == 0) {
+      ni[0] = ni[1] * f0;
+      ni[N] = ni[N - 1] * fN;
+      ns[0] = 0.0;
+      ns[N] = 0.0;
+    }
+    __syncthreads();
+    for (int i = tid; i < N; i += kFieldBlock) {
+      Gi[i] = a[i] * ni[i] - b[i] * ni[i + 1];
+      Gs[i] = -Ds * (ns[i + 1] - ns[i]) / dx;
+    }
+    __syncthreads();
+    for (int i = 1 + tid; i < N; i += kFieldBlock) {
+      ni[i] = ni[i] - h * (Gi[i] - Gi[i - 1]) / dx;
+      ns[i] = ns[i] - h * (Gs[i] - Gs[i - 1]) / dx;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    ni[0] = ni[1] * f0;
+    ni[N] = ni[N - 1] * fN;
+    ns[0] = 0.0;
+    ns[N] = 0.0;
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
                                                        const double* __restrict__ mom,
                                                        double* ni, double* ns, double* Te,
@@ -45,7 +130,9 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
   if (advance) t = t + fp.dt;
   __syncthreads();
 
-  if (fp.fluid_on) {
+  if (fp.fluid_on && fp.ion_scheme == 1) {
+    sg_heavies(fp, N, E, ni, ns, s_f);
+  } else if (fp.fluid_on) {
     const double dx = fp.dx, dt = fp.dt, mu = fp.mu_i, Di = fp.D_i, Ds = fp.D_s;
     const double e0 = E[0] > 0.0 ? E[0] : 0.0;
     const double eN = E[N - 1] < 0.0 ? E[N - 1] : 0.0;

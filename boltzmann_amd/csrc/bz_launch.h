@@ -65,6 +65,9 @@ int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t
                     const uint32_t* perm, const int32_t* targets, int32_t n_keys,
                     const bz_resample_policy_t* pol);
 size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys);
+// per-block partial sums of the push: node histograms (max_nodes) or
+// per-instance moments (batched swarms, up to max_keys instances)
+int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys);
 
 }  // namespace bz
 
@@ -79,6 +82,14 @@ int launch_resample_grouped(const LaunchEnv& env, const bz_state_t* in, const bz
                             const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
                             const uint32_t* perm, const uint32_t* skeys, const int32_t* targets,
                             int32_t n_keys, const bz_resample_policy_t* pol, int32_t** plan_out);
+int64_t mlp_workspace_floats(int64_t n);
+int launch_mlp_forward(const LaunchEnv& env, const bz_mlp_t* net, const float* x, float* y,
+                       int64_t n);
+int launch_mlp_grad(const LaunchEnv& env, const bz_mlp_t* net, const float* x, const float* t,
+                    int64_t n, float* ws, float* grad, float* loss);
+int launch_mlp_adam(const LaunchEnv& env, const bz_adam_t* opt, float* master, float* m, float* v,
+                    const float* grad, uint16_t* wbf);
+int launch_mlp_round(const LaunchEnv& env, const float* master, uint16_t* wbf);
 void* sort_scratch_keys(const LaunchEnv& env);   // keys-per-position buffer
 void* sort_scratch_perm(const LaunchEnv& env);   // position -> index buffer
 }  // namespace bz

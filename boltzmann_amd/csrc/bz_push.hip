@@ -426,6 +426,22 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(bz_params_t p, const T* __res
 
 namespace bz {
 
+int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys) {
+  // up to 8 resident blocks per CU, each with its row of sums and 8 counters
+  // (4 doubles); the field step also keeps two flux arrays here
+  auto rows = [num_cu](int64_t row_doubles) {
+    // a block's LDS holds its row, so at most 160 KiB / row blocks fit a CU
+    int64_t per_cu = (160 * 1024) / (row_doubles * 8);
+    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+    return (int64_t)num_cu * per_cu * row_doubles;
+  };
+  const int64_t hist = rows(3ll * max_nodes + 4);
+  const int64_t inst = rows((int64_t)kImN * max_keys + 4);
+  int64_t d = hist > inst ? hist : inst;
+  if (d < 2ll * (max_nodes + 1)) d = 2ll * (max_nodes + 1);
+  return d;
+}
+
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
   auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS>;

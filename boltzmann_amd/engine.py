@@ -191,7 +191,10 @@ class KineticEngine:
     def resample(self, force: bool = False) -> bool:
         """Bin, sort and (when triggered) split / merge into the other buffer."""
         self.bin_sort()
-        if not force and not self.trigger(self.counts.cpu().numpy()):
+        always = not self.policy.ref_trigger and self.policy.trigger_at <= 0
+        # the counts are read back (a host synchronisation) only when the
+        # trigger can decline
+        if not force and not always and not self.trigger(self.counts.cpu().numpy()):
             return False
         src, dst = self.bufs[self.cur], self.bufs[1 - self.cur]
         _lib.check(self.lib.bz_resample(self.ctx, C.byref(src.struct), C.byref(dst.struct),

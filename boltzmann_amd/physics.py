@@ -96,6 +96,8 @@ class RunConfig:
     seed: int = 12345
     fluid_on: bool = True
     poisson_on: bool = True
+    ion_scheme: int = 0           # 0 reference central fluxes, 1 Scharfetter-Gummel sub-cycled
+    max_substeps: int = 256
     capacity_factor: float = 2.0
     xsec: XSecModel = field(default_factory=XSecModel)
 
@@ -272,6 +274,8 @@ def field_params(cfg: RunConfig) -> _lib.FieldParams:
     fp.n = cfg.n_cells
     fp.fluid_on = 1 if cfg.fluid_on else 0
     fp.poisson_on = 1 if cfg.poisson_on else 0
+    fp.ion_scheme = cfg.ion_scheme
+    fp.max_substeps = cfg.max_substeps
     return fp
 
 

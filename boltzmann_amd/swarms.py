@@ -168,19 +168,32 @@ class SwarmBatch:
     def control(self):
         self._control_call()
 
-    def capture(self, n_steps: int, accumulate: bool, with_control: bool = True) -> int:
-        """Capture n_steps steps (even, so the buffers return to the same
-        parity) plus one population control into a hipGraph."""
-        if n_steps % 2:
-            raise ValueError("capture an even number of steps")
+    def capture(self, n_steps: int, accumulate: bool, with_control: bool = True,
+                control_every: int = 0) -> int:
+        """Capture n_steps steps into a hipGraph.
+
+        Every step and every control swaps the ping-pong buffers, so the
+        captured sequence must swap an even number of times to leave the
+        state where the next replay expects it.  control_every > 0 puts a
+        population control after every control_every steps; otherwise
+        with_control appends two controls (the second one finds every
+        instance inside its band and only copies)."""
+        if control_every:
+            if n_steps % control_every:
+                raise ValueError("n_steps must be a multiple of control_every")
+            plan = ([0] * control_every + [1]) * (n_steps // control_every)
+        else:
+            plan = [0] * n_steps + ([1, 1] if with_control else [])
+        if len(plan) % 2:
+            raise ValueError("the captured sequence must swap the buffers an even number of times")
         cur0 = self.cur
         _lib.check(self.lib.bz_graph_begin(self.ctx))
         try:
-            for _ in range(n_steps):
-                self._step_call(accumulate)
-            if with_control:
-                self._control_call()
-                self._control_call()  # second control: back to the same buffer
+            for op in plan:
+                if op:
+                    self._control_call()
+                else:
+                    self._step_call(accumulate)
         finally:
             gid = C.c_int32(-1)
             _lib.check(self.lib.bz_graph_end(self.ctx, C.byref(gid)))

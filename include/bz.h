@@ -130,6 +130,12 @@ typedef struct {
   int32_t fluid_on; /* 0: heavies frozen (0-D test problems) */
   int32_t poisson_on;
   int32_t skip_moments; /* 1: leave ni / Te / g2 untouched (stand-alone solves) */
+  int32_t ion_scheme;   /* 0: the reference's explicit central fluxes, one step
+                           (heavies_kernel_1D.py:25-61); 1: Scharfetter-Gummel
+                           fluxes, sub-cycled to the explicit stability bound,
+                           with the zero-inflow wall condition of the reference
+                           restated for those fluxes */
+  int32_t max_substeps; /* scheme 1: cap on the sub-steps of one field step */
 } bz_field_params_t;
 
 /* Resampling policy (ES-WPNC merge / split). */
@@ -284,6 +290,48 @@ int bz_graph_begin(bz_ctx_t ctx);
 int bz_graph_end(bz_ctx_t ctx, int32_t* id);
 int bz_graph_launch(bz_ctx_t ctx, int32_t id);
 int bz_graph_destroy(bz_ctx_t ctx, int32_t id);
+
+/* ---- Learned closure (no counterpart in the reference) ----------------
+ * A perceptron 3 -> 64 -> 64 -> 64 -> 3 with SiLU hidden activations maps
+ * [log10 E/N (Td), mean energy (eV), log10 n_e (m^-3)] to
+ * [log10 k_ion (m^3/s), log10 k_exc (m^3/s), log10 mu*N (1/(V m s))].
+ * Parameters are flat, row-major W[out][in]:
+ *   W1[64][3] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[3][64] b4[3].
+ * Every pass reads BF16 weights (round-to-nearest-even copy of an FP32
+ * master) and accumulates in FP32.  Inputs / outputs are row-major [n][3]
+ * FP32 in physical units; the affine normalisation is part of the network. */
+#define BZ_MLP_IN 3
+#define BZ_MLP_HIDDEN 64
+#define BZ_MLP_OUT 3
+#define BZ_MLP_NPARAM 8771
+
+typedef struct {
+  const uint16_t* w;      /* BF16 parameters on the device, BZ_MLP_NPARAM */
+  double in_shift[3];     /* x_n = (x - in_shift) * in_scale */
+  double in_scale[3];
+  double out_shift[3];    /* y = y_n * out_scale + out_shift */
+  double out_scale[3];
+} bz_mlp_t;
+
+typedef struct {
+  float lr, beta1, beta2, eps;
+  float bias1, bias2;     /* 1 - beta1^t, 1 - beta2^t of this update */
+} bz_adam_t;
+
+/* y[n][3] = net(x[n][3]). */
+int bz_mlp_forward(bz_ctx_t ctx, const bz_mlp_t* net, const float* x, float* y, int64_t n);
+/* Floats of scratch that bz_mlp_grad needs for a batch of n samples. */
+int64_t bz_mlp_workspace_floats(int64_t n);
+/* Gradient of L = (1/2n) sum |y_n - t_n|^2 in normalised output units
+ * (t_n = (t - out_shift) / out_scale) w.r.t. the BF16 parameters, reduced
+ * in a fixed order (deterministic).  grad [BZ_MLP_NPARAM] FP32, loss [1]. */
+int bz_mlp_grad(bz_ctx_t ctx, const bz_mlp_t* net, const float* x, const float* t, int64_t n,
+                float* ws, float* grad, float* loss);
+/* Adam update of the FP32 master and its moments; refreshes the BF16 copy. */
+int bz_mlp_adam(bz_ctx_t ctx, const bz_adam_t* opt, float* master, float* m, float* v,
+                const float* grad, uint16_t* w);
+/* BF16 copy of an FP32 master (round to nearest even). */
+int bz_mlp_round(bz_ctx_t ctx, const float* master, uint16_t* w);
 
 /* Host helper: Philox4x32-10 block (for tests and stream generation). */
 void bz_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

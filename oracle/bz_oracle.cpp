@@ -722,10 +722,56 @@ int64_t orc_resample_f32(const orc_params* P, const orc_policy* pol, int nb, con
 // Grid step: moment update, heavies, Poisson (FP64).
 //   chol = [diag(N-1), lower(N-1)] of the banded Cholesky factor (upper form,
 //   as returned by scipy.linalg.cholesky_banded for boltzmann.py:744-755).
+// Bernoulli function B(x) = x / (e^x - 1) of the Scharfetter-Gummel flux.
+static double orc_bernoulli(double x) {
+  if (std::fabs(x) < 1e-4) return 1.0 - 0.5 * x + x * x * (1.0 / 12.0);
+  return x / std::expm1(x);
+}
+
+// Heavies with Scharfetter-Gummel fluxes, sub-cycled to the positivity bound
+// h (a_i + b_{i-1}) / dx <= 1; wall nodes carry no inflow (the reference's
+// wall rule of heavies_kernel_1D.py:26-35 restated for these fluxes).
+static void orc_sg_heavies(int N, double dx, double dt, double mu, double Di, double Ds,
+                           int max_sub, const double* E, double* ni, double* ns) {
+  std::vector<double> a(N), b(N), Gi(N), Gs(N);
+  const double dd = Di / dx;
+  for (int i = 0; i < N; ++i) {
+    const double pe = mu * E[i] * dx / Di;
+    a[i] = dd * orc_bernoulli(-pe);
+    b[i] = dd * orc_bernoulli(pe);
+  }
+  double r = 2.0 * Ds / (dx * dx);
+  for (int i = 1; i < N; ++i) r = std::max(r, (a[i] + b[i - 1]) / dx);
+  int m = (int)std::ceil(dt * r);
+  if (m < 1) m = 1;
+  if (max_sub > 0 && m > max_sub) m = max_sub;
+  const double h = dt / m;
+  const double f0 = std::exp(-(mu * std::max(E[0], 0.0) * dx / Di));
+  const double fN = std::exp(mu * std::min(E[N - 1], 0.0) * dx / Di);
+  auto bc = [&]() {
+    ni[0] = ni[1] * f0;
+    ni[N] = ni[N - 1] * fN;
+    ns[0] = 0.0;
+    ns[N] = 0.0;
+  };
+  for (int s = 0; s < m; ++s) {
+    bc();
+    for (int i = 0; i < N; ++i) {
+      Gi[i] = a[i] * ni[i] - b[i] * ni[i + 1];
+      Gs[i] = -Ds * (ns[i + 1] - ns[i]) / dx;
+    }
+    for (int i = 1; i < N; ++i) {
+      ni[i] = ni[i] - h * (Gi[i] - Gi[i - 1]) / dx;
+      ns[i] = ns[i] - h * (Gs[i] - Gs[i - 1]) / dx;
+    }
+  }
+  bc();
+}
+
 void orc_field_step(int N, double dx, double dt, double eps, double v0, double freq, double mu_i,
-                    double D_i, double D_s, int fluid_on, int poisson_on, double t,
-                    const double* mom, double* ni, double* ns, double* Te, double* g2, double* V,
-                    double* E, const double* chol) {
+                    double D_i, double D_s, int fluid_on, int poisson_on, int ion_scheme,
+                    int max_substeps, double t, const double* mom, double* ni, double* ns,
+                    double* Te, double* g2, double* V, double* E, const double* chol) {
   const int nn = N + 1;
   const double* ne = mom;
   const double* nsrc = mom + nn;
@@ -736,7 +782,9 @@ void orc_field_step(int N, double dx, double dt, double eps, double v0, double f
     Te[i] = (te != te) ? 0.0 : te;
     g2[i] = g2[i] + nsrc[i];
   }
-  if (fluid_on) {
+  if (fluid_on && ion_scheme == 1) {
+    orc_sg_heavies(N, dx, dt, mu_i, D_i, D_s, max_substeps, E, ni, ns);
+  } else if (fluid_on) {
     std::vector<double> Ji(N), Js(N);
     auto bc = [&]() {
       ni[0] = ni[1] * ((D_i / dx - 0.5 * mu_i * std::max(E[0], 0.0)) /

@@ -176,13 +176,14 @@ def deposit(setup: Setup, state):
 
 
 def field_step(N, dx, dt, eps, v0, freq, mu_i, D_i, D_s, fluid_on, poisson_on, t, mom, ni, ns,
-               Te, g2, V, E, chol):
-    """In-place FP64 grid step (arrays are modified)."""
+               Te, g2, V, E, chol, ion_scheme=0, max_substeps=0):
+    """FP64 grid step on copies of the arrays; returns (ni, ns, Te, g2, V, E)."""
     lib = load()
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (mom, ni, ns, Te, g2, V, E, chol)]
     lib.orc_field_step(C.c_int(N), *[C.c_double(v) for v in (dx, dt, eps, v0, freq, mu_i, D_i,
                                                                D_s)],
-                       C.c_int(fluid_on), C.c_int(poisson_on), C.c_double(t),
+                       C.c_int(fluid_on), C.c_int(poisson_on), C.c_int(ion_scheme),
+                       C.c_int(max_substeps), C.c_double(t),
                        *[_ptr(a) for a in arrs])
     return arrs[1:7]
 

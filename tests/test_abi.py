@@ -54,5 +54,7 @@ def test_struct_sizes_match_header_layout():
     # 19 doubles + u64 + 6 int32 + int64
     assert ctypes.sizeof(_lib.Params) == 19 * 8 + 8 + 6 * 4 + 8
     assert ctypes.sizeof(_lib.XSec) == 4 * 8 + 4 * 4
-    assert ctypes.sizeof(_lib.FieldParams) == 8 * 8 + 4 * 4
+    assert ctypes.sizeof(_lib.FieldParams) == 8 * 8 + 6 * 4
+    assert ctypes.sizeof(_lib.Mlp) == 8 + 12 * 8
+    assert ctypes.sizeof(_lib.Adam) == 6 * 4
     assert ctypes.sizeof(_lib.ResamplePolicy) == 6 * 8 + 4 * 4

@@ -325,3 +325,82 @@ def test_moment_update():
     assert out[2][0] == 0.0  # nan_to_num
     np.testing.assert_allclose(out[2][1:], 2.0)
     np.testing.assert_allclose(out[3], nsrc)
+
+
+def _heavies(N, dx, dt, mu, Di, Ds, ni, E, scheme, ns=None, max_sub=0):
+    out = O.field_step(N, dx, dt, 5.5263e5, 0.0, 1.0, mu, Di, Ds, 1, 0, 0.0,
+                       np.zeros(3 * (N + 1)), ni.copy(),
+                       np.zeros(N + 1) if ns is None else ns.copy(), np.zeros(N + 1),
+                       np.zeros(N + 1), np.zeros(N - 1), E.copy(), np.zeros(2 * (N - 1)),
+                       ion_scheme=scheme, max_substeps=max_sub)
+    return out[0], out[1]
+
+
+def test_sg_heavies_reduce_to_reference_without_field():
+    """E = 0: B(0) = 1, the Scharfetter-Gummel flux is the central diffusion
+    flux, and one sub-step suffices, so both schemes agree."""
+    N, dx = 50, 0.05
+    rng = np.random.default_rng(7)
+    ni = rng.uniform(0.5, 1.5, N + 1) * 1e9
+    ns = rng.uniform(0.5, 1.5, N + 1) * 1e8
+    E = np.zeros(N)
+    a, sa = _heavies(N, dx, 1e-9, 14.4, 0.64, 0.75, ni, E, 0, ns)
+    b, sb = _heavies(N, dx, 1e-9, 14.4, 0.64, 0.75, ni, E, 1, ns)
+    np.testing.assert_allclose(b, a, rtol=1e-13)
+    np.testing.assert_allclose(sb, sa, rtol=1e-13)
+
+
+def test_sg_heavies_positive_where_central_scheme_fails():
+    """Cell Peclet number 50: the reference's central update produces
+    negative densities; the sub-cycled Scharfetter-Gummel update stays
+    non-negative and conserves the interior mass when both walls push ions
+    inwards (zero wall flux)."""
+    N, dx, mu, Di = 64, 0.01, 1.44e4, 643.0
+    x = np.arange(N) / (N - 1)
+    E = 500.0 * (1.0 - 2.0 * x)                   # E[0] > 0, E[N-1] < 0
+    ni = np.full(N + 1, 1e9)
+    ni[:N // 4] = 1e6                               # a steep front facing the drift
+    dt = 2e-10
+    c, _ = _heavies(N, dx, dt, mu, Di, 1.0, ni, E, 0)
+    assert c[1:N].min() < 0
+    s, _ = _heavies(N, dx, dt, mu, Di, 1.0, ni, E, 1)
+    assert s.min() >= 0
+    m0 = ni[1:N].sum()
+    assert abs(s[1:N].sum() - m0) <= 1e-12 * m0
+
+
+def test_sg_heavies_drift_speed():
+    """Uniform field, Pe >> 1: a pulse drifts at mu E (upwind limit) and its
+    mass is conserved while it is far from the walls."""
+    N, dx, mu, Di, E0 = 400, 0.01, 1.44e4, 1.0, 50.0
+    xc = (np.arange(N + 1)) * dx
+    ni = np.exp(-((xc - 1.0) / 0.1) ** 2) * 1e9
+    E = np.full(N, E0)
+    dt = 0.2 * dx / (mu * E0)                       # 0.2 cells per step
+    n = ni.copy()
+    steps = 200
+    for _ in range(steps):
+        n, _ = _heavies(N, dx, dt, mu, Di, 1.0, n, E, 1)
+    shift = (np.sum(xc * n) / np.sum(n)) - (np.sum(xc * ni) / np.sum(ni))
+    assert abs(shift / (mu * E0 * dt * steps) - 1) < 1e-3
+    assert abs(n[1:N].sum() / ni[1:N].sum() - 1) < 1e-12
+
+
+def test_sg_heavies_substep_count():
+    """The number of sub-steps follows the explicit bound
+    h (a_i + b_{i-1}) / dx <= 1: a field step twice the bound equals two
+    manual half steps."""
+    N, dx, mu, Di = 32, 0.01, 1.44e4, 643.0
+    rng = np.random.default_rng(9)
+    E = rng.uniform(-300, 300, N)
+    ni = rng.uniform(0.5, 1.5, N + 1) * 1e9
+    Bf = lambda v: np.where(np.abs(v) < 1e-4, 1 - v / 2 + v * v / 12, v / np.expm1(v))
+    pe = mu * E * dx / Di
+    a = Di / dx * Bf(-pe)
+    b = Di / dx * Bf(pe)
+    rate = max(np.max((a[1:] + b[:-1]) / dx), 2 * 1.0 / dx ** 2)
+    dt = 2.0 / rate * 0.999                         # exactly two sub-steps
+    full, _ = _heavies(N, dx, dt, mu, Di, 1.0, ni, E, 1)
+    half, _ = _heavies(N, dx, dt / 2, mu, Di, 1.0, ni, E, 1)
+    half, _ = _heavies(N, dx, dt / 2, mu, Di, 1.0, half, E, 1)
+    np.testing.assert_allclose(full, half, rtol=1e-13)

@@ -210,19 +210,23 @@ def test_trajectory_0d(gpu, prec):
     eng.close()
 
 
-def test_field_parity(gpu):
+@pytest.mark.parametrize("N,scheme,emax", [(256, 0, 50.0), (256, 1, 800.0), (4096, 1, 2000.0),
+                                            (4096, 0, 5.0)])
+def test_field_parity(gpu, N, scheme, emax):
+    """Moment update, heavies (reference central fluxes or the sub-cycled
+    Scharfetter-Gummel fluxes) and the Poisson solve, up to 4096 cells."""
     import torch
     from boltzmann_amd import physics as ph
     from boltzmann_amd.engine import KineticEngine
-    cfg = _cfg(ph, 1, 8, 1, 100, n_cells=256, gas=3.22e21, dt=1.844e-10, rf_voltage=200.0)
+    cfg = _cfg(ph, 1, 8, 1, 100, n_cells=N, gas=3.22e21, dt=1.844e-10, rf_voltage=200.0,
+               ion_scheme=scheme)
     eng = KineticEngine(cfg, capacity=1024)
-    N = cfg.n_cells
     rng = np.random.default_rng(7)
     ne = rng.uniform(0.8e9, 1.2e9, N + 1)
     nsrc = rng.uniform(0, 1e4, N + 1)
     nrg = ne * rng.uniform(1, 5, N + 1)
     mom = np.concatenate([ne, nsrc, nrg])
-    E0 = rng.uniform(-50, 50, N)
+    E0 = rng.uniform(-emax, emax, N)
     eng.moments.copy_(torch.from_numpy(mom))
     eng.E.copy_(torch.from_numpy(E0))
     ni0 = eng.ni.cpu().numpy().copy()
@@ -239,7 +243,10 @@ def test_field_parity(gpu):
     chol_ref = np.stack([c[1], np.concatenate([[0.0], c[0, 1:]])])
     out = O.field_step(N, fp.dx, fp.dt, fp.eps, fp.v0, fp.freq, fp.mu_i, fp.D_i, fp.D_s, 1, 1,
                        3.3e-9 + cfg.dt, mom, ni0, np.zeros(N + 1), np.zeros(N + 1),
-                       np.zeros(N + 1), np.zeros(N - 1), E0.copy(), chol_ref)
+                       np.zeros(N + 1), np.zeros(N - 1), E0.copy(), chol_ref,
+                       ion_scheme=scheme, max_substeps=fp.max_substeps)
+    if scheme == 1:
+        assert out[0].min() >= 0
     names = ("ni", "ns", "Te", "g2", "V", "E")
     for nm, want in zip(names, out):
         got = getattr(eng, nm).cpu().numpy()
