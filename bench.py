@@ -276,6 +276,107 @@ def bench_swarms(args):
     return out
 
 
+def bench_closure(args):
+    """Closure-accelerated 1-D run (1024 cells, 4M electrons) against the
+    plain run: simulated time per wall-clock hour of each, and the closure's
+    ionization-rate error judged by the kinetic step at every update.  Both
+    runs use free-flight collisions, exact at either step length.  The
+    closure network is trained first, on the GPU, on batched-swarm samples."""
+    import numpy as np
+    import torch
+
+    from boltzmann_amd import _lib, configs
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.closure_run import ClosureDischarge, swarm_training_set, train_closure
+    from boltzmann_amd.engine import KineticEngine
+
+    t0 = time.perf_counter()
+    x, t = swarm_training_set(n_samples=args.closure_samples)
+    t_data = time.perf_counter() - t0
+    net, rep = train_closure(x, t, epochs=args.closure_epochs)
+    params = {"master": net.parameters(), "norm": dict(net.norm)}
+    net.close()
+    cfg = configs.c3().with_(xs_mode=_lib.BZ_XS_FLIGHT)
+    st = ph.initial_state(cfg)
+    n_sub = 8
+    steps = max(n_sub, (args.steps // n_sub) * n_sub)
+    warm = max(n_sub, (args.warmup // n_sub) * n_sub)
+    # plain: a kinetic step every dt
+    eng = KineticEngine(cfg)
+    eng.load_state(st)
+    eng.initialize_fields()
+    nn = cfg.n_nodes
+    for i in range(warm):
+        eng.step(i)
+    acc_ne = torch.zeros(1, dtype=torch.float64, device=eng.dev)
+    acc_nrg = torch.zeros(1, dtype=torch.float64, device=eng.dev)
+    acc_ion = torch.zeros(1, dtype=torch.float64, device=eng.dev)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    for i in range(warm, warm + steps):
+        eng.step(i)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    # moments of the plain run (diagnostic pass, outside the timed region)
+    for i in range(warm + steps, warm + 2 * steps):
+        eng.step(i)
+        m = eng.moments
+        acc_ne += m[:nn].sum()
+        acc_ion += m[nn:2 * nn].sum()
+        acc_nrg += m[2 * nn:].sum()
+    plain_s = tb - ta
+    plain = {"ms_per_step": plain_s / steps * 1e3,
+             "sim_seconds_per_hour": cfg.dt * steps / plain_s * 3600,
+             "mean_energy_ev": float(acc_nrg.item() / acc_ne.item()),
+             "ionization_per_step": float(acc_ion.item() / steps),
+             "particles_end": eng.count()}
+    eng.close()
+    # closure: a kinetic step every n_sub sub-steps
+    run = ClosureDischarge(cfg, params, n_sub=n_sub,
+                           resample_every=max(1, cfg.resample_every // n_sub))
+    run.load(st)
+    for _ in range(warm // n_sub):
+        run.macro_step()
+    torch.cuda.synchronize()
+    run.judge.zero_()
+    tc = time.perf_counter()
+    for _ in range(steps // n_sub):
+        run.macro_step()
+    torch.cuda.synchronize()
+    td = time.perf_counter()
+    for _ in range(steps // n_sub):
+        run.macro_step()
+    rep_c = run.report()
+    closure_s = td - tc
+    closure = {"ms_per_sim_step": closure_s / steps * 1e3,
+               "sim_seconds_per_hour": cfg.dt * steps / closure_s * 3600,
+               "particles_end": run.eng.count(), **rep_c,
+               "ionization_per_step": rep_c["ionization_kinetic"] / (2 * steps)}
+    run.close()
+    speedup = closure["sim_seconds_per_hour"] / plain["sim_seconds_per_hour"]
+    e_mean = abs(closure["mean_energy_ev"] / plain["mean_energy_ev"] - 1)
+    e_ion = abs(closure["ionization_per_step"] / plain["ionization_per_step"] - 1)
+    return {
+        "metric": "simulated time per hour, closure-accelerated vs plain 1-D run (1 MI355X)",
+        "value": closure["sim_seconds_per_hour"],
+        "unit": "simulated s / wall h",
+        "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": closure["ms_per_sim_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (uniform Maxwellian discharge; closure trained on synthetic swarms)",
+        "config": {"workload": cfg.name + "_free_flight", "cells": cfg.n_cells,
+                   "particles_start": cfg.n_particles, "dt_s": cfg.dt, "sub_steps": n_sub,
+                   "parallelism": "single GPU"},
+        "speedup_vs_plain": speedup,
+        "rate_error": {"closure_vs_kinetic_ionization": closure["ionization_rate_error"],
+                       "closure_vs_kinetic_per_update": closure["ionization_rate_error_per_update"],
+                       "run_vs_plain_ionization_per_step": e_ion,
+                       "run_vs_plain_mean_energy": e_mean},
+        "plain": plain, "closure": closure,
+        "training": {"data_seconds": t_data, **rep},
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,13 +388,20 @@ def main(argv=None) -> int:
     ap.add_argument("--shape", type=int, default=-1, help="push tile shape (-1: autotune)")
     ap.add_argument("--sched", type=int, default=0,
                     help="push tile schedule: 0 static + cooperative launch, 1 ticket counter")
+    ap.add_argument("--closure-samples", type=int, default=1_000_000)
+    ap.add_argument("--closure-epochs", type=int, default=30)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the particle "
               "decomposition across GPUs is a design note, not a measured path.",
               file=sys.stderr)
         return 2
-    out = bench_swarms(args) if args.workload == "c2" else bench_engine(args)
+    if args.workload == "c2":
+        out = bench_swarms(args)
+    elif args.workload == "c3":
+        out = bench_closure(args)
+    else:
+        out = bench_engine(args)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -11,7 +11,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libbz.so"
 
 BZ_F32, BZ_F64 = 4, 8
-BZ_XS_FIT, BZ_XS_TABLE = 0, 1
+BZ_XS_FIT, BZ_XS_TABLE, BZ_XS_FLIGHT = 0, 1, 2
 BZ_RNG_PHILOX, BZ_RNG_STREAM = 0, 1
 BZ_NCOUNTERS = 16
 COUNTER_NAMES = ("births", "absorbed", "elastic", "excitation", "ionization", "null",
@@ -119,6 +119,11 @@ SIGNATURES = {
     "bz_mlp_grad": (C.c_int, [_P, C.POINTER(Mlp), _P, _P, C.c_int64, _P, _P, _P]),
     "bz_mlp_adam": (C.c_int, [_P, C.POINTER(Adam), _P, _P, _P, _P, _P]),
     "bz_mlp_round": (C.c_int, [_P, _P, _P]),
+    "bz_closure_source": (C.c_int, [_P, C.POINTER(Mlp), C.c_int32, _P, _P, C.c_double,
+                                    C.c_double, _P, _P]),
+    "bz_field_substep": (C.c_int, [_P, C.POINTER(FieldParams), _P, _P, _P, _P, _P, _P, _P, _P,
+                                   _P, _P]),
+    "bz_closure_commit": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P]),
     "bz_graph_begin": (C.c_int, [_P]),
     "bz_graph_end": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "bz_graph_launch": (C.c_int, [_P, C.c_int32]),

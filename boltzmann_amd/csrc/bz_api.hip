@@ -288,7 +288,10 @@ static int check_params(const bz_ctx* c, const bz_params_t* p) {
     if (p->bnf < 1) return fail("bnf must be >= 1");
     if (!(p->dx > 0.0) || !(p->L > 0.0)) return fail("dx and L must be > 0");
   }
-  if (p->xs_mode != BZ_XS_FIT && p->xs_mode != BZ_XS_TABLE) return fail("bad xs_mode");
+  if (p->xs_mode != BZ_XS_FIT && p->xs_mode != BZ_XS_TABLE && p->xs_mode != BZ_XS_FLIGHT)
+    return fail("bad xs_mode");
+  if (p->xs_mode == BZ_XS_FLIGHT && !(p->nu_max > 0.0))
+    return fail("free-flight collisions need nu_max > 0");
   if (p->rng_mode != BZ_RNG_PHILOX && p->rng_mode != BZ_RNG_STREAM) return fail("bad rng_mode");
   return 0;
 }
@@ -302,7 +305,7 @@ int bz_push_mcc(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const v
   if (!in || !out || !moments) return fail("null state or moments pointer");
   if (in->x == out->x || in->w == out->w) return fail("in and out state must not alias");
   if (p->dim == 1 && !E) return fail("1-D push needs the field array E");
-  if (p->xs_mode == BZ_XS_TABLE) {
+  if (p->xs_mode != BZ_XS_FIT) {
     if (!xs || !xs->energy || !xs->sig || !xs->slope || !xs->hint || xs->n < 2 ||
         xs->n_hint < 1)
       return fail("table mode needs a cross-section table");
@@ -333,6 +336,39 @@ int bz_field_step(bz_ctx_t c, const bz_field_params_t* fp, const double* moments
   if (!moments || !ni || !ns || !Te || !g2 || !V || !E || !E_t || !chol)
     return fail("null field array");
   return launch_field(c->env, fp, moments, ni, ns, Te, g2, V, E, E_t, chol, advance_time);
+}
+
+int bz_field_substep(bz_ctx_t c, const bz_field_params_t* fp, double* moments,
+                     const double* src, double* ni, double* ns, double* Te, double* g2,
+                     double* V, double* E, void* E_t, const double* chol) {
+  BZ_CTX(c);
+  if (!fp) return fail("null field params");
+  if (fp->n < 3) return fail("field needs N >= 3 cells");
+  if (fp->n + 1 > c->env.max_nodes) return fail("N+1 exceeds context max_nodes");
+  if (!moments || !src || !ni || !ns || !Te || !g2 || !V || !E || !E_t || !chol)
+    return fail("null field array");
+  bz_field_params_t q = *fp;
+  q.skip_moments = 1;
+  return launch_field(c->env, &q, moments, ni, ns, Te, g2, V, E, E_t, chol, 1, src);
+}
+
+int bz_closure_source(bz_ctx_t c, const bz_mlp_t* net, int32_t n_cells, const double* E,
+                      const double* moments, double gas_density_m3, double dt, double* src,
+                      double* acc) {
+  BZ_CTX(c);
+  if (!net || !net->w) return fail("closure source: null network");
+  if (n_cells < 1) return fail("closure source: need cells");
+  if (!E || !moments || !src) return fail("closure source: null pointer");
+  if (!(gas_density_m3 > 0.0) || !(dt > 0.0)) return fail("closure source: need N_gas, dt > 0");
+  return launch_closure_source(c->env, net, n_cells, E, moments, gas_density_m3, dt, src, acc);
+}
+
+int bz_closure_commit(bz_ctx_t c, int32_t n_nodes, const double* moments, double* acc,
+                      double* ni, double* g2, double* Te, double* judge) {
+  BZ_CTX(c);
+  if (n_nodes < 1) return fail("closure commit: need nodes");
+  if (!moments || !acc || !ni || !g2 || !Te || !judge) return fail("closure commit: null pointer");
+  return launch_closure_commit(c->env, n_nodes, moments, acc, ni, g2, Te, judge);
 }
 
 int bz_bin_sort(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,

@@ -19,12 +19,10 @@ struct U4 {
   uint32_t a, b, c, d;
 };
 
-__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __umulhi(a, b);
-#else
-  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
-#endif
+// Full 32x32 -> 64 product: one v_mad_u64_u32 on the device instead of a
+// separate mul_hi / mul_lo pair.
+__host__ __device__ __forceinline__ uint64_t mulwide32(uint32_t a, uint32_t b) {
+  return (uint64_t)a * (uint64_t)b;
 }
 
 __host__ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -35,20 +33,26 @@ __host__ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
     }
-    const uint32_t hi0 = mulhi32(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    const uint32_t hi1 = mulhi32(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-    const uint32_t n0 = hi1 ^ c1 ^ k0;
-    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    const uint64_t p0 = mulwide32(0xD2511F53u, c0);
+    const uint64_t p1 = mulwide32(0xCD9E8D57u, c2);
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
     c0 = n0;
-    c1 = lo1;
+    c1 = (uint32_t)p1;
     c2 = n2;
-    c3 = lo0;
+    c3 = (uint32_t)p0;
   }
   return U4{c0, c1, c2, c3};
 }
 
 // Stream purposes (top byte of counter word 3).
-enum : uint32_t { kPurposeCollide = 0, kPurposeGas = 1, kPurposeResample = 2, kPurposeInit = 3 };
+enum : uint32_t {
+  kPurposeCollide = 0,
+  kPurposeGas = 1,
+  kPurposeResample = 2,
+  kPurposeInit = 3,
+  kPurposeFlight = 4
+};
 
 // counter = {index, block, step_lo, (step_hi << 8) | purpose}, key = seed.
 __host__ __device__ __forceinline__ U4 draw(uint64_t seed, uint32_t purpose, uint32_t index,

@@ -95,14 +95,16 @@ __device__ void sg_heavies(const bz_field_params_t& fp, int N, const double* E, 
   __syncthreads();
 }
 
+// mom is not __restrict__: a closure sub-step updates its ne through ne_rw
 __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
-                                                       const double* __restrict__ mom,
+                                                       const double* mom,
                                                        double* ni, double* ns, double* Te,
                                                        double* g2, double* V, double* E,
                                                        void* E_t, int prec,
                                                        const double* __restrict__ chol,
                                                        double* Ji, double* Js, Clock* clk,
-                                                       int advance) {
+                                                       int advance, const double* src,
+                                                       double* ne_rw) {
   extern __shared__ double s_f[];  // rhs, tempy, 1/diag, lower [N-1 each], scan maps
   const int N = fp.n;
   const int nn = N + 1;
@@ -123,6 +125,15 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
       if (te != te) te = 0.0;  // nan_to_num
       Te[i] = te;
       g2[i] = g2[i] + nsrc[i];
+    }
+  }
+  if (src) {
+    // closure sub-step: ionization from the learned rate makes ion-electron pairs
+    for (int i = tid; i < nn; i += kFieldBlock) {
+      const double q = src[i];
+      ni[i] = ni[i] + q;
+      g2[i] = g2[i] + q;
+      ne_rw[i] = ne_rw[i] + q;
     }
   }
   __syncthreads();
@@ -288,7 +299,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
 
 int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double* moments,
                  double* ni, double* ns, double* Te, double* g2, double* V, double* E,
-                 void* E_t, const double* chol, int advance) {
+                 void* E_t, const double* chol, int advance, const double* src) {
   const int N = fp->n;
   // rhs, y, 1/diag, lower band (N-1 each) + scan maps (2 per thread)
   const size_t smem = ((size_t)4 * (N - 1) + 2 * kFieldBlock) * sizeof(double);
@@ -303,8 +314,73 @@ int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double
   double* Ji = env.partials;
   double* Js = env.partials + (N + 1);
   hipLaunchKernelGGL(k_field, dim3(1), dim3(kFieldBlock), smem, env.stream, *fp, moments, ni, ns,
-                     Te, g2, V, E, E_t, env.precision, chol, Ji, Js, env.clock, advance);
+                     Te, g2, V, E, E_t, env.precision, chol, Ji, Js, env.clock, advance, src,
+                     const_cast<double*>(moments));
   return check_launch("k_field");
+}
+
+}  // namespace bz
+
+// ---------------------------------------------------------------------------
+// Kinetic update of a closure-driven run: the sub-steps since the last update
+// added the learned ionization acc to ni; the kinetic step counted nsrc over
+// the same interval.  Replace one by the other (ni += nsrc - acc, g2 alike),
+// refresh Te = 2/3 nrg/ne, and add the judge's sums:
+//   judge[0] += sum nsrc, judge[1] += sum acc, judge[2] += |sum nsrc - sum acc|,
+//   judge[3] += 1, judge[4] += sum ne, judge[5] += sum nrg.
+// Sums run in a fixed order (deterministic).  acc is cleared.
+namespace bz {
+
+__global__ __launch_bounds__(1024) void k_closure_commit(int nn, const double* __restrict__ mom,
+                                                         double* __restrict__ acc,
+                                                         double* __restrict__ ni,
+                                                         double* __restrict__ g2,
+                                                         double* __restrict__ Te,
+                                                         double* __restrict__ judge) {
+  __shared__ double s_red[4][16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double* ne = mom;
+  const double* nsrc = mom + nn;
+  const double* nrg = mom + 2 * nn;
+  double s_k = 0.0, s_a = 0.0, s_n = 0.0, s_e = 0.0;
+  for (int i = tid; i < nn; i += 1024) {
+    const double d = nsrc[i] - acc[i];
+    ni[i] = ni[i] + d;
+    g2[i] = g2[i] + d;
+    double te = ((2.0 / 3.0) * nrg[i]) / ne[i];
+    if (te != te) te = 0.0;
+    Te[i] = te;
+    s_k += nsrc[i];
+    s_a += acc[i];
+    s_n += ne[i];
+    s_e += nrg[i];
+    acc[i] = 0.0;
+  }
+  double v[4] = {s_k, s_a, s_n, s_e};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double r = wave_sum(v[q]);
+    if (lane == 0) s_red[q][wid] = r;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < 4; ++q)
+      for (int w = 0; w < 16; ++w) t[q] += s_red[q][w];
+    judge[0] += t[0];
+    judge[1] += t[1];
+    judge[2] += fabs(t[0] - t[1]);
+    judge[3] += 1.0;
+    judge[4] += t[2];
+    judge[5] += t[3];
+  }
+}
+
+int launch_closure_commit(const LaunchEnv& env, int nn, const double* mom, double* acc, double* ni,
+                          double* g2, double* Te, double* judge) {
+  hipLaunchKernelGGL(k_closure_commit, dim3(1), dim3(1024), 0, env.stream, nn, mom, acc, ni, g2,
+                     Te, judge);
+  return check_launch("k_closure_commit");
 }
 
 }  // namespace bz

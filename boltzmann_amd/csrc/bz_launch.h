@@ -56,7 +56,11 @@ int launch_reduction(const LaunchEnv& env, int r, int M, int RM, double* dw, con
                      const double* dx, int64_t ptcl);
 int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double* moments,
                  double* ni, double* ns, double* Te, double* g2, double* V, double* E,
-                 void* E_t, const double* chol, int advance);
+                 void* E_t, const double* chol, int advance, const double* src = nullptr);
+int launch_closure_source(const LaunchEnv& env, const bz_mlp_t* net, int N, const double* E,
+                          const double* mom, double gas_m3, double dt, double* src, double* acc);
+int launch_closure_commit(const LaunchEnv& env, int nn, const double* mom, double* acc, double* ni,
+                          double* g2, double* Te, double* judge);
 int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                     const double* thr, int nb, int32_t* counts, int32_t* offsets,
                     uint32_t* perm);

@@ -49,6 +49,8 @@ __device__ __forceinline__ double dlog(double v) { return log(v); }
 __device__ __forceinline__ float dlog(float v) { return logf(v); }
 __device__ __forceinline__ double dsqrt(double v) { return sqrt(v); }
 __device__ __forceinline__ float dsqrt(float v) { return sqrtf(v); }
+__device__ __forceinline__ double dlog1p(double v) { return log1p(v); }
+__device__ __forceinline__ float dlog1p(float v) { return log1pf(v); }
 __device__ __forceinline__ void dsincos(double v, double* s, double* c) { sincos(v, s, c); }
 __device__ __forceinline__ void dsincos(float v, float* s, float* c) { sincosf(v, s, c); }
 
@@ -103,10 +105,25 @@ __device__ __forceinline__ int xs_bucket<float>(float e, int bits) {
   return (int)(__float_as_uint(e) >> (23 - bits));
 }
 
+// Cumulative collision probabilities r_j = sum_{c<=j} (sigma_c N v) * scale of
+// the tabulated model, sigma linear in E between table nodes.
+template <typename T>
+__device__ __forceinline__ void table_rates(T e, T vmag, const bz_params_t& p,
+                                            const bz_xsec_t& xs, T scale, T& r0, T& r1, T& r2,
+                                            bool& over);
+
 template <typename T>
 __device__ __forceinline__ void table_thresholds(T e, T vmag, const bz_params_t& p,
                                                  const bz_xsec_t& xs, T& r0, T& r1, T& r2,
                                                  bool& over) {
+  table_rates<T>(e, vmag, p, xs, (T)p.pc, r0, r1, r2, over);
+  over = r2 > (T)p.p_null;
+}
+
+template <typename T>
+__device__ __forceinline__ void table_rates(T e, T vmag, const bz_params_t& p,
+                                            const bz_xsec_t& xs, T scale, T& r0, T& r1, T& r2,
+                                            bool& over) {
   const T* E = (const T*)xs.energy;
   const T* S = (const T*)xs.sig;
   const T* D = (const T*)xs.slope;
@@ -127,11 +144,10 @@ __device__ __forceinline__ void table_thresholds(T e, T vmag, const bz_params_t&
   T s2 = S[2 * n + j] + D[2 * n + j] * d;
   if (e < (T)p.e_exc) s1 = (T)0;
   if (e < (T)p.e_ion) s2 = (T)0;
-  const T pc = (T)p.pc;
-  r0 = (s0 * vmag) * pc;
-  r1 = r0 + (s1 * vmag) * pc;
-  r2 = r1 + (s2 * vmag) * pc;
-  over = r2 > (T)p.p_null;
+  r0 = (s0 * vmag) * scale;
+  r1 = r0 + (s1 * vmag) * scale;
+  r2 = r1 + (s2 * vmag) * scale;
+  over = r2 > (T)1;
 }
 
 template <typename T>
@@ -188,8 +204,9 @@ struct StepOut {
   T e_sec;      // secondary energy, eV
   int alive;
   int born;
-  int chan;
+  int chan;     // kChanIon when a secondary was born, else the last event's channel
   int over;
+  int n_el, n_exc, n_null;  // events of the step (several in free-flight mode)
 };
 
 // One particle through collision + push + walls
@@ -204,6 +221,7 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
   o.born = 0;
   o.chan = kChanNone;
   o.over = 0;
+  o.n_el = o.n_exc = o.n_null = 0;
   o.e_post = (T)0;
   o.e_sec = (T)0;
   o.s = in;
@@ -248,13 +266,15 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
 
   const T R0 = U.u[0];
   T tx = (T)0, ty = (T)0, tz = (T)0;
+  // FP32 takes u[1], u[2] from the first Philox block, FP64 from the second
+  constexpr bool kAnglesInSecond = sizeof(T) == 8;
   if (R0 < r0) {
     o.chan = kChanElastic;
-    if (RNG != BZ_RNG_STREAM) U.second(p.seed, i, step);
+    if (RNG != BZ_RNG_STREAM && kAnglesInSecond) U.second(p.seed, i, step);
     iso_dir<T>(vmag * (T)p.fel, U.u[1], U.u[2], vx, vy, vz);
   } else if (XS == BZ_XS_TABLE && R0 < r1) {
     o.chan = kChanExc;
-    if (RNG != BZ_RNG_STREAM) U.second(p.seed, i, step);
+    if (RNG != BZ_RNG_STREAM && kAnglesInSecond) U.second(p.seed, i, step);
     const T enew = e - (T)p.e_exc;
     iso_dir<T>(dsqrt(((T)2 * enew) / (T)p.jm), U.u[1], U.u[2], vx, vy, vz);
   } else if ((XS == BZ_XS_TABLE && R0 < r2) || (XS == BZ_XS_FIT && r0 < R0 && R0 < r2)) {
@@ -276,6 +296,9 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
     o.e_sec = (T)p.kev * (tx * tx + ty * ty + tz * tz);
   }
 
+  o.n_el = o.chan == kChanElastic;
+  o.n_exc = o.chan == kChanExc;
+  o.n_null = o.chan == kChanNone;
   T x = in.x;
   T ef;
   if (DIM == 1) {
@@ -289,6 +312,161 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
   vx = vx - (T)p.afac * ((T)100 * ef);
   if (DIM == 1) o.alive = ((T)0 < x && x < (T)p.L) ? 1 : 0;
   else o.alive = 1;
+  o.p.x = x;
+  o.p.vx = vx;
+  o.p.vy = vy;
+  o.p.vz = vz;
+  o.p.w = in.w;
+  o.e_post = (T)p.kev * (vx * vx + vy * vy + vz * vz);
+  return o;
+}
+
+// Uniforms of free-flight event k of particle i: flight, channel and the two
+// scattering angles; the secondary's two angles come from a further block.
+template <typename T>
+struct FlightDraws;
+template <>
+struct FlightDraws<double> {
+  __device__ __forceinline__ static void event(uint64_t seed, uint32_t i, uint64_t step,
+                                               uint32_t k, double u[4]) {
+    const U4 r = draw(seed, kPurposeFlight, i, 3 * k, step);
+    const U4 q = draw(seed, kPurposeFlight, i, 3 * k + 1, step);
+    u[0] = u53(r.a, r.b);
+    u[1] = u53(r.c, r.d);
+    u[2] = u53(q.a, q.b);
+    u[3] = u53(q.c, q.d);
+  }
+  __device__ __forceinline__ static void secondary(uint64_t seed, uint32_t i, uint64_t step,
+                                                   uint32_t k, double u[2]) {
+    const U4 r = draw(seed, kPurposeFlight, i, 3 * k + 2, step);
+    u[0] = u53(r.a, r.b);
+    u[1] = u53(r.c, r.d);
+  }
+};
+template <>
+struct FlightDraws<float> {
+  __device__ __forceinline__ static void event(uint64_t seed, uint32_t i, uint64_t step,
+                                               uint32_t k, float u[4]) {
+    const U4 r = draw(seed, kPurposeFlight, i, 2 * k, step);
+    u[0] = u24(r.a);
+    u[1] = u24(r.b);
+    u[2] = u24(r.c);
+    u[3] = u24(r.d);
+  }
+  __device__ __forceinline__ static void secondary(uint64_t seed, uint32_t i, uint64_t step,
+                                                   uint32_t k, float u[2]) {
+    const U4 r = draw(seed, kPurposeFlight, i, 2 * k + 1, step);
+    u[0] = u24(r.a);
+    u[1] = u24(r.b);
+  }
+};
+
+constexpr int kMaxFlights = 256;
+
+// One particle through a step of exact free-flight null-collision MC: flight
+// times are drawn from exp(nu_max); at the end of each flight a real
+// collision of channel j happens with probability nu_j(e) / nu_max, so any
+// number of collisions fits in one step and the step length is not limited
+// by the collision frequency.  The field is the cell-constant E of the
+// particle's starting cell for the whole step (as in the single-collision
+// step).  Each flight moves x with the velocity at its start and then
+// applies the acceleration; a particle leaving the gap at a flight's end is
+// absorbed.  One secondary at most per step: a further ionizing event in
+// the same step is treated as a null event.  The secondary starts at the
+// collision point and is not moved in its birth step.  Tabulated sigma and
+// the Philox stream only.
+template <typename T, int DIM>
+__device__ __forceinline__ StepOut<T> flight_particle(const P5<T> in, uint32_t i, uint64_t step,
+                                                      const bz_params_t& p, const bz_xsec_t& xs,
+                                                      const T* __restrict__ E, T e_local) {
+  StepOut<T> o;
+  o.alive = 0;
+  o.born = 0;
+  o.chan = kChanNone;
+  o.over = 0;
+  o.n_el = o.n_exc = o.n_null = 0;
+  o.e_post = (T)0;
+  o.e_sec = (T)0;
+  o.s = in;
+  if (!(in.w > (T)0)) {
+    o.p = in;
+    return o;
+  }
+  T ef;
+  if (DIM == 1) {
+    int c = (int)((double)in.x / p.dx);
+    c = c < 0 ? 0 : (c >= p.n_cells ? p.n_cells - 1 : c);
+    ef = E[c];
+  } else {
+    ef = e_local;
+  }
+  const T acc = (T)(p.afac / p.dt) * ((T)100 * ef);  // dv_x/dt, m/s^2
+  const T inv_nu = (T)(1.0 / p.nu_max);
+  T x = in.x, vx = in.vx, vy = in.vy, vz = in.vz;
+  T tau = (T)p.dt;
+  int alive = 1;
+  for (int k = 0; k < kMaxFlights; ++k) {
+    T u[4];
+    FlightDraws<T>::event(p.seed, i, step, (uint32_t)k, u);
+    const T tf = -dlog1p(-u[0]) * inv_nu;
+    const bool hit = tf < tau;
+    const T h = hit ? tf : tau;
+    if (DIM == 1) x = x + ((T)100 * h) * vx;
+    vx = vx - acc * h;
+    if (DIM == 1 && !((T)0 < x && x < (T)p.L)) {
+      alive = 0;
+      break;
+    }
+    if (!hit) {
+      tau = (T)0;
+      break;
+    }
+    tau = tau - tf;
+    // collision at the end of the flight
+    const T s = vx * vx + vy * vy + vz * vz;
+    const T vmag = dsqrt(s);
+    const T e = (T)p.kev * s;
+    T r0, r1, r2;
+    bool over = false;
+    table_rates<T>(e, vmag, p, xs, inv_nu, r0, r1, r2, over);
+    o.over |= over ? 1 : 0;
+    const T R = u[1];
+    if (R < r0) {
+      ++o.n_el;
+      o.chan = kChanElastic;
+      iso_dir<T>(vmag * (T)p.fel, u[2], u[3], vx, vy, vz);
+    } else if (R < r1) {
+      ++o.n_exc;
+      o.chan = kChanExc;
+      const T enew = e - (T)p.e_exc;
+      iso_dir<T>(dsqrt(((T)2 * enew) / (T)p.jm), u[2], u[3], vx, vy, vz);
+    } else if (R < r2 && !o.born) {
+      const T eej = fabs((T)0.5 * (e - (T)p.e_ion));
+      const T vnew = dsqrt(((T)2 * eej) / (T)p.jm);
+      iso_dir<T>(vnew, u[2], u[3], vx, vy, vz);
+      T q[2], tx, ty, tz;
+      FlightDraws<T>::secondary(p.seed, i, step, (uint32_t)k, q);
+      iso_dir<T>(vnew, q[0], q[1], tx, ty, tz);
+      o.born = 1;
+      o.s.x = x;
+      o.s.vx = tx;
+      o.s.vy = ty;
+      o.s.vz = tz;
+      o.s.w = in.w;
+      o.e_sec = (T)p.kev * (tx * tx + ty * ty + tz * tz);
+    } else {
+      ++o.n_null;
+    }
+  }
+  if (alive && tau > (T)0) {
+    // flight budget exhausted (nu_max dt far above kMaxFlights): rest ballistic
+    if (DIM == 1) x = x + ((T)100 * tau) * vx;
+    vx = vx - acc * tau;
+    if (DIM == 1 && !((T)0 < x && x < (T)p.L)) alive = 0;
+    o.over = 1;
+  }
+  if (o.born) o.chan = kChanIon;
+  o.alive = alive;
   o.p.x = x;
   o.p.vx = vx;
   o.p.vy = vy;

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   if (tid < 8) s_cnt[tid] = 0ull;
   // stage the cross-section table in LDS (read-only for the whole launch)
   bz_xsec_t xs = a.xs;
-  if (XS == BZ_XS_TABLE) {
+  if (XS != BZ_XS_FIT) {
     const int n = a.xs.n;
     T* s_tab = (T*)(s_dep + dep_words);
     int32_t* s_hint = (int32_t*)(s_tab + 7 * n);
@@ -184,12 +184,15 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
         inst = lo;
         e_local = s_ie[inst];
       }
-      o[k] = step_particle<T, XS, DIM, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, e_local);
+      if (XS == BZ_XS_FLIGHT)
+        o[k] = flight_particle<T, DIM>(cur[k], (uint32_t)i, step, p, xs, E, e_local);
+      else
+        o[k] = step_particle<T, XS, DIM, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, e_local);
       if (i < n && cur[k].w > (T)0) {
-        c_el += o[k].chan == kChanElastic;
-        c_ex += o[k].chan == kChanExc;
+        c_el += o[k].n_el;
+        c_ex += o[k].n_exc;
         c_io += o[k].chan == kChanIon;
-        c_null += o[k].chan == kChanNone;
+        c_null += o[k].n_null;
         c_abs += !o[k].alive;
         c_over += o[k].over;
       }
@@ -450,7 +453,7 @@ static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
                                    : 3 * (DIM == 1 ? a.p.n_nodes : 1);
   const int part_doubles = DIM == 2 ? a.n_inst * kImN : 3 * (DIM == 1 ? a.p.n_nodes : 1);
   size_t smem = (size_t)dep_doubles * sizeof(double);
-  if (XS == BZ_XS_TABLE) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
+  if (XS != BZ_XS_FIT) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
   if (smem > 160 * 1024) return fail("push: deposit + table need %zu B of LDS (> 160 KiB)", smem);
   if (smem > 64 * 1024) {
     if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -501,6 +504,11 @@ static int launch_push_prec(const LaunchEnv& env, PushArgs& a, int shape, int* g
     if (p.xs_mode != BZ_XS_TABLE || stream)
       return fail("batched swarms use the tabulated model and the Philox stream");
     return launch_push_shape<T, BZ_XS_TABLE, 2, BZ_RNG_PHILOX>(env, a, shape, grid);
+  }
+  if (p.xs_mode == BZ_XS_FLIGHT) {
+    if (stream) return fail("free-flight collisions draw from the Philox stream only");
+    return p.dim == 1 ? launch_push_shape<T, BZ_XS_FLIGHT, 1, BZ_RNG_PHILOX>(env, a, shape, grid)
+                      : launch_push_shape<T, BZ_XS_FLIGHT, 0, BZ_RNG_PHILOX>(env, a, shape, grid);
   }
   if (p.xs_mode == BZ_XS_TABLE) {
     if (p.dim == 1)

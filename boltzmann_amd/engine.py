@@ -63,7 +63,7 @@ class KineticEngine:
         # collision model
         self.table = None
         self.xs = None
-        if cfg.xs_mode == _lib.BZ_XS_TABLE:
+        if cfg.xs_mode != _lib.BZ_XS_FIT:
             self.table = ph.build_table(cfg.xsec, cfg.gas_density_m3, cfg.np_dtype)
             t = self.table
             self._tab_dev = [torch.from_numpy(a).to(self.dev) for a in

@@ -154,7 +154,7 @@ def push_params(cfg: RunConfig, rng_mode: int = _lib.BZ_RNG_PHILOX,
     p.L_dx = cfg.length_cm + p.dx
     p.dep_M = float(cfg.bnf * (n + 1))
     p.nn = cfg.gas_density_m3
-    p.e_ion = cfg.xsec.e_ion if cfg.xs_mode == _lib.BZ_XS_TABLE else E_ION
+    p.e_ion = cfg.xsec.e_ion if cfg.xs_mode != _lib.BZ_XS_FIT else E_ION
     p.e_exc = cfg.xsec.e_exc
     if table is not None:
         p.nu_max = table.nu_max

@@ -36,7 +36,9 @@ enum { BZ_F32 = 4, BZ_F64 = 8 };
 enum {
   BZ_XS_FIT = 0,   /* reference argon fits, direct P = 1-exp(-nu dt)
                       (kernels/electron_kernel_numba.py:66-195) */
-  BZ_XS_TABLE = 1  /* tabulated sigma(E), 3 channels, null collision */
+  BZ_XS_TABLE = 1, /* tabulated sigma(E), 3 channels, null collision */
+  BZ_XS_FLIGHT = 2 /* tabulated sigma(E), exact free-flight null collision:
+                      any number of collisions per step (Philox stream only) */
 };
 /* random stream */
 enum {
@@ -332,6 +334,32 @@ int bz_mlp_adam(bz_ctx_t ctx, const bz_adam_t* opt, float* master, float* m, flo
                 const float* grad, uint16_t* w);
 /* BF16 copy of an FP32 master (round to nearest even). */
 int bz_mlp_round(bz_ctx_t ctx, const float* master, uint16_t* w);
+
+/* Closure source: ionization per node produced in one sub-step of length dt
+ * by the learned rate, src_i = k_ion(E/N_i, <e>_i, n_e,i) N_gas ne_i dt in
+ * the moments' density units.  E/N_i uses the node field (mean |E| of the
+ * adjacent cells, E in V/cm), <e>_i = nrg_i / ne_i and n_e,i = 1e6 ne_i
+ * (cm^-3 -> m^-3), all from moments [3][N+1] (the kinetic deposit layout).
+ * src [N+1] is written; acc [N+1] (optional) accumulates it. */
+int bz_closure_source(bz_ctx_t ctx, const bz_mlp_t* net, int32_t n_cells, const double* E,
+                      const double* moments, double gas_density_m3, double dt, double* src,
+                      double* acc);
+/* Field sub-step driven by a closure source: ni += src, g2 += src and the
+ * electron density ne (moments row 0) += src (pair creation), then the
+ * heavies and the Poisson solve of bz_field_step, t += dt.  The kinetic
+ * moment update is not applied. */
+int bz_field_substep(bz_ctx_t ctx, const bz_field_params_t* fp, double* moments,
+                     const double* src, double* ni, double* ns, double* Te, double* g2,
+                     double* V, double* E, void* E_t, const double* chol);
+
+/* Kinetic update of a closure-driven run.  The sub-steps since the last
+ * update added the learned ionization acc [N+1] to ni; the kinetic step
+ * counted nsrc (moments row 1) over the same interval: ni += nsrc - acc,
+ * g2 alike, Te = 2/3 nrg/ne, and the judge's running sums (FP64, fixed
+ * order) judge[0..5] += sum nsrc, sum acc, |sum nsrc - sum acc|, 1, sum ne,
+ * sum nrg.  acc is cleared. */
+int bz_closure_commit(bz_ctx_t ctx, int32_t n_nodes, const double* moments, double* acc,
+                      double* ni, double* g2, double* Te, double* judge);
 
 /* Host helper: Philox4x32-10 block (for tests and stream generation). */
 void bz_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

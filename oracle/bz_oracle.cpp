@@ -110,8 +110,11 @@ double orc_resample_uniform(uint64_t seed, uint64_t step, uint32_t key, uint32_t
 // Parameters (filled by oracle/oracle.py from the physical inputs)
 // ---------------------------------------------------------------------------
 typedef struct {
-  double dt, xfac, afac, kev, jm, fel, L, dx, half_dx, L_dx, dep_M, nn, e_ion, e_exc, pc, e_field0;
+  double dt, xfac, afac, kev, jm, fel, L, dx, half_dx, L_dx, dep_M, nn, e_ion, e_exc, pc, e_field0,
+      nu_max;
   int32_t dim, xs_mode, n_cells, n_nodes, bnf, nthreads;
+  uint64_t seed;  // Philox key of the free-flight draws (xs_mode 2)
+  uint64_t step;  // step number keying those draws
 } orc_params;
 
 typedef struct {
@@ -332,6 +335,142 @@ int one_particle(int64_t i, const orc_params& P, const T* x, const T* vx, const 
   return k;
 }
 
+inline float m_log1p(float v) { return log1pf(v); }
+inline double m_log1p(double v) { return std::log1p(v); }
+
+// Free-flight draws: event k of particle i uses Philox blocks 3k, 3k+1 (FP64)
+// or 2k (FP32) of purpose 4; the secondary's angles come from block 3k+2 /
+// 2k+1.
+void flight_event(uint64_t seed, uint32_t i, uint64_t step, uint32_t k, double u[4]) {
+  uint32_t r[4], q[4];
+  orc_draw(seed, 4, i, 3 * k, step, r);
+  orc_draw(seed, 4, i, 3 * k + 1, step, q);
+  u[0] = orc_u53(r[0], r[1]);
+  u[1] = orc_u53(r[2], r[3]);
+  u[2] = orc_u53(q[0], q[1]);
+  u[3] = orc_u53(q[2], q[3]);
+}
+void flight_event(uint64_t seed, uint32_t i, uint64_t step, uint32_t k, float u[4]) {
+  uint32_t r[4];
+  orc_draw(seed, 4, i, 2 * k, step, r);
+  for (int j = 0; j < 4; ++j) u[j] = orc_u24(r[j]);
+}
+void flight_secondary(uint64_t seed, uint32_t i, uint64_t step, uint32_t k, double u[2]) {
+  uint32_t r[4];
+  orc_draw(seed, 4, i, 3 * k + 2, step, r);
+  u[0] = orc_u53(r[0], r[1]);
+  u[1] = orc_u53(r[2], r[3]);
+}
+void flight_secondary(uint64_t seed, uint32_t i, uint64_t step, uint32_t k, float u[2]) {
+  uint32_t r[4];
+  orc_draw(seed, 4, i, 2 * k + 1, step, r);
+  u[0] = orc_u24(r[0]);
+  u[1] = orc_u24(r[1]);
+}
+
+struct Events {
+  int el, exc, ion, null_, alive;
+};
+
+// Exact free-flight null collision over one step (any number of events):
+// flights ~ exp(nu_max); at a flight's end channel j with probability
+// nu_j(e)/nu_max; E of the starting cell for the whole step; x moves with
+// the flight's initial velocity, then the acceleration applies; absorbed
+// when a flight ends outside the gap; one secondary per step at most
+// (further ionizing events are null), born at the collision point and not
+// moved in its birth step.
+template <typename T>
+int one_particle_flight(int64_t i, uint64_t step, const orc_params& P, const T* x, const T* vx,
+                        const T* vy, const T* vz, const T* w, const T* E, const orc_table* tab,
+                        Rec<T> out[2], T e_out[2], Events* ev) {
+  *ev = Events{0, 0, 0, 0, 0};
+  const T wi = w[i];
+  if (!(wi > (T)0)) return 0;
+  T ef;
+  if (P.dim == 1) {
+    int cell = (int)((double)x[i] / P.dx);
+    if (cell < 0) cell = 0;
+    if (cell >= P.n_cells) cell = P.n_cells - 1;
+    ef = E[cell];
+  } else {
+    ef = E ? E[0] : (T)P.e_field0;
+  }
+  const T acc = (T)(P.afac / P.dt) * ((T)100 * ef);
+  const T inv_nu = (T)(1.0 / P.nu_max);
+  T px = x[i], pvx = vx[i], pvy = vy[i], pvz = vz[i];
+  T tau = (T)P.dt;
+  int alive = 1, born = 0;
+  Rec<T> sec = {0, 0, 0, 0, 0};
+  T e_sec = 0;
+  for (int k = 0; k < 256; ++k) {
+    T u[4];
+    flight_event(P.seed, (uint32_t)i, step, (uint32_t)k, u);
+    const T tf = -m_log1p(-u[0]) * inv_nu;
+    const bool hit = tf < tau;
+    const T h = hit ? tf : tau;
+    if (P.dim == 1) px = px + ((T)100 * h) * pvx;
+    pvx = pvx - acc * h;
+    if (P.dim == 1 && !((T)0 < px && px < (T)P.L)) {
+      alive = 0;
+      break;
+    }
+    if (!hit) {
+      tau = (T)0;
+      break;
+    }
+    tau = tau - tf;
+    T s2 = pvx * pvx + pvy * pvy + pvz * pvz;
+    const T v_mag = m_sqrt(s2);
+    const T e = (T)P.kev * s2;
+    T sg[3];
+    table_cross_sections<T>(e, P, *tab, sg);
+    const T r0 = (sg[0] * v_mag) * inv_nu;
+    const T r1 = r0 + (sg[1] * v_mag) * inv_nu;
+    const T r2 = r1 + (sg[2] * v_mag) * inv_nu;
+    const T R = u[1];
+    if (R < r0) {
+      ++ev->el;
+      scatter<T>(v_mag * (T)P.fel, u[2], u[3], &pvx, &pvy, &pvz);
+    } else if (R < r1) {
+      ++ev->exc;
+      const T e_new = e - (T)P.e_exc;
+      scatter<T>(m_sqrt(((T)2 * e_new) / (T)P.jm), u[2], u[3], &pvx, &pvy, &pvz);
+    } else if (R < r2 && !born) {
+      T e_ej = (T)0.5 * (e - (T)P.e_ion);
+      if (e_ej < (T)0) e_ej = -e_ej;
+      const T sp = m_sqrt(((T)2 * e_ej) / (T)P.jm);
+      scatter<T>(sp, u[2], u[3], &pvx, &pvy, &pvz);
+      T q[2], tx, ty, tz;
+      flight_secondary(P.seed, (uint32_t)i, step, (uint32_t)k, q);
+      scatter<T>(sp, q[0], q[1], &tx, &ty, &tz);
+      born = 1;
+      ev->ion = 1;
+      sec = Rec<T>{px, tx, ty, tz, wi};
+      e_sec = (T)P.kev * (tx * tx + ty * ty + tz * tz);
+    } else {
+      ++ev->null_;
+    }
+  }
+  if (alive && tau > (T)0) {
+    if (P.dim == 1) px = px + ((T)100 * tau) * pvx;
+    pvx = pvx - acc * tau;
+    if (P.dim == 1 && !((T)0 < px && px < (T)P.L)) alive = 0;
+  }
+  ev->alive = alive;
+  int k = 0;
+  if (alive) {
+    out[k] = Rec<T>{px, pvx, pvy, pvz, wi};
+    e_out[k] = (T)P.kev * (pvx * pvx + pvy * pvy + pvz * pvz);
+    ++k;
+  }
+  if (born) {
+    out[k] = sec;
+    e_out[k] = e_sec;
+    ++k;
+  }
+  return k;
+}
+
 template <typename T>
 int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const T* vy,
                   const T* vz, const T* w, const T* E, const T* R, int64_t rs, const T* G,
@@ -357,13 +496,27 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
     for (int64_t i = lo; i < hi; ++i) {
       Rec<T> o[2];
       T eo[2];
-      int chan, ion;
-      const int m = one_particle<T>(i, *P, x, vx, vy, vz, w, E, R, rs, G, tab, o, eo, &chan, &ion);
-      if (w[i] > (T)0) {
-        cc[chan == 0 ? 5 : (chan == 1 ? 2 : (chan == 2 ? 3 : 4))] += 1;
-        if (chan == 3) cc[0] += 1;
-        const bool alive_primary = (m == 2) || (m == 1 && chan != 3);
-        if (!alive_primary) cc[1] += 1;
+      int chan, ion, m;
+      if (P->xs_mode == 2) {
+        Events ev;
+        m = one_particle_flight<T>(i, P->step, *P, x, vx, vy, vz, w, E, tab, o, eo, &ev);
+        ion = ev.alive && ev.ion;
+        if (w[i] > (T)0) {
+          cc[2] += ev.el;
+          cc[3] += ev.exc;
+          cc[4] += ev.ion;
+          cc[0] += ev.ion;
+          cc[5] += ev.null_;
+          if (!ev.alive) cc[1] += 1;
+        }
+      } else {
+        m = one_particle<T>(i, *P, x, vx, vy, vz, w, E, R, rs, G, tab, o, eo, &chan, &ion);
+        if (w[i] > (T)0) {
+          cc[chan == 0 ? 5 : (chan == 1 ? 2 : (chan == 2 ? 3 : 4))] += 1;
+          if (chan == 3) cc[0] += 1;
+          const bool alive_primary = (m == 2) || (m == 1 && chan != 3);
+          if (!alive_primary) cc[1] += 1;
+        }
       }
       for (int q = 0; q < m; ++q) {
         const int nd = node_of((double)o[q].x, *P);

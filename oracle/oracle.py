@@ -23,8 +23,9 @@ _lib = None
 class OrcParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "dt", "xfac", "afac", "kev", "jm", "fel", "L", "dx", "half_dx", "L_dx", "dep_M", "nn",
-        "e_ion", "e_exc", "pc", "e_field0")] + [
-        (n, C.c_int32) for n in ("dim", "xs_mode", "n_cells", "n_nodes", "bnf", "nthreads")]
+        "e_ion", "e_exc", "pc", "e_field0", "nu_max")] + [
+        (n, C.c_int32) for n in ("dim", "xs_mode", "n_cells", "n_nodes", "bnf", "nthreads")] + [
+        ("seed", C.c_uint64), ("step", C.c_uint64)]
 
 
 class OrcTable(C.Structure):
@@ -74,7 +75,7 @@ class Setup:
     """Physical inputs of one oracle run."""
     dim: int
     dt: float
-    xs_mode: int = 1           # 0 reference fits, 1 table
+    xs_mode: int = 1           # 0 reference fits, 1 table, 2 table with free flights
     n_cells: int = 1
     length_cm: float = 2.54
     bnf: int = 4
@@ -85,6 +86,8 @@ class Setup:
     nu_max: float = 0.0
     e_field0: float = 0.0
     nthreads: int = 1
+    seed: int = 12345          # free-flight mode draws its own Philox stream
+    step: int = 0
 
     def params(self) -> OrcParams:
         p = OrcParams()
@@ -114,6 +117,9 @@ class Setup:
         p.n_nodes = n + 1 if self.dim == 1 else 1
         p.bnf = self.bnf
         p.nthreads = self.nthreads
+        p.nu_max = self.nu_max
+        p.seed = self.seed
+        p.step = self.step
         return p
 
 
@@ -133,7 +139,8 @@ def resample_uniform(seed: int, step: int, key: int, m: int) -> float:
 def push_step(setup: Setup, state, E, R, G=None, table=None, capacity=None):
     """One collision + push + wall + deposit + compaction step.
 
-    state: (x, vx, vy, vz, w) arrays of one dtype; R: [7, stride] uniforms;
+    state: (x, vx, vy, vz, w) arrays of one dtype; R: [7, stride] uniforms
+    (None in free-flight mode, which draws Philox(seed, step) itself);
     G: optional [3, stride] gas velocities; table: (energy, sig[3,n], slope[3,n]).
     Returns (new_state, moments[3, n_nodes], counters[8])."""
     lib = load()
@@ -153,7 +160,8 @@ def push_step(setup: Setup, state, E, R, G=None, table=None, capacity=None):
         keep += [e, s, sl]
         tab = OrcTable(energy=_ptr(e), sig=_ptr(s), slope=_ptr(sl), n=e.size)
     Ea = None if E is None else np.ascontiguousarray(E, dtype=dt)
-    Ra = np.ascontiguousarray(R, dtype=dt)
+    # free-flight mode (xs_mode 2) draws its own stream: R may be None
+    Ra = np.zeros((7, 1), dtype=dt) if R is None else np.ascontiguousarray(R, dtype=dt)
     Ga = None if G is None else np.ascontiguousarray(G, dtype=dt)
     rs = Ra.shape[1]
     if Ga is not None and Ga.shape[1] != rs:

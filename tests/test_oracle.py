@@ -404,3 +404,77 @@ def test_sg_heavies_substep_count():
     half, _ = _heavies(N, dx, dt / 2, mu, Di, 1.0, ni, E, 1)
     half, _ = _heavies(N, dx, dt / 2, mu, Di, 1.0, half, E, 1)
     np.testing.assert_allclose(full, half, rtol=1e-13)
+
+
+def _flight_setup(dim, dt, gas, n_cells=1, e_field0=0.0, dtype=np.float64):
+    from boltzmann_amd import physics as ph
+    tab = ph.build_table(ph.XSecModel(), gas, dtype)
+    s = O.Setup(dim=dim, dt=dt, xs_mode=2, n_cells=n_cells, length_cm=2.5,
+                gas_density_m3=gas, nu_max=tab.nu_max, e_field0=e_field0, seed=77, step=3)
+    return s, (tab.energy, tab.sig, tab.slope), tab
+
+
+def test_free_flight_elastic_frequency_is_exact_for_long_steps():
+    """Mono-energetic 5 eV electrons (below every inelastic threshold), no
+    field: the number of real (elastic) collisions per particle and step is
+    Poisson(N sigma v dt) whatever nu_max dt is -- here nu_max dt ~ 9 and
+    nu dt = 2, where one null-collision draw per step would cap it at 1."""
+    n = 20000
+    KEV = 0.5 * O.M_E * O.J_EV
+    v = np.sqrt(5.0 / KEV)
+    gas = 3.22e21
+    nu = gas * 1e-19 * v
+    dt = 2.0 / nu
+    s, tab, t = _flight_setup(0, dt, gas)
+    assert t.nu_max * dt > 5
+    d = np.random.default_rng(3).normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    st = (np.zeros(n), *(v * d), np.ones(n))
+    out, mom, cnt = O.push_step(s, st, None, None, table=tab, capacity=2 * n)
+    mean = cnt[2] / n
+    assert abs(mean / 2.0 - 1) < 0.015, mean          # 0.5% statistical error
+    # each elastic event scales the speed by sqrt(1 - 2 m/M) exactly
+    fel = math.sqrt(1 - 2 * 1.36e-5)
+    sp = np.sqrt(out[1] ** 2 + out[2] ** 2 + out[3] ** 2)
+    k = np.log(sp / v) / np.log(fel)
+    assert np.all(np.abs(k - np.round(k)) < 1e-6)
+    assert int(np.round(k).sum()) == cnt[2]
+    assert cnt[4] == 0 and cnt[3] == 0
+
+
+def test_free_flight_ballistic_without_gas():
+    """Negligible gas: one flight spans the step, x' = x + 100 dt vx and
+    vx' = vx - dt (q/m) 100 E, the single-collision step's push."""
+    n = 1000
+    s, tab, _ = _flight_setup(1, 1e-10, 1e3, n_cells=10)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(1.0, 1.5, n)
+    v = rng.normal(0, 1e5, (3, n))
+    E = np.full(10, 50.0)
+    out, _, cnt = O.push_step(s, (x, *v, np.ones(n)), E, None, table=tab, capacity=2 * n)
+    assert cnt[2] + cnt[3] + cnt[4] == 0
+    np.testing.assert_allclose(out[0], x + 100 * 1e-10 * v[0], rtol=1e-13)
+    a = (1e-10 * (O.Q_E / O.M_E)) / 1e-10
+    np.testing.assert_allclose(out[1], v[0] - (a * (100 * 50.0)) * 1e-10, rtol=1e-12)
+
+
+def test_free_flight_reduces_to_null_collision_rates():
+    """Hot electrons with ionization: event totals match the single-draw
+    null-collision step when nu_max dt is small (both sample the same rates)."""
+    n = 40000
+    KEV = 0.5 * O.M_E * O.J_EV
+    rng = np.random.default_rng(5)
+    e = rng.uniform(20, 60, n)
+    d = rng.normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    v = np.sqrt(e / KEV) * d
+    gas = 3.22e21
+    s, tab, t = _flight_setup(0, 1e-12, gas)
+    st = (np.zeros(n), *v, np.ones(n))
+    _, _, c_ff = O.push_step(s, st, None, None, table=tab, capacity=2 * n)
+    s1 = O.Setup(dim=0, dt=1e-12, xs_mode=1, gas_density_m3=gas, nu_max=t.nu_max)
+    _, _, c_nc = O.push_step(s1, st, None, O.collision_uniforms(5, 0, n, np.float64), table=tab,
+                             capacity=2 * n)
+    for k in (2, 3, 4):
+        a, b = int(c_ff[k]), int(c_nc[k])
+        assert abs(a - b) <= 5 * math.sqrt(a + b) + 5, (k, a, b)

@@ -108,6 +108,58 @@ def test_push_parity(gpu, name, dim, prec, xs, rng_mode):
     eng.close()
 
 
+FLIGHT_CASES = [("flight-f32-1d", 1, 4), ("flight-f64-1d", 1, 8), ("flight-f32-0d", 0, 4),
+                ("flight-f64-0d", 0, 8)]
+
+
+@pytest.mark.parametrize("name,dim,prec", FLIGHT_CASES, ids=[c[0] for c in FLIGHT_CASES])
+def test_flight_parity(gpu, name, dim, prec):
+    """Free-flight collisions (about 12 flights per particle and step) over
+    two steps: event counts, births, absorptions bit-exact; states and
+    moments to rounding."""
+    import torch
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    n = 20000
+    cfg = _cfg(ph, dim, prec, 2, n, gas=3.2e23)
+    eng = KineticEngine(cfg, capacity=3 * n + 64)
+    dt_np = cfg.np_dtype
+    st = _state(n, dt_np, dim, seed=sum(map(ord, name)))
+    eng.load_state(st)
+    E = np.random.default_rng(1).uniform(-400, 400, cfg.n_cells).astype(dt_np)
+    if dim == 1:
+        eng.E_t.copy_(torch.from_numpy(E))
+    cur = st
+    for step in range(2):
+        eng.push()
+        _lib_step = eng.lib.bz_clock_advance(eng.ctx, cfg.dt)
+        assert _lib_step == 0
+        got = eng.fetch_state()
+        mom = eng.moments.cpu().numpy().reshape(3, -1)
+        cnt = eng.counters()
+        s = _setup(cfg, eng.table)
+        s.seed, s.step = cfg.seed, step
+        want, wmom, wcnt = O.push_step(s, cur, E if dim == 1 else None, None,
+                                       table=_tab(eng.table), capacity=3 * n + 64)
+        if step == 0:
+            # ~12 flights per particle (nu_max dt), most ending in null events
+            assert wcnt[5] > 5 * n and wcnt[2] > n // 2 and wcnt[4] > 0
+            base = dict(cnt)
+            for k in base:
+                base[k] = 0
+        assert cnt["births"] - base["births"] == wcnt[0]
+        assert cnt["absorbed"] - base["absorbed"] == wcnt[1]
+        assert cnt["elastic"] - base["elastic"] == wcnt[2]
+        assert cnt["excitation"] - base["excitation"] == wcnt[3]
+        assert cnt["null"] - base["null"] == wcnt[5]
+        _compare_state(got, want, dt_np, dim)
+        rt = 1e-12 if prec == 8 else 1e-6
+        np.testing.assert_allclose(mom, wmom, rtol=rt, atol=rt * np.abs(wmom).max())
+        base = dict(cnt)
+        cur = want
+    eng.close()
+
+
 @pytest.mark.parametrize("shape", [0, 1, 2, 3])
 def test_schedules_and_shapes_agree(gpu, shape):
     """Static (cooperative) and ticket tile schedules, for every tile shape,
