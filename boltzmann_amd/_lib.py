@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libbz.so"
 BZ_F32, BZ_F64 = 4, 8
 BZ_XS_FIT, BZ_XS_TABLE, BZ_XS_FLIGHT = 0, 1, 2
 BZ_RNG_PHILOX, BZ_RNG_STREAM = 0, 1
+BZ_PUSH_REF, BZ_PUSH_LEAPFROG = 0, 1
 BZ_NCOUNTERS = 16
 COUNTER_NAMES = ("births", "absorbed", "elastic", "excitation", "ionization", "null",
                  "overflow", "numax", "split", "merge", "resample", "particle_steps")
@@ -38,7 +39,7 @@ class Params(C.Structure):
         ("seed", C.c_uint64),
         ("dim", C.c_int32), ("xs_mode", C.c_int32), ("rng_mode", C.c_int32),
         ("n_cells", C.c_int32), ("n_nodes", C.c_int32), ("bnf", C.c_int32),
-        ("r_stride", C.c_int64),
+        ("r_stride", C.c_int64), ("push_order", C.c_int32), ("pad_", C.c_int32),
     ]
 
 

@@ -293,6 +293,8 @@ static int check_params(const bz_ctx* c, const bz_params_t* p) {
   if (p->xs_mode == BZ_XS_FLIGHT && !(p->nu_max > 0.0))
     return fail("free-flight collisions need nu_max > 0");
   if (p->rng_mode != BZ_RNG_PHILOX && p->rng_mode != BZ_RNG_STREAM) return fail("bad rng_mode");
+  if (p->push_order != BZ_PUSH_REF && p->push_order != BZ_PUSH_LEAPFROG)
+    return fail("bad push_order");
   return 0;
 }
 

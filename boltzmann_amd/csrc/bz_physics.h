@@ -302,14 +302,21 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
   T x = in.x;
   T ef;
   if (DIM == 1) {
-    x = x + (T)p.xfac * vx;
     int c = (int)((double)in.x / p.dx);
     c = c < 0 ? 0 : (c >= p.n_cells ? p.n_cells - 1 : c);
     ef = E[c];
   } else {
     ef = e_local;  // 0-D: uniform field of the swarm (or of its instance)
   }
-  vx = vx - (T)p.afac * ((T)100 * ef);
+  if (p.push_order == BZ_PUSH_LEAPFROG) {
+    // kick with E at the step's starting cell, then drift with the new v
+    vx = vx - (T)p.afac * ((T)100 * ef);
+    if (DIM == 1) x = x + (T)p.xfac * vx;
+  } else {
+    // reference order: drift with the post-collision v, then kick
+    if (DIM == 1) x = x + (T)p.xfac * vx;
+    vx = vx - (T)p.afac * ((T)100 * ef);
+  }
   if (DIM == 1) o.alive = ((T)0 < x && x < (T)p.L) ? 1 : 0;
   else o.alive = 1;
   o.p.x = x;
@@ -411,8 +418,13 @@ __device__ __forceinline__ StepOut<T> flight_particle(const P5<T> in, uint32_t i
     const T tf = -dlog1p(-u[0]) * inv_nu;
     const bool hit = tf < tau;
     const T h = hit ? tf : tau;
-    if (DIM == 1) x = x + ((T)100 * h) * vx;
-    vx = vx - acc * h;
+    if (p.push_order == BZ_PUSH_LEAPFROG) {
+      vx = vx - acc * h;
+      if (DIM == 1) x = x + ((T)100 * h) * vx;
+    } else {
+      if (DIM == 1) x = x + ((T)100 * h) * vx;
+      vx = vx - acc * h;
+    }
     if (DIM == 1 && !((T)0 < x && x < (T)p.L)) {
       alive = 0;
       break;
@@ -460,8 +472,13 @@ __device__ __forceinline__ StepOut<T> flight_particle(const P5<T> in, uint32_t i
   }
   if (alive && tau > (T)0) {
     // flight budget exhausted (nu_max dt far above kMaxFlights): rest ballistic
-    if (DIM == 1) x = x + ((T)100 * tau) * vx;
-    vx = vx - acc * tau;
+    if (p.push_order == BZ_PUSH_LEAPFROG) {
+      vx = vx - acc * tau;
+      if (DIM == 1) x = x + ((T)100 * tau) * vx;
+    } else {
+      if (DIM == 1) x = x + ((T)100 * tau) * vx;
+      vx = vx - acc * tau;
+    }
     if (DIM == 1 && !((T)0 < x && x < (T)p.L)) alive = 0;
     o.over = 1;
   }

@@ -97,6 +97,7 @@ class RunConfig:
     fluid_on: bool = True
     poisson_on: bool = True
     ion_scheme: int = 0           # 0 reference central fluxes, 1 Scharfetter-Gummel sub-cycled
+    push_order: int = _lib.BZ_PUSH_REF  # BZ_PUSH_LEAPFROG: kick then drift
     max_substeps: int = 256
     capacity_factor: float = 2.0
     xsec: XSecModel = field(default_factory=XSecModel)
@@ -172,6 +173,7 @@ def push_params(cfg: RunConfig, rng_mode: int = _lib.BZ_RNG_PHILOX,
     p.n_nodes = n + 1 if cfg.dim == 1 else 1
     p.bnf = cfg.bnf
     p.r_stride = 0
+    p.push_order = cfg.push_order
     return p
 
 

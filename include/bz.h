@@ -47,6 +47,14 @@ enum {
                         (the reference's cp_R_vec / gas_v arrays) */
 };
 
+/* order of the two updates of the push */
+enum {
+  BZ_PUSH_REF = 0,     /* x += 100 dt v, then v -= dt q/m E (electron_kernel_numba.py:257-264):
+                          both use the state at the start of the step (explicit Euler) */
+  BZ_PUSH_LEAPFROG = 1 /* v -= dt q/m E(x), then x += 100 dt v with the new v
+                          (kick-drift leapfrog, v staggered by half a step) */
+};
+
 /* counters accumulated by the kernels (uint64 each) */
 enum {
   BZ_CNT_BIRTHS = 0,
@@ -103,6 +111,8 @@ typedef struct {
   int32_t n_nodes;  /* N+1: deposit nodes */
   int32_t bnf;      /* fine bins per node of the deposit */
   int64_t r_stride; /* BZ_RNG_STREAM: column stride of R and gas_v */
+  int32_t push_order; /* BZ_PUSH_*: order of the position and velocity updates */
+  int32_t pad_;
 } bz_params_t;
 
 /* Tabulated sigma(E)*N_gas, three channels (0 elastic, 1 excitation,

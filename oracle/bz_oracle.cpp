@@ -115,6 +115,8 @@ typedef struct {
   int32_t dim, xs_mode, n_cells, n_nodes, bnf, nthreads;
   uint64_t seed;  // Philox key of the free-flight draws (xs_mode 2)
   uint64_t step;  // step number keying those draws
+  int32_t push_order;  // 0: x then v (reference), 1: v then x (leapfrog)
+  int32_t pad;
 } orc_params;
 
 typedef struct {
@@ -310,7 +312,6 @@ int one_particle(int64_t i, const orc_params& P, const T* x, const T* vx, const 
   T nx = x[i];
   T ef;
   if (P.dim == 1) {
-    nx = nx + (T)P.xfac * nvx;
     int cell = (int)((double)x[i] / P.dx);
     if (cell < 0) cell = 0;
     if (cell >= P.n_cells) cell = P.n_cells - 1;
@@ -318,7 +319,14 @@ int one_particle(int64_t i, const orc_params& P, const T* x, const T* vx, const 
   } else {
     ef = E ? E[0] : (T)P.e_field0;
   }
-  nvx = nvx - (T)P.afac * ((T)100 * ef);
+  if (P.push_order == 1) {
+    nvx = nvx - (T)P.afac * ((T)100 * ef);
+    if (P.dim == 1) nx = nx + (T)P.xfac * nvx;
+  } else {
+    // electron_kernel_numba.py:257-264: X-ADV with the post-collision v, then V-ADV
+    if (P.dim == 1) nx = nx + (T)P.xfac * nvx;
+    nvx = nvx - (T)P.afac * ((T)100 * ef);
+  }
   const bool alive = (P.dim == 0) || ((T)0 < nx && nx < (T)P.L);
   int k = 0;
   if (alive) {
@@ -408,8 +416,13 @@ int one_particle_flight(int64_t i, uint64_t step, const orc_params& P, const T* 
     const T tf = -m_log1p(-u[0]) * inv_nu;
     const bool hit = tf < tau;
     const T h = hit ? tf : tau;
-    if (P.dim == 1) px = px + ((T)100 * h) * pvx;
-    pvx = pvx - acc * h;
+    if (P.push_order == 1) {
+      pvx = pvx - acc * h;
+      if (P.dim == 1) px = px + ((T)100 * h) * pvx;
+    } else {
+      if (P.dim == 1) px = px + ((T)100 * h) * pvx;
+      pvx = pvx - acc * h;
+    }
     if (P.dim == 1 && !((T)0 < px && px < (T)P.L)) {
       alive = 0;
       break;
@@ -452,8 +465,13 @@ int one_particle_flight(int64_t i, uint64_t step, const orc_params& P, const T* 
     }
   }
   if (alive && tau > (T)0) {
-    if (P.dim == 1) px = px + ((T)100 * tau) * pvx;
-    pvx = pvx - acc * tau;
+    if (P.push_order == 1) {
+      pvx = pvx - acc * tau;
+      if (P.dim == 1) px = px + ((T)100 * tau) * pvx;
+    } else {
+      if (P.dim == 1) px = px + ((T)100 * tau) * pvx;
+      pvx = pvx - acc * tau;
+    }
     if (P.dim == 1 && !((T)0 < px && px < (T)P.L)) alive = 0;
   }
   ev->alive = alive;

@@ -25,7 +25,8 @@ class OrcParams(C.Structure):
         "dt", "xfac", "afac", "kev", "jm", "fel", "L", "dx", "half_dx", "L_dx", "dep_M", "nn",
         "e_ion", "e_exc", "pc", "e_field0", "nu_max")] + [
         (n, C.c_int32) for n in ("dim", "xs_mode", "n_cells", "n_nodes", "bnf", "nthreads")] + [
-        ("seed", C.c_uint64), ("step", C.c_uint64)]
+        ("seed", C.c_uint64), ("step", C.c_uint64), ("push_order", C.c_int32),
+        ("pad", C.c_int32)]
 
 
 class OrcTable(C.Structure):
@@ -88,6 +89,7 @@ class Setup:
     nthreads: int = 1
     seed: int = 12345          # free-flight mode draws its own Philox stream
     step: int = 0
+    push_order: int = 0        # 0 reference (x then v), 1 leapfrog (v then x)
 
     def params(self) -> OrcParams:
         p = OrcParams()
@@ -120,6 +122,7 @@ class Setup:
         p.nu_max = self.nu_max
         p.seed = self.seed
         p.step = self.step
+        p.push_order = self.push_order
         return p
 
 

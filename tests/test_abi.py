@@ -51,8 +51,8 @@ def test_error_path_without_device():
 
 def test_struct_sizes_match_header_layout():
     from boltzmann_amd import _lib
-    # 19 doubles + u64 + 6 int32 + int64
-    assert ctypes.sizeof(_lib.Params) == 19 * 8 + 8 + 6 * 4 + 8
+    # 19 doubles + u64 + 6 int32 + int64 + 2 int32
+    assert ctypes.sizeof(_lib.Params) == 19 * 8 + 8 + 6 * 4 + 8 + 2 * 4
     assert ctypes.sizeof(_lib.XSec) == 4 * 8 + 4 * 4
     assert ctypes.sizeof(_lib.FieldParams) == 8 * 8 + 6 * 4
     assert ctypes.sizeof(_lib.Mlp) == 8 + 12 * 8

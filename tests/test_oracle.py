@@ -67,20 +67,24 @@ def _table(dtype, gas=3.2e22):
 KEV = 0.5 * O.M_E * O.J_EV
 
 
+@pytest.mark.parametrize("order", [0, 1])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_free_push_closed_form(dtype):
-    """R0 = 1-eps: no collision; x' = x + 100 dt v, vx' = vx - dt q/m 100 E[int(x/dx)]."""
+def test_free_push_closed_form(dtype, order):
+    """R0 = 1-eps: no collision.  Reference order (0): x' = x + 100 dt v,
+    vx' = vx - dt q/m 100 E[int(x/dx)]; leapfrog (1): vx' first, then
+    x' = x + 100 dt vx'."""
     n, N, L, dt = 2000, 64, 2.5, 2e-11
     t, tab = _table(dtype)
-    s = O.Setup(dim=1, dt=dt, n_cells=N, length_cm=L, nu_max=t.nu_max, gas_density_m3=3.2e22)
+    s = O.Setup(dim=1, dt=dt, n_cells=N, length_cm=L, nu_max=t.nu_max, gas_density_m3=3.2e22,
+                push_order=order)
     st = _state(n, dtype, 1, L)
     E = np.linspace(-50, 80, N).astype(dtype)
     R = np.full((7, n), 0.9999, dtype=dtype)
     out, mom, cnt = O.push_step(s, st, E, R, table=tab)
     x, vx, vy, vz, w = [a.astype(np.float64) for a in st]
     cell = np.clip((x / (L / N)).astype(int), 0, N - 1)
-    xn = x + 100 * dt * vx
     vxn = vx - dt * (O.Q_E / O.M_E) * (100 * E.astype(np.float64)[cell])
+    xn = x + 100 * dt * (vxn if order == 1 else vx)
     keep = (xn > 0) & (xn < L)
     assert out[0].size == keep.sum()
     assert cnt[1] == (~keep).sum() and cnt[5] == n and cnt[0] == 0
@@ -90,6 +94,29 @@ def test_free_push_closed_form(dtype):
     assert np.all(np.abs(out[1] - vxn[keep]) <= rt * (np.abs(vx[keep]) + kick) + 1e-9)
     np.testing.assert_array_equal(out[2], st[2][keep])
     np.testing.assert_array_equal(out[4], st[4][keep])
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_uniform_field_trajectory(order):
+    """k collisionless steps in a uniform field a: v_k = v0 - k a dt and
+    x_k = x0 + 100 dt (k v0 - a dt S), S = k(k-1)/2 for the reference order
+    (position first) and k(k+1)/2 for the leapfrog (velocity first)."""
+    n, N, L, dt, k = 64, 16, 2.5, 1e-12, 25
+    s = O.Setup(dim=1, dt=dt, n_cells=N, length_cm=L, gas_density_m3=1e-30,
+                nu_max=1e-30, push_order=order)
+    rng = np.random.default_rng(3)
+    x0 = rng.uniform(1.0, 1.5, n)
+    v0 = rng.normal(0, 1e5, n)
+    st = (x0.copy(), v0.copy(), np.zeros(n), np.zeros(n), np.ones(n))
+    E = np.full(N, 7.0)
+    t = _table(np.float64, gas=1e-30)[1]
+    R = np.full((7, n), 0.9999)
+    for _ in range(k):
+        st, _, _ = O.push_step(s, st, E, R, table=t)
+    a = (O.Q_E / O.M_E) * 100 * 7.0
+    S = k * (k - 1) / 2 if order == 0 else k * (k + 1) / 2
+    np.testing.assert_allclose(st[1], v0 - k * a * dt, rtol=1e-12, atol=1e-6)
+    np.testing.assert_allclose(st[0], x0 + 100 * dt * (k * v0 - a * dt * S), rtol=1e-12)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])

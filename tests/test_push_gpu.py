@@ -39,7 +39,8 @@ def _setup(cfg, table):
     return O.Setup(dim=cfg.dim, dt=cfg.dt, xs_mode=cfg.xs_mode, n_cells=cfg.n_cells,
                    length_cm=cfg.length_cm, bnf=cfg.bnf, gas_density_m3=cfg.gas_density_m3,
                    nu_max=table.nu_max if table is not None else 0.0,
-                   e_field0=cfg.e_field0_vcm() if cfg.dim == 0 else 0.0)
+                   e_field0=cfg.e_field0_vcm() if cfg.dim == 0 else 0.0,
+                   push_order=cfg.push_order)
 
 
 def _tab(table):
@@ -70,15 +71,21 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("name,dim,prec,xs", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("rng_mode", ["stream", "philox"])
-def test_push_parity(gpu, name, dim, prec, xs, rng_mode):
+# the update order of the push only matters where positions move (1-D)
+PUSH_CASES = [(*c, rng, 0) for c in CASES for rng in ("stream", "philox")] + \
+    [(*c, "philox", 1) for c in CASES if c[1] == 1]
+
+
+@pytest.mark.parametrize("name,dim,prec,xs,rng_mode,order", PUSH_CASES,
+                         ids=[f"{c[0]}-{c[4]}-{'leapfrog' if c[5] else 'ref-order'}"
+                              for c in PUSH_CASES])
+def test_push_parity(gpu, name, dim, prec, xs, rng_mode, order):
     import torch
     from boltzmann_amd import physics as ph
     from boltzmann_amd.engine import KineticEngine
     n = 30000
     gas = 3.2e24 if xs == 1 else 3.22e23
-    cfg = _cfg(ph, dim, prec, xs, n, gas=gas)
+    cfg = _cfg(ph, dim, prec, xs, n, gas=gas, push_order=order)
     eng = KineticEngine(cfg, capacity=2 * n + 64)
     dt_np = cfg.np_dtype
     st = _state(n, dt_np, dim, seed=sum(map(ord, name)))
@@ -112,8 +119,13 @@ FLIGHT_CASES = [("flight-f32-1d", 1, 4), ("flight-f64-1d", 1, 8), ("flight-f32-0
                 ("flight-f64-0d", 0, 8)]
 
 
-@pytest.mark.parametrize("name,dim,prec", FLIGHT_CASES, ids=[c[0] for c in FLIGHT_CASES])
-def test_flight_parity(gpu, name, dim, prec):
+FLIGHT_ORDERS = [(*c, 0) for c in FLIGHT_CASES] + [(*c, 1) for c in FLIGHT_CASES if c[1] == 1]
+
+
+@pytest.mark.parametrize("name,dim,prec,order", FLIGHT_ORDERS,
+                         ids=[f"{c[0]}-{'leapfrog' if c[3] else 'ref-order'}"
+                              for c in FLIGHT_ORDERS])
+def test_flight_parity(gpu, name, dim, prec, order):
     """Free-flight collisions (about 12 flights per particle and step) over
     two steps: event counts, births, absorptions bit-exact; states and
     moments to rounding."""
@@ -121,7 +133,7 @@ def test_flight_parity(gpu, name, dim, prec):
     from boltzmann_amd import physics as ph
     from boltzmann_amd.engine import KineticEngine
     n = 20000
-    cfg = _cfg(ph, dim, prec, 2, n, gas=3.2e23)
+    cfg = _cfg(ph, dim, prec, 2, n, gas=3.2e23, push_order=order)
     eng = KineticEngine(cfg, capacity=3 * n + 64)
     dt_np = cfg.np_dtype
     st = _state(n, dt_np, dim, seed=sum(map(ord, name)))
