@@ -155,12 +155,14 @@ class ClosureDischarge:
         k = self.resample_every
         if k and self.cfg.target_ppc > 0 and self.macro % k == 0:
             e.resample()
-        # kinetic update over n_sub * dt
+        # kinetic update over n_sub * dt (1-D: in place, births appended)
         src_, dst = e.bufs[e.cur], e.bufs[1 - e.cur]
         _lib.check(e.lib.bz_push_mcc(e.ctx, C.byref(src_.struct), C.byref(dst.struct),
                                      _p(e.E_t), C.byref(self.kparams), C.byref(e.xs), None, None,
                                      _p(e.moments)), "bz_push_mcc")
-        e.cur = 1 - e.cur
+        e._since_compact += 1
+        if e.compact_every and e._since_compact >= e.compact_every:
+            e.compact()
         net = self.net.struct()
         for _ in range(self.n_sub):
             _lib.check(e.lib.bz_closure_source(e.ctx, C.byref(net), self.cfg.n_cells, _p(e.E),

@@ -101,7 +101,8 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   e.max_keys = max_keys;
   e.push_shape = 0;
   // smallest push tile is 256 particles
-  e.status_words = capacity / 256 + 128;
+  // look-back tile words, or the per-block birth counts of the in-place push
+  e.status_words = capacity / 256 + 4096;
   e.partials_doubles = push_partials_doubles(e.num_cu, max_nodes, max_keys);
   e.sort_ws_bytes = (int64_t)sort_workspace_bytes(capacity, max_keys);
   hipError_t st = hipMalloc((void**)&e.clock, sizeof(Clock));
@@ -326,6 +327,13 @@ int bz_deposit(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p, double* mo
   if (rc) return rc;
   if (!s || !moments) return fail("null state or moments pointer");
   return launch_deposit(c->env, s, p, moments);
+}
+
+int bz_compact(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out) {
+  BZ_CTX(c);
+  if (!in || !out) return fail("null state");
+  if (in->w == out->w) return fail("bz_compact needs two distinct state buffers");
+  return launch_compact(c->env, in, out);
 }
 
 int bz_field_step(bz_ctx_t c, const bz_field_params_t* fp, const double* moments, double* ni,

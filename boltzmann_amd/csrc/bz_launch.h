@@ -52,6 +52,7 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
                 const void* G, double* moments, int shape);
 int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                    double* moments);
+int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out);
 int launch_reduction(const LaunchEnv& env, int r, int M, int RM, double* dw, const double* dv,
                      const double* dx, int64_t ptcl);
 int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double* moments,

@@ -1,13 +1,24 @@
-// Fused electron step for gfx950: MCC + push + walls + node deposit +
-// compaction with births, in one persistent pass over the SoA state.
+// Fused electron step for gfx950: MCC + push + walls + node deposit, in one
+// pass over the SoA state, with births and the removal of absorbed particles
+// handled in one of two ways.
 //
-// Data flow per tile of BLOCK*ITEMS particles (index-contiguous):
+// 1-D discharges (in place, the reference's electron_kernel layout):
+//   each block owns a contiguous run of tiles of BLOCK*ITEMS particles
 //   HBM read  x, vx, vy, vz, w            (5 words / particle)
 //   registers collision, push, walls, secondary
 //   LDS       per-block FP64 node histograms (ne, ni_src, nrg)
+//   HBM write x, vx, vy, vz back to the same slot; w = 0 for an absorbed
+//             particle (w of a survivor is unchanged and not rewritten)
+//   births    compacted per block into a staging buffer, then appended
+//             after the live range by k_births_append in parent order
+//   No block waits on another, so any grid size and residency works.
+//   Absorbed slots stay as holes (w = 0) until a compaction or a resampling
+//   pass rewrites the state.
+// Batched and single 0-D swarms (compacting):
 //   scan      per-particle output count (alive + born) -> block scan ->
 //             decoupled look-back across tiles -> output slot
-//   HBM write survivors, each secondary right after its parent
+//   HBM write survivors, each secondary right after its parent, so every
+//             swarm instance stays contiguous
 // Node histograms are flushed once per block to a partial buffer and summed
 // over blocks in a fixed order by k_dep_reduce, so the moments do not depend
 // on scheduling across blocks.
@@ -32,7 +43,12 @@ struct PushArgs {
   const int64_t* inst_off;
   const void* inst_E;  // [n_inst] field of each instance, V/cm
   int n_inst;
+  uint32_t* block_births;  // in-place mode: births of each block
 };
+
+// per-block counter row of the push (u32): births, absorbed, elastic,
+// excitation, ionization, null, dropped, nu_max exceeded, live inputs
+constexpr int kCntRow = 10;
 
 // per-instance moments of a batched step (FP64, per block then summed)
 enum : int {
@@ -75,9 +91,10 @@ __host__ __device__ inline size_t table_lds_bytes(int n, int n_hint, int word) {
   return ((size_t)7 * n * word + (size_t)n_hint * 4 + 15) & ~(size_t)15;
 }
 
-template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS, bool INPLACE>
 __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   static_assert(ITEMS >= 1 && ITEMS <= 4, "packed scan holds 4 fields of 16 bits");
+  static_assert(!INPLACE || DIM == 1, "the in-place layout is the 1-D discharge's");
   constexpr int NW = BLOCK / kWave;
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int LB = 1024 / BLOCK > 0 ? 1024 / BLOCK : 1;  // look-back window 1024 tiles
@@ -89,7 +106,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   __shared__ uint32_t s_lbmin;
   __shared__ uint32_t s_lbred[NW];
   __shared__ uint32_t s_next;
-  __shared__ unsigned long long s_cnt[8];
+  __shared__ unsigned long long s_cnt[kCntRow];
 
   const bz_params_t& p = a.p;
   const int tid = threadIdx.x;
@@ -107,7 +124,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     for (int j = tid; j <= ni_; j += BLOCK) s_off[j] = a.inst_off[j];
     for (int j = tid; j < ni_; j += BLOCK) s_ie[j] = ((const T*)a.inst_E)[j];
   }
-  if (tid < 8) s_cnt[tid] = 0ull;
+  if (tid < kCntRow) s_cnt[tid] = 0ull;
   // stage the cross-section table in LDS (read-only for the whole launch)
   bz_xsec_t xs = a.xs;
   if (XS != BZ_XS_FIT) {
@@ -133,10 +150,15 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   const uint64_t step = a.clk->step;
   const uint32_t epoch = a.clk->epoch;
   const uint32_t ntiles = (uint32_t)((n + TILE - 1) / TILE);
-  if (blockIdx.x == 0 && tid == 0) {
-    if (n == 0) a.clk->n_next = 0;
-    atomicAdd(&a.clk->cnt[BZ_CNT_PSTEPS], (unsigned long long)n);
-  }
+  if (!INPLACE && blockIdx.x == 0 && tid == 0 && n == 0) a.clk->n_next = 0;
+  // in-place mode: this block's contiguous run of tiles [t, t_end)
+  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const uint32_t t_first = INPLACE ? (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)ntiles)
+                                   : 0u;
+  const uint32_t t_end = INPLACE ? (t_first + per < ntiles ? t_first + per : ntiles) : ntiles;
+  T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
+  const int64_t stage0 = (int64_t)t_first * TILE;  // births staged from here, in order
+  uint32_t bsum = 0;                               // births of this block so far
 
   const T* in[5] = {(const T*)a.in[0], (const T*)a.in[1], (const T*)a.in[2], (const T*)a.in[3],
                     (const T*)a.in[4]};
@@ -146,25 +168,32 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   const T e_uniform = (DIM == 0) ? (E ? E[0] : (T)p.e_field0) : (T)0;
 
   uint32_t c_el = 0, c_ex = 0, c_io = 0, c_abs = 0, c_null = 0, c_over = 0, c_drop = 0;
+  uint32_t c_live = 0;
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;  // DIM 0 deposit in registers
 
   // Tile schedule.  sched 0: static order b, b+G, ... (cooperative launch, so
   // every block is resident).  sched 1: tiles claimed from a ticket counter by
   // running blocks (any residency).  Either way a tile's look-back only waits
   // on tiles owned by blocks that are running, so every wait ends.
-  if (tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : blockIdx.x;
-  __syncthreads();
-  uint32_t t = s_next;
-  __syncthreads();  // s_next is rewritten at the top of the loop
+  uint32_t t;
+  if (INPLACE) {
+    t = t_first;
+  } else {
+    if (tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : blockIdx.x;
+    __syncthreads();
+    t = s_next;
+    __syncthreads();  // s_next is rewritten at the top of the loop
+  }
   P5<T> cur[ITEMS];
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k)
-    load5<T, WITHX>(in, (int64_t)t * TILE + k * BLOCK + tid, n, cur[k]);
+    load5<T, WITHX>(in, (int64_t)t * TILE + k * BLOCK + tid, t < t_end ? n : 0, cur[k]);
 
-  while (t < ntiles) {
+  while (t < t_end) {
     const int64_t base = (int64_t)t * TILE;
     // claim the next tile now; it is read after the scan barriers below
-    if (tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : t + gridDim.x;
+    if (!INPLACE && tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : t + gridDim.x;
+    uint32_t live_in = 0;  // bit k: item k held a live particle
 
     StepOut<T> o[ITEMS];
     uint64_t packed = 0;
@@ -189,6 +218,8 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       else
         o[k] = step_particle<T, XS, DIM, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, e_local);
       if (i < n && cur[k].w > (T)0) {
+        live_in |= 1u << k;
+        ++c_live;
         c_el += o[k].n_el;
         c_ex += o[k].n_exc;
         c_io += o[k].chan == kChanIon;
@@ -196,7 +227,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
         c_abs += !o[k].alive;
         c_over += o[k].over;
       }
-      const uint32_t cnt = (uint32_t)(o[k].alive + o[k].born);
+      const uint32_t cnt = INPLACE ? (uint32_t)o[k].born : (uint32_t)(o[k].alive + o[k].born);
       packed |= (uint64_t)cnt << (16 * k);
       if (DIM == 2) {
         // per-instance moments of this step; a wave whose lanes share one
@@ -261,7 +292,24 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       }
     }
 
+    if (INPLACE) {
+      // survivors back to their own slot (w unchanged), absorbed ones emptied
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        if (!((live_in >> k) & 1u)) continue;
+        const int64_t i = base + (int64_t)k * BLOCK + tid;
+        if (o[k].alive) {
+          st[0][i] = o[k].p.x;
+          st[1][i] = o[k].p.vx;
+          st[2][i] = o[k].p.vy;
+          st[3][i] = o[k].p.vz;
+        } else {
+          st[4][i] = (T)0;
+        }
+      }
+    }
     // block exclusive scan of the packed counts, in index order k-major
+    // (compacting: outputs per particle; in place: births per particle)
     const uint64_t incl = wave_incl_scan(packed);
     if (lane == kWave - 1) s_wsum[wid] = incl;
     __syncthreads();
@@ -273,7 +321,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     __syncthreads();
     const uint64_t excl = (wid ? s_wsum[wid - 1] : 0ull) + incl - packed;
     const uint64_t tot = s_wsum[NW - 1];
-    const uint32_t t_next = s_next;
+    const uint32_t t_next = INPLACE ? t + 1 : s_next;
     uint32_t field_base[ITEMS];
     uint32_t agg = 0;
 #pragma unroll
@@ -281,10 +329,26 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       field_base[k] = agg;
       agg += (uint32_t)((tot >> (16 * k)) & 0xFFFFu);
     }
-    // prefetch the next tile so its HBM latency overlaps the look-back
+    // prefetch the next tile so its HBM latency overlaps the stores / look-back
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k)
-      load5<T, WITHX>(in, (int64_t)t_next * TILE + k * BLOCK + tid, n, cur[k]);
+      load5<T, WITHX>(in, (int64_t)t_next * TILE + k * BLOCK + tid, t_next < t_end ? n : 0,
+                      cur[k]);
+    if (INPLACE) {
+      // births staged in parent order after the block's earlier births
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        if (!o[k].born) continue;
+        const int64_t ps =
+            stage0 + bsum + field_base[k] + (int64_t)((excl >> (16 * k)) & 0xFFFFu);
+        store5<T, true>(a.out, ps, o[k].s);
+      }
+      bsum += agg;
+      t = t_next;
+      // s_wsum is rewritten only after the next tile's first barrier
+      __syncthreads();
+      continue;
+    }
     const uint32_t pre = lookback_block<BLOCK, LB>(a.status, t, agg, epoch, &s_lbmin, s_lbred);
     if (tid == 0 && t == ntiles - 1) a.clk->n_next = (int64_t)(pre + agg);
     const int64_t tile_pre = pre;
@@ -307,11 +371,12 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     // look-back's barriers, which every thread has passed
   }
 
+  if (INPLACE && tid == 0) a.block_births[blockIdx.x] = bsum;
   // counters: wave sums -> LDS -> one row per block, summed by k_dep_reduce
   {
-    const uint32_t v[8] = {c_io, c_abs, c_el, c_ex, c_io, c_null, c_drop, c_over};
+    const uint32_t v[kCntRow] = {c_io, c_abs, c_el, c_ex, c_io, c_null, c_drop, c_over, c_live, 0};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < kCntRow; ++q) {
       const uint32_t s = wave_sum(v[q]);
       if (lane == 0 && s) atomicAdd(&s_cnt[q], (unsigned long long)s);
     }
@@ -339,7 +404,44 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   double* part = a.partials + (size_t)blockIdx.x * m;
   for (int j = tid; j < m; j += BLOCK) part[j] = s_dep[j];
   uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * m);
-  if (tid < 8) bcnt[(size_t)blockIdx.x * 8 + tid] = (uint32_t)s_cnt[tid];
+  if (tid < kCntRow) bcnt[(size_t)blockIdx.x * kCntRow + tid] = (uint32_t)s_cnt[tid];
+}
+
+// In-place mode, after k_push: append the births staged by each block after
+// the live range, block after block, so they follow their parents' order.
+// Block b sums the birth counts of blocks 0..b-1 itself (no serial chain);
+// the last block sets the slot count of the new state.
+template <typename T>
+__global__ __launch_bounds__(256) void k_births_append(PushArgs a, int64_t tile) {
+  __shared__ uint32_t s_red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int b = blockIdx.x;
+  const int64_t n = a.clk->n;
+  const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
+  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  uint32_t s = 0;
+  for (int j = tid; j < b; j += 256) s += a.block_births[j];
+  s = wave_sum(s);
+  if (lane == 0) s_red[wid] = s;
+  __syncthreads();
+  const uint64_t pre = (uint64_t)s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  const uint32_t c = a.block_births[b];
+  const int64_t src0 = (int64_t)min((uint64_t)b * per, (uint64_t)ntiles) * tile;
+  T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
+  const T* sg[5] = {(const T*)a.out[0], (const T*)a.out[1], (const T*)a.out[2],
+                    (const T*)a.out[3], (const T*)a.out[4]};
+  for (uint32_t j = tid; j < c; j += 256) {
+    const int64_t dst = n + (int64_t)pre + j;
+    if (dst >= a.capacity) break;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) st[f][dst] = sg[f][src0 + j];
+  }
+  if (b == (int)gridDim.x - 1 && tid == 0) {
+    const int64_t want = n + (int64_t)pre + c;
+    a.clk->n_next = want < a.capacity ? want : a.capacity;
+    if (want > a.capacity)
+      atomicAdd(&a.clk->cnt[BZ_CNT_OVERFLOW], (unsigned long long)(want - a.capacity));
+  }
 }
 
 // Sum the per-block partial histograms in block order and commit the clock.
@@ -360,11 +462,12 @@ __global__ __launch_bounds__(256) void k_dep_reduce(const double* __restrict__ p
   if (commit && blockIdx.x == 0 && threadIdx.x < 64) {
     // per-block counter rows written by the push (after its partials)
     const uint32_t* bcnt = (const uint32_t*)(partials + (size_t)nblocks * m);
-    const int slot[8] = {BZ_CNT_BIRTHS, BZ_CNT_ABSORBED, BZ_CNT_ELASTIC, BZ_CNT_EXCITATION,
-                         BZ_CNT_IONIZATION, BZ_CNT_NULL, BZ_CNT_OVERFLOW, BZ_CNT_NUMAX};
-    for (int q = 0; q < 8; ++q) {
+    const int slot[9] = {BZ_CNT_BIRTHS, BZ_CNT_ABSORBED, BZ_CNT_ELASTIC, BZ_CNT_EXCITATION,
+                         BZ_CNT_IONIZATION, BZ_CNT_NULL, BZ_CNT_OVERFLOW, BZ_CNT_NUMAX,
+                         BZ_CNT_PSTEPS};
+    for (int q = 0; q < 9; ++q) {
       unsigned long long s = 0;
-      for (int b = lane; b < nblocks; b += 64) s += bcnt[(size_t)b * 8 + q];
+      for (int b = lane; b < nblocks; b += 64) s += bcnt[(size_t)b * kCntRow + q];
       s = wave_sum(s);
       if (lane == 0 && s) clk->cnt[slot[q]] += s;
     }
@@ -377,6 +480,96 @@ __global__ __launch_bounds__(256) void k_dep_reduce(const double* __restrict__ p
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// Stable compaction of the live slots (w > 0).  Block b owns a contiguous run
+// of slots; k_compact_count counts its live ones, k_compact_scatter sums the
+// counts of blocks 0..b-1 itself and writes its live particles in slot order.
+// ---------------------------------------------------------------------------
+constexpr int kCompBlock = 256;
+constexpr int kCompItems = 4;
+constexpr int kCompTile = kCompBlock * kCompItems;
+
+__device__ __forceinline__ void comp_range(int64_t n, int64_t* lo, int64_t* hi) {
+  const int64_t tiles = (n + kCompTile - 1) / kCompTile;
+  const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
+  int64_t a = (int64_t)blockIdx.x * per * kCompTile;
+  int64_t b = a + per * kCompTile;
+  *lo = a < n ? a : n;
+  *hi = b < n ? b : n;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCompBlock) void k_compact_count(const T* __restrict__ w,
+                                                              const Clock* clk,
+                                                              uint32_t* block_cnt) {
+  __shared__ uint32_t s_red[kCompBlock / kWave];
+  int64_t lo, hi;
+  comp_range(clk->n, &lo, &hi);
+  uint32_t c = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kCompBlock) c += w[i] > (T)0 ? 1u : 0u;
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) block_cnt[blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCompBlock) void k_compact_scatter(bz_state_t in, bz_state_t out,
+                                                                Clock* clk,
+                                                                const uint32_t* block_cnt) {
+  constexpr int NW = kCompBlock / kWave;
+  __shared__ uint32_t s_red[NW];
+  __shared__ uint64_t s_wsum[NW];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t n = clk->n;
+  int64_t lo, hi;
+  comp_range(n, &lo, &hi);
+  uint32_t s = 0;
+  for (int j = tid; j < (int)blockIdx.x; j += kCompBlock) s += block_cnt[j];
+  s = wave_sum(s);
+  if (lane == 0) s_red[wid] = s;
+  __syncthreads();
+  int64_t pos0 = (int64_t)s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  const T* src[5] = {(const T*)in.x, (const T*)in.vx, (const T*)in.vy, (const T*)in.vz,
+                     (const T*)in.w};
+  T* dst[5] = {(T*)out.x, (T*)out.vx, (T*)out.vy, (T*)out.vz, (T*)out.w};
+  for (int64_t base = lo; base < hi; base += kCompTile) {
+    uint64_t packed = 0;
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      const int64_t i = base + k * kCompBlock + tid;
+      const bool live = i < hi && src[4][i] > (T)0;
+      packed |= (uint64_t)(live ? 1u : 0u) << (16 * k);
+    }
+    const uint64_t incl = wave_incl_scan(packed);
+    if (lane == kWave - 1) s_wsum[wid] = incl;
+    __syncthreads();
+    uint64_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      if (w < wid) before += s_wsum[w];
+      tot += s_wsum[w];
+    }
+    const uint64_t excl = before + incl - packed;
+    uint32_t field_base = 0;
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      if ((packed >> (16 * k)) & 1u) {
+        const int64_t i = base + k * kCompBlock + tid;
+        const int64_t d = pos0 + field_base + (int64_t)((excl >> (16 * k)) & 0xFFFFu);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) dst[f][d] = src[f][i];
+      }
+      field_base += (uint32_t)((tot >> (16 * k)) & 0xFFFFu);
+    }
+    pos0 += field_base;
+    __syncthreads();  // s_wsum is rewritten by the next sub-tile
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == 0) clk->n_next = pos0;
+}
+
+__global__ void k_clock_commit_next(Clock* clk) { clk->n = clk->n_next; }
 
 // Deposit of a state without pushing it (initial moments).
 template <typename T, int DIM, int BLOCK>
@@ -438,8 +631,8 @@ int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys) {
     per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
     return (int64_t)num_cu * per_cu * row_doubles;
   };
-  const int64_t hist = rows(3ll * max_nodes + 4);
-  const int64_t inst = rows((int64_t)kImN * max_keys + 4);
+  const int64_t hist = rows(3ll * max_nodes + kCntRow / 2);
+  const int64_t inst = rows((int64_t)kImN * max_keys + kCntRow / 2);
   int64_t d = hist > inst ? hist : inst;
   if (d < 2ll * (max_nodes + 1)) d = 2ll * (max_nodes + 1);
   return d;
@@ -447,7 +640,9 @@ int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys) {
 
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
-  auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS>;
+  // the 1-D discharges update in place; swarms compact
+  constexpr bool INPLACE = DIM == 1;
+  auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS, INPLACE>;
   // same layout as the kernel's dep_words (the field slots are a double each)
   const int dep_doubles = DIM == 2 ? a.n_inst * kImN + (a.n_inst + 1) + a.n_inst
                                    : 3 * (DIM == 1 ? a.p.n_nodes : 1);
@@ -464,25 +659,36 @@ static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, smem) != hipSuccess ||
       occ < 1)
     return fail("push kernel does not fit on a CU (LDS %zu B)", smem);
-  // every block resident: occupancy-sized grid, launched cooperatively
+  // every block resident: occupancy-sized grid
   int grid = env.num_cu * (occ > 8 ? 8 : occ);
-  // partial histograms [grid][part] doubles followed by [grid][8] u32 counters
-  const int64_t cap_parts = env.partials_doubles / (part_doubles + 4);
+  // partial histograms [grid][part] doubles followed by [grid][kCntRow] u32 counters
+  const int64_t cap_parts = env.partials_doubles / (part_doubles + kCntRow / 2);
   if (grid > cap_parts) grid = (int)cap_parts;
+  if (INPLACE && grid > 2 * env.status_words) grid = (int)(2 * env.status_words);
   if (grid < 1) return fail("deposit partial buffer too small");
   a.sched = env.push_sched;
+  a.block_births = (uint32_t*)env.status;
   void* args[] = {&a};
   timing_begin(env, 0);
   hipError_t e = hipSuccess;
-  if (a.sched == 0)
+  if (!INPLACE && a.sched == 0)
+    // look-back waits on other blocks' tiles: every block must be resident
     e = hipLaunchCooperativeKernel((const void*)kern, dim3(grid), dim3(BLOCK), args,
                                    (unsigned)smem, env.stream);
   else
     hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
   timing_end(env, 0);
   if (e != hipSuccess) return fail("k_push cooperative launch: %s", hipGetErrorString(e));
+  int rc = check_launch("k_push");
+  if (rc) return rc;
+  if (INPLACE) {
+    hipLaunchKernelGGL(k_births_append<T>, dim3(grid), dim3(256), 0, env.stream, a,
+                       (int64_t)(BLOCK * ITEMS));
+    rc = check_launch("k_births_append");
+    if (rc) return rc;
+  }
   *grid_out = grid;
-  return check_launch("k_push");
+  return 0;
 }
 
 template <typename T, int XS, int DIM, int RNG>
@@ -582,6 +788,28 @@ static int launch_deposit_t(const LaunchEnv& env, const bz_state_t* s, const bz_
                      (const double*)env.partials, grid, 3 * nn, moments, env.clock, 0,
                      (int64_t)env.capacity);
   return check_launch("k_dep_reduce");
+}
+
+template <typename T>
+static int launch_compact_t(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out) {
+  int grid = env.num_cu * 8;
+  if (grid > 2 * env.status_words) grid = (int)(2 * env.status_words);
+  uint32_t* cnt = (uint32_t*)env.status;
+  hipLaunchKernelGGL(k_compact_count<T>, dim3(grid), dim3(kCompBlock), 0, env.stream,
+                     (const T*)in->w, (const Clock*)env.clock, cnt);
+  int rc = check_launch("k_compact_count");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_compact_scatter<T>, dim3(grid), dim3(kCompBlock), 0, env.stream, *in,
+                     *out, env.clock, (const uint32_t*)cnt);
+  rc = check_launch("k_compact_scatter");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_clock_commit_next, dim3(1), dim3(1), 0, env.stream, env.clock);
+  return check_launch("k_clock_commit_next");
+}
+
+int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out) {
+  return env.precision == BZ_F64 ? launch_compact_t<double>(env, in, out)
+                                 : launch_compact_t<float>(env, in, out);
 }
 
 int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,

@@ -2,9 +2,14 @@
 
 PyTorch supplies device memory and the stream; every stage runs in the
 hand-written HIP kernels of libbz.so.  The particle state lives in two SoA
-buffers (ping-pong); the live count, step and time live in the library's
-device clock, so a step is enqueued without any host synchronisation and can
-be captured in a hipGraph.
+buffers; the slot count, step and time live in the library's device clock,
+so a step is enqueued without any host synchronisation and can be captured
+in a hipGraph.
+
+1-D runs push in place: an absorbed particle leaves a hole (w = 0) and the
+secondaries are appended after the slots in use.  Resampling rewrites the
+state without holes; otherwise the state is compacted every
+`compact_every` steps.  0-D runs compact in every push.
 """
 from __future__ import annotations
 
@@ -41,7 +46,8 @@ class KineticEngine:
     """One run of the electron kinetic step on one GPU."""
 
     def __init__(self, cfg: ph.RunConfig, device: int = 0, capacity: Optional[int] = None,
-                 rng_mode: int = _lib.BZ_RNG_PHILOX, max_keys: Optional[int] = None):
+                 rng_mode: int = _lib.BZ_RNG_PHILOX, max_keys: Optional[int] = None,
+                 compact_every: int = 16):
         if not torch.cuda.is_available():
             raise _lib.BzError("no GPU visible: the electron step runs only in the HIP kernels")
         self.lib = _lib.load()
@@ -60,6 +66,9 @@ class KineticEngine:
         _lib.check(self.lib.bz_set_stream(self.ctx, C.c_void_p(self.stream.cuda_stream)))
         self.bufs = [DeviceState(self.capacity, self.tdtype, self.dev) for _ in range(2)]
         self.cur = 0
+        self.inplace = cfg.dim == 1
+        self.compact_every = int(compact_every)
+        self._since_compact = 0
         # collision model
         self.table = None
         self.xs = None
@@ -129,19 +138,39 @@ class KineticEngine:
         return n.value, s.value, t.value, list(cnt)
 
     def count(self) -> int:
+        """Slots in use (1-D: live particles and holes)."""
         return self.clock()[0]
+
+    def live(self) -> int:
+        """Live particles (w > 0); synchronises."""
+        n = self.count()
+        return int((self.state.arrays[4][:n] > 0).sum().item())
 
     def counters(self) -> dict:
         cnt = self.clock()[3]
         return {k: int(cnt[i]) for i, k in enumerate(_lib.COUNTER_NAMES)}
 
-    def fetch_state(self):
+    def fetch_slots(self):
+        """All slots in use, holes (w = 0) included: the arrays the next push
+        reads (its random stream is keyed by slot index)."""
         n = self.count()
         return tuple(a[:n].cpu().numpy() for a in self.state.arrays)
 
+    def fetch_state(self):
+        """Live particles in slot order (holes dropped): survivors, then the
+        secondaries of the last pushes in parent order."""
+        n = self.count()
+        arrs = [a[:n].cpu().numpy() for a in self.state.arrays]
+        live = arrs[4] > 0
+        if live.all():
+            return tuple(arrs)
+        return tuple(np.ascontiguousarray(a[live]) for a in arrs)
+
     # ------------------------------------------------------------------ stages
     def push(self, R: Optional["torch.Tensor"] = None, G: Optional["torch.Tensor"] = None):
-        """Fused collision + push + deposit + compaction into the other buffer."""
+        """Fused collision + push + deposit: in place with births appended
+        (1-D, the other buffer stages the births), or compacting into the
+        other buffer (0-D)."""
         p = self.params
         if R is not None:
             p = _lib.Params.from_buffer_copy(self.params)
@@ -153,7 +182,18 @@ class KineticEngine:
             self.ctx, C.byref(src.struct), C.byref(dst.struct), _tptr(E), C.byref(p),
             C.byref(self.xs) if self.xs is not None else None, _tptr(R), _tptr(G),
             _tptr(self.moments)), "bz_push_mcc")
+        if self.inplace:
+            self._since_compact += 1
+        else:
+            self.cur = 1 - self.cur
+
+    def compact(self):
+        """Remove the holes (w = 0), keeping slot order, into the other buffer."""
+        src, dst = self.bufs[self.cur], self.bufs[1 - self.cur]
+        _lib.check(self.lib.bz_compact(self.ctx, C.byref(src.struct), C.byref(dst.struct)),
+                   "bz_compact")
         self.cur = 1 - self.cur
+        self._since_compact = 0
 
     def deposit(self):
         _lib.check(self.lib.bz_deposit(self.ctx, C.byref(self.state.struct), C.byref(self.params),
@@ -203,6 +243,7 @@ class KineticEngine:
                                         _tptr(self.targets), self.n_keys,
                                         C.byref(self.policy)), "bz_resample")
         self.cur = 1 - self.cur
+        self._since_compact = 0  # the resampled state holds no holes
         return True
 
     def initialize_fields(self):
@@ -220,6 +261,8 @@ class KineticEngine:
             self.resample()
         self.push()
         self.field_step(advance=True)
+        if self.inplace and self.compact_every and self._since_compact >= self.compact_every:
+            self.compact()
 
     def synchronize(self):
         _lib.check(self.lib.bz_synchronize(self.ctx))

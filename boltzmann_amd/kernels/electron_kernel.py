@@ -3,12 +3,14 @@
 The reference refreshes cp_R_vec with uniforms, runs collisions + push on the
 AoS rows in place, appends each ionization's secondary at Nc + slot and
 leaves the number of births in d_curr_count[0].  Here the same stage runs in
-the fused HIP kernel (reference argon fits, direct collision probability):
-cp_R_vec / d_R_vec receive the uniforms used (Philox stream), the AoS rows
-[0, n_out) receive the compacted survivors with each secondary right after
-its parent, rows [n_out, Nc + births) are zeroed, and d_curr_count[0] is the
-number of births, so the reference's post-processing compaction is a no-op.
-The node moments of the pushed state are kept for post_processing.
+the fused HIP kernel (reference argon fits, direct collision probability),
+which leaves the rows the same way: cp_R_vec / d_R_vec receive the uniforms
+used (Philox stream), rows [0, Nc) are pushed in place (an absorbed row gets
+w = 0, x = 1e-8 and v = 0, electron_kernel_numba.py:266-277), the secondaries
+fill rows [Nc, Nc + births) in parent order (the reference's atomic slots in
+one admissible order), and d_curr_count[0] is the number of births.
+post_processing then compacts the rows as the reference does.  The node
+moments of the pushed state are kept for post_processing.
 """
 from __future__ import annotations
 
@@ -97,22 +99,32 @@ def electron_kernel(Nc, d_data_ar, d_tosum_ar, d_collct_ar, d_E_ar, cp_R_vec, d_
         nout = C.c_int64()
         _lib.check(s.lib.bz_clock_get(s.ctx, C.byref(nout), None, None, c1))
         births = int(c1[_lib.COUNTER_NAMES.index("births")] - c0[_lib.COUNTER_NAMES.index("births")])
-        m = nout.value
-        # write back the compacted state in the reference layout
-        d_data_ar[:m, 0].copy_(b[0][:m])
+        m = nout.value  # n + births (capped at the capacity)
+        # rows [0, n) in place, births at [n, m), in the reference layout
+        d_data_ar[:m, 0].copy_(a[0][:m])
         d_data_ar[:m, 1:3].zero_()
         for k, col in ((1, 3), (2, 4), (3, 5)):
-            d_data_ar[:m, col].copy_(b[k][:m])
-        d_tosum_ar[:m, 0].copy_(b[4][:m])
+            d_data_ar[:m, col].copy_(a[k][:m])
+        d_tosum_ar[:m, 0].copy_(a[4][:m])
         d_tosum_ar[:m, 1:3].zero_()
-        v2 = b[1][:m] * b[1][:m] + b[2][:m] * b[2][:m] + b[3][:m] * b[3][:m]
+        dead = a[4][:m] <= 0
+        d_data_ar[:m, 0].masked_fill_(dead, 1e-8)
+        for col in (3, 4, 5):
+            d_data_ar[:m, col].masked_fill_(dead, 0.0)
+        v2 = d_data_ar[:m, 3] ** 2 + d_data_ar[:m, 4] ** 2 + d_data_ar[:m, 5] ** 2
         d_tosum_ar[:m, 3].copy_(p.kev * v2)
-        end = min(n + births, cap)
-        if end > m:
-            d_data_ar[m:end].zero_()
-            d_tosum_ar[m:end].zero_()
         births_total += births
         s.step += 1
+        if int(dt_ratio) > 1:
+            # the next sub-step pushes the compacted rows
+            alive = torch.nonzero(d_tosum_ar[:m, 0] > 0).flatten()
+            k = int(alive.numel())
+            d_data_ar[:k] = d_data_ar[alive]
+            d_tosum_ar[:k] = d_tosum_ar[alive]
+            if m > k:
+                d_data_ar[k:m].zero_()
+                d_tosum_ar[k:m].zero_()
+            m = k
         n = m
     d_curr_count[0] = births_total
     s.cached[(d_data_ar.data_ptr(), Nc + births_total)] = (s.moments.clone(), n)

@@ -2,9 +2,10 @@
 
 Returns (ne_ar, ni_src, ns_src, nrg_ar, counter_g0..g3, Nnew).  After
 electron_kernel the node moments were already summed by the fused HIP
-kernel and the rows compacted, so they are returned as computed there; for a
-state that was not pushed (the set-up deposit, boltzmann.py:363-371) the
-deposit runs in bz_deposit and the zero-weight rows are compacted out.
+kernel, so they are returned as computed there and the rows are compacted
+(w > 0, stable: survivors, then secondaries) as reshuffle_data_atomics does
+(post_processing_1D.py:26-38); for a state that was not pushed (the set-up
+deposit, boltzmann.py:363-371) the deposit runs in bz_deposit.
 The collision-counter reduction (post_processing_1D.py:56) accumulates an
 array the reference never resets or writes out; its counters come back as
 zeros.
@@ -32,7 +33,14 @@ def post_processing(data_out, data_out_np, data_out2, data_out_np2, data_out3, d
     s = S.session(dev, _lib.BZ_F64, cap, RN)
     key = (big_data_ar.data_ptr(), Nc + nni)
     if key in s.cached:
-        mom, nnew = s.cached.pop(key)
+        mom, m = s.cached.pop(key)
+        alive = torch.nonzero(big_tosum_ar[:m, 0] > 0).flatten()
+        nnew = int(alive.numel())
+        if need_to_reshuffle and nnew < m:
+            big_data_ar[:nnew] = big_data_ar[alive]
+            big_tosum_ar[:nnew] = big_tosum_ar[alive]
+            big_data_ar[nnew:m].zero_()
+            big_tosum_ar[nnew:m].zero_()
     else:
         bnf = int(big_N) // RN
         if bnf != _deposit_bins["bnf"]:

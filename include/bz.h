@@ -200,18 +200,31 @@ int bz_timing_enable(bz_ctx_t ctx, int on);
 int bz_timing_get(bz_ctx_t ctx, double* ms3, int64_t* launches3);
 
 /* Fused electron step: MCC collision (elastic / excitation / ionization with
- * a secondary born beside its parent), leapfrog push in E(x), absorbing walls,
- * node deposit of (sum w, sum w*ionized, sum w*energy) and stream compaction.
- *   in  -> out: survivors in input order; each secondary directly follows its
- *               parent.  Count read from / written to the clock.
+ * a secondary), push in E(x), absorbing walls and node deposit of
+ * (sum w, sum w*ionized, sum w*energy).  The slot count is read from and
+ * written to the clock.
+ *   1-D (dim 1), in place, as the reference's electron_kernel leaves its
+ *     arrays: `in` slots [0, n) are updated where they lie, an absorbed
+ *     particle's slot keeps w = 0 (a hole, skipped by every stage), and the
+ *     secondaries are appended at [n, n + births) in parent order.  `out` is
+ *     scratch (births are staged there) and its contents are clobbered.
+ *     bz_compact removes the holes (the reference's post_processing
+ *     reshuffle, kernels/post_processing_1D.py:26-38).
+ *   0-D (dim 0), in -> out: survivors in input order, each secondary
+ *     directly after its parent.
  *   E          : [n_cells] field, V/cm, context precision (NULL in 0-D: e_field0)
  *   moments    : [3][n_nodes] FP64 output (overwritten)
  * Replaces electron_kernel (kernels/electron_kernel_numba.py:291-303) plus the
- * deposit and compaction of post_processing (kernels/post_processing_1D.py:40-68,
+ * deposit of post_processing (kernels/post_processing_1D.py:40-68,
  * kernels/reduction/cuda_reduction/kernels-new.cu:56-132). */
 int bz_push_mcc(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
                 const void* E, const bz_params_t* p, const bz_xsec_t* xs,
                 const void* R, const void* gas_v, double* moments);
+
+/* Stable compaction: the particles of `in` with w > 0, in slot order, to
+ * `out`; the clock's count becomes the live count.  Replaces
+ * reshuffle_data_atomics (kernels/post_processing_1D.py:26-38). */
+int bz_compact(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out);
 
 /* The collision uniforms the Philox stream draws for particles 0..n-1 at
  * `step`, written as R[c*stride + i] (columns 0,1,2,3,5 used, 4 and 6 zero)

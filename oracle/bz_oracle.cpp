@@ -115,8 +115,9 @@ typedef struct {
   int32_t dim, xs_mode, n_cells, n_nodes, bnf, nthreads;
   uint64_t seed;  // Philox key of the free-flight draws (xs_mode 2)
   uint64_t step;  // step number keying those draws
-  int32_t push_order;  // 0: x then v (reference), 1: v then x (leapfrog)
-  int32_t pad;
+  int32_t push_order;   // 0: x then v (reference), 1: v then x (leapfrog)
+  int32_t births_last;  // 1: secondaries after all survivors (reference layout),
+                        // 0: each secondary right after its parent
 } orc_params;
 
 typedef struct {
@@ -503,7 +504,8 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
   std::vector<int64_t> chunk_cnt(nt + 1, 0);
   std::vector<std::vector<double>> part(nt, std::vector<double>(3 * nn, 0.0));
   std::vector<std::vector<uint64_t>> ccnt(nt, std::vector<uint64_t>(8, 0));
-  std::vector<std::vector<Rec<T>>> buf(nt);
+  std::vector<std::vector<Rec<T>>> buf(nt), sec(nt);
+  const bool births_last = P->births_last != 0;
 #pragma omp parallel for num_threads(nt) schedule(static, 1)
   for (int k = 0; k < nt; ++k) {
     const int64_t lo = n * k / nt, hi = n * (k + 1) / nt;
@@ -514,11 +516,12 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
     for (int64_t i = lo; i < hi; ++i) {
       Rec<T> o[2];
       T eo[2];
-      int chan, ion, m;
+      int chan, ion, m, born;
       if (P->xs_mode == 2) {
         Events ev;
         m = one_particle_flight<T>(i, P->step, *P, x, vx, vy, vz, w, E, tab, o, eo, &ev);
         ion = ev.alive && ev.ion;
+        born = ev.ion;
         if (w[i] > (T)0) {
           cc[2] += ev.el;
           cc[3] += ev.exc;
@@ -529,6 +532,7 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
         }
       } else {
         m = one_particle<T>(i, *P, x, vx, vy, vz, w, E, R, rs, G, tab, o, eo, &chan, &ion);
+        born = chan == 3;
         if (w[i] > (T)0) {
           cc[chan == 0 ? 5 : (chan == 1 ? 2 : (chan == 2 ? 3 : 4))] += 1;
           if (chan == 3) cc[0] += 1;
@@ -536,35 +540,42 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
           if (!alive_primary) cc[1] += 1;
         }
       }
+      // the secondary, when one was born, is the last record
       for (int q = 0; q < m; ++q) {
         const int nd = node_of((double)o[q].x, *P);
         const double wd = (double)o[q].w;
         pm[nd] += wd;
         if (q == 0 && ion) pm[nn + nd] += wd;
         pm[2 * nn + nd] += wd * (double)eo[q];
-        b.push_back(o[q]);
+        const bool is_sec = births_last && born && q == m - 1;
+        (is_sec ? sec[k] : b).push_back(o[q]);
       }
     }
     chunk_cnt[k + 1] = (int64_t)b.size();
   }
   for (int k = 0; k < nt; ++k) chunk_cnt[k + 1] += chunk_cnt[k];
-  const int64_t total = chunk_cnt[nt];
+  const int64_t n_surv = chunk_cnt[nt];
+  std::vector<int64_t> sec_cnt(nt + 1, 0);
+  for (int k = 0; k < nt; ++k) sec_cnt[k + 1] = sec_cnt[k] + (int64_t)sec[k].size();
+  const int64_t total = n_surv + sec_cnt[nt];
   uint64_t dropped = 0;
+  auto put = [&](int64_t pos, const Rec<T>& r) {
+    if (pos < cap) {
+      ox[pos] = r.x;
+      ovx[pos] = r.vx;
+      ovy[pos] = r.vy;
+      ovz[pos] = r.vz;
+      ow[pos] = r.w;
+      return 0;
+    }
+    return 1;
+  };
 #pragma omp parallel for num_threads(nt) schedule(static, 1) reduction(+ : dropped)
   for (int k = 0; k < nt; ++k) {
     int64_t pos = chunk_cnt[k];
-    for (const Rec<T>& r : buf[k]) {
-      if (pos < cap) {
-        ox[pos] = r.x;
-        ovx[pos] = r.vx;
-        ovy[pos] = r.vy;
-        ovz[pos] = r.vz;
-        ow[pos] = r.w;
-      } else {
-        ++dropped;
-      }
-      ++pos;
-    }
+    for (const Rec<T>& r : buf[k]) dropped += put(pos++, r);
+    pos = n_surv + sec_cnt[k];
+    for (const Rec<T>& r : sec[k]) dropped += put(pos++, r);
   }
   for (int j = 0; j < 3 * nn; ++j) {
     double s = 0.0;

@@ -26,7 +26,7 @@ class OrcParams(C.Structure):
         "e_ion", "e_exc", "pc", "e_field0", "nu_max")] + [
         (n, C.c_int32) for n in ("dim", "xs_mode", "n_cells", "n_nodes", "bnf", "nthreads")] + [
         ("seed", C.c_uint64), ("step", C.c_uint64), ("push_order", C.c_int32),
-        ("pad", C.c_int32)]
+        ("births_last", C.c_int32)]
 
 
 class OrcTable(C.Structure):
@@ -90,6 +90,9 @@ class Setup:
     seed: int = 12345          # free-flight mode draws its own Philox stream
     step: int = 0
     push_order: int = 0        # 0 reference (x then v), 1 leapfrog (v then x)
+    # secondaries after all survivors (the 1-D layout) or right after their
+    # parents (the 0-D swarm layout); None: by dimension
+    births_last: Optional[bool] = None
 
     def params(self) -> OrcParams:
         p = OrcParams()
@@ -123,6 +126,8 @@ class Setup:
         p.seed = self.seed
         p.step = self.step
         p.push_order = self.push_order
+        bl = self.births_last if self.births_last is not None else self.dim == 1
+        p.births_last = 1 if bl else 0
         return p
 
 

@@ -119,6 +119,49 @@ def test_uniform_field_trajectory(order):
     np.testing.assert_allclose(st[0], x0 + 100 * dt * (k * v0 - a * dt * S), rtol=1e-12)
 
 
+@pytest.mark.parametrize("xs_mode", [1, 2])
+def test_births_last_layout(xs_mode):
+    """The 1-D layout (survivors in slot order, then every secondary in
+    parent order, as the reference's electron_kernel appends them at Nc and
+    its compaction keeps them after the survivors) is a reordering of the
+    parent-adjacent layout: survivors are the records not preceded by an
+    ionizing parent's record."""
+    n = 6000
+    t, tab = _table(np.float64, gas=3.2e24)
+    kw = dict(dim=1, dt=2e-12, xs_mode=xs_mode, n_cells=16, length_cm=2.5, nu_max=t.nu_max,
+              gas_density_m3=3.2e24)
+    rng = np.random.default_rng(4)
+    sp = math.sqrt(40.0 / KEV)
+    d = rng.normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    st = (rng.uniform(0.2, 2.3, n), *(sp * d), rng.uniform(0.5, 2, n))
+    E = np.linspace(-30, 30, 16)
+    R = rng.uniform(0, 1, size=(7, n))
+    a, ma, ca = O.push_step(O.Setup(**kw, births_last=False), st, E, R, table=tab)
+    b, mb, cb = O.push_step(O.Setup(**kw, births_last=True), st, E, R, table=tab)
+    assert np.array_equal(ca, cb) and cb[0] > 50
+    np.testing.assert_array_equal(ma, mb)
+    births = int(cb[0])
+    n_surv = a[0].size - births
+    # secondaries start at the parent's pre-push (or collision) x with its
+    # weight; in the adjacent layout they are the records whose weight and
+    # position repeat the previous parent's, so split by the births_last
+    # layout instead: its tail is exactly the births
+    tail = np.stack([v[n_surv:] for v in b])
+    head = np.stack([v[:n_surv] for v in b])
+    A = np.stack(a)
+    # every record of the births_last layout appears in the adjacent layout
+    # in the same relative order within each group
+    def subseq(sub, full):
+        j = 0
+        for col in range(full.shape[1]):
+            if j < sub.shape[1] and np.array_equal(full[:, col], sub[:, j]):
+                j += 1
+        return j == sub.shape[1]
+    assert subseq(head, A) and subseq(tail, A)
+    assert head.shape[1] + tail.shape[1] == A.shape[1]
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_elastic_scatter_geometry(dtype):
     """R0 = 0 forces the first channel: speed scales by sqrt(1-2 m_e/M),
@@ -153,7 +196,7 @@ def test_channel_energy_balance(dtype):
     n = 20000
     t, tab = _table(dtype, gas=3.2e24)
     s = O.Setup(dim=1, dt=1e-12, n_cells=16, length_cm=2.5, nu_max=t.nu_max,
-                gas_density_m3=3.2e24)
+                gas_density_m3=3.2e24, births_last=False)
     rng = np.random.default_rng(11)
     sp = math.sqrt(40.0 / KEV)
     dirs = rng.normal(size=(3, n))

@@ -168,7 +168,10 @@ def test_flight_parity(gpu, name, dim, prec, order):
         rt = 1e-12 if prec == 8 else 1e-6
         np.testing.assert_allclose(mom, wmom, rtol=rt, atol=rt * np.abs(wmom).max())
         base = dict(cnt)
-        cur = want
+        # the next step reads the slots as they lie (1-D: holes included;
+        # the random stream is keyed by slot index)
+        cur = eng.fetch_slots()
+        np.testing.assert_array_equal(cur[4][cur[4] > 0], want[4])
     eng.close()
 
 
@@ -229,20 +232,24 @@ def test_empty_and_capacity(gpu):
     eng.load_state(tuple(np.zeros(0, dtype=np.float32) for _ in range(5)))
     eng.push()
     assert eng.count() == 0
-    # all particles ionize (R0 = 0 below the ionization threshold sum is elastic,
-    # so use energies where the stream forces ionization through R0 just below r2)
-    n = 4096
-    st = _state(n, np.float32, 1, seed=9, dead_frac=0.0)
-    eng.load_state(st)
-    R = np.random.default_rng(3).uniform(0, 1, size=(7, n)).astype(np.float32)
-    eng.push(R=torch.from_numpy(R).to(eng.dev))
-    c = eng.counters()
-    n_out = eng.count()
-    assert n_out <= 4096
-    want, _, wcnt = O.push_step(_setup(cfg, eng.table), st, np.zeros(48, np.float32), R,
-                                table=_tab(eng.table), capacity=4096)
-    assert n_out == want[0].size
-    assert c["overflow"] == int(wcnt[6])
+    # in place: the births are appended after the n slots; those beyond the
+    # capacity are dropped (and counted), the rest keep parent order
+    for n in (4096, 4092):
+        st = _state(n, np.float32, 1, seed=9, dead_frac=0.0)
+        eng.load_state(st)
+        eng.lib.bz_counters_reset(eng.ctx)
+        R = np.random.default_rng(3).uniform(0, 1, size=(7, n)).astype(np.float32)
+        eng.push(R=torch.from_numpy(R).to(eng.dev))
+        c = eng.counters()
+        want, _, wcnt = O.push_step(_setup(cfg, eng.table), st, np.zeros(48, np.float32), R,
+                                    table=_tab(eng.table), capacity=2 * n + 16)
+        births = int(wcnt[0])
+        kept = max(0, min(births, 4096 - n))
+        assert births > kept  # the capacity is reached
+        assert eng.count() == n + kept
+        assert c["overflow"] == births - kept and c["births"] == births
+        n_keep = want[0].size - births + kept
+        _compare_state(eng.fetch_state(), tuple(a[:n_keep] for a in want), np.float32)
     eng.close()
 
 
