@@ -1,9 +1,10 @@
-"""Tile-shape autotuner for the fused push kernel.
+"""Tile-shape and grid autotuner for the fused push kernel.
 
 Each candidate shape (block size x particles per thread) is run for a few
 real steps of the workload and timed with the library's HIP events around
-the push kernel; the fastest shape is kept on the context.  Steps taken
-here advance the simulation like warm-up steps.
+the push kernel; the fastest shape is kept on the context.  For the 1-D
+push the grid is tuned next: blocks per CU from 1 up to what fits.  Steps
+taken here advance the simulation like warm-up steps.
 """
 from __future__ import annotations
 
@@ -13,7 +14,7 @@ from typing import Callable
 from . import _lib
 
 # shape index -> (threads per block, particles per thread), as compiled in bz_push.hip
-PUSH_SHAPES = {0: (256, 4), 1: (512, 2), 2: (256, 2), 3: (1024, 1)}
+PUSH_SHAPES = {0: (256, 4), 1: (512, 2), 2: (256, 2), 3: (1024, 1), 4: (1024, 2)}
 
 
 def push_time_ms(lib, ctx) -> tuple[float, int]:
@@ -23,26 +24,46 @@ def push_time_ms(lib, ctx) -> tuple[float, int]:
     return ms[0], int(nl[0])
 
 
-def tune_shapes(lib, ctx, step: Callable[[], None], steps: int = 10) -> dict:
-    """Time `steps` calls of `step` per push shape; keep the fastest."""
+GRID_BPC = (0, 1, 2, 4)  # blocks per CU of the 1-D push (0: as many as fit)
+
+
+def _time(lib, ctx, step: Callable[[], None], steps: int) -> float:
+    step()  # the first call of a setting may set kernel attributes
+    _lib.check(lib.bz_timing_enable(ctx, 1))
+    for _ in range(steps):
+        step()
+    ms, n = push_time_ms(lib, ctx)
+    _lib.check(lib.bz_timing_enable(ctx, 0))
+    return ms / max(n, 1)
+
+
+def tune_shapes(lib, ctx, step: Callable[[], None], steps: int = 10,
+                grids: bool = False) -> dict:
+    """Time `steps` calls of `step` per push shape; keep the fastest.  With
+    `grids`, then time the best shape per blocks-per-CU setting."""
     results = {}
     used = 0
+    _lib.check(lib.bz_set_push_grid(ctx, 0))
     for shape in PUSH_SHAPES:
         _lib.check(lib.bz_set_push_shape(ctx, shape))
-        step()  # first call of a shape may set kernel attributes
-        used += 1
-        _lib.check(lib.bz_timing_enable(ctx, 1))
-        for _ in range(steps):
-            step()
-            used += 1
-        ms, n = push_time_ms(lib, ctx)
-        _lib.check(lib.bz_timing_enable(ctx, 0))
-        results[shape] = ms / max(n, 1)
+        results[shape] = _time(lib, ctx, step, steps)
+        used += steps + 1
     best = min(results, key=results.get)
     _lib.check(lib.bz_set_push_shape(ctx, best))
-    return {"best": best, "shape": PUSH_SHAPES[best],
-            "avg_ms": {str(PUSH_SHAPES[k]): v for k, v in results.items()},
-            "steps_used": used}
+    out = {"best": best, "shape": PUSH_SHAPES[best],
+           "avg_ms": {str(PUSH_SHAPES[k]): v for k, v in results.items()}}
+    if grids:
+        gres = {0: results[best]}
+        for bpc in GRID_BPC[1:]:
+            _lib.check(lib.bz_set_push_grid(ctx, bpc))
+            gres[bpc] = _time(lib, ctx, step, steps)
+            used += steps + 1
+        gbest = min(gres, key=gres.get)
+        _lib.check(lib.bz_set_push_grid(ctx, gbest))
+        out["blocks_per_cu"] = gbest
+        out["grid_avg_ms"] = {str(k): v for k, v in gres.items()}
+    out["steps_used"] = used
+    return out
 
 
 def tune_push(engine, steps: int = 10, start_index: int = 0) -> dict:
@@ -53,7 +74,7 @@ def tune_push(engine, steps: int = 10, start_index: int = 0) -> dict:
         engine.step(index[0])
         index[0] += 1
 
-    return tune_shapes(engine.lib, engine.ctx, step, steps)
+    return tune_shapes(engine.lib, engine.ctx, step, steps, grids=engine.cfg.dim == 1)
 
 
 def tune_swarms(batch, steps: int = 10) -> dict:

@@ -100,9 +100,11 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   e.max_nodes = max_nodes;
   e.max_keys = max_keys;
   e.push_shape = 0;
+  e.push_bpc = 0;
   // smallest push tile is 256 particles
-  // look-back tile words, or the per-block birth counts of the in-place push
-  e.status_words = capacity / 256 + 4096;
+  // look-back tile words, or the birth counts of the in-place 1-D push (u32:
+  // 4096 per block, then one per tile and wave: at most capacity/64 + 64)
+  e.status_words = capacity / 128 + 4096;
   e.partials_doubles = push_partials_doubles(e.num_cu, max_nodes, max_keys);
   e.sort_ws_bytes = (int64_t)sort_workspace_bytes(capacity, max_keys);
   hipError_t st = hipMalloc((void**)&e.clock, sizeof(Clock));
@@ -162,8 +164,15 @@ int bz_set_stream(bz_ctx_t c, void* stream) {
 
 int bz_set_push_shape(bz_ctx_t c, int shape) {
   if (!c) return fail("null context");
-  if (shape < 0 || shape > 3) return fail("push shape must be 0..3");
+  if (shape < 0 || shape > 4) return fail("push shape must be 0..4");
   c->env.push_shape = shape;
+  return 0;
+}
+
+int bz_set_push_grid(bz_ctx_t c, int blocks_per_cu) {
+  if (!c) return fail("null context");
+  if (blocks_per_cu < 0 || blocks_per_cu > 8) return fail("blocks per CU must be 0..8");
+  c->env.push_bpc = blocks_per_cu;
   return 0;
 }
 

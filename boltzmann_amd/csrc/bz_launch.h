@@ -27,6 +27,7 @@ struct LaunchEnv {
   int32_t max_keys;
   int push_shape;            // tile shape picked by the autotuner
   int push_sched;            // 0 static tiles + cooperative launch, 1 ticket counter
+  int push_bpc;              // 1-D push: blocks per CU (0: as many as fit)
   // optional kernel timing: event pairs per timed kernel class
   struct Timing* timing;
 };

@@ -49,6 +49,9 @@ struct PushArgs {
 // per-block counter row of the push (u32): births, absorbed, elastic,
 // excitation, ionization, null, dropped, nu_max exceeded, live inputs
 constexpr int kCntRow = 10;
+// 1-D push: per-block birth counts, then per-(tile, wave) counts, in the
+// context's status words
+constexpr int kMaxPushBlocks = 4096;
 
 // per-instance moments of a batched step (FP64, per block then summed)
 enum : int {
@@ -91,10 +94,30 @@ __host__ __device__ inline size_t table_lds_bytes(int n, int n_hint, int word) {
   return ((size_t)7 * n * word + (size_t)n_hint * 4 + 15) & ~(size_t)15;
 }
 
-template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS, bool INPLACE>
+// 1-D push: the table follows the [3][nn] histograms at a 16-byte boundary
+__host__ __device__ inline int table_lds_offset_doubles(int nn) { return (3 * nn + 1) & ~1; }
+
+// Copy m words global -> LDS with 16-byte accesses where both ends allow
+// (the device tables are 256-byte aligned allocations; dst is 16-byte
+// aligned by layout), scalar for the rest.
+template <typename T, int BLOCK>
+__device__ __forceinline__ void stage_lds(T* dst, const T* __restrict__ src, int m) {
+  constexpr int V = 16 / sizeof(T);
+  const bool vec = ((((uintptr_t)dst) | ((uintptr_t)src)) & 15) == 0;
+  int done = 0;
+  if (vec) {
+    const int nv = m / V;
+    const int4* s4 = (const int4*)src;
+    int4* d4 = (int4*)dst;
+    for (int j = threadIdx.x; j < nv; j += BLOCK) d4[j] = s4[j];
+    done = nv * V;
+  }
+  for (int j = done + threadIdx.x; j < m; j += BLOCK) dst[j] = src[j];
+}
+
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   static_assert(ITEMS >= 1 && ITEMS <= 4, "packed scan holds 4 fields of 16 bits");
-  static_assert(!INPLACE || DIM == 1, "the in-place layout is the 1-D discharge's");
   constexpr int NW = BLOCK / kWave;
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int LB = 1024 / BLOCK > 0 ? 1024 / BLOCK : 1;  // look-back window 1024 tiles
@@ -150,15 +173,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   const uint64_t step = a.clk->step;
   const uint32_t epoch = a.clk->epoch;
   const uint32_t ntiles = (uint32_t)((n + TILE - 1) / TILE);
-  if (!INPLACE && blockIdx.x == 0 && tid == 0 && n == 0) a.clk->n_next = 0;
-  // in-place mode: this block's contiguous run of tiles [t, t_end)
-  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
-  const uint32_t t_first = INPLACE ? (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)ntiles)
-                                   : 0u;
-  const uint32_t t_end = INPLACE ? (t_first + per < ntiles ? t_first + per : ntiles) : ntiles;
-  T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
-  const int64_t stage0 = (int64_t)t_first * TILE;  // births staged from here, in order
-  uint32_t bsum = 0;                               // births of this block so far
+  if (blockIdx.x == 0 && tid == 0 && n == 0) a.clk->n_next = 0;
 
   const T* in[5] = {(const T*)a.in[0], (const T*)a.in[1], (const T*)a.in[2], (const T*)a.in[3],
                     (const T*)a.in[4]};
@@ -175,25 +190,19 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   // every block is resident).  sched 1: tiles claimed from a ticket counter by
   // running blocks (any residency).  Either way a tile's look-back only waits
   // on tiles owned by blocks that are running, so every wait ends.
-  uint32_t t;
-  if (INPLACE) {
-    t = t_first;
-  } else {
-    if (tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : blockIdx.x;
-    __syncthreads();
-    t = s_next;
-    __syncthreads();  // s_next is rewritten at the top of the loop
-  }
+  if (tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : blockIdx.x;
+  __syncthreads();
+  uint32_t t = s_next;
+  __syncthreads();  // s_next is rewritten at the top of the loop
   P5<T> cur[ITEMS];
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k)
-    load5<T, WITHX>(in, (int64_t)t * TILE + k * BLOCK + tid, t < t_end ? n : 0, cur[k]);
+    load5<T, WITHX>(in, (int64_t)t * TILE + k * BLOCK + tid, n, cur[k]);
 
-  while (t < t_end) {
+  while (t < ntiles) {
     const int64_t base = (int64_t)t * TILE;
     // claim the next tile now; it is read after the scan barriers below
-    if (!INPLACE && tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : t + gridDim.x;
-    uint32_t live_in = 0;  // bit k: item k held a live particle
+    if (tid == 0) s_next = a.sched ? atomicAdd(&a.clk->ticket, 1u) : t + gridDim.x;
 
     StepOut<T> o[ITEMS];
     uint64_t packed = 0;
@@ -218,7 +227,6 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       else
         o[k] = step_particle<T, XS, DIM, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, e_local);
       if (i < n && cur[k].w > (T)0) {
-        live_in |= 1u << k;
         ++c_live;
         c_el += o[k].n_el;
         c_ex += o[k].n_exc;
@@ -227,7 +235,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
         c_abs += !o[k].alive;
         c_over += o[k].over;
       }
-      const uint32_t cnt = INPLACE ? (uint32_t)o[k].born : (uint32_t)(o[k].alive + o[k].born);
+      const uint32_t cnt = (uint32_t)(o[k].alive + o[k].born);
       packed |= (uint64_t)cnt << (16 * k);
       if (DIM == 2) {
         // per-instance moments of this step; a wave whose lanes share one
@@ -292,24 +300,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       }
     }
 
-    if (INPLACE) {
-      // survivors back to their own slot (w unchanged), absorbed ones emptied
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        if (!((live_in >> k) & 1u)) continue;
-        const int64_t i = base + (int64_t)k * BLOCK + tid;
-        if (o[k].alive) {
-          st[0][i] = o[k].p.x;
-          st[1][i] = o[k].p.vx;
-          st[2][i] = o[k].p.vy;
-          st[3][i] = o[k].p.vz;
-        } else {
-          st[4][i] = (T)0;
-        }
-      }
-    }
     // block exclusive scan of the packed counts, in index order k-major
-    // (compacting: outputs per particle; in place: births per particle)
     const uint64_t incl = wave_incl_scan(packed);
     if (lane == kWave - 1) s_wsum[wid] = incl;
     __syncthreads();
@@ -321,7 +312,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     __syncthreads();
     const uint64_t excl = (wid ? s_wsum[wid - 1] : 0ull) + incl - packed;
     const uint64_t tot = s_wsum[NW - 1];
-    const uint32_t t_next = INPLACE ? t + 1 : s_next;
+    const uint32_t t_next = s_next;
     uint32_t field_base[ITEMS];
     uint32_t agg = 0;
 #pragma unroll
@@ -329,26 +320,10 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       field_base[k] = agg;
       agg += (uint32_t)((tot >> (16 * k)) & 0xFFFFu);
     }
-    // prefetch the next tile so its HBM latency overlaps the stores / look-back
+    // prefetch the next tile so its HBM latency overlaps the look-back
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k)
-      load5<T, WITHX>(in, (int64_t)t_next * TILE + k * BLOCK + tid, t_next < t_end ? n : 0,
-                      cur[k]);
-    if (INPLACE) {
-      // births staged in parent order after the block's earlier births
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        if (!o[k].born) continue;
-        const int64_t ps =
-            stage0 + bsum + field_base[k] + (int64_t)((excl >> (16 * k)) & 0xFFFFu);
-        store5<T, true>(a.out, ps, o[k].s);
-      }
-      bsum += agg;
-      t = t_next;
-      // s_wsum is rewritten only after the next tile's first barrier
-      __syncthreads();
-      continue;
-    }
+      load5<T, WITHX>(in, (int64_t)t_next * TILE + k * BLOCK + tid, n, cur[k]);
     const uint32_t pre = lookback_block<BLOCK, LB>(a.status, t, agg, epoch, &s_lbmin, s_lbred);
     if (tid == 0 && t == ntiles - 1) a.clk->n_next = (int64_t)(pre + agg);
     const int64_t tile_pre = pre;
@@ -371,7 +346,6 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     // look-back's barriers, which every thread has passed
   }
 
-  if (INPLACE && tid == 0) a.block_births[blockIdx.x] = bsum;
   // counters: wave sums -> LDS -> one row per block, summed by k_dep_reduce
   {
     const uint32_t v[kCntRow] = {c_io, c_abs, c_el, c_ex, c_io, c_null, c_drop, c_over, c_live, 0};
@@ -405,43 +379,6 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   for (int j = tid; j < m; j += BLOCK) part[j] = s_dep[j];
   uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * m);
   if (tid < kCntRow) bcnt[(size_t)blockIdx.x * kCntRow + tid] = (uint32_t)s_cnt[tid];
-}
-
-// In-place mode, after k_push: append the births staged by each block after
-// the live range, block after block, so they follow their parents' order.
-// Block b sums the birth counts of blocks 0..b-1 itself (no serial chain);
-// the last block sets the slot count of the new state.
-template <typename T>
-__global__ __launch_bounds__(256) void k_births_append(PushArgs a, int64_t tile) {
-  __shared__ uint32_t s_red[4];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int b = blockIdx.x;
-  const int64_t n = a.clk->n;
-  const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
-  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
-  uint32_t s = 0;
-  for (int j = tid; j < b; j += 256) s += a.block_births[j];
-  s = wave_sum(s);
-  if (lane == 0) s_red[wid] = s;
-  __syncthreads();
-  const uint64_t pre = (uint64_t)s_red[0] + s_red[1] + s_red[2] + s_red[3];
-  const uint32_t c = a.block_births[b];
-  const int64_t src0 = (int64_t)min((uint64_t)b * per, (uint64_t)ntiles) * tile;
-  T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
-  const T* sg[5] = {(const T*)a.out[0], (const T*)a.out[1], (const T*)a.out[2],
-                    (const T*)a.out[3], (const T*)a.out[4]};
-  for (uint32_t j = tid; j < c; j += 256) {
-    const int64_t dst = n + (int64_t)pre + j;
-    if (dst >= a.capacity) break;
-#pragma unroll
-    for (int f = 0; f < 5; ++f) st[f][dst] = sg[f][src0 + j];
-  }
-  if (b == (int)gridDim.x - 1 && tid == 0) {
-    const int64_t want = n + (int64_t)pre + c;
-    a.clk->n_next = want < a.capacity ? want : a.capacity;
-    if (want > a.capacity)
-      atomicAdd(&a.clk->cnt[BZ_CNT_OVERFLOW], (unsigned long long)(want - a.capacity));
-  }
 }
 
 // Sum the per-block partial histograms in block order and commit the clock.
@@ -478,6 +415,218 @@ __global__ __launch_bounds__(256) void k_dep_reduce(const double* __restrict__ p
       clk->epoch = clk->epoch + 1;
       clk->ticket = 0;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1-D discharges: the push in place, with no barrier in the tile loop.
+//
+// Block b owns tiles [t_first, t_end) (contiguous); within a tile, wave w
+// owns the WSPAN = 64*ITEMS consecutive slots starting at t*TILE + w*WSPAN,
+// item k of lane l being slot t*TILE + w*WSPAN + k*64 + l.  Every wave walks
+// its slots tile after tile on its own: the next tile's particles are loaded
+// before this tile's collision physics, survivors are written back to their
+// slots, and births are staged in the wave's own slot range of the staging
+// buffer in slot order, with the count per (tile, wave) stored for
+// k_births_append_1d.  The node histograms live in LDS (FP64 atomics) and
+// are flushed once per block after the loop.
+// ---------------------------------------------------------------------------
+template <typename T, int XS, int RNG, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
+  constexpr int NW = BLOCK / kWave;
+  constexpr int WSPAN = kWave * ITEMS;
+  constexpr int TILE = BLOCK * ITEMS;
+  extern __shared__ double s_dep[];
+  __shared__ unsigned long long s_cnt[kCntRow];
+  __shared__ uint32_t s_births;
+
+  const bz_params_t& p = a.p;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  const int nn = p.n_nodes;
+  for (int j = tid; j < 3 * nn; j += BLOCK) s_dep[j] = 0.0;
+  if (tid < kCntRow) s_cnt[tid] = 0ull;
+  if (tid == 0) s_births = 0;
+  bz_xsec_t xs = a.xs;
+  if (XS != BZ_XS_FIT) {
+    const int n = a.xs.n;
+    T* s_tab = (T*)(s_dep + table_lds_offset_doubles(nn));
+    int32_t* s_hint = (int32_t*)(s_tab + 7 * n);
+    stage_lds<T, BLOCK>(s_tab, (const T*)a.xs.energy, n);
+    stage_lds<T, BLOCK>(s_tab + n, (const T*)a.xs.sig, 3 * n);
+    stage_lds<T, BLOCK>(s_tab + 4 * n, (const T*)a.xs.slope, 3 * n);
+    for (int j = tid; j < a.xs.n_hint; j += BLOCK) s_hint[j] = a.xs.hint[j];
+    xs.energy = s_tab;
+    xs.sig = s_tab + n;
+    xs.slope = s_tab + 4 * n;
+    xs.hint = s_hint;
+  }
+
+  const int64_t n = a.clk->n;
+  const uint64_t step = a.clk->step;
+  const uint32_t ntiles = (uint32_t)((n + TILE - 1) / TILE);
+  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const uint32_t t_first = (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)ntiles);
+  const uint32_t t_end = t_first + per < ntiles ? t_first + per : ntiles;
+  const T* in[5] = {(const T*)a.in[0], (const T*)a.in[1], (const T*)a.in[2], (const T*)a.in[3],
+                    (const T*)a.in[4]};
+  T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
+  const T* __restrict__ E = (const T*)a.E;
+  const T* R = (const T*)a.R;
+  const T* G = (const T*)a.G;
+  uint32_t* wave_births = a.block_births + kMaxPushBlocks;
+
+  uint32_t c_el = 0, c_ex = 0, c_io = 0, c_abs = 0, c_null = 0, c_over = 0, c_live = 0;
+  uint32_t w_births = 0;  // this wave's births over its tiles
+  __syncthreads();  // histograms cleared, table staged
+
+  P5<T> cur[ITEMS];
+  {
+    const int64_t b0 = (int64_t)t_first * TILE + wid * WSPAN;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+      load5<T, true>(in, b0 + k * kWave + lane, t_first < t_end ? n : 0, cur[k]);
+  }
+  for (uint32_t t = t_first; t < t_end; ++t) {
+    const int64_t bw = (int64_t)t * TILE + wid * WSPAN;
+    // next tile first, so its HBM latency hides behind this tile's physics
+    P5<T> nxt[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+      load5<T, true>(in, bw + TILE + k * kWave + lane, t + 1 < t_end ? n : 0, nxt[k]);
+    uint32_t tb = 0;  // births of this wave in this tile
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int64_t i = bw + k * kWave + lane;
+      const bool live = i < n && cur[k].w > (T)0;
+      StepOut<T> o;
+      if (XS == BZ_XS_FLIGHT)
+        o = flight_particle<T, 1>(cur[k], (uint32_t)i, step, p, xs, E, (T)0);
+      else
+        o = step_particle<T, XS, 1, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, (T)0);
+      if (live) {
+        ++c_live;
+        c_el += o.n_el;
+        c_ex += o.n_exc;
+        c_io += o.chan == kChanIon;
+        c_null += o.n_null;
+        c_abs += !o.alive;
+        c_over += o.over;
+        if (o.alive) {
+          st[0][i] = o.p.x;
+          st[1][i] = o.p.vx;
+          st[2][i] = o.p.vy;
+          st[3][i] = o.p.vz;
+          const double w = (double)o.p.w;
+          const int nd = dep_node((double)o.p.x, p);
+          atomicAdd(&s_dep[nd], w);
+          if (o.chan == kChanIon) atomicAdd(&s_dep[nn + nd], w);
+          atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
+        } else {
+          st[4][i] = (T)0;
+        }
+      }
+      const bool born = live && o.born;
+      const uint64_t m = __ballot(born);
+      if (born) {
+        const uint32_t r = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        store5<T, true>(a.out, bw + tb + r, o.s);
+        const double w = (double)o.s.w;
+        const int nd = dep_node((double)o.s.x, p);
+        atomicAdd(&s_dep[nd], w);
+        atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+      }
+      tb += (uint32_t)__popcll(m);
+    }
+    if (lane == 0) wave_births[(size_t)t * NW + wid] = tb;
+    w_births += tb;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) cur[k] = nxt[k];
+  }
+
+  {
+    const uint32_t v[kCntRow] = {c_io, c_abs, c_el, c_ex, c_io, c_null, 0u, c_over, c_live, 0u};
+#pragma unroll
+    for (int q = 0; q < kCntRow; ++q) {
+      const uint32_t s = wave_sum(v[q]);
+      if (lane == 0 && s) atomicAdd(&s_cnt[q], (unsigned long long)s);
+    }
+    if (lane == 0 && w_births) atomicAdd(&s_births, w_births);
+  }
+  __syncthreads();
+  const int m = 3 * nn;
+  double* part = a.partials + (size_t)blockIdx.x * m;
+  for (int j = tid; j < m; j += BLOCK) part[j] = s_dep[j];
+  uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * m);
+  if (tid < kCntRow) bcnt[(size_t)blockIdx.x * kCntRow + tid] = (uint32_t)s_cnt[tid];
+  if (tid == 0) a.block_births[blockIdx.x] = s_births;
+}
+
+// After k_push_1d: the births of block b go after the slots in use, behind
+// the births of blocks 0..b-1 (summed here by block b itself), in the
+// order (tile, wave, slot), which is parent order.  The last block sets the
+// slot count of the new state; births past the capacity are dropped and
+// counted.
+template <typename T>
+__global__ __launch_bounds__(256) void k_births_append_1d(PushArgs a, int tile, int nw) {
+  __shared__ uint32_t s_red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int b = blockIdx.x;
+  const int64_t n = a.clk->n;
+  const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
+  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const uint32_t t_first = (uint32_t)min((uint64_t)b * per, (uint64_t)ntiles);
+  const uint32_t t_end = t_first + per < ntiles ? t_first + per : ntiles;
+  uint32_t s = 0;
+  for (int j = tid; j < b; j += 256) s += a.block_births[j];
+  s = wave_sum(s);
+  if (lane == 0) s_red[wid] = s;
+  __syncthreads();
+  const int64_t pre = (int64_t)s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  const uint32_t mine = a.block_births[b];
+  T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
+  const T* sg[5] = {(const T*)a.out[0], (const T*)a.out[1], (const T*)a.out[2],
+                    (const T*)a.out[3], (const T*)a.out[4]};
+  const uint32_t* wave_births = a.block_births + kMaxPushBlocks;
+  const int wspan = tile / nw;
+  const int64_t entries = (int64_t)(t_end - t_first) * nw;
+  int64_t run = 0;  // births of this block's earlier entries
+  if (mine) {
+    for (int64_t e0 = 0; e0 < entries; e0 += 256) {
+      const int64_t e = e0 + tid;
+      const uint32_t c = e < entries ? wave_births[(size_t)t_first * nw + e] : 0u;
+      // block exclusive scan of c
+      uint32_t incl = c;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+      }
+      __syncthreads();
+      if (lane == 63) s_red[wid] = incl;
+      __syncthreads();
+      uint32_t before = 0, tot = 0;
+      for (int w = 0; w < 4; ++w) {
+        if (w < wid) before += s_red[w];
+        tot += s_red[w];
+      }
+      const int64_t dst0 = n + pre + run + before + incl - c;
+      const int64_t src0 = (int64_t)(t_first + e / nw) * tile + (e % nw) * wspan;
+      for (uint32_t j = 0; j < c; ++j) {
+        const int64_t dst = dst0 + j;
+        if (dst >= a.capacity) break;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) st[f][dst] = sg[f][src0 + j];
+      }
+      run += tot;
+    }
+  }
+  if (b == (int)gridDim.x - 1 && tid == 0) {
+    const int64_t want = n + pre + mine;
+    a.clk->n_next = want < a.capacity ? want : a.capacity;
+    if (want > a.capacity)
+      atomicAdd(&a.clk->cnt[BZ_CNT_OVERFLOW], (unsigned long long)(want - a.capacity));
   }
 }
 
@@ -638,16 +787,12 @@ int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys) {
   return d;
 }
 
-template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
-static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
-  // the 1-D discharges update in place; swarms compact
-  constexpr bool INPLACE = DIM == 1;
-  auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS, INPLACE>;
-  // same layout as the kernel's dep_words (the field slots are a double each)
-  const int dep_doubles = DIM == 2 ? a.n_inst * kImN + (a.n_inst + 1) + a.n_inst
-                                   : 3 * (DIM == 1 ? a.p.n_nodes : 1);
-  const int part_doubles = DIM == 2 ? a.n_inst * kImN : 3 * (DIM == 1 ? a.p.n_nodes : 1);
-  size_t smem = (size_t)dep_doubles * sizeof(double);
+// 1-D discharges: k_push_1d in place, then k_births_append_1d
+template <typename T, int XS, int RNG, int BLOCK, int ITEMS>
+static int launch_push_1d_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
+  auto kern = k_push_1d<T, XS, RNG, BLOCK, ITEMS>;
+  const int part_doubles = 3 * a.p.n_nodes;
+  size_t smem = (size_t)table_lds_offset_doubles(a.p.n_nodes) * sizeof(double);
   if (XS != BZ_XS_FIT) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
   if (smem > 160 * 1024) return fail("push: deposit + table need %zu B of LDS (> 160 KiB)", smem);
   if (smem > 64 * 1024) {
@@ -659,36 +804,75 @@ static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, smem) != hipSuccess ||
       occ < 1)
     return fail("push kernel does not fit on a CU (LDS %zu B)", smem);
-  // every block resident: occupancy-sized grid
-  int grid = env.num_cu * (occ > 8 ? 8 : occ);
-  // partial histograms [grid][part] doubles followed by [grid][kCntRow] u32 counters
+  int bpc = occ > 8 ? 8 : occ;
+  if (env.push_bpc > 0 && env.push_bpc < bpc) bpc = env.push_bpc;
+  int grid = env.num_cu * bpc;
   const int64_t cap_parts = env.partials_doubles / (part_doubles + kCntRow / 2);
   if (grid > cap_parts) grid = (int)cap_parts;
-  if (INPLACE && grid > 2 * env.status_words) grid = (int)(2 * env.status_words);
+  if (grid > kMaxPushBlocks) grid = kMaxPushBlocks;
   if (grid < 1) return fail("deposit partial buffer too small");
-  a.sched = env.push_sched;
+  // per-(tile, wave) birth counts follow the per-block ones in the status words
+  const int64_t max_tiles = env.capacity / (BLOCK * ITEMS) + 2;
+  if (kMaxPushBlocks + max_tiles * (BLOCK / kWave) > 2 * env.status_words)
+    return fail("status words too few for the birth counts");
   a.block_births = (uint32_t*)env.status;
-  void* args[] = {&a};
   timing_begin(env, 0);
-  hipError_t e = hipSuccess;
-  if (!INPLACE && a.sched == 0)
-    // look-back waits on other blocks' tiles: every block must be resident
-    e = hipLaunchCooperativeKernel((const void*)kern, dim3(grid), dim3(BLOCK), args,
-                                   (unsigned)smem, env.stream);
-  else
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
   timing_end(env, 0);
-  if (e != hipSuccess) return fail("k_push cooperative launch: %s", hipGetErrorString(e));
-  int rc = check_launch("k_push");
+  int rc = check_launch("k_push_1d");
   if (rc) return rc;
-  if (INPLACE) {
-    hipLaunchKernelGGL(k_births_append<T>, dim3(grid), dim3(256), 0, env.stream, a,
-                       (int64_t)(BLOCK * ITEMS));
-    rc = check_launch("k_births_append");
-    if (rc) return rc;
-  }
+  hipLaunchKernelGGL(k_births_append_1d<T>, dim3(grid), dim3(256), 0, env.stream, a,
+                     BLOCK * ITEMS, BLOCK / kWave);
+  rc = check_launch("k_births_append_1d");
+  if (rc) return rc;
   *grid_out = grid;
   return 0;
+}
+
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
+static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
+  // the 1-D discharges update in place; swarms compact
+  if constexpr (DIM == 1) {
+    return launch_push_1d_t<T, XS, RNG, BLOCK, ITEMS>(env, a, grid_out);
+  } else {
+    auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS>;
+    // same layout as the kernel's dep_words (the field slots are a double each)
+    const int dep_doubles = DIM == 2 ? a.n_inst * kImN + (a.n_inst + 1) + a.n_inst
+                                     : 3 * (DIM == 1 ? a.p.n_nodes : 1);
+    const int part_doubles = DIM == 2 ? a.n_inst * kImN : 3 * (DIM == 1 ? a.p.n_nodes : 1);
+    size_t smem = (size_t)dep_doubles * sizeof(double);
+    if (XS != BZ_XS_FIT) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
+    if (smem > 160 * 1024) return fail("push: deposit + table need %zu B of LDS (> 160 KiB)", smem);
+    if (smem > 64 * 1024) {
+      if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)smem) != hipSuccess)
+        return fail("hipFuncSetAttribute(dynamic LDS) failed");
+    }
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, smem) != hipSuccess ||
+        occ < 1)
+      return fail("push kernel does not fit on a CU (LDS %zu B)", smem);
+    // every block resident: occupancy-sized grid
+    int grid = env.num_cu * (occ > 8 ? 8 : occ);
+    // partial histograms [grid][part] doubles followed by [grid][kCntRow] u32 counters
+    const int64_t cap_parts = env.partials_doubles / (part_doubles + kCntRow / 2);
+    if (grid > cap_parts) grid = (int)cap_parts;
+    if (grid < 1) return fail("deposit partial buffer too small");
+    a.sched = env.push_sched;
+    void* args[] = {&a};
+    timing_begin(env, 0);
+    hipError_t e = hipSuccess;
+    if (a.sched == 0)
+      // look-back waits on other blocks' tiles: every block must be resident
+      e = hipLaunchCooperativeKernel((const void*)kern, dim3(grid), dim3(BLOCK), args,
+                                     (unsigned)smem, env.stream);
+    else
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
+    timing_end(env, 0);
+    if (e != hipSuccess) return fail("k_push cooperative launch: %s", hipGetErrorString(e));
+    *grid_out = grid;
+    return check_launch("k_push");
+  }
 }
 
 template <typename T, int XS, int DIM, int RNG>
@@ -698,6 +882,7 @@ static int launch_push_shape(const LaunchEnv& env, PushArgs& a, int shape, int* 
     case 1: return launch_push_t<T, XS, DIM, RNG, 512, 2>(env, a, grid);
     case 2: return launch_push_t<T, XS, DIM, RNG, 256, 2>(env, a, grid);
     case 3: return launch_push_t<T, XS, DIM, RNG, 1024, 1>(env, a, grid);
+    case 4: return launch_push_t<T, XS, DIM, RNG, 1024, 2>(env, a, grid);
     default: return fail("unknown push shape %d", shape);
   }
 }

@@ -174,10 +174,16 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity,
                   int32_t max_nodes, int32_t max_keys);
 int bz_ctx_destroy(bz_ctx_t ctx);
 int bz_set_stream(bz_ctx_t ctx, void* hip_stream);
-/* Tile shape of the fused push (0..3), chosen by the autotuner; no
+/* Tile shape of the fused push (0..4: threads x particles per thread
+ * 256x4, 512x2, 256x2, 1024x1, 1024x2), chosen by the autotuner; no
  * counterpart in the reference (its launch shape is fixed at 1024 x 64,
  * 1bin/boltzmann.py:139-140). */
 int bz_set_push_shape(bz_ctx_t ctx, int shape);
+/* Grid of the 1-D push: at most `blocks_per_cu` blocks per CU (1..8), or
+ * as many as fit (0, default).  Fewer blocks spread the per-block set-up
+ * (table staging, histogram flush) over more particles; chosen by the
+ * autotuner.  No counterpart in the reference. */
+int bz_set_push_grid(bz_ctx_t ctx, int blocks_per_cu);
 /* Tile schedule of the fused push: 0 static tile order under a cooperative
  * launch (default), 1 tiles claimed from a ticket counter by running blocks.
  * No counterpart in the reference. */

@@ -1,0 +1,131 @@
+"""The HIP path on the inputs of the committed golden fixtures
+(tests/golden/*.npz), through the C ABI, against the outputs stored there.
+
+Integer results (event counts, births, absorptions, keys, counts, offsets,
+the sort permutation, the resampled count and selection) must be
+identical; per-particle floating point within 1e-12 (FP64) / 2e-6 (FP32) of
+the velocity scale; node moments within 1e-12 / 1e-6 relative; the grid step
+within 1e-10 relative (two triangular solves as blocked scans); the closure
+network within the tolerances of its own parity tests."""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+sys.path.insert(0, str(GOLD))
+import make_golden as G  # noqa: E402
+
+
+def load(name):
+    return np.load(GOLD / f"{name}.npz", allow_pickle=False)
+
+
+def _close_state(got, want, dtype, dim=1):
+    """x is compared in 1-D only: 0-D swarms carry no position."""
+    assert got[0].size == want.shape[1]
+    rt = 1e-12 if dtype == np.float64 else 2e-6
+    vs = np.sqrt(sum(want[k].astype(np.float64) ** 2 for k in (1, 2, 3))) + 1.0
+    for k in range(0 if dim == 1 else 1, 5):
+        g = np.asarray(got[k], dtype=np.float64)
+        w = want[k].astype(np.float64)
+        scale = vs if k in (1, 2, 3) else np.abs(w) + 1e-30
+        bad = np.flatnonzero(np.abs(g - w) > rt * scale)
+        assert bad.size == 0, (k, bad[:5], g[bad[:5]], w[bad[:5]])
+
+
+@pytest.mark.parametrize("name", list(G.PUSH_CASES))
+def test_push_fixture_on_gpu(gpu, name):
+    import torch
+    from boltzmann_amd.engine import KineticEngine
+    f = load(name)
+    cfg = G.push_config(name)
+    dtype = cfg.np_dtype
+    eng = KineticEngine(cfg, capacity=4 * G.N_PUSH)
+    eng.load_state(tuple(f[k] for k in ("x", "vx", "vy", "vz", "w")))
+    if cfg.dim == 1:
+        eng.E_t.copy_(torch.from_numpy(f["E"]))
+    stream = G.PUSH_CASES[name][4] == "stream"
+    R = torch.from_numpy(f["R"]).to(eng.dev) if stream else None
+    Gv = torch.from_numpy(f["G"]).to(eng.dev) if f["G"].size else None
+    eng.push(R=R, G=Gv)
+    c = eng.counters()
+    want = f["counters"]
+    assert c["births"] == want[0] and c["absorbed"] == want[1]
+    assert (c["elastic"], c["excitation"], c["ionization"], c["null"]) == tuple(want[2:6])
+    _close_state(eng.fetch_state(), f["out"], dtype, cfg.dim)
+    rt = 1e-12 if dtype == np.float64 else 1e-6
+    mom = eng.moments.cpu().numpy().reshape(3, -1)
+    np.testing.assert_allclose(mom, f["moments"], rtol=rt, atol=rt * np.abs(f["moments"]).max())
+    eng.close()
+
+
+@pytest.mark.parametrize("nb", [3, 5])
+def test_resample_fixture_on_gpu(gpu, nb):
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    f = load(f"resample_{nb}bin_f64")
+    st = tuple(f[k] for k in ("x", "vx", "vy", "vz", "w"))
+    n = st[0].size
+    cfg = ph.RunConfig(name="golden", dim=1, n_particles=n, precision=8, nb=nb, n_cells=16,
+                       length_cm=G.L, dt=1e-11, target_ppc=100, seed=G.SEED,
+                       bin_fractions=ph.bin_fractions(nb), bin_thresholds=ph.bin_thresholds(nb))
+    eng = KineticEngine(cfg, capacity=4 * n + 1024)
+    np.testing.assert_array_equal(eng.targets_host, f["targets"])
+    eng.load_state(st, step=3)
+    eng.bin_sort()
+    nk = eng.n_keys
+    np.testing.assert_array_equal(eng.counts.cpu().numpy()[:nk + 1], f["counts"])
+    np.testing.assert_array_equal(eng.offsets.cpu().numpy()[:nk + 2], f["offsets"])
+    np.testing.assert_array_equal(eng.perm.cpu().numpy()[:n].astype(np.uint32), f["perm"])
+    assert eng.resample(force=True)
+    c = eng.counters()
+    assert c["split"] == f["counters"][8] and c["merge"] == f["counters"][9]
+    got = eng.fetch_state()
+    np.testing.assert_array_equal(got[0], f["out"][0])  # copied positions: same selection
+    _close_state(got, f["out"], np.float64)
+    eng.close()
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_field_fixture_on_gpu(gpu, scheme):
+    import torch
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    f = load(f"field_scheme{scheme}")
+    inp = G.field_inputs()
+    N = inp["N"]
+    cfg = ph.RunConfig(name="golden", dim=1, n_particles=16, n_cells=N, length_cm=G.L,
+                       dt=inp["dt"], gas_density_m3=3.2e22, rf_voltage=inp["v0"],
+                       rf_freq=inp["freq"], ion_scheme=scheme, max_substeps=256)
+    eng = KineticEngine(cfg, capacity=1024)
+    fp = eng.fp
+    fp.mu_i, fp.D_i, fp.D_s = inp["mu_i"], inp["D_i"], inp["D_s"]
+    eng.moments.copy_(torch.from_numpy(f["mom"]))
+    eng.ni.copy_(torch.from_numpy(f["ni_in"]))
+    eng.E.copy_(torch.from_numpy(f["E_in"]))
+    eng.lib.bz_clock_set(eng.ctx, 0, 0, inp["t"] - inp["dt"])
+    eng.field_step(advance=True)
+    for k, arr in (("ni", eng.ni), ("Te", eng.Te), ("g2", eng.g2), ("V", eng.V), ("E", eng.E)):
+        want = f[k]
+        got = arr.cpu().numpy()
+        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-10 * np.abs(want).max(),
+                                   err_msg=k)
+    eng.close()
+
+
+def test_mlp_fixture_on_gpu(gpu):
+    from boltzmann_amd.closure import ClosureNet
+    f = load("mlp")
+    inp = G.mlp_inputs()
+    net = ClosureNet(seed=0)
+    net.norm = inp["norm"]
+    net.set_parameters(f["master"])
+    np.testing.assert_array_equal(net.bf16_bits(), f["bits"])
+    y = net.predict(f["x"])
+    tol = 2e-5 * inp["norm"]["out_scale"] + 1e-6 * np.abs(f["y"])
+    assert np.all(np.abs(y - f["y"]) <= tol)
+    net.close()

@@ -175,35 +175,45 @@ def test_flight_parity(gpu, name, dim, prec, order):
     eng.close()
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 def test_schedules_and_shapes_agree(gpu, shape):
-    """Static (cooperative) and ticket tile schedules, for every tile shape,
-    give bit-identical states and counters; the FP64 node moments agree to
-    rounding (blocks sum different tiles, in LDS atomic order)."""
+    """Every tile shape and grid gives bit-identical slots and counters: the
+    1-D push at any blocks-per-CU setting (against shape 0 at the full
+    grid), the compacting 0-D push under the static (cooperative) and the
+    ticket schedule.  The FP64 node moments agree to rounding (blocks sum
+    different particles, in LDS atomic order)."""
+    import torch
     from boltzmann_amd import _lib
     from boltzmann_amd import physics as ph
     from boltzmann_amd.engine import KineticEngine
     n = 70_000
-    cfg = _cfg(ph, 1, 4, 1, n, gas=3.2e24)
-    outs = []
-    for sched in (0, 1):
+
+    def run(dim, shape_, sched, bpc):
+        cfg = _cfg(ph, dim, 4, 1, n, gas=3.2e24)
         eng = KineticEngine(cfg, capacity=2 * n + 64)
-        _lib.check(eng.lib.bz_set_push_shape(eng.ctx, shape))
+        _lib.check(eng.lib.bz_set_push_shape(eng.ctx, shape_))
         _lib.check(eng.lib.bz_set_push_schedule(eng.ctx, sched))
-        eng.load_state(_state(n, np.float32, 1, seed=77))
-        import torch
-        eng.E_t.copy_(torch.linspace(-300, 300, cfg.n_cells))
+        _lib.check(eng.lib.bz_set_push_grid(eng.ctx, bpc))
+        eng.load_state(_state(n, np.float32, dim, seed=77))
+        if dim == 1:
+            eng.E_t.copy_(torch.linspace(-300, 300, cfg.n_cells))
         for _ in range(3):
             eng.push()
-            eng.field_step(advance=False)
+            if dim == 1:
+                eng.field_step(advance=False)
             _lib.check(eng.lib.bz_clock_advance(eng.ctx, cfg.dt))
-        outs.append((eng.fetch_state(), eng.moments.cpu().numpy(), eng.counters()))
+        out = (eng.fetch_slots(), eng.moments.cpu().numpy(), eng.counters())
         eng.close()
-    (a, ma, ca), (b, mb, cb) = outs
-    for u, v in zip(a, b):
-        np.testing.assert_array_equal(u, v)
-    np.testing.assert_allclose(ma, mb, rtol=1e-14, atol=1e-14 * np.abs(ma).max())
-    assert ca == cb
+        return out
+
+    for dim, runs in ((1, [(0, 0, 0), (shape, 0, 0), (shape, 0, 1), (shape, 0, 3)]),
+                      (0, [(shape, 0, 0), (shape, 1, 0)])):
+        (a, ma, ca), *rest = [run(dim, *r) for r in runs]
+        for b, mb, cb in rest:
+            for u, v in zip(a, b):
+                np.testing.assert_array_equal(u, v)
+            np.testing.assert_allclose(mb, ma, rtol=1e-14, atol=1e-14 * np.abs(ma).max())
+            assert cb == ca
 
 
 @pytest.mark.parametrize("prec", [4, 8])
