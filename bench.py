@@ -116,11 +116,43 @@ def cpu_baseline_swarms(scfg, seconds: float = 10.0):
                       f"oracle push+MCC (FP64 like the reference, OpenMP {cores} threads)"}
 
 
-def _roofline(bytes_per_pstep, psteps_per_launch, avg_s, kernel):
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "traffic.json")
+
+
+def measured_traffic(workload: str, kernel: str, psteps_per_launch: float):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs recorded in profiles/traffic.json (per particle-step,
+    already scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    if rec.get("kernel") != kernel:
+        return None
+    per = rec["fetch_bytes_per_particle_step"] + rec["write_bytes_per_particle_step"]
+    return per * psteps_per_launch
+
+
+def copy_probe(mib: int) -> None:
+    """Three device-to-device copies of `mib` MiB: a kernel of known traffic."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+
+
+def _roofline(bytes_per_pstep, psteps_per_launch, avg_s, kernel, workload=None):
     achieved = bytes_per_pstep * psteps_per_launch / avg_s / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
-            "kernel_avg_us": avg_s * 1e6, "bytes_per_particle_step": bytes_per_pstep}
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(workload, kernel, psteps_per_launch) if workload else None,
+            "kernel": kernel, "kernel_avg_us": avg_s * 1e6,
+            "bytes_per_particle_step": bytes_per_pstep}
 
 
 def bench_engine(args):
@@ -182,7 +214,8 @@ def bench_engine(args):
         "config": {"workload": cfg.name, "particles_start": cfg.n_particles,
                    "particles_end": eng.count(), "cells": cfg.n_cells if cfg.dim == 1 else 0,
                    "dt_s": cfg.dt, "parallelism": "single GPU", "push_schedule": args.sched},
-        "roofline": _roofline(bytes_per_pstep, psteps / args.steps, avg_push_s, "k_push"),
+        "roofline": _roofline(bytes_per_pstep, psteps / args.steps, avg_push_s,
+                              "k_push_1d" if cfg.dim == 1 else "k_push", cfg.name),
         "counters": {k: c1[k] - c0[k] for k in ("births", "absorbed", "ionization", "overflow",
                                                 "resample", "split", "merge")},
         "kernel_ms": {"push": ms[0], "sort": ms[1], "resample": ms[2],
@@ -388,6 +421,9 @@ def main(argv=None) -> int:
     ap.add_argument("--shape", type=int, default=-1, help="push tile shape (-1: autotune)")
     ap.add_argument("--sched", type=int, default=0,
                     help="push tile schedule: 0 static + cooperative launch, 1 ticket counter")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device to device 3 times: the plain "
+                         "copy kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counts")
     ap.add_argument("--closure-samples", type=int, default=1_000_000)
     ap.add_argument("--closure-epochs", type=int, default=30)
     args = ap.parse_args(argv)
@@ -402,6 +438,8 @@ def main(argv=None) -> int:
         out = bench_closure(args)
     else:
         out = bench_engine(args)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
     print(json.dumps(out), flush=True)
     return 0
 
