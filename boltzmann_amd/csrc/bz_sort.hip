@@ -94,40 +94,74 @@ struct KeyArgs {
   int n_keys;  // live keys; dead particles get key n_keys
 };
 
+// Key per slot.  Each thread takes kKeyItems slots, all loads issued
+// before use.  The per-key counts are read off the sorted keys afterwards
+// (k_key_counts), so this pass needs neither atomics nor an LDS histogram.
+constexpr int kKeyItems = 4;
+
 template <typename T>
-__global__ __launch_bounds__(kSortBlock) void k_keys(KeyArgs a, const Clock* clk, uint32_t* keys,
-                                                     int32_t* counts) {
-  extern __shared__ int32_t s_cnt[];  // n_keys + 1
-  for (int j = threadIdx.x; j <= a.n_keys; j += kSortBlock) s_cnt[j] = 0;
-  __syncthreads();
+__global__ __launch_bounds__(kSortBlock) void k_keys(KeyArgs a, const Clock* clk, uint32_t* keys) {
   const int64_t n = clk->n;
+  const int lane = threadIdx.x & 63;
+  const int64_t base = (int64_t)blockIdx.x * kSortBlock * kKeyItems + (threadIdx.x - lane) * kKeyItems;
+  if (base >= n) return;
   const T* x = (const T*)a.x;
   const T* vx = (const T*)a.vx;
   const T* vy = (const T*)a.vy;
   const T* vz = (const T*)a.vz;
   const T* w = (const T*)a.w;
-  for (int64_t i = (int64_t)blockIdx.x * kSortBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kSortBlock) {
-    uint32_t key;
-    if (!(w[i] > (T)0)) {
-      key = (uint32_t)a.n_keys;
-    } else {
-      int cell = 0;
-      if (a.dim == 1) {
-        cell = (int)(((double)x[i] + 0.5 * a.dx) / a.dx);
-        cell = cell < 0 ? 0 : (cell >= a.n_nodes ? a.n_nodes - 1 : cell);
-      }
-      const T e = (T)a.kev * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
-      int b = 0;
-      for (int k = 0; k < a.nb - 1; ++k) b += (e >= (T)a.thr[k]) ? 1 : 0;
-      key = (uint32_t)(cell * a.nb + b);
-    }
-    keys[i] = key;
-    atomicAdd(&s_cnt[key], 1);
+  T qx[kKeyItems], qa[kKeyItems], qb[kKeyItems], qc[kKeyItems], qw[kKeyItems];
+#pragma unroll
+  for (int k = 0; k < kKeyItems; ++k) {
+    const int64_t i = base + k * kWave + lane;
+    const bool v = i < n;
+    qw[k] = v ? w[i] : (T)0;
+    qx[k] = v && a.dim == 1 ? x[i] : (T)0;
+    qa[k] = v ? vx[i] : (T)0;
+    qb[k] = v ? vy[i] : (T)0;
+    qc[k] = v ? vz[i] : (T)0;
   }
-  __syncthreads();
-  for (int j = threadIdx.x; j <= a.n_keys; j += kSortBlock)
-    if (s_cnt[j]) atomicAdd(&counts[j], s_cnt[j]);
+#pragma unroll
+  for (int k = 0; k < kKeyItems; ++k) {
+    const int64_t i = base + k * kWave + lane;
+    uint32_t key = 0xFFFFFFFFu;  // past the end: not counted
+    if (i < n) {
+      if (!(qw[k] > (T)0)) {
+        key = (uint32_t)a.n_keys;
+      } else {
+        int cell = 0;
+        if (a.dim == 1) {
+          cell = (int)(((double)qx[k] + 0.5 * a.dx) / a.dx);
+          cell = cell < 0 ? 0 : (cell >= a.n_nodes ? a.n_nodes - 1 : cell);
+        }
+        const T e = (T)a.kev * (qa[k] * qa[k] + qb[k] * qb[k] + qc[k] * qc[k]);
+        int b = 0;
+        for (int q = 0; q < a.nb - 1; ++q) b += (e >= (T)a.thr[q]) ? 1 : 0;
+        key = (uint32_t)(cell * a.nb + b);
+      }
+      keys[i] = key;
+    }
+  }
+}
+
+// Per-key counts from the sorted keys: the first position of every run of
+// equal keys finds the run's end by binary search and writes its length
+// (keys that do not occur keep the zero written before).
+__global__ __launch_bounds__(256) void k_key_counts(const uint32_t* __restrict__ skeys,
+                                                    const Clock* clk, int32_t* counts) {
+  const int64_t n = clk->n;
+  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * 256) {
+    const uint32_t k = skeys[s];
+    if (s > 0 && skeys[s - 1] == k) continue;
+    int64_t lo = s + 1, hi = n;  // first position in [s+1, n] with a larger key
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (skeys[mid] > k) hi = mid;
+      else lo = mid + 1;
+    }
+    counts[k] = (int32_t)(lo - s);
+  }
 }
 
 __global__ void k_zero_i32(int32_t* p, int64_t m) {
@@ -319,7 +353,7 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
                     uint32_t* perm) {
   const int n_nodes = p->dim == 1 ? p->n_nodes : 1;
   const int n_keys = n_nodes * nb;
-  if (n_keys + 1 > 40000) return fail("bin_sort: at most 39999 keys (LDS histogram)");
+  if (n_keys + 1 > 65535) return fail("bin_sort: at most 65534 keys (two 8-bit passes)");
   const int64_t cap = env.capacity;
   SortWs ws = carve(env.sort_ws, cap, env.max_keys);
   KeyArgs ka;
@@ -338,19 +372,13 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
   timing_begin(env, 1);
   hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, counts,
                      (int64_t)n_keys + 1);
-  const size_t smem = (size_t)(n_keys + 1) * sizeof(int32_t);
   auto kk = env.precision == BZ_F64 ? k_keys<double> : k_keys<float>;
-  if (smem > 64 * 1024) {
-    if (hipFuncSetAttribute((const void*)kk, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem) != hipSuccess)
-      return fail("hipFuncSetAttribute(keys LDS) failed");
-  }
-  hipLaunchKernelGGL(kk, dim3(env.num_cu * 2), dim3(kSortBlock), smem, env.stream, ka,
-                     (const Clock*)env.clock, ws.keys_a, counts);
+  // one thread per kKeyItems slots of the capacity; blocks past the count exit
+  const int64_t kgrid = (cap + (int64_t)kSortBlock * kKeyItems - 1) / ((int64_t)kSortBlock * kKeyItems);
+  hipLaunchKernelGGL(kk, dim3((unsigned)(kgrid > 0 ? kgrid : 1)), dim3(kSortBlock), 0, env.stream,
+                     ka, (const Clock*)env.clock, ws.keys_a);
   int rc = check_launch("k_keys");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, env.stream, (const int32_t*)counts,
-                     offsets, n_keys + 1);
   // passes: 8 bits each over key range [0, n_keys]
   const int passes = (n_keys + 1) <= 256 ? 1 : 2;
   const int64_t tiles = ws.max_tiles;
@@ -382,8 +410,14 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
     kin = kout;
     vin = vout;
   }
+  // counts of every key from the sorted keys, then their exclusive scan
+  hipLaunchKernelGGL(k_key_counts, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
+                     (const uint32_t*)kin, (const Clock*)env.clock, counts);
+  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, env.stream, (const int32_t*)counts,
+                     offsets, n_keys + 1);
+  rc = check_launch("key counts");
   timing_end(env, 1);
-  return 0;
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -516,35 +550,57 @@ __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* cou
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void copy_particle(const ResArgs& a, uint32_t src, int64_t dst) {
-#pragma unroll
-  for (int f = 0; f < 5; ++f) ((T*)a.out[f])[dst] = ((const T*)a.in[f])[src];
-}
+// Copy segments (and the unmerged tail of clipped merges), per sorted
+// position.  Each thread takes kCopyItems consecutive positions, with every
+// load of a level (keys and permutation, then the plan, then the particle
+// words) issued before any of it is used, so the gather's dependent loads
+// overlap instead of running one particle at a time.
+constexpr int kCopyItems = 4;
 
-// Copy segments (and the unmerged tail of clipped merges), per sorted position.
 template <typename T>
 __global__ __launch_bounds__(256) void k_res_copy(ResArgs a, const uint32_t* skeys,
                                                   const int32_t* offsets, const uint32_t* perm,
                                                   const int32_t* plan, const int32_t* counts,
                                                   const Clock* clk, int64_t cap) {
   const int64_t n_live = offsets[a.n_keys];
-  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < n_live;
-       s += (int64_t)gridDim.x * 256) {
-    const int k = key_at(skeys, s);
-    const int kind = plan[4 * k];
-    const int64_t j = s - offsets[k];
-    int64_t dst = -1;
-    if (kind == kCopy) {
-      dst = plan[4 * k + 2] + j;
-    } else if (kind == kMerge) {
-      const int c = counts[k];
-      const int ce = (a.pol.max_ppc > 0 && c > a.pol.max_ppc) ? a.pol.max_ppc : c;
-      const int keep = plan[4 * k + 1] - (c - ce);
-      if (j >= ce) dst = plan[4 * k + 2] + keep + (j - ce);
-    }
-    if (dst >= 0 && dst < cap) copy_particle<T>(a, perm[s], dst);
+  const int64_t s0 = (int64_t)blockIdx.x * 256 * kCopyItems + threadIdx.x;
+  if (s0 >= n_live) return;
+  int key[kCopyItems];
+  uint32_t src[kCopyItems];
+#pragma unroll
+  for (int k = 0; k < kCopyItems; ++k) {
+    const int64_t s = s0 + k * 256;
+    key[k] = s < n_live ? key_at(skeys, s) : -1;
+    src[k] = s < n_live ? perm[s] : 0u;
   }
+  int64_t dst[kCopyItems];
+#pragma unroll
+  for (int k = 0; k < kCopyItems; ++k) {
+    dst[k] = -1;
+    if (key[k] < 0) continue;
+    const int kk = key[k];
+    const int kind = plan[4 * kk];
+    const int64_t j = s0 + k * 256 - offsets[kk];
+    if (kind == kCopy) {
+      dst[k] = plan[4 * kk + 2] + j;
+    } else if (kind == kMerge) {
+      const int c = counts[kk];
+      const int ce = (a.pol.max_ppc > 0 && c > a.pol.max_ppc) ? a.pol.max_ppc : c;
+      const int keep = plan[4 * kk + 1] - (c - ce);
+      if (j >= ce) dst[k] = plan[4 * kk + 2] + keep + (j - ce);
+    }
+    if (dst[k] >= cap) dst[k] = -1;
+  }
+  T v[kCopyItems][5];
+#pragma unroll
+  for (int k = 0; k < kCopyItems; ++k)
+#pragma unroll
+    for (int f = 0; f < 5; ++f) v[k][f] = dst[k] >= 0 ? ((const T*)a.in[f])[src[k]] : (T)0;
+#pragma unroll
+  for (int k = 0; k < kCopyItems; ++k)
+    if (dst[k] >= 0)
+#pragma unroll
+      for (int f = 0; f < 5; ++f) ((T*)a.out[f])[dst[k]] = v[k][f];
 }
 
 // block-wide helpers for the per-segment kernels (256 threads)
@@ -852,9 +908,10 @@ static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const b
                      perm, targets, ws.plan);
   hipLaunchKernelGGL(k_res_plan, dim3(1), dim3(1024), 0, env.stream, a, counts, targets, ws.plan,
                      nb, env.clock);
-  hipLaunchKernelGGL(k_res_copy<T>, dim3(env.num_cu * 4), dim3(256), 0, env.stream, a, skeys,
-                     offsets, perm, (const int32_t*)ws.plan, counts, (const Clock*)env.clock,
-                     cap);
+  const int64_t cgrid = (cap + 256 * kCopyItems - 1) / (256 * kCopyItems);
+  hipLaunchKernelGGL(k_res_copy<T>, dim3((unsigned)(cgrid > 0 ? cgrid : 1)), dim3(256), 0,
+                     env.stream, a, skeys, offsets, perm, (const int32_t*)ws.plan, counts,
+                     (const Clock*)env.clock, cap);
   hipLaunchKernelGGL(k_res_split<T>, dim3(n_keys), dim3(256), (size_t)kSplitLds * 16,
                      env.stream, a, counts, offsets,
                      perm, targets, (const int32_t*)ws.plan, ws, (const Clock*)env.clock, cap);
