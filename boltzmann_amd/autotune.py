@@ -1,10 +1,11 @@
 """Tile-shape and grid autotuner for the fused push kernel.
 
 Each candidate shape (block size x particles per thread) is run for a few
-real steps of the workload and timed with the library's HIP events around
-the push kernel; the fastest shape is kept on the context.  For the 1-D
-push the grid is tuned next: blocks per CU from 1 up to what fits.  Steps
-taken here advance the simulation like warm-up steps.
+real steps of the workload and timed with HIP events; the fastest is kept on
+the context.  For the 1-D push the grid is tuned next, blocks per CU from 1
+up to what fits, timing whole steps (the grid also sets how many partial
+histograms the deposit reduction sums).  Steps taken here advance the
+simulation like warm-up steps.
 """
 from __future__ import annotations
 
@@ -27,8 +28,19 @@ def push_time_ms(lib, ctx) -> tuple[float, int]:
 GRID_BPC = (0, 1, 2, 4)  # blocks per CU of the 1-D push (0: as many as fit)
 
 
-def _time(lib, ctx, step: Callable[[], None], steps: int) -> float:
+def _time(lib, ctx, step: Callable[[], None], steps: int, whole: bool = False) -> float:
+    """Average ms of the push kernel (library events), or of whole steps
+    (events on the current stream around the steps)."""
     step()  # the first call of a setting may set kernel attributes
+    if whole:
+        import torch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / steps
     _lib.check(lib.bz_timing_enable(ctx, 1))
     for _ in range(steps):
         step()
@@ -53,10 +65,10 @@ def tune_shapes(lib, ctx, step: Callable[[], None], steps: int = 10,
     out = {"best": best, "shape": PUSH_SHAPES[best],
            "avg_ms": {str(PUSH_SHAPES[k]): v for k, v in results.items()}}
     if grids:
-        gres = {0: results[best]}
-        for bpc in GRID_BPC[1:]:
+        gres = {}
+        for bpc in GRID_BPC:
             _lib.check(lib.bz_set_push_grid(ctx, bpc))
-            gres[bpc] = _time(lib, ctx, step, steps)
+            gres[bpc] = _time(lib, ctx, step, steps, whole=True)
             used += steps + 1
         gbest = min(gres, key=gres.get)
         _lib.check(lib.bz_set_push_grid(ctx, gbest))

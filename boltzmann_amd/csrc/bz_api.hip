@@ -27,13 +27,13 @@ int check_launch(const char* what) {
 void timing_begin(const LaunchEnv& env, int kind) {
   Timing* t = env.timing;
   if (!t || !t->on || t->used[kind] >= t->cap) return;
-  hipEventRecord(t->start[kind][t->used[kind]], env.stream);
+  (void)hipEventRecord(t->start[kind][t->used[kind]], env.stream);
 }
 
 void timing_end(const LaunchEnv& env, int kind) {
   Timing* t = env.timing;
   if (!t || !t->on || t->used[kind] >= t->cap) return;
-  hipEventRecord(t->stop[kind][t->used[kind]], env.stream);
+  (void)hipEventRecord(t->stop[kind][t->used[kind]], env.stream);
   t->used[kind] += 1;
   t->launches[kind] += 1;
 }
@@ -106,11 +106,17 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   // 4096 per block, then one per tile and wave: at most capacity/64 + 64)
   e.status_words = capacity / 128 + 4096;
   e.partials_doubles = push_partials_doubles(e.num_cu, max_nodes, max_keys);
+  const int64_t dep_m = dep_max_row(max_nodes, max_keys);
+  e.dep_cols = (int)((dep_m + 63) / 64);
+  e.dep_l2_doubles = 16 * dep_m;
   e.sort_ws_bytes = (int64_t)sort_workspace_bytes(capacity, max_keys);
   hipError_t st = hipMalloc((void**)&e.clock, sizeof(Clock));
   if (st == hipSuccess) st = hipMalloc((void**)&e.status, e.status_words * sizeof(uint64_t));
   if (st == hipSuccess) st = hipMalloc((void**)&e.partials, e.partials_doubles * sizeof(double));
   if (st == hipSuccess) st = hipMalloc(&e.sort_ws, e.sort_ws_bytes);
+  if (st == hipSuccess) st = hipMalloc((void**)&e.dep_l2, e.dep_l2_doubles * sizeof(double));
+  if (st == hipSuccess) st = hipMalloc((void**)&e.dep_cols_done, e.dep_cols * sizeof(uint32_t));
+  if (st == hipSuccess) st = hipMemset(e.dep_cols_done, 0, e.dep_cols * sizeof(uint32_t));
   if (st == hipSuccess) st = hipHostMalloc((void**)&c->host_clock, sizeof(Clock));
   if (st == hipSuccess) st = hipMemset(e.clock, 0, sizeof(Clock));
   if (st == hipSuccess) st = hipMemset(e.status, 0, e.status_words * sizeof(uint64_t));
@@ -133,13 +139,15 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
 
 int bz_ctx_destroy(bz_ctx_t c) {
   if (!c) return 0;
-  hipSetDevice(c->device);
-  hipDeviceSynchronize();
-  if (c->env.clock) hipFree(c->env.clock);
-  if (c->env.status) hipFree(c->env.status);
-  if (c->env.partials) hipFree(c->env.partials);
-  if (c->env.sort_ws) hipFree(c->env.sort_ws);
-  if (c->host_clock) hipHostFree(c->host_clock);
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  if (c->env.clock) (void)hipFree(c->env.clock);
+  if (c->env.status) (void)hipFree(c->env.status);
+  if (c->env.partials) (void)hipFree(c->env.partials);
+  if (c->env.sort_ws) (void)hipFree(c->env.sort_ws);
+  if (c->env.dep_l2) (void)hipFree(c->env.dep_l2);
+  if (c->env.dep_cols_done) (void)hipFree(c->env.dep_cols_done);
+  if (c->host_clock) (void)hipHostFree(c->host_clock);
   graph_release(c);
   if (Timing* t = c->env.timing) {
     for (int k = 0; k < kTimedKinds; ++k) {

@@ -8,7 +8,6 @@
 
 namespace bz {
 
-constexpr int kFieldBlock = 1024;
 
 // Bernoulli function B(x) = x / (e^x - 1), B(0) = 1.
 __device__ __forceinline__ double bernoulli(double x) {
@@ -27,6 +26,7 @@ __device__ __forceinline__ double bernoulli(double x) {
 // drift outflow otherwise (n_0 = n_1).  Metastables diffuse with the
 // reference's central flux and zero wall density, on the same sub-steps.
 // Uses LDS: a, b, Gi, Gs [N each] (the Poisson scratch, not yet in use).
+template <int BLOCK>
 __device__ void sg_heavies(const bz_field_params_t& fp, int N, const double* E, double* ni,
                            double* ns, double* lds) {
   __shared__ double s_rate;
@@ -38,7 +38,7 @@ __device__ void sg_heavies(const bz_field_params_t& fp, int N, const double* E, 
   double* Gi = lds + 2 * N;
   double* Gs = lds + 3 * N;
   const double dd = Di / dx;
-  for (int i = tid; i < N; i += kFieldBlock) {
+  for (int i = tid; i < N; i += BLOCK) {
     const double pe = mu * E[i] * dx / Di;
     a[i] = dd * bernoulli(-pe);
     b[i] = dd * bernoulli(pe);
@@ -47,7 +47,7 @@ __device__ void sg_heavies(const bz_field_params_t& fp, int N, const double* E, 
   __syncthreads();
   // largest diagonal rate over interior nodes (max is order independent)
   double r = 0.0;
-  for (int i = 1 + tid; i < N; i += kFieldBlock) r = fmax(r, (a[i] + b[i - 1]) / dx);
+  for (int i = 1 + tid; i < N; i += BLOCK) r = fmax(r, (a[i] + b[i - 1]) / dx);
   r = fmax(r, 2.0 * Ds / (dx * dx));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
@@ -75,12 +75,12 @@ __device__ void sg_heavies(const bz_field_params_t& fp, int N, const double* E, 
       ns[N] = 0.0;
     }
     __syncthreads();
-    for (int i = tid; i < N; i += kFieldBlock) {
+    for (int i = tid; i < N; i += BLOCK) {
       Gi[i] = a[i] * ni[i] - b[i] * ni[i + 1];
       Gs[i] = -Ds * (ns[i + 1] - ns[i]) / dx;
     }
     __syncthreads();
-    for (int i = 1 + tid; i < N; i += kFieldBlock) {
+    for (int i = 1 + tid; i < N; i += BLOCK) {
       ni[i] = ni[i] - h * (Gi[i] - Gi[i - 1]) / dx;
       ns[i] = ns[i] - h * (Gs[i] - Gs[i - 1]) / dx;
     }
@@ -96,7 +96,8 @@ __device__ void sg_heavies(const bz_field_params_t& fp, int N, const double* E, 
 }
 
 // mom is not __restrict__: a closure sub-step updates its ne through ne_rw
-__global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_field(bz_field_params_t fp,
                                                        const double* mom,
                                                        double* ni, double* ns, double* Te,
                                                        double* g2, double* V, double* E,
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
 
   // host moment update (boltzmann.py:643-647)
   if (!fp.skip_moments) {
-    for (int i = tid; i < nn; i += kFieldBlock) {
+    for (int i = tid; i < nn; i += BLOCK) {
       ni[i] = ni[i] + nsrc[i];
       double te = ((2.0 / 3.0) * nrg[i]) / ne[i];
       if (te != te) te = 0.0;  // nan_to_num
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
   }
   if (src) {
     // closure sub-step: ionization from the learned rate makes ion-electron pairs
-    for (int i = tid; i < nn; i += kFieldBlock) {
+    for (int i = tid; i < nn; i += BLOCK) {
       const double q = src[i];
       ni[i] = ni[i] + q;
       g2[i] = g2[i] + q;
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
   __syncthreads();
 
   if (fp.fluid_on && fp.ion_scheme == 1) {
-    sg_heavies(fp, N, E, ni, ns, s_f);
+    sg_heavies<BLOCK>(fp, N, E, ni, ns, s_f);
   } else if (fp.fluid_on) {
     const double dx = fp.dx, dt = fp.dt, mu = fp.mu_i, Di = fp.D_i, Ds = fp.D_s;
     const double e0 = E[0] > 0.0 ? E[0] : 0.0;
@@ -155,12 +156,12 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
       ns[N] = 0.0;
     }
     __syncthreads();
-    for (int i = tid; i < N; i += kFieldBlock) {
+    for (int i = tid; i < N; i += BLOCK) {
       Ji[i] = mu * (0.5 * (ni[i] + ni[i + 1])) * E[i] - Di * (ni[i + 1] - ni[i]) / dx;
       Js[i] = -Ds * (ns[i + 1] - ns[i]) / dx;
     }
     __syncthreads();
-    for (int i = 1 + tid; i < N; i += kFieldBlock) {
+    for (int i = 1 + tid; i < N; i += BLOCK) {
       ni[i] = ni[i] - dt * (Ji[i] - Ji[i - 1]) / dx;
       ns[i] = ns[i] - dt * (Js[i] - Js[i - 1]) / dx;
     }
@@ -180,9 +181,9 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
     const int M = N - 1;
     double* inv = s_f + 2 * M;  // 1/diag
     double* lw = s_f + 3 * M;   // lower band
-    double* sa = s_f + 4 * M;   // per-thread scan maps [kFieldBlock]
-    double* sb = sa + kFieldBlock;
-    for (int i = tid; i < M; i += kFieldBlock) {
+    double* sa = s_f + 4 * M;   // per-thread scan maps [BLOCK]
+    double* sb = sa + BLOCK;
+    for (int i = tid; i < M; i += BLOCK) {
       rhs[i] = (1.0 / fp.eps) * (ni[i + 1] - ne[i + 1]);
       inv[i] = 1.0 / diag[i];
       lw[i] = low[i];
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
     // y_i = A_i y_{i-1} + B_i; each thread composes the maps of its chunk,
     // a block scan composes across chunks, then each chunk is re-run from
     // its incoming value with the reference's per-row expression.
-    const int chunk = (M + kFieldBlock - 1) / kFieldBlock;
+    const int chunk = (M + BLOCK - 1) / BLOCK;
     const int lo = tid * chunk;
     const int hi = lo + chunk < M ? lo + chunk : M;
     // forward: U^T y = rhs (EF_kernel_1D.py:31-33)
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
       sa[tid] = A;
       sb[tid] = B;
       __syncthreads();
-      for (int d = 1; d < kFieldBlock; d <<= 1) {
+      for (int d = 1; d < BLOCK; d <<= 1) {
         double pa = 1.0, pb = 0.0;
         if (tid >= d) {
           pa = sa[tid - d];
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
       sa[tid] = A;
       sb[tid] = B;
       __syncthreads();
-      for (int d = 1; d < kFieldBlock; d <<= 1) {
+      for (int d = 1; d < BLOCK; d <<= 1) {
         double pa = 1.0, pb = 0.0;
         if (tid >= d) {
           pa = sa[tid - d];
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
       }
       __syncthreads();
     }
-    for (int i = tid; i < N; i += kFieldBlock) {
+    for (int i = tid; i < N; i += BLOCK) {
       double e;
       if (i == 0) e = -(V[0] - vl) / dx;
       else if (i == N - 1) e = -(0.0 - V[N - 2]) / dx;
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
     }
     __syncthreads();
   }
-  for (int i = tid; i < N; i += kFieldBlock) {
+  for (int i = tid; i < N; i += BLOCK) {
     if (prec == BZ_F64) ((double*)E_t)[i] = E[i];
     else ((float*)E_t)[i] = (float)E[i];
   }
@@ -297,26 +298,38 @@ __global__ __launch_bounds__(kFieldBlock) void k_field(bz_field_params_t fp,
   }
 }
 
-int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double* moments,
-                 double* ni, double* ns, double* Te, double* g2, double* V, double* E,
-                 void* E_t, const double* chol, int advance, const double* src) {
+template <int BLOCK>
+static int launch_field_t(const LaunchEnv& env, const bz_field_params_t* fp,
+                          const double* moments, double* ni, double* ns, double* Te, double* g2,
+                          double* V, double* E, void* E_t, const double* chol, int advance,
+                          const double* src) {
   const int N = fp->n;
   // rhs, y, 1/diag, lower band (N-1 each) + scan maps (2 per thread)
-  const size_t smem = ((size_t)4 * (N - 1) + 2 * kFieldBlock) * sizeof(double);
+  const size_t smem = ((size_t)4 * (N - 1) + 2 * BLOCK) * sizeof(double);
   if (smem > 160 * 1024) return fail("field solve limited to N <= 4865 cells (LDS)");
   if (smem > 64 * 1024) {
-    if (hipFuncSetAttribute((const void*)k_field, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem) != hipSuccess)
+    if (hipFuncSetAttribute((const void*)k_field<BLOCK>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) != hipSuccess)
       return fail("hipFuncSetAttribute(field LDS) failed");
   }
   // fluxes live in the partial buffer (free between push launches)
   if (env.partials_doubles < 2ll * (N + 1)) return fail("workspace too small for fluxes");
   double* Ji = env.partials;
   double* Js = env.partials + (N + 1);
-  hipLaunchKernelGGL(k_field, dim3(1), dim3(kFieldBlock), smem, env.stream, *fp, moments, ni, ns,
-                     Te, g2, V, E, E_t, env.precision, chol, Ji, Js, env.clock, advance, src,
+  hipLaunchKernelGGL(k_field<BLOCK>, dim3(1), dim3(BLOCK), smem, env.stream, *fp, moments, ni,
+                     ns, Te, g2, V, E, E_t, env.precision, chol, Ji, Js, env.clock, advance, src,
                      const_cast<double*>(moments));
   return check_launch("k_field");
+}
+
+// One block: 256 threads while the grid is small (fewer threads in every
+// barrier and a shorter scan), 1024 for the large grids.
+int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double* moments,
+                 double* ni, double* ns, double* Te, double* g2, double* V, double* E,
+                 void* E_t, const double* chol, int advance, const double* src) {
+  if (fp->n <= 1024)
+    return launch_field_t<256>(env, fp, moments, ni, ns, Te, g2, V, E, E_t, chol, advance, src);
+  return launch_field_t<1024>(env, fp, moments, ni, ns, Te, g2, V, E, E_t, chol, advance, src);
 }
 
 }  // namespace bz

@@ -21,6 +21,10 @@ struct LaunchEnv {
   int64_t status_words;
   double* partials;          // per-block partial histograms
   int64_t partials_doubles;
+  double* dep_l2;            // level-2 rows of the partial reduction
+  int64_t dep_l2_doubles;
+  uint32_t* dep_cols_done;   // per output column: level-2 rows written (reset after use)
+  int dep_cols;
   void* sort_ws;             // sort / resample workspace
   int64_t sort_ws_bytes;
   int32_t max_nodes;
@@ -54,6 +58,12 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
 int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                    double* moments);
 int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out);
+// sum the push's per-block partial rows (m doubles each) into out[m];
+// commit != 0 also sums the counter rows and commits the clock
+int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int commit,
+                      int64_t cap);
+// widest partial row any reduction of a context with these sizes produces
+int64_t dep_max_row(int32_t max_nodes, int32_t max_keys);
 int launch_reduction(const LaunchEnv& env, int r, int M, int RM, double* dw, const double* dv,
                      const double* dx, int64_t ptcl);
 int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double* moments,

@@ -381,41 +381,109 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   if (tid < kCntRow) bcnt[(size_t)blockIdx.x * kCntRow + tid] = (uint32_t)s_cnt[tid];
 }
 
-// Sum the per-block partial histograms in block order and commit the clock.
-// One wave per output: lane l sums blocks l, l+64, ... in order, then a fixed
-// butterfly combines the lanes, so the result does not depend on timing.
-__global__ __launch_bounds__(256) void k_dep_reduce(const double* __restrict__ partials,
-                                                    int nblocks, int m,
-                                                    double* __restrict__ out, Clock* clk,
-                                                    int commit, int64_t cap) {
+// Sum the per-block partial histograms, in two levels inside one launch.
+// Block (cx, cy) owns outputs [64 cx, 64 cx + 64) and partial rows
+// [cy rpc, (cy + 1) rpc): lane l of wave w sums rows w, w+16, ... of its
+// chunk for output l (coalesced across the lanes), the 16 wave sums are added
+// in wave order and stored as level-2 row cy.  The last block to finish a
+// column cx (an acquire-release counter) adds the level-2 rows in the same
+// fixed pattern, so the result does not depend on timing.
+constexpr int kRedWaves = 16;
+constexpr int kRedThreads = kRedWaves * 64;
+
+__device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks, int m, int rpc,
+                                double* l2, uint32_t* col_done, double* __restrict__ out,
+                                int cx, int cy, int ny, double (*s_part)[64], int* s_last) {
   const int lane = threadIdx.x & 63;
-  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wid = threadIdx.x >> 6;
+  const int j = cx * 64 + lane;
+  const int r0 = cy * rpc;
+  const int r1 = r0 + rpc < nblocks ? r0 + rpc : nblocks;
+  // four independent partial sums per thread keep four row loads in flight
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   if (j < m) {
-    double s = 0.0;
-    for (int b = lane; b < nblocks; b += 64) s += partials[(size_t)b * m + j];
+    int b = r0 + wid;
+    for (; b + 3 * kRedWaves < r1; b += 4 * kRedWaves) {
+      s0 += partials[(size_t)b * m + j];
+      s1 += partials[(size_t)(b + kRedWaves) * m + j];
+      s2 += partials[(size_t)(b + 2 * kRedWaves) * m + j];
+      s3 += partials[(size_t)(b + 3 * kRedWaves) * m + j];
+    }
+    for (; b < r1; b += kRedWaves) s0 += partials[(size_t)b * m + j];
+  }
+  s_part[wid][lane] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (wid == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kRedWaves; ++w) t += s_part[w][lane];
+    if (ny == 1) {
+      if (j < m) out[j] = t;
+    } else if (j < m) {
+      l2[(size_t)cy * m + j] = t;
+      __threadfence();  // the level-2 row is visible before the column count moves
+    }
+  }
+  if (ny > 1) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t before = __hip_atomic_fetch_add(&col_done[cx], 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+      *s_last = before == (uint32_t)(ny - 1);
+    }
+    __syncthreads();
+    if (*s_last) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      double t = 0.0;
+      if (j < m)
+        for (int y = wid; y < ny; y += kRedWaves) t += __hip_atomic_load(
+            &l2[(size_t)y * m + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_part[wid][lane] = t;
+      __syncthreads();
+      if (wid == 0) {
+        double u = 0.0;
+#pragma unroll
+        for (int w = 0; w < kRedWaves; ++w) u += s_part[w][lane];
+        if (j < m) out[j] = u;
+        if (lane == 0) col_done[cx] = 0u;  // ready for the next launch
+      }
+    }
+  }
+}
+
+// Counters of the push's per-block rows into the clock and the slot count
+// commit (one wave).
+__device__ void dep_commit(const double* __restrict__ partials, int nblocks, int m, Clock* clk,
+                           int64_t cap) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t* bcnt = (const uint32_t*)(partials + (size_t)nblocks * m);
+  const int slot[9] = {BZ_CNT_BIRTHS, BZ_CNT_ABSORBED, BZ_CNT_ELASTIC, BZ_CNT_EXCITATION,
+                       BZ_CNT_IONIZATION, BZ_CNT_NULL, BZ_CNT_OVERFLOW, BZ_CNT_NUMAX,
+                       BZ_CNT_PSTEPS};
+  for (int q = 0; q < 9; ++q) {
+    unsigned long long s = 0;
+    for (int b = lane; b < nblocks; b += 64) s += bcnt[(size_t)b * kCntRow + q];
     s = wave_sum(s);
-    if (lane == 0) out[j] = s;
+    if (lane == 0 && s) clk->cnt[slot[q]] += s;
   }
-  if (commit && blockIdx.x == 0 && threadIdx.x < 64) {
-    // per-block counter rows written by the push (after its partials)
-    const uint32_t* bcnt = (const uint32_t*)(partials + (size_t)nblocks * m);
-    const int slot[9] = {BZ_CNT_BIRTHS, BZ_CNT_ABSORBED, BZ_CNT_ELASTIC, BZ_CNT_EXCITATION,
-                         BZ_CNT_IONIZATION, BZ_CNT_NULL, BZ_CNT_OVERFLOW, BZ_CNT_NUMAX,
-                         BZ_CNT_PSTEPS};
-    for (int q = 0; q < 9; ++q) {
-      unsigned long long s = 0;
-      for (int b = lane; b < nblocks; b += 64) s += bcnt[(size_t)b * kCntRow + q];
-      s = wave_sum(s);
-      if (lane == 0 && s) clk->cnt[slot[q]] += s;
-    }
-    if (lane == 0) {
-      // outputs past the capacity were dropped (and counted) by the push
-      const int64_t nx = clk->n_next;
-      clk->n = nx < cap ? nx : cap;
-      clk->epoch = clk->epoch + 1;
-      clk->ticket = 0;
-    }
+  if (lane == 0) {
+    // outputs past the capacity were dropped (and counted) by the push
+    const int64_t nx = __hip_atomic_load(&clk->n_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    clk->n = nx < cap ? nx : cap;
+    clk->epoch = clk->epoch + 1;
+    clk->ticket = 0;
   }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dep_reduce(
+    const double* __restrict__ partials, int nblocks, int m, int rpc, double* l2,
+    uint32_t* col_done, double* __restrict__ out, Clock* clk, int commit, int64_t cap) {
+  __shared__ double s_part[kRedWaves][64];
+  __shared__ int s_last;
+  dep_reduce_part(partials, nblocks, m, rpc, l2, col_done, out, blockIdx.x, blockIdx.y,
+                  gridDim.y, s_part, &s_last);
+  if (commit && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 64)
+    dep_commit(partials, nblocks, m, clk, cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -563,27 +631,29 @@ __global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
   if (tid == 0) a.block_births[blockIdx.x] = s_births;
 }
 
-// After k_push_1d: the births of block b go after the slots in use, behind
-// the births of blocks 0..b-1 (summed here by block b itself), in the
-// order (tile, wave, slot), which is parent order.  The last block sets the
-// slot count of the new state; births past the capacity are dropped and
-// counted.
-template <typename T>
-__global__ __launch_bounds__(256) void k_births_append_1d(PushArgs a, int tile, int nw) {
-  __shared__ uint32_t s_red[4];
+// After k_push_1d: the births of push block b go after the slots in use,
+// behind the births of blocks 0..b-1 (summed here by this block itself), in
+// the order (tile, wave, slot), which is parent order.  The block handling
+// the last push block sets the slot count of the new state; births past the
+// capacity are dropped and counted.  NT threads.
+template <typename T, int NT>
+__device__ void births_part(const PushArgs& a, int b, int nblk, int tile, int nw,
+                            uint32_t* s_red) {
+  constexpr int NWV = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int b = blockIdx.x;
   const int64_t n = a.clk->n;
   const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
-  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const uint32_t per = (ntiles + nblk - 1) / nblk;
   const uint32_t t_first = (uint32_t)min((uint64_t)b * per, (uint64_t)ntiles);
   const uint32_t t_end = t_first + per < ntiles ? t_first + per : ntiles;
   uint32_t s = 0;
-  for (int j = tid; j < b; j += 256) s += a.block_births[j];
+  for (int j = tid; j < b; j += NT) s += a.block_births[j];
   s = wave_sum(s);
   if (lane == 0) s_red[wid] = s;
   __syncthreads();
-  const int64_t pre = (int64_t)s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  int64_t pre = 0;
+#pragma unroll
+  for (int w = 0; w < NWV; ++w) pre += s_red[w];
   const uint32_t mine = a.block_births[b];
   T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
   const T* sg[5] = {(const T*)a.out[0], (const T*)a.out[1], (const T*)a.out[2],
@@ -593,7 +663,7 @@ __global__ __launch_bounds__(256) void k_births_append_1d(PushArgs a, int tile, 
   const int64_t entries = (int64_t)(t_end - t_first) * nw;
   int64_t run = 0;  // births of this block's earlier entries
   if (mine) {
-    for (int64_t e0 = 0; e0 < entries; e0 += 256) {
+    for (int64_t e0 = 0; e0 < entries; e0 += NT) {
       const int64_t e = e0 + tid;
       const uint32_t c = e < entries ? wave_births[(size_t)t_first * nw + e] : 0u;
       // block exclusive scan of c
@@ -607,7 +677,8 @@ __global__ __launch_bounds__(256) void k_births_append_1d(PushArgs a, int tile, 
       if (lane == 63) s_red[wid] = incl;
       __syncthreads();
       uint32_t before = 0, tot = 0;
-      for (int w = 0; w < 4; ++w) {
+#pragma unroll
+      for (int w = 0; w < NWV; ++w) {
         if (w < wid) before += s_red[w];
         tot += s_red[w];
       }
@@ -622,12 +693,18 @@ __global__ __launch_bounds__(256) void k_births_append_1d(PushArgs a, int tile, 
       run += tot;
     }
   }
-  if (b == (int)gridDim.x - 1 && tid == 0) {
+  if (b == nblk - 1 && tid == 0) {
     const int64_t want = n + pre + mine;
     a.clk->n_next = want < a.capacity ? want : a.capacity;
     if (want > a.capacity)
       atomicAdd(&a.clk->cnt[BZ_CNT_OVERFLOW], (unsigned long long)(want - a.capacity));
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_births_append_1d(PushArgs a, int tile, int nw) {
+  __shared__ uint32_t s_red[4];
+  births_part<T, 256>(a, blockIdx.x, gridDim.x, tile, nw, s_red);
 }
 
 // ---------------------------------------------------------------------------
@@ -770,6 +847,31 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(bz_params_t p, const T* __res
 #include "bz_launch.h"
 
 namespace bz {
+
+// Launch of k_dep_reduce: rows per chunk so that the level-2 rows fit.
+int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int commit,
+                      int64_t cap) {
+  if (m < 1) return 0;
+  // one level up to 256 rows (at most 16 rows per thread), two above
+  int64_t rpc = 256;
+  const int64_t need = (int64_t)nblocks * m;
+  if (need > rpc * env.dep_l2_doubles) rpc = (need + env.dep_l2_doubles - 1) / env.dep_l2_doubles;
+  if (rpc > 65535) return fail("deposit reduction: too many partial rows");
+  const int ny = (int)((nblocks + rpc - 1) / rpc);
+  if ((int64_t)ny * m > env.dep_l2_doubles && ny > 1) return fail("deposit reduction: level-2 buffer");
+  const int nx = (m + 63) / 64;
+  if (nx > env.dep_cols) return fail("deposit reduction: %d output columns > %d", nx, env.dep_cols);
+  hipLaunchKernelGGL(k_dep_reduce, dim3(nx, ny > 0 ? ny : 1), dim3(kRedWaves * 64), 0, env.stream,
+                     (const double*)env.partials, nblocks, m, (int)rpc, env.dep_l2, env.dep_cols_done,
+                     out, env.clock, commit, cap);
+  return check_launch("k_dep_reduce");
+}
+
+int64_t dep_max_row(int32_t max_nodes, int32_t max_keys) {
+  int64_t m = 4ll * max_nodes;  // the reference reduction interface: r = 4 columns
+  if ((int64_t)kImN * max_keys > m) m = (int64_t)kImN * max_keys;
+  return m;
+}
 
 int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys) {
   // up to 8 resident blocks per CU, each with its row of sums and 8 counters
@@ -943,10 +1045,7 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
                                          : launch_push_prec<float>(env, a, shape, &grid);
   if (rc) return rc;
   const int m = 3 * (p->dim == 1 ? p->n_nodes : 1);
-  hipLaunchKernelGGL(k_dep_reduce, dim3((m + 3) / 4), dim3(256), 0, env.stream,
-                     (const double*)env.partials, grid, m, moments, env.clock, 1,
-                     (int64_t)env.capacity);
-  return check_launch("k_dep_reduce");
+  return launch_dep_reduce(env, grid, m, moments, 1, (int64_t)env.capacity);
 }
 
 template <typename T, int DIM>
@@ -969,10 +1068,7 @@ static int launch_deposit_t(const LaunchEnv& env, const bz_state_t* s, const bz_
                      (const Clock*)env.clock, env.partials);
   int rc = check_launch("k_deposit");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_dep_reduce, dim3((3 * nn + 3) / 4), dim3(256), 0, env.stream,
-                     (const double*)env.partials, grid, 3 * nn, moments, env.clock, 0,
-                     (int64_t)env.capacity);
-  return check_launch("k_dep_reduce");
+  return launch_dep_reduce(env, grid, 3 * nn, moments, 0, (int64_t)env.capacity);
 }
 
 template <typename T>
@@ -1064,9 +1160,7 @@ int launch_reduction(const LaunchEnv& env, int r, int M, int RM, double* dw, con
                      ptcl, env.partials);
   int rc = check_launch("k_reduction");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_dep_reduce, dim3((m + 3) / 4), dim3(256), 0, env.stream,
-                     (const double*)env.partials, grid, m, dw, env.clock, 0, (int64_t)0);
-  rc = check_launch("k_dep_reduce");
+  rc = launch_dep_reduce(env, grid, m, dw, 0, (int64_t)0);
   if (rc) return rc;
   if ((int64_t)M * r > m)
     hipLaunchKernelGGL(k_zero_rows, dim3(64), dim3(256), 0, env.stream, dw, (int64_t)m,
@@ -1145,10 +1239,7 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
                                          : launch_push_prec<float>(env, a, shape, &grid);
   if (rc) return rc;
   const int m = n_inst * kImN;
-  hipLaunchKernelGGL(k_dep_reduce, dim3((m + 3) / 4), dim3(256), 0, env.stream,
-                     (const double*)env.partials, grid, m, inst_moments, env.clock, 1,
-                     (int64_t)env.capacity);
-  return check_launch("k_dep_reduce");
+  return launch_dep_reduce(env, grid, m, inst_moments, 1, (int64_t)env.capacity);
 }
 
 }  // namespace bz

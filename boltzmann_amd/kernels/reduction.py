@@ -23,7 +23,9 @@ def _reduction(r: int, p: int, M: int, RM: int, dw, dv, dx, ptcl: int):
         if not t.is_contiguous():
             raise ValueError(f"{name} must be contiguous")
     dev = dw.device.index or 0
-    s = S.session(dev, _lib.BZ_F64, max(int(ptcl), 1), 1, 1)
+    # a context whose node count covers RM rows of r columns (its partial
+    # and level-2 buffers are sized from it)
+    s = S.session(dev, _lib.BZ_F64, max(int(ptcl), 1), (int(RM) * int(r) + 3) // 4 + 1, 1)
     _lib.check(s.lib.bz_reduction(s.ctx, int(r), int(M), int(RM), S.ptr(dw), S.ptr(dv),
                                   S.ptr(dx), int(ptcl)), "bz_reduction")
 
