@@ -4,17 +4,20 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0|c2|c3|c4]
 
 A step is one kinetic update of the whole particle population.  For the
-discharges and the single swarm (c0, c1, c3, c4) that is the fused
-collision + push + deposit + compaction kernel, the deposit reduction, the
-device field step (moment update, fluid ions, Poisson) and, on the steps the
-workload asks for it, binning + merge/split resampling.  For the batched
+discharges (c1, c3, c4) that is the fused collision + push + deposit kernel
+(in place, k_push_1d), the births append, the deposit reduction, the device
+field step (moment update, fluid ions, Poisson) and, on the steps the
+workload asks for it, binning + merge/split resampling or the periodic
+compaction.  For the single swarm (c0) the push compacts.  For the batched
 swarms (c2) it is one launch over all 256 instances, replayed from a
 captured hipGraph with population control every 50 steps.  The state is
 synthetic (Maxwellian, uniform density) and resident in HBM before timing.
 
 Prints one JSON line.  The roofline object is for the dominant kernel
-(k_push), timed with HIP events on the library's stream; the CPU baseline is
-the oracle port timed on a bounded sample of the same workload on the host.
+(k_push_1d, or k_push for the swarms), timed with HIP events on the
+library's stream; its `traffic` is the HBM bytes per launch measured with
+rocprofv3 (profiles/traffic.json).  The CPU baseline is the oracle port
+timed on a bounded sample of the same workload on the host.
 """
 from __future__ import annotations
 
@@ -75,13 +78,14 @@ def cpu_baseline(cfg, seconds: float = 10.0):
     sub = cfg.with_(n_particles=n, precision=_lib.BZ_F64)
     st = ph.initial_state(sub)
     table = ph.build_table(cfg.xsec, cfg.gas_density_m3, sub.np_dtype) \
-        if cfg.xs_mode == 1 else None
+        if cfg.xs_mode != _lib.BZ_XS_FIT else None
     cores = _cores()
     setup = O.Setup(dim=cfg.dim, dt=cfg.dt, xs_mode=cfg.xs_mode,
                     n_cells=cfg.n_cells, length_cm=cfg.length_cm, bnf=cfg.bnf,
                     gas_density_m3=cfg.gas_density_m3,
                     nu_max=table.nu_max if table is not None else 0.0,
-                    e_field0=cfg.e_field0_vcm() if cfg.dim == 0 else 0.0, nthreads=cores)
+                    e_field0=cfg.e_field0_vcm() if cfg.dim == 0 else 0.0, nthreads=cores,
+                    push_order=cfg.push_order)
     tab = None if table is None else (table.energy, table.sig, table.slope)
     E = np.zeros(cfg.n_cells, dtype=np.float64) if cfg.dim == 1 else None
     rate, done = _time_oracle(setup, st, E, n, tab, cfg.seed, seconds, 2 * n)

@@ -11,7 +11,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libbz.so"
 
 BZ_F32, BZ_F64 = 4, 8
-BZ_XS_FIT, BZ_XS_TABLE, BZ_XS_FLIGHT = 0, 1, 2
+BZ_XS_FIT, BZ_XS_TABLE, BZ_XS_FLIGHT, BZ_XS_DIRECT = 0, 1, 2, 3
 BZ_RNG_PHILOX, BZ_RNG_STREAM = 0, 1
 BZ_PUSH_REF, BZ_PUSH_LEAPFROG = 0, 1
 BZ_NCOUNTERS = 16

@@ -12,14 +12,19 @@ c3  learned closure on the c1 discharge with 1024 cells and 4M electrons.
 c4  1-D discharge, 5 bins, 4096 cells, 32M electrons, FP32, 300 V,
     resampling every step towards 8000 per cell, 10 periods x 800 steps.
 
-All workloads share the gas of c0: the synthetic argon-like cross sections
-at N_gas = 3.2e22 m^-3 (about 1 Torr at 300 K).  The 1-D discharges push with
-the kick-drift leapfrog (BZ_PUSH_LEAPFROG): at 400 or 800 steps per RF
-period (omega_pe dt ~ 0.3) the reference's update order, explicit Euler in
-(x, v), heats the plasma until it is lost to the walls within a few RF
-periods; the leapfrog holds these discharges steady.  The reference order
-stays selectable (push_order=BZ_PUSH_REF) and is what the parity runs of the
-reference's own set-up use.
+The 1-D discharges use the reference's gas, 0.1 Torr argon (3.22e15 cm^-3,
+boltzmann.py:47-49), with the synthetic argon-like cross sections of c0 and
+the reference's direct collision probability P = 1 - exp(-nu_t(E) dt) per
+particle (BZ_XS_DIRECT): at 400-800 steps per RF period nu_t dt stays near
+0.1 for most electrons, while a single null-collision draw bounded by the
+table's largest frequency (nu_max dt ~ 1.3) would undercount collisions.
+They push with the kick-drift leapfrog (BZ_PUSH_LEAPFROG): at these steps
+(omega_pe dt ~ 0.3) the reference's update order, explicit Euler in (x, v),
+heats the plasma until it is lost to the walls within a few RF periods; the
+leapfrog holds the discharges steady.  The reference order stays selectable
+(push_order=BZ_PUSH_REF) and is what the parity runs of the reference's own
+set-up use.  The 0-D swarms (c0, c2) keep the null-collision draw: there
+nu_max dt is 0.07.
 """
 from __future__ import annotations
 
@@ -40,7 +45,8 @@ def c0() -> RunConfig:
 def c1() -> RunConfig:
     return RunConfig(name="c1_1d_3bin_256cells_1M", dim=1, n_particles=1_000_000,
                      precision=_lib.BZ_F32, nb=3, n_cells=256, length_cm=2.5,
-                     dt=1.0 / (RF * 400), steps=20 * 400, gas_density_m3=3.2e22,
+                     dt=1.0 / (RF * 400), steps=20 * 400, gas_density_m3=3.22e21,
+                     xs_mode=_lib.BZ_XS_DIRECT,
                      te_ev=2.0, density_cm3=1e9, rf_voltage=200.0, rf_freq=RF,
                      target_ppc=4000, resample_every=10, trigger_at=2.0, merge_at=2.0,
                      bin_fractions=bin_fractions(3), bin_thresholds=bin_thresholds(3),
@@ -54,7 +60,8 @@ def c3() -> RunConfig:
 def c4() -> RunConfig:
     return RunConfig(name="c4_1d_5bin_4096cells_32M", dim=1, n_particles=32_000_000,
                      precision=_lib.BZ_F32, nb=5, n_cells=4096, length_cm=2.5,
-                     dt=1.0 / (RF * 800), steps=10 * 800, gas_density_m3=3.2e22,
+                     dt=1.0 / (RF * 800), steps=10 * 800, gas_density_m3=3.22e21,
+                     xs_mode=_lib.BZ_XS_DIRECT,
                      te_ev=2.0, density_cm3=1e9, rf_voltage=300.0, rf_freq=RF,
                      target_ppc=8000, resample_every=1, trigger_at=0.0, merge_at=1.5,
                      bin_fractions=bin_fractions(5), bin_thresholds=bin_thresholds(5),

@@ -306,7 +306,8 @@ static int check_params(const bz_ctx* c, const bz_params_t* p) {
     if (p->bnf < 1) return fail("bnf must be >= 1");
     if (!(p->dx > 0.0) || !(p->L > 0.0)) return fail("dx and L must be > 0");
   }
-  if (p->xs_mode != BZ_XS_FIT && p->xs_mode != BZ_XS_TABLE && p->xs_mode != BZ_XS_FLIGHT)
+  if (p->xs_mode != BZ_XS_FIT && p->xs_mode != BZ_XS_TABLE && p->xs_mode != BZ_XS_FLIGHT &&
+      p->xs_mode != BZ_XS_DIRECT)
     return fail("bad xs_mode");
   if (p->xs_mode == BZ_XS_FLIGHT && !(p->nu_max > 0.0))
     return fail("free-flight collisions need nu_max > 0");

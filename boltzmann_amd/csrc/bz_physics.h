@@ -49,6 +49,8 @@ __device__ __forceinline__ double dlog(double v) { return log(v); }
 __device__ __forceinline__ float dlog(float v) { return logf(v); }
 __device__ __forceinline__ double dsqrt(double v) { return sqrt(v); }
 __device__ __forceinline__ float dsqrt(float v) { return sqrtf(v); }
+__device__ __forceinline__ double dexpm1(double v) { return expm1(v); }
+__device__ __forceinline__ float dexpm1(float v) { return expm1f(v); }
 __device__ __forceinline__ double dlog1p(double v) { return log1p(v); }
 __device__ __forceinline__ float dlog1p(float v) { return log1pf(v); }
 __device__ __forceinline__ void dsincos(double v, double* s, double* c) { sincos(v, s, c); }
@@ -118,6 +120,25 @@ __device__ __forceinline__ void table_thresholds(T e, T vmag, const bz_params_t&
                                                  bool& over) {
   table_rates<T>(e, vmag, p, xs, (T)p.pc, r0, r1, r2, over);
   over = r2 > (T)p.p_null;
+}
+
+// Direct model: cumulative frequencies nu_j of the tabulated sigma, then
+// P = 1 - exp(-nu_t dt) and r_j = nu_j * P / nu_t (the reference's
+// P * sigma_j / sigma_t, electron_kernel_numba.py:186-212).
+template <typename T>
+__device__ __forceinline__ void direct_thresholds(T e, T vmag, const bz_params_t& p,
+                                                  const bz_xsec_t& xs, T& r0, T& r1, T& r2) {
+  bool unused;
+  table_rates<T>(e, vmag, p, xs, (T)1, r0, r1, r2, unused);
+  if (r2 > (T)0) {
+    const T P = -dexpm1(-(T)p.dt * r2);
+    const T pcst = P * ((T)1 / r2);
+    r0 = r0 * pcst;
+    r1 = r1 * pcst;
+    r2 = r2 * pcst;
+  } else {
+    r0 = r1 = r2 = (T)0;
+  }
 }
 
 template <typename T>
@@ -261,6 +282,7 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
   T r0, r1, r2;
   bool over = false;
   if (XS == BZ_XS_FIT) fit_thresholds<T>(e, vmag, p, r0, r1, r2);
+  else if (XS == BZ_XS_DIRECT) direct_thresholds<T>(e, vmag, p, xs, r0, r1, r2);
   else table_thresholds<T>(e, vmag, p, xs, r0, r1, r2, over);
   o.over = over ? 1 : 0;
 
@@ -272,12 +294,12 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
     o.chan = kChanElastic;
     if (RNG != BZ_RNG_STREAM && kAnglesInSecond) U.second(p.seed, i, step);
     iso_dir<T>(vmag * (T)p.fel, U.u[1], U.u[2], vx, vy, vz);
-  } else if (XS == BZ_XS_TABLE && R0 < r1) {
+  } else if (XS != BZ_XS_FIT && R0 < r1) {
     o.chan = kChanExc;
     if (RNG != BZ_RNG_STREAM && kAnglesInSecond) U.second(p.seed, i, step);
     const T enew = e - (T)p.e_exc;
     iso_dir<T>(dsqrt(((T)2 * enew) / (T)p.jm), U.u[1], U.u[2], vx, vy, vz);
-  } else if ((XS == BZ_XS_TABLE && R0 < r2) || (XS == BZ_XS_FIT && r0 < R0 && R0 < r2)) {
+  } else if ((XS != BZ_XS_FIT && R0 < r2) || (XS == BZ_XS_FIT && r0 < R0 && R0 < r2)) {
     o.chan = kChanIon;
     if (RNG != BZ_RNG_STREAM) {
       U.second(p.seed, i, step);

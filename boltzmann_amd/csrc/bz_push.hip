@@ -1003,6 +1003,13 @@ static int launch_push_prec(const LaunchEnv& env, PushArgs& a, int shape, int* g
     return p.dim == 1 ? launch_push_shape<T, BZ_XS_FLIGHT, 1, BZ_RNG_PHILOX>(env, a, shape, grid)
                       : launch_push_shape<T, BZ_XS_FLIGHT, 0, BZ_RNG_PHILOX>(env, a, shape, grid);
   }
+  if (p.xs_mode == BZ_XS_DIRECT) {
+    if (p.dim == 1)
+      return stream ? launch_push_shape<T, BZ_XS_DIRECT, 1, BZ_RNG_STREAM>(env, a, 0, grid)
+                    : launch_push_shape<T, BZ_XS_DIRECT, 1, BZ_RNG_PHILOX>(env, a, shape, grid);
+    return stream ? launch_push_shape<T, BZ_XS_DIRECT, 0, BZ_RNG_STREAM>(env, a, 0, grid)
+                  : launch_push_shape<T, BZ_XS_DIRECT, 0, BZ_RNG_PHILOX>(env, a, shape, grid);
+  }
   if (p.xs_mode == BZ_XS_TABLE) {
     if (p.dim == 1)
       return stream ? launch_push_shape<T, BZ_XS_TABLE, 1, BZ_RNG_STREAM>(env, a, 0, grid)

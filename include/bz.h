@@ -37,8 +37,12 @@ enum {
   BZ_XS_FIT = 0,   /* reference argon fits, direct P = 1-exp(-nu dt)
                       (kernels/electron_kernel_numba.py:66-195) */
   BZ_XS_TABLE = 1, /* tabulated sigma(E), 3 channels, null collision */
-  BZ_XS_FLIGHT = 2 /* tabulated sigma(E), exact free-flight null collision:
-                      any number of collisions per step (Philox stream only) */
+  BZ_XS_FLIGHT = 2, /* tabulated sigma(E), exact free-flight null collision:
+                       any number of collisions per step (Philox stream only) */
+  BZ_XS_DIRECT = 3  /* tabulated sigma(E), the reference's direct collision
+                       probability P = 1 - exp(-nu_t(E) dt) per particle
+                       (electron_kernel_numba.py:186-212): accurate while the
+                       particle's own nu_t dt is small, whatever nu_max dt is */
 };
 /* random stream */
 enum {

@@ -158,6 +158,8 @@ inline double m_p10(double v) { return pow(10.0, v); }
 inline float m_p10(float v) { return powf(10.0f, v); }
 inline double m_exp(double v) { return exp(v); }
 inline float m_exp(float v) { return expf(v); }
+inline double m_expm1(double v) { return expm1(v); }
+inline float m_expm1(float v) { return expm1f(v); }
 inline double m_sqrt(double v) { return sqrt(v); }
 inline float m_sqrt(float v) { return sqrtf(v); }
 inline double m_sin(double v) { return sin(v); }
@@ -274,6 +276,26 @@ int one_particle(int64_t i, const orc_params& P, const T* x, const T* vx, const 
     const T g2 = g0 + s2 * pcst;
     if (R0 < g0) c = 1;
     else if (g0 < R0 && R0 < g2) c = 3;
+  } else if (P.xs_mode == 3) {
+    // direct: P = 1 - exp(-nu_t dt), thresholds nu_j * P / nu_t
+    // (electron_kernel_numba.py:186-212 with the tabulated sigma)
+    T s[3];
+    table_cross_sections<T>(e, P, *tab, s);
+    T g0 = s[0] * v_mag;
+    T g1 = g0 + s[1] * v_mag;
+    T g2 = g1 + s[2] * v_mag;
+    if (g2 > (T)0) {
+      const T Pc = -m_expm1(-(T)P.dt * g2);
+      const T pcst = Pc * ((T)1 / g2);
+      g0 = g0 * pcst;
+      g1 = g1 * pcst;
+      g2 = g2 * pcst;
+    } else {
+      g0 = g1 = g2 = (T)0;
+    }
+    if (R0 < g0) c = 1;
+    else if (R0 < g1) c = 2;
+    else if (R0 < g2) c = 3;
   } else {
     T s[3];
     table_cross_sections<T>(e, P, *tab, s);

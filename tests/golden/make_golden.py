@@ -36,6 +36,7 @@ PUSH_CASES = {
     "push_fit_f64_1d_ref": (1, 0, np.float64, 0, "stream", 48, 3.22e23, 2e-11),
     "push_table_f64_0d": (0, 1, np.float64, 0, "philox", 1, 3.2e24, 2e-11),
     "push_flight_f32_1d_leapfrog": (1, 2, np.float32, 1, "philox", 48, 3.2e23, 2e-11),
+    "push_direct_f32_1d_leapfrog": (1, 3, np.float32, 1, "philox", 48, 3.22e23, 2e-11),
 }
 N_PUSH = 1000
 L = 2.5

@@ -162,6 +162,37 @@ def test_births_last_layout(xs_mode):
     assert head.shape[1] + tail.shape[1] == A.shape[1]
 
 
+@pytest.mark.parametrize("gas", [3.2e21, 3.2e23])
+def test_direct_collision_probability(gas):
+    """Direct model (xs_mode 3): a mono-energetic beam at 20 eV collides
+    with probability 1 - exp(-nu_t dt), nu_t = N v (s_el + s_exc + s_ion),
+    split between channels in proportion to the frequencies, also where
+    nu_t dt is of order one (there a single null-collision draw with the
+    table's nu_max would undercount).  Uniforms on a regular grid make the
+    counts exact up to one per channel."""
+    from boltzmann_amd import physics as ph
+    n, dt, e0 = 100_000, 2e-11, 20.0
+    t, tab = _table(np.float64, gas=gas)
+    s = O.Setup(dim=0, dt=dt, xs_mode=3, gas_density_m3=gas, nu_max=t.nu_max)
+    sp = math.sqrt(e0 / KEV)
+    d = np.random.default_rng(2).normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    st = (np.zeros(n), *(sp * d), np.ones(n))
+    R = np.random.default_rng(3).uniform(0, 1, size=(7, n))
+    R[0] = (np.arange(n) + 0.5) / n
+    _, _, cnt = O.push_step(s, st, None, R, table=tab, capacity=2 * n)
+    sig = ph.XSecModel().sigma(np.array([e0]))[:, 0] * gas
+    nu = sig * sp
+    p = -math.expm1(-nu.sum() * dt)
+    want = n * p * nu / nu.sum()
+    got = [int(cnt[2]), int(cnt[3]), int(cnt[4])]
+    for g, w in zip(got, want):
+        assert abs(g - w) <= 2 + 1e-4 * w, (got, want)
+    # the null-collision draw with the table's nu_max would give n * nu dt-ish
+    # counts capped by p_null: far off when nu_max dt is large
+    assert p > 0.01
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_elastic_scatter_geometry(dtype):
     """R0 = 0 forces the first channel: speed scales by sqrt(1-2 m_e/M),

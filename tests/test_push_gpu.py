@@ -68,6 +68,9 @@ CASES = [
     ("fit-f64-1d", 1, 8, 0),
     ("fit-f32-1d", 1, 4, 0),
     ("fit-f64-0d", 0, 8, 0),
+    ("direct-f32-1d", 1, 4, 3),
+    ("direct-f64-1d", 1, 8, 3),
+    ("direct-f64-0d", 0, 8, 3),
 ]
 
 
