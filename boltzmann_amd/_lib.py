@@ -68,7 +68,8 @@ class ResamplePolicy(C.Structure):
 
 class Mlp(C.Structure):
     _fields_ = [("w", C.c_void_p), ("in_shift", C.c_double * 3), ("in_scale", C.c_double * 3),
-                ("out_shift", C.c_double * 3), ("out_scale", C.c_double * 3)]
+                ("out_shift", C.c_double * 3), ("out_scale", C.c_double * 3),
+                ("in_lo", C.c_double * 3), ("in_hi", C.c_double * 3)]
 
 
 class Adam(C.Structure):

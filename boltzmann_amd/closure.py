@@ -76,7 +76,8 @@ class ClosureNet:
         self.loss_buf = torch.zeros(1, **f32)
         self.ws = None
         self.norm = {"in_shift": np.zeros(3), "in_scale": np.ones(3),
-                     "out_shift": np.zeros(3), "out_scale": np.ones(3)}
+                     "out_shift": np.zeros(3), "out_scale": np.ones(3),
+                     "in_lo": np.zeros(3), "in_hi": np.zeros(3)}
         self.t = 0
         self.refresh()
 
@@ -90,6 +91,8 @@ class ClosureNet:
         s.w = self.w.data_ptr()
         for k in ("in_shift", "in_scale", "out_shift", "out_scale"):
             getattr(s, k)[:] = [float(v) for v in self.norm[k]]
+        for k in ("in_lo", "in_hi"):
+            getattr(s, k)[:] = [float(v) for v in self.norm.get(k, np.zeros(3))]
         return s
 
     def fit_normalisation(self, x: np.ndarray, t: np.ndarray):
@@ -97,7 +100,9 @@ class ClosureNet:
         xs = np.std(x, axis=0)
         ts = np.std(t, axis=0)
         self.norm = {"in_shift": np.mean(x, axis=0), "in_scale": 1.0 / np.where(xs > 0, xs, 1.0),
-                     "out_shift": np.mean(t, axis=0), "out_scale": np.where(ts > 0, ts, 1.0)}
+                     "out_shift": np.mean(t, axis=0), "out_scale": np.where(ts > 0, ts, 1.0),
+                     "in_lo": np.min(x, axis=0).astype(np.float64),
+                     "in_hi": np.max(x, axis=0).astype(np.float64)}
 
     def set_parameters(self, theta: np.ndarray):
         self.master.copy_(torch.from_numpy(np.asarray(theta, np.float32)))
@@ -190,6 +195,8 @@ class ClosureNet:
         with np.load(path, allow_pickle=False) as z:
             self.norm = {k: np.array(z[k], dtype=np.float64)
                          for k in ("in_shift", "in_scale", "out_shift", "out_scale")}
+            for k in ("in_lo", "in_hi"):
+                self.norm[k] = np.array(z[k], dtype=np.float64) if k in z else np.zeros(3)
             self.set_parameters(z["master"])
 
     def close(self):

@@ -11,14 +11,17 @@ mu N = |<v_x>| / (E/N), and a log-uniform electron density (the swarms have
 no electron-electron collisions: the rates do not depend on it, and the
 network has to learn that).
 
-Closure-accelerated run.  The kinetic step (exact free-flight collisions,
-so its length is not bound by the collision frequency) runs once per
-n_sub fluid / Poisson sub-steps and advances the particles by n_sub * dt.
-Between kinetic updates the learned rate supplies the ionization source
-of the sub-steps from the local E/N of each sub-step and the last kinetic
+Closure-accelerated run.  The kinetic update runs once per n_sub fluid /
+Poisson sub-steps and advances the particles through the whole interval,
+in n_kin pushes of h = n_sub * dt / n_kin with free-flight collisions
+(exact at any push length) and a Poisson solve on the new electron density
+between them; n_kin keeps omega_pe h below 2, the leapfrog's stability
+bound (a single push of n_sub * dt at the discharge's density is not
+stable).  Between kinetic updates the learned rate supplies the ionization source of
+the sub-steps from the local E/N of each sub-step and the last kinetic
 mean energy and density.  At each kinetic update the kinetic ionization
-count over the same interval replaces the closure's (the kinetic step
-stays the reference), and the difference is the closure's rate error,
+estimate for the interval replaces the closure's (the kinetic step stays
+the reference), and the difference is the closure's rate error,
 accumulated in FP64 on the device.
 """
 from __future__ import annotations
@@ -126,20 +129,34 @@ class ClosureDischarge:
     """A 1-D discharge advanced with the closure between kinetic updates."""
 
     def __init__(self, cfg: ph.RunConfig, net_params: dict, n_sub: int = 8,
-                 resample_every: Optional[int] = None):
+                 resample_every: Optional[int] = None, n_kin: Optional[int] = None):
         if cfg.dim != 1 or cfg.xs_mode != _lib.BZ_XS_FLIGHT:
             raise ValueError("the closure run needs a 1-D discharge with free-flight collisions")
         self.cfg = cfg
         self.n_sub = n_sub
+        if n_kin is None:
+            # pushes per kinetic update: omega_pe h <= 1.8 at 1.5 times the
+            # initial density (the bulk density of these discharges grows by
+            # some 30 %)
+            wpe = 56.4 * math.sqrt(1.5 * cfg.density_cm3 * 1e6)  # rad/s, n in m^-3
+            n_kin = max(1, math.ceil(wpe * n_sub * cfg.dt / 1.8))
+            while n_sub % n_kin:
+                n_kin += 1
+        self.n_kin = int(n_kin)
         self.eng = KineticEngine(cfg)
-        # kinetic step length n_sub * dt (free flights: exact at any length)
-        self.kparams = ph.push_params(cfg.with_(dt=n_sub * cfg.dt), _lib.BZ_RNG_PHILOX,
-                                      self.eng.table)
+        # kinetic push length h (free flights: exact at any length)
+        self.kparams = ph.push_params(cfg.with_(dt=n_sub * cfg.dt / self.n_kin),
+                                      _lib.BZ_RNG_PHILOX, self.eng.table)
+        # Poisson only between the pushes of one update (no time advance)
+        self.fp_poisson = _lib.FieldParams.from_buffer_copy(self.eng.fp)
+        self.fp_poisson.fluid_on = 0
+        self.fp_poisson.skip_moments = 1
         self.net = ClosureNet(ctx=self.eng.ctx)
         self.net.norm = {k: np.asarray(v, np.float64) for k, v in net_params["norm"].items()}
         self.net.set_parameters(net_params["master"])
         nn = cfg.n_nodes
         f64 = dict(dtype=torch.float64, device=self.eng.dev)
+        self.ksrc = torch.zeros(nn, **f64)
         self.src = torch.zeros(nn, **f64)
         self.acc = torch.zeros(nn, **f64)
         self.judge = torch.zeros(8, **f64)
@@ -155,14 +172,31 @@ class ClosureDischarge:
         k = self.resample_every
         if k and self.cfg.target_ppc > 0 and self.macro % k == 0:
             e.resample()
-        # kinetic update over n_sub * dt (1-D: in place, births appended)
-        src_, dst = e.bufs[e.cur], e.bufs[1 - e.cur]
-        _lib.check(e.lib.bz_push_mcc(e.ctx, C.byref(src_.struct), C.byref(dst.struct),
-                                     _p(e.E_t), C.byref(self.kparams), C.byref(e.xs), None, None,
-                                     _p(e.moments)), "bz_push_mcc")
-        e._since_compact += 1
-        if e.compact_every and e._since_compact >= e.compact_every:
-            e.compact()
+        # kinetic update over n_sub * dt: n_kin pushes (in place, births
+        # appended), the field re-solved on the kinetic density between them
+        nn = self.cfg.n_nodes
+        for kk in range(self.n_kin):
+            if kk:
+                _lib.check(e.lib.bz_field_step(
+                    e.ctx, C.byref(self.fp_poisson), _p(e.moments), _p(e.ni), _p(e.ns),
+                    _p(e.Te), _p(e.g2), _p(e.V), _p(e.E), _p(e.E_t), _p(e.chol), 0),
+                    "bz_field_step")
+            src_, dst = e.bufs[e.cur], e.bufs[1 - e.cur]
+            _lib.check(e.lib.bz_push_mcc(e.ctx, C.byref(src_.struct), C.byref(dst.struct),
+                                         _p(e.E_t), C.byref(self.kparams), C.byref(e.xs), None,
+                                         None, _p(e.moments)), "bz_push_mcc")
+            if self.n_kin > 1:
+                if kk == 0:
+                    self.ksrc.copy_(e.moments[nn:2 * nn])
+                else:
+                    self.ksrc += e.moments[nn:2 * nn]
+            e._since_compact += 1
+            if e.compact_every and e._since_compact >= e.compact_every:
+                e.compact()
+        if self.n_kin > 1:
+            # the update's ionization source: every push of it
+            e.moments[nn:2 * nn].copy_(self.ksrc)
+
         net = self.net.struct()
         for _ in range(self.n_sub):
             _lib.check(e.lib.bz_closure_source(e.ctx, C.byref(net), self.cfg.n_cells, _p(e.E),

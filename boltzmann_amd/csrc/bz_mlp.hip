@@ -37,7 +37,14 @@ constexpr int kRow = kH + 1;            // padded LDS activation row
 // targets are normalised with out_inv = 1 / out_scale
 struct Norm {
   float in_shift[kIn], in_scale[kIn], out_shift[kOut], out_scale[kOut], out_inv[kOut];
+  double in_lo[kIn], in_hi[kIn];
 };
+
+// a feature clamped to the trained domain (none where lo >= hi)
+__device__ __forceinline__ double clamp_feature(const Norm& nm, int k, double v) {
+  if (nm.in_lo[k] < nm.in_hi[k]) v = fmin(fmax(v, nm.in_lo[k]), nm.in_hi[k]);
+  return v;
+}
 
 __device__ __forceinline__ float bf16_f32(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);
@@ -315,6 +322,8 @@ Norm norm_of(const bz_mlp_t* net) {
   for (int i = 0; i < kIn; ++i) {
     nm.in_shift[i] = (float)net->in_shift[i];
     nm.in_scale[i] = (float)net->in_scale[i];
+    nm.in_lo[i] = net->in_lo[i];
+    nm.in_hi[i] = net->in_hi[i];
   }
   for (int o = 0; o < kOut; ++o) {
     nm.out_shift[o] = (float)net->out_shift[o];
@@ -422,9 +431,9 @@ __global__ void __launch_bounds__(256) k_closure_source(Norm nm, const uint16_t*
     const double me = ne > 0.0 ? mom[2 * nn + node] / ne : 0.0;
     const double lne = log10(fmax(ne * 1e6, 1.0));
     float x[kIn];
-    x[0] = ((float)log10(en_td) - nm.in_shift[0]) * nm.in_scale[0];
-    x[1] = ((float)me - nm.in_shift[1]) * nm.in_scale[1];
-    x[2] = ((float)lne - nm.in_shift[2]) * nm.in_scale[2];
+    x[0] = ((float)clamp_feature(nm, 0, log10(en_td)) - nm.in_shift[0]) * nm.in_scale[0];
+    x[1] = ((float)clamp_feature(nm, 1, me) - nm.in_shift[1]) * nm.in_scale[1];
+    x[2] = ((float)clamp_feature(nm, 2, lne) - nm.in_shift[2]) * nm.in_scale[2];
     float z = b1, a, d;
 #pragma unroll
     for (int i = 0; i < kIn; ++i) z = fmaf(w1[i], x[i], z);

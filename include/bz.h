@@ -346,6 +346,8 @@ typedef struct {
   double in_scale[3];
   double out_shift[3];    /* y = y_n * out_scale + out_shift */
   double out_scale[3];
+  double in_lo[3];        /* input domain of the training set: bz_closure_source */
+  double in_hi[3];        /* clamps its features to it (no clamp where lo >= hi) */
 } bz_mlp_t;
 
 typedef struct {
@@ -372,7 +374,9 @@ int bz_mlp_round(bz_ctx_t ctx, const float* master, uint16_t* w);
  * by the learned rate, src_i = k_ion(E/N_i, <e>_i, n_e,i) N_gas ne_i dt in
  * the moments' density units.  E/N_i uses the node field (mean |E| of the
  * adjacent cells, E in V/cm), <e>_i = nrg_i / ne_i and n_e,i = 1e6 ne_i
- * (cm^-3 -> m^-3), all from moments [3][N+1] (the kinetic deposit layout).
+ * (cm^-3 -> m^-3), all from moments [3][N+1] (the kinetic deposit layout);
+ * each feature is clamped to the network's input domain [in_lo, in_hi], so
+ * sheath fields beyond the trained range get the rate at its edge.
  * src [N+1] is written; acc [N+1] (optional) accumulates it. */
 int bz_closure_source(bz_ctx_t ctx, const bz_mlp_t* net, int32_t n_cells, const double* E,
                       const double* moments, double gas_density_m3, double dt, double* src,

@@ -55,6 +55,6 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.Params) == 19 * 8 + 8 + 6 * 4 + 8 + 2 * 4
     assert ctypes.sizeof(_lib.XSec) == 4 * 8 + 4 * 4
     assert ctypes.sizeof(_lib.FieldParams) == 8 * 8 + 6 * 4
-    assert ctypes.sizeof(_lib.Mlp) == 8 + 12 * 8
+    assert ctypes.sizeof(_lib.Mlp) == 8 + 18 * 8
     assert ctypes.sizeof(_lib.Adam) == 6 * 4
     assert ctypes.sizeof(_lib.ResamplePolicy) == 6 * 8 + 4 * 4

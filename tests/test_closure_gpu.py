@@ -27,7 +27,9 @@ def _net():
     net = ClosureNet(seed=4)
     net.norm = {"in_shift": np.array([1.5, 6.0, 16.0]), "in_scale": np.array([0.9, 0.12, 0.6]),
                 "out_shift": np.array([-17.0, -16.0, 24.0]),
-                "out_scale": np.array([2.0, 1.5, 0.4])}
+                "out_scale": np.array([2.0, 1.5, 0.4]),
+                # a trained domain narrower than the test grid's features: clamped
+                "in_lo": np.array([1.0, 2.0, 15.5]), "in_hi": np.array([2.0, 10.0, 17.5])}
     return net
 
 
@@ -41,6 +43,10 @@ def _want_src(net, E, mom, N, gas, dt):
     me = np.where(ne > 0, mom[2 * nn:] / np.where(ne > 0, ne, 1), 0.0)
     lne = np.log10(np.maximum(ne * 1e6, 1.0))
     x = np.stack([np.log10(en_td), me, lne], 1)
+    lo, hi = net.norm.get("in_lo", np.zeros(3)), net.norm.get("in_hi", np.zeros(3))
+    for k in range(3):
+        if lo[k] < hi[k]:
+            x[:, k] = np.clip(x[:, k], lo[k], hi[k])
     y = M.forward(M.bf16_value(net.bf16_bits()), net.norm, x)
     return 10.0 ** y[:, 0] * gas * ne * dt
 
