@@ -53,8 +53,10 @@ __device__ __forceinline__ double dexpm1(double v) { return expm1(v); }
 __device__ __forceinline__ float dexpm1(float v) { return expm1f(v); }
 __device__ __forceinline__ double dlog1p(double v) { return log1p(v); }
 __device__ __forceinline__ float dlog1p(float v) { return log1pf(v); }
-__device__ __forceinline__ void dsincos(double v, double* s, double* c) { sincos(v, s, c); }
-__device__ __forceinline__ void dsincos(float v, float* s, float* c) { sincosf(v, s, c); }
+// sin(pi v), cos(pi v): the argument reduction is exact, so an angle 2*pi*u
+// with u in [0,1) costs no large-argument reduction.
+__device__ __forceinline__ void dsincospi(double v, double* s, double* c) { sincospi(v, s, c); }
+__device__ __forceinline__ void dsincospi(float v, float* s, float* c) { sincospif(v, s, c); }
 
 template <typename T>
 __device__ __forceinline__ T fit_poly(const double (*coef)[3], int seg, T L) {
@@ -155,7 +157,14 @@ __device__ __forceinline__ void table_rates(T e, T vmag, const bz_params_t& p,
   int b = xs_bucket<T>(ec, xs.hint_bits) - xs.hint_base;
   b = b < 0 ? 0 : (b >= xs.n_hint ? xs.n_hint - 1 : b);
   int j = xs.hint[b];
-  for (int it = 0; it < 8; ++it) {
+  // a hint bucket spans one or two table intervals at the tables' spacing:
+  // two branch-free steps, then a loop the host bounds to 7 steps in all
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int q = j + 1 <= n - 2 ? j + 1 : n - 2;  // in range, so read unconditionally
+    j += (q == j + 1 && E[q] <= ec) ? 1 : 0;
+  }
+  for (int it = 2; it < 8; ++it) {
     if (j + 1 <= n - 2 && E[j + 1] <= ec) ++j;
     else break;
   }
@@ -176,9 +185,8 @@ __device__ __forceinline__ void iso_dir(T vnew, T ra, T rb, T& vx, T& vy, T& vz)
   const T cchi = (T)2 * ra - (T)1;
   T s2 = (T)1 - cchi * cchi;
   const T schi = dsqrt(s2 > (T)0 ? s2 : (T)0);
-  const T phi = (T)6.283185307179586 * rb;
   T sphi, cphi;
-  dsincos(phi, &sphi, &cphi);
+  dsincospi((T)2 * rb, &sphi, &cphi);  // phi = 2 pi rb
   vx = vnew * schi * cphi;
   vy = vnew * schi * sphi;
   vz = vnew * cchi;
@@ -196,10 +204,10 @@ __device__ __forceinline__ void gas_velocity<double>(uint64_t seed, uint32_t i, 
   const double g0 = u53(a.a, a.b), g1 = u53(a.c, a.d), g2 = u53(b.a, b.b), g3 = u53(b.c, b.d);
   const double r1 = sqrt(-2.0 * log(1.0 - g0)), r2 = sqrt(-2.0 * log(1.0 - g2));
   double s, c;
-  sincos(6.283185307179586 * g1, &s, &c);
+  sincospi(2.0 * g1, &s, &c);
   gx = sigma * (r1 * c);
   gy = sigma * (r1 * s);
-  gz = sigma * (r2 * cos(6.283185307179586 * g3));
+  gz = sigma * (r2 * cospi(2.0 * g3));
 }
 template <>
 __device__ __forceinline__ void gas_velocity<float>(uint64_t seed, uint32_t i, uint64_t step,
@@ -209,10 +217,10 @@ __device__ __forceinline__ void gas_velocity<float>(uint64_t seed, uint32_t i, u
   const float g0 = u24(a.a), g1 = u24(a.b), g2 = u24(a.c), g3 = u24(a.d);
   const float r1 = sqrtf(-2.0f * logf(1.0f - g0)), r2 = sqrtf(-2.0f * logf(1.0f - g2));
   float s, c;
-  sincosf(6.2831853f * g1, &s, &c);
+  sincospif(2.0f * g1, &s, &c);
   gx = sigma * (r1 * c);
   gy = sigma * (r1 * s);
-  gz = sigma * (r2 * cosf(6.2831853f * g3));
+  gz = sigma * (r2 * cospif(2.0f * g3));
 }
 
 enum : int { kChanNone = 0, kChanElastic = 1, kChanExc = 2, kChanIon = 3 };
@@ -259,6 +267,9 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
       gz = G[2 * st + i];
     }
   } else if (p.gas_sigma > 0.0) {
+    // a uniform branch; the empty asm keeps the compiler from evaluating the
+    // draw for every particle and selecting zero when the gas is cold
+    asm volatile("" ::: "memory");
     gas_velocity<T>(p.seed, i, step, (T)p.gas_sigma, gx, gy, gz);
   }
   T vx = in.vx, vy = in.vy, vz = in.vz;
@@ -288,34 +299,37 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
 
   const T R0 = U.u[0];
   T tx = (T)0, ty = (T)0, tz = (T)0;
-  // FP32 takes u[1], u[2] from the first Philox block, FP64 from the second
-  constexpr bool kAnglesInSecond = sizeof(T) == 8;
-  if (R0 < r0) {
-    o.chan = kChanElastic;
-    if (RNG != BZ_RNG_STREAM && kAnglesInSecond) U.second(p.seed, i, step);
-    iso_dir<T>(vmag * (T)p.fel, U.u[1], U.u[2], vx, vy, vz);
-  } else if (XS != BZ_XS_FIT && R0 < r1) {
-    o.chan = kChanExc;
-    if (RNG != BZ_RNG_STREAM && kAnglesInSecond) U.second(p.seed, i, step);
-    const T enew = e - (T)p.e_exc;
-    iso_dir<T>(dsqrt(((T)2 * enew) / (T)p.jm), U.u[1], U.u[2], vx, vy, vz);
-  } else if ((XS != BZ_XS_FIT && R0 < r2) || (XS == BZ_XS_FIT && r0 < R0 && R0 < r2)) {
+  if (R0 < r0) o.chan = kChanElastic;
+  else if (XS != BZ_XS_FIT && R0 < r1) o.chan = kChanExc;
+  else if ((XS != BZ_XS_FIT && R0 < r2) || (XS == BZ_XS_FIT && r0 < R0 && R0 < r2))
     o.chan = kChanIon;
+  if (o.chan != kChanNone) {
+    // one scatter for every channel (a wave with mixed channels runs it
+    // once); FP32 takes the angles u[1], u[2] from the first Philox block,
+    // FP64 from the second; an ionization also draws the secondary's angles
+    constexpr bool kAnglesInSecond = sizeof(T) == 8;
     if (RNG != BZ_RNG_STREAM) {
-      U.second(p.seed, i, step);
-      U.third(p.seed, i, step);
+      if (kAnglesInSecond || o.chan == kChanIon) U.second(p.seed, i, step);
+      if (o.chan == kChanIon) U.third(p.seed, i, step);
     }
-    const T eej = fabs((T)0.5 * (e - (T)p.e_ion));
-    const T vnew = dsqrt(((T)2 * eej) / (T)p.jm);
+    T vnew;
+    if (o.chan == kChanElastic) {
+      vnew = vmag * (T)p.fel;
+    } else {
+      const T enew = o.chan == kChanExc ? e - (T)p.e_exc : fabs((T)0.5 * (e - (T)p.e_ion));
+      vnew = dsqrt(((T)2 * enew) / (T)p.jm);
+    }
     iso_dir<T>(vnew, U.u[1], U.u[2], vx, vy, vz);
-    iso_dir<T>(vnew, U.u[3], U.u[4], tx, ty, tz);
-    o.born = 1;
-    o.s.x = in.x;
-    o.s.vx = tx;
-    o.s.vy = ty;
-    o.s.vz = tz;
-    o.s.w = in.w;
-    o.e_sec = (T)p.kev * (tx * tx + ty * ty + tz * tz);
+    if (o.chan == kChanIon) {
+      iso_dir<T>(vnew, U.u[3], U.u[4], tx, ty, tz);
+      o.born = 1;
+      o.s.x = in.x;
+      o.s.vx = tx;
+      o.s.vy = ty;
+      o.s.vz = tz;
+      o.s.w = in.w;
+      o.e_sec = (T)p.kev * (tx * tx + ty * ty + tz * tz);
+    }
   }
 
   o.n_el = o.chan == kChanElastic;
@@ -519,7 +533,13 @@ __device__ __forceinline__ StepOut<T> flight_particle(const P5<T> in, uint32_t i
 // by integer division by bnf (kernels-new.cu:63 and :8-50).
 __device__ __forceinline__ int dep_node(double x, const bz_params_t& p) {
   const int b = (int)(((x + p.half_dx) / p.L_dx) * p.dep_M);
-  int k = b / p.bnf;
+  // k = b / bnf without the integer divide: the FP32 quotient is within one
+  // of it for |b| < 2^24 and one step corrects it; for b < 0 either rounding
+  // ends at node 0 after the clamp
+  const int bnf = p.bnf;
+  int k = (int)((float)b * (1.0f / (float)bnf));
+  if (k * bnf > b) --k;
+  else if ((k + 1) * bnf <= b) ++k;
   return k < 0 ? 0 : (k >= p.n_nodes ? p.n_nodes - 1 : k);
 }
 

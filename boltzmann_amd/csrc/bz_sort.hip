@@ -95,8 +95,9 @@ struct KeyArgs {
 };
 
 // Key per slot.  Each thread takes kKeyItems slots, all loads issued
-// before use.  The per-key counts are read off the sorted keys afterwards
-// (k_key_counts), so this pass needs neither atomics nor an LDS histogram.
+// before use.  The per-key offsets and counts are read off the sorted keys
+// afterwards (k_key_bounds), so this pass needs neither atomics nor an LDS
+// histogram.
 constexpr int kKeyItems = 4;
 
 template <typename T>
@@ -144,58 +145,35 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(KeyArgs a, const Clock* clk
   }
 }
 
-// Per-key counts from the sorted keys: the first position of every run of
-// equal keys finds the run's end by binary search and writes its length
-// (keys that do not occur keep the zero written before).
-__global__ __launch_bounds__(256) void k_key_counts(const uint32_t* __restrict__ skeys,
-                                                    const Clock* clk, int32_t* counts) {
+// Offsets of every key from the sorted keys in one pass: position s sits
+// between keys kp = skeys[s-1] (or -1) and kc = skeys[s] (or m past the
+// end), so s is the first position of every key in (kp, kc]; keys that do
+// not occur get the start of the next one.  offsets[k] = lower bound of k,
+// k = 0..m, with offsets[m] = n.
+__global__ __launch_bounds__(256) void k_key_bounds(const uint32_t* __restrict__ skeys,
+                                                    const Clock* clk, int m,
+                                                    int32_t* offsets) {
   const int64_t n = clk->n;
-  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < n;
+  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s <= n;
        s += (int64_t)gridDim.x * 256) {
-    const uint32_t k = skeys[s];
-    if (s > 0 && skeys[s - 1] == k) continue;
-    int64_t lo = s + 1, hi = n;  // first position in [s+1, n] with a larger key
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (skeys[mid] > k) hi = mid;
-      else lo = mid + 1;
-    }
-    counts[k] = (int32_t)(lo - s);
+    const int64_t kp = s > 0 ? (int64_t)skeys[s - 1] : -1;
+    int64_t kc = s < n ? (int64_t)skeys[s] : (int64_t)m;
+    kc = kc < m ? kc : m;
+    for (int64_t k = kp + 1; k <= kc; ++k) offsets[k] = (int32_t)s;
   }
+}
+
+// counts[k] = offsets[k+1] - offsets[k], k = 0..m-1
+__global__ __launch_bounds__(256) void k_key_counts(const int32_t* __restrict__ offsets, int m,
+                                                    int32_t* counts) {
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < m; k += gridDim.x * 256)
+    counts[k] = offsets[k + 1] - offsets[k];
 }
 
 __global__ void k_zero_i32(int32_t* p, int64_t m) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
        j += (int64_t)gridDim.x * blockDim.x)
     p[j] = 0;
-}
-
-// exclusive scan of counts[0..m) -> offsets[0..m], single block
-__global__ __launch_bounds__(1024) void k_scan_small(const int32_t* in, int32_t* out, int m) {
-  __shared__ int32_t s_w[16];
-  __shared__ int32_t s_carry;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  if (tid == 0) s_carry = 0;
-  __syncthreads();
-  for (int base = 0; base < m; base += 1024) {
-    const int i = base + tid;
-    const int v = i < m ? in[i] : 0;
-    const int inc = wave_incl_scan(v);
-    if (lane == 63) s_w[wid] = inc;
-    __syncthreads();
-    if (wid == 0) {
-      int t = lane < 16 ? s_w[lane] : 0;
-      t = wave_incl_scan(t);
-      if (lane < 16) s_w[lane] = t;
-    }
-    __syncthreads();
-    const int excl = s_carry + (wid ? s_w[wid - 1] : 0) + inc - v;
-    if (i < m) out[i] = excl;
-    __syncthreads();
-    if (tid == 0) s_carry += s_w[15];
-    __syncthreads();
-  }
-  if (tid == 0) out[m] = s_carry;
 }
 
 // ---------------------------------------------------------------------------
@@ -370,8 +348,6 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
   ka.dim = p->dim;
   ka.n_keys = n_keys;
   timing_begin(env, 1);
-  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, counts,
-                     (int64_t)n_keys + 1);
   auto kk = env.precision == BZ_F64 ? k_keys<double> : k_keys<float>;
   // one thread per kKeyItems slots of the capacity; blocks past the count exit
   const int64_t kgrid = (cap + (int64_t)kSortBlock * kKeyItems - 1) / ((int64_t)kSortBlock * kKeyItems);
@@ -394,7 +370,7 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
     if (passes == 2 && ps == 1) {
       kout = ws.keys_a;
     }
-    hipLaunchKernelGGL(k_zero_i32, dim3(256), dim3(256), 0, env.stream, (int32_t*)ws.hist, m);
+    // every upsweep block writes all 256 of its counts: no clearing needed
     hipLaunchKernelGGL(k_upsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
                        (const uint32_t*)kin, (const Clock*)env.clock, shift, ws.hist, tiles);
     hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)nch), dim3(1024), 0, env.stream,
@@ -410,11 +386,12 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
     kin = kout;
     vin = vout;
   }
-  // counts of every key from the sorted keys, then their exclusive scan
-  hipLaunchKernelGGL(k_key_counts, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
-                     (const uint32_t*)kin, (const Clock*)env.clock, counts);
-  hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, env.stream, (const int32_t*)counts,
-                     offsets, n_keys + 1);
+  // offsets of every key from the sorted keys, then the counts
+  const int mk = n_keys + 1;  // key values 0..n_keys (n_keys: dead slots)
+  hipLaunchKernelGGL(k_key_bounds, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
+                     (const uint32_t*)kin, (const Clock*)env.clock, mk, offsets);
+  hipLaunchKernelGGL(k_key_counts, dim3((mk + 255) / 256), dim3(256), 0, env.stream,
+                     (const int32_t*)offsets, mk, counts);
   rc = check_launch("key counts");
   timing_end(env, 1);
   return rc;

@@ -162,10 +162,27 @@ inline double m_expm1(double v) { return expm1(v); }
 inline float m_expm1(float v) { return expm1f(v); }
 inline double m_sqrt(double v) { return sqrt(v); }
 inline float m_sqrt(float v) { return sqrtf(v); }
-inline double m_sin(double v) { return sin(v); }
-inline float m_sin(float v) { return sinf(v); }
-inline double m_cos(double v) { return cos(v); }
-inline float m_cos(float v) { return cosf(v); }
+// sin(pi v), cos(pi v) with an exact argument reduction: n = rint(2v) and
+// r = v - n/2 in [-1/4, 1/4] are exact, pi*r is one rounding, and the
+// quadrant n mod 4 maps sin/cos(pi r) to the result.  FP32 callers round the
+// FP64 value once.
+inline void m_sincospi(double v, double* s, double* c) {
+  const double n = nearbyint(2.0 * v);
+  const double r = v - 0.5 * n;
+  const double sr = sin(3.141592653589793 * r), cr = cos(3.141592653589793 * r);
+  switch (((long long)n) & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+  }
+}
+inline void m_sincospi(float v, float* s, float* c) {
+  double sd, cd;
+  m_sincospi((double)v, &sd, &cd);
+  *s = (float)sd;
+  *c = (float)cd;
+}
 
 template <typename T>
 T poly(double a, double b, double c, T L) {
@@ -223,9 +240,10 @@ void scatter(T speed, T ra, T rb, T* vx, T* vy, T* vz) {
   T q = (T)1 - cos_chi * cos_chi;
   if (q < (T)0) q = (T)0;
   const T sin_chi = m_sqrt(q);
-  const T phi = (T)(2.0 * 3.141592653589793) * rb;
-  *vx = speed * sin_chi * m_cos(phi);
-  *vy = speed * sin_chi * m_sin(phi);
+  T sphi, cphi;
+  m_sincospi((T)2 * rb, &sphi, &cphi);  // phi = 2 pi rb
+  *vx = speed * sin_chi * cphi;
+  *vy = speed * sin_chi * sphi;
   *vz = speed * cos_chi;
 }
 
