@@ -5,16 +5,16 @@
 
 A step is one kinetic update of the whole particle population.  For the
 discharges (c1, c3, c4) that is the fused collision + push + deposit kernel
-(in place, k_push_1d), the births append, the deposit reduction, the device
+(in place, k_push_inplace), the births append, the deposit reduction, the device
 field step (moment update, fluid ions, Poisson) and, on the steps the
 workload asks for it, binning + merge/split resampling or the periodic
 compaction.  For the single swarm (c0) the push compacts.  For the batched
-swarms (c2) it is one launch over all 256 instances, replayed from a
-captured hipGraph with population control every 50 steps.  The state is
+swarms (c2) it is one in-place launch over all 256 instances, replayed from
+a captured hipGraph with population control every 50 steps.  The state is
 synthetic (Maxwellian, uniform density) and resident in HBM before timing.
 
 Prints one JSON line.  The roofline object is for the dominant kernel
-(k_push_1d, or k_push for the swarms), timed with HIP events on the
+(k_push_inplace; k_push for the single 0-D swarm), timed with HIP events on the
 library's stream; its `traffic` is the HBM bytes per launch measured with
 rocprofv3 (profiles/traffic.json).  The CPU baseline is the oracle port
 timed on a bounded sample of the same workload on the host.
@@ -219,7 +219,7 @@ def bench_engine(args):
                    "particles_end": eng.count(), "cells": cfg.n_cells if cfg.dim == 1 else 0,
                    "dt_s": cfg.dt, "parallelism": "single GPU", "push_schedule": args.sched},
         "roofline": _roofline(bytes_per_pstep, psteps / args.steps, avg_push_s,
-                              "k_push_1d" if cfg.dim == 1 else "k_push", cfg.name),
+                              "k_push_inplace" if cfg.dim == 1 else "k_push", cfg.name),
         "counters": {k: c1[k] - c0[k] for k in ("births", "absorbed", "ionization", "overflow",
                                                 "resample", "split", "merge")},
         "kernel_ms": {"push": ms[0], "sort": ms[1], "resample": ms[2],
@@ -249,7 +249,6 @@ def bench_swarms(args):
     else:
         tuned = tune_swarms(b, steps=10)
     per_graph = 2 * scfg.control_every
-    cur_cap = b.cur
     gid = b.capture(per_graph, accumulate=True, control_every=scfg.control_every)
     reps_w = max(1, args.warmup // per_graph)
     reps = max(1, args.steps // per_graph)
@@ -266,8 +265,6 @@ def bench_swarms(args):
     _lib.check(b.lib.bz_timing_get(b.ctx, ms, nl))
     _lib.check(b.lib.bz_timing_enable(b.ctx, 0))
     n_mid = b.count()
-    if b.cur != cur_cap:  # restore the buffer parity the graph was captured with
-        b.step(accumulate=True)
     b.synchronize()
     c0 = b.counters()
     torch.cuda.synchronize()
@@ -280,7 +277,7 @@ def bench_swarms(args):
     steps = reps * per_graph
     elapsed = t1 - t0
     word = 8 if scfg.precision == _lib.BZ_F64 else 4
-    bytes_per_pstep = 2 * 4 * word  # vx, vy, vz, w read + write
+    bytes_per_pstep = 8 * word  # x (instance), vx, vy, vz, w read; vx, vy, vz written
     avg_push_s = ms[0] / 1e3 / max(int(nl[0]), 1)
     res = b.results()
     out = {
@@ -301,7 +298,7 @@ def bench_swarms(args):
                    "en_td": [scfg.en_td_lo, scfg.en_td_hi], "dt_s": scfg.dt,
                    "graph": f"{per_graph} steps + control every {scfg.control_every}",
                    "parallelism": "single GPU"},
-        "roofline": _roofline(bytes_per_pstep, 0.5 * (n_before + n_mid), avg_push_s, "k_push"),
+        "roofline": _roofline(bytes_per_pstep, 0.5 * (n_before + n_mid), avg_push_s, "k_push_inplace"),
         "counters": {k: c1[k] - c0[k] for k in ("births", "ionization", "split", "merge")},
         "sample_results": {"en_td": [float(res["en_td"][i]) for i in (0, 128, 255)],
                            "mean_energy_ev": [float(res["mean_energy"][i]) for i in (0, 128, 255)]},

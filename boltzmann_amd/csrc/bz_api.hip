@@ -475,8 +475,8 @@ int bz_swarm_step(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const
   if (p->xs_mode != BZ_XS_TABLE || p->rng_mode != BZ_RNG_PHILOX)
     return fail("swarm step: tabulated model and Philox stream only");
   if (n_inst < 1 || n_inst > 1024) return fail("swarm step: 1 <= n_inst <= 1024");
-  if (!in || !out || !xs || !inst_off || !inst_E || !inst_moments)
-    return fail("swarm step: null pointer");
+  if (!in || !out || !xs || !inst_E || !inst_moments) return fail("swarm step: null pointer");
+  if (accumulate && !inst_acc) return fail("swarm step: accumulate needs inst_acc");
   if (in->vx == out->vx) return fail("in and out state must not alias");
   return launch_swarm_step(c->env, in, out, p, xs, n_inst, inst_off, inst_E, inst_moments,
                            inst_acc, accumulate);

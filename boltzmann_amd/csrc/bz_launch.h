@@ -60,8 +60,9 @@ int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t*
 int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out);
 // sum the push's per-block partial rows (m doubles each) into out[m];
 // commit != 0 also sums the counter rows and commits the clock
+// sums the per-block partial rows into out (and adds them to acc when given)
 int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int commit,
-                      int64_t cap);
+                      int64_t cap, double* acc = nullptr);
 // widest partial row any reduction of a context with these sizes produces
 int64_t dep_max_row(int32_t max_nodes, int32_t max_keys);
 int launch_reduction(const LaunchEnv& env, int r, int M, int RM, double* dw, const double* dv,
@@ -76,6 +77,9 @@ int launch_closure_commit(const LaunchEnv& env, int nn, const double* mom, doubl
 int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                     const double* thr, int nb, int32_t* counts, int32_t* offsets,
                     uint32_t* perm);
+// batched swarms: stable sort of the slots by instance (held in x)
+int launch_group_sort(const LaunchEnv& env, const bz_state_t* s, int n_inst, int32_t* counts,
+                      int32_t* offsets, uint32_t* perm, const uint32_t** skeys);
 int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
                     const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
                     const uint32_t* perm, const int32_t* targets, int32_t n_keys,
@@ -88,10 +92,10 @@ int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys);
 }  // namespace bz
 
 namespace bz {
+// batched swarms in place (x = instance), births staged in out
 int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
-                      const bz_params_t* p, const bz_xsec_t* xs, int n_inst,
-                      const int64_t* inst_off, const void* inst_E, double* inst_moments,
-                      int shape);
+                      const bz_params_t* p, const bz_xsec_t* xs, int n_inst, const void* inst_E,
+                      double* inst_moments, double* acc, int shape);
 // resampling kernels on a state already grouped by key (no sort): counts,
 // offsets (int32, n_keys+1 / n_keys+2) and keys-per-position built by the caller
 int launch_resample_grouped(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,

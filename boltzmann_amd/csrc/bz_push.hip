@@ -117,13 +117,13 @@ __device__ __forceinline__ void stage_lds(T* dst, const T* __restrict__ src, int
 
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
+  static_assert(DIM == 0, "the compacting push serves the single 0-D swarm");
   static_assert(ITEMS >= 1 && ITEMS <= 4, "packed scan holds 4 fields of 16 bits");
   constexpr int NW = BLOCK / kWave;
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int LB = 1024 / BLOCK > 0 ? 1024 / BLOCK : 1;  // look-back window 1024 tiles
-  constexpr bool WITHX = DIM == 1;
-  // dynamic LDS: DIM 1 node histograms [3][n_nodes]; DIM 0 [3]; DIM 2 instance
-  // moments [n_inst][kImN], offsets [n_inst+1] and fields [n_inst]; then the table
+  constexpr bool WITHX = false;
+  // dynamic LDS: the three sums, then the table
   extern __shared__ double s_dep[];
   __shared__ uint64_t s_wsum[NW];
   __shared__ uint32_t s_lbmin;
@@ -135,18 +135,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wid = tid / kWave;
-  const int nn = (DIM == 1) ? p.n_nodes : 1;
-  const int ni_ = (DIM == 2) ? a.n_inst : 0;
-  const int dep_words = (DIM == 2) ? ni_ * kImN + (ni_ + 1) + ni_ : 3 * nn;
-  for (int j = tid; j < 3 * nn && DIM != 2; j += BLOCK) s_dep[j] = 0.0;
-  double* s_im = s_dep;
-  int64_t* s_off = (int64_t*)(s_dep + ni_ * kImN);
-  T* s_ie = (T*)(s_off + ni_ + 1);
-  if (DIM == 2) {
-    for (int j = tid; j < ni_ * kImN; j += BLOCK) s_im[j] = 0.0;
-    for (int j = tid; j <= ni_; j += BLOCK) s_off[j] = a.inst_off[j];
-    for (int j = tid; j < ni_; j += BLOCK) s_ie[j] = ((const T*)a.inst_E)[j];
-  }
+  const int dep_words = 3;
   if (tid < kCntRow) s_cnt[tid] = 0ull;
   // stage the cross-section table in LDS (read-only for the whole launch)
   bz_xsec_t xs = a.xs;
@@ -209,19 +198,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const int64_t i = base + (int64_t)k * BLOCK + tid;
-      int inst = 0;
-      T e_local = e_uniform;
-      if (DIM == 2) {
-        // instance of particle i: largest k with off[k] <= i
-        int lo = 0, hi = ni_ - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_off[mid] <= i) lo = mid;
-          else hi = mid - 1;
-        }
-        inst = lo;
-        e_local = s_ie[inst];
-      }
+      const T e_local = e_uniform;
       if (XS == BZ_XS_FLIGHT)
         o[k] = flight_particle<T, DIM>(cur[k], (uint32_t)i, step, p, xs, E, e_local);
       else
@@ -237,66 +214,17 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       }
       const uint32_t cnt = (uint32_t)(o[k].alive + o[k].born);
       packed |= (uint64_t)cnt << (16 * k);
-      if (DIM == 2) {
-        // per-instance moments of this step; a wave whose lanes share one
-        // instance (the common case, instances are contiguous) sums first
-        double mv[kImN];
-        const double w = o[k].alive ? (double)o[k].p.w : 0.0;
-        const double ep = (double)o[k].e_post;
-        const double ws = o[k].born ? (double)o[k].s.w : 0.0;
-        const double es = (double)o[k].e_sec;
-        const double wc = (double)cur[k].w;
-        mv[kImW] = w + ws;
-        mv[kImWE] = w * ep + ws * es;
-        mv[kImWVX] = w * (double)o[k].p.vx + ws * (double)o[k].s.vx;
-        mv[kImEl] = o[k].chan == kChanElastic ? wc : 0.0;
-        mv[kImExc] = o[k].chan == kChanExc ? wc : 0.0;
-        mv[kImIon] = o[k].chan == kChanIon ? wc : 0.0;
-        mv[kImBirth] = (double)o[k].born;
-        mv[kImWE2] = w * ep * ep + ws * es * es;
-        const bool valid = i < n;
-        const int inst0 = __shfl(inst, 0, kWave);
-        const bool uniform = __all(!valid || inst == inst0);
-        if (uniform) {
-#pragma unroll
-          for (int q = 0; q < kImN; ++q) {
-            const double sq = wave_sum(valid ? mv[q] : 0.0);
-            if (lane == 0 && sq != 0.0) atomicAdd(&s_im[inst0 * kImN + q], sq);
-          }
-        } else if (valid) {
-#pragma unroll
-          for (int q = 0; q < kImN; ++q)
-            if (mv[q] != 0.0) atomicAdd(&s_im[inst * kImN + q], mv[q]);
-        }
-        continue;
-      }
       // deposit: survivors at their new position, secondaries where born
       if (o[k].alive) {
         const double w = (double)o[k].p.w;
-        const double ni = (o[k].chan == kChanIon) ? w : 0.0;
-        const double en = w * (double)o[k].e_post;
-        if (DIM == 1) {
-          const int nd = dep_node((double)o[k].p.x, p);
-          atomicAdd(&s_dep[nd], w);
-          if (ni != 0.0) atomicAdd(&s_dep[nn + nd], ni);
-          atomicAdd(&s_dep[2 * nn + nd], en);
-        } else {
-          acc0 += w;
-          acc1 += ni;
-          acc2 += en;
-        }
+        acc0 += w;
+        acc1 += (o[k].chan == kChanIon) ? w : 0.0;
+        acc2 += w * (double)o[k].e_post;
       }
       if (o[k].born) {
         const double w = (double)o[k].s.w;
-        const double en = w * (double)o[k].e_sec;
-        if (DIM == 1) {
-          const int nd = dep_node((double)o[k].s.x, p);
-          atomicAdd(&s_dep[nd], w);
-          atomicAdd(&s_dep[2 * nn + nd], en);
-        } else {
-          acc0 += w;
-          acc2 += en;
-        }
+        acc0 += w;
+        acc2 += w * (double)o[k].e_sec;
       }
     }
 
@@ -355,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
       if (lane == 0 && s) atomicAdd(&s_cnt[q], (unsigned long long)s);
     }
   }
-  if (DIM == 0) {
+  {
     // per-wave sums combined in wave order (no LDS atomics: reproducible)
     __shared__ double s_w0[3][NW];
     acc0 = wave_sum(acc0);
@@ -374,7 +302,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
     }
   }
   __syncthreads();
-  const int m = (DIM == 2) ? ni_ * kImN : 3 * nn;
+  const int m = 3;
   double* part = a.partials + (size_t)blockIdx.x * m;
   for (int j = tid; j < m; j += BLOCK) part[j] = s_dep[j];
   uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * m);
@@ -393,7 +321,8 @@ constexpr int kRedThreads = kRedWaves * 64;
 
 __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks, int m, int rpc,
                                 double* l2, uint32_t* col_done, double* __restrict__ out,
-                                int cx, int cy, int ny, double (*s_part)[64], int* s_last) {
+                                double* acc, int cx, int cy, int ny, double (*s_part)[64],
+                                int* s_last) {
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int j = cx * 64 + lane;
@@ -418,7 +347,10 @@ __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks
 #pragma unroll
     for (int w = 0; w < kRedWaves; ++w) t += s_part[w][lane];
     if (ny == 1) {
-      if (j < m) out[j] = t;
+      if (j < m) {
+        out[j] = t;
+        if (acc) acc[j] += t;
+      }
     } else if (j < m) {
       l2[(size_t)cy * m + j] = t;
       __threadfence();  // the level-2 row is visible before the column count moves
@@ -444,7 +376,10 @@ __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks
         double u = 0.0;
 #pragma unroll
         for (int w = 0; w < kRedWaves; ++w) u += s_part[w][lane];
-        if (j < m) out[j] = u;
+        if (j < m) {
+          out[j] = u;
+          if (acc) acc[j] += u;
+        }
         if (lane == 0) col_done[cx] = 0u;  // ready for the next launch
       }
     }
@@ -477,17 +412,19 @@ __device__ void dep_commit(const double* __restrict__ partials, int nblocks, int
 
 __global__ __launch_bounds__(kRedThreads) void k_dep_reduce(
     const double* __restrict__ partials, int nblocks, int m, int rpc, double* l2,
-    uint32_t* col_done, double* __restrict__ out, Clock* clk, int commit, int64_t cap) {
+    uint32_t* col_done, double* __restrict__ out, double* acc, Clock* clk, int commit,
+    int64_t cap) {
   __shared__ double s_part[kRedWaves][64];
   __shared__ int s_last;
-  dep_reduce_part(partials, nblocks, m, rpc, l2, col_done, out, blockIdx.x, blockIdx.y,
+  dep_reduce_part(partials, nblocks, m, rpc, l2, col_done, out, acc, blockIdx.x, blockIdx.y,
                   gridDim.y, s_part, &s_last);
   if (commit && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 64)
     dep_commit(partials, nblocks, m, clk, cap);
 }
 
 // ---------------------------------------------------------------------------
-// 1-D discharges: the push in place, with no barrier in the tile loop.
+// The push in place, with no barrier in the tile loop: the 1-D discharges
+// (DIM 1) and the batched 0-D swarms (DIM 2).
 //
 // Block b owns tiles [t_first, t_end) (contiguous); within a tile, wave w
 // owns the WSPAN = 64*ITEMS consecutive slots starting at t*TILE + w*WSPAN,
@@ -496,11 +433,25 @@ __global__ __launch_bounds__(kRedThreads) void k_dep_reduce(
 // before this tile's collision physics, survivors are written back to their
 // slots, and births are staged in the wave's own slot range of the staging
 // buffer in slot order, with the count per (tile, wave) stored for
-// k_births_append_1d.  The node histograms live in LDS (FP64 atomics) and
-// are flushed once per block after the loop.
+// k_births_append.  The sums live in LDS (FP64 atomics) and are flushed once
+// per block after the loop: node histograms [3][n_nodes] (DIM 1) or
+// per-instance moments [n_inst][kImN] (DIM 2).
+//
+// Swarms: a 0-D particle has no position, so x holds its instance number
+// (an exact small integer in either precision); the field of the instance is
+// staged in LDS, x and w are never rewritten (no walls in 0-D), and a
+// secondary inherits its parent's x, i.e. its instance.
 // ---------------------------------------------------------------------------
-template <typename T, int XS, int RNG, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
+// LDS doubles before the staged table: DIM 1 histograms, DIM 2 moments and fields
+__host__ __device__ inline int inplace_lds_doubles(int dim, int nn, int n_inst, int word) {
+  if (dim == 1) return table_lds_offset_doubles(nn);
+  const int d = n_inst * kImN + (n_inst * word + 7) / 8;
+  return (d + 1) & ~1;
+}
+
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
+  static_assert(DIM == 1 || DIM == 2, "in-place push: 1-D discharges and batched swarms");
   constexpr int NW = BLOCK / kWave;
   constexpr int WSPAN = kWave * ITEMS;
   constexpr int TILE = BLOCK * ITEMS;
@@ -512,14 +463,19 @@ __global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wid = tid / kWave;
-  const int nn = p.n_nodes;
-  for (int j = tid; j < 3 * nn; j += BLOCK) s_dep[j] = 0.0;
+  const int nn = DIM == 1 ? p.n_nodes : 0;
+  const int n_inst = DIM == 2 ? a.n_inst : 0;
+  const int sums = DIM == 1 ? 3 * nn : n_inst * kImN;
+  for (int j = tid; j < sums; j += BLOCK) s_dep[j] = 0.0;
+  T* s_ie = (T*)(s_dep + sums);  // DIM 2: field of every instance
+  if (DIM == 2)
+    for (int j = tid; j < n_inst; j += BLOCK) s_ie[j] = ((const T*)a.inst_E)[j];
   if (tid < kCntRow) s_cnt[tid] = 0ull;
   if (tid == 0) s_births = 0;
   bz_xsec_t xs = a.xs;
   if (XS != BZ_XS_FIT) {
     const int n = a.xs.n;
-    T* s_tab = (T*)(s_dep + table_lds_offset_doubles(nn));
+    T* s_tab = (T*)(s_dep + inplace_lds_doubles(DIM, nn, n_inst, (int)sizeof(T)));
     int32_t* s_hint = (int32_t*)(s_tab + 7 * n);
     stage_lds<T, BLOCK>(s_tab, (const T*)a.xs.energy, n);
     stage_lds<T, BLOCK>(s_tab + n, (const T*)a.xs.sig, 3 * n);
@@ -547,7 +503,14 @@ __global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
 
   uint32_t c_el = 0, c_ex = 0, c_io = 0, c_abs = 0, c_null = 0, c_over = 0, c_live = 0;
   uint32_t w_births = 0;  // this wave's births over its tiles
-  __syncthreads();  // histograms cleared, table staged
+  // DIM 2: per-lane running moments of the wave's current instance m_inst
+  // (wave-uniform, -1: none); summed over the wave and added to LDS only
+  // when the wave moves on to another instance
+  double macc[kImN];
+#pragma unroll
+  for (int q = 0; q < kImN; ++q) macc[q] = 0.0;
+  int m_inst = -1;
+  __syncthreads();  // sums cleared, fields and table staged
 
   P5<T> cur[ITEMS];
   {
@@ -568,11 +531,19 @@ __global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
     for (int k = 0; k < ITEMS; ++k) {
       const int64_t i = bw + k * kWave + lane;
       const bool live = i < n && cur[k].w > (T)0;
+      int inst = 0;
+      T e_local = (T)0;
+      if (DIM == 2) {
+        inst = (int)cur[k].x;
+        inst = inst < 0 ? 0 : (inst >= n_inst ? n_inst - 1 : inst);
+        e_local = s_ie[inst];
+      }
+      constexpr int PD = DIM == 1 ? 1 : 0;  // physics dimension
       StepOut<T> o;
       if (XS == BZ_XS_FLIGHT)
-        o = flight_particle<T, 1>(cur[k], (uint32_t)i, step, p, xs, E, (T)0);
+        o = flight_particle<T, PD>(cur[k], (uint32_t)i, step, p, xs, E, e_local);
       else
-        o = step_particle<T, XS, 1, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, (T)0);
+        o = step_particle<T, XS, PD, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, e_local);
       if (live) {
         ++c_live;
         c_el += o.n_el;
@@ -581,7 +552,11 @@ __global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
         c_null += o.n_null;
         c_abs += !o.alive;
         c_over += o.over;
-        if (o.alive) {
+        if (DIM == 2) {
+          st[1][i] = o.p.vx;
+          st[2][i] = o.p.vy;
+          st[3][i] = o.p.vz;
+        } else if (o.alive) {
           st[0][i] = o.p.x;
           st[1][i] = o.p.vx;
           st[2][i] = o.p.vy;
@@ -595,15 +570,61 @@ __global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
           st[4][i] = (T)0;
         }
       }
+      if (DIM == 2) {
+        // per-instance moments of this step; a wave whose lanes share one
+        // instance (the common case: instances are contiguous between
+        // population controls) sums in registers first
+        double mv[kImN];
+        const double w = live ? (double)o.p.w : 0.0;
+        const double ep = (double)o.e_post;
+        const double ws = live && o.born ? (double)o.s.w : 0.0;
+        const double es = (double)o.e_sec;
+        const double wc = live ? (double)cur[k].w : 0.0;
+        mv[kImW] = w + ws;
+        mv[kImWE] = w * ep + ws * es;
+        mv[kImWVX] = w * (double)o.p.vx + ws * (double)o.s.vx;
+        mv[kImEl] = o.chan == kChanElastic ? wc : 0.0;
+        mv[kImExc] = o.chan == kChanExc ? wc : 0.0;
+        mv[kImIon] = o.chan == kChanIon ? wc : 0.0;
+        mv[kImBirth] = live ? (double)o.born : 0.0;
+        mv[kImWE2] = w * ep * ep + ws * es * es;
+        const uint64_t lm = __ballot(live);
+        if (lm) {
+          if (!__all(!live || inst == m_inst)) {
+            // the wave leaves m_inst: flush its running sums
+            if (m_inst >= 0) {
+#pragma unroll
+              for (int q = 0; q < kImN; ++q) {
+                const double sq = wave_sum(macc[q]);
+                macc[q] = 0.0;
+                if (lane == 0 && sq != 0.0) atomicAdd(&s_dep[m_inst * kImN + q], sq);
+              }
+            }
+            const int inst0 = __shfl(inst, (int)__builtin_ctzll(lm), kWave);
+            m_inst = __all(!live || inst == inst0) ? inst0 : -1;
+          }
+          if (m_inst >= 0) {
+#pragma unroll
+            for (int q = 0; q < kImN; ++q) macc[q] += mv[q];
+          } else if (live) {
+            // lanes of several instances (a boundary or the births)
+#pragma unroll
+            for (int q = 0; q < kImN; ++q)
+              if (mv[q] != 0.0) atomicAdd(&s_dep[inst * kImN + q], mv[q]);
+          }
+        }
+      }
       const bool born = live && o.born;
       const uint64_t m = __ballot(born);
       if (born) {
         const uint32_t r = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         store5<T, true>(a.out, bw + tb + r, o.s);
-        const double w = (double)o.s.w;
-        const int nd = dep_node((double)o.s.x, p);
-        atomicAdd(&s_dep[nd], w);
-        atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+        if (DIM == 1) {
+          const double w = (double)o.s.w;
+          const int nd = dep_node((double)o.s.x, p);
+          atomicAdd(&s_dep[nd], w);
+          atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+        }
       }
       tb += (uint32_t)__popcll(m);
     }
@@ -613,6 +634,13 @@ __global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
     for (int k = 0; k < ITEMS; ++k) cur[k] = nxt[k];
   }
 
+  if (DIM == 2 && m_inst >= 0) {
+#pragma unroll
+    for (int q = 0; q < kImN; ++q) {
+      const double sq = wave_sum(macc[q]);
+      if (lane == 0 && sq != 0.0) atomicAdd(&s_dep[m_inst * kImN + q], sq);
+    }
+  }
   {
     const uint32_t v[kCntRow] = {c_io, c_abs, c_el, c_ex, c_io, c_null, 0u, c_over, c_live, 0u};
 #pragma unroll
@@ -623,15 +651,14 @@ __global__ __launch_bounds__(BLOCK) void k_push_1d(PushArgs a) {
     if (lane == 0 && w_births) atomicAdd(&s_births, w_births);
   }
   __syncthreads();
-  const int m = 3 * nn;
-  double* part = a.partials + (size_t)blockIdx.x * m;
-  for (int j = tid; j < m; j += BLOCK) part[j] = s_dep[j];
-  uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * m);
+  double* part = a.partials + (size_t)blockIdx.x * sums;
+  for (int j = tid; j < sums; j += BLOCK) part[j] = s_dep[j];
+  uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * sums);
   if (tid < kCntRow) bcnt[(size_t)blockIdx.x * kCntRow + tid] = (uint32_t)s_cnt[tid];
   if (tid == 0) a.block_births[blockIdx.x] = s_births;
 }
 
-// After k_push_1d: the births of push block b go after the slots in use,
+// After k_push_inplace: the births of push block b go after the slots in use,
 // behind the births of blocks 0..b-1 (summed here by this block itself), in
 // the order (tile, wave, slot), which is parent order.  The block handling
 // the last push block sets the slot count of the new state; births past the
@@ -702,7 +729,7 @@ __device__ void births_part(const PushArgs& a, int b, int nblk, int tile, int nw
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_births_append_1d(PushArgs a, int tile, int nw) {
+__global__ __launch_bounds__(256) void k_births_append(PushArgs a, int tile, int nw) {
   __shared__ uint32_t s_red[4];
   births_part<T, 256>(a, blockIdx.x, gridDim.x, tile, nw, s_red);
 }
@@ -850,7 +877,7 @@ namespace bz {
 
 // Launch of k_dep_reduce: rows per chunk so that the level-2 rows fit.
 int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int commit,
-                      int64_t cap) {
+                      int64_t cap, double* acc) {
   if (m < 1) return 0;
   // one level up to 256 rows (at most 16 rows per thread), two above
   int64_t rpc = 256;
@@ -863,7 +890,7 @@ int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int
   if (nx > env.dep_cols) return fail("deposit reduction: %d output columns > %d", nx, env.dep_cols);
   hipLaunchKernelGGL(k_dep_reduce, dim3(nx, ny > 0 ? ny : 1), dim3(kRedWaves * 64), 0, env.stream,
                      (const double*)env.partials, nblocks, m, (int)rpc, env.dep_l2, env.dep_cols_done,
-                     out, env.clock, commit, cap);
+                     out, acc, env.clock, commit, cap);
   return check_launch("k_dep_reduce");
 }
 
@@ -889,14 +916,15 @@ int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys) {
   return d;
 }
 
-// 1-D discharges: k_push_1d in place, then k_births_append_1d
-template <typename T, int XS, int RNG, int BLOCK, int ITEMS>
-static int launch_push_1d_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
-  auto kern = k_push_1d<T, XS, RNG, BLOCK, ITEMS>;
-  const int part_doubles = 3 * a.p.n_nodes;
-  size_t smem = (size_t)table_lds_offset_doubles(a.p.n_nodes) * sizeof(double);
+// 1-D discharges and batched swarms: k_push_inplace, then k_births_append
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
+static int launch_push_inplace_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
+  auto kern = k_push_inplace<T, XS, DIM, RNG, BLOCK, ITEMS>;
+  const int nn = DIM == 1 ? a.p.n_nodes : 0;
+  const int part_doubles = DIM == 1 ? 3 * nn : a.n_inst * kImN;
+  size_t smem = (size_t)inplace_lds_doubles(DIM, nn, a.n_inst, (int)sizeof(T)) * sizeof(double);
   if (XS != BZ_XS_FIT) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
-  if (smem > 160 * 1024) return fail("push: deposit + table need %zu B of LDS (> 160 KiB)", smem);
+  if (smem > 160 * 1024) return fail("push: sums + table need %zu B of LDS (> 160 KiB)", smem);
   if (smem > 64 * 1024) {
     if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)smem) != hipSuccess)
@@ -912,7 +940,7 @@ static int launch_push_1d_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
   const int64_t cap_parts = env.partials_doubles / (part_doubles + kCntRow / 2);
   if (grid > cap_parts) grid = (int)cap_parts;
   if (grid > kMaxPushBlocks) grid = kMaxPushBlocks;
-  if (grid < 1) return fail("deposit partial buffer too small");
+  if (grid < 1) return fail("partial sum buffer too small");
   // per-(tile, wave) birth counts follow the per-block ones in the status words
   const int64_t max_tiles = env.capacity / (BLOCK * ITEMS) + 2;
   if (kMaxPushBlocks + max_tiles * (BLOCK / kWave) > 2 * env.status_words)
@@ -921,11 +949,11 @@ static int launch_push_1d_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
   timing_begin(env, 0);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
   timing_end(env, 0);
-  int rc = check_launch("k_push_1d");
+  int rc = check_launch("k_push_inplace");
   if (rc) return rc;
-  hipLaunchKernelGGL(k_births_append_1d<T>, dim3(grid), dim3(256), 0, env.stream, a,
+  hipLaunchKernelGGL(k_births_append<T>, dim3(grid), dim3(256), 0, env.stream, a,
                      BLOCK * ITEMS, BLOCK / kWave);
-  rc = check_launch("k_births_append_1d");
+  rc = check_launch("k_births_append");
   if (rc) return rc;
   *grid_out = grid;
   return 0;
@@ -933,16 +961,15 @@ static int launch_push_1d_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
 
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
-  // the 1-D discharges update in place; swarms compact
-  if constexpr (DIM == 1) {
-    return launch_push_1d_t<T, XS, RNG, BLOCK, ITEMS>(env, a, grid_out);
+  // the 1-D discharges and the batched swarms update in place; a single
+  // 0-D swarm compacts
+  if constexpr (DIM != 0) {
+    return launch_push_inplace_t<T, XS, DIM, RNG, BLOCK, ITEMS>(env, a, grid_out);
   } else {
     auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS>;
-    // same layout as the kernel's dep_words (the field slots are a double each)
-    const int dep_doubles = DIM == 2 ? a.n_inst * kImN + (a.n_inst + 1) + a.n_inst
-                                     : 3 * (DIM == 1 ? a.p.n_nodes : 1);
-    const int part_doubles = DIM == 2 ? a.n_inst * kImN : 3 * (DIM == 1 ? a.p.n_nodes : 1);
-    size_t smem = (size_t)dep_doubles * sizeof(double);
+    // the kernel's three sums, then the table
+    const int part_doubles = 3;
+    size_t smem = (size_t)4 * sizeof(double);
     if (XS != BZ_XS_FIT) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
     if (smem > 160 * 1024) return fail("push: deposit + table need %zu B of LDS (> 160 KiB)", smem);
     if (smem > 64 * 1024) {
@@ -1219,9 +1246,8 @@ int launch_fill_uniforms(const LaunchEnv& env, void* R, int64_t stride, int64_t 
 namespace bz {
 
 int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
-                      const bz_params_t* p, const bz_xsec_t* xs, int n_inst,
-                      const int64_t* inst_off, const void* inst_E, double* inst_moments,
-                      int shape) {
+                      const bz_params_t* p, const bz_xsec_t* xs, int n_inst, const void* inst_E,
+                      double* inst_moments, double* acc, int shape) {
   PushArgs a;
   a.p = *p;
   a.xs = *xs;
@@ -1238,15 +1264,15 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
   a.status = env.status;
   a.partials = env.partials;
   a.capacity = env.capacity;
-  a.inst_off = inst_off;
+  a.inst_off = nullptr;
   a.inst_E = inst_E;
   a.n_inst = n_inst;
   int grid = 0;
   const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &grid)
                                          : launch_push_prec<float>(env, a, shape, &grid);
   if (rc) return rc;
-  const int m = n_inst * kImN;
-  return launch_dep_reduce(env, grid, m, inst_moments, 1, (int64_t)env.capacity);
+  return launch_dep_reduce(env, grid, n_inst * kImN, inst_moments, 1, (int64_t)env.capacity,
+                           acc);
 }
 
 }  // namespace bz

@@ -326,6 +326,54 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(const uint32_t* keys_i
   }
 }
 
+// Stable sort of the keys in ws.keys_a (slots [0, n)) over key values
+// 0..n_keys, then the offsets and counts of every key.  The permutation goes
+// to perm; the sorted keys end in keys_b (one 8-bit pass) or keys_a (two),
+// returned in *skeys.
+static int sort_keys(const LaunchEnv& env, const SortWs& ws, int n_keys, int32_t* counts,
+                     int32_t* offsets, uint32_t* perm, const uint32_t** skeys) {
+  if (n_keys + 1 > 65535) return fail("sort: at most 65534 keys (two 8-bit passes)");
+  const int passes = (n_keys + 1) <= 256 ? 1 : 2;
+  const int64_t tiles = ws.max_tiles;
+  const int64_t m = 256 * tiles;
+  const int64_t nch = (m + kScanChunk - 1) / kScanChunk;
+  // the first of two passes parks the permutation in vals_b, or in sel when
+  // the caller's perm is vals_b itself (sel is free until the resampling)
+  uint32_t* park = perm == ws.vals_b ? ws.sel : ws.vals_b;
+  uint32_t* kin = ws.keys_a;
+  const uint32_t* vin = nullptr;
+  int rc = 0;
+  for (int ps = 0; ps < passes; ++ps) {
+    const int shift = 8 * ps;
+    const bool last = ps == passes - 1;
+    uint32_t* kout = ps == 0 ? ws.keys_b : ws.keys_a;
+    uint32_t* vout = last ? perm : park;
+    // every upsweep block writes all 256 of its counts: no clearing needed
+    hipLaunchKernelGGL(k_upsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
+                       (const uint32_t*)kin, (const Clock*)env.clock, shift, ws.hist, tiles);
+    hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)nch), dim3(1024), 0, env.stream,
+                       (const uint32_t*)ws.hist, m, ws.chunks);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, env.stream, ws.chunks, nch);
+    hipLaunchKernelGGL(k_chunk_apply, dim3((unsigned)nch), dim3(1024), 0, env.stream, ws.hist, m,
+                       (const uint32_t*)ws.chunks);
+    hipLaunchKernelGGL(k_downsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
+                       (const uint32_t*)kin, vin, kout, vout, (const Clock*)env.clock, shift,
+                       (const uint32_t*)ws.hist, tiles, ps == 0 ? 1 : 0);
+    rc = check_launch("radix pass");
+    if (rc) return rc;
+    kin = kout;
+    vin = vout;
+  }
+  // offsets of every key from the sorted keys, then the counts
+  const int mk = n_keys + 1;  // key values 0..n_keys (n_keys: dead slots)
+  hipLaunchKernelGGL(k_key_bounds, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
+                     (const uint32_t*)kin, (const Clock*)env.clock, mk, offsets);
+  hipLaunchKernelGGL(k_key_counts, dim3((mk + 255) / 256), dim3(256), 0, env.stream,
+                     (const int32_t*)offsets, mk, counts);
+  if (skeys) *skeys = kin;
+  return check_launch("key counts");
+}
+
 int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                     const double* thr, int nb, int32_t* counts, int32_t* offsets,
                     uint32_t* perm) {
@@ -355,46 +403,43 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
                      ka, (const Clock*)env.clock, ws.keys_a);
   int rc = check_launch("k_keys");
   if (rc) return rc;
-  // passes: 8 bits each over key range [0, n_keys]
-  const int passes = (n_keys + 1) <= 256 ? 1 : 2;
-  const int64_t tiles = ws.max_tiles;
-  const int64_t m = 256 * tiles;
-  const int64_t nch = (m + kScanChunk - 1) / kScanChunk;
-  uint32_t* kin = ws.keys_a;
-  const uint32_t* vin = nullptr;
-  for (int ps = 0; ps < passes; ++ps) {
-    const int shift = 8 * ps;
-    const bool last = ps == passes - 1;
-    uint32_t* kout = last ? (passes == 1 ? ws.keys_b : ws.keys_b) : ws.keys_b;
-    uint32_t* vout = last ? perm : ws.vals_b;
-    if (passes == 2 && ps == 1) {
-      kout = ws.keys_a;
-    }
-    // every upsweep block writes all 256 of its counts: no clearing needed
-    hipLaunchKernelGGL(k_upsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
-                       (const uint32_t*)kin, (const Clock*)env.clock, shift, ws.hist, tiles);
-    hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)nch), dim3(1024), 0, env.stream,
-                       (const uint32_t*)ws.hist, m, ws.chunks);
-    hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, env.stream, ws.chunks, nch);
-    hipLaunchKernelGGL(k_chunk_apply, dim3((unsigned)nch), dim3(1024), 0, env.stream, ws.hist, m,
-                       (const uint32_t*)ws.chunks);
-    hipLaunchKernelGGL(k_downsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
-                       (const uint32_t*)kin, vin, kout, vout, (const Clock*)env.clock, shift,
-                       (const uint32_t*)ws.hist, tiles, ps == 0 ? 1 : 0);
-    rc = check_launch("radix pass");
-    if (rc) return rc;
-    kin = kout;
-    vin = vout;
-  }
-  // offsets of every key from the sorted keys, then the counts
-  const int mk = n_keys + 1;  // key values 0..n_keys (n_keys: dead slots)
-  hipLaunchKernelGGL(k_key_bounds, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
-                     (const uint32_t*)kin, (const Clock*)env.clock, mk, offsets);
-  hipLaunchKernelGGL(k_key_counts, dim3((mk + 255) / 256), dim3(256), 0, env.stream,
-                     (const int32_t*)offsets, mk, counts);
-  rc = check_launch("key counts");
+  rc = sort_keys(env, ws, n_keys, counts, offsets, perm, nullptr);
   timing_end(env, 1);
   return rc;
+}
+
+// Batched swarms: key = instance (held in x), n_inst for a dead slot.
+template <typename T>
+__global__ __launch_bounds__(256) void k_inst_keys(const T* __restrict__ x,
+                                                   const T* __restrict__ w, const Clock* clk,
+                                                   int n_inst, uint32_t* keys) {
+  const int64_t n = clk->n;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    uint32_t key = (uint32_t)n_inst;
+    if (w[i] > (T)0) {
+      int k = (int)x[i];
+      key = (uint32_t)(k < 0 ? 0 : (k >= n_inst ? n_inst - 1 : k));
+    }
+    keys[i] = key;
+  }
+}
+
+int launch_group_sort(const LaunchEnv& env, const bz_state_t* s, int n_inst, int32_t* counts,
+                      int32_t* offsets, uint32_t* perm, const uint32_t** skeys) {
+  SortWs ws = carve(env.sort_ws, env.capacity, env.max_keys);
+  const unsigned grid = (unsigned)(env.num_cu * 8);
+  if (env.precision == BZ_F64)
+    hipLaunchKernelGGL(k_inst_keys<double>, dim3(grid), dim3(256), 0, env.stream,
+                       (const double*)s->x, (const double*)s->w, (const Clock*)env.clock, n_inst,
+                       ws.keys_a);
+  else
+    hipLaunchKernelGGL(k_inst_keys<float>, dim3(grid), dim3(256), 0, env.stream,
+                       (const float*)s->x, (const float*)s->w, (const Clock*)env.clock, n_inst,
+                       ws.keys_a);
+  const int rc = check_launch("k_inst_keys");
+  if (rc) return rc;
+  return sort_keys(env, ws, n_inst, counts, offsets, perm, skeys);
 }
 
 // ---------------------------------------------------------------------------

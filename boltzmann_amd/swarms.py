@@ -2,10 +2,13 @@
 
 Hundreds of independent swarms, each at its own reduced field E/N and with
 its own particle count, advance together: one fused launch steps every
-instance (bz_swarm_step), and a captured hipGraph replays many steps with no
-host involvement.  Population control (bz_swarm_control) keeps each swarm
+instance in place (bz_swarm_step; a 0-D particle has no position, so x holds
+its instance number and secondaries appended at the end inherit it), and a
+captured hipGraph replays many steps with no host involvement.  Population
+control (bz_swarm_control) sorts the slots by instance and keeps each swarm
 near its initial size with the ES-WPNC merge / split, which conserves
-weight, momentum and energy.  Per instance the run yields the mean energy,
+weight, momentum and energy; afterwards instance k owns the slot range
+[off[k], off[k+1]).  Per instance the run yields the mean energy,
 the mobility, the rate coefficients of every collision channel and the
 EEDF, the inputs of the learned closure.
 
@@ -120,6 +123,8 @@ class SwarmBatch:
         self.counts_ws = torch.zeros(self.n_inst + 1, dtype=torch.int32, device=self.dev)
         self.offsets_ws = torch.zeros(self.n_inst + 2, dtype=torch.int32, device=self.dev)
         self.policy = _lib.ResamplePolicy(0.5, 1.5, 1.0, 1.5, 0.0, 1.0, 0, 0, 0, 0)
+        # regrouping only: no instance is split or merged
+        self.group_policy = _lib.ResamplePolicy(0.0, 1e300, 1.0, 1e300, 0.0, 1.0, 0, 0, 0, 0)
         self.acc_steps = 0
         self.graphs = {}
         self.steps_done = 0
@@ -136,27 +141,29 @@ class SwarmBatch:
                 v = rng.normal(0.0, sv, n)
                 self.bufs[self.cur][k][:n].copy_(torch.from_numpy(v).to(self.tdtype))
             self.bufs[self.cur][4][:n].fill_(1.0)
-            self.bufs[self.cur][0][:n].zero_()
+            inst = np.repeat(np.arange(self.n_inst), self.counts0)
+            self.bufs[self.cur][0][:n].copy_(torch.from_numpy(inst).to(self.tdtype))
         self.stream.synchronize()
         _lib.check(self.lib.bz_clock_set(self.ctx, n, 0, 0.0))
 
     def _step_call(self, accumulate: bool):
+        # in place; the other buffer stages the secondaries
         src, dst = self.structs[self.cur], self.structs[1 - self.cur]
         _lib.check(self.lib.bz_swarm_step(self.ctx, C.byref(src), C.byref(dst),
                                           C.byref(self.params), C.byref(self.xs), self.n_inst,
-                                          _tptr(self.inst_off), _tptr(self.inst_E),
+                                          None, _tptr(self.inst_E),
                                           _tptr(self.moments), _tptr(self.acc),
                                           1 if accumulate else 0), "bz_swarm_step")
         _lib.check(self.lib.bz_clock_advance(self.ctx, self.cfg.dt))
-        self.cur = 1 - self.cur
 
-    def _control_call(self):
+    def _control_call(self, policy=None):
         src, dst = self.structs[self.cur], self.structs[1 - self.cur]
+        pol = self.policy if policy is None else policy
         _lib.check(self.lib.bz_swarm_control(self.ctx, C.byref(src), C.byref(dst),
                                              C.byref(self.params), self.n_inst,
                                              _tptr(self.inst_off), _tptr(self.targets),
                                              _tptr(self.counts_ws), _tptr(self.offsets_ws),
-                                             C.byref(self.policy)), "bz_swarm_control")
+                                             C.byref(pol)), "bz_swarm_control")
         self.cur = 1 - self.cur
 
     def step(self, accumulate: bool = False):
@@ -168,14 +175,19 @@ class SwarmBatch:
     def control(self):
         self._control_call()
 
+    def group(self):
+        """Sort the slots by instance without splitting or merging anything,
+        so that offsets() describes the state (what eedf() reads)."""
+        self._control_call(self.group_policy)
+
     def capture(self, n_steps: int, accumulate: bool, with_control: bool = True,
                 control_every: int = 0) -> int:
         """Capture n_steps steps into a hipGraph.
 
-        Every step and every control swaps the ping-pong buffers, so the
-        captured sequence must swap an even number of times to leave the
-        state where the next replay expects it.  control_every > 0 puts a
-        population control after every control_every steps; otherwise
+        A step works in place; every control swaps the ping-pong buffers, so
+        the captured sequence must hold an even number of controls to leave
+        the state where the next replay expects it.  control_every > 0 puts
+        a population control after every control_every steps; otherwise
         with_control appends two controls (the second one finds every
         instance inside its band and only copies)."""
         if control_every:
@@ -184,8 +196,8 @@ class SwarmBatch:
             plan = ([0] * control_every + [1]) * (n_steps // control_every)
         else:
             plan = [0] * n_steps + ([1, 1] if with_control else [])
-        if len(plan) % 2:
-            raise ValueError("the captured sequence must swap the buffers an even number of times")
+        if sum(plan) % 2:
+            raise ValueError("the captured sequence must hold an even number of controls")
         cur0 = self.cur
         _lib.check(self.lib.bz_graph_begin(self.ctx))
         try:
@@ -198,11 +210,15 @@ class SwarmBatch:
             gid = C.c_int32(-1)
             _lib.check(self.lib.bz_graph_end(self.ctx, C.byref(gid)))
         self.cur = cur0
-        self.graphs[gid.value] = (n_steps, accumulate)
+        self.graphs[gid.value] = (n_steps, accumulate, cur0)
         return gid.value
 
     def replay(self, gid: int, times: int = 1):
-        n_steps, accumulate = self.graphs[gid]
+        n_steps, accumulate, cur0 = self.graphs[gid]
+        if self.cur != cur0:
+            # the state sits in the other buffer (an odd number of controls
+            # since the capture): one regroup moves it back
+            self.group()
         for _ in range(times):
             _lib.check(self.lib.bz_graph_launch(self.ctx, gid))
             self.steps_done += n_steps
@@ -236,8 +252,15 @@ class SwarmBatch:
         return {k: int(cnt[i]) for i, k in enumerate(_lib.COUNTER_NAMES)}
 
     def offsets(self) -> np.ndarray:
+        """Slot range of every instance as the last control() / group() left
+        it (steps since then append secondaries after the last range)."""
         self.synchronize()
         return self.inst_off.cpu().numpy()
+
+    def instance_counts(self) -> np.ndarray:
+        """Live particles per instance, from the instance numbers in x."""
+        x, _, _, _, w = self.state()
+        return np.bincount(x[w > 0].astype(np.int64), minlength=self.n_inst)
 
     def state(self):
         self.synchronize()
@@ -273,7 +296,10 @@ class SwarmBatch:
         }
 
     def eedf(self, n_bins: int = 200, e_lo: float = 0.01, e_hi: float = 1000.0):
-        """EEDF per instance on log-spaced bins, normalised to unit integral (1/eV)."""
+        """EEDF per instance on log-spaced bins, normalised to unit integral (1/eV).
+        The state is regrouped by instance first (group(): a stable sort, no
+        particle changes)."""
+        self.group()
         with torch.cuda.stream(self.stream):
             out = torch.zeros(self.n_inst * n_bins, dtype=torch.float64, device=self.dev)
         s = self.structs[self.cur]

@@ -294,9 +294,13 @@ int bz_resample(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
 
 /* ---- Batched 0-D swarms (no counterpart in the reference: its 0-D test
  * problems are single runs made by hand-editing the driver, README.md:7-8) ----
- * All instances live in one SoA state; instance k owns [inst_off[k],
- * inst_off[k+1]) (int64 device array, n_inst+1 entries, updated in place).
- * x is not used.  inst_E: [n_inst] field (V/cm) in the context precision.
+ * All instances live in one SoA state.  A 0-D particle has no position, so
+ * x holds its instance number k (0 <= k < n_inst, exact in either
+ * precision).  inst_E: [n_inst] field (V/cm) in the context precision.
+ * bz_swarm_step advances every slot in place (velocities only: 0-D has no
+ * walls, x and w are kept) and appends the secondaries after the slots in
+ * use, in parent order, each in its parent's instance; `out` is staging for
+ * them.  inst_off is neither read nor written by the step (it may be NULL).
  * inst_moments: [n_inst][8] FP64 of this step: sum w, sum w e, sum w vx,
  * weights of elastic / excitation / ionization events, births, sum w e^2.
  * inst_acc (optional): running sums of inst_moments when accumulate != 0. */
@@ -305,14 +309,18 @@ int bz_swarm_step(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
                   const void* inst_E, double* inst_moments, double* inst_acc,
                   int32_t accumulate);
 /* Population control of every instance towards targets[k] with the ES-WPNC
- * merge / split of bz_resample (instances are already grouped: no sort).
- * counts_ws [n_inst+1] and offsets_ws [n_inst+2] are int32 device scratch. */
+ * merge / split of bz_resample: the slots of `in` are sorted by instance
+ * (stable), each instance is merged, split or copied into `out`, and
+ * inst_off [n_inst+1] (int64 device array) receives the new layout: instance
+ * k owns [inst_off[k], inst_off[k+1]) of `out`.  counts_ws [n_inst+1] and
+ * offsets_ws [n_inst+2] are int32 device scratch. */
 int bz_swarm_control(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
                      const bz_params_t* p, int32_t n_inst, int64_t* inst_off,
                      const int32_t* targets, int32_t* counts_ws, int32_t* offsets_ws,
                      const bz_resample_policy_t* pol);
 /* EEDF per instance: sum of weights in n_bins log-spaced energy bins on
- * [e_lo, e_hi] eV (out [n_inst][n_bins] FP64). */
+ * [e_lo, e_hi] eV (out [n_inst][n_bins] FP64).  Reads the grouped layout
+ * inst_off that bz_swarm_control wrote for s. */
 int bz_swarm_eedf(bz_ctx_t ctx, const bz_state_t* s, int32_t n_inst, const int64_t* inst_off,
                   int32_t n_bins, double e_lo, double e_hi, double kev, double* out);
 

@@ -27,38 +27,50 @@ def _batch(prec, **kw):
 
 @pytest.mark.parametrize("prec", [8, 4])
 def test_batched_step_matches_oracle_per_instance(gpu, prec):
+    """One launch over every instance against the oracle run instance by
+    instance.  The step works in place: slot i keeps its particle (x holds
+    the instance number, w is kept) and the secondaries are appended after
+    the slots in use in parent order, each carrying its parent's instance.
+    The Philox stream is keyed by the slot, so the oracle of instance k gets
+    the uniforms of k's slots."""
     b = _batch(prec)
     dtype = np.float64 if prec == 8 else np.float32
     st = b.state()
-    off = b.offsets()
     tab = (b.table.energy, b.table.sig, b.table.slope)
+    rt_w = 1e-12 if prec == 8 else 1e-6
+    rt_v = 1e-12 if prec == 8 else 2e-6
     for step in range(3):
+        n0 = st[0].size
         b.step(accumulate=False)
         got = b.state()
-        goff = b.offsets()
         mom = b.moments.cpu().numpy().reshape(b.n_inst, 8)
-        R = O.collision_uniforms(b.cfg.seed, step, st[0].size, dtype)
-        outs = []
+        R = O.collision_uniforms(b.cfg.seed, step, n0, dtype)
+        inst = st[0].astype(np.int64)
+        g_inst_births = got[0][n0:].astype(np.int64)
+        np.testing.assert_array_equal(got[0][:n0], st[0])  # x (instance) kept in place
+        np.testing.assert_array_equal(got[4][:n0], st[4])  # no walls in 0-D
+        n_births = 0
         for k in range(b.n_inst):
-            sl = slice(off[k], off[k + 1])
+            sk = np.flatnonzero(inst == k)
             s = O.Setup(dim=0, dt=b.cfg.dt, xs_mode=1, gas_density_m3=b.cfg.gas_density_m3,
-                        nu_max=b.table.nu_max, e_field0=float(b.e_vcm[k]))
-            part = tuple(np.ascontiguousarray(a[sl]) for a in st)
-            o, wm, wc = O.push_step(s, part, None, np.ascontiguousarray(R[:, sl]), table=tab,
-                                    capacity=4 * part[0].size + 16)
-            outs.append(o)
-            assert goff[k + 1] - goff[k] == o[0].size, (step, k)
-            assert mom[k, 6] == wc[0]
-            rt = 1e-12 if prec == 8 else 1e-6
-            assert abs(mom[k, 0] - wm[0, 0]) <= rt * wm[0, 0]
-            assert abs(mom[k, 1] - wm[2, 0]) <= rt * wm[2, 0]
-        want = tuple(np.concatenate([o[j] for o in outs]) for j in range(5))
-        rt = 1e-12 if prec == 8 else 2e-6
-        vs = np.sqrt(sum(want[j].astype(np.float64) ** 2 for j in (1, 2, 3))) + 1
-        for j in (1, 2, 3):
-            assert np.all(np.abs(got[j] - want[j]) <= rt * vs), (step, j)
-        np.testing.assert_array_equal(got[4], want[4])
-        st, off = got, goff
+                        nu_max=b.table.nu_max, e_field0=float(b.e_vcm[k]), births_last=True)
+            part = tuple(np.ascontiguousarray(a[sk]) for a in st)
+            o, wm, wc = O.push_step(s, part, None, np.ascontiguousarray(R[:, sk]), table=tab,
+                                    capacity=4 * sk.size + 16)
+            bk = np.flatnonzero(g_inst_births == k) + n0
+            assert o[0].size == sk.size + bk.size, (step, k)
+            assert mom[k, 6] == wc[0] == bk.size
+            n_births += bk.size
+            assert abs(mom[k, 0] - wm[0, 0]) <= rt_w * wm[0, 0]
+            assert abs(mom[k, 1] - wm[2, 0]) <= rt_w * wm[2, 0]
+            slots = np.concatenate([sk, bk])
+            vs = np.sqrt(sum(o[j].astype(np.float64) ** 2 for j in (1, 2, 3))) + 1
+            for j in (1, 2, 3):
+                assert np.all(np.abs(got[j][slots] - o[j]) <= rt_v * vs), (step, k, j)
+            np.testing.assert_array_equal(got[4][slots], o[4])
+            np.testing.assert_array_equal(got[0][slots], o[0])
+        assert got[0].size == n0 + n_births
+        st = got
     b.close()
 
 
@@ -68,18 +80,20 @@ def test_population_control_conserves(gpu):
     for _ in range(120):
         b.step()
     st = b.state()
-    off = b.offsets()
-    c0 = np.diff(off)
+    c0 = b.instance_counts()
     assert c0[0] > 1.5 * 400  # grew by ionization
     b.control()
     st2 = b.state()
     off2 = b.offsets()
     c1 = np.diff(off2)
+    # grouped by instance after the control
+    np.testing.assert_array_equal(st2[0], np.repeat(np.arange(3), c1).astype(st2[0].dtype))
     for k, tgt in enumerate((400, 300, 200)):
         # merged back to the target when at or above 1.5x, untouched otherwise
         assert c1[k] == (tgt if c0[k] >= 1.5 * tgt else c0[k]), (k, c0[k], c1[k])
     for k in range(3):
-        a = [x[off[k]:off[k + 1]] for x in st]
+        mine = st[0] == k
+        a = [x[mine] for x in st]
         z = [x[off2[k]:off2[k + 1]] for x in st2]
         W0, W1 = a[4].sum(), z[4].sum()
         assert abs(W1 - W0) <= 1e-12 * W0
