@@ -95,8 +95,40 @@ __device__ void sg_heavies(const bz_field_params_t& fp, int N, const double* E, 
   __syncthreads();
 }
 
-// mom is not __restrict__: a closure sub-step updates its ne through ne_rw
+// Exclusive scan of affine maps y -> A y + B over the block (thread order):
+// returns the value entering thread tid's chunk, i.e. the composition of the
+// maps of threads 0..tid-1 applied to 0 (0 for thread 0).  Waves scan with
+// shuffles; the wave totals are composed in wave order.
 template <int BLOCK>
+__device__ double affine_scan_in(double A, double B, double* s_wa, double* s_wb) {
+  constexpr int NW = BLOCK / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double pa = __shfl_up(A, off, 64), pb = __shfl_up(B, off, 64);
+    if (lane >= off) {
+      B = A * pb + B;
+      A = A * pa;
+    }
+  }
+  if (lane == 63) {
+    s_wa[wid] = A;
+    s_wb[wid] = B;
+  }
+  __syncthreads();
+  double y = 0.0;  // value entering this wave
+  for (int w = 0; w < wid && w < NW; ++w) y = s_wa[w] * y + s_wb[w];
+  const double ia = __shfl_up(A, 1, 64), ib = __shfl_up(B, 1, 64);
+  if (lane > 0) y = ia * y + ib;
+  __syncthreads();  // s_wa, s_wb are reused by the next scan
+  return y;
+}
+
+// mom is not __restrict__: a closure sub-step updates its ne through ne_rw.
+// STAGE: the grid is small enough for ni and ns to live in LDS through the
+// heavy-species sub-steps and the Poisson right-hand side (one global read
+// and one write instead of round trips on every sub-step).
+template <int BLOCK, bool STAGE>
 __global__ __launch_bounds__(BLOCK) void k_field(bz_field_params_t fp,
                                                        const double* mom,
                                                        double* ni, double* ns, double* Te,
@@ -106,7 +138,10 @@ __global__ __launch_bounds__(BLOCK) void k_field(bz_field_params_t fp,
                                                        double* Ji, double* Js, Clock* clk,
                                                        int advance, const double* src,
                                                        double* ne_rw) {
-  extern __shared__ double s_f[];  // rhs, tempy, 1/diag, lower [N-1 each], scan maps
+  // a, b, Gi, Gs [N each] of the heavy species, then rhs, tempy, 1/diag,
+  // lower [N-1 each] of the Poisson solve (STAGE: ni, ns [N+1 each] after)
+  extern __shared__ double s_f[];
+  __shared__ double s_wa[BLOCK / 64], s_wb[BLOCK / 64];
   const int N = fp.n;
   const int nn = N + 1;
   const int tid = threadIdx.x;
@@ -142,8 +177,27 @@ __global__ __launch_bounds__(BLOCK) void k_field(bz_field_params_t fp,
   if (advance) t = t + fp.dt;
   __syncthreads();
 
+  // ion density read by the Poisson right-hand side
+  const double* ni_p = ni;
   if (fp.fluid_on && fp.ion_scheme == 1) {
-    sg_heavies<BLOCK>(fp, N, E, ni, ns, s_f);
+    if (STAGE) {
+      // after the heavy-species scratch a, b, Gi, Gs [N each]
+      double* s_ni = s_f + 4 * N;
+      double* s_ns = s_ni + nn;
+      for (int i = tid; i < nn; i += BLOCK) {
+        s_ni[i] = ni[i];
+        s_ns[i] = ns[i];
+      }
+      __syncthreads();
+      sg_heavies<BLOCK>(fp, N, E, s_ni, s_ns, s_f);
+      for (int i = tid; i < nn; i += BLOCK) {
+        ni[i] = s_ni[i];
+        ns[i] = s_ns[i];
+      }
+      ni_p = s_ni;
+    } else {
+      sg_heavies<BLOCK>(fp, N, E, ni, ns, s_f);
+    }
   } else if (fp.fluid_on) {
     const double dx = fp.dx, dt = fp.dt, mu = fp.mu_i, Di = fp.D_i, Ds = fp.D_s;
     const double e0 = E[0] > 0.0 ? E[0] : 0.0;
@@ -181,10 +235,8 @@ __global__ __launch_bounds__(BLOCK) void k_field(bz_field_params_t fp,
     const int M = N - 1;
     double* inv = s_f + 2 * M;  // 1/diag
     double* lw = s_f + 3 * M;   // lower band
-    double* sa = s_f + 4 * M;   // per-thread scan maps [BLOCK]
-    double* sb = sa + BLOCK;
     for (int i = tid; i < M; i += BLOCK) {
-      rhs[i] = (1.0 / fp.eps) * (ni[i + 1] - ne[i + 1]);
+      rhs[i] = (1.0 / fp.eps) * (ni_p[i + 1] - ne[i + 1]);
       inv[i] = 1.0 / diag[i];
       lw[i] = low[i];
     }
@@ -214,24 +266,7 @@ __global__ __launch_bounds__(BLOCK) void k_field(bz_field_params_t fp,
         B = ai * B + bi;
         A = ai * A;
       }
-      sa[tid] = A;
-      sb[tid] = B;
-      __syncthreads();
-      for (int d = 1; d < BLOCK; d <<= 1) {
-        double pa = 1.0, pb = 0.0;
-        if (tid >= d) {
-          pa = sa[tid - d];
-          pb = sb[tid - d];
-        }
-        __syncthreads();
-        if (tid >= d) {
-          const double ca = sa[tid], cb = sb[tid];
-          sa[tid] = ca * pa;
-          sb[tid] = ca * pb + cb;
-        }
-        __syncthreads();
-      }
-      double y = tid > 0 ? sb[tid - 1] : 0.0;
+      double y = affine_scan_in<BLOCK>(A, B, s_wa, s_wb);
       for (int i = lo; i < hi; ++i) {
         y = (i == 0) ? rhs[0] / diag[0] : inv[i] * (rhs[i] - lw[i] * y);
         ty[i] = y;
@@ -254,24 +289,7 @@ __global__ __launch_bounds__(BLOCK) void k_field(bz_field_params_t fp,
         B = ai * B + bi;
         A = ai * A;
       }
-      sa[tid] = A;
-      sb[tid] = B;
-      __syncthreads();
-      for (int d = 1; d < BLOCK; d <<= 1) {
-        double pa = 1.0, pb = 0.0;
-        if (tid >= d) {
-          pa = sa[tid - d];
-          pb = sb[tid - d];
-        }
-        __syncthreads();
-        if (tid >= d) {
-          const double ca = sa[tid], cb = sb[tid];
-          sa[tid] = ca * pa;
-          sb[tid] = ca * pb + cb;
-        }
-        __syncthreads();
-      }
-      double v = tid > 0 ? sb[tid - 1] : 0.0;
+      double v = affine_scan_in<BLOCK>(A, B, s_wa, s_wb);
       for (int r = lo; r < hi; ++r) {
         const int i = M - 1 - r;
         v = (r == 0) ? ty[i] / diag[i] : inv[i] * (ty[i] - lw[i + 1] * v);
@@ -298,17 +316,20 @@ __global__ __launch_bounds__(BLOCK) void k_field(bz_field_params_t fp,
   }
 }
 
-template <int BLOCK>
+template <int BLOCK, bool STAGE>
 static int launch_field_t(const LaunchEnv& env, const bz_field_params_t* fp,
                           const double* moments, double* ni, double* ns, double* Te, double* g2,
                           double* V, double* E, void* E_t, const double* chol, int advance,
                           const double* src) {
   const int N = fp->n;
-  // rhs, y, 1/diag, lower band (N-1 each) + scan maps (2 per thread)
-  const size_t smem = ((size_t)4 * (N - 1) + 2 * BLOCK) * sizeof(double);
-  if (smem > 160 * 1024) return fail("field solve limited to N <= 4865 cells (LDS)");
+  // heavy-species scratch a, b, Gi, Gs (N each), reused by the Poisson
+  // solve for rhs, y, 1/diag, lower band (N-1 each) [+ staged ni, ns]
+  size_t smem = (size_t)4 * N * sizeof(double);
+  if (STAGE) smem += (size_t)2 * (N + 1) * sizeof(double);
+  // 1 KiB of the 160 KiB stays for the kernel's own LDS variables
+  if (smem > 159 * 1024) return fail("field solve limited to N <= 5088 cells (LDS)");
   if (smem > 64 * 1024) {
-    if (hipFuncSetAttribute((const void*)k_field<BLOCK>,
+    if (hipFuncSetAttribute((const void*)k_field<BLOCK, STAGE>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) != hipSuccess)
       return fail("hipFuncSetAttribute(field LDS) failed");
   }
@@ -316,20 +337,23 @@ static int launch_field_t(const LaunchEnv& env, const bz_field_params_t* fp,
   if (env.partials_doubles < 2ll * (N + 1)) return fail("workspace too small for fluxes");
   double* Ji = env.partials;
   double* Js = env.partials + (N + 1);
-  hipLaunchKernelGGL(k_field<BLOCK>, dim3(1), dim3(BLOCK), smem, env.stream, *fp, moments, ni,
-                     ns, Te, g2, V, E, E_t, env.precision, chol, Ji, Js, env.clock, advance, src,
-                     const_cast<double*>(moments));
+  hipLaunchKernelGGL((k_field<BLOCK, STAGE>), dim3(1), dim3(BLOCK), smem, env.stream, *fp,
+                     moments, ni, ns, Te, g2, V, E, E_t, env.precision, chol, Ji, Js, env.clock,
+                     advance, src, const_cast<double*>(moments));
   return check_launch("k_field");
 }
 
 // One block: 256 threads while the grid is small (fewer threads in every
-// barrier and a shorter scan), 1024 for the large grids.
+// barrier and a shorter scan; ni and ns staged in LDS), 1024 for the large
+// grids.
 int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double* moments,
                  double* ni, double* ns, double* Te, double* g2, double* V, double* E,
                  void* E_t, const double* chol, int advance, const double* src) {
   if (fp->n <= 1024)
-    return launch_field_t<256>(env, fp, moments, ni, ns, Te, g2, V, E, E_t, chol, advance, src);
-  return launch_field_t<1024>(env, fp, moments, ni, ns, Te, g2, V, E, E_t, chol, advance, src);
+    return launch_field_t<256, true>(env, fp, moments, ni, ns, Te, g2, V, E, E_t, chol, advance,
+                                     src);
+  return launch_field_t<1024, false>(env, fp, moments, ni, ns, Te, g2, V, E, E_t, chol, advance,
+                                     src);
 }
 
 }  // namespace bz

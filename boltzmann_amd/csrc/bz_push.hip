@@ -328,19 +328,22 @@ __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks
   const int j = cx * 64 + lane;
   const int r0 = cy * rpc;
   const int r1 = r0 + rpc < nblocks ? r0 + rpc : nblocks;
-  // four independent partial sums per thread keep four row loads in flight
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  // up to 16 rows per thread with every load issued before the first add,
+  // so a 256-row chunk costs one memory round trip
+  double sum = 0.0;
   if (j < m) {
-    int b = r0 + wid;
-    for (; b + 3 * kRedWaves < r1; b += 4 * kRedWaves) {
-      s0 += partials[(size_t)b * m + j];
-      s1 += partials[(size_t)(b + kRedWaves) * m + j];
-      s2 += partials[(size_t)(b + 2 * kRedWaves) * m + j];
-      s3 += partials[(size_t)(b + 3 * kRedWaves) * m + j];
+    for (int b0 = r0 + wid; b0 < r1; b0 += 16 * kRedWaves) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int b = b0 + u * kRedWaves;
+        v[u] = b < r1 ? partials[(size_t)b * m + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) sum += v[u];
     }
-    for (; b < r1; b += kRedWaves) s0 += partials[(size_t)b * m + j];
   }
-  s_part[wid][lane] = (s0 + s1) + (s2 + s3);
+  s_part[wid][lane] = sum;
   __syncthreads();
   if (wid == 0) {
     double t = 0.0;
@@ -386,22 +389,24 @@ __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks
   }
 }
 
-// Counters of the push's per-block rows into the clock and the slot count
-// commit (one wave).
+// Counters of the push's per-block rows into the clock (wave q sums counter
+// q, all at once) and the slot count commit.  Runs in a block of its own.
 __device__ void dep_commit(const double* __restrict__ partials, int nblocks, int m, Clock* clk,
                            int64_t cap) {
   const int lane = threadIdx.x & 63;
+  const int q = threadIdx.x >> 6;
   const uint32_t* bcnt = (const uint32_t*)(partials + (size_t)nblocks * m);
   const int slot[9] = {BZ_CNT_BIRTHS, BZ_CNT_ABSORBED, BZ_CNT_ELASTIC, BZ_CNT_EXCITATION,
                        BZ_CNT_IONIZATION, BZ_CNT_NULL, BZ_CNT_OVERFLOW, BZ_CNT_NUMAX,
                        BZ_CNT_PSTEPS};
-  for (int q = 0; q < 9; ++q) {
+  static_assert(kRedWaves >= 9, "one wave per counter");
+  if (q < 9) {
     unsigned long long s = 0;
     for (int b = lane; b < nblocks; b += 64) s += bcnt[(size_t)b * kCntRow + q];
     s = wave_sum(s);
     if (lane == 0 && s) clk->cnt[slot[q]] += s;
   }
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     // outputs past the capacity were dropped (and counted) by the push
     const int64_t nx = __hip_atomic_load(&clk->n_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     clk->n = nx < cap ? nx : cap;
@@ -416,10 +421,13 @@ __global__ __launch_bounds__(kRedThreads) void k_dep_reduce(
     int64_t cap) {
   __shared__ double s_part[kRedWaves][64];
   __shared__ int s_last;
+  if (commit && blockIdx.x == gridDim.x - 1) {
+    // the extra column of blocks: counters and the slot count
+    if (blockIdx.y == 0) dep_commit(partials, nblocks, m, clk, cap);
+    return;
+  }
   dep_reduce_part(partials, nblocks, m, rpc, l2, col_done, out, acc, blockIdx.x, blockIdx.y,
                   gridDim.y, s_part, &s_last);
-  if (commit && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 64)
-    dep_commit(partials, nblocks, m, clk, cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -888,7 +896,8 @@ int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int
   if ((int64_t)ny * m > env.dep_l2_doubles && ny > 1) return fail("deposit reduction: level-2 buffer");
   const int nx = (m + 63) / 64;
   if (nx > env.dep_cols) return fail("deposit reduction: %d output columns > %d", nx, env.dep_cols);
-  hipLaunchKernelGGL(k_dep_reduce, dim3(nx, ny > 0 ? ny : 1), dim3(kRedWaves * 64), 0, env.stream,
+  hipLaunchKernelGGL(k_dep_reduce, dim3(nx + (commit ? 1 : 0), ny > 0 ? ny : 1),
+                     dim3(kRedWaves * 64), 0, env.stream,
                      (const double*)env.partials, nblocks, m, (int)rpc, env.dep_l2, env.dep_cols_done,
                      out, acc, env.clock, commit, cap);
   return check_launch("k_dep_reduce");
