@@ -149,17 +149,35 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(KeyArgs a, const Clock* clk
 // between keys kp = skeys[s-1] (or -1) and kc = skeys[s] (or m past the
 // end), so s is the first position of every key in (kp, kc]; keys that do
 // not occur get the start of the next one.  offsets[k] = lower bound of k,
-// k = 0..m, with offsets[m] = n.
+// k = 0..m, with offsets[m] = n.  A wave covers 256 consecutive positions
+// (4 per lane, loads issued together); the left neighbour comes from the
+// next lane down.
+constexpr int kBoundsTile = 1024;  // positions per 256-thread block
+
 __global__ __launch_bounds__(256) void k_key_bounds(const uint32_t* __restrict__ skeys,
                                                     const Clock* clk, int m,
                                                     int32_t* offsets) {
   const int64_t n = clk->n;
-  for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s <= n;
-       s += (int64_t)gridDim.x * 256) {
-    const int64_t kp = s > 0 ? (int64_t)skeys[s - 1] : -1;
-    int64_t kc = s < n ? (int64_t)skeys[s] : (int64_t)m;
-    kc = kc < m ? kc : m;
-    for (int64_t k = kp + 1; k <= kc; ++k) offsets[k] = (int32_t)s;
+  const int lane = threadIdx.x & 63;
+  const int64_t base = (int64_t)blockIdx.x * kBoundsTile + (threadIdx.x - lane) * 4;
+  if (base > n) return;  // whole wave past the end (wave-uniform)
+  int kc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t s = base + k * 64 + lane;
+    kc[k] = s < n ? (int)skeys[s] : m;
+  }
+  const int before = (lane == 0 && base > 0) ? (int)skeys[base - 1] : -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int up = __shfl_up(kc[k], 1, 64);
+    const int last = __shfl(k > 0 ? kc[k > 0 ? k - 1 : 0] : 0, 63, 64);
+    const int kp = lane > 0 ? up : (k > 0 ? last : before);
+    const int64_t s = base + k * 64 + lane;
+    if (s <= n) {
+      const int hi = kc[k] < m ? kc[k] : m;
+      for (int q = kp + 1; q <= hi; ++q) offsets[q] = (int32_t)s;
+    }
   }
 }
 
@@ -185,11 +203,41 @@ __global__ __launch_bounds__(kSortBlock) void k_upsweep(const uint32_t* keys, co
   s_h[threadIdx.x] = 0;
   __syncthreads();
   const int64_t n = clk->n;
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
-  for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = base + r * kSortBlock + threadIdx.x;
-    if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 255u], 1u);
+  // thread t counts the kSortItems consecutive keys from base + t*kSortItems
+  // (the tile's histogram does not depend on the split) and adds one run of
+  // equal digits at a time: the keys arrive nearly sorted, so this cuts the
+  // LDS atomics on a few hot words by the run length
+  const int64_t i0 = (int64_t)blockIdx.x * kSortTile + (int64_t)threadIdx.x * kSortItems;
+  uint32_t kv[kSortItems];
+  if (i0 + kSortItems <= n) {
+    const uint4* k4 = (const uint4*)(keys + i0);
+#pragma unroll
+    for (int q = 0; q < kSortItems / 4; ++q) {
+      const uint4 v = k4[q];
+      kv[4 * q] = v.x;
+      kv[4 * q + 1] = v.y;
+      kv[4 * q + 2] = v.z;
+      kv[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) kv[r] = i0 + r < n ? keys[i0 + r] : 0u;
   }
+  uint32_t cur = 0u, run = 0u;
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (i0 + r < n) {
+      const uint32_t d = (kv[r] >> shift) & 255u;
+      if (run && d == cur) {
+        ++run;
+      } else {
+        if (run) atomicAdd(&s_h[cur], run);
+        cur = d;
+        run = 1u;
+      }
+    }
+  }
+  if (run) atomicAdd(&s_h[cur], run);
   __syncthreads();
   hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
 }
@@ -366,7 +414,9 @@ static int sort_keys(const LaunchEnv& env, const SortWs& ws, int n_keys, int32_t
   }
   // offsets of every key from the sorted keys, then the counts
   const int mk = n_keys + 1;  // key values 0..n_keys (n_keys: dead slots)
-  hipLaunchKernelGGL(k_key_bounds, dim3(env.num_cu * 4), dim3(256), 0, env.stream,
+  // positions 0..n (n itself closes the last keys); waves past n exit
+  const int64_t bgrid = (env.capacity + kBoundsTile) / kBoundsTile;
+  hipLaunchKernelGGL(k_key_bounds, dim3((unsigned)bgrid), dim3(256), 0, env.stream,
                      (const uint32_t*)kin, (const Clock*)env.clock, mk, offsets);
   hipLaunchKernelGGL(k_key_counts, dim3((mk + 255) / 256), dim3(256), 0, env.stream,
                      (const int32_t*)offsets, mk, counts);
