@@ -298,7 +298,8 @@ def bench_swarms(args):
                    "en_td": [scfg.en_td_lo, scfg.en_td_hi], "dt_s": scfg.dt,
                    "graph": f"{per_graph} steps + control every {scfg.control_every}",
                    "parallelism": "single GPU"},
-        "roofline": _roofline(bytes_per_pstep, 0.5 * (n_before + n_mid), avg_push_s, "k_push_inplace"),
+        "roofline": _roofline(bytes_per_pstep, 0.5 * (n_before + n_mid), avg_push_s, "k_push_inplace",
+                              scfg.name),
         "counters": {k: c1[k] - c0[k] for k in ("births", "ionization", "split", "merge")},
         "sample_results": {"en_td": [float(res["en_td"][i]) for i in (0, 128, 255)],
                            "mean_energy_ev": [float(res["mean_energy"][i]) for i in (0, 128, 255)]},
