@@ -105,6 +105,40 @@ def test_population_control_conserves(gpu):
     b.close()
 
 
+def test_population_control_many_instances(gpu):
+    """300 instances: the instance keys need both 8-bit radix passes.  After
+    the control every instance is one contiguous range in instance order,
+    grown instances are merged to their target and the rest are copied, with
+    weight and energy conserved per instance."""
+    n_inst = 300
+    rng = np.random.default_rng(5)
+    counts = tuple(int(c) for c in rng.integers(20, 60, n_inst))
+    en = tuple(float(e) for e in np.where(np.arange(n_inst) % 2 == 0, 1000.0, 5.0))
+    b = _batch(8, en=en, counts=counts, te=30.0, dt=2e-11, capacity_factor=40.0)
+    for _ in range(100):
+        b.step()
+    st = b.state()
+    c0 = b.instance_counts()
+    assert c0[0::2].sum() > 1.5 * sum(counts[0::2])  # the strong-field half grew
+    b.control()
+    st2 = b.state()
+    off2 = b.offsets()
+    c1 = np.diff(off2)
+    np.testing.assert_array_equal(st2[0], np.repeat(np.arange(n_inst), c1).astype(st2[0].dtype))
+    for k in range(n_inst):
+        tgt = counts[k]
+        assert c1[k] == (tgt if c0[k] >= 1.5 * tgt else c0[k]), (k, c0[k], c1[k])
+        mine = st[0] == k
+        W0 = st[4][mine].sum()
+        W1 = st2[4][off2[k]:off2[k + 1]].sum()
+        assert abs(W1 - W0) <= 1e-12 * W0
+        E0 = np.sum(st[4][mine] * (st[1][mine] ** 2 + st[2][mine] ** 2 + st[3][mine] ** 2))
+        z = slice(off2[k], off2[k + 1])
+        E1 = np.sum(st2[4][z] * (st2[1][z] ** 2 + st2[2][z] ** 2 + st2[3][z] ** 2))
+        assert abs(E1 - E0) <= 1e-12 * E0
+    b.close()
+
+
 def test_graph_replay_matches_eager(gpu):
     b1 = _batch(4)
     b2 = _batch(4)
