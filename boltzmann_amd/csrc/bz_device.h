@@ -217,6 +217,7 @@ __device__ __forceinline__ uint32_t lookback_block(uint64_t* status, uint32_t t,
 struct Clock {
   int64_t n;         // live particles (current state buffer)
   int64_t n_next;    // written by the producing kernel, committed by the next
+  int64_t n_prev;    // slot count an in-place push started from (its births go after)
   uint64_t step;     // keys the random stream
   double t;          // simulated time, s
   uint32_t epoch;    // look-back epoch, bumped per scan launch

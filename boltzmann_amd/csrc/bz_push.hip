@@ -2,26 +2,29 @@
 // pass over the SoA state, with births and the removal of absorbed particles
 // handled in one of two ways.
 //
-// 1-D discharges (in place, the reference's electron_kernel layout):
+// 1-D discharges and batched 0-D swarms (in place; for the discharges the
+// reference's electron_kernel layout):
 //   each block owns a contiguous run of tiles of BLOCK*ITEMS particles
 //   HBM read  x, vx, vy, vz, w            (5 words / particle)
 //   registers collision, push, walls, secondary
-//   LDS       per-block FP64 node histograms (ne, ni_src, nrg)
-//   HBM write x, vx, vy, vz back to the same slot; w = 0 for an absorbed
-//             particle (w of a survivor is unchanged and not rewritten)
+//   LDS       per-block FP64 node histograms (ne, ni_src, nrg), or the
+//             per-instance moments of the swarms (x holds the instance)
+//   HBM write x, vx, vy, vz back to the same slot (swarms: vx, vy, vz);
+//             w = 0 for an absorbed particle (w of a survivor is unchanged
+//             and not rewritten)
 //   births    compacted per block into a staging buffer, then appended
-//             after the live range by k_births_append in parent order
+//             after the live range by k_push_post in parent order, in the
+//             same launch that sums the partial rows and the counters
 //   No block waits on another, so any grid size and residency works.
 //   Absorbed slots stay as holes (w = 0) until a compaction or a resampling
 //   pass rewrites the state.
-// Batched and single 0-D swarms (compacting):
+// The single 0-D swarm (compacting):
 //   scan      per-particle output count (alive + born) -> block scan ->
 //             decoupled look-back across tiles -> output slot
-//   HBM write survivors, each secondary right after its parent, so every
-//             swarm instance stays contiguous
-// Node histograms are flushed once per block to a partial buffer and summed
-// over blocks in a fixed order by k_dep_reduce, so the moments do not depend
-// on scheduling across blocks.
+//   HBM write survivors, each secondary right after its parent
+// Histograms are flushed once per block to a partial buffer and summed over
+// blocks in a fixed order (k_push_post / k_dep_reduce), so the moments do not
+// depend on scheduling across blocks.
 #include "bz_physics.h"
 
 namespace bz {
@@ -392,7 +395,7 @@ __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks
 // Counters of the push's per-block rows into the clock (wave q sums counter
 // q, all at once) and the slot count commit.  Runs in a block of its own.
 __device__ void dep_commit(const double* __restrict__ partials, int nblocks, int m, Clock* clk,
-                           int64_t cap) {
+                           int64_t cap, bool set_n) {
   const int lane = threadIdx.x & 63;
   const int q = threadIdx.x >> 6;
   const uint32_t* bcnt = (const uint32_t*)(partials + (size_t)nblocks * m);
@@ -407,9 +410,12 @@ __device__ void dep_commit(const double* __restrict__ partials, int nblocks, int
     if (lane == 0 && s) clk->cnt[slot[q]] += s;
   }
   if (threadIdx.x == 0) {
-    // outputs past the capacity were dropped (and counted) by the push
-    const int64_t nx = __hip_atomic_load(&clk->n_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    clk->n = nx < cap ? nx : cap;
+    if (set_n) {
+      // outputs past the capacity were dropped (and counted) by the push
+      const int64_t nx = __hip_atomic_load(&clk->n_next, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      clk->n = nx < cap ? nx : cap;
+    }
     clk->epoch = clk->epoch + 1;
     clk->ticket = 0;
   }
@@ -423,7 +429,7 @@ __global__ __launch_bounds__(kRedThreads) void k_dep_reduce(
   __shared__ int s_last;
   if (commit && blockIdx.x == gridDim.x - 1) {
     // the extra column of blocks: counters and the slot count
-    if (blockIdx.y == 0) dep_commit(partials, nblocks, m, clk, cap);
+    if (blockIdx.y == 0) dep_commit(partials, nblocks, m, clk, cap, true);
     return;
   }
   dep_reduce_part(partials, nblocks, m, rpc, l2, col_done, out, acc, blockIdx.x, blockIdx.y,
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(kRedThreads) void k_dep_reduce(
 // before this tile's collision physics, survivors are written back to their
 // slots, and births are staged in the wave's own slot range of the staging
 // buffer in slot order, with the count per (tile, wave) stored for
-// k_births_append.  The sums live in LDS (FP64 atomics) and are flushed once
+// k_push_post.  The sums live in LDS (FP64 atomics) and are flushed once
 // per block after the loop: node histograms [3][n_nodes] (DIM 1) or
 // per-instance moments [n_inst][kImN] (DIM 2).
 //
@@ -497,6 +503,9 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
 
   const int64_t n = a.clk->n;
   const uint64_t step = a.clk->step;
+  // the births stage reads the count from n_prev: the slot count itself is
+  // committed while it runs
+  if (blockIdx.x == 0 && tid == 0) a.clk->n_prev = n;
   const uint32_t ntiles = (uint32_t)((n + TILE - 1) / TILE);
   const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
   const uint32_t t_first = (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)ntiles);
@@ -676,7 +685,7 @@ __device__ void births_part(const PushArgs& a, int b, int nblk, int tile, int nw
                             uint32_t* s_red) {
   constexpr int NWV = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int64_t n = a.clk->n;
+  const int64_t n = a.clk->n_prev;
   const uint32_t ntiles = (uint32_t)((n + tile - 1) / tile);
   const uint32_t per = (ntiles + nblk - 1) / nblk;
   const uint32_t t_first = (uint32_t)min((uint64_t)b * per, (uint64_t)ntiles);
@@ -730,16 +739,36 @@ __device__ void births_part(const PushArgs& a, int b, int nblk, int tile, int nw
   }
   if (b == nblk - 1 && tid == 0) {
     const int64_t want = n + pre + mine;
-    a.clk->n_next = want < a.capacity ? want : a.capacity;
+    a.clk->n = want < a.capacity ? want : a.capacity;
     if (want > a.capacity)
       atomicAdd(&a.clk->cnt[BZ_CNT_OVERFLOW], (unsigned long long)(want - a.capacity));
   }
 }
 
+// After the in-place push, one launch: blocks [0, nbirth) append the births
+// of push block b (births_part), the next nx*ny blocks sum the partial rows
+// (dep_reduce_part), and the last block adds the counters to the clock.
+// The slot count is committed by the births of the last push block.
 template <typename T>
-__global__ __launch_bounds__(256) void k_births_append(PushArgs a, int tile, int nw) {
-  __shared__ uint32_t s_red[4];
-  births_part<T, 256>(a, blockIdx.x, gridDim.x, tile, nw, s_red);
+__global__ __launch_bounds__(kRedThreads) void k_push_post(PushArgs a, int tile, int nw,
+                                                           int nbirth, int m, int rpc, int nx,
+                                                           int ny, double* l2,
+                                                           uint32_t* col_done,
+                                                           double* __restrict__ out,
+                                                           double* acc) {
+  __shared__ double s_part[kRedWaves][64];
+  __shared__ int s_last;
+  __shared__ uint32_t s_red[kRedWaves];
+  const int b = blockIdx.x;
+  if (b < nbirth) {
+    births_part<T, kRedThreads>(a, b, nbirth, tile, nw, s_red);
+  } else if (b < nbirth + nx * ny) {
+    const int r = b - nbirth;
+    dep_reduce_part(a.partials, nbirth, m, rpc, l2, col_done, out, acc, r % nx, r / nx, ny,
+                    s_part, &s_last);
+  } else {
+    dep_commit(a.partials, nbirth, m, a.clk, a.capacity, false);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -884,9 +913,10 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(bz_params_t p, const T* __res
 namespace bz {
 
 // Launch of k_dep_reduce: rows per chunk so that the level-2 rows fit.
-int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int commit,
-                      int64_t cap, double* acc) {
-  if (m < 1) return 0;
+// Geometry of the two-level reduction of nblocks rows of m sums: 64 outputs
+// per block column (nx), row chunks of rpc rows (ny).
+static int red_geometry(const LaunchEnv& env, int nblocks, int m, int* rpc_out, int* nx_out,
+                        int* ny_out) {
   // one level up to 256 rows (at most 16 rows per thread), two above
   int64_t rpc = 256;
   const int64_t need = (int64_t)nblocks * m;
@@ -896,9 +926,21 @@ int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int
   if ((int64_t)ny * m > env.dep_l2_doubles && ny > 1) return fail("deposit reduction: level-2 buffer");
   const int nx = (m + 63) / 64;
   if (nx > env.dep_cols) return fail("deposit reduction: %d output columns > %d", nx, env.dep_cols);
-  hipLaunchKernelGGL(k_dep_reduce, dim3(nx + (commit ? 1 : 0), ny > 0 ? ny : 1),
+  *rpc_out = (int)rpc;
+  *nx_out = nx;
+  *ny_out = ny > 0 ? ny : 1;
+  return 0;
+}
+
+int launch_dep_reduce(const LaunchEnv& env, int nblocks, int m, double* out, int commit,
+                      int64_t cap, double* acc) {
+  if (m < 1) return 0;
+  int rpc = 0, nx = 0, ny = 0;
+  const int rc = red_geometry(env, nblocks, m, &rpc, &nx, &ny);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dep_reduce, dim3(nx + (commit ? 1 : 0), ny),
                      dim3(kRedWaves * 64), 0, env.stream,
-                     (const double*)env.partials, nblocks, m, (int)rpc, env.dep_l2, env.dep_cols_done,
+                     (const double*)env.partials, nblocks, m, rpc, env.dep_l2, env.dep_cols_done,
                      out, acc, env.clock, commit, cap);
   return check_launch("k_dep_reduce");
 }
@@ -925,9 +967,17 @@ int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys) {
   return d;
 }
 
-// 1-D discharges and batched swarms: k_push_inplace, then k_births_append
+// What the push launch leaves for the stage after it.
+struct PushLaunch {
+  int grid;     // push blocks (partial rows)
+  int tile;     // in-place push: slots per tile
+  int nw;       // in-place push: waves per block
+  bool inplace;
+};
+
+// 1-D discharges and batched swarms: k_push_inplace (k_push_post follows)
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
-static int launch_push_inplace_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
+static int launch_push_inplace_t(const LaunchEnv& env, PushArgs& a, PushLaunch* pl) {
   auto kern = k_push_inplace<T, XS, DIM, RNG, BLOCK, ITEMS>;
   const int nn = DIM == 1 ? a.p.n_nodes : 0;
   const int part_doubles = DIM == 1 ? 3 * nn : a.n_inst * kImN;
@@ -958,22 +1008,19 @@ static int launch_push_inplace_t(const LaunchEnv& env, PushArgs& a, int* grid_ou
   timing_begin(env, 0);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
   timing_end(env, 0);
-  int rc = check_launch("k_push_inplace");
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_births_append<T>, dim3(grid), dim3(256), 0, env.stream, a,
-                     BLOCK * ITEMS, BLOCK / kWave);
-  rc = check_launch("k_births_append");
-  if (rc) return rc;
-  *grid_out = grid;
-  return 0;
+  pl->grid = grid;
+  pl->tile = BLOCK * ITEMS;
+  pl->nw = BLOCK / kWave;
+  pl->inplace = true;
+  return check_launch("k_push_inplace");
 }
 
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
-static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
+static int launch_push_t(const LaunchEnv& env, PushArgs& a, PushLaunch* pl) {
   // the 1-D discharges and the batched swarms update in place; a single
   // 0-D swarm compacts
   if constexpr (DIM != 0) {
-    return launch_push_inplace_t<T, XS, DIM, RNG, BLOCK, ITEMS>(env, a, grid_out);
+    return launch_push_inplace_t<T, XS, DIM, RNG, BLOCK, ITEMS>(env, a, pl);
   } else {
     auto kern = k_push<T, XS, DIM, RNG, BLOCK, ITEMS>;
     // the kernel's three sums, then the table
@@ -1008,13 +1055,16 @@ static int launch_push_t(const LaunchEnv& env, PushArgs& a, int* grid_out) {
       hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, a);
     timing_end(env, 0);
     if (e != hipSuccess) return fail("k_push cooperative launch: %s", hipGetErrorString(e));
-    *grid_out = grid;
+    pl->grid = grid;
+    pl->tile = 0;
+    pl->nw = 0;
+    pl->inplace = false;
     return check_launch("k_push");
   }
 }
 
 template <typename T, int XS, int DIM, int RNG>
-static int launch_push_shape(const LaunchEnv& env, PushArgs& a, int shape, int* grid) {
+static int launch_push_shape(const LaunchEnv& env, PushArgs& a, int shape, PushLaunch* grid) {
   switch (shape) {
     case 0: return launch_push_t<T, XS, DIM, RNG, 256, 4>(env, a, grid);
     case 1: return launch_push_t<T, XS, DIM, RNG, 512, 2>(env, a, grid);
@@ -1026,7 +1076,7 @@ static int launch_push_shape(const LaunchEnv& env, PushArgs& a, int shape, int* 
 }
 
 template <typename T>
-static int launch_push_prec(const LaunchEnv& env, PushArgs& a, int shape, int* grid) {
+static int launch_push_prec(const LaunchEnv& env, PushArgs& a, int shape, PushLaunch* grid) {
   const bz_params_t& p = a.p;
   const bool stream = p.rng_mode == BZ_RNG_STREAM;
   if (a.n_inst > 0) {
@@ -1060,6 +1110,26 @@ static int launch_push_prec(const LaunchEnv& env, PushArgs& a, int shape, int* g
                 : launch_push_shape<T, BZ_XS_FIT, 0, BZ_RNG_PHILOX>(env, a, 0, grid);
 }
 
+// The stage after the push: births + reduction + counters in one launch
+// (in place), or the reduction with the slot count commit (compacting push).
+static int launch_push_post(const LaunchEnv& env, const PushArgs& a, const PushLaunch& pl, int m,
+                            double* out, double* acc) {
+  if (!pl.inplace) return launch_dep_reduce(env, pl.grid, m, out, 1, (int64_t)env.capacity, acc);
+  int rpc = 0, nx = 0, ny = 0;
+  const int rc = red_geometry(env, pl.grid, m, &rpc, &nx, &ny);
+  if (rc) return rc;
+  const unsigned blocks = (unsigned)(pl.grid + nx * ny + 1);
+  if (env.precision == BZ_F64)
+    hipLaunchKernelGGL(k_push_post<double>, dim3(blocks), dim3(kRedThreads), 0, env.stream, a,
+                       pl.tile, pl.nw, pl.grid, m, rpc, nx, ny, env.dep_l2, env.dep_cols_done,
+                       out, acc);
+  else
+    hipLaunchKernelGGL(k_push_post<float>, dim3(blocks), dim3(kRedThreads), 0, env.stream, a,
+                       pl.tile, pl.nw, pl.grid, m, rpc, nx, ny, env.dep_l2, env.dep_cols_done,
+                       out, acc);
+  return check_launch("k_push_post");
+}
+
 int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,
                 const void* E, const bz_params_t* p, const bz_xsec_t* xs, const void* R,
                 const void* G, double* moments, int shape) {
@@ -1083,12 +1153,11 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
   a.inst_off = nullptr;
   a.inst_E = nullptr;
   a.n_inst = 0;
-  int grid = 0;
-  const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &grid)
-                                         : launch_push_prec<float>(env, a, shape, &grid);
+  PushLaunch pl;
+  const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &pl)
+                                         : launch_push_prec<float>(env, a, shape, &pl);
   if (rc) return rc;
-  const int m = 3 * (p->dim == 1 ? p->n_nodes : 1);
-  return launch_dep_reduce(env, grid, m, moments, 1, (int64_t)env.capacity);
+  return launch_push_post(env, a, pl, 3 * (p->dim == 1 ? p->n_nodes : 1), moments, nullptr);
 }
 
 template <typename T, int DIM>
@@ -1276,12 +1345,11 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
   a.inst_off = nullptr;
   a.inst_E = inst_E;
   a.n_inst = n_inst;
-  int grid = 0;
-  const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &grid)
-                                         : launch_push_prec<float>(env, a, shape, &grid);
+  PushLaunch pl;
+  const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &pl)
+                                         : launch_push_prec<float>(env, a, shape, &pl);
   if (rc) return rc;
-  return launch_dep_reduce(env, grid, n_inst * kImN, inst_moments, 1, (int64_t)env.capacity,
-                           acc);
+  return launch_push_post(env, a, pl, n_inst * kImN, inst_moments, acc);
 }
 
 }  // namespace bz
