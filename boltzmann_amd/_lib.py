@@ -106,6 +106,8 @@ SIGNATURES = {
     "bz_fill_uniforms": (C.c_int, [_P, _P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64]),
     "bz_field_step": (C.c_int, [_P, C.POINTER(FieldParams), _P, _P, _P, _P, _P, _P, _P, _P,
                                 _P, C.c_int32]),
+    "bz_bin_count": (C.c_int, [_P, C.POINTER(State), C.POINTER(Params), C.POINTER(C.c_double),
+                               C.c_int32, _P]),
     "bz_bin_sort": (C.c_int, [_P, C.POINTER(State), C.POINTER(Params), C.POINTER(C.c_double),
                               C.c_int32, _P, _P, _P]),
     "bz_resample": (C.c_int, [_P, C.POINTER(State), C.POINTER(State), C.POINTER(Params),

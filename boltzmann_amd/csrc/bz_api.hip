@@ -399,6 +399,19 @@ int bz_closure_commit(bz_ctx_t c, int32_t n_nodes, const double* moments, double
   return launch_closure_commit(c->env, n_nodes, moments, acc, ni, g2, Te, judge);
 }
 
+int bz_bin_count(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
+                 const double* thresholds_host, int32_t nb, int32_t* counts) {
+  BZ_CTX(c);
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (nb < 1 || nb > 8) return fail("nb must be in [1,8]");
+  if (nb > 1 && !thresholds_host) return fail("nb > 1 needs thresholds");
+  const int64_t n_keys = (int64_t)(p->dim == 1 ? p->n_nodes : 1) * nb;
+  if (n_keys > c->env.max_keys) return fail("n_keys %lld exceeds context max_keys", (long long)n_keys);
+  if (!s || !counts) return fail("null bin/count pointer");
+  return launch_bin_count(c->env, s, p, thresholds_host, nb, counts);
+}
+
 int bz_bin_sort(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
                 const double* thresholds_host, int32_t nb, int32_t* counts, int32_t* offsets,
                 uint32_t* perm) {
@@ -409,7 +422,7 @@ int bz_bin_sort(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
   if (nb > 1 && !thresholds_host) return fail("nb > 1 needs thresholds");
   const int64_t n_keys = (int64_t)(p->dim == 1 ? p->n_nodes : 1) * nb;
   if (n_keys > c->env.max_keys) return fail("n_keys %lld exceeds context max_keys", (long long)n_keys);
-  if (n_keys > 39999) return fail("n_keys must be <= 39999 (LDS key histogram)");
+  if (n_keys > 65534) return fail("n_keys must be <= 65534 (two 8-bit radix passes)");
   if (!s || !counts || !offsets || !perm) return fail("null bin/sort pointer");
   return launch_bin_sort(c->env, s, p, thresholds_host, nb, counts, offsets, perm);
 }

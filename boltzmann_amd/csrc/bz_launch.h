@@ -74,6 +74,9 @@ int launch_closure_source(const LaunchEnv& env, const bz_mlp_t* net, int N, cons
                           const double* mom, double gas_m3, double dt, double* src, double* acc);
 int launch_closure_commit(const LaunchEnv& env, int nn, const double* mom, double* acc, double* ni,
                           double* g2, double* Te, double* judge);
+// per-key counts of the binning keys, no sort (resample trigger input)
+int launch_bin_count(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                     const double* thr, int nb, int32_t* counts);
 int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                     const double* thr, int nb, int32_t* counts, int32_t* offsets,
                     uint32_t* perm);

@@ -100,6 +100,21 @@ struct KeyArgs {
 // histogram.
 constexpr int kKeyItems = 4;
 
+// key = cell * nb + energy bin of one slot; n_keys for a dead slot (w <= 0)
+template <typename T>
+__device__ __forceinline__ uint32_t slot_key(const KeyArgs& a, T x, T vx, T vy, T vz, T w) {
+  if (!(w > (T)0)) return (uint32_t)a.n_keys;
+  int cell = 0;
+  if (a.dim == 1) {
+    cell = (int)(((double)x + 0.5 * a.dx) / a.dx);
+    cell = cell < 0 ? 0 : (cell >= a.n_nodes ? a.n_nodes - 1 : cell);
+  }
+  const T e = (T)a.kev * (vx * vx + vy * vy + vz * vz);
+  int b = 0;
+  for (int q = 0; q < a.nb - 1; ++q) b += (e >= (T)a.thr[q]) ? 1 : 0;
+  return (uint32_t)(cell * a.nb + b);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kSortBlock) void k_keys(KeyArgs a, const Clock* clk, uint32_t* keys) {
   const int64_t n = clk->n;
@@ -125,24 +140,51 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(KeyArgs a, const Clock* clk
 #pragma unroll
   for (int k = 0; k < kKeyItems; ++k) {
     const int64_t i = base + k * kWave + lane;
-    uint32_t key = 0xFFFFFFFFu;  // past the end: not counted
-    if (i < n) {
-      if (!(qw[k] > (T)0)) {
-        key = (uint32_t)a.n_keys;
-      } else {
-        int cell = 0;
-        if (a.dim == 1) {
-          cell = (int)(((double)qx[k] + 0.5 * a.dx) / a.dx);
-          cell = cell < 0 ? 0 : (cell >= a.n_nodes ? a.n_nodes - 1 : cell);
-        }
-        const T e = (T)a.kev * (qa[k] * qa[k] + qb[k] * qb[k] + qc[k] * qc[k]);
-        int b = 0;
-        for (int q = 0; q < a.nb - 1; ++q) b += (e >= (T)a.thr[q]) ? 1 : 0;
-        key = (uint32_t)(cell * a.nb + b);
-      }
-      keys[i] = key;
+    if (i < n) keys[i] = slot_key<T>(a, qx[k], qa[k], qb[k], qc[k], qw[k]);
+  }
+}
+
+// Counts per key without sorting (the input of the resample trigger): the
+// keys of k_keys, counted in an LDS histogram per block (lds_keys counters;
+// 0: straight into the global counts) and added to counts with integer
+// atomics (exact, so the result does not depend on the order).
+template <typename T>
+__global__ __launch_bounds__(256) void k_key_hist(KeyArgs a, const Clock* clk, int32_t* counts,
+                                                  int lds_keys) {
+  extern __shared__ uint32_t s_c[];
+  const int64_t n = clk->n;
+  for (int j = threadIdx.x; j < lds_keys; j += 256) s_c[j] = 0u;
+  __syncthreads();
+  const T* x = (const T*)a.x;
+  const T* vx = (const T*)a.vx;
+  const T* vy = (const T*)a.vy;
+  const T* vz = (const T*)a.vz;
+  const T* w = (const T*)a.w;
+  const int64_t stride = (int64_t)gridDim.x * 256 * kKeyItems;
+  for (int64_t base = (int64_t)blockIdx.x * 256 * kKeyItems + threadIdx.x; base < n;
+       base += stride) {
+    T q[kKeyItems][5];
+#pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) {
+      const int64_t i = base + k * 256;
+      const bool v = i < n;
+      q[k][0] = v && a.dim == 1 ? x[i] : (T)0;
+      q[k][1] = v ? vx[i] : (T)0;
+      q[k][2] = v ? vy[i] : (T)0;
+      q[k][3] = v ? vz[i] : (T)0;
+      q[k][4] = v ? w[i] : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) {
+      if (base + k * 256 >= n) continue;
+      const uint32_t key = slot_key<T>(a, q[k][0], q[k][1], q[k][2], q[k][3], q[k][4]);
+      if (lds_keys) atomicAdd(&s_c[key], 1u);
+      else atomicAdd(&counts[key], 1);
     }
   }
+  __syncthreads();
+  for (int j = threadIdx.x; j < lds_keys; j += 256)
+    if (s_c[j]) atomicAdd(&counts[j], (int32_t)s_c[j]);
 }
 
 // Offsets of every key from the sorted keys in one pass: position s sits
@@ -424,14 +466,8 @@ static int sort_keys(const LaunchEnv& env, const SortWs& ws, int n_keys, int32_t
   return check_launch("key counts");
 }
 
-int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
-                    const double* thr, int nb, int32_t* counts, int32_t* offsets,
-                    uint32_t* perm) {
+static KeyArgs key_args(const bz_state_t* s, const bz_params_t* p, const double* thr, int nb) {
   const int n_nodes = p->dim == 1 ? p->n_nodes : 1;
-  const int n_keys = n_nodes * nb;
-  if (n_keys + 1 > 65535) return fail("bin_sort: at most 65534 keys (two 8-bit passes)");
-  const int64_t cap = env.capacity;
-  SortWs ws = carve(env.sort_ws, cap, env.max_keys);
   KeyArgs ka;
   ka.x = s->x;
   ka.vx = s->vx;
@@ -444,7 +480,37 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
   ka.nb = nb;
   ka.n_nodes = n_nodes;
   ka.dim = p->dim;
-  ka.n_keys = n_keys;
+  ka.n_keys = n_nodes * nb;
+  return ka;
+}
+
+int launch_bin_count(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                     const double* thr, int nb, int32_t* counts) {
+  const KeyArgs ka = key_args(s, p, thr, nb);
+  const int m = ka.n_keys + 1;
+  // an LDS histogram per block while it fits in 64 KiB, global atomics above
+  const int lds_keys = m <= 16384 ? m : 0;
+  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, counts, (int64_t)m);
+  if (env.precision == BZ_F64)
+    hipLaunchKernelGGL(k_key_hist<double>, dim3(env.num_cu * 2), dim3(256),
+                       (size_t)lds_keys * 4, env.stream, ka, (const Clock*)env.clock, counts,
+                       lds_keys);
+  else
+    hipLaunchKernelGGL(k_key_hist<float>, dim3(env.num_cu * 2), dim3(256),
+                       (size_t)lds_keys * 4, env.stream, ka, (const Clock*)env.clock, counts,
+                       lds_keys);
+  return check_launch("k_key_hist");
+}
+
+int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                    const double* thr, int nb, int32_t* counts, int32_t* offsets,
+                    uint32_t* perm) {
+  const int n_nodes = p->dim == 1 ? p->n_nodes : 1;
+  const int n_keys = n_nodes * nb;
+  if (n_keys + 1 > 65535) return fail("bin_sort: at most 65534 keys (two 8-bit passes)");
+  const int64_t cap = env.capacity;
+  SortWs ws = carve(env.sort_ws, cap, env.max_keys);
+  const KeyArgs ka = key_args(s, p, thr, nb);
   timing_begin(env, 1);
   auto kk = env.precision == BZ_F64 ? k_keys<double> : k_keys<float>;
   // one thread per kKeyItems slots of the capacity; blocks past the count exit

@@ -228,14 +228,22 @@ class KineticEngine:
             return bool(np.any(ok & last & (c >= 3.0 * t)))
         return bool(np.any(ok & (c >= self.policy.trigger_at * t)))
 
+    def bin_count(self):
+        """Per-key counts alone (the trigger's input), no sort."""
+        _lib.check(self.lib.bz_bin_count(self.ctx, C.byref(self.state.struct),
+                                         C.byref(self.params), self.thr, self.nb,
+                                         _tptr(self.counts)), "bz_bin_count")
+
     def resample(self, force: bool = False) -> bool:
-        """Bin, sort and (when triggered) split / merge into the other buffer."""
-        self.bin_sort()
+        """Count, and when triggered sort and split / merge into the other buffer."""
         always = not self.policy.ref_trigger and self.policy.trigger_at <= 0
         # the counts are read back (a host synchronisation) only when the
-        # trigger can decline
-        if not force and not always and not self.trigger(self.counts.cpu().numpy()):
-            return False
+        # trigger can decline; the sort runs only for a pass that happens
+        if not force and not always:
+            self.bin_count()
+            if not self.trigger(self.counts.cpu().numpy()):
+                return False
+        self.bin_sort()
         src, dst = self.bufs[self.cur], self.bufs[1 - self.cur]
         _lib.check(self.lib.bz_resample(self.ctx, C.byref(src.struct), C.byref(dst.struct),
                                         C.byref(self.params), _tptr(self.counts),

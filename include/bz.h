@@ -270,7 +270,7 @@ int bz_reduction(bz_ctx_t ctx, int32_t r, int32_t M, int32_t RM, double* dw, con
                  const double* dx, int64_t ptcl);
 
 /* Energy / cell binning and stable radix sort by key = cell*nb + ebin,
- * cell = int((x + dx/2)/dx) (1bin/boltzmann.py:658), n_keys = n_nodes*nb <= 39999.
+ * cell = int((x + dx/2)/dx) (1bin/boltzmann.py:658), n_keys = n_nodes*nb <= 65534.
  * thresholds: [nb-1] eV, ascending (bin b holds thr[b-1] <= e < thr[b]).
  * Particles with w <= 0 get the extra key n_keys and sort last.
  * counts[n_keys+1], offsets[n_keys+2] (offsets[n_keys] = live particles),
@@ -280,6 +280,12 @@ int bz_reduction(bz_ctx_t ctx, int32_t r, int32_t M, int32_t RM, double* dw, con
 int bz_bin_sort(bz_ctx_t ctx, const bz_state_t* s, const bz_params_t* p,
                 const double* thresholds_host, int32_t nb,
                 int32_t* counts, int32_t* offsets, uint32_t* perm);
+
+/* The counts of bz_bin_sort alone (counts[n_keys+1], same keys), without the
+ * sort: what the resample trigger reads (1bin/boltzmann.py:894-916 calls
+ * bin_particles for its counts); a triggered pass then calls bz_bin_sort. */
+int bz_bin_count(bz_ctx_t ctx, const bz_state_t* s, const bz_params_t* p,
+                 const double* thresholds_host, int32_t nb, int32_t* counts);
 
 /* ES-WPNC resampling of the state sorted by bz_bin_sort: per (cell, bin)
  * segment split or merge according to `pol`, written in key order to out.

@@ -66,6 +66,27 @@ def test_bin_sort_parity(gpu, prec, nb):
     eng.close()
 
 
+@pytest.mark.parametrize("prec,nb,N", [(8, 3, 300), (4, 5, 4096)])
+def test_bin_count_matches_sort(gpu, prec, nb, N):
+    """The trigger's counts without the sort equal the oracle's per-key
+    counts; 4096 cells x 5 bins exceed the per-block LDS histogram and take
+    the global-atomic path."""
+    from boltzmann_amd import physics as ph
+    rng = np.random.default_rng(7 + nb)
+    cells = list(rng.integers(0, 60 if N > 1000 else 400, N + 1))
+    cfg, eng = _engine(N, prec, nb, 100, int(sum(cells)))
+    st = _state(cells, N, 2.5, cfg.np_dtype, seed=nb, e_ev=12.0)
+    w = st[4].copy()
+    w[::19] = 0
+    st = st[:4] + (w,)
+    eng.load_state(st)
+    eng.bin_count()
+    s = O.Setup(dim=1, dt=cfg.dt, n_cells=N, length_cm=2.5)
+    c, _, _ = O.bin_sort(s, nb, ph.bin_thresholds(nb), st)
+    np.testing.assert_array_equal(eng.counts.cpu().numpy()[:c.size], c)
+    eng.close()
+
+
 RS_CASES = [
     ("f64-1bin-drift", 8, 1, 3e5),
     ("f64-3bin", 8, 3, 0.0),
