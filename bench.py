@@ -60,6 +60,17 @@ def _time_oracle(setup, state, E, n, table, seed, seconds, capacity):
     return psteps / (time.perf_counter() - t0), done
 
 
+def _one_core(setup, state, E, table, seed, seconds):
+    """The same oracle step on one host thread over the first 200k particles
+    (the per-core figure beside the all-core one)."""
+    import dataclasses
+    m = min(state[0].size, 200_000)
+    sub = tuple(a[:m].copy() for a in state)
+    s1 = dataclasses.replace(setup, nthreads=1)
+    rate, done = _time_oracle(s1, sub, E, m, table, seed, seconds, 2 * m)
+    return {"value": rate, "cores": 1, "sample": f"{m} particles x {done} steps, 1 thread"}
+
+
 def cpu_baseline(cfg, seconds: float = 10.0):
     """Oracle (C++ restatement of the reference step, OpenMP over particles)
     on a bounded sample of the same workload: push + collisions + deposit +
@@ -91,7 +102,8 @@ def cpu_baseline(cfg, seconds: float = 10.0):
     rate, done = _time_oracle(setup, st, E, n, tab, cfg.seed, seconds, 2 * n)
     return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"{n} particles x {done} steps of the oracle push+MCC+deposit+compaction "
-                      f"(FP64 like the reference, OpenMP {cores} threads)"}
+                      f"(FP64 like the reference, OpenMP {cores} threads)",
+            "one_core": _one_core(setup, st, E, tab, cfg.seed, seconds / 4)}
 
 
 def cpu_baseline_swarms(scfg, seconds: float = 10.0):
@@ -113,11 +125,12 @@ def cpu_baseline_swarms(scfg, seconds: float = 10.0):
     setup = O.Setup(dim=0, dt=scfg.dt, xs_mode=1, gas_density_m3=scfg.gas_density_m3,
                     nu_max=table.nu_max, e_field0=en * ph.TD * scfg.gas_density_m3 / 100.0,
                     nthreads=cores)
-    rate, done = _time_oracle(setup, st, None, n, (table.energy, table.sig, table.slope),
-                              scfg.seed, seconds, 2 * n)
+    tab = (table.energy, table.sig, table.slope)
+    rate, done = _time_oracle(setup, st, None, n, tab, scfg.seed, seconds, 2 * n)
     return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"one 0-D swarm of {n} particles at {en:.1f} Td x {done} steps of the "
-                      f"oracle push+MCC (FP64 like the reference, OpenMP {cores} threads)"}
+                      f"oracle push+MCC (FP64 like the reference, OpenMP {cores} threads)",
+            "one_core": _one_core(setup, st, None, tab, scfg.seed, seconds / 4)}
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
