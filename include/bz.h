@@ -202,7 +202,8 @@ int bz_counters_reset(bz_ctx_t ctx);
 int bz_clock_advance(bz_ctx_t ctx, double dt);
 
 /* Kernel timing with HIP events recorded on the context stream around each
- * launch of the fused push kernel (k_push) and of the resampling kernels.
+ * launch of the fused push kernel (k_push_inplace, or k_push for a single
+ * 0-D swarm) and of the binning / resampling kernels.
  * bz_timing_enable(ctx, 1) clears and starts, 0 stops; bz_timing_get
  * synchronises and returns the summed milliseconds and launch counts of
  * [0] push, [1] sort, [2] resample.  No counterpart in the reference. */
