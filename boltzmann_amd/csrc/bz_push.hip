@@ -395,7 +395,7 @@ __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks
 // Counters of the push's per-block rows into the clock (wave q sums counter
 // q, all at once) and the slot count commit.  Runs in a block of its own.
 __device__ void dep_commit(const double* __restrict__ partials, int nblocks, int m, Clock* clk,
-                           int64_t cap, bool set_n) {
+                           int64_t cap, bool set_n, double advance_dt = 0.0) {
   const int lane = threadIdx.x & 63;
   const int q = threadIdx.x >> 6;
   const uint32_t* bcnt = (const uint32_t*)(partials + (size_t)nblocks * m);
@@ -418,6 +418,11 @@ __device__ void dep_commit(const double* __restrict__ partials, int nblocks, int
     }
     clk->epoch = clk->epoch + 1;
     clk->ticket = 0;
+    if (advance_dt > 0.0) {
+      // a step with no field stage after it (the batched swarms)
+      clk->step = clk->step + 1;
+      clk->t = clk->t + advance_dt;
+    }
   }
 }
 
@@ -755,7 +760,7 @@ __global__ __launch_bounds__(kRedThreads) void k_push_post(PushArgs a, int tile,
                                                            int ny, double* l2,
                                                            uint32_t* col_done,
                                                            double* __restrict__ out,
-                                                           double* acc) {
+                                                           double* acc, double advance_dt) {
   __shared__ double s_part[kRedWaves][64];
   __shared__ int s_last;
   __shared__ uint32_t s_red[kRedWaves];
@@ -767,7 +772,7 @@ __global__ __launch_bounds__(kRedThreads) void k_push_post(PushArgs a, int tile,
     dep_reduce_part(a.partials, nbirth, m, rpc, l2, col_done, out, acc, r % nx, r / nx, ny,
                     s_part, &s_last);
   } else {
-    dep_commit(a.partials, nbirth, m, a.clk, a.capacity, false);
+    dep_commit(a.partials, nbirth, m, a.clk, a.capacity, false, advance_dt);
   }
 }
 
@@ -1113,7 +1118,7 @@ static int launch_push_prec(const LaunchEnv& env, PushArgs& a, int shape, PushLa
 // The stage after the push: births + reduction + counters in one launch
 // (in place), or the reduction with the slot count commit (compacting push).
 static int launch_push_post(const LaunchEnv& env, const PushArgs& a, const PushLaunch& pl, int m,
-                            double* out, double* acc) {
+                            double* out, double* acc, double advance_dt = 0.0) {
   if (!pl.inplace) return launch_dep_reduce(env, pl.grid, m, out, 1, (int64_t)env.capacity, acc);
   int rpc = 0, nx = 0, ny = 0;
   const int rc = red_geometry(env, pl.grid, m, &rpc, &nx, &ny);
@@ -1122,11 +1127,11 @@ static int launch_push_post(const LaunchEnv& env, const PushArgs& a, const PushL
   if (env.precision == BZ_F64)
     hipLaunchKernelGGL(k_push_post<double>, dim3(blocks), dim3(kRedThreads), 0, env.stream, a,
                        pl.tile, pl.nw, pl.grid, m, rpc, nx, ny, env.dep_l2, env.dep_cols_done,
-                       out, acc);
+                       out, acc, advance_dt);
   else
     hipLaunchKernelGGL(k_push_post<float>, dim3(blocks), dim3(kRedThreads), 0, env.stream, a,
                        pl.tile, pl.nw, pl.grid, m, rpc, nx, ny, env.dep_l2, env.dep_cols_done,
-                       out, acc);
+                       out, acc, advance_dt);
   return check_launch("k_push_post");
 }
 
@@ -1349,7 +1354,8 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
   const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &pl)
                                          : launch_push_prec<float>(env, a, shape, &pl);
   if (rc) return rc;
-  return launch_push_post(env, a, pl, n_inst * kImN, inst_moments, acc);
+  // the step ends here (no field stage): the clock advances by dt
+  return launch_push_post(env, a, pl, n_inst * kImN, inst_moments, acc, p->dt);
 }
 
 }  // namespace bz

@@ -9,7 +9,9 @@ in a hipGraph.
 1-D runs push in place: an absorbed particle leaves a hole (w = 0) and the
 secondaries are appended after the slots in use.  Resampling rewrites the
 state without holes; otherwise the state is compacted every
-`compact_every` steps.  0-D runs compact in every push.
+`compact_every` steps (holes come only from absorption at the walls, ~0.02 %
+of the electrons per step in the discharges, so 64 steps leave ~1 %).  0-D
+runs compact in every push.
 """
 from __future__ import annotations
 
@@ -47,7 +49,7 @@ class KineticEngine:
 
     def __init__(self, cfg: ph.RunConfig, device: int = 0, capacity: Optional[int] = None,
                  rng_mode: int = _lib.BZ_RNG_PHILOX, max_keys: Optional[int] = None,
-                 compact_every: int = 16):
+                 compact_every: int = 64):
         if not torch.cuda.is_available():
             raise _lib.BzError("no GPU visible: the electron step runs only in the HIP kernels")
         self.lib = _lib.load()

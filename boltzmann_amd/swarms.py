@@ -147,14 +147,14 @@ class SwarmBatch:
         _lib.check(self.lib.bz_clock_set(self.ctx, n, 0, 0.0))
 
     def _step_call(self, accumulate: bool):
-        # in place; the other buffer stages the secondaries
+        # in place (the other buffer stages the secondaries); the step also
+        # advances the device clock by dt
         src, dst = self.structs[self.cur], self.structs[1 - self.cur]
         _lib.check(self.lib.bz_swarm_step(self.ctx, C.byref(src), C.byref(dst),
                                           C.byref(self.params), C.byref(self.xs), self.n_inst,
                                           None, _tptr(self.inst_E),
                                           _tptr(self.moments), _tptr(self.acc),
                                           1 if accumulate else 0), "bz_swarm_step")
-        _lib.check(self.lib.bz_clock_advance(self.ctx, self.cfg.dt))
 
     def _control_call(self, policy=None):
         src, dst = self.structs[self.cur], self.structs[1 - self.cur]

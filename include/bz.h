@@ -307,7 +307,8 @@ int bz_resample(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
  * bz_swarm_step advances every slot in place (velocities only: 0-D has no
  * walls, x and w are kept) and appends the secondaries after the slots in
  * use, in parent order, each in its parent's instance; `out` is staging for
- * them.  inst_off is neither read nor written by the step (it may be NULL).
+ * them.  The step then advances the clock (step += 1, t += p->dt).
+ * inst_off is neither read nor written by the step (it may be NULL).
  * inst_moments: [n_inst][8] FP64 of this step: sum w, sum w e, sum w vx,
  * weights of elastic / excitation / ionization events, births, sum w e^2.
  * inst_acc (optional): running sums of inst_moments when accumulate != 0. */
