@@ -101,6 +101,7 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   e.max_keys = max_keys;
   e.push_shape = 0;
   e.push_bpc = 0;
+  e.push_queue = 0;
   // smallest push tile is 256 particles
   // look-back tile words, or the birth counts of the in-place 1-D push (u32:
   // 4096 per block, then one per tile and wave: at most capacity/64 + 64)
@@ -181,6 +182,13 @@ int bz_set_push_grid(bz_ctx_t c, int blocks_per_cu) {
   if (!c) return fail("null context");
   if (blocks_per_cu < 0 || blocks_per_cu > 8) return fail("blocks per CU must be 0..8");
   c->env.push_bpc = blocks_per_cu;
+  return 0;
+}
+
+int bz_set_push_queue(bz_ctx_t c, int mode) {
+  if (!c) return fail("null context");
+  if (mode < -1 || mode > 1) return fail("push queue mode must be -1, 0 or 1");
+  c->env.push_queue = mode;
   return 0;
 }
 

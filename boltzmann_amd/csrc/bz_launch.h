@@ -32,6 +32,7 @@ struct LaunchEnv {
   int push_shape;            // tile shape picked by the autotuner
   int push_sched;            // 0 static tiles + cooperative launch, 1 ticket counter
   int push_bpc;              // 1-D push: blocks per CU (0: as many as fit)
+  int push_queue;            // 1-D push candidate queue: 0 auto, 1 always, -1 never
   // optional kernel timing: event pairs per timed kernel class
   struct Timing* timing;
 };

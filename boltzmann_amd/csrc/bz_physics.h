@@ -180,6 +180,50 @@ __device__ __forceinline__ void table_rates(T e, T vmag, const bz_params_t& p,
   over = r2 > (T)1;
 }
 
+// Upper bound of the collision probability a particle can have in energy
+// bucket b of the table's hint (the bucket holds energies with the same top
+// bits, below bucket_upper): the largest sigma of every channel over the
+// table nodes that bracket the bucket, times the speed at the bucket's upper
+// energy, gives nu_up >= nu_t(e) for every e in it; the bound is P =
+// 1 - exp(-nu_up dt) (direct) or nu_up pc (null collision), raised by 1e-4
+// relative to cover the rounding of the per-particle evaluation.  A uniform
+// R0 at or above it means no collision, exactly as the full evaluation
+// decides.  A bucket whose null-collision bound passes P_null returns 2 (every
+// particle of it takes the full path, which counts nu_max overruns).
+template <typename T>
+__device__ __forceinline__ double bucket_upper(int b_abs, int bits);
+template <>
+__device__ __forceinline__ double bucket_upper<double>(int b_abs, int bits) {
+  return __longlong_as_double((long long)((uint64_t)(b_abs + 1) << (52 - bits)));
+}
+template <>
+__device__ __forceinline__ double bucket_upper<float>(int b_abs, int bits) {
+  return (double)__uint_as_float((uint32_t)(b_abs + 1) << (23 - bits));
+}
+
+template <typename T, int XS>
+__device__ double bucket_bound(int b, const bz_xsec_t& xs, const bz_params_t& p) {
+  const T* E = (const T*)xs.energy;
+  const T* S = (const T*)xs.sig;
+  const int n = xs.n;
+  const int j0 = xs.hint[b];
+  int j1 = b + 1 < xs.n_hint ? xs.hint[b + 1] + 1 : n - 1;
+  j1 = j1 < n - 1 ? j1 : n - 1;
+  double sig_up = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    double m = 0.0;
+    for (int j = j0; j <= j1; ++j) m = fmax(m, (double)S[c * n + j]);
+    sig_up += m;
+  }
+  double e_hi = bucket_upper<T>(b + xs.hint_base, xs.hint_bits);
+  e_hi = fmin(e_hi, (double)E[n - 1]);
+  const double nu_up = sig_up * sqrt(e_hi / p.kev);
+  const double P = XS == BZ_XS_DIRECT ? -expm1(-p.dt * nu_up) : nu_up * p.pc;
+  const double Pm = P * (1.0 + 1e-4) + 1e-30;
+  if (XS != BZ_XS_DIRECT && Pm > p.p_null) return 2.0;
+  return Pm;
+}
+
 template <typename T>
 __device__ __forceinline__ void iso_dir(T vnew, T ra, T rb, T& vx, T& vy, T& vz) {
   const T cchi = (T)2 * ra - (T)1;
@@ -237,6 +281,42 @@ struct StepOut {
   int over;
   int n_el, n_exc, n_null;  // events of the step (several in free-flight mode)
 };
+
+// Push and walls after the collision (electron_kernel_numba.py:257-287):
+// velocity (vx, vy, vz) after the collision, field of the particle's cell
+// (1-D) or of its swarm, the configured update order; fills o.p, o.alive
+// and o.e_post.
+template <typename T, int DIM>
+__device__ __forceinline__ void push_tail(const P5<T>& in, T vx, T vy, T vz,
+                                          const bz_params_t& p, const T* __restrict__ E,
+                                          T e_local, StepOut<T>& o) {
+  T x = in.x;
+  T ef;
+  if (DIM == 1) {
+    int c = (int)((double)in.x / p.dx);
+    c = c < 0 ? 0 : (c >= p.n_cells ? p.n_cells - 1 : c);
+    ef = E[c];
+  } else {
+    ef = e_local;  // 0-D: uniform field of the swarm (or of its instance)
+  }
+  if (p.push_order == BZ_PUSH_LEAPFROG) {
+    // kick with E at the step's starting cell, then drift with the new v
+    vx = vx - (T)p.afac * ((T)100 * ef);
+    if (DIM == 1) x = x + (T)p.xfac * vx;
+  } else {
+    // reference order: drift with the post-collision v, then kick
+    if (DIM == 1) x = x + (T)p.xfac * vx;
+    vx = vx - (T)p.afac * ((T)100 * ef);
+  }
+  if (DIM == 1) o.alive = ((T)0 < x && x < (T)p.L) ? 1 : 0;
+  else o.alive = 1;
+  o.p.x = x;
+  o.p.vx = vx;
+  o.p.vy = vy;
+  o.p.vz = vz;
+  o.p.w = in.w;
+  o.e_post = (T)p.kev * (vx * vx + vy * vy + vz * vz);
+}
 
 // One particle through collision + push + walls
 // (electron_kernel_numba.py:39-287; dead particles are dropped, not pushed).
@@ -335,32 +415,7 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
   o.n_el = o.chan == kChanElastic;
   o.n_exc = o.chan == kChanExc;
   o.n_null = o.chan == kChanNone;
-  T x = in.x;
-  T ef;
-  if (DIM == 1) {
-    int c = (int)((double)in.x / p.dx);
-    c = c < 0 ? 0 : (c >= p.n_cells ? p.n_cells - 1 : c);
-    ef = E[c];
-  } else {
-    ef = e_local;  // 0-D: uniform field of the swarm (or of its instance)
-  }
-  if (p.push_order == BZ_PUSH_LEAPFROG) {
-    // kick with E at the step's starting cell, then drift with the new v
-    vx = vx - (T)p.afac * ((T)100 * ef);
-    if (DIM == 1) x = x + (T)p.xfac * vx;
-  } else {
-    // reference order: drift with the post-collision v, then kick
-    if (DIM == 1) x = x + (T)p.xfac * vx;
-    vx = vx - (T)p.afac * ((T)100 * ef);
-  }
-  if (DIM == 1) o.alive = ((T)0 < x && x < (T)p.L) ? 1 : 0;
-  else o.alive = 1;
-  o.p.x = x;
-  o.p.vx = vx;
-  o.p.vy = vy;
-  o.p.vz = vz;
-  o.p.w = in.w;
-  o.e_post = (T)p.kev * (vx * vx + vy * vy + vz * vz);
+  push_tail<T, DIM>(in, vx, vy, vz, p, E, e_local, o);
   return o;
 }
 

@@ -47,6 +47,7 @@ struct PushArgs {
   const void* inst_E;  // [n_inst] field of each instance, V/cm
   int n_inst;
   uint32_t* block_births;  // in-place mode: births of each block
+  int queue_mode;          // 1-D push candidate queue: 1 always, 0 where walks are long
 };
 
 // per-block counter row of the push (u32): births, absorbed, elastic,
@@ -461,6 +462,25 @@ __global__ __launch_bounds__(kRedThreads) void k_dep_reduce(
 // staged in LDS, x and w are never rewritten (no walls in 0-D), and a
 // secondary inherits its parent's x, i.e. its instance.
 // ---------------------------------------------------------------------------
+// Candidate queue of the 1-D push: slots per wave (a batch of 64 runs as soon
+// as 64 are queued, so one tile item never overflows it).
+constexpr int kQCap = 128;
+// tile items per wave below which the push does not queue
+constexpr uint32_t kQueueMinItems = 16;
+
+// the 1-D push queues its collision candidates (tabulated sigma, Philox)
+template <int XS, int DIM, int RNG>
+struct QueuedPush {
+  static constexpr bool value =
+      DIM == 1 && (XS == BZ_XS_TABLE || XS == BZ_XS_DIRECT) && RNG == BZ_RNG_PHILOX;
+};
+
+// LDS bytes after the staged table: per-bucket bounds [n_hint] and the
+// per-wave candidate queues [NW][kQCap]
+__host__ __device__ inline size_t queue_lds_bytes(int n_hint, int word, int nw) {
+  return (((size_t)n_hint * word + 15) & ~(size_t)15) + (size_t)nw * kQCap * 4;
+}
+
 // LDS doubles before the staged table: DIM 1 histograms, DIM 2 moments and fields
 __host__ __device__ inline int inplace_lds_doubles(int dim, int nn, int n_inst, int word) {
   if (dim == 1) return table_lds_offset_doubles(nn);
@@ -468,7 +488,7 @@ __host__ __device__ inline int inplace_lds_doubles(int dim, int nn, int n_inst, 
   return (d + 1) & ~1;
 }
 
-template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS, bool QUEUE>
 __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   static_assert(DIM == 1 || DIM == 2, "in-place push: 1-D discharges and batched swarms");
   constexpr int NW = BLOCK / kWave;
@@ -478,6 +498,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   __shared__ unsigned long long s_cnt[kCntRow];
   __shared__ uint32_t s_births;
 
+  constexpr bool kQueue = QUEUE && QueuedPush<XS, DIM, RNG>::value;
   const bz_params_t& p = a.p;
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
@@ -485,6 +506,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   const int nn = DIM == 1 ? p.n_nodes : 0;
   const int n_inst = DIM == 2 ? a.n_inst : 0;
   const int sums = DIM == 1 ? 3 * nn : n_inst * kImN;
+  T* s_pup = nullptr;       // kQueue: collision-probability bound per hint bucket
+  uint32_t* s_q = nullptr;  // kQueue: candidate slots, kQCap per wave
   for (int j = tid; j < sums; j += BLOCK) s_dep[j] = 0.0;
   T* s_ie = (T*)(s_dep + sums);  // DIM 2: field of every instance
   if (DIM == 2)
@@ -504,6 +527,10 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
     xs.sig = s_tab + n;
     xs.slope = s_tab + 4 * n;
     xs.hint = s_hint;
+    if (kQueue) {
+      s_pup = (T*)((char*)s_tab + table_lds_bytes(n, a.xs.n_hint, (int)sizeof(T)));
+      s_q = (uint32_t*)((char*)s_pup + (((size_t)a.xs.n_hint * sizeof(T) + 15) & ~(size_t)15));
+    }
   }
 
   const int64_t n = a.clk->n;
@@ -532,7 +559,16 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
 #pragma unroll
   for (int q = 0; q < kImN; ++q) macc[q] = 0.0;
   int m_inst = -1;
+  // the queued path (below) only where a wave walks enough tiles for its
+  // batches to fill: the last, partial batch re-reads its particles, a
+  // latency a short walk would pay
+  const bool queued = kQueue && !(p.gas_sigma > 0.0) &&
+                      (a.queue_mode > 0 || per * ITEMS >= kQueueMinItems);
   __syncthreads();  // sums cleared, fields and table staged
+  if (kQueue && queued) {
+    for (int b = tid; b < xs.n_hint; b += BLOCK) s_pup[b] = (T)bucket_bound<T, XS>(b, xs, p);
+    __syncthreads();
+  }
 
   P5<T> cur[ITEMS];
   {
@@ -541,6 +577,146 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
     for (int k = 0; k < ITEMS; ++k)
       load5<T, true>(in, b0 + k * kWave + lane, t_first < t_end ? n : 0, cur[k]);
   }
+  // Queued 1-D path (cold gas).  A live particle whose uniform R0 is at or
+  // above its energy bucket's bound cannot collide: it is pushed right away
+  // (the null branch of step_particle).  The others are queued per wave in
+  // slot order and run through step_particle 64 at a time, so a wave does
+  // the sigma interpolation, the probability and the scatter on full lanes.
+  // Births are staged as in the general path: in the (tile, wave) slot span
+  // of their parent, in slot order.  The wave's birth count of every tile
+  // goes to wave_births once, in tile order, from lane 0: t_open is the tile
+  // the batches are in, c_open its births so far; every tile below t_open
+  // has been written.
+  if constexpr (kQueue) {
+   if (queued) {
+    uint32_t* q = s_q + wid * kQCap;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const T e_bot = ((const T*)xs.energy)[0], e_top = ((const T*)xs.energy)[xs.n - 1];
+    int qn = 0;                 // queued slots (wave-uniform)
+    uint32_t t_open = t_first;  // wave-uniform
+    uint32_t c_open = 0;
+    auto advance_to = [&](uint32_t t_new) {
+      if (lane == 0) {
+        wave_births[(size_t)t_open * NW + wid] = c_open;
+        for (uint32_t tt = t_open + 1; tt < t_new; ++tt) wave_births[(size_t)tt * NW + wid] = 0u;
+      }
+      t_open = t_new;
+      c_open = 0;
+    };
+    auto run_batch = [&](int nb) {
+      const bool act = lane < nb;
+      const uint32_t i = act ? q[lane] : 0u;
+      P5<T> c;
+      load5<T, true>(in, (int64_t)i, act ? n : 0, c);
+      const StepOut<T> o = step_particle<T, XS, 1, RNG>(c, i, step, p, xs, E, R, G, (T)0);
+      if (act) {
+        ++c_live;
+        c_el += o.n_el;
+        c_ex += o.n_exc;
+        c_io += o.chan == kChanIon;
+        c_null += o.n_null;
+        c_abs += !o.alive;
+        c_over += o.over;
+        if (o.alive) {
+          st[0][i] = o.p.x;
+          st[1][i] = o.p.vx;
+          st[2][i] = o.p.vy;
+          st[3][i] = o.p.vz;
+          const double w = (double)o.p.w;
+          const int nd = dep_node((double)o.p.x, p);
+          atomicAdd(&s_dep[nd], w);
+          if (o.chan == kChanIon) atomicAdd(&s_dep[nn + nd], w);
+          atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
+        } else {
+          st[4][i] = (T)0;
+        }
+      }
+      // births: the batch's slots rise, so equal tiles are runs of lanes
+      // (segments), taken in order
+      const bool born = act && o.born;
+      const uint64_t m = __ballot(born);
+      const uint32_t te = act ? i / (uint32_t)TILE : 0xFFFFFFFFu;
+      const uint32_t tprev = __shfl_up(te, 1, kWave);
+      uint64_t heads = __ballot(act && (lane == 0 || te != tprev));
+      while (heads) {
+        const int s0 = (int)__builtin_ctzll(heads);
+        heads &= heads - 1ull;
+        const int s1 = heads ? (int)__builtin_ctzll(heads) : 64;
+        const uint64_t below_s1 = s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull);
+        const uint64_t seg = ~((1ull << s0) - 1ull) & below_s1;
+        const uint32_t ts = __shfl(te, s0, kWave);
+        if (ts != t_open) advance_to(ts);
+        if (born && ((seg >> lane) & 1ull)) {
+          const uint32_t r = c_open + (uint32_t)__popcll(m & lt & seg);
+          store5<T, true>(a.out, (int64_t)ts * TILE + wid * WSPAN + r, o.s);
+          const double w = (double)o.s.w;
+          const int nd = dep_node((double)o.s.x, p);
+          atomicAdd(&s_dep[nd], w);
+          atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+        }
+        c_open += (uint32_t)__popcll(m & seg);
+      }
+      w_births += (uint32_t)__popcll(m);
+    };
+    for (uint32_t t = t_first; t < t_end; ++t) {
+      const int64_t bw = (int64_t)t * TILE + wid * WSPAN;
+      P5<T> nxt[ITEMS];
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k)
+        load5<T, true>(in, bw + TILE + k * kWave + lane, t + 1 < t_end ? n : 0, nxt[k]);
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        const int64_t i = bw + k * kWave + lane;
+        const bool live = i < n && cur[k].w > (T)0;
+        bool cand = false;
+        if (live) {
+          const T s2 = cur[k].vx * cur[k].vx + cur[k].vy * cur[k].vy + cur[k].vz * cur[k].vz;
+          const T e = (T)p.kev * s2;
+          if (!(e < e_top)) {
+            cand = true;  // beyond the table: the speed is not bounded
+          } else {
+            const T ec = e < e_bot ? e_bot : e;
+            int b = xs_bucket<T>(ec, xs.hint_bits) - xs.hint_base;
+            b = b < 0 ? 0 : (b >= xs.n_hint ? xs.n_hint - 1 : b);
+            Uniforms<T> U;
+            U.first(p.seed, (uint32_t)i, step);
+            cand = U.u[0] < s_pup[b];
+          }
+        }
+        if (live && !cand) {
+          StepOut<T> o;
+          push_tail<T, 1>(cur[k], cur[k].vx, cur[k].vy, cur[k].vz, p, E, (T)0, o);
+          ++c_live;
+          ++c_null;
+          if (o.alive) {
+            st[0][i] = o.p.x;
+            st[1][i] = o.p.vx;
+            const double w = (double)o.p.w;
+            const int nd = dep_node((double)o.p.x, p);
+            atomicAdd(&s_dep[nd], w);
+            atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
+          } else {
+            ++c_abs;
+            st[4][i] = (T)0;
+          }
+        }
+        const uint64_t cm = __ballot(live && cand);
+        if (live && cand) q[qn + (int)__popcll(cm & lt)] = (uint32_t)i;
+        qn += (int)__popcll(cm);
+        if (qn >= 64) {
+          run_batch(64);
+          if (lane < qn - 64) q[lane] = q[64 + lane];
+          qn -= 64;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) cur[k] = nxt[k];
+    }
+    if (qn > 0) run_batch(qn);
+    if (t_first < t_end) advance_to(t_end);  // the last tile's count, zeros after it
+   }
+  }
+  if (!queued)
   for (uint32_t t = t_first; t < t_end; ++t) {
     const int64_t bw = (int64_t)t * TILE + wid * WSPAN;
     // next tile first, so its HBM latency hides behind this tile's physics
@@ -981,13 +1157,15 @@ struct PushLaunch {
 };
 
 // 1-D discharges and batched swarms: k_push_inplace (k_push_post follows)
-template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
-static int launch_push_inplace_t(const LaunchEnv& env, PushArgs& a, PushLaunch* pl) {
-  auto kern = k_push_inplace<T, XS, DIM, RNG, BLOCK, ITEMS>;
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS, bool QUEUE>
+static int launch_push_inplace_q(const LaunchEnv& env, PushArgs& a, PushLaunch* pl) {
+  auto kern = k_push_inplace<T, XS, DIM, RNG, BLOCK, ITEMS, QUEUE>;
   const int nn = DIM == 1 ? a.p.n_nodes : 0;
   const int part_doubles = DIM == 1 ? 3 * nn : a.n_inst * kImN;
   size_t smem = (size_t)inplace_lds_doubles(DIM, nn, a.n_inst, (int)sizeof(T)) * sizeof(double);
   if (XS != BZ_XS_FIT) smem += table_lds_bytes(a.xs.n, a.xs.n_hint, (int)sizeof(T));
+  if (QUEUE && QueuedPush<XS, DIM, RNG>::value)
+    smem += queue_lds_bytes(a.xs.n_hint, (int)sizeof(T), BLOCK / kWave);
   if (smem > 160 * 1024) return fail("push: sums + table need %zu B of LDS (> 160 KiB)", smem);
   if (smem > 64 * 1024) {
     if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1018,6 +1196,23 @@ static int launch_push_inplace_t(const LaunchEnv& env, PushArgs& a, PushLaunch* 
   pl->nw = BLOCK / kWave;
   pl->inplace = true;
   return check_launch("k_push_inplace");
+}
+
+// The queued variant where it can pay: forced, or when a full state would
+// give each wave at least kQueueMinItems tile items (the kernel still
+// checks the walk of the actual count).
+template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
+static int launch_push_inplace_t(const LaunchEnv& env, PushArgs& a, PushLaunch* pl) {
+  if constexpr (QueuedPush<XS, DIM, RNG>::value) {
+    const int64_t walk = env.capacity / ((int64_t)BLOCK * env.num_cu);  // items per wave, 1 block/CU
+    const bool want = env.push_queue > 0 || (env.push_queue == 0 && walk >= kQueueMinItems);
+    if (want && !(a.p.gas_sigma > 0.0)) {
+      a.queue_mode = env.push_queue > 0 ? 1 : 0;
+      return launch_push_inplace_q<T, XS, DIM, RNG, BLOCK, ITEMS, true>(env, a, pl);
+    }
+  }
+  a.queue_mode = 0;
+  return launch_push_inplace_q<T, XS, DIM, RNG, BLOCK, ITEMS, false>(env, a, pl);
 }
 
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>

@@ -188,6 +188,14 @@ int bz_set_push_shape(bz_ctx_t ctx, int shape);
  * (table staging, histogram flush) over more particles; chosen by the
  * autotuner.  No counterpart in the reference. */
 int bz_set_push_grid(bz_ctx_t ctx, int blocks_per_cu);
+/* Candidate queue of the 1-D push (tabulated sigma, Philox stream, cold
+ * gas): a particle whose uniform is at or above the collision-probability
+ * bound of its energy bucket is pushed at once; the others are queued per
+ * wave and collided 64 at a time.  The results are the same either way.
+ * 0 (default): when a wave walks at least 16 tile items (a kernel variant
+ * with the queue is launched when the capacity allows that many), 1: always,
+ * -1: never.  No counterpart in the reference. */
+int bz_set_push_queue(bz_ctx_t ctx, int mode);
 /* Tile schedule of the fused push: 0 static tile order under a cooperative
  * launch (default), 1 tiles claimed from a ticket counter by running blocks.
  * No counterpart in the reference. */
