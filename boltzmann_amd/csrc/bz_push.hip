@@ -468,17 +468,70 @@ constexpr int kQCap = 128;
 // tile items per wave below which the push does not queue
 constexpr uint32_t kQueueMinItems = 16;
 
-// the 1-D push queues its collision candidates (tabulated sigma, Philox)
+// the in-place push queues its collision candidates (tabulated sigma,
+// Philox stream; the swarms use the null-collision table)
 template <int XS, int DIM, int RNG>
 struct QueuedPush {
   static constexpr bool value =
-      DIM == 1 && (XS == BZ_XS_TABLE || XS == BZ_XS_DIRECT) && RNG == BZ_RNG_PHILOX;
+      (XS == BZ_XS_TABLE || (DIM == 1 && XS == BZ_XS_DIRECT)) && RNG == BZ_RNG_PHILOX;
 };
 
 // LDS bytes after the staged table: per-bucket bounds [n_hint] and the
 // per-wave candidate queues [NW][kQCap]
 __host__ __device__ inline size_t queue_lds_bytes(int n_hint, int word, int nw) {
   return (((size_t)n_hint * word + 15) & ~(size_t)15) + (size_t)nw * kQCap * 4;
+}
+
+// The kImN per-instance moments of one swarm particle's step (all zero for a
+// lane that is not counted): survivor and secondary weight, energy, energy
+// squared and x momentum, the weight of the event's channel, the birth.
+template <typename T>
+__device__ __forceinline__ void swarm_moments(const StepOut<T>& o, T w_in, bool counted,
+                                              double mv[kImN]) {
+  const double w = counted ? (double)o.p.w : 0.0;
+  const double ep = (double)o.e_post;
+  const double ws = counted && o.born ? (double)o.s.w : 0.0;
+  const double es = (double)o.e_sec;
+  const double wc = counted ? (double)w_in : 0.0;
+  mv[kImW] = w + ws;
+  mv[kImWE] = w * ep + ws * es;
+  mv[kImWVX] = w * (double)o.p.vx + ws * (double)o.s.vx;
+  mv[kImEl] = o.chan == kChanElastic ? wc : 0.0;
+  mv[kImExc] = o.chan == kChanExc ? wc : 0.0;
+  mv[kImIon] = o.chan == kChanIon ? wc : 0.0;
+  mv[kImBirth] = counted ? (double)o.born : 0.0;
+  mv[kImWE2] = w * ep * ep + ws * es * es;
+}
+
+// Adds one item's moments to the wave's per-lane running sums of instance
+// m_inst (wave-uniform, -1: none).  When the counted lanes leave m_inst the
+// sums are added to the LDS moments first; lanes of several instances (an
+// instance boundary, the births) add theirs to LDS directly.
+__device__ __forceinline__ void swarm_accumulate(const double mv[kImN], bool counted, int inst,
+                                                 int& m_inst, double macc[kImN], double* s_im,
+                                                 int lane) {
+  const uint64_t lm = __ballot(counted);
+  if (!lm) return;
+  if (!__all(!counted || inst == m_inst)) {
+    if (m_inst >= 0) {
+#pragma unroll
+      for (int q = 0; q < kImN; ++q) {
+        const double sq = wave_sum(macc[q]);
+        macc[q] = 0.0;
+        if (lane == 0 && sq != 0.0) atomicAdd(&s_im[m_inst * kImN + q], sq);
+      }
+    }
+    const int inst0 = __shfl(inst, (int)__builtin_ctzll(lm), kWave);
+    m_inst = __all(!counted || inst == inst0) ? inst0 : -1;
+  }
+  if (m_inst >= 0) {
+#pragma unroll
+    for (int q = 0; q < kImN; ++q) macc[q] += mv[q];
+  } else if (counted) {
+#pragma unroll
+    for (int q = 0; q < kImN; ++q)
+      if (mv[q] != 0.0) atomicAdd(&s_im[inst * kImN + q], mv[q]);
+  }
 }
 
 // LDS doubles before the staged table: DIM 1 histograms, DIM 2 moments and fields
@@ -587,6 +640,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   // goes to wave_births once, in tile order, from lane 0: t_open is the tile
   // the batches are in, c_open its births so far; every tile below t_open
   // has been written.
+  constexpr int PD = DIM == 1 ? 1 : 0;  // physics dimension
   if constexpr (kQueue) {
    if (queued) {
     uint32_t* q = s_q + wid * kQCap;
@@ -608,7 +662,14 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
       const uint32_t i = act ? q[lane] : 0u;
       P5<T> c;
       load5<T, true>(in, (int64_t)i, act ? n : 0, c);
-      const StepOut<T> o = step_particle<T, XS, 1, RNG>(c, i, step, p, xs, E, R, G, (T)0);
+      int inst = 0;
+      T e_local = (T)0;
+      if (DIM == 2) {
+        inst = (int)c.x;
+        inst = inst < 0 ? 0 : (inst >= n_inst ? n_inst - 1 : inst);
+        e_local = s_ie[inst];
+      }
+      const StepOut<T> o = step_particle<T, XS, PD, RNG>(c, i, step, p, xs, E, R, G, e_local);
       if (act) {
         ++c_live;
         c_el += o.n_el;
@@ -617,7 +678,17 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
         c_null += o.n_null;
         c_abs += !o.alive;
         c_over += o.over;
-        if (o.alive) {
+        if (DIM == 2) {
+          // velocities only; this batch's instances are mixed: per-lane sums
+          st[1][i] = o.p.vx;
+          st[2][i] = o.p.vy;
+          st[3][i] = o.p.vz;
+          double mv[kImN];
+          swarm_moments<T>(o, c.w, true, mv);
+#pragma unroll
+          for (int qq = 0; qq < kImN; ++qq)
+            if (mv[qq] != 0.0) atomicAdd(&s_dep[inst * kImN + qq], mv[qq]);
+        } else if (o.alive) {
           st[0][i] = o.p.x;
           st[1][i] = o.p.vx;
           st[2][i] = o.p.vy;
@@ -649,10 +720,12 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
         if (born && ((seg >> lane) & 1ull)) {
           const uint32_t r = c_open + (uint32_t)__popcll(m & lt & seg);
           store5<T, true>(a.out, (int64_t)ts * TILE + wid * WSPAN + r, o.s);
-          const double w = (double)o.s.w;
-          const int nd = dep_node((double)o.s.x, p);
-          atomicAdd(&s_dep[nd], w);
-          atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+          if (DIM == 1) {
+            const double w = (double)o.s.w;
+            const int nd = dep_node((double)o.s.x, p);
+            atomicAdd(&s_dep[nd], w);
+            atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+          }
         }
         c_open += (uint32_t)__popcll(m & seg);
       }
@@ -683,12 +756,26 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
             cand = U.u[0] < s_pup[b];
           }
         }
-        if (live && !cand) {
-          StepOut<T> o;
-          push_tail<T, 1>(cur[k], cur[k].vx, cur[k].vy, cur[k].vz, p, E, (T)0, o);
+        const bool plain = live && !cand;
+        int inst = 0;
+        T e_local = (T)0;
+        if (DIM == 2) {
+          inst = (int)cur[k].x;
+          inst = inst < 0 ? 0 : (inst >= n_inst ? n_inst - 1 : inst);
+          e_local = s_ie[inst];
+        }
+        StepOut<T> o;
+        o.born = 0;
+        o.chan = kChanNone;
+        o.e_sec = (T)0;
+        o.s = cur[k];
+        push_tail<T, PD>(cur[k], cur[k].vx, cur[k].vy, cur[k].vz, p, E, e_local, o);
+        if (plain) {
           ++c_live;
           ++c_null;
-          if (o.alive) {
+          if (DIM == 2) {
+            st[1][i] = o.p.vx;  // only vx moves without a collision
+          } else if (o.alive) {
             st[0][i] = o.p.x;
             st[1][i] = o.p.vx;
             const double w = (double)o.p.w;
@@ -699,6 +786,13 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
             ++c_abs;
             st[4][i] = (T)0;
           }
+        }
+        if (DIM == 2) {
+          // running moments of the wave's instance (the queued candidates
+          // add theirs in their batch)
+          double mv[kImN];
+          swarm_moments<T>(o, cur[k].w, plain, mv);
+          swarm_accumulate(mv, plain, inst, m_inst, macc, s_dep, lane);
         }
         const uint64_t cm = __ballot(live && cand);
         if (live && cand) q[qn + (int)__popcll(cm & lt)] = (uint32_t)i;
@@ -773,44 +867,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
         // instance (the common case: instances are contiguous between
         // population controls) sums in registers first
         double mv[kImN];
-        const double w = live ? (double)o.p.w : 0.0;
-        const double ep = (double)o.e_post;
-        const double ws = live && o.born ? (double)o.s.w : 0.0;
-        const double es = (double)o.e_sec;
-        const double wc = live ? (double)cur[k].w : 0.0;
-        mv[kImW] = w + ws;
-        mv[kImWE] = w * ep + ws * es;
-        mv[kImWVX] = w * (double)o.p.vx + ws * (double)o.s.vx;
-        mv[kImEl] = o.chan == kChanElastic ? wc : 0.0;
-        mv[kImExc] = o.chan == kChanExc ? wc : 0.0;
-        mv[kImIon] = o.chan == kChanIon ? wc : 0.0;
-        mv[kImBirth] = live ? (double)o.born : 0.0;
-        mv[kImWE2] = w * ep * ep + ws * es * es;
-        const uint64_t lm = __ballot(live);
-        if (lm) {
-          if (!__all(!live || inst == m_inst)) {
-            // the wave leaves m_inst: flush its running sums
-            if (m_inst >= 0) {
-#pragma unroll
-              for (int q = 0; q < kImN; ++q) {
-                const double sq = wave_sum(macc[q]);
-                macc[q] = 0.0;
-                if (lane == 0 && sq != 0.0) atomicAdd(&s_dep[m_inst * kImN + q], sq);
-              }
-            }
-            const int inst0 = __shfl(inst, (int)__builtin_ctzll(lm), kWave);
-            m_inst = __all(!live || inst == inst0) ? inst0 : -1;
-          }
-          if (m_inst >= 0) {
-#pragma unroll
-            for (int q = 0; q < kImN; ++q) macc[q] += mv[q];
-          } else if (live) {
-            // lanes of several instances (a boundary or the births)
-#pragma unroll
-            for (int q = 0; q < kImN; ++q)
-              if (mv[q] != 0.0) atomicAdd(&s_dep[inst * kImN + q], mv[q]);
-          }
-        }
+        swarm_moments<T>(o, cur[k].w, live, mv);
+        swarm_accumulate(mv, live, inst, m_inst, macc, s_dep, lane);
       }
       const bool born = live && o.born;
       const uint64_t m = __ballot(born);

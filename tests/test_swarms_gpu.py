@@ -25,15 +25,19 @@ def _batch(prec, **kw):
     return b
 
 
+@pytest.mark.parametrize("queue", [0, 1], ids=["auto", "queued"])
 @pytest.mark.parametrize("prec", [8, 4])
-def test_batched_step_matches_oracle_per_instance(gpu, prec):
+def test_batched_step_matches_oracle_per_instance(gpu, prec, queue):
     """One launch over every instance against the oracle run instance by
     instance.  The step works in place: slot i keeps its particle (x holds
     the instance number, w is kept) and the secondaries are appended after
     the slots in use in parent order, each carrying its parent's instance.
     The Philox stream is keyed by the slot, so the oracle of instance k gets
-    the uniforms of k's slots."""
+    the uniforms of k's slots.  `queued` forces the candidate queue (the
+    batch is too small for the push to choose it)."""
+    from boltzmann_amd import _lib
     b = _batch(prec)
+    _lib.check(b.lib.bz_set_push_queue(b.ctx, queue))
     dtype = np.float64 if prec == 8 else np.float32
     st = b.state()
     tab = (b.table.energy, b.table.sig, b.table.slope)
