@@ -585,9 +585,18 @@ __device__ __forceinline__ StepOut<T> flight_particle(const P5<T> in, uint32_t i
 }
 
 // Node of the deposit: fine bin int((x+dx/2)/(L+dx) * M), then fine -> node
-// by integer division by bnf (kernels-new.cu:63 and :8-50).
+// by integer division by bnf (kernels-new.cu:63 and :8-50).  The quotient
+// is formed with the reciprocal of L+dx: it and the correctly rounded
+// quotient differ by a few ulp, so away from an integer (more than 1e-12
+// relative) both truncate to the same bin; next to one the division runs.
 __device__ __forceinline__ int dep_node(double x, const bz_params_t& p) {
-  const int b = (int)(((x + p.half_dx) / p.L_dx) * p.dep_M);
+  const double xh = x + p.half_dx;
+  const double yf = (xh * (1.0 / p.L_dx)) * p.dep_M;
+  const double fy = floor(yf);
+  const double margin = 1e-12 * yf;
+  int b;
+  if (yf >= 0.0 && yf - fy > margin && fy + 1.0 - yf > margin) b = (int)fy;
+  else b = (int)((xh / p.L_dx) * p.dep_M);
   // k = b / bnf without the integer divide: the FP32 quotient is within one
   // of it for |b| < 2^24 and one step corrects it; for b < 0 either rounding
   // ends at node 0 after the clamp
