@@ -642,173 +642,173 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   // has been written.
   constexpr int PD = DIM == 1 ? 1 : 0;  // physics dimension
   if constexpr (kQueue) {
-   if (queued) {
-    uint32_t* q = s_q + wid * kQCap;
-    const uint64_t lt = (1ull << lane) - 1ull;
-    const T e_bot = ((const T*)xs.energy)[0], e_top = ((const T*)xs.energy)[xs.n - 1];
-    int qn = 0;                 // queued slots (wave-uniform)
-    uint32_t t_open = t_first;  // wave-uniform
-    uint32_t c_open = 0;
-    auto advance_to = [&](uint32_t t_new) {
-      if (lane == 0) {
-        wave_births[(size_t)t_open * NW + wid] = c_open;
-        for (uint32_t tt = t_open + 1; tt < t_new; ++tt) wave_births[(size_t)tt * NW + wid] = 0u;
-      }
-      t_open = t_new;
-      c_open = 0;
-    };
-    auto run_batch = [&](int nb) {
-      const bool act = lane < nb;
-      const uint32_t i = act ? q[lane] : 0u;
-      P5<T> c;
-      load5<T, true>(in, (int64_t)i, act ? n : 0, c);
-      int inst = 0;
-      T e_local = (T)0;
-      if (DIM == 2) {
-        inst = (int)c.x;
-        inst = inst < 0 ? 0 : (inst >= n_inst ? n_inst - 1 : inst);
-        e_local = s_ie[inst];
-      }
-      const StepOut<T> o = step_particle<T, XS, PD, RNG>(c, i, step, p, xs, E, R, G, e_local);
-      if (act) {
-        ++c_live;
-        c_el += o.n_el;
-        c_ex += o.n_exc;
-        c_io += o.chan == kChanIon;
-        c_null += o.n_null;
-        c_abs += !o.alive;
-        c_over += o.over;
-        if (DIM == 2) {
-          // velocities only; this batch's instances are mixed: per-lane sums
-          st[1][i] = o.p.vx;
-          st[2][i] = o.p.vy;
-          st[3][i] = o.p.vz;
-          double mv[kImN];
-          swarm_moments<T>(o, c.w, true, mv);
-#pragma unroll
-          for (int qq = 0; qq < kImN; ++qq)
-            if (mv[qq] != 0.0) atomicAdd(&s_dep[inst * kImN + qq], mv[qq]);
-        } else if (o.alive) {
-          st[0][i] = o.p.x;
-          st[1][i] = o.p.vx;
-          st[2][i] = o.p.vy;
-          st[3][i] = o.p.vz;
-          const double w = (double)o.p.w;
-          const int nd = dep_node((double)o.p.x, p);
-          atomicAdd(&s_dep[nd], w);
-          if (o.chan == kChanIon) atomicAdd(&s_dep[nn + nd], w);
-          atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
-        } else {
-          st[4][i] = (T)0;
+    if (queued) {
+      uint32_t* q = s_q + wid * kQCap;
+      const uint64_t lt = (1ull << lane) - 1ull;
+      const T e_bot = ((const T*)xs.energy)[0], e_top = ((const T*)xs.energy)[xs.n - 1];
+      int qn = 0;                 // queued slots (wave-uniform)
+      uint32_t t_open = t_first;  // wave-uniform
+      uint32_t c_open = 0;
+      auto advance_to = [&](uint32_t t_new) {
+        if (lane == 0) {
+          wave_births[(size_t)t_open * NW + wid] = c_open;
+          for (uint32_t tt = t_open + 1; tt < t_new; ++tt) wave_births[(size_t)tt * NW + wid] = 0u;
         }
-      }
-      // births: the batch's slots rise, so equal tiles are runs of lanes
-      // (segments), taken in order
-      const bool born = act && o.born;
-      const uint64_t m = __ballot(born);
-      const uint32_t te = act ? i / (uint32_t)TILE : 0xFFFFFFFFu;
-      const uint32_t tprev = __shfl_up(te, 1, kWave);
-      uint64_t heads = __ballot(act && (lane == 0 || te != tprev));
-      while (heads) {
-        const int s0 = (int)__builtin_ctzll(heads);
-        heads &= heads - 1ull;
-        const int s1 = heads ? (int)__builtin_ctzll(heads) : 64;
-        const uint64_t below_s1 = s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull);
-        const uint64_t seg = ~((1ull << s0) - 1ull) & below_s1;
-        const uint32_t ts = __shfl(te, s0, kWave);
-        if (ts != t_open) advance_to(ts);
-        if (born && ((seg >> lane) & 1ull)) {
-          const uint32_t r = c_open + (uint32_t)__popcll(m & lt & seg);
-          store5<T, true>(a.out, (int64_t)ts * TILE + wid * WSPAN + r, o.s);
-          if (DIM == 1) {
-            const double w = (double)o.s.w;
-            const int nd = dep_node((double)o.s.x, p);
-            atomicAdd(&s_dep[nd], w);
-            atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
-          }
-        }
-        c_open += (uint32_t)__popcll(m & seg);
-      }
-      w_births += (uint32_t)__popcll(m);
-    };
-    for (uint32_t t = t_first; t < t_end; ++t) {
-      const int64_t bw = (int64_t)t * TILE + wid * WSPAN;
-      P5<T> nxt[ITEMS];
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k)
-        load5<T, true>(in, bw + TILE + k * kWave + lane, t + 1 < t_end ? n : 0, nxt[k]);
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        const int64_t i = bw + k * kWave + lane;
-        const bool live = i < n && cur[k].w > (T)0;
-        bool cand = false;
-        if (live) {
-          const T s2 = cur[k].vx * cur[k].vx + cur[k].vy * cur[k].vy + cur[k].vz * cur[k].vz;
-          const T e = (T)p.kev * s2;
-          if (!(e < e_top)) {
-            cand = true;  // beyond the table: the speed is not bounded
-          } else {
-            const T ec = e < e_bot ? e_bot : e;
-            int b = xs_bucket<T>(ec, xs.hint_bits) - xs.hint_base;
-            b = b < 0 ? 0 : (b >= xs.n_hint ? xs.n_hint - 1 : b);
-            Uniforms<T> U;
-            U.first(p.seed, (uint32_t)i, step);
-            cand = U.u[0] < s_pup[b];
-          }
-        }
-        const bool plain = live && !cand;
+        t_open = t_new;
+        c_open = 0;
+      };
+      auto run_batch = [&](int nb) {
+        const bool act = lane < nb;
+        const uint32_t i = act ? q[lane] : 0u;
+        P5<T> c;
+        load5<T, true>(in, (int64_t)i, act ? n : 0, c);
         int inst = 0;
         T e_local = (T)0;
         if (DIM == 2) {
-          inst = (int)cur[k].x;
+          inst = (int)c.x;
           inst = inst < 0 ? 0 : (inst >= n_inst ? n_inst - 1 : inst);
           e_local = s_ie[inst];
         }
-        StepOut<T> o;
-        o.born = 0;
-        o.chan = kChanNone;
-        o.e_sec = (T)0;
-        o.s = cur[k];
-        push_tail<T, PD>(cur[k], cur[k].vx, cur[k].vy, cur[k].vz, p, E, e_local, o);
-        if (plain) {
+        const StepOut<T> o = step_particle<T, XS, PD, RNG>(c, i, step, p, xs, E, R, G, e_local);
+        if (act) {
           ++c_live;
-          ++c_null;
+          c_el += o.n_el;
+          c_ex += o.n_exc;
+          c_io += o.chan == kChanIon;
+          c_null += o.n_null;
+          c_abs += !o.alive;
+          c_over += o.over;
           if (DIM == 2) {
-            st[1][i] = o.p.vx;  // only vx moves without a collision
+            // velocities only; this batch's instances are mixed: per-lane sums
+            st[1][i] = o.p.vx;
+            st[2][i] = o.p.vy;
+            st[3][i] = o.p.vz;
+            double mv[kImN];
+            swarm_moments<T>(o, c.w, true, mv);
+  #pragma unroll
+            for (int qq = 0; qq < kImN; ++qq)
+              if (mv[qq] != 0.0) atomicAdd(&s_dep[inst * kImN + qq], mv[qq]);
           } else if (o.alive) {
             st[0][i] = o.p.x;
             st[1][i] = o.p.vx;
+            st[2][i] = o.p.vy;
+            st[3][i] = o.p.vz;
             const double w = (double)o.p.w;
             const int nd = dep_node((double)o.p.x, p);
             atomicAdd(&s_dep[nd], w);
+            if (o.chan == kChanIon) atomicAdd(&s_dep[nn + nd], w);
             atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
           } else {
-            ++c_abs;
             st[4][i] = (T)0;
           }
         }
-        if (DIM == 2) {
-          // running moments of the wave's instance (the queued candidates
-          // add theirs in their batch)
-          double mv[kImN];
-          swarm_moments<T>(o, cur[k].w, plain, mv);
-          swarm_accumulate(mv, plain, inst, m_inst, macc, s_dep, lane);
+        // births: the batch's slots rise, so equal tiles are runs of lanes
+        // (segments), taken in order
+        const bool born = act && o.born;
+        const uint64_t m = __ballot(born);
+        const uint32_t te = act ? i / (uint32_t)TILE : 0xFFFFFFFFu;
+        const uint32_t tprev = __shfl_up(te, 1, kWave);
+        uint64_t heads = __ballot(act && (lane == 0 || te != tprev));
+        while (heads) {
+          const int s0 = (int)__builtin_ctzll(heads);
+          heads &= heads - 1ull;
+          const int s1 = heads ? (int)__builtin_ctzll(heads) : 64;
+          const uint64_t below_s1 = s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull);
+          const uint64_t seg = ~((1ull << s0) - 1ull) & below_s1;
+          const uint32_t ts = __shfl(te, s0, kWave);
+          if (ts != t_open) advance_to(ts);
+          if (born && ((seg >> lane) & 1ull)) {
+            const uint32_t r = c_open + (uint32_t)__popcll(m & lt & seg);
+            store5<T, true>(a.out, (int64_t)ts * TILE + wid * WSPAN + r, o.s);
+            if (DIM == 1) {
+              const double w = (double)o.s.w;
+              const int nd = dep_node((double)o.s.x, p);
+              atomicAdd(&s_dep[nd], w);
+              atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+            }
+          }
+          c_open += (uint32_t)__popcll(m & seg);
         }
-        const uint64_t cm = __ballot(live && cand);
-        if (live && cand) q[qn + (int)__popcll(cm & lt)] = (uint32_t)i;
-        qn += (int)__popcll(cm);
-        if (qn >= 64) {
-          run_batch(64);
-          if (lane < qn - 64) q[lane] = q[64 + lane];
-          qn -= 64;
+        w_births += (uint32_t)__popcll(m);
+      };
+      for (uint32_t t = t_first; t < t_end; ++t) {
+        const int64_t bw = (int64_t)t * TILE + wid * WSPAN;
+        P5<T> nxt[ITEMS];
+  #pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+          load5<T, true>(in, bw + TILE + k * kWave + lane, t + 1 < t_end ? n : 0, nxt[k]);
+  #pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+          const int64_t i = bw + k * kWave + lane;
+          const bool live = i < n && cur[k].w > (T)0;
+          bool cand = false;
+          if (live) {
+            const T s2 = cur[k].vx * cur[k].vx + cur[k].vy * cur[k].vy + cur[k].vz * cur[k].vz;
+            const T e = (T)p.kev * s2;
+            if (!(e < e_top)) {
+              cand = true;  // beyond the table: the speed is not bounded
+            } else {
+              const T ec = e < e_bot ? e_bot : e;
+              int b = xs_bucket<T>(ec, xs.hint_bits) - xs.hint_base;
+              b = b < 0 ? 0 : (b >= xs.n_hint ? xs.n_hint - 1 : b);
+              Uniforms<T> U;
+              U.first(p.seed, (uint32_t)i, step);
+              cand = U.u[0] < s_pup[b];
+            }
+          }
+          const bool plain = live && !cand;
+          int inst = 0;
+          T e_local = (T)0;
+          if (DIM == 2) {
+            inst = (int)cur[k].x;
+            inst = inst < 0 ? 0 : (inst >= n_inst ? n_inst - 1 : inst);
+            e_local = s_ie[inst];
+          }
+          StepOut<T> o;
+          o.born = 0;
+          o.chan = kChanNone;
+          o.e_sec = (T)0;
+          o.s = cur[k];
+          push_tail<T, PD>(cur[k], cur[k].vx, cur[k].vy, cur[k].vz, p, E, e_local, o);
+          if (plain) {
+            ++c_live;
+            ++c_null;
+            if (DIM == 2) {
+              st[1][i] = o.p.vx;  // only vx moves without a collision
+            } else if (o.alive) {
+              st[0][i] = o.p.x;
+              st[1][i] = o.p.vx;
+              const double w = (double)o.p.w;
+              const int nd = dep_node((double)o.p.x, p);
+              atomicAdd(&s_dep[nd], w);
+              atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
+            } else {
+              ++c_abs;
+              st[4][i] = (T)0;
+            }
+          }
+          if (DIM == 2) {
+            // running moments of the wave's instance (the queued candidates
+            // add theirs in their batch)
+            double mv[kImN];
+            swarm_moments<T>(o, cur[k].w, plain, mv);
+            swarm_accumulate(mv, plain, inst, m_inst, macc, s_dep, lane);
+          }
+          const uint64_t cm = __ballot(live && cand);
+          if (live && cand) q[qn + (int)__popcll(cm & lt)] = (uint32_t)i;
+          qn += (int)__popcll(cm);
+          if (qn >= 64) {
+            run_batch(64);
+            if (lane < qn - 64) q[lane] = q[64 + lane];
+            qn -= 64;
+          }
         }
+  #pragma unroll
+        for (int k = 0; k < ITEMS; ++k) cur[k] = nxt[k];
       }
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) cur[k] = nxt[k];
+      if (qn > 0) run_batch(qn);
+      if (t_first < t_end) advance_to(t_end);  // the last tile's count, zeros after it
     }
-    if (qn > 0) run_batch(qn);
-    if (t_first < t_end) advance_to(t_end);  // the last tile's count, zeros after it
-   }
   }
   if (!queued)
   for (uint32_t t = t_first; t < t_end; ++t) {
@@ -1162,7 +1162,8 @@ static int red_geometry(const LaunchEnv& env, int nblocks, int m, int* rpc_out, 
   if (need > rpc * env.dep_l2_doubles) rpc = (need + env.dep_l2_doubles - 1) / env.dep_l2_doubles;
   if (rpc > 65535) return fail("deposit reduction: too many partial rows");
   const int ny = (int)((nblocks + rpc - 1) / rpc);
-  if ((int64_t)ny * m > env.dep_l2_doubles && ny > 1) return fail("deposit reduction: level-2 buffer");
+  if ((int64_t)ny * m > env.dep_l2_doubles && ny > 1)
+    return fail("deposit reduction: level-2 buffer");
   const int nx = (m + 63) / 64;
   if (nx > env.dep_cols) return fail("deposit reduction: %d output columns > %d", nx, env.dep_cols);
   *rpc_out = (int)rpc;
@@ -1262,7 +1263,8 @@ static int launch_push_inplace_q(const LaunchEnv& env, PushArgs& a, PushLaunch* 
 template <typename T, int XS, int DIM, int RNG, int BLOCK, int ITEMS>
 static int launch_push_inplace_t(const LaunchEnv& env, PushArgs& a, PushLaunch* pl) {
   if constexpr (QueuedPush<XS, DIM, RNG>::value) {
-    const int64_t walk = env.capacity / ((int64_t)BLOCK * env.num_cu);  // items per wave, 1 block/CU
+    // tile items per wave of a full state at one block per CU
+    const int64_t walk = env.capacity / ((int64_t)BLOCK * env.num_cu);
     const bool want = env.push_queue > 0 || (env.push_queue == 0 && walk >= kQueueMinItems);
     if (want && !(a.p.gas_sigma > 0.0)) {
       a.queue_mode = env.push_queue > 0 ? 1 : 0;
