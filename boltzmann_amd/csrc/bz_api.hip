@@ -47,7 +47,22 @@ struct bz_ctx {
   LaunchEnv env;
   Clock* host_clock;  // pinned mirror
   void* graphs;       // captured graphs (GraphSlots), created on first capture
+  // state (its x array) whose binning keys the last 1-D push wrote into the
+  // sort input, with the binning and energy factor they were made with; any
+  // other call that changes a state or the sort workspace clears it
+  const void* keys_state;
+  BinKeys keys_bk;
+  double keys_kev;
 };
+
+static bool same_bins(const BinKeys& a, const BinKeys& b) {
+  if (a.dx != b.dx || a.nb != b.nb || a.n_nodes != b.n_nodes || a.n_keys != b.n_keys ||
+      a.dim != b.dim)
+    return false;
+  for (int k = 0; k < 8; ++k)
+    if (a.thr[k] != b.thr[k]) return false;
+  return true;
+}
 
 static void graph_release(bz_ctx* c);
 
@@ -102,6 +117,7 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   e.push_shape = 0;
   e.push_bpc = 0;
   e.push_queue = 0;
+  e.push_keys_nb = 0;
   // smallest push tile is 256 particles
   // look-back tile words, or the birth counts of the in-place 1-D push (u32:
   // 4096 per block, then one per tile and wave: at most capacity/64 + 64)
@@ -185,6 +201,17 @@ int bz_set_push_grid(bz_ctx_t c, int blocks_per_cu) {
   return 0;
 }
 
+int bz_set_push_keys(bz_ctx_t c, int32_t nb, const double* thresholds_host) {
+  if (!c) return fail("null context");
+  if (nb < 0 || nb > 8) return fail("push keys: nb must be in [0,8]");
+  if (nb > 1 && !thresholds_host) return fail("push keys: nb > 1 needs thresholds");
+  c->env.push_keys_nb = nb;
+  for (int k = 0; k < 8; ++k)
+    c->env.push_keys_thr[k] = (thresholds_host && k < nb - 1) ? thresholds_host[k] : 0.0;
+  c->keys_state = nullptr;
+  return 0;
+}
+
 int bz_set_push_queue(bz_ctx_t c, int mode) {
   if (!c) return fail("null context");
   if (mode < -1 || mode > 1) return fail("push queue mode must be -1, 0 or 1");
@@ -256,6 +283,7 @@ int bz_synchronize(bz_ctx_t c) {
 int bz_clock_set(bz_ctx_t c, int64_t n, uint64_t step, double t) {
   BZ_CTX(c);
   if (n < 0 || n > c->env.capacity) return fail("count %lld outside [0, capacity]", (long long)n);
+  c->keys_state = nullptr;
   BZ_HIP(hipStreamSynchronize(c->env.stream));
   BZ_HIP(hipMemcpy(c->host_clock, c->env.clock, sizeof(Clock), hipMemcpyDeviceToHost));
   c->host_clock->n = n;
@@ -344,7 +372,14 @@ int bz_push_mcc(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const v
     if (!R) return fail("stream mode needs the uniform array R");
     if (p->r_stride < 1) return fail("stream mode needs r_stride >= 1");
   }
-  return launch_push(c->env, in, out, E, p, xs, R, gas_v, moments, c->env.push_shape);
+  c->keys_state = nullptr;
+  rc = launch_push(c->env, in, out, E, p, xs, R, gas_v, moments, c->env.push_shape);
+  if (!rc && p->dim == 1 && c->env.push_keys_nb > 0) {
+    c->keys_state = in->x;
+    c->keys_bk = bin_keys(p, c->env.push_keys_thr, c->env.push_keys_nb);
+    c->keys_kev = p->kev;
+  }
+  return rc;
 }
 
 int bz_deposit(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p, double* moments) {
@@ -359,6 +394,7 @@ int bz_compact(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out) {
   BZ_CTX(c);
   if (!in || !out) return fail("null state");
   if (in->w == out->w) return fail("bz_compact needs two distinct state buffers");
+  c->keys_state = nullptr;
   return launch_compact(c->env, in, out);
 }
 
@@ -432,7 +468,12 @@ int bz_bin_sort(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
   if (n_keys > c->env.max_keys) return fail("n_keys %lld exceeds context max_keys", (long long)n_keys);
   if (n_keys > 65534) return fail("n_keys must be <= 65534 (two 8-bit radix passes)");
   if (!s || !counts || !offsets || !perm) return fail("null bin/sort pointer");
-  return launch_bin_sort(c->env, s, p, thresholds_host, nb, counts, offsets, perm);
+  // the keys the last push wrote serve when they are this state's, made
+  // with this binning
+  const bool ready = c->keys_state && c->keys_state == s->x && c->keys_kev == p->kev &&
+                     same_bins(c->keys_bk, bin_keys(p, thresholds_host, nb));
+  c->keys_state = nullptr;  // the sort overwrites its input keys
+  return launch_bin_sort(c->env, s, p, thresholds_host, nb, counts, offsets, perm, ready);
 }
 
 int bz_resample(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const bz_params_t* p,
@@ -445,6 +486,7 @@ int bz_resample(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const b
     return fail("null resample pointer");
   if (in->x == out->x) return fail("in and out state must not alias");
   if (n_keys < 1 || n_keys > c->env.max_keys) return fail("bad n_keys");
+  c->keys_state = nullptr;
   return launch_resample(c->env, in, out, p, counts, offsets, perm, targets, n_keys, pol);
 }
 
@@ -499,6 +541,7 @@ int bz_swarm_step(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const
   if (!in || !out || !xs || !inst_E || !inst_moments) return fail("swarm step: null pointer");
   if (accumulate && !inst_acc) return fail("swarm step: accumulate needs inst_acc");
   if (in->vx == out->vx) return fail("in and out state must not alias");
+  c->keys_state = nullptr;
   return launch_swarm_step(c->env, in, out, p, xs, n_inst, inst_off, inst_E, inst_moments,
                            inst_acc, accumulate);
 }
@@ -512,6 +555,7 @@ int bz_swarm_control(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out,
   if (!in || !out || !p || !inst_off || !targets || !counts_ws || !offsets_ws || !pol)
     return fail("swarm control: null pointer");
   if (in->vx == out->vx) return fail("in and out state must not alias");
+  c->keys_state = nullptr;
   return launch_swarm_control(c->env, in, out, p, n_inst, inst_off, targets, counts_ws,
                               offsets_ws, pol);
 }
@@ -604,6 +648,7 @@ int bz_graph_begin(bz_ctx_t c) {
   GraphSlots* g = slots_of(c);
   if (!g) return fail("too many contexts with graphs on this thread");
   if (g->capturing) return fail("capture already in progress");
+  c->keys_state = nullptr;  // keys are reused within one capture only
   g->saved_sched = c->env.push_sched;
   c->env.push_sched = 1;  // cooperative launches are not captured: ticket schedule
   g->saved_timing = c->env.timing ? c->env.timing->on : 0;
@@ -620,6 +665,7 @@ int bz_graph_end(bz_ctx_t c, int32_t* id) {
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(c->env.stream, &graph);
   g->capturing = 0;
+  c->keys_state = nullptr;
   c->env.push_sched = g->saved_sched;
   if (c->env.timing) c->env.timing->on = g->saved_timing;
   if (e != hipSuccess) return fail("hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -645,6 +691,7 @@ int bz_graph_launch(bz_ctx_t c, int32_t id) {
   BZ_CTX(c);
   GraphSlots* g = slots_of(c);
   if (!g || id < 0 || id >= kMaxGraphs || !g->used[id]) return fail("bad graph id");
+  c->keys_state = nullptr;  // a replay may change states and the sort input
   BZ_HIP(hipGraphLaunch(g->exec[id], c->env.stream));
   return 0;
 }

@@ -213,6 +213,30 @@ __device__ __forceinline__ uint32_t lookback_block(uint64_t* status, uint32_t t,
   return excl;
 }
 
+// Binning key of one live particle: cell = int((x + dx/2)/dx) clamped to the
+// nodes (1-D; 0 otherwise), energy bin = number of thresholds <= e, key =
+// cell * nb + bin (the key of bz_bin_sort).
+struct BinKeys {
+  double dx;
+  double thr[8];
+  int nb;
+  int n_nodes;
+  int n_keys;  // the key of a dead slot
+  int dim;
+};
+
+template <typename T>
+__device__ __forceinline__ uint32_t bin_key(const BinKeys& k, T x, T e) {
+  int cell = 0;
+  if (k.dim == 1) {
+    cell = (int)(((double)x + 0.5 * k.dx) / k.dx);
+    cell = cell < 0 ? 0 : (cell >= k.n_nodes ? k.n_nodes - 1 : cell);
+  }
+  int b = 0;
+  for (int q = 0; q < k.nb - 1; ++q) b += (e >= (T)k.thr[q]) ? 1 : 0;
+  return (uint32_t)(cell * k.nb + b);
+}
+
 // Device clock kept inside the context.
 struct Clock {
   int64_t n;         // live particles (current state buffer)

@@ -33,6 +33,8 @@ struct LaunchEnv {
   int push_sched;            // 0 static tiles + cooperative launch, 1 ticket counter
   int push_bpc;              // 1-D push: blocks per CU (0: as many as fit)
   int push_queue;            // 1-D push candidate queue: 0 auto, 1 always, -1 never
+  int push_keys_nb;          // 1-D push writes binning keys for nb bins (0: off)
+  double push_keys_thr[8];   // their energy thresholds
   // optional kernel timing: event pairs per timed kernel class
   struct Timing* timing;
 };
@@ -80,7 +82,7 @@ int launch_bin_count(const LaunchEnv& env, const bz_state_t* s, const bz_params_
                      const double* thr, int nb, int32_t* counts);
 int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                     const double* thr, int nb, int32_t* counts, int32_t* offsets,
-                    uint32_t* perm);
+                    uint32_t* perm, bool keys_ready = false);
 // batched swarms: stable sort of the slots by instance (held in x)
 int launch_group_sort(const LaunchEnv& env, const bz_state_t* s, int n_inst, int32_t* counts,
                       int32_t* offsets, uint32_t* perm, const uint32_t** skeys);
@@ -114,6 +116,8 @@ int launch_mlp_grad(const LaunchEnv& env, const bz_mlp_t* net, const float* x, c
 int launch_mlp_adam(const LaunchEnv& env, const bz_adam_t* opt, float* master, float* m, float* v,
                     const float* grad, uint16_t* wbf);
 int launch_mlp_round(const LaunchEnv& env, const float* master, uint16_t* wbf);
+void* sort_input_keys(const LaunchEnv& env);     // key per slot, input of the sort
+BinKeys bin_keys(const bz_params_t* p, const double* thr, int nb);
 void* sort_scratch_keys(const LaunchEnv& env);   // keys-per-position buffer
 void* sort_scratch_perm(const LaunchEnv& env);   // position -> index buffer
 }  // namespace bz

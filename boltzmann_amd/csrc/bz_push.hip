@@ -48,6 +48,10 @@ struct PushArgs {
   int n_inst;
   uint32_t* block_births;  // in-place mode: births of each block
   int queue_mode;          // 1-D push candidate queue: 1 always, 0 where walks are long
+  // 1-D push: binning key of every slot it leaves (NULL: none), the input
+  // of the next bin sort
+  uint32_t* keys;
+  BinKeys bk;
 };
 
 // per-block counter row of the push (u32): births, absorbed, elastic,
@@ -701,6 +705,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           } else {
             st[4][i] = (T)0;
           }
+          if (DIM == 1 && a.keys)
+            a.keys[i] = o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post) : (uint32_t)a.bk.n_keys;
         }
         // births: the batch's slots rise, so equal tiles are runs of lanes
         // (segments), taken in order
@@ -787,6 +793,10 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
               st[4][i] = (T)0;
             }
           }
+          // keys of the slots settled here (candidates get theirs in their batch)
+          if (DIM == 1 && a.keys && i < n && !(live && cand))
+            a.keys[i] = plain && o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post)
+                                         : (uint32_t)a.bk.n_keys;
           if (DIM == 2) {
             // running moments of the wave's instance (the queued candidates
             // add theirs in their batch)
@@ -862,6 +872,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           st[4][i] = (T)0;
         }
       }
+      if (DIM == 1 && a.keys && i < n)
+        a.keys[i] = live && o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post) : (uint32_t)a.bk.n_keys;
       if (DIM == 2) {
         // per-instance moments of this step; a wave whose lanes share one
         // instance (the common case: instances are contiguous between
@@ -970,8 +982,16 @@ __device__ void births_part(const PushArgs& a, int b, int nblk, int tile, int nw
       for (uint32_t j = 0; j < c; ++j) {
         const int64_t dst = dst0 + j;
         if (dst >= a.capacity) break;
+        T v[5];
 #pragma unroll
-        for (int f = 0; f < 5; ++f) st[f][dst] = sg[f][src0 + j];
+        for (int f = 0; f < 5; ++f) {
+          v[f] = sg[f][src0 + j];
+          st[f][dst] = v[f];
+        }
+        if (a.keys) {
+          const T e = (T)a.p.kev * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+          a.keys[dst] = bin_key<T>(a.bk, v[0], e);
+        }
       }
       run += tot;
     }
@@ -1413,6 +1433,11 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
   a.inst_off = nullptr;
   a.inst_E = nullptr;
   a.n_inst = 0;
+  a.keys = nullptr;
+  if (p->dim == 1 && env.push_keys_nb > 0) {
+    a.keys = (uint32_t*)sort_input_keys(env);
+    a.bk = bin_keys(p, env.push_keys_thr, env.push_keys_nb);
+  }
   PushLaunch pl;
   const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &pl)
                                          : launch_push_prec<float>(env, a, shape, &pl);
@@ -1605,6 +1630,7 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
   a.inst_off = nullptr;
   a.inst_E = inst_E;
   a.n_inst = n_inst;
+  a.keys = nullptr;
   PushLaunch pl;
   const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &pl)
                                          : launch_push_prec<float>(env, a, shape, &pl);

@@ -85,13 +85,8 @@ struct KeyArgs {
   const void* vy;
   const void* vz;
   const void* w;
-  double dx;
   double kev;
-  double thr[8];
-  int nb;
-  int n_nodes;
-  int dim;
-  int n_keys;  // live keys; dead particles get key n_keys
+  BinKeys bk;  // live keys 0..n_keys-1; dead particles get key n_keys
 };
 
 // Key per slot.  Each thread takes kKeyItems slots, all loads issued
@@ -103,16 +98,9 @@ constexpr int kKeyItems = 4;
 // key = cell * nb + energy bin of one slot; n_keys for a dead slot (w <= 0)
 template <typename T>
 __device__ __forceinline__ uint32_t slot_key(const KeyArgs& a, T x, T vx, T vy, T vz, T w) {
-  if (!(w > (T)0)) return (uint32_t)a.n_keys;
-  int cell = 0;
-  if (a.dim == 1) {
-    cell = (int)(((double)x + 0.5 * a.dx) / a.dx);
-    cell = cell < 0 ? 0 : (cell >= a.n_nodes ? a.n_nodes - 1 : cell);
-  }
+  if (!(w > (T)0)) return (uint32_t)a.bk.n_keys;
   const T e = (T)a.kev * (vx * vx + vy * vy + vz * vz);
-  int b = 0;
-  for (int q = 0; q < a.nb - 1; ++q) b += (e >= (T)a.thr[q]) ? 1 : 0;
-  return (uint32_t)(cell * a.nb + b);
+  return bin_key<T>(a.bk, x, e);
 }
 
 template <typename T>
@@ -132,7 +120,7 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(KeyArgs a, const Clock* clk
     const int64_t i = base + k * kWave + lane;
     const bool v = i < n;
     qw[k] = v ? w[i] : (T)0;
-    qx[k] = v && a.dim == 1 ? x[i] : (T)0;
+    qx[k] = v && a.bk.dim == 1 ? x[i] : (T)0;
     qa[k] = v ? vx[i] : (T)0;
     qb[k] = v ? vy[i] : (T)0;
     qc[k] = v ? vz[i] : (T)0;
@@ -168,7 +156,7 @@ __global__ __launch_bounds__(256) void k_key_hist(KeyArgs a, const Clock* clk, i
     for (int k = 0; k < kKeyItems; ++k) {
       const int64_t i = base + k * 256;
       const bool v = i < n;
-      q[k][0] = v && a.dim == 1 ? x[i] : (T)0;
+      q[k][0] = v && a.bk.dim == 1 ? x[i] : (T)0;
       q[k][1] = v ? vx[i] : (T)0;
       q[k][2] = v ? vy[i] : (T)0;
       q[k][3] = v ? vz[i] : (T)0;
@@ -466,28 +454,34 @@ static int sort_keys(const LaunchEnv& env, const SortWs& ws, int n_keys, int32_t
   return check_launch("key counts");
 }
 
-static KeyArgs key_args(const bz_state_t* s, const bz_params_t* p, const double* thr, int nb) {
+BinKeys bin_keys(const bz_params_t* p, const double* thr, int nb) {
+  BinKeys bk;
   const int n_nodes = p->dim == 1 ? p->n_nodes : 1;
+  bk.dx = p->dx;
+  for (int k = 0; k < 8; ++k) bk.thr[k] = (thr && k < nb - 1) ? thr[k] : 0.0;
+  bk.nb = nb;
+  bk.n_nodes = n_nodes;
+  bk.n_keys = n_nodes * nb;
+  bk.dim = p->dim;
+  return bk;
+}
+
+static KeyArgs key_args(const bz_state_t* s, const bz_params_t* p, const double* thr, int nb) {
   KeyArgs ka;
   ka.x = s->x;
   ka.vx = s->vx;
   ka.vy = s->vy;
   ka.vz = s->vz;
   ka.w = s->w;
-  ka.dx = p->dx;
   ka.kev = p->kev;
-  for (int k = 0; k < 8; ++k) ka.thr[k] = (thr && k < nb - 1) ? thr[k] : 0.0;
-  ka.nb = nb;
-  ka.n_nodes = n_nodes;
-  ka.dim = p->dim;
-  ka.n_keys = n_nodes * nb;
+  ka.bk = bin_keys(p, thr, nb);
   return ka;
 }
 
 int launch_bin_count(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                      const double* thr, int nb, int32_t* counts) {
   const KeyArgs ka = key_args(s, p, thr, nb);
-  const int m = ka.n_keys + 1;
+  const int m = ka.bk.n_keys + 1;
   // an LDS histogram per block while it fits in 64 KiB, global atomics above
   const int lds_keys = m <= 16384 ? m : 0;
   hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, counts, (int64_t)m);
@@ -504,7 +498,7 @@ int launch_bin_count(const LaunchEnv& env, const bz_state_t* s, const bz_params_
 
 int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                     const double* thr, int nb, int32_t* counts, int32_t* offsets,
-                    uint32_t* perm) {
+                    uint32_t* perm, bool keys_ready) {
   const int n_nodes = p->dim == 1 ? p->n_nodes : 1;
   const int n_keys = n_nodes * nb;
   if (n_keys + 1 > 65535) return fail("bin_sort: at most 65534 keys (two 8-bit passes)");
@@ -512,13 +506,18 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
   SortWs ws = carve(env.sort_ws, cap, env.max_keys);
   const KeyArgs ka = key_args(s, p, thr, nb);
   timing_begin(env, 1);
-  auto kk = env.precision == BZ_F64 ? k_keys<double> : k_keys<float>;
-  // one thread per kKeyItems slots of the capacity; blocks past the count exit
-  const int64_t kgrid = (cap + (int64_t)kSortBlock * kKeyItems - 1) / ((int64_t)kSortBlock * kKeyItems);
-  hipLaunchKernelGGL(kk, dim3((unsigned)(kgrid > 0 ? kgrid : 1)), dim3(kSortBlock), 0, env.stream,
-                     ka, (const Clock*)env.clock, ws.keys_a);
-  int rc = check_launch("k_keys");
-  if (rc) return rc;
+  int rc = 0;
+  if (!keys_ready) {
+    // (otherwise the push that left this state wrote the keys already)
+    auto kk = env.precision == BZ_F64 ? k_keys<double> : k_keys<float>;
+    // one thread per kKeyItems slots of the capacity; blocks past the count exit
+    const int64_t kgrid =
+        (cap + (int64_t)kSortBlock * kKeyItems - 1) / ((int64_t)kSortBlock * kKeyItems);
+    hipLaunchKernelGGL(kk, dim3((unsigned)(kgrid > 0 ? kgrid : 1)), dim3(kSortBlock), 0,
+                       env.stream, ka, (const Clock*)env.clock, ws.keys_a);
+    rc = check_launch("k_keys");
+    if (rc) return rc;
+  }
   rc = sort_keys(env, ws, n_keys, counts, offsets, perm, nullptr);
   timing_end(env, 1);
   return rc;
@@ -1080,6 +1079,10 @@ int launch_resample_grouped(const LaunchEnv& env, const bz_state_t* in, const bz
                                      pol, skeys, plan_out);
   return launch_resample_t<float>(env, in, out, p, counts, offsets, perm, targets, n_keys, pol,
                                   skeys, plan_out);
+}
+
+void* sort_input_keys(const LaunchEnv& env) {
+  return carve(env.sort_ws, env.capacity, env.max_keys).keys_a;
 }
 
 void* sort_scratch_keys(const LaunchEnv& env) {

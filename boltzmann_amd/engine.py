@@ -49,7 +49,7 @@ class KineticEngine:
 
     def __init__(self, cfg: ph.RunConfig, device: int = 0, capacity: Optional[int] = None,
                  rng_mode: int = _lib.BZ_RNG_PHILOX, max_keys: Optional[int] = None,
-                 compact_every: int = 64):
+                 compact_every: int = 64, push_keys: Optional[bool] = None):
         if not torch.cuda.is_available():
             raise _lib.BzError("no GPU visible: the electron step runs only in the HIP kernels")
         self.lib = _lib.load()
@@ -116,6 +116,16 @@ class KineticEngine:
         self.thr = (C.c_double * 8)(*thr)
         self.policy = _lib.ResamplePolicy(cfg.split_below, cfg.merge_at, cfg.merge_keep,
                                           cfg.trigger_at, cfg.min_wt, cfg.split_to, 0, 0, 0, 0)
+        # the 1-D push writes the binning keys of the next sort (4 bytes per
+        # slot) when a sort follows every push: resampling every step with no
+        # trigger to decline it
+        if push_keys is None:
+            always = not self.policy.ref_trigger and self.policy.trigger_at <= 0
+            push_keys = (cfg.dim == 1 and cfg.target_ppc > 0 and cfg.resample_every == 1
+                         and always)
+        self.push_keys = bool(push_keys) and cfg.dim == 1
+        _lib.check(self.lib.bz_set_push_keys(self.ctx, self.nb if self.push_keys else 0,
+                                             self.thr), "bz_set_push_keys")
 
     # ------------------------------------------------------------------ state
     @property

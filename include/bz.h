@@ -196,6 +196,17 @@ int bz_set_push_grid(bz_ctx_t ctx, int blocks_per_cu);
  * with the queue is launched when the capacity allows that many), 1: always,
  * -1: never.  No counterpart in the reference. */
 int bz_set_push_queue(bz_ctx_t ctx, int mode);
+/* Binning keys written by the 1-D push: with nb in [1,8] (nb-1 thresholds,
+ * host pointer) every 1-D bz_push_mcc also writes the bz_bin_sort key of
+ * each slot it leaves (holes and births included), and the next bz_bin_sort
+ * of that state with the same nb, thresholds, dx and energy factor sorts
+ * those keys instead of recomputing them from the state.  Any other call
+ * that changes a state or the sort workspace (clock set, compact,
+ * resample, swarm calls, graph capture or replay) drops them.  A caller
+ * that writes the state arrays itself between the push and the sort calls
+ * this function again first.  nb = 0 turns it off (default).  No
+ * counterpart in the reference. */
+int bz_set_push_keys(bz_ctx_t ctx, int32_t nb, const double* thresholds);
 /* Tile schedule of the fused push: 0 static tile order under a cooperative
  * launch (default), 1 tiles claimed from a ticket counter by running blocks.
  * No counterpart in the reference. */
