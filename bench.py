@@ -8,15 +8,19 @@ discharges (c1, c3, c4) that is the fused collision + push + deposit kernel
 (in place, k_push_inplace), the births append, the deposit reduction, the device
 field step (moment update, fluid ions, Poisson) and, on the steps the
 workload asks for it, binning + merge/split resampling or the periodic
-compaction.  For the single swarm (c0) the push compacts.  For the batched
-swarms (c2) it is one in-place launch over all 256 instances, replayed from
-a captured hipGraph with population control every 50 steps.  The state is
-synthetic (Maxwellian, uniform density) and resident in HBM before timing.
+compaction.  The discharges replay the pushes and field steps between two
+resampling checks from captured hipGraphs (the run is the same step for
+step; --no-graphs launches every step directly).  For the single swarm (c0)
+the push compacts.  For the batched swarms (c2) it is one in-place launch
+over all 256 instances, replayed from a captured hipGraph with population
+control every 50 steps.  The state is synthetic (Maxwellian, uniform
+density) and resident in HBM before timing.
 
 Prints one JSON line.  The roofline object is for the dominant kernel
 (k_push_inplace; k_push for the single 0-D swarm), timed with HIP events on the
-library's stream; its `traffic` is the HBM bytes per launch measured with
-rocprofv3 (profiles/traffic.json).  The CPU baseline is the oracle port
+library's stream over directly launched steps right before the timed region
+(graph replays record no events); its `traffic` is the HBM bytes per launch
+measured with rocprofv3 (profiles/traffic.json).  The CPU baseline is the oracle port
 timed on a bounded sample of the same workload on the host.
 """
 from __future__ import annotations
@@ -192,24 +196,30 @@ def bench_engine(args):
         from boltzmann_amd.autotune import tune_push
         tuned = tune_push(eng, steps=10)
         index += tuned["steps_used"]
-    for _ in range(args.warmup):
-        eng.step(index)
-        index += 1
+    graphs = not args.no_graphs
+    index = eng.run(index, args.warmup, graphs=graphs)
     torch.cuda.synchronize()
-    c0 = eng.counters()
+    # kernel times from directly launched steps with the library's HIP events
+    # on its stream (graph replays record none)
+    n_timed = max(10, min(50, args.steps // 4))
+    ct0 = eng.counters()
     _lib.check(eng.lib.bz_timing_enable(eng.ctx, 1))
+    index = eng.run(index, n_timed, graphs=False)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.step(index)
-        index += 1
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
     ms = (C.c_double * 3)()
     nl = (C.c_int64 * 3)()
     _lib.check(eng.lib.bz_timing_get(eng.ctx, ms, nl))
+    _lib.check(eng.lib.bz_timing_enable(eng.ctx, 0))
+    ct1 = eng.counters()
+    c0 = eng.counters()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    index = eng.run(index, args.steps, graphs=graphs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     c1 = eng.counters()
     psteps = c1["particle_steps"] - c0["particle_steps"]
+    psteps_timed = ct1["particle_steps"] - ct0["particle_steps"]
     elapsed = t1 - t0
     word = 8 if cfg.precision == _lib.BZ_F64 else 4
     # read + write of the carried components (0-D runs carry no position)
@@ -230,13 +240,15 @@ def bench_engine(args):
         "data": "synthetic (Maxwellian electrons, uniform density, tabulated argon-like sigma)",
         "config": {"workload": cfg.name, "particles_start": cfg.n_particles,
                    "particles_end": eng.count(), "cells": cfg.n_cells if cfg.dim == 1 else 0,
-                   "dt_s": cfg.dt, "parallelism": "single GPU", "push_schedule": args.sched},
-        "roofline": _roofline(bytes_per_pstep, psteps / args.steps, avg_push_s,
+                   "dt_s": cfg.dt, "parallelism": "single GPU", "push_schedule": args.sched,
+                   "graphs": graphs and cfg.dim == 1 and cfg.resample_every != 1},
+        "roofline": _roofline(bytes_per_pstep, psteps_timed / n_timed, avg_push_s,
                               "k_push_inplace" if cfg.dim == 1 else "k_push", cfg.name),
         "counters": {k: c1[k] - c0[k] for k in ("births", "absorbed", "ionization", "overflow",
                                                 "resample", "split", "merge")},
         "kernel_ms": {"push": ms[0], "sort": ms[1], "resample": ms[2],
-                      "launches": [int(nl[0]), int(nl[1]), int(nl[2])]},
+                      "launches": [int(nl[0]), int(nl[1]), int(nl[2])],
+                      "over_steps": n_timed},
         "autotune": tuned,
     }
     if not args.no_cpu:
@@ -439,6 +451,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every step directly instead of replaying captured blocks")
     ap.add_argument("--shape", type=int, default=-1, help="push tile shape (-1: autotune)")
     ap.add_argument("--sched", type=int, default=0,
                     help="push tile schedule: 0 static + cooperative launch, 1 ticket counter")

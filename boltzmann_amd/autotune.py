@@ -86,7 +86,9 @@ def tune_push(engine, steps: int = 10, start_index: int = 0) -> dict:
         engine.step(index[0])
         index[0] += 1
 
-    return tune_shapes(engine.lib, engine.ctx, step, steps, grids=engine.cfg.dim == 1)
+    out = tune_shapes(engine.lib, engine.ctx, step, steps, grids=engine.cfg.dim == 1)
+    engine.drop_graphs()  # blocks captured before hold the old launch shape
+    return out
 
 
 def tune_swarms(batch, steps: int = 10) -> dict:

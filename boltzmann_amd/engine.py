@@ -71,6 +71,9 @@ class KineticEngine:
         self.inplace = cfg.dim == 1
         self.compact_every = int(compact_every)
         self._since_compact = 0
+        # captured blocks of steps: (state buffer, steps) -> graph id
+        self._graphs = {}
+        self._cap_stream = None
         # collision model
         self.table = None
         self.xs = None
@@ -283,6 +286,72 @@ class KineticEngine:
         self.field_step(advance=True)
         if self.inplace and self.compact_every and self._since_compact >= self.compact_every:
             self.compact()
+
+    def run(self, index: int, n_steps: int, graphs: bool = True) -> int:
+        """n_steps steps from step `index`: the sequence step(index),
+        step(index + 1), ... makes, with the pushes and field steps between
+        two resampling checks or compactions replayed from a captured
+        hipGraph (one per block length and state buffer), so the host
+        enqueues one launch per block instead of two per step.  Returns the
+        next step index."""
+        k = self.cfg.resample_every if self.cfg.target_ppc > 0 else 0
+        # a check every step leaves one push per block: launched directly,
+        # which also keeps the keys that push writes for the next sort
+        if not graphs or not self.inplace or k == 1:
+            for _ in range(n_steps):
+                self.step(index)
+                index += 1
+            return index
+        end = index + n_steps
+        while index < end:
+            if k and index % k == 0:
+                self.resample()
+            m = end - index
+            if k:
+                m = min(m, k - index % k)
+            if self.compact_every:
+                m = max(1, min(m, self.compact_every - self._since_compact))
+            self._replay(m)
+            index += m
+            if self.compact_every and self._since_compact >= self.compact_every:
+                self.compact()
+        return index
+
+    def _replay(self, m: int):
+        key = (self.cur, m)
+        gid = self._graphs.get(key)
+        if gid is None:
+            gid = self._capture(m)
+            self._graphs[key] = gid
+        _lib.check(self.lib.bz_graph_launch(self.ctx, gid), "bz_graph_launch")
+        self._since_compact += m
+
+    def _capture(self, m: int) -> int:
+        """Capture m pushes + field steps on a side stream (capture needs a
+        created stream); the graph is replayed on the engine's stream."""
+        if self._cap_stream is None:
+            self._cap_stream = torch.cuda.Stream(self.dev)
+        since = self._since_compact
+        _lib.check(self.lib.bz_set_stream(self.ctx, C.c_void_p(self._cap_stream.cuda_stream)))
+        gid = C.c_int32(-1)
+        try:
+            _lib.check(self.lib.bz_graph_begin(self.ctx), "bz_graph_begin")
+            try:
+                for _ in range(m):
+                    self.push()
+                    self.field_step(advance=True)
+            finally:
+                _lib.check(self.lib.bz_graph_end(self.ctx, C.byref(gid)), "bz_graph_end")
+        finally:
+            self._since_compact = since
+            _lib.check(self.lib.bz_set_stream(self.ctx, C.c_void_p(self.stream.cuda_stream)))
+        return gid.value
+
+    def drop_graphs(self):
+        """Forget the captured blocks (after a change of the push knobs)."""
+        for gid in self._graphs.values():
+            _lib.check(self.lib.bz_graph_destroy(self.ctx, gid), "bz_graph_destroy")
+        self._graphs = {}
 
     def synchronize(self):
         _lib.check(self.lib.bz_synchronize(self.ctx))
