@@ -224,6 +224,8 @@ def bench_engine(args):
     word = 8 if cfg.precision == _lib.BZ_F64 else 4
     # read + write of the carried components (0-D runs carry no position)
     bytes_per_pstep = 2 * (5 if cfg.dim == 1 else 4) * word
+    if eng.push_keys:
+        bytes_per_pstep += 4  # the binning key of the next sort, written by the push
     avg_push_s = ms[0] / 1e3 / max(int(nl[0]), 1)
     out = {
         "metric": "particle-steps/s on 1 MI355X",

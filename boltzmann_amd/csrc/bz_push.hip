@@ -48,6 +48,7 @@ struct PushArgs {
   int n_inst;
   uint32_t* block_births;  // in-place mode: births of each block
   int queue_mode;          // 1-D push candidate queue: 1 always, 0 where walks are long
+  int e_lds_off;           // 1-D in-place push: LDS byte offset of the staged field (0: global)
   // 1-D push: binning key of every slot it leaves (NULL: none), the input
   // of the next bin sort
   uint32_t* keys;
@@ -565,6 +566,23 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   const int sums = DIM == 1 ? 3 * nn : n_inst * kImN;
   T* s_pup = nullptr;       // kQueue: collision-probability bound per hint bucket
   uint32_t* s_q = nullptr;  // kQueue: candidate slots, kQCap per wave
+  // The slot count first, then the first tile's particles: their round
+  // trips overlap the staging of the sums, the table and the field below.
+  const int64_t n = a.clk->n;
+  const uint64_t step = a.clk->step;
+  const uint32_t ntiles = (uint32_t)((n + TILE - 1) / TILE);
+  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const uint32_t t_first = (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)ntiles);
+  const uint32_t t_end = t_first + per < ntiles ? t_first + per : ntiles;
+  const T* in[5] = {(const T*)a.in[0], (const T*)a.in[1], (const T*)a.in[2], (const T*)a.in[3],
+                    (const T*)a.in[4]};
+  P5<T> cur[ITEMS];
+  {
+    const int64_t b0 = (int64_t)t_first * TILE + wid * WSPAN;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+      load5<T, true>(in, b0 + k * kWave + lane, t_first < t_end ? n : 0, cur[k]);
+  }
   for (int j = tid; j < sums; j += BLOCK) s_dep[j] = 0.0;
   T* s_ie = (T*)(s_dep + sums);  // DIM 2: field of every instance
   if (DIM == 2)
@@ -589,20 +607,20 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
       s_q = (uint32_t*)((char*)s_pup + (((size_t)a.xs.n_hint * sizeof(T) + 15) & ~(size_t)15));
     }
   }
+  // 1-D: the field of every cell in LDS (where it fits), so the kick reads
+  // no global memory
+  const T* s_E = nullptr;
+  if (DIM == 1 && a.e_lds_off) {
+    T* e = (T*)((char*)s_dep + a.e_lds_off);
+    stage_lds<T, BLOCK>(e, (const T*)a.E, p.n_cells);
+    s_E = e;
+  }
 
-  const int64_t n = a.clk->n;
-  const uint64_t step = a.clk->step;
   // the births stage reads the count from n_prev: the slot count itself is
   // committed while it runs
   if (blockIdx.x == 0 && tid == 0) a.clk->n_prev = n;
-  const uint32_t ntiles = (uint32_t)((n + TILE - 1) / TILE);
-  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
-  const uint32_t t_first = (uint32_t)min((uint64_t)blockIdx.x * per, (uint64_t)ntiles);
-  const uint32_t t_end = t_first + per < ntiles ? t_first + per : ntiles;
-  const T* in[5] = {(const T*)a.in[0], (const T*)a.in[1], (const T*)a.in[2], (const T*)a.in[3],
-                    (const T*)a.in[4]};
   T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
-  const T* __restrict__ E = (const T*)a.E;
+  const T* E = s_E ? s_E : (const T*)a.E;
   const T* R = (const T*)a.R;
   const T* G = (const T*)a.G;
   uint32_t* wave_births = a.block_births + kMaxPushBlocks;
@@ -627,13 +645,6 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
     __syncthreads();
   }
 
-  P5<T> cur[ITEMS];
-  {
-    const int64_t b0 = (int64_t)t_first * TILE + wid * WSPAN;
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k)
-      load5<T, true>(in, b0 + k * kWave + lane, t_first < t_end ? n : 0, cur[k]);
-  }
   // Queued 1-D path (cold gas).  A live particle whose uniform R0 is at or
   // above its energy bucket's bound cannot collide: it is pushed right away
   // (the null branch of step_particle).  The others are queued per wave in
@@ -1246,15 +1257,30 @@ static int launch_push_inplace_q(const LaunchEnv& env, PushArgs& a, PushLaunch* 
   if (QUEUE && QueuedPush<XS, DIM, RNG>::value)
     smem += queue_lds_bytes(a.xs.n_hint, (int)sizeof(T), BLOCK / kWave);
   if (smem > 160 * 1024) return fail("push: sums + table need %zu B of LDS (> 160 KiB)", smem);
-  if (smem > 64 * 1024) {
+  // 1-D: the field staged after the rest, where it fits without costing blocks
+  const size_t smem_e = smem + (DIM == 1 ? (((size_t)a.p.n_cells * sizeof(T) + 15) & ~(size_t)15)
+                                         : (size_t)0);
+  const size_t smem_max = DIM == 1 && smem_e <= 160 * 1024 ? smem_e : smem;
+  if (smem_max > 64 * 1024) {
     if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem) != hipSuccess)
+                            (int)smem_max) != hipSuccess)
       return fail("hipFuncSetAttribute(dynamic LDS) failed");
   }
   int occ = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, smem) != hipSuccess ||
       occ < 1)
     return fail("push kernel does not fit on a CU (LDS %zu B)", smem);
+  a.e_lds_off = 0;
+  if (smem_max > smem) {
+    int occ_e = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_e, kern, BLOCK, smem_max) ==
+            hipSuccess &&
+        (occ_e > 8 ? 8 : occ_e) >= (occ > 8 ? 8 : occ)) {
+      a.e_lds_off = (int)smem;  // 16-byte aligned: every part above is
+      smem = smem_max;
+      occ = occ_e;
+    }
+  }
   int bpc = occ > 8 ? 8 : occ;
   if (env.push_bpc > 0 && env.push_bpc < bpc) bpc = env.push_bpc;
   int grid = env.num_cu * bpc;
@@ -1434,6 +1460,7 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
   a.inst_E = nullptr;
   a.n_inst = 0;
   a.keys = nullptr;
+  a.e_lds_off = 0;
   if (p->dim == 1 && env.push_keys_nb > 0) {
     a.keys = (uint32_t*)sort_input_keys(env);
     a.bk = bin_keys(p, env.push_keys_thr, env.push_keys_nb);
@@ -1631,6 +1658,7 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
   a.inst_E = inst_E;
   a.n_inst = n_inst;
   a.keys = nullptr;
+  a.e_lds_off = 0;
   PushLaunch pl;
   const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &pl)
                                          : launch_push_prec<float>(env, a, shape, &pl);
