@@ -347,9 +347,13 @@ __global__ __launch_bounds__(1024) void k_chunk_apply(uint32_t* data, int64_t m,
   }
 }
 
-// Stable scatter of one tile: items are visited in index order (rounds of
-// 256, one per thread); equal digits keep their order through a wave-level
-// match (8 ballots) and per-wave counts combined in wave order.
+// Stable scatter of one tile.  Wave w owns the tile's w-th quarter
+// (kSortItems rows of 64 consecutive items), so the stable order is wave,
+// then row, then lane.  Pass 1, no barrier: each wave ranks its rows with a
+// wave-level match (8 ballots) against its own running digit counts in LDS
+// and keeps every item's rank within the wave.  One barrier, then each digit's
+// counts are turned into exclusive offsets across the waves; one more, and
+// every item goes to tile base + wave offset + rank.
 __global__ __launch_bounds__(kSortBlock) void k_downsweep(const uint32_t* keys_in,
                                                           const uint32_t* vals_in,
                                                           uint32_t* keys_out, uint32_t* vals_out,
@@ -357,23 +361,29 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(const uint32_t* keys_i
                                                           const uint32_t* hist, int64_t tiles,
                                                           int identity_vals) {
   constexpr int NW = kSortBlock / kWave;
-  __shared__ uint32_t s_run[256];
+  static_assert(kSortBlock == 256, "one digit per thread in the offset pass");
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_wc[NW][256];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int64_t n = clk->n;
   const int64_t t = blockIdx.x;
-  s_run[tid] = 0;
   s_base[tid] = hist[(int64_t)tid * tiles + t];
-  for (int w = 0; w < NW; ++w) s_wc[w][tid] = 0;
-  __syncthreads();
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int j = 0; j < 256 / kWave; ++j) s_wc[wid][j * kWave + lane] = 0u;
+  const int64_t w0 = t * kSortTile + (int64_t)wid * (kSortItems * kWave);
+  uint32_t key[kSortItems], val[kSortItems], pre[kSortItems];
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = t * kSortTile + r * kSortBlock + tid;
+    const int64_t i = w0 + r * kWave + lane;
     const bool valid = i < n;
-    const uint32_t key = valid ? keys_in[i] : 0u;
-    const uint32_t val = valid ? (identity_vals ? (uint32_t)i : vals_in[i]) : 0u;
-    const uint32_t d = (key >> shift) & 255u;
+    key[r] = valid ? keys_in[i] : 0u;
+    val[r] = valid ? (identity_vals ? (uint32_t)i : vals_in[i]) : 0u;
+  }
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const bool valid = w0 + r * kWave + lane < n;
+    const uint32_t d = (key[r] >> shift) & 255u;
     uint64_t m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -381,26 +391,29 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(const uint32_t* keys_i
       m &= ((d >> b) & 1u) ? bb : ~bb;
     }
     const uint32_t rank = (uint32_t)__popcll(m & lt);
-    const uint32_t cnt = (uint32_t)__popcll(m);
-    const bool leader = valid && rank == 0;
-    if (leader) s_wc[wid][d] = cnt;
-    __syncthreads();
-    if (valid) {
-      uint32_t off = s_base[d] + s_run[d] + rank;
-      for (int w = 0; w < wid; ++w) off += s_wc[w][d];
-      keys_out[off] = key;
-      vals_out[off] = val;
+    const uint32_t before = s_wc[wid][d];  // this wave's items of digit d in earlier rows
+    pre[r] = before + rank;
+    if (valid && rank == 0) s_wc[wid][d] = before + (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  {
+    uint32_t acc = 0u;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t c = s_wc[w][tid];
+      s_wc[w][tid] = acc;
+      acc += c;
     }
-    __syncthreads();
-    {
-      uint32_t s = 0;
-      for (int w = 0; w < NW; ++w) {
-        s += s_wc[w][tid];
-        s_wc[w][tid] = 0;
-      }
-      s_run[tid] += s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (w0 + r * kWave + lane < n) {
+      const uint32_t d = (key[r] >> shift) & 255u;
+      const uint32_t off = s_base[d] + s_wc[wid][d] + pre[r];
+      keys_out[off] = key[r];
+      vals_out[off] = val[r];
     }
-    __syncthreads();
   }
 }
 
