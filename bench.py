@@ -355,10 +355,14 @@ def bench_closure(args):
     from boltzmann_amd.closure_run import ClosureDischarge, swarm_training_set, train_closure
     from boltzmann_amd.engine import KineticEngine
 
+    def log(msg):
+        print(f"[c3] {msg}", file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
-    x, t = swarm_training_set(n_samples=args.closure_samples)
+    x, t = swarm_training_set(n_samples=args.closure_samples, log=log)
     t_data = time.perf_counter() - t0
-    net, rep = train_closure(x, t, epochs=args.closure_epochs)
+    net, rep = train_closure(x, t, epochs=args.closure_epochs, log=log)
+    log(f"trained: {rep['holdout_trained']}")
     params = {"master": net.parameters(), "norm": dict(net.norm)}
     net.close()
     cfg = configs.c3()

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 from typing import Optional
 
 import numpy as np
@@ -174,7 +175,7 @@ class ClosureNet:
                 k += 1
             hist.append(float(acc.item()) / steps_per_epoch)
             if log_every and (ep % log_every == 0 or ep == epochs - 1):
-                print(f"epoch {ep}: loss {hist[-1]:.4e}", flush=True)
+                print(f"epoch {ep}: loss {hist[-1]:.4e}", file=sys.stderr, flush=True)
         return hist
 
     def loss(self, x: np.ndarray, t: np.ndarray) -> float:
