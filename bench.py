@@ -341,7 +341,7 @@ def bench_swarms(args):
 def bench_closure(args):
     """Closure-accelerated 1-D run (1024 cells, 4M electrons) against the
     plain run: simulated time per wall-clock hour of each, and the closure's
-    ionization-rate error judged by the kinetic step at every update.  The
+    ionization-rate error checked against the kinetic step at every update.  The
     plain run is the workload as configured (direct collision probability,
     nu dt ~ 0.1).  The closure run's kinetic updates span each interval with
     a few longer free-flight pushes (exact collisions at any push length,
@@ -408,7 +408,7 @@ def bench_closure(args):
     for _ in range(warm // n_sub):
         run.macro_step()
     torch.cuda.synchronize()
-    run.judge.zero_()
+    run.tally.zero_()
     tc = time.perf_counter()
     for _ in range(steps // n_sub):
         run.macro_step()

@@ -159,7 +159,7 @@ class ClosureDischarge:
         self.ksrc = torch.zeros(nn, **f64)
         self.src = torch.zeros(nn, **f64)
         self.acc = torch.zeros(nn, **f64)
-        self.judge = torch.zeros(8, **f64)
+        self.tally = torch.zeros(8, **f64)
         self.resample_every = cfg.resample_every if resample_every is None else resample_every
         self.macro = 0
 
@@ -207,12 +207,12 @@ class ClosureDischarge:
                                               _p(e.ni), _p(e.ns), _p(e.Te), _p(e.g2), _p(e.V),
                                               _p(e.E), _p(e.E_t), _p(e.chol)), "bz_field_substep")
         _lib.check(e.lib.bz_closure_commit(e.ctx, self.cfg.n_nodes, _p(e.moments), _p(self.acc),
-                                           _p(e.ni), _p(e.g2), _p(e.Te), _p(self.judge)),
+                                           _p(e.ni), _p(e.g2), _p(e.Te), _p(self.tally)),
                    "bz_closure_commit")
         self.macro += 1
 
     def report(self) -> dict:
-        j = self.judge.cpu().numpy()
+        j = self.tally.cpu().numpy()
         kin, pred = j[0], j[1]
         return {"kinetic_updates": int(j[3]),
                 "ionization_kinetic": float(kin), "ionization_closure": float(pred),

@@ -436,11 +436,11 @@ int bz_closure_source(bz_ctx_t c, const bz_mlp_t* net, int32_t n_cells, const do
 }
 
 int bz_closure_commit(bz_ctx_t c, int32_t n_nodes, const double* moments, double* acc,
-                      double* ni, double* g2, double* Te, double* judge) {
+                      double* ni, double* g2, double* Te, double* tally) {
   BZ_CTX(c);
   if (n_nodes < 1) return fail("closure commit: need nodes");
-  if (!moments || !acc || !ni || !g2 || !Te || !judge) return fail("closure commit: null pointer");
-  return launch_closure_commit(c->env, n_nodes, moments, acc, ni, g2, Te, judge);
+  if (!moments || !acc || !ni || !g2 || !Te || !tally) return fail("closure commit: null pointer");
+  return launch_closure_commit(c->env, n_nodes, moments, acc, ni, g2, Te, tally);
 }
 
 int bz_bin_count(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,

@@ -362,9 +362,9 @@ int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double
 // Kinetic update of a closure-driven run: the sub-steps since the last update
 // added the learned ionization acc to ni; the kinetic step counted nsrc over
 // the same interval.  Replace one by the other (ni += nsrc - acc, g2 alike),
-// refresh Te = 2/3 nrg/ne, and add the judge's sums:
-//   judge[0] += sum nsrc, judge[1] += sum acc, judge[2] += |sum nsrc - sum acc|,
-//   judge[3] += 1, judge[4] += sum ne, judge[5] += sum nrg.
+// refresh Te = 2/3 nrg/ne, and add the tally's sums:
+//   tally[0] += sum nsrc, tally[1] += sum acc, tally[2] += |sum nsrc - sum acc|,
+//   tally[3] += 1, tally[4] += sum ne, tally[5] += sum nrg.
 // Sums run in a fixed order (deterministic).  acc is cleared.
 namespace bz {
 
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(1024) void k_closure_commit(int nn, const double* _
                                                          double* __restrict__ ni,
                                                          double* __restrict__ g2,
                                                          double* __restrict__ Te,
-                                                         double* __restrict__ judge) {
+                                                         double* __restrict__ tally) {
   __shared__ double s_red[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const double* ne = mom;
@@ -404,19 +404,19 @@ __global__ __launch_bounds__(1024) void k_closure_commit(int nn, const double* _
     double t[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q = 0; q < 4; ++q)
       for (int w = 0; w < 16; ++w) t[q] += s_red[q][w];
-    judge[0] += t[0];
-    judge[1] += t[1];
-    judge[2] += fabs(t[0] - t[1]);
-    judge[3] += 1.0;
-    judge[4] += t[2];
-    judge[5] += t[3];
+    tally[0] += t[0];
+    tally[1] += t[1];
+    tally[2] += fabs(t[0] - t[1]);
+    tally[3] += 1.0;
+    tally[4] += t[2];
+    tally[5] += t[3];
   }
 }
 
 int launch_closure_commit(const LaunchEnv& env, int nn, const double* mom, double* acc, double* ni,
-                          double* g2, double* Te, double* judge) {
+                          double* g2, double* Te, double* tally) {
   hipLaunchKernelGGL(k_closure_commit, dim3(1), dim3(1024), 0, env.stream, nn, mom, acc, ni, g2,
-                     Te, judge);
+                     Te, tally);
   return check_launch("k_closure_commit");
 }
 

@@ -76,7 +76,7 @@ int launch_field(const LaunchEnv& env, const bz_field_params_t* fp, const double
 int launch_closure_source(const LaunchEnv& env, const bz_mlp_t* net, int N, const double* E,
                           const double* mom, double gas_m3, double dt, double* src, double* acc);
 int launch_closure_commit(const LaunchEnv& env, int nn, const double* mom, double* acc, double* ni,
-                          double* g2, double* Te, double* judge);
+                          double* g2, double* Te, double* tally);
 // per-key counts of the binning keys, no sort (resample trigger input)
 int launch_bin_count(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                      const double* thr, int nb, int32_t* counts);

@@ -32,7 +32,7 @@ def _reduction(r: int, p: int, M: int, RM: int, dw, dv, dx, ptcl: int):
 
 def init_reduction(pyk: bool = False):
     """Select the reduction backend (the reference's -p flag picks pykokkos;
-    this build has the HIP backend only)."""
+    this library has the HIP backend only)."""
     global reduction
     reduction = _reduction
     return reduction

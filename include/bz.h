@@ -427,11 +427,11 @@ int bz_field_substep(bz_ctx_t ctx, const bz_field_params_t* fp, double* moments,
 /* Kinetic update of a closure-driven run.  The sub-steps since the last
  * update added the learned ionization acc [N+1] to ni; the kinetic step
  * counted nsrc (moments row 1) over the same interval: ni += nsrc - acc,
- * g2 alike, Te = 2/3 nrg/ne, and the judge's running sums (FP64, fixed
- * order) judge[0..5] += sum nsrc, sum acc, |sum nsrc - sum acc|, 1, sum ne,
+ * g2 alike, Te = 2/3 nrg/ne, and the tally's running sums (FP64, fixed
+ * order) tally[0..5] += sum nsrc, sum acc, |sum nsrc - sum acc|, 1, sum ne,
  * sum nrg.  acc is cleared. */
 int bz_closure_commit(bz_ctx_t ctx, int32_t n_nodes, const double* moments, double* acc,
-                      double* ni, double* g2, double* Te, double* judge);
+                      double* ni, double* g2, double* Te, double* tally);
 
 /* Host helper: Philox4x32-10 block (for tests and stream generation). */
 void bz_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -2,8 +2,8 @@
 
 -ffp-contract=off keeps every expression in plain IEEE source order, which
 is what the device kernels are compiled with as well.  x86-64-v3 keeps the
-library loadable on any current x86 host (the GPU box's CPU may differ from
-the build container's).
+library loadable on any current x86 host (the host that runs it may differ from
+the one that built it).
 """
 from __future__ import annotations
 

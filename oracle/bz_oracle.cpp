@@ -14,7 +14,7 @@
 //   binning                   1bin/kernels/bin_particles.py:7-31 (3bin :39-78, 5bin :86-140)
 //   resampling                1bin/boltzmann.py:799-916 (3bin :813-1113, 5bin :843-1334)
 //
-// plus the definitions this build adds on top of the reference: the Philox
+// plus the definitions this library adds on top of the reference: the Philox
 // stream, the tabulated three-channel null-collision model, the canonical
 // particle order (secondary right after its parent; segments in key order)
 // and the order-independent fixed-point selection of the resampler.  It is

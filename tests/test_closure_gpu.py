@@ -116,15 +116,15 @@ def test_field_substep_and_commit(gpu):
         g = getattr(eng, nm).cpu().numpy()
         np.testing.assert_allclose(g, w, rtol=1e-10, atol=1e-10 * np.abs(w).max(), err_msg=nm)
     np.testing.assert_allclose(eng.moments.cpu().numpy()[:N + 1], m2[:N + 1], rtol=1e-15)
-    # commit: ni += nsrc - acc; judge sums
+    # commit: ni += nsrc - acc; tally sums
     acc = torch.from_numpy(src * 0.5).cuda()
-    judge = torch.zeros(8, dtype=torch.float64, device="cuda")
+    tally = torch.zeros(8, dtype=torch.float64, device="cuda")
     ni1 = eng.ni.cpu().numpy()
     _lib.check(eng.lib.bz_closure_commit(eng.ctx, N + 1, p(eng.moments), p(acc), p(eng.ni),
-                                         p(eng.g2), p(eng.Te), p(judge)))
+                                         p(eng.g2), p(eng.Te), p(tally)))
     nsrc = mom[N + 1:2 * N + 2]
     np.testing.assert_allclose(eng.ni.cpu().numpy(), ni1 + nsrc - 0.5 * src, rtol=1e-13)
-    j = judge.cpu().numpy()
+    j = tally.cpu().numpy()
     assert j[0] == pytest.approx(nsrc.sum(), rel=1e-13)
     assert j[1] == pytest.approx(0.5 * src.sum(), rel=1e-13)
     assert j[2] == pytest.approx(abs(nsrc.sum() - 0.5 * src.sum()), rel=1e-12)
