@@ -30,6 +30,10 @@ struct SortWs {
   int32_t* copies;      // resample: split copies per sorted position
   int32_t* plan;        // resample: [kind, new_count, new_off, hw_count] per key
   uint32_t* sel;        // resample: selected sorted position per new particle
+  // resample: [n_split, n_merge], then the split keys at [2, 2 + kstride)
+  // and the merge keys at [2 + kstride, 2 + 2 kstride)
+  int32_t* klist;
+  int32_t kstride;
   int64_t max_tiles;
   int64_t max_chunks;
 };
@@ -55,6 +59,8 @@ static SortWs carve(void* base, int64_t cap, int32_t max_keys) {
   w.copies = (int32_t*)take(4 * cap);
   w.plan = (int32_t*)take(16 * (size_t)(max_keys + 1));
   w.sel = (uint32_t*)take(4 * cap);
+  w.kstride = max_keys + 1;
+  w.klist = (int32_t*)take(4 * (2 + 2 * (size_t)w.kstride));
   return w;
 }
 
@@ -73,6 +79,7 @@ size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys) {
   add(4 * cap);
   add(16 * (size_t)(max_keys + 1));
   add(4 * cap);
+  add(4 * (2 + 2 * ((size_t)max_keys + 1)));
   return b + 1024;
 }
 
@@ -623,8 +630,10 @@ __global__ __launch_bounds__(256) void k_res_hw(ResArgs a, const int32_t* counts
 // Phase B (single block): kind and new count per key, trigger, offsets.
 __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* counts,
                                                    const int32_t* targets, int32_t* plan,
-                                                   int nb, Clock* clk) {
+                                                   int nb, Clock* clk, int32_t* klist,
+                                                   int kstride) {
   __shared__ int s_fire;
+  __shared__ int s_nlist[2];  // split and merge keys listed so far
   __shared__ int s_w[16];
   __shared__ int s_carry;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -632,6 +641,8 @@ __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* cou
   if (tid == 0) {
     s_fire = (!P.ref_trigger && P.trigger_at <= 0.0) ? 1 : 0;  // <= 0: every check fires
     s_carry = 0;
+    s_nlist[0] = 0;
+    s_nlist[1] = 0;
   }
   __syncthreads();
   for (int k = tid; k < a.n_keys; k += 1024) {
@@ -661,6 +672,7 @@ __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* cou
             kind = kSplit;
             nc = c + draws;
             ++n_split;
+            klist[2 + atomicAdd(&s_nlist[0], 1)] = k;
           }
         } else if ((double)c >= P.merge_at * (double)t && keep >= 1) {
           const int ce = (P.max_ppc > 0 && c > P.max_ppc) ? P.max_ppc : c;
@@ -668,6 +680,7 @@ __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* cou
             kind = kMerge;
             nc = keep + (c - ce);
             ++n_merge;
+            klist[2 + kstride + atomicAdd(&s_nlist[1], 1)] = k;
           }
         }
       }
@@ -697,6 +710,8 @@ __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* cou
   if (tid == 0) {
     clk->n_next = s_carry;
     if (fire) atomicAdd(&clk->cnt[BZ_CNT_RESAMPLE], 1ull);
+    klist[0] = s_nlist[0];
+    klist[1] = s_nlist[1];
   }
 }
 
@@ -826,12 +841,10 @@ __device__ __forceinline__ double res_uniform(uint64_t seed, uint64_t step, uint
 constexpr int kSplitLds = 3968;  // 3968 * 16 B stays under the 64 KiB default LDS limit
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* counts,
-                                                   const int32_t* offsets, const uint32_t* perm,
-                                                   const int32_t* targets, const int32_t* plan,
-                                                   SortWs ws, const Clock* clk, int64_t cap) {
-  const int k = blockIdx.x;
-  if (plan[4 * k] != kSplit) return;
+__device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int32_t* counts,
+                                              const int32_t* offsets, const uint32_t* perm,
+                                              const int32_t* plan, const SortWs& ws,
+                                              const Clock* clk, int64_t cap) {
   extern __shared__ int64_t s_split[];  // cdf[kSplitLds], copies/offs as int32 after
   __shared__ double s_red[4];
   __shared__ int64_t s_w64[4];
@@ -910,12 +923,10 @@ __global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* cou
 // sampling of the weight CDF, each with weight W/K; velocities rescaled about
 // the mean so that weight, momentum and energy are conserved.
 template <typename T>
-__global__ __launch_bounds__(256) void k_res_merge(ResArgs a, const int32_t* counts,
-                                                   const int32_t* offsets, const uint32_t* perm,
-                                                   const int32_t* targets, const int32_t* plan,
-                                                   SortWs ws, const Clock* clk, int64_t cap) {
-  const int k = blockIdx.x;
-  if (plan[4 * k] != kMerge) return;
+__device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int32_t* counts,
+                                              const int32_t* offsets, const uint32_t* perm,
+                                              const int32_t* targets, const int32_t* plan,
+                                              const SortWs& ws, const Clock* clk, int64_t cap) {
   __shared__ double s_red[4];
   __shared__ int64_t s_w64[4];
   const int c = counts[k];
@@ -1016,6 +1027,34 @@ __global__ __launch_bounds__(256) void k_res_merge(ResArgs a, const int32_t* cou
   }
 }
 
+// Split / merge over the keys k_res_plan listed: a grid sized to what is
+// resident walks the list, so the many keys that only copy cost no block
+// (the split's 62 KiB of LDS lets two blocks share a CU).
+template <typename T>
+__global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* counts,
+                                                   const int32_t* offsets, const uint32_t* perm,
+                                                   const int32_t* plan, SortWs ws,
+                                                   const Clock* clk, int64_t cap) {
+  const int n_list = ws.klist[0];
+  for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
+    __syncthreads();  // the previous key is done with the LDS
+    res_split_key<T>(ws.klist[2 + li], a, counts, offsets, perm, plan, ws, clk, cap);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_res_merge(ResArgs a, const int32_t* counts,
+                                                   const int32_t* offsets, const uint32_t* perm,
+                                                   const int32_t* targets, const int32_t* plan,
+                                                   SortWs ws, const Clock* clk, int64_t cap) {
+  const int n_list = ws.klist[1];
+  for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
+    __syncthreads();
+    res_merge_key<T>(ws.klist[2 + ws.kstride + li], a, counts, offsets, perm, targets, plan, ws,
+                     clk, cap);
+  }
+}
+
 __global__ void k_res_commit(Clock* clk, int64_t cap) {
   int64_t n = clk->n_next;
   if (n > cap) {
@@ -1057,15 +1096,18 @@ static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const b
   hipLaunchKernelGGL(k_res_hw<T>, dim3(n_keys), dim3(256), 0, env.stream, a, counts, offsets,
                      perm, targets, ws.plan);
   hipLaunchKernelGGL(k_res_plan, dim3(1), dim3(1024), 0, env.stream, a, counts, targets, ws.plan,
-                     nb, env.clock);
+                     nb, env.clock, ws.klist, ws.kstride);
   const int64_t cgrid = (cap + 256 * kCopyItems - 1) / (256 * kCopyItems);
   hipLaunchKernelGGL(k_res_copy<T>, dim3((unsigned)(cgrid > 0 ? cgrid : 1)), dim3(256), 0,
                      env.stream, a, skeys, offsets, perm, (const int32_t*)ws.plan, counts,
                      (const Clock*)env.clock, cap);
-  hipLaunchKernelGGL(k_res_split<T>, dim3(n_keys), dim3(256), (size_t)kSplitLds * 16,
-                     env.stream, a, counts, offsets,
-                     perm, targets, (const int32_t*)ws.plan, ws, (const Clock*)env.clock, cap);
-  hipLaunchKernelGGL(k_res_merge<T>, dim3(n_keys), dim3(256), 0, env.stream, a, counts, offsets,
+  // at most n_keys listed keys; blocks beyond the list exit at once
+  const int sgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
+  const int mgrid = n_keys < 8 * env.num_cu ? n_keys : 8 * env.num_cu;
+  hipLaunchKernelGGL(k_res_split<T>, dim3(sgrid), dim3(256), (size_t)kSplitLds * 16, env.stream,
+                     a, counts, offsets, perm, (const int32_t*)ws.plan, ws,
+                     (const Clock*)env.clock, cap);
+  hipLaunchKernelGGL(k_res_merge<T>, dim3(mgrid), dim3(256), 0, env.stream, a, counts, offsets,
                      perm, targets, (const int32_t*)ws.plan, ws, (const Clock*)env.clock, cap);
   hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, cap);
   timing_end(env, 2);
