@@ -117,3 +117,42 @@ def test_push_keys_dropped_by_other_calls(gpu):
     cnt = a.counts.cpu().numpy()
     assert cnt[:-1].sum() == 0 and cnt[-1] == a.count()
     a.close()
+
+
+def test_push_keys_with_births_past_capacity(gpu):
+    """Births that do not fit the capacity are dropped and counted; the keys
+    the push wrote still cover exactly the slots kept."""
+    from boltzmann_amd import _lib
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    n = 200_000
+    cfg = ph.RunConfig(name="keys-cap", dim=1, n_particles=n, precision=4,
+                       xs_mode=_lib.BZ_XS_TABLE, nb=3, n_cells=48, length_cm=2.5, dt=2e-11,
+                       gas_density_m3=3.2e24, target_ppc=100, bin_fractions=ph.bin_fractions(3),
+                       bin_thresholds=ph.bin_thresholds(3))
+    cap = n + 64  # far fewer free slots than births
+    a = KineticEngine(cfg, capacity=cap, push_keys=True)
+    b = KineticEngine(cfg, capacity=cap, push_keys=False)
+    st = _state(n, cfg.np_dtype, seed=9, dead_frac=0.0)
+    for e in (a, b):
+        e.load_state(st)
+        e.push()
+    assert a.count() == cap and a.counters()["overflow"] > 0
+    a.bin_sort()
+    b.bin_sort()
+    for g, w in zip(_sorted(a), _sorted(b)):
+        np.testing.assert_array_equal(g, w)
+    a.close()
+    b.close()
+
+
+def test_push_keys_empty_state(gpu):
+    from boltzmann_amd import physics as ph
+    cfg, a = _engine(4, 1, 3, 1000, True, 0)
+    a.load_state(tuple(np.zeros(0, dtype=cfg.np_dtype) for _ in range(5)))
+    a.push()
+    a.bin_sort()
+    assert a.count() == 0
+    assert int(a.counts.cpu().numpy().sum()) == 0
+    a.close()
+    _ = ph
