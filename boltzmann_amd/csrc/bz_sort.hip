@@ -768,16 +768,33 @@ __global__ __launch_bounds__(256) void k_res_copy(ResArgs a, const uint32_t* ske
       for (int f = 0; f < 5; ++f) ((T*)a.out[f])[dst[k]] = v[k][f];
 }
 
-// block-wide helpers for the per-segment kernels (256 threads)
-__device__ __forceinline__ double block_sum_d(double v, double* s_red) {
-  v = wave_sum(v);
+// block-wide helpers for the per-segment kernels (kSegBlock threads)
+constexpr int kSegBlock = 512;
+constexpr int kSegWaves = kSegBlock / 64;
+
+// N sums at once over the block (one set of barriers); s_red holds N * kSegWaves
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double* s_red) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
   __syncthreads();
-  if (lane == 0) s_red[wid] = v;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < N; ++q) s_red[q * kSegWaves + wid] = v[q];
   __syncthreads();
-  const double r = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    double r = s_red[q * kSegWaves];
+    for (int w = 1; w < kSegWaves; ++w) r += s_red[q * kSegWaves + w];
+    v[q] = r;
+  }
   __syncthreads();
-  return r;
+}
+__device__ __forceinline__ double block_sum_d(double v, double* s_red) {
+  double t[1] = {v};
+  block_sum_n<1>(t, s_red);
+  return t[0];
 }
 __device__ __forceinline__ double block_max_d(double v, double* s_red) {
 #pragma unroll
@@ -786,18 +803,19 @@ __device__ __forceinline__ double block_max_d(double v, double* s_red) {
   __syncthreads();
   if (lane == 0) s_red[wid] = v;
   __syncthreads();
-  const double r = fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+  double r = s_red[0];
+  for (int w = 1; w < kSegWaves; ++w) r = fmax(r, s_red[w]);
   __syncthreads();
   return r;
 }
 
 // Inclusive fixed-point prefix of q over sorted positions [s0, s0+c) into cdf.
-// Returns the total.  q(j) is supplied by the functor.
+// Returns the total.  q(j) is supplied by the functor; s_w64 holds kSegWaves.
 template <typename Q>
 __device__ int64_t block_fixed_scan(int64_t s0, int c, int64_t* cdf, Q q, int64_t* s_w64) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int64_t carry = 0;
-  for (int base = 0; base < c; base += 256) {
+  for (int base = 0; base < c; base += kSegBlock) {
     const int j = base + threadIdx.x;
     const int64_t v = j < c ? q(j) : 0;
     const int64_t inc = wave_incl_scan(v);
@@ -807,7 +825,7 @@ __device__ int64_t block_fixed_scan(int64_t s0, int c, int64_t* cdf, Q q, int64_
     int64_t before = carry;
     for (int w = 0; w < wid; ++w) before += s_w64[w];
     if (j < c) cdf[s0 + j] = before + inc;
-    carry += s_w64[0] + s_w64[1] + s_w64[2] + s_w64[3];
+    for (int w = 0; w < kSegWaves; ++w) carry += s_w64[w];
   }
   __syncthreads();
   return carry;
@@ -846,9 +864,9 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
                                               const int32_t* plan, const SortWs& ws,
                                               const Clock* clk, int64_t cap) {
   extern __shared__ int64_t s_split[];  // cdf[kSplitLds], copies/offs as int32 after
-  __shared__ double s_red[4];
-  __shared__ int64_t s_w64[4];
-  __shared__ int s_wi[4];
+  __shared__ double s_red[kSegWaves];
+  __shared__ int64_t s_w64[kSegWaves];
+  __shared__ int s_wi[kSegWaves];
   const int c = counts[k];
   const int64_t s0 = offsets[k];
   const uint64_t step = clk->step;
@@ -860,7 +878,7 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
   const T* Wt = (const T*)a.in[4];
   // fixed-point scale from the largest heavy weight
   double wmax = 0.0;
-  for (int j = threadIdx.x; j < c; j += 256) {
+  for (int j = threadIdx.x; j < c; j += kSegBlock) {
     const double w = (double)Wt[perm[s0 + j]];
     if (w > minw) wmax = fmax(wmax, w);
     copies[j] = 0;
@@ -875,7 +893,7 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
   };
   const int64_t Wq = block_fixed_scan(0, c, cdf, q, s_w64);
   const int draws = plan[4 * k + 1] - c;
-  for (int m = threadIdx.x; m < draws; m += 256) {
+  for (int m = threadIdx.x; m < draws; m += kSegBlock) {
     const double u = res_uniform(a.seed, step, (uint32_t)k, (uint32_t)m);
     const int64_t thr = (int64_t)(u * (double)Wq);
     atomicAdd(&copies[search_cdf(cdf, 0, c, thr, true)], 1);
@@ -884,7 +902,7 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
   // exclusive offsets of (1 + copies) in segment order
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int carry = 0;
-  for (int base = 0; base < c; base += 256) {
+  for (int base = 0; base < c; base += kSegBlock) {
     const int j = base + threadIdx.x;
     const int reps = j < c ? 1 + copies[j] : 0;
     const int inc = wave_incl_scan(reps);
@@ -894,12 +912,12 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
     int before = carry;
     for (int w = 0; w < wid; ++w) before += s_wi[w];
     if (j < c) offs[j] = before + inc - reps;
-    carry += s_wi[0] + s_wi[1] + s_wi[2] + s_wi[3];
+    for (int w = 0; w < kSegWaves; ++w) carry += s_wi[w];
   }
   __syncthreads();
   const int total = carry;  // = c + draws
   const int64_t out0 = plan[4 * k + 2];
-  for (int o = threadIdx.x; o < total; o += 256) {
+  for (int o = threadIdx.x; o < total; o += kSegBlock) {
     const int64_t dst = out0 + o;
     if (dst >= cap) break;
     // largest j with offs[j] <= o
@@ -927,8 +945,8 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
                                               const int32_t* offsets, const uint32_t* perm,
                                               const int32_t* targets, const int32_t* plan,
                                               const SortWs& ws, const Clock* clk, int64_t cap) {
-  __shared__ double s_red[4];
-  __shared__ int64_t s_w64[4];
+  __shared__ double s_red[5 * kSegWaves];
+  __shared__ int64_t s_w64[kSegWaves];
   const int c = counts[k];
   const int ce = (a.pol.max_ppc > 0 && c > a.pol.max_ppc) ? a.pol.max_ppc : c;
   const int K = plan[4 * k + 1] - (c - ce);
@@ -942,7 +960,7 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
   const T* Wt = (const T*)a.in[4];
   // FP64 moments of the old set
   double w0 = 0, px = 0, py = 0, pz = 0, en = 0, wmax = 0;
-  for (int j = threadIdx.x; j < ce; j += 256) {
+  for (int j = threadIdx.x; j < ce; j += kSegBlock) {
     const uint32_t i = perm[s0 + j];
     const double w = (double)Wt[i];
     const double vx = VX[i], vy = VY[i], vz = VZ[i];
@@ -953,14 +971,13 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
     en += w * (vx * vx + vy * vy + vz * vz);
     wmax = fmax(wmax, w);
   }
-  const double W = block_sum_d(w0, s_red);
-  const double Px = block_sum_d(px, s_red), Py = block_sum_d(py, s_red),
-               Pz = block_sum_d(pz, s_red);
-  const double En = block_sum_d(en, s_red);
+  double m5[5] = {w0, px, py, pz, en};
+  block_sum_n<5>(m5, s_red);
+  const double W = m5[0], Px = m5[1], Py = m5[2], Pz = m5[3], En = m5[4];
   wmax = block_max_d(wmax, s_red);
   const double mx = Px / W, my = Py / W, mz = Pz / W;
   double c2 = 0;
-  for (int j = threadIdx.x; j < ce; j += 256) {
+  for (int j = threadIdx.x; j < ce; j += kSegBlock) {
     const uint32_t i = perm[s0 + j];
     const double w = (double)Wt[i];
     const double dx_ = VX[i] - mx, dy_ = VY[i] - my, dz_ = VZ[i] - mz;
@@ -976,7 +993,7 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
   const double wbar = W / (double)K;
   // stratified selection: r_k = (1/K)(U_k + k), searchsorted left
   double sx = 0, sy = 0, sz = 0, se = 0;
-  for (int m = threadIdx.x; m < K; m += 256) {
+  for (int m = threadIdx.x; m < K; m += kSegBlock) {
     const double u = res_uniform(a.seed, step, (uint32_t)k, (uint32_t)m);
     const double r = (1.0 / (double)K) * (u + (double)m);
     const int64_t thr = (int64_t)(r * (double)Wq);
@@ -990,8 +1007,9 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
     sz += vz;
     se += vx * vx + vy * vy + vz * vz;
   }
-  const double Sx = block_sum_d(sx, s_red), Sy = block_sum_d(sy, s_red),
-               Sz = block_sum_d(sz, s_red), Se = block_sum_d(se, s_red);
+  double s4[4] = {sx, sy, sz, se};
+  block_sum_n<4>(s4, s_red);
+  const double Sx = s4[0], Sy = s4[1], Sz = s4[2], Se = s4[3];
   double hx, hy, hz, f;
   if (a.pol.ref_rescale) {
     // boltzmann.py:835-869: hats normalised by the original weight, factor from total energy
@@ -1006,7 +1024,7 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
     hy = Sy / (double)K;
     hz = Sz / (double)K;
     double cn = 0;
-    for (int m = threadIdx.x; m < K; m += 256) {
+    for (int m = threadIdx.x; m < K; m += kSegBlock) {
       if (out0 + m >= cap) break;
       const uint32_t i = perm[ws.sel[out0 + m]];
       const double dx_ = VX[i] - hx, dy_ = VY[i] - hy, dz_ = VZ[i] - hz;
@@ -1015,7 +1033,7 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
     const double C_new = wbar * block_sum_d(cn, s_red);
     f = C_new > 0 ? sqrt(C_old / C_new) : 1.0;
   }
-  for (int m = threadIdx.x; m < K; m += 256) {
+  for (int m = threadIdx.x; m < K; m += kSegBlock) {
     const int64_t dst = out0 + m;
     if (dst >= cap) break;
     const uint32_t i = perm[ws.sel[dst]];
@@ -1031,7 +1049,7 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
 // resident walks the list, so the many keys that only copy cost no block
 // (the split's 62 KiB of LDS lets two blocks share a CU).
 template <typename T>
-__global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* counts,
+__global__ __launch_bounds__(kSegBlock) void k_res_split(ResArgs a, const int32_t* counts,
                                                    const int32_t* offsets, const uint32_t* perm,
                                                    const int32_t* plan, SortWs ws,
                                                    const Clock* clk, int64_t cap) {
@@ -1043,7 +1061,7 @@ __global__ __launch_bounds__(256) void k_res_split(ResArgs a, const int32_t* cou
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_res_merge(ResArgs a, const int32_t* counts,
+__global__ __launch_bounds__(kSegBlock) void k_res_merge(ResArgs a, const int32_t* counts,
                                                    const int32_t* offsets, const uint32_t* perm,
                                                    const int32_t* targets, const int32_t* plan,
                                                    SortWs ws, const Clock* clk, int64_t cap) {
@@ -1103,12 +1121,13 @@ static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const b
                      (const Clock*)env.clock, cap);
   // at most n_keys listed keys; blocks beyond the list exit at once
   const int sgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
-  const int mgrid = n_keys < 8 * env.num_cu ? n_keys : 8 * env.num_cu;
-  hipLaunchKernelGGL(k_res_split<T>, dim3(sgrid), dim3(256), (size_t)kSplitLds * 16, env.stream,
-                     a, counts, offsets, perm, (const int32_t*)ws.plan, ws,
+  const int mgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
+  hipLaunchKernelGGL(k_res_split<T>, dim3(sgrid), dim3(kSegBlock), (size_t)kSplitLds * 16,
+                     env.stream, a, counts, offsets, perm, (const int32_t*)ws.plan, ws,
                      (const Clock*)env.clock, cap);
-  hipLaunchKernelGGL(k_res_merge<T>, dim3(mgrid), dim3(256), 0, env.stream, a, counts, offsets,
-                     perm, targets, (const int32_t*)ws.plan, ws, (const Clock*)env.clock, cap);
+  hipLaunchKernelGGL(k_res_merge<T>, dim3(mgrid), dim3(kSegBlock), 0, env.stream, a, counts,
+                     offsets, perm, targets, (const int32_t*)ws.plan, ws,
+                     (const Clock*)env.clock, cap);
   hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, cap);
   timing_end(env, 2);
   return check_launch("resample");
