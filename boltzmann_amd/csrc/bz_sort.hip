@@ -940,6 +940,8 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
 // Merge: keep K = int(merge_keep*target) particles chosen by stratified
 // sampling of the weight CDF, each with weight W/K; velocities rescaled about
 // the mean so that weight, momentum and energy are conserved.
+constexpr int kMergeLds = 6144;  // 48 KiB of CDF
+
 template <typename T>
 __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int32_t* counts,
                                               const int32_t* offsets, const uint32_t* perm,
@@ -989,7 +991,11 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
   frexp(wmax, &ew);
   const int S = 61 - ew - ceil_log2_i64(ce);
   auto q = [&](int j) -> int64_t { return (int64_t)ldexp((double)Wt[perm[s0 + j]], S); };
-  const int64_t Wq = block_fixed_scan(s0, ce, ws.cdf, q, s_w64);
+  // the CDF in LDS when it fits (the selection's binary searches read it)
+  extern __shared__ int64_t s_mcdf[];
+  const bool lds = ce <= kMergeLds;
+  int64_t* cdf = lds ? s_mcdf : ws.cdf + s0;
+  const int64_t Wq = block_fixed_scan(0, ce, cdf, q, s_w64);
   const double wbar = W / (double)K;
   // stratified selection: r_k = (1/K)(U_k + k), searchsorted left
   double sx = 0, sy = 0, sz = 0, se = 0;
@@ -997,7 +1003,7 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
     const double u = res_uniform(a.seed, step, (uint32_t)k, (uint32_t)m);
     const double r = (1.0 / (double)K) * (u + (double)m);
     const int64_t thr = (int64_t)(r * (double)Wq);
-    const int j = search_cdf(ws.cdf, s0, ce, thr, false);
+    const int j = search_cdf(cdf, 0, ce, thr, false);
     if (out0 + m >= cap) continue;  // overflow: counted at commit
     ws.sel[out0 + m] = (uint32_t)(s0 + j);
     const uint32_t i = perm[s0 + j];
@@ -1125,8 +1131,8 @@ static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const b
   hipLaunchKernelGGL(k_res_split<T>, dim3(sgrid), dim3(kSegBlock), (size_t)kSplitLds * 16,
                      env.stream, a, counts, offsets, perm, (const int32_t*)ws.plan, ws,
                      (const Clock*)env.clock, cap);
-  hipLaunchKernelGGL(k_res_merge<T>, dim3(mgrid), dim3(kSegBlock), 0, env.stream, a, counts,
-                     offsets, perm, targets, (const int32_t*)ws.plan, ws,
+  hipLaunchKernelGGL(k_res_merge<T>, dim3(mgrid), dim3(kSegBlock), (size_t)kMergeLds * 8,
+                     env.stream, a, counts, offsets, perm, targets, (const int32_t*)ws.plan, ws,
                      (const Clock*)env.clock, cap);
   hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, cap);
   timing_end(env, 2);
