@@ -18,6 +18,15 @@ from . import _lib
 PUSH_SHAPES = {0: (256, 4), 1: (512, 2), 2: (256, 2), 3: (1024, 1), 4: (1024, 2)}
 
 
+def pick_within(results: dict, order, tie: float = None):
+    """Among the candidates timed within `tie` (relative) of the fastest,
+    the first by `order` (then by time)."""
+    tie = TIE if tie is None else tie
+    fastest = min(results.values())
+    return min((k for k in results if results[k] <= fastest * (1.0 + tie)),
+               key=lambda k: (order(k), results[k]))
+
+
 def push_time_ms(lib, ctx) -> tuple[float, int]:
     ms = (C.c_double * 3)()
     nl = (C.c_int64 * 3)()
@@ -26,6 +35,7 @@ def push_time_ms(lib, ctx) -> tuple[float, int]:
 
 
 GRID_BPC = (0, 1, 2, 4)  # blocks per CU of the 1-D push (0: as many as fit)
+TIE = 0.02  # relative difference below which two trial timings count as equal
 
 
 def _time(lib, ctx, step: Callable[[], None], steps: int, whole: bool = False) -> float:
@@ -60,7 +70,10 @@ def tune_shapes(lib, ctx, step: Callable[[], None], steps: int = 10,
         _lib.check(lib.bz_set_push_shape(ctx, shape))
         results[shape] = _time(lib, ctx, step, steps)
         used += steps + 1
-    best = min(results, key=results.get)
+    # the fastest, or a shape with a smaller tile within TIE of it: the trial
+    # steps run at the starting count, and a smaller tile keeps the waves
+    # evenly loaded when the count drifts (a discharge losing electrons)
+    best = pick_within(results, lambda k: PUSH_SHAPES[k][0] * PUSH_SHAPES[k][1])
     _lib.check(lib.bz_set_push_shape(ctx, best))
     out = {"best": best, "shape": PUSH_SHAPES[best],
            "avg_ms": {str(PUSH_SHAPES[k]): v for k, v in results.items()}}
@@ -70,7 +83,8 @@ def tune_shapes(lib, ctx, step: Callable[[], None], steps: int = 10,
             _lib.check(lib.bz_set_push_grid(ctx, bpc))
             gres[bpc] = _time(lib, ctx, step, steps, whole=True)
             used += steps + 1
-        gbest = min(gres, key=gres.get)
+        # within TIE, the fewest blocks (0, as many as fit, counts as the most)
+        gbest = pick_within(gres, lambda k: k if k > 0 else 99)
         _lib.check(lib.bz_set_push_grid(ctx, gbest))
         out["blocks_per_cu"] = gbest
         out["grid_avg_ms"] = {str(k): v for k, v in gres.items()}

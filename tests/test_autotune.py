@@ -1,0 +1,24 @@
+"""Autotuner choice rule (host logic, no GPU)."""
+from boltzmann_amd.autotune import PUSH_SHAPES, pick_within
+
+
+def _tile(k):
+    return PUSH_SHAPES[k][0] * PUSH_SHAPES[k][1]
+
+
+def test_smaller_tile_wins_a_tie():
+    # (1024, 2) is 0.1 % faster than (1024, 1): a tie, the smaller tile is kept
+    t = {0: 0.0323, 1: 0.0261, 2: 0.0315, 3: 0.025377, 4: 0.025345}
+    assert pick_within(t, _tile) == 3
+
+
+def test_clear_winner_kept():
+    t = {0: 0.0323, 1: 0.0261, 2: 0.0315, 3: 0.0300, 4: 0.0250}
+    assert pick_within(t, _tile) == 4
+
+
+def test_grid_prefers_fewest_blocks_within_tie():
+    g = {0: 0.04447, 1: 0.04103, 2: 0.04123, 4: 0.04082}
+    assert pick_within(g, lambda k: k if k > 0 else 99) == 1
+    g = {0: 0.030, 1: 0.041, 2: 0.040, 4: 0.039}
+    assert pick_within(g, lambda k: k if k > 0 else 99) == 0
