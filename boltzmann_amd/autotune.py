@@ -35,7 +35,7 @@ def push_time_ms(lib, ctx) -> tuple[float, int]:
 
 
 GRID_BPC = (0, 1, 2, 4)  # blocks per CU of the 1-D push (0: as many as fit)
-TIE = 0.02  # relative difference below which two trial timings count as equal
+TIE = 0.005  # relative difference below which two trial timings count as equal
 
 
 def _time(lib, ctx, step: Callable[[], None], steps: int, whole: bool = False) -> float:

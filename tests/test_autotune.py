@@ -18,7 +18,7 @@ def test_clear_winner_kept():
 
 
 def test_grid_prefers_fewest_blocks_within_tie():
-    g = {0: 0.04447, 1: 0.04103, 2: 0.04123, 4: 0.04082}
+    g = {0: 0.04447, 1: 0.04090, 2: 0.04123, 4: 0.04082}
     assert pick_within(g, lambda k: k if k > 0 else 99) == 1
     g = {0: 0.030, 1: 0.041, 2: 0.040, 4: 0.039}
     assert pick_within(g, lambda k: k if k > 0 else 99) == 0
