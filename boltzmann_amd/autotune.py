@@ -36,6 +36,8 @@ def push_time_ms(lib, ctx) -> tuple[float, int]:
 
 GRID_BPC = (0, 1, 2, 4)  # blocks per CU of the 1-D push (0: as many as fit)
 TIE = 0.005  # relative difference below which two trial timings count as equal
+TIE_SHORT = 0.02  # the same where each CU walks few tiles
+SHORT_WALK_TILES = 16  # tiles of 1024 particles per CU below which a walk is short
 
 
 def _time(lib, ctx, step: Callable[[], None], steps: int, whole: bool = False) -> float:
@@ -60,20 +62,24 @@ def _time(lib, ctx, step: Callable[[], None], steps: int, whole: bool = False) -
 
 
 def tune_shapes(lib, ctx, step: Callable[[], None], steps: int = 10,
-                grids: bool = False) -> dict:
-    """Time `steps` calls of `step` per push shape; keep the fastest.  With
-    `grids`, then time the best shape per blocks-per-CU setting."""
+                grids: bool = False, tie: float = None, max_tile: int = 0) -> dict:
+    """Time `steps` calls of `step` per push shape (tiles of at most
+    `max_tile` particles if set); keep the fastest (ties within `tie` go to
+    the smaller tile).  With `grids`, then time the best shape per
+    blocks-per-CU setting."""
     results = {}
     used = 0
     _lib.check(lib.bz_set_push_grid(ctx, 0))
     for shape in PUSH_SHAPES:
+        if max_tile and PUSH_SHAPES[shape][0] * PUSH_SHAPES[shape][1] > max_tile:
+            continue
         _lib.check(lib.bz_set_push_shape(ctx, shape))
         results[shape] = _time(lib, ctx, step, steps)
         used += steps + 1
     # the fastest, or a shape with a smaller tile within TIE of it: the trial
     # steps run at the starting count, and a smaller tile keeps the waves
     # evenly loaded when the count drifts (a discharge losing electrons)
-    best = pick_within(results, lambda k: PUSH_SHAPES[k][0] * PUSH_SHAPES[k][1])
+    best = pick_within(results, lambda k: PUSH_SHAPES[k][0] * PUSH_SHAPES[k][1], tie)
     _lib.check(lib.bz_set_push_shape(ctx, best))
     out = {"best": best, "shape": PUSH_SHAPES[best],
            "avg_ms": {str(PUSH_SHAPES[k]): v for k, v in results.items()}}
@@ -100,7 +106,15 @@ def tune_push(engine, steps: int = 10, start_index: int = 0) -> dict:
         engine.step(index[0])
         index[0] += 1
 
-    out = tune_shapes(engine.lib, engine.ctx, step, steps, grids=engine.cfg.dim == 1)
+    # a short walk (a few 1024-particle tiles per CU) is where the tile size
+    # decides how evenly the waves end, and where a count that drifts after
+    # the trial moves the balance: there the 2048-particle tile is left out
+    # and timings within TIE_SHORT tie
+    import torch
+    cus = torch.cuda.get_device_properties(engine.dev).multi_processor_count
+    short = engine.count() < SHORT_WALK_TILES * 1024 * cus
+    out = tune_shapes(engine.lib, engine.ctx, step, steps, grids=engine.cfg.dim == 1,
+                      tie=TIE_SHORT if short else TIE, max_tile=1024 if short else 0)
     engine.drop_graphs()  # blocks captured before hold the old launch shape
     return out
 
