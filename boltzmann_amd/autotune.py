@@ -113,8 +113,10 @@ def tune_push(engine, steps: int = 10, start_index: int = 0) -> dict:
     import torch
     cus = torch.cuda.get_device_properties(engine.dev).multi_processor_count
     short = engine.count() < SHORT_WALK_TILES * 1024 * cus
-    out = tune_shapes(engine.lib, engine.ctx, step, steps, grids=engine.cfg.dim == 1,
-                      tie=TIE_SHORT if short else TIE, max_tile=1024 if short else 0)
+    # short walks make short launches: time more of them against the noise
+    out = tune_shapes(engine.lib, engine.ctx, step, max(steps, 40) if short else steps,
+                      grids=engine.cfg.dim == 1, tie=TIE_SHORT if short else TIE,
+                      max_tile=1024 if short else 0)
     engine.drop_graphs()  # blocks captured before hold the old launch shape
     return out
 
