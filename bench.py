@@ -36,27 +36,36 @@ import time
 HBM_PEAK_GBS = 8000.0
 
 
-def _cores() -> int:
+def _cores() -> dict:
+    """Threads the CPU baseline uses: every core this process may run on, up
+    to 16 (the host-CPU share of one GPU on the box: 16 cores per GPU), with
+    the counts it was chosen from."""
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        aff = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    share = 16
+    if env and env.isdigit() and int(env) > 0:
+        share = int(env)
+    return {"threads": max(1, min(aff, share)), "nproc": os.cpu_count() or 1,
+            "affinity": aff, "omp_num_threads": env}
 
 
 def _time_oracle(setup, state, E, n, table, seed, seconds, capacity):
-    import numpy as np
+    """Oracle steps on `state` for about `seconds`: each step draws its Philox
+    uniforms inside the timed loop (R=None), as the device push does."""
+    import dataclasses
 
     from oracle import oracle as O
-    R = O.collision_uniforms(seed, 0, n, state[0].dtype)
     done = 0
     psteps = 0
     cur = state
     t0 = time.perf_counter()
     while True:
         m = cur[0].size
-        cur, _, _ = O.push_step(setup, cur, E, R if m >= n else np.ascontiguousarray(R[:, :m]),
-                                table=table, capacity=capacity)
+        s = dataclasses.replace(setup, seed=seed, step=done)
+        cur, _, _ = O.push_step(s, cur, E, None, table=table, capacity=capacity)
         psteps += m
         done += 1
         if time.perf_counter() - t0 > seconds or done >= 1000:
@@ -75,12 +84,11 @@ def _one_core(setup, state, E, table, seed, seconds):
     return {"value": rate, "cores": 1, "sample": f"{m} particles x {done} steps, 1 thread"}
 
 
-def cpu_baseline(cfg, seconds: float = 10.0):
+def cpu_baseline(cfg, seconds: float = 10.0, E=None):
     """Oracle (C++ restatement of the reference step, OpenMP over particles)
     on a bounded sample of the same workload: push + collisions + deposit +
-    compaction of up to 1M particles, repeated for about `seconds`.  The
-    uniforms are drawn outside the timed region (the GPU draws them
-    in-kernel)."""
+    compaction of up to 1M particles in the field E of the GPU run (V/cm per
+    cell), repeated for about `seconds`, each step drawing its uniforms."""
     import numpy as np
 
     from boltzmann_amd import physics as ph
@@ -99,14 +107,20 @@ def cpu_baseline(cfg, seconds: float = 10.0):
                     n_cells=cfg.n_cells, length_cm=cfg.length_cm, bnf=cfg.bnf,
                     gas_density_m3=cfg.gas_density_m3,
                     nu_max=table.nu_max if table is not None else 0.0,
-                    e_field0=cfg.e_field0_vcm() if cfg.dim == 0 else 0.0, nthreads=cores,
-                    push_order=cfg.push_order)
+                    e_field0=cfg.e_field0_vcm() if cfg.dim == 0 else 0.0,
+                    nthreads=cores["threads"], push_order=cfg.push_order)
     tab = None if table is None else (table.energy, table.sig, table.slope)
-    E = np.zeros(cfg.n_cells, dtype=np.float64) if cfg.dim == 1 else None
+    if cfg.dim == 1:
+        E = np.zeros(cfg.n_cells) if E is None else np.asarray(E, dtype=np.float64)
+        field = "the GPU run's final field" if np.any(E) else "zero field"
+    else:
+        E, field = None, "the uniform 0-D field"
     rate, done = _time_oracle(setup, st, E, n, tab, cfg.seed, seconds, 2 * n)
-    return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+    return {"value": rate, "unit": "particle-steps/s", "cores": cores["threads"], "kind": "port",
+            "host": cores,
             "sample": f"{n} particles x {done} steps of the oracle push+MCC+deposit+compaction "
-                      f"(FP64 like the reference, OpenMP {cores} threads)",
+                      f"in {field} (FP64 like the reference, Philox uniforms drawn in the "
+                      f"loop, OpenMP {cores['threads']} threads)",
             "one_core": _one_core(setup, st, E, tab, cfg.seed, seconds / 4)}
 
 
@@ -128,12 +142,14 @@ def cpu_baseline_swarms(scfg, seconds: float = 10.0):
     cores = _cores()
     setup = O.Setup(dim=0, dt=scfg.dt, xs_mode=1, gas_density_m3=scfg.gas_density_m3,
                     nu_max=table.nu_max, e_field0=en * ph.TD * scfg.gas_density_m3 / 100.0,
-                    nthreads=cores)
+                    nthreads=cores["threads"])
     tab = (table.energy, table.sig, table.slope)
     rate, done = _time_oracle(setup, st, None, n, tab, scfg.seed, seconds, 2 * n)
-    return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+    return {"value": rate, "unit": "particle-steps/s", "cores": cores["threads"], "kind": "port",
+            "host": cores,
             "sample": f"one 0-D swarm of {n} particles at {en:.1f} Td x {done} steps of the "
-                      f"oracle push+MCC (FP64 like the reference, OpenMP {cores} threads)",
+                      f"oracle push+MCC (FP64 like the reference, Philox uniforms drawn in the "
+                      f"loop, OpenMP {cores['threads']} threads)",
             "one_core": _one_core(setup, st, None, tab, scfg.seed, seconds / 4)}
 
 
@@ -150,9 +166,10 @@ def measured_traffic(workload: str, kernel: str, psteps_per_launch: float):
             rec = json.load(f)[workload]
     except (OSError, KeyError, ValueError):
         return None
-    if rec.get("kernel") != kernel:
+    rec = rec.get(kernel) if "kernel" not in rec else (rec if rec["kernel"] == kernel else None)
+    if not rec:
         return None
-    per = rec["fetch_bytes_per_particle_step"] + rec["write_bytes_per_particle_step"]
+    per = rec["fetch_bytes_per_unit"] + rec["write_bytes_per_unit"]
     return per * psteps_per_launch
 
 
@@ -167,13 +184,18 @@ def copy_probe(mib: int) -> None:
     torch.cuda.synchronize()
 
 
-def _roofline(bytes_per_pstep, psteps_per_launch, avg_s, kernel, workload=None):
-    achieved = bytes_per_pstep * psteps_per_launch / avg_s / 1e9
+def _roofline(bytes_per_unit, units_per_launch, avg_s, kernel, workload=None, unit="particle"):
+    achieved = bytes_per_unit * units_per_launch / avg_s / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(workload, kernel, psteps_per_launch) if workload else None,
+            "traffic": measured_traffic(workload, kernel, units_per_launch) if workload else None,
             "kernel": kernel, "kernel_avg_us": avg_s * 1e6,
-            "bytes_per_particle_step": bytes_per_pstep}
+            "bytes_per_unit": bytes_per_unit, "units_per_launch": units_per_launch,
+            "work_unit": unit}
+
+
+# timed kinds of the library's event timing (bz_timing_get_n)
+T_PUSH, T_SORT, T_RESAMPLE, T_COPY, T_DOWN = range(5)
 
 
 def bench_engine(args):
@@ -206,11 +228,13 @@ def bench_engine(args):
     _lib.check(eng.lib.bz_timing_enable(eng.ctx, 1))
     index = eng.run(index, n_timed, graphs=False)
     torch.cuda.synchronize()
-    ms = (C.c_double * 3)()
-    nl = (C.c_int64 * 3)()
-    _lib.check(eng.lib.bz_timing_get(eng.ctx, ms, nl))
+    ms = (C.c_double * 5)()
+    nl = (C.c_int64 * 5)()
+    _lib.check(eng.lib.bz_timing_get_n(eng.ctx, 5, ms, nl))
     _lib.check(eng.lib.bz_timing_enable(eng.ctx, 0))
     ct1 = eng.counters()
+    # every block the timed run replays is captured before the clock starts
+    n_graphs = eng.precapture() if graphs else 0
     c0 = eng.counters()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -219,14 +243,36 @@ def bench_engine(args):
     t1 = time.perf_counter()
     c1 = eng.counters()
     psteps = c1["particle_steps"] - c0["particle_steps"]
-    psteps_timed = ct1["particle_steps"] - ct0["particle_steps"]
     elapsed = t1 - t0
     word = 8 if cfg.precision == _lib.BZ_F64 else 4
-    # read + write of the carried components (0-D runs carry no position)
-    bytes_per_pstep = 2 * (5 if cfg.dim == 1 else 4) * word
+    # push: read + write of the carried components (0-D runs carry no position)
+    bytes_push = 2 * (5 if cfg.dim == 1 else 4) * word
     if eng.push_keys:
-        bytes_per_pstep += 4  # the binning key of the next sort, written by the push
-    avg_push_s = ms[0] / 1e3 / max(int(nl[0]), 1)
+        bytes_push += 4  # the binning key of the next sort, written by the push
+    dct = {k: ct1[k] - ct0[k] for k in ct1}
+
+    def avg(kind):
+        return ms[kind] / 1e3 / max(int(nl[kind]), 1)
+
+    kernels = {}
+    if nl[T_PUSH]:
+        kernels["push"] = _roofline(bytes_push, dct["particle_steps"] / nl[T_PUSH], avg(T_PUSH),
+                                    "k_push_inplace" if cfg.dim == 1 else "k_push", cfg.name,
+                                    "particle-step")
+    if nl[T_COPY]:
+        # sorted key 4 + permutation 4 + the five words gathered 20, written 20
+        kernels["res_copy"] = _roofline(8 + 10 * word, dct["copied"] / nl[T_COPY], avg(T_COPY),
+                                        "k_res_copy", cfg.name, "copied particle")
+    if nl[T_DOWN]:
+        passes = nl[T_DOWN] / max(nl[T_SORT], 1)
+        # per slot and pass: key read 4, key + value written 8, value read 4
+        # on every pass but the first (the first pass's values are the indices)
+        per = (12 + 16 * (passes - 1)) / passes
+        kernels["downsweep"] = _roofline(per, dct["sorted"] / max(nl[T_SORT], 1), avg(T_DOWN),
+                                         "k_downsweep", cfg.name, "sorted slot per pass")
+    # the roofline line names the kernel with the most time per step
+    share = {"push": ms[T_PUSH], "res_copy": ms[T_COPY], "downsweep": ms[T_DOWN]}
+    dom = max((k for k in kernels), key=lambda k: share[k])
     out = {
         "metric": "particle-steps/s on 1 MI355X",
         "value": psteps / elapsed,
@@ -242,20 +288,23 @@ def bench_engine(args):
         "data": "synthetic (Maxwellian electrons, uniform density, tabulated argon-like sigma)",
         "config": {"workload": cfg.name, "particles_start": cfg.n_particles,
                    "particles_end": eng.count(), "cells": cfg.n_cells if cfg.dim == 1 else 0,
+                   "energy_bins": cfg.nb, "resample_every": cfg.resample_every,
                    "dt_s": cfg.dt, "parallelism": "single GPU", "push_schedule": args.sched,
-                   "graphs": graphs and cfg.dim == 1 and cfg.resample_every != 1},
-        "roofline": _roofline(bytes_per_pstep, psteps_timed / n_timed, avg_push_s,
-                              "k_push_inplace" if cfg.dim == 1 else "k_push", cfg.name),
+                   "graphs": n_graphs},
+        "roofline": kernels[dom],
+        "roofline_kernels": kernels,
         "counters": {k: c1[k] - c0[k] for k in ("births", "absorbed", "ionization", "overflow",
-                                                "resample", "split", "merge")},
-        "kernel_ms": {"push": ms[0], "sort": ms[1], "resample": ms[2],
-                      "launches": [int(nl[0]), int(nl[1]), int(nl[2])],
-                      "over_steps": n_timed},
+                                                "resample", "split", "merge", "copied",
+                                                "sorted")},
+        "kernel_ms": {"push": ms[T_PUSH], "sort": ms[T_SORT], "resample": ms[T_RESAMPLE],
+                      "res_copy": ms[T_COPY], "downsweep": ms[T_DOWN],
+                      "launches": [int(v) for v in nl], "over_steps": n_timed},
         "autotune": tuned,
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    E_run = eng.E.cpu().numpy() if cfg.dim == 1 else None
     eng.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, E_run)
     return out
 
 
@@ -326,7 +375,7 @@ def bench_swarms(args):
                    "graph": f"{per_graph} steps + control every {scfg.control_every}",
                    "parallelism": "single GPU"},
         "roofline": _roofline(bytes_per_pstep, 0.5 * (n_before + n_mid), avg_push_s, "k_push_inplace",
-                              scfg.name),
+                              scfg.name, "particle-step"),
         "counters": {k: c1[k] - c0[k] for k in ("births", "ionization", "split", "merge")},
         "sample_results": {"en_td": [float(res["en_td"][i]) for i in (0, 128, 255)],
                            "mean_energy_ev": [float(res["mean_energy"][i]) for i in (0, 128, 255)]},
@@ -452,9 +501,10 @@ def bench_closure(args):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--workload", default="c4",
+                    help="c4 (default: the largest single-GPU discharge), c0, c1, c2, c3")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",

@@ -16,7 +16,8 @@ BZ_RNG_PHILOX, BZ_RNG_STREAM = 0, 1
 BZ_PUSH_REF, BZ_PUSH_LEAPFROG = 0, 1
 BZ_NCOUNTERS = 16
 COUNTER_NAMES = ("births", "absorbed", "elastic", "excitation", "ionization", "null",
-                 "overflow", "numax", "split", "merge", "resample", "particle_steps")
+                 "overflow", "numax", "split", "merge", "resample", "particle_steps",
+                 "copied", "sorted")
 
 
 class BzError(RuntimeError):
@@ -100,6 +101,7 @@ SIGNATURES = {
     "bz_clock_advance": (C.c_int, [_P, C.c_double]),
     "bz_timing_enable": (C.c_int, [_P, C.c_int]),
     "bz_timing_get": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "bz_timing_get_n": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "bz_push_mcc": (C.c_int, [_P, C.POINTER(State), C.POINTER(State), _P, C.POINTER(Params),
                               C.POINTER(XSec), _P, _P, _P]),
     "bz_deposit": (C.c_int, [_P, C.POINTER(State), C.POINTER(Params), _P]),

@@ -1,5 +1,9 @@
 """Build libbz.so (gfx950 HIP kernels + C ABI) in-tree with hipcc.
 
+Each translation unit is compiled to an object in parallel (no device code
+crosses translation units, so no relocatable device code is needed), then
+the objects are linked into the shared library.
+
 Usage: python -m boltzmann_amd.build   (or build() from Python)
 """
 from __future__ import annotations
@@ -7,11 +11,13 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libbz.so"
+OBJ = PKG / "_build"
 SOURCES = ["bz_api.hip", "bz_push.hip", "bz_field.hip", "bz_sort.hip", "bz_swarm.hip",
            "bz_mlp.hip"]
 ARCH = os.environ.get("BZ_OFFLOAD_ARCH", "gfx950")
@@ -20,7 +26,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     f"--offload-arch={ARCH}",
     # identical IEEE evaluation order on device and in the CPU check
     "-ffp-contract=off",
@@ -31,16 +36,28 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> Path:
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc, *HIPCC_FLAGS, *[str(CSRC / s) for s in SOURCES], "-o", str(tmp)]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode})")
+        raise RuntimeError(f"hipcc failed ({res.returncode}): {cmd[-1]}")
+
+
+def build(verbose: bool = False) -> Path:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    OBJ.mkdir(exist_ok=True)
+    objs = [OBJ / (Path(s).stem + ".o") for s in SOURCES]
+    jobs = [[hipcc, *HIPCC_FLAGS, "-c", str(CSRC / s), "-o", str(o)]
+            for s, o in zip(SOURCES, objs)]
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(_run, j, verbose) for j in jobs]:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objs],
+          "-o", str(tmp)], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -39,7 +39,9 @@ struct LaunchEnv {
   struct Timing* timing;
 };
 
-constexpr int kTimedKinds = 3;  // 0 push, 1 sort, 2 resample
+// 0 push, 1 sort, 2 resample (whole stages); 3 the resampling copy and
+// 4 each radix downsweep (single kernels inside stages 2 and 1)
+constexpr int kTimedKinds = 5;
 struct Timing {
   int on;
   int cap;

@@ -218,9 +218,12 @@ __global__ __launch_bounds__(256) void k_key_bounds(const uint32_t* __restrict__
   }
 }
 
-// counts[k] = offsets[k+1] - offsets[k], k = 0..m-1
+// counts[k] = offsets[k+1] - offsets[k], k = 0..m-1; the sort's size is
+// added to the sorted-slots counter
 __global__ __launch_bounds__(256) void k_key_counts(const int32_t* __restrict__ offsets, int m,
-                                                    int32_t* counts) {
+                                                    int32_t* counts, Clock* clk) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    clk->cnt[BZ_CNT_SORTED] += (unsigned long long)clk->n;
   for (int k = blockIdx.x * 256 + threadIdx.x; k < m; k += gridDim.x * 256)
     counts[k] = offsets[k + 1] - offsets[k];
 }
@@ -454,9 +457,11 @@ static int sort_keys(const LaunchEnv& env, const SortWs& ws, int n_keys, int32_t
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, env.stream, ws.chunks, nch);
     hipLaunchKernelGGL(k_chunk_apply, dim3((unsigned)nch), dim3(1024), 0, env.stream, ws.hist, m,
                        (const uint32_t*)ws.chunks);
+    timing_begin(env, 4);
     hipLaunchKernelGGL(k_downsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
                        (const uint32_t*)kin, vin, kout, vout, (const Clock*)env.clock, shift,
                        (const uint32_t*)ws.hist, tiles, ps == 0 ? 1 : 0);
+    timing_end(env, 4);
     rc = check_launch("radix pass");
     if (rc) return rc;
     kin = kout;
@@ -469,7 +474,7 @@ static int sort_keys(const LaunchEnv& env, const SortWs& ws, int n_keys, int32_t
   hipLaunchKernelGGL(k_key_bounds, dim3((unsigned)bgrid), dim3(256), 0, env.stream,
                      (const uint32_t*)kin, (const Clock*)env.clock, mk, offsets);
   hipLaunchKernelGGL(k_key_counts, dim3((mk + 255) / 256), dim3(256), 0, env.stream,
-                     (const int32_t*)offsets, mk, counts);
+                     (const int32_t*)offsets, mk, counts, env.clock);
   if (skeys) *skeys = kin;
   return check_launch("key counts");
 }
@@ -658,6 +663,7 @@ __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* cou
   __syncthreads();
   const int fire = s_fire;
   int n_split = 0, n_merge = 0;
+  unsigned long long n_copied = 0;  // particles k_res_copy moves unchanged
   for (int base = 0; base < a.n_keys; base += 1024) {
     const int k = base + tid;
     int kind = kCopy, nc = 0;
@@ -686,6 +692,9 @@ __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* cou
       }
       plan[4 * k + 0] = kind;
       plan[4 * k + 1] = nc;
+      if (kind == kCopy) n_copied += (unsigned)c;
+      else if (kind == kMerge && P.max_ppc > 0 && c > P.max_ppc)
+        n_copied += (unsigned)(c - P.max_ppc);
     }
     const int inc = wave_incl_scan(nc);
     if (lane == 63) s_w[wid] = inc;
@@ -703,9 +712,11 @@ __global__ __launch_bounds__(1024) void k_res_plan(ResArgs a, const int32_t* cou
   }
   n_split = wave_sum(n_split);
   n_merge = wave_sum(n_merge);
+  n_copied = wave_sum(n_copied);
   if (lane == 0) {
     if (n_split) atomicAdd(&clk->cnt[BZ_CNT_SPLIT], (unsigned long long)n_split);
     if (n_merge) atomicAdd(&clk->cnt[BZ_CNT_MERGE], (unsigned long long)n_merge);
+    if (n_copied) atomicAdd(&clk->cnt[BZ_CNT_COPIED], n_copied);
   }
   if (tid == 0) {
     clk->n_next = s_carry;
@@ -1122,9 +1133,11 @@ static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const b
   hipLaunchKernelGGL(k_res_plan, dim3(1), dim3(1024), 0, env.stream, a, counts, targets, ws.plan,
                      nb, env.clock, ws.klist, ws.kstride);
   const int64_t cgrid = (cap + 256 * kCopyItems - 1) / (256 * kCopyItems);
+  timing_begin(env, 3);
   hipLaunchKernelGGL(k_res_copy<T>, dim3((unsigned)(cgrid > 0 ? cgrid : 1)), dim3(256), 0,
                      env.stream, a, skeys, offsets, perm, (const int32_t*)ws.plan, counts,
                      (const Clock*)env.clock, cap);
+  timing_end(env, 3);
   // at most n_keys listed keys; blocks beyond the list exit at once
   const int sgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
   const int mgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;

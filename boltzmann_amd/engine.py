@@ -347,6 +347,28 @@ class KineticEngine:
             _lib.check(self.lib.bz_set_stream(self.ctx, C.c_void_p(self.stream.cuda_stream)))
         return gid.value
 
+    def precapture(self) -> int:
+        """Capture every block `run` can replay (each block length up to the
+        resampling period, or the compaction period without resampling, on
+        either state buffer) before a timed run, so that the run itself only
+        replays.  Returns the number of graphs held."""
+        k = self.cfg.resample_every if self.cfg.target_ppc > 0 else 0
+        if not self.inplace or k == 1:
+            return len(self._graphs)  # every step is launched directly
+        limit = k if k else self.compact_every
+        if self.compact_every:
+            limit = min(limit, self.compact_every)
+        cur = self.cur
+        try:
+            for c in (0, 1):
+                self.cur = c
+                for m in range(1, max(limit, 1) + 1):
+                    if (c, m) not in self._graphs:
+                        self._graphs[(c, m)] = self._capture(m)
+        finally:
+            self.cur = cur
+        return len(self._graphs)
+
     def drop_graphs(self):
         """Forget the captured blocks (after a change of the push knobs)."""
         for gid in self._graphs.values():

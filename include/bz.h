@@ -73,6 +73,8 @@ enum {
   BZ_CNT_MERGE = 9,     /* resampling: segments merged */
   BZ_CNT_RESAMPLE = 10, /* resampling passes that fired */
   BZ_CNT_PSTEPS = 11,   /* particle-steps pushed (sum of input counts) */
+  BZ_CNT_COPIED = 12,   /* resampling: particles copied unchanged (copy segments) */
+  BZ_CNT_SORTED = 13,   /* slots ordered by the binning sorts (sum of sort sizes) */
   BZ_NCOUNTERS = 16
 };
 
@@ -228,6 +230,10 @@ int bz_clock_advance(bz_ctx_t ctx, double dt);
  * [0] push, [1] sort, [2] resample.  No counterpart in the reference. */
 int bz_timing_enable(bz_ctx_t ctx, int on);
 int bz_timing_get(bz_ctx_t ctx, double* ms3, int64_t* launches3);
+/* The same for the first n (<= 5) timed kinds: [0] push, [1] sort,
+ * [2] resample, [3] the resampling copy kernel alone (k_res_copy), [4] each
+ * radix downsweep pass (k_downsweep).  No counterpart in the reference. */
+int bz_timing_get_n(bz_ctx_t ctx, int n, double* ms, int64_t* launches);
 
 /* Fused electron step: MCC collision (elastic / excitation / ionization with
  * a secondary), push in E(x), absorbing walls and node deposit of

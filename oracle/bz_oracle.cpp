@@ -70,33 +70,39 @@ static float orc_u24(uint32_t a) { return (float)(a >> 8) / 16777216.0f; }
 
 // The collision uniforms of step `step` for particles 0..n-1, laid out as the
 // reference's R matrix transposed: R[c*stride + i], columns 0,1,2,3,5 used.
-void orc_collision_uniforms(uint64_t seed, uint64_t step, int64_t n, int prec, void* R,
-                            int64_t stride) {
-  for (int64_t i = 0; i < n; ++i) {
+static void orc_uniforms_at(uint64_t seed, uint64_t step, int64_t n, int prec, void* R,
+                            int64_t stride, uint32_t first) {
+  for (int64_t j = 0; j < n; ++j) {
+    const uint32_t i = first + (uint32_t)j;
     uint32_t b0[4], b1[4], b2[4];
-    orc_draw(seed, 0, (uint32_t)i, 0, step, b0);
-    orc_draw(seed, 0, (uint32_t)i, 1, step, b1);
+    orc_draw(seed, 0, i, 0, step, b0);
+    orc_draw(seed, 0, i, 1, step, b1);
     if (prec == 8) {
-      orc_draw(seed, 0, (uint32_t)i, 2, step, b2);
+      orc_draw(seed, 0, i, 2, step, b2);
       double* r = (double*)R;
-      r[0 * stride + i] = orc_u53(b0[0], b0[1]);
-      r[1 * stride + i] = orc_u53(b0[2], b0[3]);
-      r[2 * stride + i] = orc_u53(b1[0], b1[1]);
-      r[3 * stride + i] = orc_u53(b1[2], b1[3]);
-      r[4 * stride + i] = 0.0;
-      r[5 * stride + i] = orc_u53(b2[0], b2[1]);
-      r[6 * stride + i] = 0.0;
+      r[0 * stride + j] = orc_u53(b0[0], b0[1]);
+      r[1 * stride + j] = orc_u53(b0[2], b0[3]);
+      r[2 * stride + j] = orc_u53(b1[0], b1[1]);
+      r[3 * stride + j] = orc_u53(b1[2], b1[3]);
+      r[4 * stride + j] = 0.0;
+      r[5 * stride + j] = orc_u53(b2[0], b2[1]);
+      r[6 * stride + j] = 0.0;
     } else {
       float* r = (float*)R;
-      r[0 * stride + i] = orc_u24(b0[0]);
-      r[1 * stride + i] = orc_u24(b0[1]);
-      r[2 * stride + i] = orc_u24(b0[2]);
-      r[3 * stride + i] = orc_u24(b0[3]);
-      r[4 * stride + i] = 0.0f;
-      r[5 * stride + i] = orc_u24(b1[0]);
-      r[6 * stride + i] = 0.0f;
+      r[0 * stride + j] = orc_u24(b0[0]);
+      r[1 * stride + j] = orc_u24(b0[1]);
+      r[2 * stride + j] = orc_u24(b0[2]);
+      r[3 * stride + j] = orc_u24(b0[3]);
+      r[4 * stride + j] = 0.0f;
+      r[5 * stride + j] = orc_u24(b1[0]);
+      r[6 * stride + j] = 0.0f;
     }
   }
+}
+
+void orc_collision_uniforms(uint64_t seed, uint64_t step, int64_t n, int prec, void* R,
+                            int64_t stride) {
+  orc_uniforms_at(seed, step, n, prec, R, stride, 0u);
 }
 
 // Resampling draws: uniform m of segment `key` at step `step` (FP64).
@@ -281,8 +287,23 @@ int one_particle(int64_t i, const orc_params& P, const T* x, const T* vx, const 
   T e = ux * ux + uy * uy + uz * uz;
   const T v_mag = m_sqrt(e);
   e = (T)P.kev * e;
-  const T R0 = R[i], R1 = R[rs + i], R2 = R[2 * rs + i], R3 = R[3 * rs + i],
-          R5 = R[5 * rs + i];
+  T R0, R1, R2, R3, R5;
+  if (R) {
+    R0 = R[i];
+    R1 = R[rs + i];
+    R2 = R[2 * rs + i];
+    R3 = R[3 * rs + i];
+    R5 = R[5 * rs + i];
+  } else {
+    // no stream given: the Philox uniforms of (seed, step, i), drawn here
+    T u[7];
+    orc_uniforms_at(P.seed, P.step, 1, sizeof(T) == 8 ? 8 : 4, u, 1, (uint32_t)i);
+    R0 = u[0];
+    R1 = u[1];
+    R2 = u[2];
+    R3 = u[3];
+    R5 = u[5];
+  }
   int c = 0;
   if (P.xs_mode == 0) {
     T s0, s2;

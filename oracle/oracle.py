@@ -168,10 +168,11 @@ def push_step(setup: Setup, state, E, R, G=None, table=None, capacity=None):
         keep += [e, s, sl]
         tab = OrcTable(energy=_ptr(e), sig=_ptr(s), slope=_ptr(sl), n=e.size)
     Ea = None if E is None else np.ascontiguousarray(E, dtype=dt)
-    # free-flight mode (xs_mode 2) draws its own stream: R may be None
-    Ra = np.zeros((7, 1), dtype=dt) if R is None else np.ascontiguousarray(R, dtype=dt)
+    # R None: the oracle draws the Philox uniforms of (seed, step, slot) itself,
+    # as free-flight mode (xs_mode 2) always does
+    Ra = None if R is None else np.ascontiguousarray(R, dtype=dt)
     Ga = None if G is None else np.ascontiguousarray(G, dtype=dt)
-    rs = Ra.shape[1]
+    rs = Ra.shape[1] if Ra is not None else (Ga.shape[1] if Ga is not None else 1)
     if Ga is not None and Ga.shape[1] != rs:
         raise ValueError("R and G must share the column stride")
     fn = lib.orc_push_step_f64 if dt == np.float64 else lib.orc_push_step_f32
