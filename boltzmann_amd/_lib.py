@@ -14,10 +14,11 @@ BZ_F32, BZ_F64 = 4, 8
 BZ_XS_FIT, BZ_XS_TABLE, BZ_XS_FLIGHT, BZ_XS_DIRECT = 0, 1, 2, 3
 BZ_RNG_PHILOX, BZ_RNG_STREAM = 0, 1
 BZ_PUSH_REF, BZ_PUSH_LEAPFROG = 0, 1
+BZ_REF_DEAD_IONIZER, BZ_REF_ENERGY_ROUNDTRIP = 1, 2
 BZ_NCOUNTERS = 16
 COUNTER_NAMES = ("births", "absorbed", "elastic", "excitation", "ionization", "null",
                  "overflow", "numax", "split", "merge", "resample", "particle_steps",
-                 "copied", "sorted")
+                 "copied", "sorted", "dep_range")
 
 
 class BzError(RuntimeError):
@@ -40,7 +41,7 @@ class Params(C.Structure):
         ("seed", C.c_uint64),
         ("dim", C.c_int32), ("xs_mode", C.c_int32), ("rng_mode", C.c_int32),
         ("n_cells", C.c_int32), ("n_nodes", C.c_int32), ("bnf", C.c_int32),
-        ("r_stride", C.c_int64), ("push_order", C.c_int32), ("pad_", C.c_int32),
+        ("r_stride", C.c_int64), ("push_order", C.c_int32), ("ref_flags", C.c_int32),
     ]
 
 
@@ -64,7 +65,8 @@ class ResamplePolicy(C.Structure):
                 ("merge_keep", C.c_double), ("trigger_at", C.c_double),
                 ("min_wt", C.c_double), ("split_to", C.c_double),
                 ("ref_trigger", C.c_int32), ("max_ppc", C.c_int32),
-                ("ref_rescale", C.c_int32), ("pad", C.c_int32)]
+                ("ref_rescale", C.c_int32), ("pad", C.c_int32),
+                ("uniforms", C.c_void_p), ("u_stride", C.c_int64)]
 
 
 class Mlp(C.Structure):
@@ -116,6 +118,9 @@ SIGNATURES = {
                               C.c_int32, _P, _P, _P]),
     "bz_resample": (C.c_int, [_P, C.POINTER(State), C.POINTER(State), C.POINTER(Params),
                               _P, _P, _P, _P, C.c_int32, C.POINTER(ResamplePolicy)]),
+    "bz_resample_inplace": (C.c_int, [_P, C.POINTER(State), C.POINTER(Params),
+                                      C.POINTER(C.c_double), C.c_int32, _P, C.c_int32,
+                                      C.POINTER(ResamplePolicy)]),
     "bz_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint32)]),
     "bz_swarm_step": (C.c_int, [_P, C.POINTER(State), C.POINTER(State), C.POINTER(Params),

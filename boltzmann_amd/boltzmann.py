@@ -46,26 +46,35 @@ def reference_config(Nc: int, gridpts: int, bnf: int, in_dt: float, nb: int = 1,
         capacity_factor=4.0)
 
 
-def liu_initial_state(cfg: ph.RunConfig, seed: int):
-    """Per node: target particles at the node (+-dx/4 at the ends), Maxwellian
-    at 0.5 eV, weight n(x)/ppc with n(x) = P (1e7 + 1e9 (1-x/L)^2 (x/L)^2)."""
+def liu_initial_state(cfg: ph.RunConfig, seed: int, num_nodes: int = 1):
+    """The reference's prescribed initial condition (IC_flag == 1,
+    1bin/boltzmann.py:242-333): per node ii, ppc particles at the node
+    position (+dx/4 at the first node, -dx/4 at the last), velocities
+    N(0, 0.5 * kTm) per component (Te = 0.5 eV, kTm the 1 eV variance,
+    :253,:267,:293-295), weight ne(x_ii) / ppc / num_nodes (:297) with
+    ne(x) = P (1e7 + 1e9 (1-x/L)^2 (x/L)^2) (:246), ppc the sum of the
+    per-bin targets (3bin :282).  Then the rows are shuffled by one
+    permutation applied to every array (:315-318, 'shuffling for
+    efficiency'); the reference's permutation is CuPy's seeded shuffle, this
+    one numpy's (a uniformly random permutation either way).  Returns the
+    state and ne at the nodes."""
     N = cfg.n_cells
     rng = np.random.default_rng(seed)
     xs = np.linspace(0, cfg.length_cm, N + 1)
     ne = P_TORR * (1e7 + 1e9 * (1 - xs / cfg.length_cm) ** 2 * (xs / cfg.length_cm) ** 2)
     dx = cfg.dx
-    # particles per node = sum of the per-bin targets (boltzmann.py:278)
     ppc = max(cfg.target_ppc, 1)
     x, v, w = [], [], []
     for ii in range(N + 1):
         pos = xs[ii] + (dx / 4 if ii == 0 else (-dx / 4 if ii == N else 0.0))
         x.append(np.full(ppc, pos))
         v.append(rng.normal(0.0, math.sqrt(0.5 * KTM), size=(3, ppc)))
-        w.append(np.full(ppc, ne[ii] / ppc))
+        w.append(np.full(ppc, ne[ii] / ppc / num_nodes))
     x = np.concatenate(x)
     v = np.concatenate(v, axis=1)
     w = np.concatenate(w)
-    return (x, v[0], v[1], v[2], w), ne
+    perm = rng.permutation(x.size)
+    return (x[perm], v[0][perm], v[1][perm], v[2][perm], w[perm]), ne
 
 
 class BoltzmannDSMC:
@@ -86,12 +95,15 @@ class BoltzmannDSMC:
         self.steps_per_cycle = int(in_spc)
         self.num_nodes = int(num_nodes)
         self.curr_t = 0.0
-        state, ne0 = liu_initial_state(self.cfg, seed)
+        state, ne0 = liu_initial_state(self.cfg, seed, self.num_nodes)
         self.eng = KineticEngine(self.cfg, device=device)
         tt = np.array([int(f * int(Nc) / self.N) for f in self.cfg.bin_fractions],
                       dtype=np.int32)
         self.eng.targets_host = np.tile(tt, self.N + 1)
         self.eng.targets.copy_(torch.from_numpy(self.eng.targets_host))
+        # the reference's own behaviours: the dead ionizer's flag at node 0
+        # and the (e*w)/w binning energy
+        self.eng.params.ref_flags = _lib.BZ_REF_DEAD_IONIZER | _lib.BZ_REF_ENERGY_ROUNDTRIP
         pol = self.eng.policy
         pol.ref_trigger = 1
         pol.max_ppc = int(10.0 * max(self.eng.targets_host.max(), 1))

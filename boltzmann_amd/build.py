@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libbz.so"
 OBJ = PKG / "_build"
 SOURCES = ["bz_api.hip", "bz_push.hip", "bz_field.hip", "bz_sort.hip", "bz_swarm.hip",
-           "bz_mlp.hip"]
+           "bz_mlp.hip", "bz_reduction.hip"]
 ARCH = os.environ.get("BZ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

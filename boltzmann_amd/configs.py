@@ -65,7 +65,7 @@ def c4() -> RunConfig:
                      te_ev=2.0, density_cm3=1e9, rf_voltage=300.0, rf_freq=RF,
                      target_ppc=8000, resample_every=1, trigger_at=0.0, merge_at=1.5,
                      bin_fractions=bin_fractions(5), bin_thresholds=bin_thresholds(5),
-                     capacity_factor=1.5, ion_scheme=1, push_order=_lib.BZ_PUSH_LEAPFROG)
+                     capacity_factor=2.5, ion_scheme=1, push_order=_lib.BZ_PUSH_LEAPFROG)
 
 
 WORKLOADS = {"c0": c0, "c1": c1, "c3": c3, "c4": c4}

@@ -53,11 +53,14 @@ struct bz_ctx {
   const void* keys_state;
   BinKeys keys_bk;
   double keys_kev;
+  // the clock's weight bound was taken from the state loaded by the last
+  // clock set (merges raise it on the device afterwards)
+  bool wmax_valid;
 };
 
 static bool same_bins(const BinKeys& a, const BinKeys& b) {
   if (a.dx != b.dx || a.nb != b.nb || a.n_nodes != b.n_nodes || a.n_keys != b.n_keys ||
-      a.dim != b.dim)
+      a.dim != b.dim || a.ew != b.ew)
     return false;
   for (int k = 0; k < 8; ++k)
     if (a.thr[k] != b.thr[k]) return false;
@@ -134,6 +137,12 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   if (st == hipSuccess) st = hipMalloc((void**)&e.dep_l2, e.dep_l2_doubles * sizeof(double));
   if (st == hipSuccess) st = hipMalloc((void**)&e.dep_cols_done, e.dep_cols * sizeof(uint32_t));
   if (st == hipSuccess) st = hipMemset(e.dep_cols_done, 0, e.dep_cols * sizeof(uint32_t));
+  if (st == hipSuccess)
+    st = hipMalloc((void**)&e.dep_m1_lo, (size_t)max_nodes * sizeof(unsigned long long));
+  if (st == hipSuccess) st = hipMalloc((void**)&e.dep_m1_hi, (size_t)max_nodes * sizeof(uint32_t));
+  if (st == hipSuccess)
+    st = hipMemset(e.dep_m1_lo, 0, (size_t)max_nodes * sizeof(unsigned long long));
+  if (st == hipSuccess) st = hipMemset(e.dep_m1_hi, 0, (size_t)max_nodes * sizeof(uint32_t));
   if (st == hipSuccess) st = hipHostMalloc((void**)&c->host_clock, sizeof(Clock));
   if (st == hipSuccess) st = hipMemset(e.clock, 0, sizeof(Clock));
   if (st == hipSuccess) st = hipMemset(e.status, 0, e.status_words * sizeof(uint64_t));
@@ -164,6 +173,8 @@ int bz_ctx_destroy(bz_ctx_t c) {
   if (c->env.sort_ws) (void)hipFree(c->env.sort_ws);
   if (c->env.dep_l2) (void)hipFree(c->env.dep_l2);
   if (c->env.dep_cols_done) (void)hipFree(c->env.dep_cols_done);
+  if (c->env.dep_m1_lo) (void)hipFree(c->env.dep_m1_lo);
+  if (c->env.dep_m1_hi) (void)hipFree(c->env.dep_m1_hi);
   if (c->host_clock) (void)hipHostFree(c->host_clock);
   graph_release(c);
   if (Timing* t = c->env.timing) {
@@ -311,8 +322,18 @@ int bz_clock_set(bz_ctx_t c, int64_t n, uint64_t step, double t) {
   c->host_clock->step = step;
   c->host_clock->t = t;
   c->host_clock->ticket = 0;
+  c->host_clock->w_max_bits = 0ull;
   BZ_HIP(hipMemcpy(c->env.clock, c->host_clock, sizeof(Clock), hipMemcpyHostToDevice));
+  c->wmax_valid = false;
   return 0;
+}
+
+// the weight bound of a freshly set state, before its first deposit
+static int ensure_wmax(bz_ctx_t c, const bz_state_t* s) {
+  if (c->wmax_valid) return 0;
+  const int rc = launch_dep_wmax(c->env, s);
+  if (!rc) c->wmax_valid = true;
+  return rc;
 }
 
 int bz_clock_get(bz_ctx_t c, int64_t* n, uint64_t* step, double* t, uint64_t* counters) {
@@ -393,6 +414,8 @@ int bz_push_mcc(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const v
     if (p->r_stride < 1) return fail("stream mode needs r_stride >= 1");
   }
   c->keys_state = nullptr;
+  rc = ensure_wmax(c, in);
+  if (rc) return rc;
   rc = launch_push(c->env, in, out, E, p, xs, R, gas_v, moments, c->env.push_shape);
   if (!rc && p->dim == 1 && c->env.push_keys_nb > 0) {
     c->keys_state = in->x;
@@ -407,6 +430,8 @@ int bz_deposit(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p, double* mo
   int rc = check_params(c, p);
   if (rc) return rc;
   if (!s || !moments) return fail("null state or moments pointer");
+  rc = ensure_wmax(c, s);
+  if (rc) return rc;
   return launch_deposit(c->env, s, p, moments);
 }
 
@@ -508,6 +533,30 @@ int bz_resample(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const b
   if (n_keys < 1 || n_keys > c->env.max_keys) return fail("bad n_keys");
   c->keys_state = nullptr;
   return launch_resample(c->env, in, out, p, counts, offsets, perm, targets, n_keys, pol);
+}
+
+int bz_resample_inplace(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
+                        const double* thresholds_host, int32_t nb, const int32_t* targets,
+                        int32_t n_keys, const bz_resample_policy_t* pol) {
+  BZ_CTX(c);
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (!s || !targets || !pol) return fail("null resample pointer");
+  if (nb < 1 || nb > 8) return fail("nb must be in [1,8]");
+  if (nb > 1 && !thresholds_host) return fail("nb > 1 needs thresholds");
+  const int64_t nk = (int64_t)(p->dim == 1 ? p->n_nodes : 1) * nb;
+  if (n_keys != nk) return fail("n_keys must be n_nodes * nb = %lld", (long long)nk);
+  if (n_keys < 1 || n_keys > c->env.max_keys) return fail("bad n_keys");
+  if (pol->uniforms && pol->u_stride < 1) return fail("resample uniforms need u_stride >= 1");
+  const bool ready = c->keys_state && c->keys_state == s->x && c->keys_kev == p->kev &&
+                     same_bins(c->keys_bk, bin_keys(p, thresholds_host, nb));
+  c->keys_state = nullptr;  // the state changes
+  if (!ready) {
+    rc = launch_slot_keys(c->env, s, p, thresholds_host, nb);
+    if (rc) return rc;
+  }
+  return launch_resample_inplace(c->env, s, p, (const uint32_t*)sort_input_keys(c->env), nb,
+                                 targets, n_keys, pol);
 }
 
 }  // extern "C"

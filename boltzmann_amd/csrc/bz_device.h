@@ -223,10 +223,12 @@ struct BinKeys {
   int n_nodes;
   int n_keys;  // the key of a dead slot
   int dim;
+  int ew;      // 1: bin the energy after (e*w)/w (BZ_REF_ENERGY_ROUNDTRIP)
 };
 
 template <typename T>
-__device__ __forceinline__ uint32_t bin_key(const BinKeys& k, T x, T e) {
+__device__ __forceinline__ uint32_t bin_key(const BinKeys& k, T x, T e, T w) {
+  if (k.ew) e = (e * w) / w;
   int cell = 0;
   if (k.dim == 1) {
     cell = (int)(((double)x + 0.5 * k.dx) / k.dx);
@@ -235,6 +237,21 @@ __device__ __forceinline__ uint32_t bin_key(const BinKeys& k, T x, T e) {
   int b = 0;
   for (int q = 0; q < k.nb - 1; ++q) b += (e >= (T)k.thr[q]) ? 1 : 0;
   return (uint32_t)(cell * k.nb + b);
+}
+
+// Stable compaction blocks: block b of the grid owns a contiguous run of
+// whole tiles of kCompTile slots out of n.
+constexpr int kCompBlock = 256;
+constexpr int kCompItems = 4;
+constexpr int kCompTile = kCompBlock * kCompItems;
+
+__device__ __forceinline__ void comp_range(int64_t n, int64_t* lo, int64_t* hi) {
+  const int64_t tiles = (n + kCompTile - 1) / kCompTile;
+  const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
+  int64_t a = (int64_t)blockIdx.x * per * kCompTile;
+  int64_t b = a + per * kCompTile;
+  *lo = a < n ? a : n;
+  *hi = b < n ? b : n;
 }
 
 // Device clock kept inside the context.
@@ -249,6 +266,80 @@ struct Clock {
   uint32_t resample; // resampling pass requested (set by the check)
   uint32_t pad;
   unsigned long long cnt[BZ_NCOUNTERS];
+  // bits of an upper bound of every weight (a positive double's bits order
+  // like the double): set from the state after a clock set, raised by every
+  // merge; the deposit's fixed-point scale follows from it
+  unsigned long long w_max_bits;
 };
+
+// ---------------------------------------------------------------------------
+// Deterministic node moments of the 1-D deposit.  Each contribution v >= 0
+// (a weight, or a weight times an energy) is scaled by 2^S, S = 48 minus the
+// binary exponent of the weight bound, truncated to an integer and added to
+// a 64-bit word; a wrap of that word (the returned old value says so) adds a
+// carry to a 32-bit carry word.  Integer sums do not depend on the order the
+// adds arrive in, so the node sums are the same on every run.  The push
+// keeps the weight and energy sums (moments 0 and 2) per block in LDS, 64-bit
+// words [2][nodes] and one carry word per node holding both moments' carries
+// in 16-bit fields; the ionization sum (moment 1, added only where a particle
+// ionizes) goes to global 64-bit words and carry words shared by all blocks.
+// Resolution 2^-48 of the weight bound per contribution; range 2^80 units
+// (2^32 bound-weights or bound-weight-eV) per node and block in LDS, 2^96 in
+// the global words.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dep_scale_of(unsigned long long wmax_bits) {
+  if (!wmax_bits) return 1.0;
+  int e;
+  frexp(__longlong_as_double((long long)wmax_bits), &e);
+  return ldexp(1.0, 48 - e);
+}
+
+// v * scale as a 64-bit word and the carries above it
+__device__ __forceinline__ unsigned long long dep_fixed_split(double v, double scale,
+                                                              uint32_t* qh) {
+  const double t = v * scale;  // v >= 0; scale is a power of two
+  if (t < 18446744073709551616.0) {
+    *qh = 0;
+    return (unsigned long long)t;
+  }
+  const double h = floor(t * 5.421010862427522170e-20);  // beyond 64 bits
+  *qh = (uint32_t)h;
+  return (unsigned long long)(t - h * 18446744073709551616.0);
+}
+
+// LDS moments: slot 0 (weight) or 1 (energy) of [2][nn] words, 16-bit carry
+// fields of the node's carry word
+__device__ __forceinline__ void dep_fixed_add(unsigned long long* lo, uint32_t* hi, int slot,
+                                              int nn, int node, double v, double scale) {
+  uint32_t qh;
+  const unsigned long long q = dep_fixed_split(v, scale, &qh);
+  const unsigned long long old = atomicAdd(&lo[slot * nn + node], q);
+  const uint32_t carry = qh + (old + q < old ? 1u : 0u);
+  if (carry) atomicAdd(&hi[node], carry << (16 * slot));
+}
+
+__device__ __forceinline__ double dep_fixed_value(unsigned long long lo, uint32_t carries,
+                                                  double inv_scale) {
+  return ((double)carries * 18446744073709551616.0 + (double)lo) * inv_scale;
+}
+
+__device__ __forceinline__ uint32_t dep_carries(uint32_t hi, int slot) {
+  return (hi >> (16 * slot)) & 0xFFFFu;
+}
+
+// a carry field past half its range: the block's sums are close to wrapping
+__device__ __forceinline__ bool dep_near_range(uint32_t hi) {
+  return ((hi >> 15) & 1u) || ((hi >> 31) & 1u);
+}
+
+// global moment 1: the node's 64-bit word and its own 32-bit carry word
+__device__ __forceinline__ void dep_fixed_add_global(unsigned long long* lo, uint32_t* hi,
+                                                     int node, double v, double scale) {
+  uint32_t qh;
+  const unsigned long long q = dep_fixed_split(v, scale, &qh);
+  const unsigned long long old = atomicAdd(&lo[node], q);
+  const uint32_t carry = qh + (old + q < old ? 1u : 0u);
+  if (carry) atomicAdd(&hi[node], carry);
+}
 
 }  // namespace bz

@@ -33,6 +33,8 @@ struct LaunchEnv {
   int push_sched;            // 0 static tiles + cooperative launch, 1 ticket counter
   int push_bpc;              // 1-D push: blocks per CU (0: as many as fit)
   int push_queue;            // 1-D push candidate queue: 0 auto, 1 always, -1 never
+  unsigned long long* dep_m1_lo;  // 1-D push: global fixed-point ionization sums [max_nodes]
+  uint32_t* dep_m1_hi;            //   and their carry words (zero between pushes)
   int push_keys_nb;          // 1-D push writes binning keys for nb bins (0: off)
   double push_keys_thr[8];   // their energy thresholds
   // optional kernel timing: event pairs per timed kernel class
@@ -62,6 +64,8 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
                 const void* G, double* moments, int shape);
 int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                    double* moments);
+// the weight bound of a state into the clock (the deposit's fixed-point scale)
+int launch_dep_wmax(const LaunchEnv& env, const bz_state_t* s);
 int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out);
 // sum the push's per-block partial rows (m doubles each) into out[m];
 // commit != 0 also sums the counter rows and commits the clock
@@ -93,6 +97,13 @@ int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t
                     const uint32_t* perm, const int32_t* targets, int32_t n_keys,
                     const bz_resample_policy_t* pol);
 size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys);
+// in-place resampling (reference layout) of a state whose slot keys are in
+// the sort input (written by the push, or by launch_slot_keys)
+int launch_resample_inplace(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                            const uint32_t* keys, int nb, const int32_t* targets,
+                            int32_t n_keys, const bz_resample_policy_t* pol);
+int launch_slot_keys(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                     const double* thr, int nb);
 // per-block partial sums of the push: node histograms (max_nodes) or
 // per-instance moments (batched swarms, up to max_keys instances)
 int64_t push_partials_doubles(int num_cu, int32_t max_nodes, int32_t max_keys);

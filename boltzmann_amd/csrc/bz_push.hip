@@ -53,6 +53,10 @@ struct PushArgs {
   // of the next bin sort
   uint32_t* keys;
   BinKeys bk;
+  // 1-D: the ionization moment's global fixed-point words (k_push_post
+  // reads and clears them)
+  unsigned long long* m1_lo;
+  uint32_t* m1_hi;
 };
 
 // per-block counter row of the push (u32): births, absorbed, elastic,
@@ -328,10 +332,28 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs a) {
 constexpr int kRedWaves = 16;
 constexpr int kRedThreads = kRedWaves * 64;
 
+// m1 (optional): the 1-D push's global fixed-point ionization words, added
+// to outputs [nn, 2 nn) at scale m1_inv and cleared for the next push.
+struct M1Fold {
+  unsigned long long* lo;
+  uint32_t* hi;
+  int nn;
+  double inv;
+};
+
+__device__ __forceinline__ double m1_take(const M1Fold& f, int j) {
+  if (!f.lo || j < f.nn || j >= 2 * f.nn) return 0.0;
+  const int k = j - f.nn;
+  const double v = dep_fixed_value(f.lo[k], f.hi[k], f.inv);
+  f.lo[k] = 0ull;
+  f.hi[k] = 0u;
+  return v;
+}
+
 __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks, int m, int rpc,
                                 double* l2, uint32_t* col_done, double* __restrict__ out,
                                 double* acc, int cx, int cy, int ny, double (*s_part)[64],
-                                int* s_last) {
+                                int* s_last, const M1Fold& m1 = M1Fold{nullptr, nullptr, 0, 0.0}) {
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int j = cx * 64 + lane;
@@ -360,6 +382,7 @@ __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks
     for (int w = 0; w < kRedWaves; ++w) t += s_part[w][lane];
     if (ny == 1) {
       if (j < m) {
+        t += m1_take(m1, j);
         out[j] = t;
         if (acc) acc[j] += t;
       }
@@ -389,6 +412,7 @@ __device__ void dep_reduce_part(const double* __restrict__ partials, int nblocks
 #pragma unroll
         for (int w = 0; w < kRedWaves; ++w) u += s_part[w][lane];
         if (j < m) {
+          u += m1_take(m1, j);
           out[j] = u;
           if (acc) acc[j] += u;
         }
@@ -539,9 +563,10 @@ __device__ __forceinline__ void swarm_accumulate(const double mv[kImN], bool cou
   }
 }
 
-// LDS doubles before the staged table: DIM 1 histograms, DIM 2 moments and fields
+// LDS doubles before the staged table: DIM 1 fixed-point histograms (2 x nn
+// 64-bit sums, nn 32-bit carry words), DIM 2 moments and fields
 __host__ __device__ inline int inplace_lds_doubles(int dim, int nn, int n_inst, int word) {
-  if (dim == 1) return table_lds_offset_doubles(nn);
+  if (dim == 1) return (2 * nn + (nn + 1) / 2 + 1) & ~1;
   const int d = n_inst * kImN + (n_inst * word + 7) / 8;
   return (d + 1) & ~1;
 }
@@ -583,7 +608,16 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
     for (int k = 0; k < ITEMS; ++k)
       load5<T, true>(in, b0 + k * kWave + lane, t_first < t_end ? n : 0, cur[k]);
   }
-  for (int j = tid; j < sums; j += BLOCK) s_dep[j] = 0.0;
+  // DIM 1: fixed-point node sums (deterministic), DIM 2: FP64 moments
+  unsigned long long* s_lo = (unsigned long long*)s_dep;
+  uint32_t* s_hi = (uint32_t*)(s_lo + 2 * nn);
+  const double dscale = DIM == 1 ? dep_scale_of(a.clk->w_max_bits) : 1.0;
+  if (DIM == 1) {
+    for (int j = tid; j < 2 * nn; j += BLOCK) s_lo[j] = 0ull;
+    for (int j = tid; j < nn; j += BLOCK) s_hi[j] = 0u;
+  } else {
+    for (int j = tid; j < sums; j += BLOCK) s_dep[j] = 0.0;
+  }
   T* s_ie = (T*)(s_dep + sums);  // DIM 2: field of every instance
   if (DIM == 2)
     for (int j = tid; j < n_inst; j += BLOCK) s_ie[j] = ((const T*)a.inst_E)[j];
@@ -710,14 +744,16 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
             st[3][i] = o.p.vz;
             const double w = (double)o.p.w;
             const int nd = dep_node((double)o.p.x, p);
-            atomicAdd(&s_dep[nd], w);
-            if (o.chan == kChanIon) atomicAdd(&s_dep[nn + nd], w);
-            atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
+            dep_fixed_add(s_lo, s_hi, 0, nn, nd, w, dscale);
+            if (o.chan == kChanIon) dep_fixed_add_global(a.m1_lo, a.m1_hi, nd, w, dscale);
+            dep_fixed_add(s_lo, s_hi, 1, nn, nd, w * (double)o.e_post, dscale);
           } else {
             st[4][i] = (T)0;
+            if (DIM == 1 && o.chan == kChanIon && (p.ref_flags & BZ_REF_DEAD_IONIZER))
+              dep_fixed_add_global(a.m1_lo, a.m1_hi, 0, 1.0, dscale);  // the reference's flag
           }
           if (DIM == 1 && a.keys)
-            a.keys[i] = o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post) : (uint32_t)a.bk.n_keys;
+            a.keys[i] = o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post, o.p.w) : (uint32_t)a.bk.n_keys;
         }
         // births: the batch's slots rise, so equal tiles are runs of lanes
         // (segments), taken in order
@@ -740,8 +776,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
             if (DIM == 1) {
               const double w = (double)o.s.w;
               const int nd = dep_node((double)o.s.x, p);
-              atomicAdd(&s_dep[nd], w);
-              atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+              dep_fixed_add(s_lo, s_hi, 0, nn, nd, w, dscale);
+              dep_fixed_add(s_lo, s_hi, 1, nn, nd, w * (double)o.e_sec, dscale);
             }
           }
           c_open += (uint32_t)__popcll(m & seg);
@@ -797,8 +833,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
               st[1][i] = o.p.vx;
               const double w = (double)o.p.w;
               const int nd = dep_node((double)o.p.x, p);
-              atomicAdd(&s_dep[nd], w);
-              atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
+              dep_fixed_add(s_lo, s_hi, 0, nn, nd, w, dscale);
+              dep_fixed_add(s_lo, s_hi, 1, nn, nd, w * (double)o.e_post, dscale);
             } else {
               ++c_abs;
               st[4][i] = (T)0;
@@ -806,7 +842,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           }
           // keys of the slots settled here (candidates get theirs in their batch)
           if (DIM == 1 && a.keys && i < n && !(live && cand))
-            a.keys[i] = plain && o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post)
+            a.keys[i] = plain && o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post, o.p.w)
                                          : (uint32_t)a.bk.n_keys;
           if (DIM == 2) {
             // running moments of the wave's instance (the queued candidates
@@ -876,15 +912,18 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           st[3][i] = o.p.vz;
           const double w = (double)o.p.w;
           const int nd = dep_node((double)o.p.x, p);
-          atomicAdd(&s_dep[nd], w);
-          if (o.chan == kChanIon) atomicAdd(&s_dep[nn + nd], w);
-          atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_post);
+          dep_fixed_add(s_lo, s_hi, 0, nn, nd, w, dscale);
+          if (o.chan == kChanIon) dep_fixed_add_global(a.m1_lo, a.m1_hi, nd, w, dscale);
+          dep_fixed_add(s_lo, s_hi, 1, nn, nd, w * (double)o.e_post, dscale);
         } else {
           st[4][i] = (T)0;
+          if (DIM == 1 && o.chan == kChanIon && (p.ref_flags & BZ_REF_DEAD_IONIZER))
+            dep_fixed_add_global(a.m1_lo, a.m1_hi, 0, 1.0, dscale);  // the reference's flag
         }
       }
       if (DIM == 1 && a.keys && i < n)
-        a.keys[i] = live && o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post) : (uint32_t)a.bk.n_keys;
+        a.keys[i] = live && o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post, o.p.w)
+                                   : (uint32_t)a.bk.n_keys;
       if (DIM == 2) {
         // per-instance moments of this step; a wave whose lanes share one
         // instance (the common case: instances are contiguous between
@@ -901,8 +940,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
         if (DIM == 1) {
           const double w = (double)o.s.w;
           const int nd = dep_node((double)o.s.x, p);
-          atomicAdd(&s_dep[nd], w);
-          atomicAdd(&s_dep[2 * nn + nd], w * (double)o.e_sec);
+          dep_fixed_add(s_lo, s_hi, 0, nn, nd, w, dscale);
+          dep_fixed_add(s_lo, s_hi, 1, nn, nd, w * (double)o.e_sec, dscale);
         }
       }
       tb += (uint32_t)__popcll(m);
@@ -931,7 +970,21 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   }
   __syncthreads();
   double* part = a.partials + (size_t)blockIdx.x * sums;
-  for (int j = tid; j < sums; j += BLOCK) part[j] = s_dep[j];
+  if (DIM == 1) {
+    // rows 0 and 2 from LDS; row 1 is summed in the global words
+    const double inv = 1.0 / dscale;
+    for (int j = tid; j < sums; j += BLOCK) {
+      const int m = j / nn, k = j - m * nn;
+      part[j] = m == 1 ? 0.0
+                       : dep_fixed_value(s_lo[(m >> 1) * nn + k], dep_carries(s_hi[k], m >> 1),
+                                         inv);
+    }
+    bool near = false;
+    for (int k = tid; k < nn; k += BLOCK) near |= dep_near_range(s_hi[k]);
+    if (__any(near) && lane == 0) atomicAdd(&a.clk->cnt[BZ_CNT_DEP_RANGE], 1ull);
+  } else {
+    for (int j = tid; j < sums; j += BLOCK) part[j] = s_dep[j];
+  }
   uint32_t* bcnt = (uint32_t*)(a.partials + (size_t)gridDim.x * sums);
   if (tid < kCntRow) bcnt[(size_t)blockIdx.x * kCntRow + tid] = (uint32_t)s_cnt[tid];
   if (tid == 0) a.block_births[blockIdx.x] = s_births;
@@ -1001,7 +1054,7 @@ __device__ void births_part(const PushArgs& a, int b, int nblk, int tile, int nw
         }
         if (a.keys) {
           const T e = (T)a.p.kev * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
-          a.keys[dst] = bin_key<T>(a.bk, v[0], e);
+          a.keys[dst] = bin_key<T>(a.bk, v[0], e, v[4]);
         }
       }
       run += tot;
@@ -1034,8 +1087,10 @@ __global__ __launch_bounds__(kRedThreads) void k_push_post(PushArgs a, int tile,
     births_part<T, kRedThreads>(a, b, nbirth, tile, nw, s_red);
   } else if (b < nbirth + nx * ny) {
     const int r = b - nbirth;
+    M1Fold f{a.m1_lo, a.m1_hi, a.p.n_nodes, 1.0 / dep_scale_of(a.clk->w_max_bits)};
+    if (!a.m1_lo) f.lo = nullptr;
     dep_reduce_part(a.partials, nbirth, m, rpc, l2, col_done, out, acc, r % nx, r / nx, ny,
-                    s_part, &s_last);
+                    s_part, &s_last, f);
   } else {
     dep_commit(a.partials, nbirth, m, a.clk, a.capacity, false, advance_dt);
   }
@@ -1046,18 +1101,6 @@ __global__ __launch_bounds__(kRedThreads) void k_push_post(PushArgs a, int tile,
 // of slots; k_compact_count counts its live ones, k_compact_scatter sums the
 // counts of blocks 0..b-1 itself and writes its live particles in slot order.
 // ---------------------------------------------------------------------------
-constexpr int kCompBlock = 256;
-constexpr int kCompItems = 4;
-constexpr int kCompTile = kCompBlock * kCompItems;
-
-__device__ __forceinline__ void comp_range(int64_t n, int64_t* lo, int64_t* hi) {
-  const int64_t tiles = (n + kCompTile - 1) / kCompTile;
-  const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
-  int64_t a = (int64_t)blockIdx.x * per * kCompTile;
-  int64_t b = a + per * kCompTile;
-  *lo = a < n ? a : n;
-  *hi = b < n ? b : n;
-}
 
 template <typename T>
 __global__ __launch_bounds__(kCompBlock) void k_compact_count(const T* __restrict__ w,
@@ -1131,17 +1174,26 @@ __global__ __launch_bounds__(kCompBlock) void k_compact_scatter(bz_state_t in, b
 
 __global__ void k_clock_commit_next(Clock* clk) { clk->n = clk->n_next; }
 
-// Deposit of a state without pushing it (initial moments).
+// Deposit of a state without pushing it (initial moments).  1-D: the
+// fixed-point node sums of the push (deterministic); 0-D: per-wave sums
+// added in wave order.
 template <typename T, int DIM, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_deposit(bz_params_t p, const T* __restrict__ x,
                                                   const T* __restrict__ vx,
                                                   const T* __restrict__ vy,
                                                   const T* __restrict__ vz,
-                                                  const T* __restrict__ w, const Clock* clk,
+                                                  const T* __restrict__ w, Clock* clk,
                                                   double* partials) {
   extern __shared__ double s_dep[];
+  __shared__ double s_w0[BLOCK / kWave][2];
   const int nn = DIM == 1 ? p.n_nodes : 1;
-  for (int j = threadIdx.x; j < 3 * nn; j += BLOCK) s_dep[j] = 0.0;
+  unsigned long long* s_lo = (unsigned long long*)s_dep;
+  uint32_t* s_hi = (uint32_t*)(s_lo + 2 * nn);
+  const double dscale = dep_scale_of(clk->w_max_bits);
+  if (DIM == 1) {
+    for (int j = threadIdx.x; j < 2 * nn; j += BLOCK) s_lo[j] = 0ull;
+    for (int j = threadIdx.x; j < nn; j += BLOCK) s_hi[j] = 0u;
+  }
   __syncthreads();
   const int64_t n = clk->n;
   double a0 = 0.0, a2 = 0.0;
@@ -1153,24 +1205,59 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(bz_params_t p, const T* __res
     const double wd = (double)wi;
     if (DIM == 1) {
       const int nd = dep_node((double)x[i], p);
-      atomicAdd(&s_dep[nd], wd);
-      atomicAdd(&s_dep[2 * nn + nd], wd * (double)e);
+      dep_fixed_add(s_lo, s_hi, 0, nn, nd, wd, dscale);
+      dep_fixed_add(s_lo, s_hi, 1, nn, nd, wd * (double)e, dscale);
     } else {
       a0 += wd;
       a2 += wd * (double)e;
     }
   }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (DIM == 0) {
     a0 = wave_sum(a0);
     a2 = wave_sum(a2);
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&s_dep[0], a0);
-      atomicAdd(&s_dep[2], a2);
+    if (lane == 0) {
+      s_w0[wid][0] = a0;
+      s_w0[wid][1] = a2;
     }
   }
   __syncthreads();
   double* part = partials + (size_t)blockIdx.x * 3 * nn;
-  for (int j = threadIdx.x; j < 3 * nn; j += BLOCK) part[j] = s_dep[j];
+  if (DIM == 1) {
+    const double inv = 1.0 / dscale;
+    for (int j = threadIdx.x; j < 3 * nn; j += BLOCK) {
+      const int m = j / nn, k = j - m * nn;  // no ionization without a push
+      part[j] = m == 1 ? 0.0
+                       : dep_fixed_value(s_lo[(m >> 1) * nn + k], dep_carries(s_hi[k], m >> 1),
+                                         inv);
+    }
+  } else if (threadIdx.x == 0) {
+    double t0 = 0.0, t2 = 0.0;
+    for (int q = 0; q < BLOCK / kWave; ++q) {
+      t0 += s_w0[q][0];
+      t2 += s_w0[q][1];
+    }
+    part[0] = t0;
+    part[1] = 0.0;
+    part[2] = t2;
+  }
+}
+
+// Upper bound of the weights of a state: the largest weight's bits, by
+// integer max (a positive double's bits order like the double), into the
+// clock's bound; the deposit's fixed-point scale follows from it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dep_wmax(const T* __restrict__ w, Clock* clk) {
+  const int64_t n = clk->n;
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double v = (double)w[i];
+    m = v > m ? v : m;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.0)
+    atomicMax(&clk->w_max_bits, (unsigned long long)__double_as_longlong(m));
 }
 
 }  // namespace bz
@@ -1461,6 +1548,8 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
   a.n_inst = 0;
   a.keys = nullptr;
   a.e_lds_off = 0;
+  a.m1_lo = p->dim == 1 ? env.dep_m1_lo : nullptr;
+  a.m1_hi = p->dim == 1 ? env.dep_m1_hi : nullptr;
   if (p->dim == 1 && env.push_keys_nb > 0) {
     a.keys = (uint32_t*)sort_input_keys(env);
     a.bk = bin_keys(p, env.push_keys_thr, env.push_keys_nb);
@@ -1478,7 +1567,7 @@ static int launch_deposit_t(const LaunchEnv& env, const bz_state_t* s, const bz_
   constexpr int BLOCK = 256;
   auto kern = k_deposit<T, DIM, BLOCK>;
   const int nn = DIM == 1 ? p->n_nodes : 1;
-  const size_t smem = (size_t)3 * nn * sizeof(double);
+  const size_t smem = (size_t)inplace_lds_doubles(1, nn, 0, (int)sizeof(T)) * sizeof(double);
   if (smem > 64 * 1024) {
     if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)smem) != hipSuccess)
@@ -1489,7 +1578,7 @@ static int launch_deposit_t(const LaunchEnv& env, const bz_state_t* s, const bz_
   if (grid > cap_parts) grid = (int)cap_parts;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), smem, env.stream, *p, (const T*)s->x,
                      (const T*)s->vx, (const T*)s->vy, (const T*)s->vz, (const T*)s->w,
-                     (const Clock*)env.clock, env.partials);
+                     env.clock, env.partials);
   int rc = check_launch("k_deposit");
   if (rc) return rc;
   return launch_dep_reduce(env, grid, 3 * nn, moments, 0, (int64_t)env.capacity);
@@ -1517,6 +1606,16 @@ int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t*
                                  : launch_compact_t<float>(env, in, out);
 }
 
+int launch_dep_wmax(const LaunchEnv& env, const bz_state_t* s) {
+  if (env.precision == BZ_F64)
+    hipLaunchKernelGGL(k_dep_wmax<double>, dim3(env.num_cu * 2), dim3(256), 0, env.stream,
+                       (const double*)s->w, env.clock);
+  else
+    hipLaunchKernelGGL(k_dep_wmax<float>, dim3(env.num_cu * 2), dim3(256), 0, env.stream,
+                       (const float*)s->w, env.clock);
+  return check_launch("k_dep_wmax");
+}
+
 int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                    double* moments) {
   if (env.precision == BZ_F64)
@@ -1524,72 +1623,6 @@ int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t*
                        : launch_deposit_t<double, 0>(env, s, p, moments);
   return p->dim == 1 ? launch_deposit_t<float, 1>(env, s, p, moments)
                      : launch_deposit_t<float, 0>(env, s, p, moments);
-}
-
-}  // namespace bz
-
-// ---------------------------------------------------------------------------
-// Reference reduction interface: r value columns histogrammed into coarse
-// rows (kernels/reduction/cuda_reduction/kernels-new.cu:56-132).  Fine bin
-// b = int(x*M), coarse row b/(M/RM); per-block LDS rows, summed over blocks
-// in a fixed order into rows 0..RM-1 of dw; the other rows are zeroed.
-// ---------------------------------------------------------------------------
-namespace bz {
-
-__global__ __launch_bounds__(256) void k_reduction(int r, int M, int RM, int bnf,
-                                                   const double* __restrict__ dv,
-                                                   const double* __restrict__ dx, int64_t ptcl,
-                                                   double* partials) {
-  extern __shared__ double s_h[];  // [RM][r]
-  const int m = RM * r;
-  for (int j = threadIdx.x; j < m; j += 256) s_h[j] = 0.0;
-  __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < ptcl;
-       i += (int64_t)gridDim.x * 256) {
-    const int b = (int)(dx[i] * (double)M);
-    int k = b / bnf;
-    k = k < 0 ? 0 : (k >= RM ? RM - 1 : k);
-    for (int j = 0; j < r; ++j) {
-      const double v = dv[(int64_t)r * i + j];
-      if (v != 0.0) atomicAdd(&s_h[r * k + j], v);
-    }
-  }
-  __syncthreads();
-  double* part = partials + (size_t)blockIdx.x * m;
-  for (int j = threadIdx.x; j < m; j += 256) part[j] = s_h[j];
-}
-
-__global__ void k_zero_rows(double* dw, int64_t from, int64_t to) {
-  for (int64_t j = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < to;
-       j += (int64_t)gridDim.x * blockDim.x)
-    dw[j] = 0.0;
-}
-
-int launch_reduction(const LaunchEnv& env, int r, int M, int RM, double* dw, const double* dv,
-                     const double* dx, int64_t ptcl) {
-  const int bnf = (M + RM - 1) / RM;  // ceil(M/RM), the redcheck block size
-  const int m = RM * r;
-  const size_t smem = (size_t)m * sizeof(double);
-  if (smem > 160 * 1024) return fail("reduction: RM*r = %d doubles exceed LDS", m);
-  if (smem > 64 * 1024) {
-    if (hipFuncSetAttribute((const void*)k_reduction, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem) != hipSuccess)
-      return fail("hipFuncSetAttribute(reduction LDS) failed");
-  }
-  int grid = env.num_cu * 2;
-  const int64_t cap_parts = env.partials_doubles / m;
-  if (grid > cap_parts) grid = (int)cap_parts;
-  if (grid < 1) return fail("reduction: partial buffer too small");
-  hipLaunchKernelGGL(k_reduction, dim3(grid), dim3(256), smem, env.stream, r, M, RM, bnf, dv, dx,
-                     ptcl, env.partials);
-  int rc = check_launch("k_reduction");
-  if (rc) return rc;
-  rc = launch_dep_reduce(env, grid, m, dw, 0, (int64_t)0);
-  if (rc) return rc;
-  if ((int64_t)M * r > m)
-    hipLaunchKernelGGL(k_zero_rows, dim3(64), dim3(256), 0, env.stream, dw, (int64_t)m,
-                       (int64_t)M * r);
-  return check_launch("k_zero_rows");
 }
 
 }  // namespace bz
@@ -1659,6 +1692,8 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
   a.n_inst = n_inst;
   a.keys = nullptr;
   a.e_lds_off = 0;
+  a.m1_lo = nullptr;
+  a.m1_hi = nullptr;
   PushLaunch pl;
   const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &pl)
                                          : launch_push_prec<float>(env, a, shape, &pl);

@@ -36,7 +36,22 @@ struct SortWs {
   int32_t kstride;
   int64_t max_tiles;
   int64_t max_chunks;
+  // in-place resampling: members of the split / merge segments, grouped by a
+  // sort of their own (keys and slots, two buffers each)
+  uint32_t* mkeys;
+  uint32_t* mslot;
+  uint32_t* mkeys2;
+  uint32_t* mslot2;
+  int32_t* ip_counts;   // [max_keys+1] slots per key
+  int32_t* ip_moff;     // [max_keys+2] member offsets per key
+  int32_t* ip_mcnt;     // [max_keys+1] members per key
+  int32_t* ip_cand;     // [max_keys] 0 untouched, 1 split candidate, 2 merge
+  uint32_t* ip_bcnt;    // members per extraction block
+  Clock* ip_mclk;       // member count (the radix kernels read it as a clock)
 };
+
+// blocks of the member extraction (contiguous slot ranges)
+constexpr int kIpBlocks = 2048;
 
 static int64_t tiles_for(int64_t cap) { return cap / kSortTile + 2; }
 
@@ -61,6 +76,16 @@ static SortWs carve(void* base, int64_t cap, int32_t max_keys) {
   w.sel = (uint32_t*)take(4 * cap);
   w.kstride = max_keys + 1;
   w.klist = (int32_t*)take(4 * (2 + 2 * (size_t)w.kstride));
+  w.mkeys = (uint32_t*)take(4 * cap);
+  w.mslot = (uint32_t*)take(4 * cap);
+  w.mkeys2 = (uint32_t*)take(4 * cap);
+  w.mslot2 = (uint32_t*)take(4 * cap);
+  w.ip_counts = (int32_t*)take(4 * ((size_t)max_keys + 1));
+  w.ip_moff = (int32_t*)take(4 * ((size_t)max_keys + 2));
+  w.ip_mcnt = (int32_t*)take(4 * ((size_t)max_keys + 1));
+  w.ip_cand = (int32_t*)take(4 * ((size_t)max_keys + 1));
+  w.ip_bcnt = (uint32_t*)take(4 * (size_t)kIpBlocks);
+  w.ip_mclk = (Clock*)take(sizeof(Clock));
   return w;
 }
 
@@ -80,6 +105,13 @@ size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys) {
   add(16 * (size_t)(max_keys + 1));
   add(4 * cap);
   add(4 * (2 + 2 * ((size_t)max_keys + 1)));
+  for (int k = 0; k < 4; ++k) add(4 * cap);
+  add(4 * ((size_t)max_keys + 1));
+  add(4 * ((size_t)max_keys + 2));
+  add(4 * ((size_t)max_keys + 1));
+  add(4 * ((size_t)max_keys + 1));
+  add(4 * (size_t)kIpBlocks);
+  add(sizeof(Clock));
   return b + 1024;
 }
 
@@ -107,7 +139,7 @@ template <typename T>
 __device__ __forceinline__ uint32_t slot_key(const KeyArgs& a, T x, T vx, T vy, T vz, T w) {
   if (!(w > (T)0)) return (uint32_t)a.bk.n_keys;
   const T e = (T)a.kev * (vx * vx + vy * vy + vz * vz);
-  return bin_key<T>(a.bk, x, e);
+  return bin_key<T>(a.bk, x, e, w);
 }
 
 template <typename T>
@@ -196,8 +228,12 @@ __global__ __launch_bounds__(256) void k_key_bounds(const uint32_t* __restrict__
                                                     int32_t* offsets) {
   const int64_t n = clk->n;
   const int lane = threadIdx.x & 63;
-  const int64_t base = (int64_t)blockIdx.x * kBoundsTile + (threadIdx.x - lane) * 4;
-  if (base > n) return;  // whole wave past the end (wave-uniform)
+  // positions 0..n (n closes the last keys), tiles of kBoundsTile walked by
+  // the grid: the work follows the count on the device, not the capacity
+  for (int64_t tb = (int64_t)blockIdx.x * kBoundsTile; tb <= n;
+       tb += (int64_t)gridDim.x * kBoundsTile) {
+  const int64_t base = tb + (threadIdx.x - lane) * 4;
+  if (base > n) continue;  // whole wave past the end (wave-uniform)
   int kc[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -215,6 +251,7 @@ __global__ __launch_bounds__(256) void k_key_bounds(const uint32_t* __restrict__
       const int hi = kc[k] < m ? kc[k] : m;
       for (int q = kp + 1; q <= hi; ++q) offsets[q] = (int32_t)s;
     }
+  }
   }
 }
 
@@ -237,56 +274,69 @@ __global__ void k_zero_i32(int32_t* p, int64_t m) {
 // ---------------------------------------------------------------------------
 // radix pass: upsweep histogram, scan, stable downsweep scatter
 // ---------------------------------------------------------------------------
+// tiles of kSortTile keys in the count the clock holds; the histogram of a
+// pass is laid out [digit][tile] with that many tiles per digit
+__device__ __forceinline__ int64_t sort_tiles(const Clock* clk) {
+  return (clk->n + kSortTile - 1) / kSortTile;
+}
+
 __global__ __launch_bounds__(kSortBlock) void k_upsweep(const uint32_t* keys, const Clock* clk,
-                                                        int shift, uint32_t* hist, int64_t tiles) {
+                                                        int shift, uint32_t* hist) {
   __shared__ uint32_t s_h[256];
-  s_h[threadIdx.x] = 0;
-  __syncthreads();
   const int64_t n = clk->n;
-  // thread t counts the kSortItems consecutive keys from base + t*kSortItems
-  // (the tile's histogram does not depend on the split) and adds one run of
-  // equal digits at a time: the keys arrive nearly sorted, so this cuts the
-  // LDS atomics on a few hot words by the run length
-  const int64_t i0 = (int64_t)blockIdx.x * kSortTile + (int64_t)threadIdx.x * kSortItems;
-  uint32_t kv[kSortItems];
-  if (i0 + kSortItems <= n) {
-    const uint4* k4 = (const uint4*)(keys + i0);
+  const int64_t tiles = sort_tiles(clk);
+  for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    __syncthreads();  // the previous tile's counts are written
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    // thread t counts the kSortItems consecutive keys from base + t*kSortItems
+    // (the tile's histogram does not depend on the split) and adds one run of
+    // equal digits at a time: nearly sorted keys cut the LDS atomics on a few
+    // hot words by the run length
+    const int64_t i0 = t * kSortTile + (int64_t)threadIdx.x * kSortItems;
+    uint32_t kv[kSortItems];
+    if (i0 + kSortItems <= n) {
+      const uint4* k4 = (const uint4*)(keys + i0);
 #pragma unroll
-    for (int q = 0; q < kSortItems / 4; ++q) {
-      const uint4 v = k4[q];
-      kv[4 * q] = v.x;
-      kv[4 * q + 1] = v.y;
-      kv[4 * q + 2] = v.z;
-      kv[4 * q + 3] = v.w;
+      for (int q = 0; q < kSortItems / 4; ++q) {
+        const uint4 v = k4[q];
+        kv[4 * q] = v.x;
+        kv[4 * q + 1] = v.y;
+        kv[4 * q + 2] = v.z;
+        kv[4 * q + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kSortItems; ++r) kv[r] = i0 + r < n ? keys[i0 + r] : 0u;
     }
-  } else {
+    uint32_t cur = 0u, run = 0u;
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) kv[r] = i0 + r < n ? keys[i0 + r] : 0u;
-  }
-  uint32_t cur = 0u, run = 0u;
-#pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    if (i0 + r < n) {
-      const uint32_t d = (kv[r] >> shift) & 255u;
-      if (run && d == cur) {
-        ++run;
-      } else {
-        if (run) atomicAdd(&s_h[cur], run);
-        cur = d;
-        run = 1u;
+    for (int r = 0; r < kSortItems; ++r) {
+      if (i0 + r < n) {
+        const uint32_t d = (kv[r] >> shift) & 255u;
+        if (run && d == cur) {
+          ++run;
+        } else {
+          if (run) atomicAdd(&s_h[cur], run);
+          cur = d;
+          run = 1u;
+        }
       }
     }
+    if (run) atomicAdd(&s_h[cur], run);
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * tiles + t] = s_h[threadIdx.x];
   }
-  if (run) atomicAdd(&s_h[cur], run);
-  __syncthreads();
-  hist[(int64_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
 // generic 3-phase exclusive scan of uint32 data[0..m)
-__global__ __launch_bounds__(1024) void k_chunk_reduce(const uint32_t* data, int64_t m,
+// (m = 256 digits x the pass's tiles, from the clock's count)
+__global__ __launch_bounds__(1024) void k_chunk_reduce(const uint32_t* data, const Clock* clk,
                                                        uint32_t* chunks) {
   __shared__ uint32_t s_w[16];
+  const int64_t m = 256 * sort_tiles(clk);
   const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+  if (base >= m) return;  // block-uniform
   uint32_t s = 0;
   for (int k = threadIdx.x; k < kScanChunk; k += 1024) {
     const int64_t i = base + k;
@@ -302,10 +352,11 @@ __global__ __launch_bounds__(1024) void k_chunk_reduce(const uint32_t* data, int
   }
 }
 
-__global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* chunks, int64_t nch) {
+__global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* chunks, const Clock* clk) {
   // sequential carry over 1024-wide block scans
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
+  const int64_t nch = (256 * sort_tiles(clk) + kScanChunk - 1) / kScanChunk;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   if (tid == 0) s_carry = 0;
   __syncthreads();
@@ -329,12 +380,14 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* chunks, int64_t n
   }
 }
 
-__global__ __launch_bounds__(1024) void k_chunk_apply(uint32_t* data, int64_t m,
+__global__ __launch_bounds__(1024) void k_chunk_apply(uint32_t* data, const Clock* clk,
                                                       const uint32_t* chunks) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t m = 256 * sort_tiles(clk);
   const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+  if (base >= m) return;  // block-uniform
   if (tid == 0) s_carry = chunks[blockIdx.x];
   __syncthreads();
   for (int k = 0; k < kScanChunk; k += 1024) {
@@ -368,7 +421,7 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(const uint32_t* keys_i
                                                           const uint32_t* vals_in,
                                                           uint32_t* keys_out, uint32_t* vals_out,
                                                           const Clock* clk, int shift,
-                                                          const uint32_t* hist, int64_t tiles,
+                                                          const uint32_t* hist,
                                                           int identity_vals) {
   constexpr int NW = kSortBlock / kWave;
   static_assert(kSortBlock == 256, "one digit per thread in the offset pass");
@@ -376,55 +429,86 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(const uint32_t* keys_i
   __shared__ uint32_t s_wc[NW][256];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int64_t n = clk->n;
-  const int64_t t = blockIdx.x;
-  s_base[tid] = hist[(int64_t)tid * tiles + t];
-#pragma unroll
-  for (int j = 0; j < 256 / kWave; ++j) s_wc[wid][j * kWave + lane] = 0u;
-  const int64_t w0 = t * kSortTile + (int64_t)wid * (kSortItems * kWave);
-  uint32_t key[kSortItems], val[kSortItems], pre[kSortItems];
-#pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = w0 + r * kWave + lane;
-    const bool valid = i < n;
-    key[r] = valid ? keys_in[i] : 0u;
-    val[r] = valid ? (identity_vals ? (uint32_t)i : vals_in[i]) : 0u;
-  }
+  const int64_t tiles = sort_tiles(clk);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    __syncthreads();  // the previous tile is done with the LDS
+    s_base[tid] = hist[(int64_t)tid * tiles + t];
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    const bool valid = w0 + r * kWave + lane < n;
-    const uint32_t d = (key[r] >> shift) & 255u;
-    uint64_t m = __ballot(valid);
+    for (int j = 0; j < 256 / kWave; ++j) s_wc[wid][j * kWave + lane] = 0u;
+    const int64_t w0 = t * kSortTile + (int64_t)wid * (kSortItems * kWave);
+    uint32_t key[kSortItems], val[kSortItems], pre[kSortItems];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t bb = __ballot((d >> b) & 1u);
-      m &= ((d >> b) & 1u) ? bb : ~bb;
+    for (int r = 0; r < kSortItems; ++r) {
+      const int64_t i = w0 + r * kWave + lane;
+      const bool valid = i < n;
+      key[r] = valid ? keys_in[i] : 0u;
+      val[r] = valid ? (identity_vals ? (uint32_t)i : vals_in[i]) : 0u;
     }
-    const uint32_t rank = (uint32_t)__popcll(m & lt);
-    const uint32_t before = s_wc[wid][d];  // this wave's items of digit d in earlier rows
-    pre[r] = before + rank;
-    if (valid && rank == 0) s_wc[wid][d] = before + (uint32_t)__popcll(m);
-  }
-  __syncthreads();
-  {
-    uint32_t acc = 0u;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      const uint32_t c = s_wc[w][tid];
-      s_wc[w][tid] = acc;
-      acc += c;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    if (w0 + r * kWave + lane < n) {
+    for (int r = 0; r < kSortItems; ++r) {
+      const bool valid = w0 + r * kWave + lane < n;
       const uint32_t d = (key[r] >> shift) & 255u;
-      const uint32_t off = s_base[d] + s_wc[wid][d] + pre[r];
-      keys_out[off] = key[r];
-      vals_out[off] = val[r];
+      uint64_t m = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const uint64_t bb = __ballot((d >> b) & 1u);
+        m &= ((d >> b) & 1u) ? bb : ~bb;
+      }
+      const uint32_t rank = (uint32_t)__popcll(m & lt);
+      const uint32_t before = s_wc[wid][d];  // this wave's items of digit d in earlier rows
+      pre[r] = before + rank;
+      if (valid && rank == 0) s_wc[wid][d] = before + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    {
+      uint32_t acc = 0u;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        const uint32_t c = s_wc[w][tid];
+        s_wc[w][tid] = acc;
+        acc += c;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+      if (w0 + r * kWave + lane < n) {
+        const uint32_t d = (key[r] >> shift) & 255u;
+        const uint32_t off = s_base[d] + s_wc[wid][d] + pre[r];
+        keys_out[off] = key[r];
+        vals_out[off] = val[r];
+      }
     }
   }
+}
+
+// One radix pass over the count in `clk`: histogram, scan, stable scatter.
+// Grids are sized to what is resident and walk the tiles the count needs.
+static void radix_pass(const LaunchEnv& env, const SortWs& ws, const Clock* clk,
+                       const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                       int shift, int identity) {
+  const int64_t tg = ws.max_tiles < (int64_t)env.num_cu * 8 ? ws.max_tiles
+                                                            : (int64_t)env.num_cu * 8;
+  hipLaunchKernelGGL(k_upsweep, dim3((unsigned)tg), dim3(kSortBlock), 0, env.stream, kin, clk,
+                     shift, ws.hist);
+  hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)ws.max_chunks), dim3(1024), 0, env.stream,
+                     (const uint32_t*)ws.hist, clk, ws.chunks);
+  hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, env.stream, ws.chunks, clk);
+  hipLaunchKernelGGL(k_chunk_apply, dim3((unsigned)ws.max_chunks), dim3(1024), 0, env.stream,
+                     ws.hist, clk, (const uint32_t*)ws.chunks);
+  timing_begin(env, 4);
+  hipLaunchKernelGGL(k_downsweep, dim3((unsigned)tg), dim3(kSortBlock), 0, env.stream, kin, vin,
+                     kout, vout, clk, shift, (const uint32_t*)ws.hist, identity);
+  timing_end(env, 4);
+}
+
+static void key_bounds(const LaunchEnv& env, const Clock* clk, const uint32_t* skeys, int mk,
+                       int32_t* offsets) {
+  const int64_t bmax = (env.capacity + kBoundsTile) / kBoundsTile;
+  const int64_t bg = bmax < (int64_t)env.num_cu * 8 ? bmax : (int64_t)env.num_cu * 8;
+  hipLaunchKernelGGL(k_key_bounds, dim3((unsigned)bg), dim3(256), 0, env.stream, skeys, clk, mk,
+                     offsets);
 }
 
 // Stable sort of the keys in ws.keys_a (slots [0, n)) over key values
@@ -435,44 +519,24 @@ static int sort_keys(const LaunchEnv& env, const SortWs& ws, int n_keys, int32_t
                      int32_t* offsets, uint32_t* perm, const uint32_t** skeys) {
   if (n_keys + 1 > 65535) return fail("sort: at most 65534 keys (two 8-bit passes)");
   const int passes = (n_keys + 1) <= 256 ? 1 : 2;
-  const int64_t tiles = ws.max_tiles;
-  const int64_t m = 256 * tiles;
-  const int64_t nch = (m + kScanChunk - 1) / kScanChunk;
   // the first of two passes parks the permutation in vals_b, or in sel when
   // the caller's perm is vals_b itself (sel is free until the resampling)
   uint32_t* park = perm == ws.vals_b ? ws.sel : ws.vals_b;
   uint32_t* kin = ws.keys_a;
   const uint32_t* vin = nullptr;
-  int rc = 0;
   for (int ps = 0; ps < passes; ++ps) {
-    const int shift = 8 * ps;
     const bool last = ps == passes - 1;
     uint32_t* kout = ps == 0 ? ws.keys_b : ws.keys_a;
     uint32_t* vout = last ? perm : park;
-    // every upsweep block writes all 256 of its counts: no clearing needed
-    hipLaunchKernelGGL(k_upsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
-                       (const uint32_t*)kin, (const Clock*)env.clock, shift, ws.hist, tiles);
-    hipLaunchKernelGGL(k_chunk_reduce, dim3((unsigned)nch), dim3(1024), 0, env.stream,
-                       (const uint32_t*)ws.hist, m, ws.chunks);
-    hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, env.stream, ws.chunks, nch);
-    hipLaunchKernelGGL(k_chunk_apply, dim3((unsigned)nch), dim3(1024), 0, env.stream, ws.hist, m,
-                       (const uint32_t*)ws.chunks);
-    timing_begin(env, 4);
-    hipLaunchKernelGGL(k_downsweep, dim3((unsigned)tiles), dim3(kSortBlock), 0, env.stream,
-                       (const uint32_t*)kin, vin, kout, vout, (const Clock*)env.clock, shift,
-                       (const uint32_t*)ws.hist, tiles, ps == 0 ? 1 : 0);
-    timing_end(env, 4);
-    rc = check_launch("radix pass");
+    radix_pass(env, ws, env.clock, kin, vin, kout, vout, 8 * ps, ps == 0 ? 1 : 0);
+    const int rc = check_launch("radix pass");
     if (rc) return rc;
     kin = kout;
     vin = vout;
   }
   // offsets of every key from the sorted keys, then the counts
   const int mk = n_keys + 1;  // key values 0..n_keys (n_keys: dead slots)
-  // positions 0..n (n itself closes the last keys); waves past n exit
-  const int64_t bgrid = (env.capacity + kBoundsTile) / kBoundsTile;
-  hipLaunchKernelGGL(k_key_bounds, dim3((unsigned)bgrid), dim3(256), 0, env.stream,
-                     (const uint32_t*)kin, (const Clock*)env.clock, mk, offsets);
+  key_bounds(env, env.clock, kin, mk, offsets);
   hipLaunchKernelGGL(k_key_counts, dim3((mk + 255) / 256), dim3(256), 0, env.stream,
                      (const int32_t*)offsets, mk, counts, env.clock);
   if (skeys) *skeys = kin;
@@ -488,6 +552,7 @@ BinKeys bin_keys(const bz_params_t* p, const double* thr, int nb) {
   bk.n_nodes = n_nodes;
   bk.n_keys = n_nodes * nb;
   bk.dim = p->dim;
+  bk.ew = (p->ref_flags & BZ_REF_ENERGY_ROUNDTRIP) ? 1 : 0;
   return bk;
 }
 
@@ -595,6 +660,8 @@ struct ResArgs {
   uint64_t seed;
   int n_keys;
   int prec;
+  const double* U;   // caller's uniforms U[key*u_stride + m], or NULL: Philox
+  int64_t u_stride;
 };
 
 // sorted keys of the final pass (keys_a for 2 passes, keys_b for 1)
@@ -862,6 +929,12 @@ __device__ __forceinline__ double res_uniform(uint64_t seed, uint64_t step, uint
   return (m & 1) ? u53(r.c, r.d) : u53(r.a, r.b);
 }
 
+// uniform m of segment `key`: the caller's stream when given, else Philox
+__device__ __forceinline__ double seg_uniform(const ResArgs& a, uint64_t step, uint32_t key,
+                                              uint32_t m) {
+  return a.U ? a.U[(int64_t)key * a.u_stride + m] : res_uniform(a.seed, step, key, m);
+}
+
 // Split: weighted draws with replacement among w > min_wt; each drawn
 // particle's weight is divided by (copies+1) and its copies follow it.
 // Segments of up to kSplitLds particles keep the CDF, copy counts and output
@@ -869,7 +942,7 @@ __device__ __forceinline__ double res_uniform(uint64_t seed, uint64_t step, uint
 // threads: output slot o belongs to the particle whose offset range holds o.
 constexpr int kSplitLds = 3968;  // 3968 * 16 B stays under the 64 KiB default LDS limit
 
-template <typename T>
+template <typename T, bool INPLACE>
 __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int32_t* counts,
                                               const int32_t* offsets, const uint32_t* perm,
                                               const int32_t* plan, const SortWs& ws,
@@ -903,19 +976,21 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
     return w > minw ? (int64_t)ldexp(w, S) : 0;
   };
   const int64_t Wq = block_fixed_scan(0, c, cdf, q, s_w64);
-  const int draws = plan[4 * k + 1] - c;
+  // key order: plan holds the segment's new count; in place: the clones
+  const int draws = INPLACE ? plan[4 * k + 1] : plan[4 * k + 1] - c;
   for (int m = threadIdx.x; m < draws; m += kSegBlock) {
-    const double u = res_uniform(a.seed, step, (uint32_t)k, (uint32_t)m);
+    const double u = seg_uniform(a, step, (uint32_t)k, (uint32_t)m);
     const int64_t thr = (int64_t)(u * (double)Wq);
     atomicAdd(&copies[search_cdf(cdf, 0, c, thr, true)], 1);
   }
   __syncthreads();
-  // exclusive offsets of (1 + copies) in segment order
+  // exclusive offsets of (1 + copies) in segment order (in place: of the
+  // clones alone, the particles themselves keep their slots)
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int carry = 0;
   for (int base = 0; base < c; base += kSegBlock) {
     const int j = base + threadIdx.x;
-    const int reps = j < c ? 1 + copies[j] : 0;
+    const int reps = j < c ? (INPLACE ? 0 : 1) + copies[j] : 0;
     const int inc = wave_incl_scan(reps);
     __syncthreads();
     if (lane == 63) s_wi[wid] = inc;
@@ -926,12 +1001,12 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
     for (int w = 0; w < kSegWaves; ++w) carry += s_wi[w];
   }
   __syncthreads();
-  const int total = carry;  // = c + draws
+  const int total = carry;  // key order: c + draws; in place: draws
   const int64_t out0 = plan[4 * k + 2];
   for (int o = threadIdx.x; o < total; o += kSegBlock) {
     const int64_t dst = out0 + o;
     if (dst >= cap) break;
-    // largest j with offs[j] <= o
+    // largest j with offs[j] <= o: the member whose output range holds o
     int lo = 0, hi = c - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -946,6 +1021,16 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
     for (int f = 0; f < 4; ++f) ((T*)a.out[f])[dst] = ((const T*)a.in[f])[src];
     ((T*)a.out[4])[dst] = w;
   }
+  if (INPLACE) {
+    // the drawn particles keep their slots with the divided weight, written
+    // once every clone has read the original
+    __syncthreads();
+    for (int j = threadIdx.x; j < c; j += kSegBlock) {
+      if (!copies[j]) continue;
+      const uint32_t src = perm[s0 + j];
+      ((T*)a.out[4])[src] = Wt[src] / (T)(1 + copies[j]);
+    }
+  }
 }
 
 // Merge: keep K = int(merge_keep*target) particles chosen by stratified
@@ -953,16 +1038,17 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
 // the mean so that weight, momentum and energy are conserved.
 constexpr int kMergeLds = 6144;  // 48 KiB of CDF
 
-template <typename T>
+template <typename T, bool INPLACE>
 __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int32_t* counts,
                                               const int32_t* offsets, const uint32_t* perm,
                                               const int32_t* targets, const int32_t* plan,
-                                              const SortWs& ws, const Clock* clk, int64_t cap) {
+                                              const SortWs& ws, Clock* clk, int64_t cap) {
   __shared__ double s_red[5 * kSegWaves];
   __shared__ int64_t s_w64[kSegWaves];
   const int c = counts[k];
   const int ce = (a.pol.max_ppc > 0 && c > a.pol.max_ppc) ? a.pol.max_ppc : c;
-  const int K = plan[4 * k + 1] - (c - ce);
+  // key order: plan holds the new count (K and the clipped tail); in place: K
+  const int K = INPLACE ? plan[4 * k + 1] : plan[4 * k + 1] - (c - ce);
   const int64_t s0 = offsets[k];
   const int64_t out0 = plan[4 * k + 2];
   const uint64_t step = clk->step;
@@ -1008,10 +1094,14 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
   int64_t* cdf = lds ? s_mcdf : ws.cdf + s0;
   const int64_t Wq = block_fixed_scan(0, ce, cdf, q, s_w64);
   const double wbar = W / (double)K;
+  // the merged weight may exceed every weight so far: raise the bound the
+  // deposit scales its fixed-point sums by
+  if (threadIdx.x == 0)
+    atomicMax(&clk->w_max_bits, (unsigned long long)__double_as_longlong((double)(T)wbar));
   // stratified selection: r_k = (1/K)(U_k + k), searchsorted left
   double sx = 0, sy = 0, sz = 0, se = 0;
   for (int m = threadIdx.x; m < K; m += kSegBlock) {
-    const double u = res_uniform(a.seed, step, (uint32_t)k, (uint32_t)m);
+    const double u = seg_uniform(a, step, (uint32_t)k, (uint32_t)m);
     const double r = (1.0 / (double)K) * (u + (double)m);
     const int64_t thr = (int64_t)(r * (double)Wq);
     const int j = search_cdf(cdf, 0, ce, thr, false);
@@ -1060,12 +1150,17 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
     ((T*)a.out[3])[dst] = (T)(mz + ((double)VZ[i] - hz) * f);
     ((T*)a.out[4])[dst] = (T)wbar;
   }
+  if (INPLACE) {
+    // the merged members become holes once every selection has been read
+    __syncthreads();
+    for (int j = threadIdx.x; j < ce; j += kSegBlock) ((T*)a.out[4])[perm[s0 + j]] = (T)0;
+  }
 }
 
 // Split / merge over the keys k_res_plan listed: a grid sized to what is
 // resident walks the list, so the many keys that only copy cost no block
 // (the split's 62 KiB of LDS lets two blocks share a CU).
-template <typename T>
+template <typename T, bool INPLACE>
 __global__ __launch_bounds__(kSegBlock) void k_res_split(ResArgs a, const int32_t* counts,
                                                    const int32_t* offsets, const uint32_t* perm,
                                                    const int32_t* plan, SortWs ws,
@@ -1073,20 +1168,20 @@ __global__ __launch_bounds__(kSegBlock) void k_res_split(ResArgs a, const int32_
   const int n_list = ws.klist[0];
   for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
     __syncthreads();  // the previous key is done with the LDS
-    res_split_key<T>(ws.klist[2 + li], a, counts, offsets, perm, plan, ws, clk, cap);
+    res_split_key<T, INPLACE>(ws.klist[2 + li], a, counts, offsets, perm, plan, ws, clk, cap);
   }
 }
 
-template <typename T>
+template <typename T, bool INPLACE>
 __global__ __launch_bounds__(kSegBlock) void k_res_merge(ResArgs a, const int32_t* counts,
                                                    const int32_t* offsets, const uint32_t* perm,
                                                    const int32_t* targets, const int32_t* plan,
-                                                   SortWs ws, const Clock* clk, int64_t cap) {
+                                                   SortWs ws, Clock* clk, int64_t cap) {
   const int n_list = ws.klist[1];
   for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
     __syncthreads();
-    res_merge_key<T>(ws.klist[2 + ws.kstride + li], a, counts, offsets, perm, targets, plan, ws,
-                     clk, cap);
+    res_merge_key<T, INPLACE>(ws.klist[2 + ws.kstride + li], a, counts, offsets, perm, targets,
+                              plan, ws, clk, cap);
   }
 }
 
@@ -1098,6 +1193,298 @@ __global__ void k_res_commit(Clock* clk, int64_t cap) {
   }
   clk->n = n;
   clk->n_next = n;
+}
+
+// ---------------------------------------------------------------------------
+// In-place resampling, the reference's layout (boltzmann.py:812-823,
+// :850-874): copy segments stay where they are, a split particle keeps its
+// slot with weight w/(c+1) and its clones are appended, a merged segment's
+// members become holes (w = 0) and its K particles are appended.  Only the
+// members of split / merge segments are grouped (extracted in slot order and
+// sorted by key), so a pass costs a key histogram, the extraction and a sort
+// of the members instead of a sort and a copy of the whole state.  Members
+// of a segment are its slots in ascending order, as after the full sort, so
+// the selections are those of the key-order resampler.
+// ---------------------------------------------------------------------------
+
+// Slots per key from the keys the push (or k_keys) wrote.  An LDS histogram
+// per block (lds_keys counters; 0: global atomics), added with integer
+// atomics: exact and order independent.
+constexpr int kIpHistBlock = 1024;  // one block per CU: the LDS histogram of every key
+
+__global__ __launch_bounds__(kIpHistBlock) void k_ip_hist(const uint32_t* __restrict__ keys,
+                                                          const Clock* clk, int32_t* counts,
+                                                          int lds_keys) {
+  extern __shared__ uint32_t s_c[];
+  const int64_t n = clk->n;
+  for (int j = threadIdx.x; j < lds_keys; j += kIpHistBlock) s_c[j] = 0u;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * kIpHistBlock * kKeyItems;
+  for (int64_t base = (int64_t)blockIdx.x * kIpHistBlock * kKeyItems + threadIdx.x; base < n;
+       base += stride) {
+    uint32_t kv[kKeyItems];
+#pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) {
+      const int64_t i = base + k * kIpHistBlock;
+      kv[k] = i < n ? keys[i] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) {
+      if (kv[k] == 0xFFFFFFFFu) continue;
+      if (lds_keys) atomicAdd(&s_c[kv[k]], 1u);
+      else atomicAdd(&counts[kv[k]], 1);
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < lds_keys; j += kIpHistBlock)
+    if (s_c[j]) atomicAdd(&counts[j], (int32_t)s_c[j]);
+}
+
+// The trigger and the candidates: cand[k] = 1 for a split candidate (its
+// heavy-particle count is not known yet), 2 for a merge, 0 otherwise; all 0
+// when the pass does not fire.  One block.
+__global__ __launch_bounds__(1024) void k_ip_plan1(ResArgs a, const int32_t* counts,
+                                                   const int32_t* targets, int nb,
+                                                   int32_t* cand) {
+  __shared__ int s_fire;
+  const bz_resample_policy_t& P = a.pol;
+  if (threadIdx.x == 0) s_fire = (!P.ref_trigger && P.trigger_at <= 0.0) ? 1 : 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < a.n_keys; k += 1024) {
+    const int c = counts[k], t = targets[k];
+    if (t <= 0 || c <= 0) continue;
+    if (P.ref_trigger) {
+      // only the last bin's maximum decides (3bin/boltzmann.py:1110-1113)
+      if ((k % nb) == nb - 1 && (double)c >= 3.0 * (double)t) s_fire = 1;
+    } else if ((double)c >= P.trigger_at * (double)t) {
+      s_fire = 1;
+    }
+  }
+  __syncthreads();
+  const int fire = s_fire;
+  for (int k = threadIdx.x; k < a.n_keys; k += 1024) {
+    const int c = counts[k], t = targets[k];
+    int kind = 0;
+    if (fire && t > 0) {
+      const int keep = (int)(P.merge_keep * (double)t);
+      if (c > 0 && (double)c < P.split_below * (double)t) {
+        if ((int)(P.split_to * (double)t) - c > 0) kind = 1;
+      } else if ((double)c >= P.merge_at * (double)t && keep >= 1) {
+        const int ce = (P.max_ppc > 0 && c > P.max_ppc) ? P.max_ppc : c;
+        if (keep < ce) kind = 2;
+      }
+    }
+    cand[k] = kind;
+  }
+  if (threadIdx.x == 0) cand[a.n_keys] = fire;  // read by the counters of k_ip_plan2
+}
+
+// Extraction of the members of candidate segments, stable: block b owns a
+// contiguous run of slots; k_ip_mcount counts its members, k_ip_mscatter
+// sums the counts of blocks 0..b-1 itself and writes (key, slot) of its
+// members in slot order.
+__device__ __forceinline__ bool ip_listed(const int32_t* cand, uint32_t key, int n_keys) {
+  return key < (uint32_t)n_keys && cand[key] != 0;
+}
+
+__global__ __launch_bounds__(kCompBlock) void k_ip_mcount(const uint32_t* __restrict__ keys,
+                                                          const Clock* clk,
+                                                          const int32_t* __restrict__ cand,
+                                                          int n_keys, uint32_t* bcnt) {
+  __shared__ uint32_t s_red[kCompBlock / kWave];
+  int64_t lo, hi;
+  comp_range(clk->n, &lo, &hi);
+  uint32_t c = 0;
+  // eight keys in flight per thread, then their eight candidate lookups
+  constexpr int U = 8;
+  for (int64_t b = lo; b < hi; b += (int64_t)U * kCompBlock) {
+    uint32_t kv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b + u * kCompBlock + threadIdx.x;
+      kv[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) c += ip_listed(cand, kv[u], n_keys) ? 1u : 0u;
+  }
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+__global__ __launch_bounds__(kCompBlock) void k_ip_mscatter(const uint32_t* __restrict__ keys,
+                                                            const Clock* clk,
+                                                            const int32_t* __restrict__ cand,
+                                                            int n_keys, const uint32_t* bcnt,
+                                                            uint32_t* mkeys, uint32_t* mslot,
+                                                            Clock* mclk) {
+  constexpr int NW = kCompBlock / kWave;
+  __shared__ uint32_t s_red[NW];
+  __shared__ uint64_t s_wsum[NW];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int64_t lo, hi;
+  comp_range(clk->n, &lo, &hi);
+  uint32_t s = 0;
+  for (int j = tid; j < (int)blockIdx.x; j += kCompBlock) s += bcnt[j];
+  s = wave_sum(s);
+  if (lane == 0) s_red[wid] = s;
+  __syncthreads();
+  int64_t pos0 = (int64_t)s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  for (int64_t base = lo; base < hi; base += kCompTile) {
+    uint64_t packed = 0;
+    uint32_t kv[kCompItems];
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      const int64_t i = base + k * kCompBlock + tid;
+      kv[k] = i < hi ? keys[i] : 0xFFFFFFFFu;
+      packed |= (uint64_t)(ip_listed(cand, kv[k], n_keys) ? 1u : 0u) << (16 * k);
+    }
+    const uint64_t incl = wave_incl_scan(packed);
+    if (lane == kWave - 1) s_wsum[wid] = incl;
+    __syncthreads();
+    uint64_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      if (w < wid) before += s_wsum[w];
+      tot += s_wsum[w];
+    }
+    const uint64_t excl = before + incl - packed;
+    uint32_t field_base = 0;
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      if ((packed >> (16 * k)) & 1u) {
+        const int64_t d = pos0 + field_base + (int64_t)((excl >> (16 * k)) & 0xFFFFu);
+        mkeys[d] = kv[k];
+        mslot[d] = (uint32_t)(base + k * kCompBlock + tid);
+      }
+      field_base += (uint32_t)((tot >> (16 * k)) & 0xFFFFu);
+    }
+    pos0 += field_base;
+    __syncthreads();  // s_wsum is rewritten by the next sub-tile
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == 0) {
+    mclk->n = pos0;
+    mclk->n_next = pos0;
+  }
+}
+
+// Heavy-particle counts (w > min_wt) of the split candidates, over their
+// grouped members; a grid sized to what is resident walks the keys.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ip_hw(ResArgs a, const int32_t* cand,
+                                               const int32_t* mcnt, const int32_t* moff,
+                                               const uint32_t* mslot, int32_t* plan) {
+  __shared__ int s_sum;
+  for (int k = blockIdx.x; k < a.n_keys; k += gridDim.x) {
+    if (cand[k] != 1) continue;  // block-uniform
+    __syncthreads();
+    if (threadIdx.x == 0) s_sum = 0;
+    __syncthreads();
+    const int c = mcnt[k];
+    int h = 0;
+    for (int j = threadIdx.x; j < c; j += 256)
+      h += wt<T>(a, mslot[moff[k] + j]) > a.pol.min_wt ? 1 : 0;
+    h = wave_sum(h);
+    if ((threadIdx.x & 63) == 0 && h) atomicAdd(&s_sum, h);
+    __syncthreads();
+    if (threadIdx.x == 0) plan[4 * k + 3] = s_sum;
+  }
+}
+
+// Final kinds, appended counts and their slots (in key order after the n
+// slots in use), the split / merge key lists in key order, the counters and
+// the new slot count.  One block; thread t owns a contiguous run of keys.
+__global__ __launch_bounds__(1024) void k_ip_plan2(ResArgs a, const int32_t* counts,
+                                                   const int32_t* targets, const int32_t* cand,
+                                                   int32_t* plan, Clock* clk, int32_t* klist,
+                                                   int kstride) {
+  __shared__ int64_t s_w[16];
+  __shared__ int s_ws[16], s_wm[16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const bz_resample_policy_t& P = a.pol;
+  const int per = (a.n_keys + 1023) / 1024;
+  const int k0 = tid * per;
+  const int k1 = min(k0 + per, a.n_keys);
+  // every thread's appended count per key first (the loads of all its keys
+  // in flight together), kept for the second pass
+  constexpr int kMaxPer = 64;  // n_keys <= 65534
+  int napp_k[kMaxPer];
+  int64_t app = 0;
+  int ns = 0, nm = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxPer; ++q) {
+    const int k = k0 + q;
+    napp_k[q] = 0;
+    if (k >= k1) continue;
+    const int kc = cand[k];
+    if (kc == 1) {
+      const int t = targets[k];
+      const int draws = (int)(P.split_to * (double)t) - counts[k];
+      if (plan[4 * k + 3] > 0 && draws > 0) {
+        napp_k[q] = draws;
+        ++ns;
+      }
+    } else if (kc == 2) {
+      napp_k[q] = (int)(P.merge_keep * (double)targets[k]);
+      ++nm;
+    }
+    app += napp_k[q];
+  }
+  // block exclusive scan of (app, ns, nm)
+  const int64_t ia = wave_incl_scan(app);
+  const int is = wave_incl_scan(ns);
+  const int im = wave_incl_scan(nm);
+  if (lane == 63) {
+    s_w[wid] = ia;
+    s_ws[wid] = is;
+    s_wm[wid] = im;
+  }
+  __syncthreads();
+  int64_t ea = ia - app;
+  int es = is - ns, em = im - nm;
+  int64_t ta = 0;
+  int ts = 0, tm = 0;
+  for (int w = 0; w < 16; ++w) {
+    if (w < wid) {
+      ea += s_w[w];
+      es += s_ws[w];
+      em += s_wm[w];
+    }
+    ta += s_w[w];
+    ts += s_ws[w];
+    tm += s_wm[w];
+  }
+  const int64_t n = clk->n;
+  int64_t off = n + ea;
+#pragma unroll
+  for (int q = 0; q < kMaxPer; ++q) {
+    const int k = k0 + q;
+    if (k >= k1) continue;
+    const int napp = napp_k[q];
+    int kind = kCopy;
+    if (napp > 0) {
+      if (cand[k] == 1) {
+        kind = kSplit;
+        klist[2 + es++] = k;
+      } else {
+        kind = kMerge;
+        klist[2 + kstride + em++] = k;
+      }
+    }
+    plan[4 * k + 0] = kind;
+    plan[4 * k + 1] = napp;
+    plan[4 * k + 2] = (int32_t)off;  // first appended slot (bounded by the capacity)
+    off += napp;
+  }
+  if (tid == 0) {
+    klist[0] = ts;
+    klist[1] = tm;
+    clk->n_next = n + ta;
+    if (ts) clk->cnt[BZ_CNT_SPLIT] += (unsigned long long)ts;
+    if (tm) clk->cnt[BZ_CNT_MERGE] += (unsigned long long)tm;
+    if (ts || tm || cand[a.n_keys] != 0) clk->cnt[BZ_CNT_RESAMPLE] += 1ull;
+  }
 }
 
 template <typename T>
@@ -1126,6 +1513,8 @@ static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const b
   a.seed = p->seed;
   a.n_keys = n_keys;
   a.prec = env.precision;
+  a.U = pol->uniforms;
+  a.u_stride = pol->u_stride;
   const int64_t cap = env.capacity;
   timing_begin(env, 2);
   hipLaunchKernelGGL(k_res_hw<T>, dim3(n_keys), dim3(256), 0, env.stream, a, counts, offsets,
@@ -1141,15 +1530,131 @@ static int launch_resample_t(const LaunchEnv& env, const bz_state_t* in, const b
   // at most n_keys listed keys; blocks beyond the list exit at once
   const int sgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
   const int mgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
-  hipLaunchKernelGGL(k_res_split<T>, dim3(sgrid), dim3(kSegBlock), (size_t)kSplitLds * 16,
+  hipLaunchKernelGGL((k_res_split<T, false>), dim3(sgrid), dim3(kSegBlock), (size_t)kSplitLds * 16,
                      env.stream, a, counts, offsets, perm, (const int32_t*)ws.plan, ws,
                      (const Clock*)env.clock, cap);
-  hipLaunchKernelGGL(k_res_merge<T>, dim3(mgrid), dim3(kSegBlock), (size_t)kMergeLds * 8,
+  hipLaunchKernelGGL((k_res_merge<T, false>), dim3(mgrid), dim3(kSegBlock), (size_t)kMergeLds * 8,
                      env.stream, a, counts, offsets, perm, targets, (const int32_t*)ws.plan, ws,
-                     (const Clock*)env.clock, cap);
+                     env.clock, cap);
   hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, cap);
   timing_end(env, 2);
   return check_launch("resample");
+}
+
+// Stable sort of the extracted members (ws.mkeys, ws.mslot; count in
+// ws.ip_mclk) by key, then the member offsets and counts of every key.
+static int sort_members(const LaunchEnv& env, const SortWs& ws, int n_keys) {
+  const int passes = (n_keys + 1) <= 256 ? 1 : 2;
+  uint32_t* kin = ws.mkeys;
+  uint32_t* vin = ws.mslot;
+  for (int ps = 0; ps < passes; ++ps) {
+    uint32_t* kout = ps == 0 ? ws.mkeys2 : ws.mkeys;
+    uint32_t* vout = ps == 0 ? ws.mslot2 : ws.mslot;
+    radix_pass(env, ws, ws.ip_mclk, kin, vin, kout, vout, 8 * ps, 0);
+    const int rc = check_launch("member radix pass");
+    if (rc) return rc;
+    kin = kout;
+    vin = vout;
+  }
+  // one pass leaves the members in the second buffers
+  if (passes == 1) {
+    hipMemcpyAsync(ws.mkeys, ws.mkeys2, 4 * (size_t)env.capacity, hipMemcpyDeviceToDevice,
+                   env.stream);
+    hipMemcpyAsync(ws.mslot, ws.mslot2, 4 * (size_t)env.capacity, hipMemcpyDeviceToDevice,
+                   env.stream);
+  }
+  const int mk = n_keys + 1;
+  key_bounds(env, ws.ip_mclk, ws.mkeys, mk, ws.ip_moff);
+  hipLaunchKernelGGL(k_key_counts, dim3((mk + 255) / 256), dim3(256), 0, env.stream,
+                     (const int32_t*)ws.ip_moff, mk, ws.ip_mcnt, ws.ip_mclk);
+  return check_launch("member offsets");
+}
+
+template <typename T>
+static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
+                                     const bz_params_t* p, const uint32_t* keys, int nb,
+                                     const int32_t* targets, int32_t n_keys,
+                                     const bz_resample_policy_t* pol) {
+  SortWs ws = carve(env.sort_ws, env.capacity, env.max_keys);
+  const int64_t cap = env.capacity;
+  if (cap > 0x7FFFFFFFll) return fail("resample in place: capacity above 2^31 slots");
+  ResArgs a;
+  const void* iv[5] = {s->x, s->vx, s->vy, s->vz, s->w};
+  for (int f = 0; f < 5; ++f) {
+    a.in[f] = iv[f];
+    a.out[f] = (void*)iv[f];  // in place
+  }
+  a.pol = *pol;
+  a.kev = p->kev;
+  a.seed = p->seed;
+  a.n_keys = n_keys;
+  a.prec = env.precision;
+  a.U = pol->uniforms;
+  a.u_stride = pol->u_stride;
+  timing_begin(env, 2);
+  // slots per key
+  const int m = n_keys + 1;
+  const int lds_keys = m <= 40960 ? m : 0;
+  if ((size_t)lds_keys * 4 > 64 * 1024 &&
+      hipFuncSetAttribute((const void*)k_ip_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          lds_keys * 4) != hipSuccess)
+    return fail("hipFuncSetAttribute(key histogram LDS) failed");
+  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, ws.ip_counts, (int64_t)m);
+  hipLaunchKernelGGL(k_ip_hist, dim3(env.num_cu), dim3(kIpHistBlock), (size_t)lds_keys * 4,
+                     env.stream, keys, (const Clock*)env.clock, ws.ip_counts, lds_keys);
+  hipLaunchKernelGGL(k_ip_plan1, dim3(1), dim3(1024), 0, env.stream, a,
+                     (const int32_t*)ws.ip_counts, targets, nb, ws.ip_cand);
+  // members of the candidate segments, in slot order, then grouped by key
+  hipLaunchKernelGGL(k_ip_mcount, dim3(kIpBlocks), dim3(kCompBlock), 0, env.stream, keys,
+                     (const Clock*)env.clock, (const int32_t*)ws.ip_cand, n_keys, ws.ip_bcnt);
+  hipLaunchKernelGGL(k_ip_mscatter, dim3(kIpBlocks), dim3(kCompBlock), 0, env.stream, keys,
+                     (const Clock*)env.clock, (const int32_t*)ws.ip_cand, n_keys,
+                     (const uint32_t*)ws.ip_bcnt, ws.mkeys, ws.mslot, ws.ip_mclk);
+  int rc = check_launch("member extraction");
+  if (rc) return rc;
+  rc = sort_members(env, ws, n_keys);
+  if (rc) return rc;
+  const int sgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
+  hipLaunchKernelGGL(k_ip_hw<T>, dim3(sgrid), dim3(256), 0, env.stream, a,
+                     (const int32_t*)ws.ip_cand, (const int32_t*)ws.ip_mcnt,
+                     (const int32_t*)ws.ip_moff, (const uint32_t*)ws.mslot, ws.plan);
+  hipLaunchKernelGGL(k_ip_plan2, dim3(1), dim3(1024), 0, env.stream, a,
+                     (const int32_t*)ws.ip_counts, targets, (const int32_t*)ws.ip_cand, ws.plan,
+                     env.clock, ws.klist, ws.kstride);
+  hipLaunchKernelGGL((k_res_split<T, true>), dim3(sgrid), dim3(kSegBlock), (size_t)kSplitLds * 16,
+                     env.stream, a, (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
+                     (const uint32_t*)ws.mslot, (const int32_t*)ws.plan, ws,
+                     (const Clock*)env.clock, cap);
+  hipLaunchKernelGGL((k_res_merge<T, true>), dim3(sgrid), dim3(kSegBlock), (size_t)kMergeLds * 8,
+                     env.stream, a, (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
+                     (const uint32_t*)ws.mslot, targets, (const int32_t*)ws.plan, ws,
+                     env.clock, cap);
+  hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, cap);
+  timing_end(env, 2);
+  return check_launch("resample in place");
+}
+
+int launch_resample_inplace(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                            const uint32_t* keys, int nb, const int32_t* targets,
+                            int32_t n_keys, const bz_resample_policy_t* pol) {
+  if (n_keys + 1 > 65535) return fail("resample: at most 65534 keys");
+  if (env.precision == BZ_F64)
+    return launch_resample_inplace_t<double>(env, s, p, keys, nb, targets, n_keys, pol);
+  return launch_resample_inplace_t<float>(env, s, p, keys, nb, targets, n_keys, pol);
+}
+
+// the slot keys of a state (k_keys) into the sort input, for a caller whose
+// state has no keys from the push
+int launch_slot_keys(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
+                     const double* thr, int nb) {
+  SortWs ws = carve(env.sort_ws, env.capacity, env.max_keys);
+  const KeyArgs ka = key_args(s, p, thr, nb);
+  auto kk = env.precision == BZ_F64 ? k_keys<double> : k_keys<float>;
+  const int64_t kgrid = (env.capacity + (int64_t)kSortBlock * kKeyItems - 1) /
+                        ((int64_t)kSortBlock * kKeyItems);
+  hipLaunchKernelGGL(kk, dim3((unsigned)(kgrid > 0 ? kgrid : 1)), dim3(kSortBlock), 0,
+                     env.stream, ka, (const Clock*)env.clock, ws.keys_a);
+  return check_launch("k_keys");
 }
 
 int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out,

@@ -49,7 +49,9 @@ class KineticEngine:
 
     def __init__(self, cfg: ph.RunConfig, device: int = 0, capacity: Optional[int] = None,
                  rng_mode: int = _lib.BZ_RNG_PHILOX, max_keys: Optional[int] = None,
-                 compact_every: int = 64, push_keys: Optional[bool] = None):
+                 compact_every: int = 64, push_keys: Optional[bool] = None,
+                 inplace_resample: Optional[bool] = None,
+                 compact_resampled_every: Optional[int] = None):
         if not torch.cuda.is_available():
             raise _lib.BzError("no GPU visible: the electron step runs only in the HIP kernels")
         self.lib = _lib.load()
@@ -71,6 +73,14 @@ class KineticEngine:
         self.inplace = cfg.dim == 1
         self.compact_every = int(compact_every)
         self._since_compact = 0
+        # 1-D runs resample in place; the holes a pass leaves are compacted
+        # away before the push after every `compact_resampled_every` passes
+        self.inplace_resample = self.inplace if inplace_resample is None else \
+            bool(inplace_resample) and self.inplace
+        if compact_resampled_every is None:
+            compact_resampled_every = 4 if cfg.resample_every == 1 else 1
+        self.compact_resampled_every = int(compact_resampled_every)
+        self._resampled_since_compact = 0
         # captured blocks of steps: (state buffer, steps) -> graph id
         self._graphs = {}
         self._cap_stream = None
@@ -209,6 +219,7 @@ class KineticEngine:
                    "bz_compact")
         self.cur = 1 - self.cur
         self._since_compact = 0
+        self._resampled_since_compact = 0
 
     def deposit(self):
         _lib.check(self.lib.bz_deposit(self.ctx, C.byref(self.state.struct), C.byref(self.params),
@@ -250,7 +261,23 @@ class KineticEngine:
                                          _tptr(self.counts)), "bz_bin_count")
 
     def resample(self, force: bool = False) -> bool:
-        """Count, and when triggered sort and split / merge into the other buffer."""
+        """One resampling check.  1-D runs resample in place (the reference's
+        layout: merged members become holes, clones and merged particles are
+        appended; the trigger is evaluated on the device, so nothing is read
+        back).  0-D runs count, and when triggered sort and split / merge into
+        the other buffer in key order.  Returns whether the pass was enqueued
+        (in place: always; the device decides whether it fires)."""
+        if self.inplace_resample:
+            pol = self.policy
+            if force:
+                pol = _lib.ResamplePolicy.from_buffer_copy(self.policy)
+                pol.ref_trigger = 0
+                pol.trigger_at = 0.0
+            _lib.check(self.lib.bz_resample_inplace(
+                self.ctx, C.byref(self.state.struct), C.byref(self.params), self.thr, self.nb,
+                _tptr(self.targets), self.n_keys, C.byref(pol)), "bz_resample_inplace")
+            self._resampled_since_compact += 1
+            return True
         always = not self.policy.ref_trigger and self.policy.trigger_at <= 0
         # the counts are read back (a host synchronisation) only when the
         # trigger can decline; the sort runs only for a pass that happens
@@ -269,6 +296,14 @@ class KineticEngine:
         self._since_compact = 0  # the resampled state holds no holes
         return True
 
+    def _compact_due(self) -> bool:
+        if not self.inplace:
+            return False
+        if self.compact_every and self._since_compact >= self.compact_every:
+            return True
+        return bool(self.compact_resampled_every and
+                    self._resampled_since_compact >= self.compact_resampled_every)
+
     def initialize_fields(self):
         """Initial deposit and Poisson solve at t (boltzmann.py:363-399)."""
         self.deposit()
@@ -277,15 +312,16 @@ class KineticEngine:
 
     def step(self, index: Optional[int] = None):
         """One kinetic step in the reference's order (boltzmann.py:570-678):
-        resampling check every `resample_every` steps, push + collisions +
-        deposit + compaction, then the grid step."""
+        resampling check every `resample_every` steps (then the compaction
+        when due, so the push writes the keys of the state it leaves), push +
+        collisions + deposit, then the grid step."""
         k = self.cfg.resample_every
         if k and self.cfg.target_ppc > 0 and index is not None and index % k == 0:
             self.resample()
+        if self._compact_due():
+            self.compact()
         self.push()
         self.field_step(advance=True)
-        if self.inplace and self.compact_every and self._since_compact >= self.compact_every:
-            self.compact()
 
     def run(self, index: int, n_steps: int, graphs: bool = True) -> int:
         """n_steps steps from step `index`: the sequence step(index),
@@ -306,6 +342,8 @@ class KineticEngine:
         while index < end:
             if k and index % k == 0:
                 self.resample()
+            if self._compact_due():
+                self.compact()
             m = end - index
             if k:
                 m = min(m, k - index % k)
@@ -313,8 +351,6 @@ class KineticEngine:
                 m = max(1, min(m, self.compact_every - self._since_compact))
             self._replay(m)
             index += m
-            if self.compact_every and self._since_compact >= self.compact_every:
-                self.compact()
         return index
 
     def _replay(self, m: int):

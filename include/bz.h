@@ -75,6 +75,7 @@ enum {
   BZ_CNT_PSTEPS = 11,   /* particle-steps pushed (sum of input counts) */
   BZ_CNT_COPIED = 12,   /* resampling: particles copied unchanged (copy segments) */
   BZ_CNT_SORTED = 13,   /* slots ordered by the binning sorts (sum of sort sizes) */
+  BZ_CNT_DEP_RANGE = 14, /* deposit blocks whose fixed-point node sums neared their range */
   BZ_NCOUNTERS = 16
 };
 
@@ -118,8 +119,20 @@ typedef struct {
   int32_t bnf;      /* fine bins per node of the deposit */
   int64_t r_stride; /* BZ_RNG_STREAM: column stride of R and gas_v */
   int32_t push_order; /* BZ_PUSH_*: order of the position and velocity updates */
-  int32_t pad_;
+  int32_t ref_flags;  /* BZ_REF_*: reference behaviours reproduced on request */
 } bz_params_t;
+
+/* Reference behaviours that are off by default (bz_params_t.ref_flags).
+ * BZ_REF_DEAD_IONIZER: a primary that ionizes and leaves the domain in the
+ *   same step adds 1 (its unscaled ionization flag) to the ionization moment
+ *   of node 0: the reference zeroes its weight and parks it at x = 1e-8
+ *   (kernels/electron_kernel_numba.py:231,267-270), scales the flag by the
+ *   weight only where w > 0 and reduces every row
+ *   (kernels/post_processing_1D.py:48-55).
+ * BZ_REF_ENERGY_ROUNDTRIP: binning uses the energy after (e*w)/w, the value
+ *   the reference's tosum[:,3] holds when it bins (post_processing_1D.py:53,62),
+ *   in the state precision. */
+enum { BZ_REF_DEAD_IONIZER = 1, BZ_REF_ENERGY_ROUNDTRIP = 2 };
 
 /* Tabulated sigma(E)*N_gas, three channels (0 elastic, 1 excitation,
  * 2 ionization), piecewise linear on n nodes.  Device pointers in the
@@ -169,6 +182,13 @@ typedef struct {
   int32_t ref_rescale;  /* 1: reference merge rescale sqrt(T/T_hat) from total energy;
                            0: central moments (conserves energy with drift) */
   int32_t pad;
+  /* The resampling draws: NULL, the Philox stream; otherwise a device array
+   * of FP64 uniforms in [0,1), U[key*u_stride + m] being draw m of segment
+   * `key` (a merge's U_m of (U_m + m)/K, a split's m-th categorical draw), the
+   * numbers the reference takes from cp.random.rand / cp.random.choice
+   * (1bin/boltzmann.py:814,847).  u_stride must exceed every segment's draws. */
+  const double* uniforms;
+  int64_t u_stride;
 } bz_resample_policy_t;
 
 const char* bz_last_error(void);
@@ -323,6 +343,24 @@ int bz_resample(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
                 const bz_params_t* p, const int32_t* counts, const int32_t* offsets,
                 const uint32_t* perm, const int32_t* targets, int32_t n_keys,
                 const bz_resample_policy_t* pol);
+
+/* ES-WPNC resampling in place, in the reference's own layout
+ * (1bin/boltzmann.py:806-877): the slots of segments that are neither split
+ * nor merged are not touched; a split particle keeps its slot with weight
+ * w/(c+1) and its c clones are appended; a merged segment's members become
+ * holes (w = 0) and its K particles (weight W/K, rescaled velocities) are
+ * appended.  The appended particles follow the slots in use in key order
+ * (cell, then energy bin), a split particle's clones in slot order.
+ * Selections, weights and velocities are those of bz_resample (segment
+ * members in ascending slot order).  The holes stay until bz_compact, the
+ * reference's compaction after resampling (1bin/boltzmann.py:879-890).
+ * Binning as bz_bin_sort (thresholds_host [nb-1]); the keys the last 1-D
+ * push wrote for this state are used when it wrote them (bz_set_push_keys).
+ * Replaces _check_for_resampling + _do_resampling (1bin/boltzmann.py:799-916,
+ * 3bin/boltzmann.py:813-1113, 5bin/boltzmann.py:843-1334). */
+int bz_resample_inplace(bz_ctx_t ctx, const bz_state_t* s, const bz_params_t* p,
+                        const double* thresholds_host, int32_t nb, const int32_t* targets,
+                        int32_t n_keys, const bz_resample_policy_t* pol);
 
 /* ---- Batched 0-D swarms (no counterpart in the reference: its 0-D test
  * problems are single runs made by hand-editing the driver, README.md:7-8) ----

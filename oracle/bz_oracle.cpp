@@ -105,6 +105,16 @@ void orc_collision_uniforms(uint64_t seed, uint64_t step, int64_t n, int prec, v
   orc_uniforms_at(seed, step, n, prec, R, stride, 0u);
 }
 
+// Gas velocity of particles 0..n-1 at `step` from the Philox stream (purpose
+// 1), G[c*stride + i], c = x, y, z: Box-Muller with std `sigma` per
+// component, the draw the device makes when the gas is warm.  The reference
+// draws it with cp.random.multivariate_normal(0, Eg*I) (1bin/boltzmann.py:589);
+// the normal deviates here are r1 cos(2 pi g1), r1 sin(2 pi g1) and
+// r2 cos(2 pi g3) with r = sqrt(-2 ln(1 - g)).  sin/cos of pi*v use the
+// exactly reduced m_sincospi below (defined after this block).
+void orc_gas_velocities(uint64_t seed, uint64_t step, int64_t n, int prec, double sigma, void* G,
+                        int64_t stride);
+
 // Resampling draws: uniform m of segment `key` at step `step` (FP64).
 double orc_resample_uniform(uint64_t seed, uint64_t step, uint32_t key, uint32_t m) {
   uint32_t b[4];
@@ -124,6 +134,8 @@ typedef struct {
   int32_t push_order;   // 0: x then v (reference), 1: v then x (leapfrog)
   int32_t births_last;  // 1: secondaries after all survivors (reference layout),
                         // 0: each secondary right after its parent
+  int32_t ref_flags;    // 1: dead ionizer's flag at node 0, 2: (e*w)/w binning energy
+  int32_t pad;
 } orc_params;
 
 typedef struct {
@@ -190,6 +202,43 @@ inline void m_sincospi(float v, float* s, float* c) {
   *c = (float)cd;
 }
 
+}  // namespace
+
+extern "C" void orc_gas_velocities(uint64_t seed, uint64_t step, int64_t n, int prec,
+                                   double sigma, void* G, int64_t stride) {
+  for (int64_t j = 0; j < n; ++j) {
+    const uint32_t i = (uint32_t)j;
+    uint32_t a[4], b[4];
+    orc_draw(seed, 1, i, 0, step, a);
+    if (prec == 8) {
+      orc_draw(seed, 1, i, 1, step, b);
+      const double g0 = orc_u53(a[0], a[1]), g1 = orc_u53(a[2], a[3]);
+      const double g2 = orc_u53(b[0], b[1]), g3 = orc_u53(b[2], b[3]);
+      const double r1 = sqrt(-2.0 * log(1.0 - g0)), r2 = sqrt(-2.0 * log(1.0 - g2));
+      double s1, c1, s3, c3;
+      m_sincospi(2.0 * g1, &s1, &c1);
+      m_sincospi(2.0 * g3, &s3, &c3);
+      double* g = (double*)G;
+      g[j] = sigma * (r1 * c1);
+      g[stride + j] = sigma * (r1 * s1);
+      g[2 * stride + j] = sigma * (r2 * c3);
+    } else {
+      const float g0 = orc_u24(a[0]), g1 = orc_u24(a[1]), g2 = orc_u24(a[2]), g3 = orc_u24(a[3]);
+      const float r1 = sqrtf(-2.0f * logf(1.0f - g0)), r2 = sqrtf(-2.0f * logf(1.0f - g2));
+      float s1, c1, s3, c3;
+      m_sincospi(2.0f * g1, &s1, &c1);
+      m_sincospi(2.0f * g3, &s3, &c3);
+      const float sg = (float)sigma;
+      float* g = (float*)G;
+      g[j] = sg * (r1 * c1);
+      g[stride + j] = sg * (r1 * s1);
+      g[2 * stride + j] = sg * (r2 * c3);
+    }
+  }
+}
+
+namespace {
+
 template <typename T>
 T poly(double a, double b, double c, T L) {
   return ((T)a * L + (T)b) * L + (T)c;
@@ -217,6 +266,20 @@ void fit_cross_sections(T e, const orc_params& P, T* s_el, T* s_ion) {
   *s_el = s0 * (T)P.nn;
   *s_ion = s2 * (T)P.nn;
 }
+
+}  // namespace
+
+// sigma_el * nn and sigma_ion * nn of the reference fits at energies e[0..n)
+extern "C" void orc_fit_sigma_f64(const orc_params* P, int64_t n, const double* e, double* s_el,
+                                  double* s_ion) {
+  for (int64_t i = 0; i < n; ++i) fit_cross_sections<double>(e[i], *P, &s_el[i], &s_ion[i]);
+}
+extern "C" void orc_fit_sigma_f32(const orc_params* P, int64_t n, const float* e, float* s_el,
+                                  float* s_ion) {
+  for (int64_t i = 0; i < n; ++i) fit_cross_sections<float>(e[i], *P, &s_el[i], &s_ion[i]);
+}
+
+namespace {
 
 template <typename T>
 void table_cross_sections(T e, const orc_params& P, const orc_table& tab, T s[3]) {
@@ -599,6 +662,10 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
           if (chan == 3) cc[0] += 1;
           const bool alive_primary = (m == 2) || (m == 1 && chan != 3);
           if (!alive_primary) cc[1] += 1;
+          // the reference reduces the rows of absorbed particles too: an
+          // ionizer that left keeps its unscaled flag 1 at x = 1e-8, node 0
+          // (electron_kernel_numba.py:231,267-270; post_processing_1D.py:48-55)
+          if (!alive_primary && chan == 3 && (P->ref_flags & 1) && P->dim == 1) pm[nn] += 1.0;
         }
       }
       // the secondary, when one was born, is the last record
@@ -726,7 +793,10 @@ int bin_key(const orc_params& P, int nb, const double* thr, T x, T vx, T vy, T v
     if (cell < 0) cell = 0;
     if (cell >= P.n_nodes) cell = P.n_nodes - 1;
   }
-  const T e = (T)P.kev * (vx * vx + vy * vy + vz * vz);
+  T e = (T)P.kev * (vx * vx + vy * vy + vz * vz);
+  // the reference bins tosum[:,3] after it was multiplied and divided by w
+  // (post_processing_1D.py:53,62)
+  if (P.ref_flags & 2) e = (e * w) / w;
   // bin_particles.py: LE < thr[0] <= ... ; bins ascend with energy
   int b = 0;
   for (int k = 0; k < nb - 1; ++k)
@@ -766,15 +836,39 @@ void bin_sort(const orc_params& P, int nb, const double* thr, int64_t n, const T
   for (int64_t i = 0; i < n; ++i) perm[fill[key[i]]++] = (uint32_t)i;  // stable
 }
 
+// Resampling uniform m of segment k: the caller's stream U[k*u_stride + m]
+// when given, the Philox stream otherwise.
+inline double res_u(const double* U, int64_t u_stride, uint64_t seed, uint64_t step, int k,
+                    int m) {
+  return U ? U[(int64_t)k * u_stride + m]
+           : orc_resample_uniform(seed, step, (uint32_t)k, (uint32_t)m);
+}
+
+// What one resampling pass does to one (cell, bin) segment.
+struct SegAction {
+  int kind = 0;                    // 0 untouched, 1 split, 2 merge
+  int ce = 0;                      // merge: members taking part (max_ppc clip)
+  std::vector<int> copies;         // split: clones of each member
+  std::vector<uint32_t> sel;       // merge: the selected slots, in draw order
+  std::vector<double> nv[3];       // merge: velocities of the merged particles
+  double wbar = 0;                 // merge: their weight
+};
+
+// Binning, trigger and the split / merge decisions and draws of every
+// segment (boltzmann.py:799-916; 3bin:813-1113; 5bin:843-1334).  Members of
+// a segment are its slots in ascending order.  Fills counts/offsets/perm.
 template <typename T>
-int64_t resample(const orc_params& P, const orc_policy& pol, int nb, const double* thr,
-                 const int32_t* targets, uint64_t seed, uint64_t step, int64_t n, const T* x,
-                 const T* vx, const T* vy, const T* vz, const T* w, T* ox, T* ovx, T* ovy,
-                 T* ovz, T* ow, int64_t cap, uint64_t* cnt) {
+bool resample_plan(const orc_params& P, const orc_policy& pol, int nb, const double* thr,
+                   const int32_t* targets, uint64_t seed, uint64_t step, int64_t n, const T* x,
+                   const T* vx, const T* vy, const T* vz, const T* w, const double* U,
+                   int64_t u_stride, std::vector<int32_t>& counts, std::vector<int32_t>& offsets,
+                   std::vector<uint32_t>& perm, std::vector<SegAction>& act, uint64_t* cnt) {
   const int nn = P.dim == 1 ? P.n_nodes : 1;
   const int n_keys = nn * nb;
-  std::vector<int32_t> counts(n_keys + 1), offsets(n_keys + 2);
-  std::vector<uint32_t> perm(n > 0 ? n : 1);
+  counts.assign(n_keys + 1, 0);
+  offsets.assign(n_keys + 2, 0);
+  perm.assign(n > 0 ? n : 1, 0);
+  act.assign(n_keys, SegAction());
   bin_sort<T>(P, nb, thr, n, x, vx, vy, vz, w, counts.data(), offsets.data(), perm.data());
   bool fire = !pol.ref_trigger && pol.trigger_at <= 0.0;  // <= 0: every check fires
   for (int k = 0; k < n_keys; ++k) {
@@ -786,43 +880,21 @@ int64_t resample(const orc_params& P, const orc_policy& pol, int nb, const doubl
       fire = true;
     }
   }
-  int64_t o = 0;
-  auto emit = [&](T a, T b, T c_, T d, T e) {
-    if (o < cap) {
-      ox[o] = a;
-      ovx[o] = b;
-      ovy[o] = c_;
-      ovz[o] = d;
-      ow[o] = e;
-    } else {
-      cnt[6] += 1;
-    }
-    ++o;
-  };
+  if (!fire) return false;
+  cnt[10] += 1;
   for (int k = 0; k < n_keys; ++k) {
     const int c = counts[k], t = targets[k];
     const uint32_t* ids = perm.data() + offsets[k];
-    int kind = 0;
-    int draws = 0, keep = 0, ce = c;
-    if (fire && t > 0) {
-      keep = (int)(pol.merge_keep * (double)t);
-      if (c > 0 && (double)c < pol.split_below * t) {
-        draws = (int)(pol.split_to * (double)t) - c;
-        int hw = 0;
-        for (int j = 0; j < c; ++j) hw += (double)w[ids[j]] > pol.min_wt;
-        if (hw > 0 && draws > 0) kind = 1;
-      } else if ((double)c >= pol.merge_at * t && keep >= 1) {
-        ce = (pol.max_ppc > 0 && c > pol.max_ppc) ? pol.max_ppc : c;
-        if (keep < ce) kind = 2;
-      }
-    }
-    if (kind == 0) {
-      for (int j = 0; j < c; ++j) {
-        const uint32_t i = ids[j];
-        emit(x[i], vx[i], vy[i], vz[i], w[i]);
-      }
-    } else if (kind == 1) {
+    SegAction& A = act[k];
+    if (t <= 0) continue;
+    const int keep = (int)(pol.merge_keep * (double)t);
+    if (c > 0 && (double)c < pol.split_below * t) {
+      const int draws = (int)(pol.split_to * (double)t) - c;
+      int hw = 0;
+      for (int j = 0; j < c; ++j) hw += (double)w[ids[j]] > pol.min_wt;
+      if (hw == 0 || draws <= 0) continue;
       // split (boltzmann.py:806-823): draws with replacement, p ~ w over w > min_wt
+      A.kind = 1;
       cnt[8] += 1;
       double wmax = 0;
       for (int j = 0; j < c; ++j)
@@ -837,21 +909,18 @@ int64_t resample(const orc_params& P, const orc_policy& pol, int nb, const doubl
         acc += wj > pol.min_wt ? (int64_t)ldexp(wj, S) : 0;
         cdf[j] = acc;
       }
-      std::vector<int> copies(c, 0);
+      A.copies.assign(c, 0);
       for (int m = 0; m < draws; ++m) {
-        const double u = orc_resample_uniform(seed, step, (uint32_t)k, (uint32_t)m);
+        const double u = res_u(U, u_stride, seed, step, k, m);
         const int64_t th = (int64_t)(u * (double)acc);
-        copies[search(cdf, th, true)] += 1;
+        A.copies[search(cdf, th, true)] += 1;
       }
-      for (int j = 0; j < c; ++j) {
-        const uint32_t i = ids[j];
-        const int reps = 1 + copies[j];
-        T wn = w[i];
-        if (reps > 1) wn = wn / (T)reps;
-        for (int r = 0; r < reps; ++r) emit(x[i], vx[i], vy[i], vz[i], wn);
-      }
-    } else {
+    } else if ((double)c >= pol.merge_at * t && keep >= 1) {
+      const int ce = (pol.max_ppc > 0 && c > pol.max_ppc) ? pol.max_ppc : c;
+      if (keep >= ce) continue;
       // merge (boltzmann.py:825-877)
+      A.kind = 2;
+      A.ce = ce;
       cnt[9] += 1;
       double W = 0, Px = 0, Py = 0, Pz = 0, En = 0, wmax = 0;
       for (int j = 0; j < ce; ++j) {
@@ -881,15 +950,15 @@ int64_t resample(const orc_params& P, const orc_policy& pol, int nb, const doubl
         cdf[j] = acc;
       }
       const int K = keep;
-      const double wbar = W / (double)K;
-      std::vector<uint32_t> sel(K);
+      A.wbar = W / (double)K;
+      A.sel.resize(K);
       double Sx = 0, Sy = 0, Sz = 0, Se = 0;
       for (int m = 0; m < K; ++m) {
-        const double u = orc_resample_uniform(seed, step, (uint32_t)k, (uint32_t)m);
+        const double u = res_u(U, u_stride, seed, step, k, m);
         const double r = (1.0 / (double)K) * (u + (double)m);
         const int64_t th = (int64_t)(r * (double)acc);
-        sel[m] = ids[search(cdf, th, false)];
-        const double a = vx[sel[m]], b = vy[sel[m]], cc = vz[sel[m]];
+        A.sel[m] = ids[search(cdf, th, false)];
+        const double a = vx[A.sel[m]], b = vy[A.sel[m]], cc = vz[A.sel[m]];
         Sx += a;
         Sy += b;
         Sz += cc;
@@ -897,11 +966,11 @@ int64_t resample(const orc_params& P, const orc_policy& pol, int nb, const doubl
       }
       double hx, hy, hz, f;
       if (pol.ref_rescale) {
-        hx = wbar * Sx / W;
-        hy = wbar * Sy / W;
-        hz = wbar * Sz / W;
+        hx = A.wbar * Sx / W;
+        hy = A.wbar * Sy / W;
+        hz = A.wbar * Sz / W;
         const double Tc = (2.0 / 3.0) * (P.kev * En) / W;
-        const double Th = (2.0 / 3.0) * (wbar * (P.kev * Se)) / W;
+        const double Th = (2.0 / 3.0) * (A.wbar * (P.kev * Se)) / W;
         f = Th > 0 ? sqrt(Tc / Th) : 1.0;
       } else {
         hx = Sx / K;
@@ -909,24 +978,128 @@ int64_t resample(const orc_params& P, const orc_policy& pol, int nb, const doubl
         hz = Sz / K;
         double Cn = 0;
         for (int m = 0; m < K; ++m) {
-          const double a = vx[sel[m]] - hx, b = vy[sel[m]] - hy, cc = vz[sel[m]] - hz;
+          const double a = vx[A.sel[m]] - hx, b = vy[A.sel[m]] - hy, cc = vz[A.sel[m]] - hz;
           Cn += a * a + b * b + cc * cc;
         }
-        Cn *= wbar;
+        Cn *= A.wbar;
         f = Cn > 0 ? sqrt(Cold / Cn) : 1.0;
       }
+      for (int q = 0; q < 3; ++q) A.nv[q].resize(K);
       for (int m = 0; m < K; ++m) {
-        const uint32_t i = sel[m];
-        emit(x[i], (T)(mx + ((double)vx[i] - hx) * f), (T)(my + ((double)vy[i] - hy) * f),
-             (T)(mz + ((double)vz[i] - hz) * f), (T)wbar);
+        const uint32_t i = A.sel[m];
+        A.nv[0][m] = mx + ((double)vx[i] - hx) * f;
+        A.nv[1][m] = my + ((double)vy[i] - hy) * f;
+        A.nv[2][m] = mz + ((double)vz[i] - hz) * f;
       }
-      for (int j = ce; j < c; ++j) {
+    }
+  }
+  return true;
+}
+
+// Key-order layout: every segment written to out in key order; a split
+// particle is followed by its clones, a merged segment is its K particles
+// followed by the members past the max_ppc clip.
+template <typename T>
+int64_t resample(const orc_params& P, const orc_policy& pol, int nb, const double* thr,
+                 const int32_t* targets, uint64_t seed, uint64_t step, int64_t n, const T* x,
+                 const T* vx, const T* vy, const T* vz, const T* w, T* ox, T* ovx, T* ovy,
+                 T* ovz, T* ow, int64_t cap, uint64_t* cnt, const double* U, int64_t u_stride) {
+  std::vector<int32_t> counts, offsets;
+  std::vector<uint32_t> perm;
+  std::vector<SegAction> act;
+  resample_plan<T>(P, pol, nb, thr, targets, seed, step, n, x, vx, vy, vz, w, U, u_stride,
+                   counts, offsets, perm, act, cnt);
+  const int n_keys = (int)act.size();
+  int64_t o = 0;
+  auto emit = [&](T a, T b, T c_, T d, T e) {
+    if (o < cap) {
+      ox[o] = a;
+      ovx[o] = b;
+      ovy[o] = c_;
+      ovz[o] = d;
+      ow[o] = e;
+    } else {
+      cnt[6] += 1;
+    }
+    ++o;
+  };
+  for (int k = 0; k < n_keys; ++k) {
+    const int c = counts[k];
+    const uint32_t* ids = perm.data() + offsets[k];
+    const SegAction& A = act[k];
+    if (A.kind == 0) {
+      for (int j = 0; j < c; ++j) {
+        const uint32_t i = ids[j];
+        emit(x[i], vx[i], vy[i], vz[i], w[i]);
+      }
+    } else if (A.kind == 1) {
+      for (int j = 0; j < c; ++j) {
+        const uint32_t i = ids[j];
+        const int reps = 1 + A.copies[j];
+        T wn = w[i];
+        if (reps > 1) wn = wn / (T)reps;
+        for (int r = 0; r < reps; ++r) emit(x[i], vx[i], vy[i], vz[i], wn);
+      }
+    } else {
+      for (size_t m = 0; m < A.sel.size(); ++m)
+        emit(x[A.sel[m]], (T)A.nv[0][m], (T)A.nv[1][m], (T)A.nv[2][m], (T)A.wbar);
+      for (int j = A.ce; j < c; ++j) {
         const uint32_t i = ids[j];
         emit(x[i], vx[i], vy[i], vz[i], w[i]);
       }
     }
   }
-  if (fire) cnt[10] += 1;
+  return o < cap ? o : cap;
+}
+
+// In-place layout, the reference's own (boltzmann.py:812-823, :850-874): a
+// split particle keeps its slot with weight w/(c+1) and its c clones are
+// appended; a merged segment's members 0..ce-1 become holes (w = 0) and its
+// K particles are appended; every other slot is left as it is.  The appended
+// particles follow the n slots in key order, a split particle's clones in
+// member order.  State arrays hold `cap` slots; returns the new slot count.
+template <typename T>
+int64_t resample_inplace(const orc_params& P, const orc_policy& pol, int nb, const double* thr,
+                         const int32_t* targets, uint64_t seed, uint64_t step, int64_t n, T* x,
+                         T* vx, T* vy, T* vz, T* w, int64_t cap, uint64_t* cnt, const double* U,
+                         int64_t u_stride) {
+  std::vector<int32_t> counts, offsets;
+  std::vector<uint32_t> perm;
+  std::vector<SegAction> act;
+  if (!resample_plan<T>(P, pol, nb, thr, targets, seed, step, n, x, vx, vy, vz, w, U, u_stride,
+                        counts, offsets, perm, act, cnt))
+    return n;
+  const int n_keys = (int)act.size();
+  int64_t o = n;
+  auto emit = [&](T a, T b, T c_, T d, T e) {
+    if (o < cap) {
+      x[o] = a;
+      vx[o] = b;
+      vy[o] = c_;
+      vz[o] = d;
+      w[o] = e;
+    } else {
+      cnt[6] += 1;
+    }
+    ++o;
+  };
+  for (int k = 0; k < n_keys; ++k) {
+    const uint32_t* ids = perm.data() + offsets[k];
+    const SegAction& A = act[k];
+    if (A.kind == 1) {
+      for (int j = 0; j < counts[k]; ++j) {
+        if (!A.copies[j]) continue;
+        const uint32_t i = ids[j];
+        const T wn = w[i] / (T)(1 + A.copies[j]);
+        for (int r = 0; r < A.copies[j]; ++r) emit(x[i], vx[i], vy[i], vz[i], wn);
+        w[i] = wn;
+      }
+    } else if (A.kind == 2) {
+      for (size_t m = 0; m < A.sel.size(); ++m)
+        emit(x[A.sel[m]], (T)A.nv[0][m], (T)A.nv[1][m], (T)A.nv[2][m], (T)A.wbar);
+      for (int j = 0; j < A.ce; ++j) w[ids[j]] = (T)0;
+    }
+  }
   return o < cap ? o : cap;
 }
 
@@ -945,21 +1118,39 @@ void orc_bin_sort_f32(const orc_params* P, int nb, const double* thr, int64_t n,
   bin_sort<float>(*P, nb, thr, n, x, vx, vy, vz, w, counts, offsets, perm);
 }
 
+// U (optional): the caller's resampling uniforms, U[key*u_stride + m]
 int64_t orc_resample_f64(const orc_params* P, const orc_policy* pol, int nb, const double* thr,
                          const int32_t* targets, uint64_t seed, uint64_t step, int64_t n,
                          const double* x, const double* vx, const double* vy, const double* vz,
                          const double* w, double* ox, double* ovx, double* ovy, double* ovz,
-                         double* ow, int64_t cap, uint64_t* cnt) {
+                         double* ow, int64_t cap, uint64_t* cnt, const double* U,
+                         int64_t u_stride) {
   return resample<double>(*P, *pol, nb, thr, targets, seed, step, n, x, vx, vy, vz, w, ox, ovx,
-                          ovy, ovz, ow, cap, cnt);
+                          ovy, ovz, ow, cap, cnt, U, u_stride);
 }
 int64_t orc_resample_f32(const orc_params* P, const orc_policy* pol, int nb, const double* thr,
                          const int32_t* targets, uint64_t seed, uint64_t step, int64_t n,
                          const float* x, const float* vx, const float* vy, const float* vz,
                          const float* w, float* ox, float* ovx, float* ovy, float* ovz, float* ow,
-                         int64_t cap, uint64_t* cnt) {
+                         int64_t cap, uint64_t* cnt, const double* U, int64_t u_stride) {
   return resample<float>(*P, *pol, nb, thr, targets, seed, step, n, x, vx, vy, vz, w, ox, ovx,
-                         ovy, ovz, ow, cap, cnt);
+                         ovy, ovz, ow, cap, cnt, U, u_stride);
+}
+int64_t orc_resample_inplace_f64(const orc_params* P, const orc_policy* pol, int nb,
+                                 const double* thr, const int32_t* targets, uint64_t seed,
+                                 uint64_t step, int64_t n, double* x, double* vx, double* vy,
+                                 double* vz, double* w, int64_t cap, uint64_t* cnt,
+                                 const double* U, int64_t u_stride) {
+  return resample_inplace<double>(*P, *pol, nb, thr, targets, seed, step, n, x, vx, vy, vz, w,
+                                  cap, cnt, U, u_stride);
+}
+int64_t orc_resample_inplace_f32(const orc_params* P, const orc_policy* pol, int nb,
+                                 const double* thr, const int32_t* targets, uint64_t seed,
+                                 uint64_t step, int64_t n, float* x, float* vx, float* vy,
+                                 float* vz, float* w, int64_t cap, uint64_t* cnt, const double* U,
+                                 int64_t u_stride) {
+  return resample_inplace<float>(*P, *pol, nb, thr, targets, seed, step, n, x, vx, vy, vz, w,
+                                 cap, cnt, U, u_stride);
 }
 
 // Grid step: moment update, heavies, Poisson (FP64).

@@ -26,7 +26,7 @@ class OrcParams(C.Structure):
         "e_ion", "e_exc", "pc", "e_field0", "nu_max")] + [
         (n, C.c_int32) for n in ("dim", "xs_mode", "n_cells", "n_nodes", "bnf", "nthreads")] + [
         ("seed", C.c_uint64), ("step", C.c_uint64), ("push_order", C.c_int32),
-        ("births_last", C.c_int32)]
+        ("births_last", C.c_int32), ("ref_flags", C.c_int32), ("pad", C.c_int32)]
 
 
 class OrcTable(C.Structure):
@@ -93,6 +93,9 @@ class Setup:
     # secondaries after all survivors (the 1-D layout) or right after their
     # parents (the 0-D swarm layout); None: by dimension
     births_last: Optional[bool] = None
+    # reference behaviours: 1 the dead ionizer's flag at node 0, 2 the
+    # (e*w)/w binning energy
+    ref_flags: int = 0
 
     def params(self) -> OrcParams:
         p = OrcParams()
@@ -128,6 +131,7 @@ class Setup:
         p.push_order = self.push_order
         bl = self.births_last if self.births_last is not None else self.dim == 1
         p.births_last = 1 if bl else 0
+        p.ref_flags = self.ref_flags
         return p
 
 
@@ -138,6 +142,28 @@ def collision_uniforms(seed: int, step: int, n: int, dtype) -> np.ndarray:
     lib.orc_collision_uniforms(C.c_uint64(seed), C.c_uint64(step), n,
                                8 if dtype == np.float64 else 4, _ptr(R), R.shape[1])
     return R
+
+
+def gas_velocities(seed: int, step: int, n: int, dtype, sigma: float) -> np.ndarray:
+    """[3, n] gas velocities of the Philox stream (purpose 1): Box-Muller with
+    std `sigma` per component, as the device draws them for a warm gas."""
+    lib = load()
+    G = np.zeros((3, max(n, 1)), dtype=dtype)
+    lib.orc_gas_velocities(C.c_uint64(seed), C.c_uint64(step), C.c_int64(n),
+                           C.c_int(8 if dtype == np.float64 else 4), C.c_double(sigma), _ptr(G),
+                           C.c_int64(G.shape[1]))
+    return G
+
+
+def fit_sigma(setup: "Setup", e: np.ndarray):
+    """(sigma_el * nn, sigma_ion * nn) of the reference argon fits at energies e."""
+    lib = load()
+    e = np.ascontiguousarray(e)
+    p = setup.params()
+    a, b = np.zeros_like(e), np.zeros_like(e)
+    fn = lib.orc_fit_sigma_f64 if e.dtype == np.float64 else lib.orc_fit_sigma_f32
+    fn(C.byref(p), C.c_int64(e.size), _ptr(e), _ptr(a), _ptr(b))
+    return a, b
 
 
 def resample_uniform(seed: int, step: int, key: int, m: int) -> float:
@@ -235,9 +261,10 @@ def bin_sort(setup: Setup, nb: int, thresholds, state):
 
 
 def resample(setup: Setup, pol: OrcPolicy, nb: int, thresholds, targets, seed: int, step: int,
-             state, capacity=None):
+             state, capacity=None, uniforms=None):
     """Binning + sort + ES-WPNC split/merge; output in key order.
-    Returns (new_state, counters[16])."""
+    uniforms: optional [n_keys, stride] caller stream (row k: segment k's
+    draws).  Returns (new_state, counters[16])."""
     lib = load()
     x, vx, vy, vz, w = [np.ascontiguousarray(a) for a in state]
     dt = x.dtype
@@ -247,9 +274,47 @@ def resample(setup: Setup, pol: OrcPolicy, nb: int, thresholds, targets, seed: i
     cap = capacity if capacity is not None else 4 * x.size + 1024
     out = [np.zeros(cap, dtype=dt) for _ in range(5)]
     cnt = np.zeros(16, dtype=np.uint64)
+    U, us = _uniforms(uniforms)
     fn = lib.orc_resample_f64 if dt == np.float64 else lib.orc_resample_f32
     fn.restype = C.c_int64
     m = fn(C.byref(p), C.byref(pol), C.c_int(nb), _ptr(thr), _ptr(tg), C.c_uint64(seed),
            C.c_uint64(step), C.c_int64(x.size), *[_ptr(a) for a in (x, vx, vy, vz, w)],
-           *[_ptr(a) for a in out], C.c_int64(cap), _ptr(cnt))
+           *[_ptr(a) for a in out], C.c_int64(cap), _ptr(cnt), _ptr(U), C.c_int64(us))
     return tuple(a[:m] for a in out), cnt
+
+
+def _uniforms(uniforms):
+    if uniforms is None:
+        return None, 0
+    U = np.ascontiguousarray(uniforms, dtype=np.float64)
+    if U.ndim != 2:
+        raise ValueError("uniforms: [n_keys, stride]")
+    return U, U.shape[1]
+
+
+def resample_inplace(setup: Setup, pol: OrcPolicy, nb: int, thresholds, targets, seed: int,
+                     step: int, state, capacity=None, uniforms=None):
+    """The reference's in-place layout: split particles keep their slot with
+    the divided weight, merged members become holes (w = 0), clones and
+    merged particles are appended after the slots in key order.  Returns
+    (slots [0, n'), holes included, counters[16])."""
+    lib = load()
+    dt = np.asarray(state[0]).dtype
+    n = int(np.asarray(state[0]).size)
+    cap = capacity if capacity is not None else 4 * n + 1024
+    arrs = []
+    for a in state:
+        b = np.zeros(cap, dtype=dt)
+        b[:n] = a
+        arrs.append(b)
+    p = setup.params()
+    thr = np.concatenate([np.asarray(thresholds, dtype=np.float64).reshape(-1), np.zeros(1)])
+    tg = np.ascontiguousarray(np.asarray(targets, dtype=np.int32))
+    cnt = np.zeros(16, dtype=np.uint64)
+    U, us = _uniforms(uniforms)
+    fn = lib.orc_resample_inplace_f64 if dt == np.float64 else lib.orc_resample_inplace_f32
+    fn.restype = C.c_int64
+    m = fn(C.byref(p), C.byref(pol), C.c_int(nb), _ptr(thr), _ptr(tg), C.c_uint64(seed),
+           C.c_uint64(step), C.c_int64(n), *[_ptr(a) for a in arrs], C.c_int64(cap), _ptr(cnt),
+           _ptr(U), C.c_int64(us))
+    return tuple(a[:m] for a in arrs), cnt

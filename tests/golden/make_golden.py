@@ -139,6 +139,16 @@ def resample_expected(inp):
     return counts, offsets, perm, out, cnt
 
 
+def resample_inplace_expected(inp):
+    """The same pass in the reference's in-place layout (every slot, holes
+    included)."""
+    s = inp["setup"]
+    out, cnt = O.resample_inplace(s, O.policy(**inp["pol"]), inp["nb"], inp["thr"],
+                                  inp["targets"], SEED, 3, inp["state"],
+                                  capacity=4 * inp["state"][0].size + 1024)
+    return out, cnt
+
+
 def field_inputs():
     from boltzmann_amd import physics as ph
     N = 64
@@ -198,11 +208,13 @@ def write_all(out_dir: Path = HERE) -> list:
     for nb in (3, 5):
         inp = resample_inputs(nb, np.float64)
         counts, offsets, perm, out, cnt = resample_expected(inp)
+        ip, _ = resample_inplace_expected(inp)
         st = inp["state"]
         name = f"resample_{nb}bin_f64"
         np.savez_compressed(out_dir / f"{name}.npz", x=st[0], vx=st[1], vy=st[2], vz=st[3],
                             w=st[4], targets=inp["targets"], counts=counts, offsets=offsets,
-                            perm=perm, out=np.stack(out), counters=cnt)
+                            perm=perm, out=np.stack(out), counters=cnt,
+                            out_inplace=np.stack(ip))
         written.append(name)
     inp = field_inputs()
     for scheme in (0, 1):

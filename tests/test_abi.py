@@ -49,12 +49,32 @@ def test_error_path_without_device():
     assert b"null context" in lib.bz_last_error()
 
 
-def test_struct_sizes_match_header_layout():
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives
+    the header's struct (a host program built from include/bz.h with gcc)."""
+    import subprocess
+
     from boltzmann_amd import _lib
-    # 19 doubles + u64 + 6 int32 + int64 + 2 int32
-    assert ctypes.sizeof(_lib.Params) == 19 * 8 + 8 + 6 * 4 + 8 + 2 * 4
-    assert ctypes.sizeof(_lib.XSec) == 4 * 8 + 4 * 4
-    assert ctypes.sizeof(_lib.FieldParams) == 8 * 8 + 6 * 4
-    assert ctypes.sizeof(_lib.Mlp) == 8 + 18 * 8
-    assert ctypes.sizeof(_lib.Adam) == 6 * 4
-    assert ctypes.sizeof(_lib.ResamplePolicy) == 6 * 8 + 4 * 4
+    pairs = {"bz_params_t": _lib.Params, "bz_xsec_t": _lib.XSec,
+             "bz_field_params_t": _lib.FieldParams, "bz_mlp_t": _lib.Mlp,
+             "bz_adam_t": _lib.Adam, "bz_resample_policy_t": _lib.ResamplePolicy,
+             "bz_state_t": _lib.State}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bz.h"', "int main(void) {"]
+    for cname, cls in pairs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                               text=True).stdout.splitlines():
+        cname, field, val = line.split()
+        got[(cname, field)] = int(val)
+    for cname, cls in pairs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

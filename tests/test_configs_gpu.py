@@ -1,0 +1,223 @@
+"""The workloads end to end on the GPU, stage by stage against the oracle.
+
+C4 scaled to 256 cells and 300k electrons (5 bins, resampling every step,
+merge at 1.5x the 1200-per-cell target, binning keys written by the push,
+compaction every 4th pass): for 6 steps every stage is checked on the
+device's own input.  The in-place resampling equals the oracle's slot for
+slot, the compaction is a stable filter of the live slots, the push equals
+the oracle's push of the compacted state in the field the device holds,
+and the field step equals the oracle's on the device's moments.
+
+C0 at its own configuration (100 Td, 1 eV Maxwellian, 100k electrons,
+FP64, dt = 1e-12 s): 5 steps of the compacting 0-D push, each against the
+oracle from the device's state.
+
+The resample trigger on the device (a pass at trigger_at = 2 and the
+reference's last-bin rule, quirk Q1) against the oracle and the reference's
+own test (3bin/boltzmann.py:1063-1113)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _live(st):
+    m = st[4] > 0
+    return tuple(np.ascontiguousarray(a[m]) for a in st)
+
+
+def _close(got, want, prec, L=2.5, what=""):
+    assert got[0].size == want[0].size, what
+    rt = 1e-12 if prec == 8 else 1e-6
+    sp = np.sqrt(sum(np.asarray(want[k], np.float64) ** 2 for k in (1, 2, 3))) + 1.0
+    for k in range(5):
+        g = np.asarray(got[k], np.float64)
+        w = np.asarray(want[k], np.float64)
+        scale = sp if k in (1, 2, 3) else (np.abs(w) + 1e-3 * L if k == 0 else np.abs(w))
+        bad = np.abs(g - w) > rt * scale
+        assert not bad.any(), (what, k, np.flatnonzero(bad)[:5], g[bad][:5], w[bad][:5])
+
+
+def _policy(cfg):
+    return O.policy(split_below=cfg.split_below, merge_at=cfg.merge_at,
+                    merge_keep=cfg.merge_keep, trigger_at=cfg.trigger_at, min_wt=cfg.min_wt,
+                    split_to=cfg.split_to)
+
+
+def test_c4_scaled_pipeline_stage_by_stage(gpu):
+    from boltzmann_amd import configs
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    cfg = configs.c4().with_(n_particles=300_000, n_cells=256, target_ppc=1200)
+    eng = KineticEngine(cfg)
+    # the holes of four passes and their appended particles fit
+    assert eng.capacity >= 2.5 * cfg.n_particles
+    assert eng.push_keys and eng.inplace_resample and eng.compact_resampled_every == 4
+    st0 = ph.initial_state(cfg)
+    eng.load_state(st0)
+    eng.initialize_fields()
+    setup = O.Setup(dim=1, dt=cfg.dt, xs_mode=cfg.xs_mode, n_cells=cfg.n_cells,
+                    length_cm=cfg.length_cm, bnf=cfg.bnf, gas_density_m3=cfg.gas_density_m3,
+                    nu_max=eng.table.nu_max, push_order=cfg.push_order)
+    tab = (eng.table.energy, eng.table.sig, eng.table.slope)
+    pol = _policy(cfg)
+    fp = eng.fp
+    fired = merges = compactions = 0
+    for i in range(6):
+        _, step, t, _ = eng.clock()
+        # resampling in place (the keys the last push wrote, or k_keys at first)
+        before = eng.fetch_slots()
+        c0 = eng.counters()
+        eng.resample()
+        after = eng.fetch_slots()
+        c1 = eng.counters()
+        want, wc = O.resample_inplace(setup, pol, cfg.nb, cfg.bin_thresholds, eng.targets_host,
+                                      cfg.seed, step, before, capacity=eng.capacity)
+        assert c1["split"] - c0["split"] == wc[8] and c1["merge"] - c0["merge"] == wc[9]
+        np.testing.assert_array_equal(after[0], want[0])
+        np.testing.assert_array_equal(after[4] > 0, want[4] > 0)
+        _close(after, want, 4, what=f"resample {i}")
+        fired += int(wc[10])
+        merges += int(wc[9])
+        # compaction when due: a stable filter of the live slots
+        if eng._compact_due():
+            eng.compact()
+            compactions += 1
+            comp = eng.fetch_slots()
+            for a, b in zip(comp, _live(after)):
+                np.testing.assert_array_equal(a, b)
+        # the push in the field the device holds
+        src = eng.fetch_slots()
+        E = eng.E_t.cpu().numpy()
+        c0 = eng.counters()
+        eng.push()
+        got = _live(eng.fetch_slots())
+        mom = eng.moments.cpu().numpy().reshape(3, -1)
+        c1 = eng.counters()
+        R = O.collision_uniforms(cfg.seed, step, src[0].size, np.float32)
+        wst, wmom, wcnt = O.push_step(setup, src, E, R, table=tab, capacity=eng.capacity)
+        for k, q in ((0, "births"), (1, "absorbed"), (2, "elastic"), (3, "excitation"),
+                     (4, "ionization")):
+            assert c1[q] - c0[q] == wcnt[k], (i, q)
+        _close(got, wst, 4, what=f"push {i}")
+        np.testing.assert_allclose(mom, wmom, rtol=1e-6, atol=1e-6 * np.abs(wmom).max())
+        # the field step on the device's moments
+        ni0, E0 = eng.ni.cpu().numpy(), eng.E.cpu().numpy()
+        eng.field_step(advance=True)
+        N = cfg.n_cells
+        ni, ns, Te, g2, V, Ew = O.field_step(
+            N, fp.dx, fp.dt, fp.eps, fp.v0, fp.freq, fp.mu_i, fp.D_i, fp.D_s, fp.fluid_on,
+            fp.poisson_on, t + cfg.dt, eng.moments.cpu().numpy(), ni0, np.zeros(N + 1),
+            np.zeros(N + 1), np.zeros(N + 1), np.zeros(N - 1), E0,
+            eng.chol.cpu().numpy(), ion_scheme=fp.ion_scheme, max_substeps=fp.max_substeps)
+        np.testing.assert_allclose(eng.E.cpu().numpy(), Ew, rtol=1e-9,
+                                   atol=1e-9 * np.abs(Ew).max())
+        np.testing.assert_allclose(eng.ni.cpu().numpy(), ni, rtol=1e-10)
+    assert fired == 6 and merges > 0 and compactions >= 1
+    assert eng.counters()["overflow"] == 0
+    eng.close()
+
+
+def test_c0_at_its_configuration(gpu):
+    from boltzmann_amd import configs
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    cfg = configs.c0()
+    assert cfg.n_particles == 100_000 and cfg.e_over_n_td == 100.0 and cfg.te_ev == 1.0
+    eng = KineticEngine(cfg)
+    st = ph.initial_state(cfg)
+    eng.load_state(st)
+    setup = O.Setup(dim=0, dt=cfg.dt, xs_mode=cfg.xs_mode, gas_density_m3=cfg.gas_density_m3,
+                    nu_max=eng.table.nu_max, e_field0=cfg.e_field0_vcm())
+    tab = (eng.table.energy, eng.table.sig, eng.table.slope)
+    coll = 0
+    for step in range(5):
+        src = eng.fetch_state()
+        c0 = eng.counters()
+        eng.push()
+        eng.field_step(advance=True)
+        got = eng.fetch_state()
+        c1 = eng.counters()
+        R = O.collision_uniforms(cfg.seed, step, src[0].size, np.float64)
+        want, wmom, wcnt = O.push_step(setup, src, None, R, table=tab, capacity=eng.capacity)
+        for k, q in ((0, "births"), (2, "elastic"), (3, "excitation"), (4, "ionization"),
+                     (5, "null")):
+            assert c1[q] - c0[q] == wcnt[k], (step, q)
+        coll += int(wcnt[2])
+        assert got[0].size == want[0].size
+        for k in (1, 2, 3, 4):
+            sp = np.sqrt(want[1] ** 2 + want[2] ** 2 + want[3] ** 2) + 1.0
+            assert np.all(np.abs(got[k] - want[k]) <= 1e-12 * (sp if k < 4 else want[4]))
+        mom = eng.moments.cpu().numpy().reshape(3, -1)
+        np.testing.assert_allclose(mom, wmom, rtol=1e-12)
+    assert coll > 0
+    eng.close()
+
+
+def _trigger_state(nb, counts, N, seed):
+    from boltzmann_amd import physics as ph
+    rng = np.random.default_rng(seed)
+    KEV = 0.5 * O.M_E * O.J_EV
+    dx = 2.5 / N
+    thr = list(ph.bin_thresholds(nb))
+    centre = [0.5 * (lo + hi) for lo, hi in zip([0.0] + thr, thr + [40.0])]
+    xs, vs = [], []
+    for c in range(N + 1):
+        for b in range(nb):
+            m = counts[c][b]
+            xs.append(np.clip(rng.uniform((c - 0.4) * dx, (c + 0.4) * dx, m), 1e-6, 2.5 - 1e-6))
+            d = rng.normal(size=(3, m))
+            d /= np.linalg.norm(d, axis=0)
+            vs.append(np.sqrt(centre[b] / KEV) * d)
+    x = np.concatenate(xs)
+    v = np.concatenate(vs, axis=1)
+    return (x, v[0], v[1], v[2], np.full(x.size, 1.3))
+
+
+@pytest.mark.parametrize("ref", [0, 1])
+def test_trigger_on_the_device(gpu, ref):
+    """trigger_at = 2: the pass fires iff some segment reaches twice its
+    target.  Reference rule (quirk Q1, 3bin/boltzmann.py:1110-1113): only
+    the last energy bin's maximum at or above three times its target fires."""
+    import torch
+
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    N, nb = 8, 3
+    targets = np.tile(np.array([70, 20, 10], np.int32), N + 1)
+    cases = []
+    base = [[50, 15, 8] for _ in range(N + 1)]
+    low_crowd = [r[:] for r in base]
+    low_crowd[3][0] = 160          # 2.3x the low-energy target: fires at 2x, not Q1
+    high_crowd = [r[:] for r in base]
+    high_crowd[5][2] = 31          # 3.1x the high-energy target: fires both ways
+    for counts, fire_new, fire_ref in ((base, False, False), (low_crowd, True, False),
+                                       (high_crowd, True, True)):
+        cases.append((counts, fire_ref if ref else fire_new))
+    s = O.Setup(dim=1, dt=1e-11, n_cells=N, length_cm=2.5)
+    for j, (counts, expect) in enumerate(cases):
+        st = tuple(np.ascontiguousarray(a, np.float64) for a in
+                   _trigger_state(nb, counts, N, seed=j))
+        cfg = ph.RunConfig(name="trig", dim=1, n_particles=st[0].size, precision=8, nb=nb,
+                           n_cells=N, length_cm=2.5, dt=1e-11, target_ppc=100,
+                           trigger_at=2.0, merge_at=2.0, bin_fractions=(0.7, 0.2, 0.1),
+                           bin_thresholds=ph.bin_thresholds(nb))
+        for inplace in (True, False):
+            eng = KineticEngine(cfg, capacity=4 * st[0].size + 1024, inplace_resample=inplace)
+            eng.targets_host = targets
+            eng.targets.copy_(torch.from_numpy(targets))
+            eng.policy.ref_trigger = ref
+            eng.load_state(st)
+            if not inplace:
+                # the key-order path decides on the host from the counts
+                eng.bin_count()
+                assert eng.trigger(eng.counts.cpu().numpy()) == expect
+            eng.resample(force=False)
+            assert eng.counters()["resample"] == int(expect), (j, inplace)
+            eng.close()
+        pol = O.policy(split_below=0.5, merge_at=2.0, merge_keep=0.6, trigger_at=2.0,
+                       min_wt=0.1, split_to=2.0, ref_trigger=ref)
+        _, wc = O.resample_inplace(s, pol, nb, ph.bin_thresholds(nb), targets, 1, 0, st)
+        assert wc[10] == int(expect)

@@ -54,6 +54,9 @@ def test_resample_fixture(nb):
     np.testing.assert_array_equal(np.stack(out), f["out"])
     np.testing.assert_array_equal(cnt, f["counters"])
     assert cnt[8] > 0 and cnt[9] > 0  # one split and one merge at least
+    ip, ci = G.resample_inplace_expected(inp)
+    np.testing.assert_array_equal(np.stack(ip), f["out_inplace"])
+    np.testing.assert_array_equal(ci, f["counters"])
 
 
 @pytest.mark.parametrize("scheme", [0, 1])

@@ -579,3 +579,197 @@ def test_free_flight_reduces_to_null_collision_rates():
     for k in (2, 3, 4):
         a, b = int(c_ff[k]), int(c_nc[k])
         assert abs(a - b) <= 5 * math.sqrt(a + b) + 5, (k, a, b)
+
+
+# The reference's argon fits as electron_kernel_numba.py:67-171 writes them:
+# log10 sigma = (a L + b) L + c, L = log10 e, one (a, b, c) per interval.
+REF_EL = [(-0.02704763, -0.23720051, -19.67900951), (-0.08847237, -0.6084786, -20.24111992),
+          (-0.37608274, -1.8167778, -21.51414308), (-1.4874467, -4.82420619, -23.55745478),
+          (-0.9870356, -4.2206026, -23.44715988), (14.28063581, 18.92275458, -14.70556113),
+          (-2.12069169, 0.7555105, -19.71943671), (-0.3585636, 0.79246666, -19.71260274),
+          (1.25262128, -0.40002029, -19.48909003), (-2.28332905, 4.89566076, -21.46789173),
+          (-1.47508661, 2.82263476, -20.1928456), (-0.11090525, -0.77506081, -17.81752307)]
+REF_EL_BRK = [0.000780396, 0.006098667, 0.0476269, 0.1015574, 0.1943527, 0.2995781, 1.097156,
+              2.339524, 4.988691, 10.63765, 22.68322]
+REF_ION = [(-2149239.99337822, 5150451.29014456, -3085665.64874862),
+           (-227229.83440255, 545171.14839945, -327016.69508374),
+           (-20173.68563579, 48599.13254342, -29290.96100603),
+           (-1838.90888364, 4491.58838596, -2763.82866252),
+           (-186.98923927, 477.04784782, -324.75732578),
+           (-24.48843901, 69.43965607, -69.145292411),
+           (-4.33101714, 14.47124519, -31.66224458),
+           (-0.68345667, 2.6053539, -21.99823511)]
+REF_ION_BRK = [15.77742, 15.81964, 15.96196, 16.44162, 18.05837, 23.50771, 41.87498]
+E_ION = 15.76
+
+
+def _ref_fit(e):
+    """numpy restatement of the interval sums (electron_kernel_numba.py:115-174):
+    each interval's indicator times its polynomial, then 10**; ionization
+    zero below 15.76 eV and 10**0 = 1 at exactly 15.76 eV, where no interval
+    holds (the first starts strictly above it)."""
+    e = np.asarray(e, dtype=np.float64)
+    L = np.log10(e)
+    lg0 = np.zeros_like(e)
+    edges = [-np.inf] + REF_EL_BRK + [np.inf]
+    for k, (a, b, c) in enumerate(REF_EL):
+        ind = (edges[k] <= e) & (e < edges[k + 1]) if k else (e < edges[1])
+        lg0 = lg0 + ind * ((a * L + b) * L + c)
+    lg2 = np.zeros_like(e)
+    iedges = [E_ION] + REF_ION_BRK + [np.inf]
+    for k, (a, b, c) in enumerate(REF_ION):
+        ind = (iedges[k] < e) & (e < iedges[k + 1]) if k == 0 else \
+            (iedges[k] <= e) & (e < iedges[k + 1])
+        lg2 = lg2 + ind * ((a * L + b) * L + c)
+    s0 = 10.0 ** lg0
+    s2 = np.where(e < E_ION, 0.0, 10.0 ** lg2)
+    return s0, s2
+
+
+def test_fit_cross_sections_all_intervals():
+    """The oracle's fits against the restatement over 1e-4 .. 1000 eV: every
+    elastic interval (12) and ionization interval (8), both sides of every
+    breakpoint, and the reference's sigma_ion = 1 m^2 at e = 15.76 eV exactly
+    (quirk Q13)."""
+    nn = 3.22e21
+    rng = np.random.default_rng(17)
+    pts = [np.exp(rng.uniform(np.log(1e-4), np.log(1e3), 4000))]
+    for b in REF_EL_BRK + REF_ION_BRK:
+        pts.append(np.array([b * (1 - 1e-12), b, b * (1 + 1e-12), b * 0.99, b * 1.01]))
+    pts.append(np.array([E_ION, np.nextafter(E_ION, 0), np.nextafter(E_ION, 100), 15.765]))
+    e = np.concatenate(pts)
+    s = O.Setup(dim=0, dt=1e-12, xs_mode=0, gas_density_m3=nn)
+    s_el, s_ion = O.fit_sigma(s, e)
+    r0, r2 = _ref_fit(e)
+    np.testing.assert_allclose(s_el, r0 * nn, rtol=1e-12)
+    np.testing.assert_allclose(s_ion, r2 * nn, rtol=1e-12, atol=0)
+    # every interval was visited
+    assert len(np.unique(np.searchsorted(REF_EL_BRK, e, side="right"))) == 12
+    above = e[e > E_ION]
+    assert len(np.unique(np.searchsorted(REF_ION_BRK, above, side="right"))) == 8
+    # Q13 and the threshold
+    q = O.fit_sigma(s, np.array([E_ION, np.nextafter(E_ION, 0)]))[1]
+    assert q[0] == nn and q[1] == 0.0
+    # a wrong coefficient anywhere would show: perturb one interval of the
+    # restatement and the comparison fails there
+    bad = list(REF_ION)
+    bad[3] = (bad[3][0], bad[3][1] * (1 + 1e-7), bad[3][2])
+    saved = REF_ION[:]
+    REF_ION[:] = bad
+    try:
+        assert not np.allclose(_ref_fit(e)[1] * nn, s_ion, rtol=1e-12)
+    finally:
+        REF_ION[:] = saved
+
+
+def test_fit_cross_sections_fp32_follow_fp64():
+    """FP32 fits (the polynomial in FP32, 10** in FP32) stay within FP32
+    rounding of the FP64 restatement, interval by interval."""
+    nn = 3.22e21
+    e = np.exp(np.random.default_rng(3).uniform(np.log(1e-3), np.log(900.0), 3000))
+    s = O.Setup(dim=0, dt=1e-12, xs_mode=0, gas_density_m3=nn)
+    e32 = e.astype(np.float32)
+    a, b = O.fit_sigma(s, e32)
+    ed = e32.astype(np.float64)
+    r0, r2 = _ref_fit(ed)
+    L = np.abs(np.log10(ed))
+
+    def bound(coef, brk, lo):
+        # FP32 rounding of the polynomial: a few ulps of its largest term,
+        # times ln 10 in sigma (the ionization fits cancel terms of ~1e6)
+        k = np.searchsorted(brk, ed, side="right")
+        t = np.array([abs(coef[j][0]) * L[i] ** 2 + abs(coef[j][1]) * L[i] + abs(coef[j][2])
+                      for i, j in enumerate(np.minimum(k, len(coef) - 1))])
+        return 4 * math.log(10) * 2.0 ** -24 * t + 1e-6
+
+    assert np.all(np.abs(a / nn / r0 - 1) <= bound(REF_EL, REF_EL_BRK, None))
+    m = r2 > 0
+    assert np.all(np.abs(b[m] / nn / r2[m] - 1) <= bound(REF_ION, REF_ION_BRK, None)[m])
+
+
+def test_gas_velocity_draw_is_normal():
+    """The gas velocities of the Philox stream (Box-Muller, purpose 1): each
+    component N(0, sigma^2) (Kolmogorov-Smirnov), uncorrelated with the
+    others, independent of the collision stream, and deterministic in
+    (seed, step, index).  The reference draws multivariate_normal(0, Eg*I)
+    (1bin/boltzmann.py:589)."""
+    from scipy import stats
+    n, sigma = 200_000, 1.5e4
+    G = O.gas_velocities(12345, 7, n, np.float64, sigma)
+    for c in range(3):
+        assert abs(G[c].mean()) < 5 * sigma / math.sqrt(n)
+        assert abs(G[c].var() / sigma ** 2 - 1) < 5 * math.sqrt(2 / n)
+        assert stats.kstest(G[c] / sigma, "norm").pvalue > 1e-3
+    cc = np.corrcoef(G)
+    assert np.all(np.abs(cc - np.eye(3)) < 5 / math.sqrt(n))
+    R = O.collision_uniforms(12345, 7, n, np.float64)
+    assert abs(np.corrcoef(G[0], R[0])[0, 1]) < 5 / math.sqrt(n)
+    np.testing.assert_array_equal(G, O.gas_velocities(12345, 7, n, np.float64, sigma))
+    assert not np.array_equal(G, O.gas_velocities(12345, 8, n, np.float64, sigma))
+    G32 = O.gas_velocities(12345, 7, n, np.float32, sigma)
+    assert abs(G32[2].var() / sigma ** 2 - 1) < 5 * math.sqrt(2 / n)
+
+
+def test_dead_ionizer_flag_at_node_0():
+    """ref_flags 1: a primary that ionizes and leaves in the same step adds
+    its unscaled flag 1 to ni_src at node 0 (the reference reduces every row,
+    the dead one parked at x = 1e-8, post_processing_1D.py:48-55).  Particles
+    packed against the right wall move out at 120 eV with no field: without
+    a collision they leave.  The ones that ionized are the births born there;
+    an ionized primary keeps e_ej = (120 - 15.76)/2 eV in a random direction,
+    so the survivors at the wall with that energy are the ionizers that
+    stayed, and the rest left: exactly that many units appear at node 0."""
+    n, L, N = 4000, 2.5, 25
+    rng = np.random.default_rng(12)
+    x = np.concatenate([rng.uniform(L - 1e-6, L - 1e-7, n // 2), rng.uniform(0.5, 2.0, n // 2)])
+    sp = math.sqrt(120.0 / KEV)
+    vx = np.concatenate([np.full(n // 2, sp), np.full(n // 2, -1e3)])
+    st = (x, vx, np.zeros(n), np.zeros(n), np.full(n, 0.7))
+    R = O.collision_uniforms(3, 0, n, np.float64)
+    s0 = O.Setup(dim=1, dt=1e-10, xs_mode=0, n_cells=N, length_cm=L, gas_density_m3=3e23)
+    s1 = O.Setup(dim=1, dt=1e-10, xs_mode=0, n_cells=N, length_cm=L, gas_density_m3=3e23,
+                 ref_flags=1)
+    out0, m0, c0 = O.push_step(s0, st, np.zeros(N), R)
+    out1, m1, c1 = O.push_step(s1, st, np.zeros(N), R)
+    nb_ = int(c1[0])
+    births_at_wall = int(np.sum(out1[0][out1[0].size - nb_:] > L - 1e-6))
+    surv = tuple(a[:out1[0].size - nb_] for a in out1)
+    e_surv = KEV * (surv[1] ** 2 + surv[2] ** 2 + surv[3] ** 2)
+    e_ej = 0.5 * (KEV * sp * sp - 15.76)
+    stayed = int(np.sum((surv[0] > L - 0.1) & (np.abs(e_surv / e_ej - 1) < 1e-9)))
+    assert births_at_wall > 10 and stayed > 10
+    assert m1[1][0] - m0[1][0] == births_at_wall - stayed
+    m1[1][0] = m0[1][0]
+    np.testing.assert_array_equal(m0, m1)
+    for a, b in zip(out0, out1):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_energy_roundtrip_binning_moves_threshold_energies():
+    """ref_flags 2: the binning energy is (e*w)/w, the value the reference's
+    tosum[:,3] holds when it bins (post_processing_1D.py:53,62).  An energy
+    one ulp below the 10 eV threshold rounds up to 10 for some weights:
+    those particles move to the next bin, and the keys follow the
+    round-tripped energy exactly."""
+    N, L, nb, thr = 4, 2.5, 3, (10.0, 15.76)
+    below = np.nextafter(10.0, 0.0)
+    v = math.sqrt(10.0 / KEV)
+    for _ in range(200):  # the speed whose energy is one ulp below 10 eV
+        if KEV * v * v == below:
+            break
+        v = np.nextafter(v, 0.0 if KEV * v * v > below else 1e9)
+    assert KEV * v * v == below
+    rng = np.random.default_rng(4)
+    m = 4000
+    w = rng.uniform(0.1, 50.0, m)
+    st = (np.full(m, 1.0), np.full(m, v), np.zeros(m), np.zeros(m), w)
+    s0 = O.Setup(dim=1, dt=1e-11, n_cells=N, length_cm=L)
+    s2 = O.Setup(dim=1, dt=1e-11, n_cells=N, length_cm=L, ref_flags=2)
+    c0, _, _ = O.bin_sort(s0, nb, thr, st)
+    c2, _, _ = O.bin_sort(s2, nb, thr, st)
+    node = int((1.0 + 0.5 * L / N) / (L / N))
+    er = (below * w) / w
+    flips = int(np.sum(er >= 10.0))
+    assert flips > 10  # about 1 % of the weights
+    assert c0[node * nb] == m and c0[node * nb + 1] == 0
+    assert c2[node * nb] == m - flips and c2[node * nb + 1] == flips

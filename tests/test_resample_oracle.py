@@ -184,3 +184,71 @@ def test_reference_trigger_quirk():
     _, cnt_new = O.resample(s, O.policy(**POL), nb, thr, targets, 1, 0, st)
     assert cnt_ref[10] == 0  # low-energy crowding alone does not fire the reference check
     assert cnt_new[10] == 1
+
+
+def test_split_draws_follow_the_weights_chi2():
+    """The split's categorical draws land on particle j with probability
+    w_j / sum(w) over w > min_wt, and never on a light particle
+    (cp.random.choice(HW_ids, ..., p=w/sum(w)), 1bin/boltzmann.py:809-814):
+    copies per particle pooled over 40 passes (steps) against the expected
+    counts, chi-square.  A uniform choice would fail it."""
+    from scipy import stats
+    N, L, target = 1, 2.5, 4000
+    st = _cell_state([120, 0], N, L, np.float64, seed=31)
+    w = np.random.default_rng(8).choice([0.05, 0.3, 1.0, 3.0, 9.0], st[4].size)
+    st = st[:4] + (w,)
+    s = O.Setup(dim=1, dt=1e-11, n_cells=N, length_cm=L)
+    pol = O.policy(**{**POL, "trigger_at": 0.0})
+    tg = np.full(2, target, np.int32)
+    n = w.size
+    copies = np.zeros(n)
+    passes = 40
+    for step in range(passes):
+        out, cnt = O.resample_inplace(s, pol, 1, (), tg, 5, step, st)
+        assert cnt[8] == 1
+        # in place: slot j keeps w / (copies + 1)
+        copies += np.rint(w / out[4][:n]) - 1
+    draws = (2 * target - n) * passes
+    heavy = w > 0.1
+    assert np.all(copies[~heavy] == 0)
+    p = w[heavy] / w[heavy].sum()
+    chi2, pv = stats.chisquare(copies[heavy], draws * p)
+    assert pv > 1e-3, (chi2, pv)
+    # the same counts against a uniform choice are rejected
+    assert stats.chisquare(copies[heavy], np.full(p.size, draws / p.size)).pvalue < 1e-6
+
+
+def test_inplace_layout_is_the_reference_layout():
+    """In-place resampling leaves untouched segments where they are, keeps a
+    split particle in its slot with w/(c+1) followed (at the end) by its
+    clones, and turns merged members into holes with the K merged particles
+    appended; after the compaction the live particles are those of the
+    key-order resampler, as a multiset, with equal totals."""
+    N, L, nb, thr = 6, 2.5, 3, (10.0, 15.76)
+    st = _cell_state([30, 500, 900, 60, 10, 700, 5], N, L, np.float64, seed=2, e_ev=8.0)
+    s = O.Setup(dim=1, dt=1e-11, n_cells=N, length_cm=L)
+    targets = np.tile(np.array([210, 60, 30], np.int32), N + 1)
+    pol = O.policy(**{**POL, "trigger_at": 0.0})
+    ip, ci = O.resample_inplace(s, pol, nb, thr, targets, 4, 1, st)
+    ko, ck = O.resample(s, pol, nb, thr, targets, 4, 1, st)
+    assert np.array_equal(ci, ck) and ci[8] > 0 and ci[9] > 0
+    n = st[0].size
+    key = _keys_numpy(st, N, L, nb, thr)
+    counts = np.bincount(key, minlength=(N + 1) * nb + 1)
+    for j in range(n):
+        k = key[j]
+        t = targets[k] if k < targets.size else 0
+        c = counts[k]
+        if t and c >= POL["merge_at"] * t:
+            assert ip[4][j] == 0.0  # merged member: a hole
+        elif t and 0 < c < 0.5 * t:
+            assert ip[4][j] > 0 and ip[4][j] <= st[4][j]  # split in place
+        else:
+            for a, b in zip(ip, st):
+                assert a[j] == b[j]  # untouched
+    live = ip[4] > 0
+    a = np.stack([v[live] for v in ip]).T
+    b = np.stack(list(ko)).T
+    assert a.shape == b.shape
+    # the same rows: lexicographic sort of both
+    np.testing.assert_array_equal(a[np.lexsort(a.T[::-1])], b[np.lexsort(b.T[::-1])])

@@ -87,3 +87,39 @@ def test_nu_max_bounds_collision_frequency():
     nu = (sig * v).sum(axis=0)
     assert nu.max() <= t.nu_max
     assert nu.max() > 0.9 * t.nu_max
+
+
+def test_reference_initial_condition():
+    """The Liu initial condition as 1bin/boltzmann.py:242-333 writes it, restated
+    here: ppc particles per node at x_ii (+dx/4 at the first node, -dx/4 at
+    the last), weight 0.1 (1e7 + 1e9 (1-x/L)^2 (x/L)^2) / ppc / num_nodes,
+    velocity variance 0.5 * 1.5 * 1.17255e11 per component, every row moved
+    by one shuffle (each particle keeps its own node's weight)."""
+    import math
+
+    from boltzmann_amd.boltzmann import liu_initial_state, reference_config
+    Nc, gridpts, nodes = 24_000, 41, 2
+    cfg = reference_config(Nc, gridpts, 4, 1e5, nb=3)
+    N, L = gridpts - 1, 2.54
+    ppc = sum(int(f * Nc / N) for f in (0.7, 0.2, 0.1))
+    assert cfg.target_ppc == ppc
+    (x, vx, vy, vz, w), ne = liu_initial_state(cfg, seed=5, num_nodes=nodes)
+    assert x.size == ppc * (N + 1)
+    dx = L / N
+    pos = np.linspace(0, L, N + 1)
+    pos[0] += dx / 4.0
+    pos[N] -= dx / 4.0
+    node = np.argmin(np.abs(x[:, None] - pos[None, :]), axis=1)
+    np.testing.assert_array_equal(x, pos[node])
+    assert np.all(np.bincount(node, minlength=N + 1) == ppc)
+    xs = np.linspace(0, L, N + 1)
+    ne_ref = 0.1 * (1e7 + 1e9 * (1 - xs / L) ** 2 * (xs / L) ** 2)
+    np.testing.assert_allclose(ne, ne_ref, rtol=1e-15)
+    np.testing.assert_allclose(w, ne_ref[node] / ppc / nodes, rtol=1e-15)
+    var = 0.5 * 1.5 * 1.17255e11
+    n = x.size
+    for v in (vx, vy, vz):
+        assert abs(v.mean()) < 5 * math.sqrt(var / n)
+        assert abs(v.var() / var - 1) < 5 * math.sqrt(2 / n)
+    # shuffled: nodes are not in blocks, and not in order
+    assert np.mean(np.diff(node) != 0) > 0.9

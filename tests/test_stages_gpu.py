@@ -30,6 +30,12 @@ def test_reduction_brute_force(gpu):
     got = dw.cpu().numpy()
     np.testing.assert_allclose(got[:RM], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
     assert np.all(got[RM:] == 0.0)
+    # signed fixed-point sums: the same bits on every call (the reference's
+    # FP64 atomics are not reproducible)
+    dw2 = torch.full((M, r), -3.0, dtype=torch.float64, device=dev)
+    red.reduction(r, 0, M, RM, dw2, torch.from_numpy(dv).to(dev), torch.from_numpy(dx).to(dev),
+                  ptcl)
+    np.testing.assert_array_equal(dw2.cpu().numpy(), got)
 
 
 def test_electron_kernel_and_post_processing(gpu):
