@@ -45,13 +45,22 @@ struct SortWs {
   int32_t* ip_counts;   // [max_keys+1] slots per key
   int32_t* ip_moff;     // [max_keys+2] member offsets per key
   int32_t* ip_mcnt;     // [max_keys+1] members per key
-  int32_t* ip_cand;     // [max_keys] 0 untouched, 1 split candidate, 2 merge
-  uint32_t* ip_bcnt;    // members per extraction block
+  int32_t* ip_cand;     // [max_keys+1] 0 untouched, 1 split candidate, 2 merge; [max_keys]: fire
+  uint32_t* ip_bcnt;    // members per extraction wave, then their exclusive prefixes
   Clock* ip_mclk;       // member count (the radix kernels read it as a clock)
+  long long* ip_chunk;  // per 1024-key chunk: totals and prefixes of the plan scans
+  int32_t* ip_tmp;      // [2][max_keys+1] chunk-local split / merge list positions
+  uint32_t* ip_bits;    // [max_keys/32 + 1] candidate keys as a bit mask
 };
 
-// blocks of the member extraction (contiguous slot ranges)
+// member extraction: waves of 4-wave blocks, each wave a contiguous slot range
 constexpr int kIpBlocks = 2048;
+constexpr int kIpWaves = kIpBlocks * 4;
+// plan scans: 1024 keys per block (n_keys <= 65534: at most 64 chunks); per
+// chunk 8 words: member total, appended, splits, merges and their prefixes
+constexpr int kKeyChunk = 1024;
+constexpr int kMaxKeyChunks = 64;
+constexpr int kIpChunkWords = 8 * kMaxKeyChunks + 8;
 
 static int64_t tiles_for(int64_t cap) { return cap / kSortTile + 2; }
 
@@ -84,8 +93,11 @@ static SortWs carve(void* base, int64_t cap, int32_t max_keys) {
   w.ip_moff = (int32_t*)take(4 * ((size_t)max_keys + 2));
   w.ip_mcnt = (int32_t*)take(4 * ((size_t)max_keys + 1));
   w.ip_cand = (int32_t*)take(4 * ((size_t)max_keys + 1));
-  w.ip_bcnt = (uint32_t*)take(4 * (size_t)kIpBlocks);
+  w.ip_bcnt = (uint32_t*)take(4 * (size_t)kIpWaves);
   w.ip_mclk = (Clock*)take(sizeof(Clock));
+  w.ip_chunk = (long long*)take(8 * (size_t)kIpChunkWords);
+  w.ip_tmp = (int32_t*)take(8 * ((size_t)max_keys + 1));
+  w.ip_bits = (uint32_t*)take(4 * ((size_t)max_keys / 32 + 2));
   return w;
 }
 
@@ -110,8 +122,11 @@ size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys) {
   add(4 * ((size_t)max_keys + 2));
   add(4 * ((size_t)max_keys + 1));
   add(4 * ((size_t)max_keys + 1));
-  add(4 * (size_t)kIpBlocks);
+  add(4 * (size_t)kIpWaves);
   add(sizeof(Clock));
+  add(8 * (size_t)kIpChunkWords);
+  add(8 * ((size_t)max_keys + 1));
+  add(4 * ((size_t)max_keys / 32 + 2));
   return b + 1024;
 }
 
@@ -1240,32 +1255,56 @@ __global__ __launch_bounds__(kIpHistBlock) void k_ip_hist(const uint32_t* __rest
     if (s_c[j]) atomicAdd(&counts[j], (int32_t)s_c[j]);
 }
 
-// The trigger and the candidates: cand[k] = 1 for a split candidate (its
-// heavy-particle count is not known yet), 2 for a merge, 0 otherwise; all 0
-// when the pass does not fire.  One block.
-__global__ __launch_bounds__(1024) void k_ip_plan1(ResArgs a, const int32_t* counts,
-                                                   const int32_t* targets, int nb,
-                                                   int32_t* cand) {
-  __shared__ int s_fire;
-  const bz_resample_policy_t& P = a.pol;
-  if (threadIdx.x == 0) s_fire = (!P.ref_trigger && P.trigger_at <= 0.0) ? 1 : 0;
+// Block-wide exclusive scan of NV values per thread (1024 threads); tot gets
+// the block totals.  s holds NV x 16 words.
+template <int NV>
+__device__ __forceinline__ void block_scan1024(long long (&v)[NV], long long (&tot)[NV],
+                                               long long (*s)[16]) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  long long inc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) inc[q] = wave_incl_scan(v[q]);
+  if (lane == 63)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) s[q][wid] = inc[q];
   __syncthreads();
-  for (int k = threadIdx.x; k < a.n_keys; k += 1024) {
-    const int c = counts[k], t = targets[k];
-    if (t <= 0 || c <= 0) continue;
-    if (P.ref_trigger) {
-      // only the last bin's maximum decides (3bin/boltzmann.py:1110-1113)
-      if ((k % nb) == nb - 1 && (double)c >= 3.0 * (double)t) s_fire = 1;
-    } else if ((double)c >= P.trigger_at * (double)t) {
-      s_fire = 1;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    long long before = 0, t = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wid) before += s[q][w];
+      t += s[q][w];
     }
+    tot[q] = t;
+    v[q] = before + inc[q] - v[q];
   }
   __syncthreads();
-  const int fire = s_fire;
-  for (int k = threadIdx.x; k < a.n_keys; k += 1024) {
-    const int c = counts[k], t = targets[k];
-    int kind = 0;
-    if (fire && t > 0) {
+}
+
+// The candidates and the trigger, one key per thread: cand[k] = 1 for a
+// split candidate (its heavy-particle count is not known yet), 2 for a
+// merge, 0 otherwise, as if the pass fired; chunk-local member offsets of the
+// candidates into moff; a key that trips the trigger sets the fire word.
+__global__ __launch_bounds__(kKeyChunk) void k_ip_cand(ResArgs a, const int32_t* counts,
+                                                       const int32_t* targets, int nb,
+                                                       int32_t* cand, int32_t* moff,
+                                                       long long* chunk, uint32_t* bits) {
+  __shared__ long long s[1][16];
+  const bz_resample_policy_t& P = a.pol;
+  const int k = blockIdx.x * kKeyChunk + threadIdx.x;
+  int kind = 0, c = 0;
+  bool trig = false;
+  if (k < a.n_keys) {
+    c = counts[k];
+    const int t = targets[k];
+    if (t > 0 && c > 0) {
+      if (P.ref_trigger)
+        // only the last bin's maximum decides (3bin/boltzmann.py:1110-1113)
+        trig = (k % nb) == nb - 1 && (double)c >= 3.0 * (double)t;
+      else
+        trig = (double)c >= P.trigger_at * (double)t;
+    }
+    if (t > 0) {
       const int keep = (int)(P.merge_keep * (double)t);
       if (c > 0 && (double)c < P.split_below * (double)t) {
         if ((int)(P.split_to * (double)t) - c > 0) kind = 1;
@@ -1276,96 +1315,155 @@ __global__ __launch_bounds__(1024) void k_ip_plan1(ResArgs a, const int32_t* cou
     }
     cand[k] = kind;
   }
-  if (threadIdx.x == 0) cand[a.n_keys] = fire;  // read by the counters of k_ip_plan2
+  // the candidate keys as bits, 64 per wave (two words)
+  const uint64_t cm = __ballot(kind != 0);
+  // (words past the last key are never read: ip_listed checks the key first)
+  if ((threadIdx.x & 63) == 0 && k < a.n_keys) {
+    bits[k >> 5] = (uint32_t)cm;
+    if (k + 32 < a.n_keys) bits[(k >> 5) + 1] = (uint32_t)(cm >> 32);
+  }
+  if (__any(trig) && (threadIdx.x & 63) == 0) atomicOr((int*)&chunk[8 * kMaxKeyChunks], 1);
+  long long v[1] = {kind ? (long long)c : 0ll}, tot[1];
+  block_scan1024<1>(v, tot, s);
+  if (k < a.n_keys) moff[k] = (int32_t)v[0];
+  if (threadIdx.x == 0) chunk[8 * blockIdx.x + 0] = tot[0];
 }
 
-// Extraction of the members of candidate segments, stable: block b owns a
-// contiguous run of slots; k_ip_mcount counts its members, k_ip_mscatter
-// sums the counts of blocks 0..b-1 itself and writes (key, slot) of its
-// members in slot order.
-__device__ __forceinline__ bool ip_listed(const int32_t* cand, uint32_t key, int n_keys) {
-  return key < (uint32_t)n_keys && cand[key] != 0;
+// One block: the trigger (every check fires when trigger_at <= 0 and the
+// reference rule is off), chunk prefixes of the member offsets and the member
+// count (nothing is listed when the pass does not fire).  The fire word is
+// cleared for the next pass; cand[n_keys] keeps the decision.
+__global__ __launch_bounds__(64) void k_ip_cand_carry(ResArgs a, int nch, int32_t* cand,
+                                                      long long* chunk, Clock* mclk) {
+  const bz_resample_policy_t& P = a.pol;
+  if (threadIdx.x != 0) return;
+  int* fw = (int*)&chunk[8 * kMaxKeyChunks];
+  const int fire = ((!P.ref_trigger && P.trigger_at <= 0.0) || *fw) ? 1 : 0;
+  *fw = 0;
+  cand[a.n_keys] = fire;
+  long long acc = 0;
+  for (int b = 0; b < nch; ++b) {
+    const long long t = chunk[8 * b + 0];
+    chunk[8 * b + 4] = acc;
+    acc += t;
+  }
+  mclk->n = fire ? acc : 0;
+  mclk->n_next = mclk->n;
 }
 
-__global__ __launch_bounds__(kCompBlock) void k_ip_mcount(const uint32_t* __restrict__ keys,
-                                                          const Clock* clk,
-                                                          const int32_t* __restrict__ cand,
-                                                          int n_keys, uint32_t* bcnt) {
-  __shared__ uint32_t s_red[kCompBlock / kWave];
+// Member offsets and counts of every key (zero for keys that are not
+// candidates, or when the pass does not fire).
+__global__ __launch_bounds__(kKeyChunk) void k_ip_cand_apply(ResArgs a, const int32_t* counts,
+                                                             const int32_t* cand,
+                                                             const long long* chunk,
+                                                             int32_t* moff, int32_t* mcnt) {
+  const int k = blockIdx.x * kKeyChunk + threadIdx.x;
+  if (k >= a.n_keys) return;
+  const int fire = cand[a.n_keys];
+  moff[k] = (int32_t)(chunk[8 * blockIdx.x + 4] + moff[k]);
+  mcnt[k] = fire && cand[k] ? counts[k] : 0;
+}
+
+// Extraction of the members of candidate segments, stable: global wave g
+// owns a contiguous run of slots (whole 512-slot rows of 8 per lane);
+// k_ip_wcount counts its members, k_ip_wscan turns the counts into
+// exclusive offsets, k_ip_wscatter writes (key, slot) of the wave's members
+// in slot order.  No barriers: each wave works alone.
+__device__ __forceinline__ bool ip_listed(const uint32_t* bits, uint32_t key, int n_keys) {
+  return key < (uint32_t)n_keys && ((bits[key >> 5] >> (key & 31)) & 1u);
+}
+
+// the candidate bits into LDS (every block of the extraction): the per-slot
+// lookups are then LDS reads instead of scattered global loads
+__device__ __forceinline__ void ip_stage_bits(const uint32_t* __restrict__ bits, int n_keys,
+                                              uint32_t* s_bits) {
+  const int words = n_keys / 32 + 1;
+  for (int j = threadIdx.x; j < words; j += 256) s_bits[j] = bits[j];
+  __syncthreads();
+}
+
+constexpr int kIpRow = 8;  // keys per lane and row
+__device__ __forceinline__ void ip_wave_range(int64_t n, int64_t* lo, int64_t* hi) {
+  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t rows = (n + 64 * kIpRow - 1) / (64 * kIpRow);
+  const int64_t per = (rows + kIpWaves - 1) / kIpWaves;
+  const int64_t a = g * per * 64 * kIpRow, b = a + per * 64 * kIpRow;
+  *lo = a < n ? a : n;
+  *hi = b < n ? b : n;
+}
+
+__global__ __launch_bounds__(256) void k_ip_wcount(const uint32_t* __restrict__ keys,
+                                                   const Clock* clk,
+                                                   const int32_t* __restrict__ cand, int n_keys,
+                                                   const uint32_t* __restrict__ bits,
+                                                   uint32_t* wcnt) {
+  extern __shared__ uint32_t s_bits[];
+  const int lane = threadIdx.x & 63;
   int64_t lo, hi;
-  comp_range(clk->n, &lo, &hi);
+  ip_wave_range(clk->n, &lo, &hi);
   uint32_t c = 0;
-  // eight keys in flight per thread, then their eight candidate lookups
-  constexpr int U = 8;
-  for (int64_t b = lo; b < hi; b += (int64_t)U * kCompBlock) {
-    uint32_t kv[U];
+  const int fire = cand[n_keys];
+  if (fire) ip_stage_bits(bits, n_keys, s_bits);  // block-uniform
+  if (fire) {
+    for (int64_t b = lo; b < hi; b += 64 * kIpRow) {
+      uint32_t kv[kIpRow];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t i = b + u * kCompBlock + threadIdx.x;
+      for (int u = 0; u < kIpRow; ++u) {
+        const int64_t i = b + u * 64 + lane;
+        kv[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int u = 0; u < kIpRow; ++u) c += ip_listed(s_bits, kv[u], n_keys) ? 1u : 0u;
+    }
+  }
+  c = wave_sum(c);
+  if (lane == 0) wcnt[(int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)] = c;
+}
+
+// exclusive scan of the kIpWaves wave counts in place (one block)
+__global__ __launch_bounds__(1024) void k_ip_wscan(uint32_t* wcnt) {
+  __shared__ long long s[1][16];
+  long long carry = 0;
+  for (int base = 0; base < kIpWaves; base += 1024) {
+    long long v[1] = {(long long)wcnt[base + threadIdx.x]}, tot[1];
+    block_scan1024<1>(v, tot, s);
+    wcnt[base + threadIdx.x] = (uint32_t)(carry + v[0]);
+    carry += tot[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ip_wscatter(const uint32_t* __restrict__ keys,
+                                                     const Clock* clk,
+                                                     const int32_t* __restrict__ cand,
+                                                     int n_keys, const uint32_t* __restrict__ bits,
+                                                     const uint32_t* wpre,
+                                                     uint32_t* mkeys, uint32_t* mslot) {
+  extern __shared__ uint32_t s_bits[];
+  const int lane = threadIdx.x & 63;
+  if (!cand[n_keys]) return;  // block-uniform
+  ip_stage_bits(bits, n_keys, s_bits);
+  int64_t lo, hi;
+  ip_wave_range(clk->n, &lo, &hi);
+  int64_t pos = wpre[(int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)];
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int64_t b = lo; b < hi; b += 64 * kIpRow) {
+    uint32_t kv[kIpRow];
+#pragma unroll
+    for (int u = 0; u < kIpRow; ++u) {
+      const int64_t i = b + u * 64 + lane;
       kv[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) c += ip_listed(cand, kv[u], n_keys) ? 1u : 0u;
-  }
-  c = wave_sum(c);
-  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) bcnt[blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
-}
-
-__global__ __launch_bounds__(kCompBlock) void k_ip_mscatter(const uint32_t* __restrict__ keys,
-                                                            const Clock* clk,
-                                                            const int32_t* __restrict__ cand,
-                                                            int n_keys, const uint32_t* bcnt,
-                                                            uint32_t* mkeys, uint32_t* mslot,
-                                                            Clock* mclk) {
-  constexpr int NW = kCompBlock / kWave;
-  __shared__ uint32_t s_red[NW];
-  __shared__ uint64_t s_wsum[NW];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  int64_t lo, hi;
-  comp_range(clk->n, &lo, &hi);
-  uint32_t s = 0;
-  for (int j = tid; j < (int)blockIdx.x; j += kCompBlock) s += bcnt[j];
-  s = wave_sum(s);
-  if (lane == 0) s_red[wid] = s;
-  __syncthreads();
-  int64_t pos0 = (int64_t)s_red[0] + s_red[1] + s_red[2] + s_red[3];
-  for (int64_t base = lo; base < hi; base += kCompTile) {
-    uint64_t packed = 0;
-    uint32_t kv[kCompItems];
-#pragma unroll
-    for (int k = 0; k < kCompItems; ++k) {
-      const int64_t i = base + k * kCompBlock + tid;
-      kv[k] = i < hi ? keys[i] : 0xFFFFFFFFu;
-      packed |= (uint64_t)(ip_listed(cand, kv[k], n_keys) ? 1u : 0u) << (16 * k);
-    }
-    const uint64_t incl = wave_incl_scan(packed);
-    if (lane == kWave - 1) s_wsum[wid] = incl;
-    __syncthreads();
-    uint64_t before = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      if (w < wid) before += s_wsum[w];
-      tot += s_wsum[w];
-    }
-    const uint64_t excl = before + incl - packed;
-    uint32_t field_base = 0;
-#pragma unroll
-    for (int k = 0; k < kCompItems; ++k) {
-      if ((packed >> (16 * k)) & 1u) {
-        const int64_t d = pos0 + field_base + (int64_t)((excl >> (16 * k)) & 0xFFFFu);
-        mkeys[d] = kv[k];
-        mslot[d] = (uint32_t)(base + k * kCompBlock + tid);
+    for (int u = 0; u < kIpRow; ++u) {
+      const bool lst = ip_listed(s_bits, kv[u], n_keys);
+      const uint64_t m = __ballot(lst);
+      if (lst) {
+        const int64_t d = pos + __popcll(m & lt);
+        mkeys[d] = kv[u];
+        mslot[d] = (uint32_t)(b + u * 64 + lane);
       }
-      field_base += (uint32_t)((tot >> (16 * k)) & 0xFFFFu);
+      pos += __popcll(m);
     }
-    pos0 += field_base;
-    __syncthreads();  // s_wsum is rewritten by the next sub-tile
-  }
-  if (blockIdx.x == gridDim.x - 1 && tid == 0) {
-    mclk->n = pos0;
-    mclk->n_next = pos0;
   }
 }
 
@@ -1392,99 +1490,84 @@ __global__ __launch_bounds__(256) void k_ip_hw(ResArgs a, const int32_t* cand,
   }
 }
 
-// Final kinds, appended counts and their slots (in key order after the n
-// slots in use), the split / merge key lists in key order, the counters and
-// the new slot count.  One block; thread t owns a contiguous run of keys.
-__global__ __launch_bounds__(1024) void k_ip_plan2(ResArgs a, const int32_t* counts,
-                                                   const int32_t* targets, const int32_t* cand,
-                                                   int32_t* plan, Clock* clk, int32_t* klist,
-                                                   int kstride) {
-  __shared__ int64_t s_w[16];
-  __shared__ int s_ws[16], s_wm[16];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+// Final kinds and appended counts, one key per thread, with chunk-local
+// prefixes of the appended particles and of the split / merge lists.
+__global__ __launch_bounds__(kKeyChunk) void k_ip_plan_local(ResArgs a, const int32_t* counts,
+                                                             const int32_t* targets,
+                                                             const int32_t* cand, int32_t* plan,
+                                                             int32_t* tmp, long long* chunk,
+                                                             int kstride) {
+  __shared__ long long s[3][16];
   const bz_resample_policy_t& P = a.pol;
-  const int per = (a.n_keys + 1023) / 1024;
-  const int k0 = tid * per;
-  const int k1 = min(k0 + per, a.n_keys);
-  // every thread's appended count per key first (the loads of all its keys
-  // in flight together), kept for the second pass
-  constexpr int kMaxPer = 64;  // n_keys <= 65534
-  int napp_k[kMaxPer];
-  int64_t app = 0;
-  int ns = 0, nm = 0;
-#pragma unroll
-  for (int q = 0; q < kMaxPer; ++q) {
-    const int k = k0 + q;
-    napp_k[q] = 0;
-    if (k >= k1) continue;
+  const int k = blockIdx.x * kKeyChunk + threadIdx.x;
+  int kind = kCopy, napp = 0;
+  if (k < a.n_keys && cand[a.n_keys]) {
     const int kc = cand[k];
     if (kc == 1) {
-      const int t = targets[k];
-      const int draws = (int)(P.split_to * (double)t) - counts[k];
+      const int draws = (int)(P.split_to * (double)targets[k]) - counts[k];
       if (plan[4 * k + 3] > 0 && draws > 0) {
-        napp_k[q] = draws;
-        ++ns;
+        kind = kSplit;
+        napp = draws;
       }
     } else if (kc == 2) {
-      napp_k[q] = (int)(P.merge_keep * (double)targets[k]);
-      ++nm;
+      kind = kMerge;
+      napp = (int)(P.merge_keep * (double)targets[k]);
     }
-    app += napp_k[q];
   }
-  // block exclusive scan of (app, ns, nm)
-  const int64_t ia = wave_incl_scan(app);
-  const int is = wave_incl_scan(ns);
-  const int im = wave_incl_scan(nm);
-  if (lane == 63) {
-    s_w[wid] = ia;
-    s_ws[wid] = is;
-    s_wm[wid] = im;
-  }
-  __syncthreads();
-  int64_t ea = ia - app;
-  int es = is - ns, em = im - nm;
-  int64_t ta = 0;
-  int ts = 0, tm = 0;
-  for (int w = 0; w < 16; ++w) {
-    if (w < wid) {
-      ea += s_w[w];
-      es += s_ws[w];
-      em += s_wm[w];
-    }
-    ta += s_w[w];
-    ts += s_ws[w];
-    tm += s_wm[w];
-  }
-  const int64_t n = clk->n;
-  int64_t off = n + ea;
-#pragma unroll
-  for (int q = 0; q < kMaxPer; ++q) {
-    const int k = k0 + q;
-    if (k >= k1) continue;
-    const int napp = napp_k[q];
-    int kind = kCopy;
-    if (napp > 0) {
-      if (cand[k] == 1) {
-        kind = kSplit;
-        klist[2 + es++] = k;
-      } else {
-        kind = kMerge;
-        klist[2 + kstride + em++] = k;
-      }
-    }
+  long long v[3] = {(long long)napp, kind == kSplit ? 1ll : 0ll, kind == kMerge ? 1ll : 0ll};
+  long long tot[3];
+  block_scan1024<3>(v, tot, s);
+  if (k < a.n_keys) {
     plan[4 * k + 0] = kind;
     plan[4 * k + 1] = napp;
-    plan[4 * k + 2] = (int32_t)off;  // first appended slot (bounded by the capacity)
-    off += napp;
+    plan[4 * k + 2] = (int32_t)v[0];
+    tmp[k] = (int32_t)(kind == kSplit ? v[1] : (kind == kMerge ? v[2] : -1));
   }
-  if (tid == 0) {
-    klist[0] = ts;
-    klist[1] = tm;
-    clk->n_next = n + ta;
-    if (ts) clk->cnt[BZ_CNT_SPLIT] += (unsigned long long)ts;
-    if (tm) clk->cnt[BZ_CNT_MERGE] += (unsigned long long)tm;
-    if (ts || tm || cand[a.n_keys] != 0) clk->cnt[BZ_CNT_RESAMPLE] += 1ull;
+  if (threadIdx.x == 0) {
+    chunk[8 * blockIdx.x + 1] = tot[0];
+    chunk[8 * blockIdx.x + 2] = tot[1];
+    chunk[8 * blockIdx.x + 3] = tot[2];
   }
+}
+
+// One block: chunk prefixes of appended / split / merge, the new slot count,
+// the list lengths and the counters.
+__global__ __launch_bounds__(64) void k_ip_plan_carry(ResArgs a, int nch, const int32_t* cand,
+                                                      long long* chunk, Clock* clk,
+                                                      int32_t* klist) {
+  if (threadIdx.x != 0) return;
+  long long app = 0, ns = 0, nm = 0;
+  for (int b = 0; b < nch; ++b) {
+    const long long ta = chunk[8 * b + 1], ts = chunk[8 * b + 2], tm = chunk[8 * b + 3];
+    chunk[8 * b + 5] = app;
+    chunk[8 * b + 6] = ns;
+    chunk[8 * b + 7] = nm;
+    app += ta;
+    ns += ts;
+    nm += tm;
+  }
+  klist[0] = (int32_t)ns;
+  klist[1] = (int32_t)nm;
+  clk->n_next = clk->n + app;
+  if (ns) clk->cnt[BZ_CNT_SPLIT] += (unsigned long long)ns;
+  if (nm) clk->cnt[BZ_CNT_MERGE] += (unsigned long long)nm;
+  if (cand[a.n_keys]) clk->cnt[BZ_CNT_RESAMPLE] += 1ull;
+}
+
+// The first appended slot of every key (after the n slots in use, in key
+// order) and the key lists in key order.
+__global__ __launch_bounds__(kKeyChunk) void k_ip_plan_apply(ResArgs a, int32_t* plan,
+                                                             const int32_t* tmp,
+                                                             const long long* chunk,
+                                                             const Clock* clk, int32_t* klist,
+                                                             int kstride) {
+  const int k = blockIdx.x * kKeyChunk + threadIdx.x;
+  if (k >= a.n_keys) return;
+  const long long* c = chunk + 8 * blockIdx.x;
+  plan[4 * k + 2] = (int32_t)(clk->n + c[5] + plan[4 * k + 2]);
+  const int kind = plan[4 * k + 0];
+  if (kind == kSplit) klist[2 + c[6] + tmp[k]] = k;
+  else if (kind == kMerge) klist[2 + kstride + c[7] + tmp[k]] = k;
 }
 
 template <typename T>
@@ -1563,11 +1646,7 @@ static int sort_members(const LaunchEnv& env, const SortWs& ws, int n_keys) {
     hipMemcpyAsync(ws.mslot, ws.mslot2, 4 * (size_t)env.capacity, hipMemcpyDeviceToDevice,
                    env.stream);
   }
-  const int mk = n_keys + 1;
-  key_bounds(env, ws.ip_mclk, ws.mkeys, mk, ws.ip_moff);
-  hipLaunchKernelGGL(k_key_counts, dim3((mk + 255) / 256), dim3(256), 0, env.stream,
-                     (const int32_t*)ws.ip_moff, mk, ws.ip_mcnt, ws.ip_mclk);
-  return check_launch("member offsets");
+  return 0;
 }
 
 template <typename T>
@@ -1600,16 +1679,32 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
                           lds_keys * 4) != hipSuccess)
     return fail("hipFuncSetAttribute(key histogram LDS) failed");
   hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, ws.ip_counts, (int64_t)m);
+  // the fire word the candidate kernel raises
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, env.stream,
+                     (int32_t*)&ws.ip_chunk[8 * kMaxKeyChunks], (int64_t)2);
   hipLaunchKernelGGL(k_ip_hist, dim3(env.num_cu), dim3(kIpHistBlock), (size_t)lds_keys * 4,
                      env.stream, keys, (const Clock*)env.clock, ws.ip_counts, lds_keys);
-  hipLaunchKernelGGL(k_ip_plan1, dim3(1), dim3(1024), 0, env.stream, a,
-                     (const int32_t*)ws.ip_counts, targets, nb, ws.ip_cand);
+  // candidates, trigger and member offsets (one key per thread, 1024-key chunks)
+  const int nch = (n_keys + kKeyChunk - 1) / kKeyChunk;
+  if (nch > kMaxKeyChunks) return fail("resample: at most %d keys", kMaxKeyChunks * kKeyChunk);
+  hipLaunchKernelGGL(k_ip_cand, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
+                     (const int32_t*)ws.ip_counts, targets, nb, ws.ip_cand, ws.ip_moff,
+                     ws.ip_chunk, ws.ip_bits);
+  hipLaunchKernelGGL(k_ip_cand_carry, dim3(1), dim3(64), 0, env.stream, a, nch, ws.ip_cand,
+                     ws.ip_chunk, ws.ip_mclk);
+  hipLaunchKernelGGL(k_ip_cand_apply, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
+                     (const int32_t*)ws.ip_counts, (const int32_t*)ws.ip_cand,
+                     (const long long*)ws.ip_chunk, ws.ip_moff, ws.ip_mcnt);
   // members of the candidate segments, in slot order, then grouped by key
-  hipLaunchKernelGGL(k_ip_mcount, dim3(kIpBlocks), dim3(kCompBlock), 0, env.stream, keys,
-                     (const Clock*)env.clock, (const int32_t*)ws.ip_cand, n_keys, ws.ip_bcnt);
-  hipLaunchKernelGGL(k_ip_mscatter, dim3(kIpBlocks), dim3(kCompBlock), 0, env.stream, keys,
+  const size_t bits_lds = 4 * ((size_t)n_keys / 32 + 1);
+  hipLaunchKernelGGL(k_ip_wcount, dim3(kIpBlocks), dim3(256), bits_lds, env.stream, keys,
                      (const Clock*)env.clock, (const int32_t*)ws.ip_cand, n_keys,
-                     (const uint32_t*)ws.ip_bcnt, ws.mkeys, ws.mslot, ws.ip_mclk);
+                     (const uint32_t*)ws.ip_bits, ws.ip_bcnt);
+  hipLaunchKernelGGL(k_ip_wscan, dim3(1), dim3(1024), 0, env.stream, ws.ip_bcnt);
+  hipLaunchKernelGGL(k_ip_wscatter, dim3(kIpBlocks), dim3(256), bits_lds, env.stream, keys,
+                     (const Clock*)env.clock, (const int32_t*)ws.ip_cand, n_keys,
+                     (const uint32_t*)ws.ip_bits, (const uint32_t*)ws.ip_bcnt, ws.mkeys,
+                     ws.mslot);
   int rc = check_launch("member extraction");
   if (rc) return rc;
   rc = sort_members(env, ws, n_keys);
@@ -1618,9 +1713,14 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
   hipLaunchKernelGGL(k_ip_hw<T>, dim3(sgrid), dim3(256), 0, env.stream, a,
                      (const int32_t*)ws.ip_cand, (const int32_t*)ws.ip_mcnt,
                      (const int32_t*)ws.ip_moff, (const uint32_t*)ws.mslot, ws.plan);
-  hipLaunchKernelGGL(k_ip_plan2, dim3(1), dim3(1024), 0, env.stream, a,
+  hipLaunchKernelGGL(k_ip_plan_local, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
                      (const int32_t*)ws.ip_counts, targets, (const int32_t*)ws.ip_cand, ws.plan,
-                     env.clock, ws.klist, ws.kstride);
+                     ws.ip_tmp, ws.ip_chunk, ws.kstride);
+  hipLaunchKernelGGL(k_ip_plan_carry, dim3(1), dim3(64), 0, env.stream, a, nch,
+                     (const int32_t*)ws.ip_cand, ws.ip_chunk, env.clock, ws.klist);
+  hipLaunchKernelGGL(k_ip_plan_apply, dim3(nch), dim3(kKeyChunk), 0, env.stream, a, ws.plan,
+                     (const int32_t*)ws.ip_tmp, (const long long*)ws.ip_chunk,
+                     (const Clock*)env.clock, ws.klist, ws.kstride);
   hipLaunchKernelGGL((k_res_split<T, true>), dim3(sgrid), dim3(kSegBlock), (size_t)kSplitLds * 16,
                      env.stream, a, (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
                      (const uint32_t*)ws.mslot, (const int32_t*)ws.plan, ws,
