@@ -171,3 +171,69 @@ int launch_reduction(const LaunchEnv& env, int r, int M, int RM, double* dw, con
 }
 
 }  // namespace bz
+
+// ---------------------------------------------------------------------------
+// The reduction for callers that cannot allocate (code under graph capture,
+// serving loops, harnesses that forbid hipMalloc): no context, and its
+// scratch (the column bounds and the blocks' integer rows) in a file-scope
+// device array of kRedStaticDoubles doubles, 8 MiB that every GPU the library
+// runs on holds until the process ends.  Its address on each device is taken
+// once, with that device current, on the first call there.  Every call on a
+// GPU shares that GPU's array, so calls that overlap on one GPU race; the
+// column bounds are cleared by a kernel at each call.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+namespace bz {
+
+constexpr int64_t kRedStaticDoubles = (int64_t)1 << 20;
+static __device__ __align__(256) double scratch[kRedStaticDoubles];
+
+constexpr int kStaticMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kStaticMaxDevices];
+static int g_scratch_cu[kStaticMaxDevices];
+
+}  // namespace bz
+
+extern "C" int bz_reduction_static(int32_t r, int32_t M, int32_t RM, double* dw,
+                                   const double* dv, const double* dx, int64_t ptcl,
+                                   void* hip_stream) {
+  using namespace bz;
+  if (r < 1 || r > 64 || M < 1 || RM < 1 || RM > M)
+    return fail("bz_reduction_static: need 1 <= r <= 64, M >= 1, 1 <= RM <= M");
+  if (ptcl < 0) return fail("bz_reduction_static: negative particle count");
+  if (!dw || (ptcl > 0 && (!dv || !dx))) return fail("bz_reduction_static: null pointer");
+  const int64_t need = 64 + 2 * (int64_t)RM * r;
+  if (need > kRedStaticDoubles)
+    return fail("bz_reduction_static: %d rows x %d columns need %lld doubles of scratch, "
+                "more than the %lld its static array holds",
+                RM, r, (long long)need, (long long)kRedStaticDoubles);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail("bz_reduction_static: no current device");
+  if (dev < 0 || dev >= kStaticMaxDevices)
+    return fail("bz_reduction_static: device %d beyond %d", dev, kStaticMaxDevices);
+  double* ptr = nullptr;
+  int ncu = 0;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      // the array's address on this device (the host-side name is only a
+      // stand-in, never passed to a kernel)
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail("bz_reduction_static: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      int n = 0;
+      const hipError_t e2 = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e2 != hipSuccess || n < 1)
+        return fail("bz_reduction_static: compute unit count: %s", hipGetErrorString(e2));
+      g_scratch_ptr[dev] = (double*)p;
+      g_scratch_cu[dev] = n;
+    }
+    ptr = g_scratch_ptr[dev];
+    ncu = g_scratch_cu[dev];
+  }
+  return launch_reduction_scratch((hipStream_t)hip_stream, ncu, r, M, RM, dw, dv, dx, ptcl, ptr,
+                                  kRedStaticDoubles);
+}

@@ -314,6 +314,16 @@ int bz_field_step(bz_ctx_t ctx, const bz_field_params_t* fp, const double* momen
  * (kernels/reduction/cuda_reduction/__init__.py:11-15). */
 int bz_reduction(bz_ctx_t ctx, int32_t r, int32_t M, int32_t RM, double* dw, const double* dv,
                  const double* dx, int64_t ptcl);
+/* The same reduction with no context and no allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch is a static 8 MiB device array per GPU, so RM*r may be at most
+ * (2^20 - 64)/2 = 524256 (sizes above are rejected with an error); calls that
+ * overlap on one GPU share that array and race.  Work is enqueued on
+ * hip_stream (NULL: the default stream) of the current device; the result
+ * equals bz_reduction's bit for bit. */
+int bz_reduction_static(int32_t r, int32_t M, int32_t RM, double* dw, const double* dv,
+                        const double* dx, int64_t ptcl, void* hip_stream);
 
 /* Energy / cell binning and stable radix sort by key = cell*nb + ebin,
  * cell = int((x + dx/2)/dx) (1bin/boltzmann.py:658), n_keys = n_nodes*nb <= 65534.
