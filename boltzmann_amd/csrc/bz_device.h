@@ -51,7 +51,8 @@ enum : uint32_t {
   kPurposeGas = 1,
   kPurposeResample = 2,
   kPurposeInit = 3,
-  kPurposeFlight = 4
+  kPurposeFlight = 4,
+  kPurposeDecide = 5
 };
 
 // counter = {index, block, step_lo, (step_hi << 8) | purpose}, key = seed.
@@ -69,43 +70,79 @@ __host__ __device__ __forceinline__ float u24(uint32_t a) {
   return (float)(a >> 8) * (1.0f / 16777216.0f);
 }
 
-// Uniform k (0..4 -> reference R columns 0,1,2,3,5) of the collision stream.
+// The collision stream of particle i at a step, 5 uniforms (reference R
+// columns 0,1,2,3,5).  u[0], the decision uniform compared with the
+// collision probabilities, comes from a draw shared by the particles 1024
+// slots apart: in FP32 the 4 particles i = 4096a + 1024b + c take word b of
+// the draw of group 1024a + c (purpose kPurposeDecide), in FP64 the 2
+// particles i = 2048a + 1024b + c take words 2b, 2b+1 of group 1024a + c.
+// u[1..4], the scattering angles (the secondary's are u[3], u[4]), come from
+// the particle's own draw, made only when it collides: FP32 one block, FP64
+// a second block for the secondary's angles.
+__host__ __device__ __forceinline__ uint32_t decision_group(uint32_t i, int per) {
+  return per == 4 ? (((i >> 12) << 10) | (i & 1023u)) : (((i >> 11) << 10) | (i & 1023u));
+}
+
+template <typename T>
+struct Decision;
+
+// Keeps the last group's draw: a thread whose slots are 1024 apart draws
+// once for 4 (FP32) or 2 (FP64) of them.
+template <>
+struct Decision<float> {
+  uint32_t g = 0xFFFFFFFFu;
+  U4 r;
+  __device__ __forceinline__ float get(uint64_t seed, uint32_t i, uint64_t step) {
+    const uint32_t gi = decision_group(i, 4);
+    if (gi != g) {
+      r = draw(seed, kPurposeDecide, gi, 0, step);
+      g = gi;
+    }
+    const uint32_t b = (i >> 10) & 3u;
+    return u24(b == 0 ? r.a : (b == 1 ? r.b : (b == 2 ? r.c : r.d)));
+  }
+};
+template <>
+struct Decision<double> {
+  uint32_t g = 0xFFFFFFFFu;
+  U4 r;
+  __device__ __forceinline__ double get(uint64_t seed, uint32_t i, uint64_t step) {
+    const uint32_t gi = decision_group(i, 2);
+    if (gi != g) {
+      r = draw(seed, kPurposeDecide, gi, 0, step);
+      g = gi;
+    }
+    return ((i >> 10) & 1u) ? u53(r.c, r.d) : u53(r.a, r.b);
+  }
+};
+
 template <typename T>
 struct Uniforms;
 
 template <>
 struct Uniforms<double> {
   double u[5];
-  __device__ __forceinline__ void first(uint64_t seed, uint32_t i, uint64_t step) {
+  __device__ __forceinline__ void scatter(uint64_t seed, uint32_t i, uint64_t step, bool ion) {
     const U4 r = draw(seed, kPurposeCollide, i, 0, step);
-    u[0] = u53(r.a, r.b);
-    u[1] = u53(r.c, r.d);
-  }
-  __device__ __forceinline__ void second(uint64_t seed, uint32_t i, uint64_t step) {
-    const U4 r = draw(seed, kPurposeCollide, i, 1, step);
-    u[2] = u53(r.a, r.b);
-    u[3] = u53(r.c, r.d);
-  }
-  __device__ __forceinline__ void third(uint64_t seed, uint32_t i, uint64_t step) {
-    const U4 r = draw(seed, kPurposeCollide, i, 2, step);
-    u[4] = u53(r.a, r.b);
+    u[1] = u53(r.a, r.b);
+    u[2] = u53(r.c, r.d);
+    if (ion) {
+      const U4 q = draw(seed, kPurposeCollide, i, 1, step);
+      u[3] = u53(q.a, q.b);
+      u[4] = u53(q.c, q.d);
+    }
   }
 };
 
 template <>
 struct Uniforms<float> {
   float u[5];
-  __device__ __forceinline__ void first(uint64_t seed, uint32_t i, uint64_t step) {
+  __device__ __forceinline__ void scatter(uint64_t seed, uint32_t i, uint64_t step, bool) {
     const U4 r = draw(seed, kPurposeCollide, i, 0, step);
-    u[0] = u24(r.a);
-    u[1] = u24(r.b);
-    u[2] = u24(r.c);
-    u[3] = u24(r.d);
-  }
-  __device__ __forceinline__ void second(uint64_t, uint32_t, uint64_t) {}
-  __device__ __forceinline__ void third(uint64_t seed, uint32_t i, uint64_t step) {
-    const U4 r = draw(seed, kPurposeCollide, i, 1, step);
-    u[4] = u24(r.a);
+    u[1] = u24(r.a);
+    u[2] = u24(r.b);
+    u[3] = u24(r.c);
+    u[4] = u24(r.d);
   }
 };
 
@@ -226,17 +263,52 @@ struct BinKeys {
   int ew;      // 1: bin the energy after (e*w)/w (BZ_REF_ENERGY_ROUNDTRIP)
 };
 
+// int(x / dx) for x >= 0, equal to the FP64 quotient's truncation, without
+// the divide: the product with the reciprocal is within a few ulps of the
+// quotient, so away from a cell edge (more than 1e-12 relative) it truncates
+// to the same integer; near an edge, and for x < 0, the quotient itself.
+__device__ __forceinline__ int cell_of(double x, double dx, double inv_dx) {
+  const double q = x * inv_dx;
+  const double t = trunc(q);
+  const double margin = 1e-12 * q;
+  if (q - t > margin && t + 1.0 - q > margin) return (int)t;
+  return (int)(x / dx);
+}
+
+// The energy thresholds of a binning in the compute type: from the kernel
+// arguments, or a copy staged in LDS (BinThrLds) by a kernel that bins
+// every particle it touches.
 template <typename T>
-__device__ __forceinline__ uint32_t bin_key(const BinKeys& k, T x, T e, T w) {
+struct BinThrArgs {
+  const BinKeys& k;
+  __device__ __forceinline__ T operator[](int q) const { return (T)k.thr[q]; }
+};
+template <typename T>
+struct BinThrLds {
+  __attribute__((address_space(3))) const T* t;
+  __device__ __forceinline__ T operator[](int q) const { return t[q]; }
+};
+
+template <typename T, typename TH>
+__device__ __forceinline__ uint32_t bin_key(const BinKeys& k, const TH& thr, T x, T e, T w) {
   if (k.ew) e = (e * w) / w;
   int cell = 0;
   if (k.dim == 1) {
-    cell = (int)(((double)x + 0.5 * k.dx) / k.dx);
+    cell = cell_of((double)x + 0.5 * k.dx, k.dx, 1.0 / k.dx);
     cell = cell < 0 ? 0 : (cell >= k.n_nodes ? k.n_nodes - 1 : cell);
   }
+  // thresholds past the binning's nb - 1 are +inf (an infinite energy,
+  // which passes them, is kept in the last bin)
   int b = 0;
-  for (int q = 0; q < k.nb - 1; ++q) b += (e >= (T)k.thr[q]) ? 1 : 0;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) b += (e >= thr[q]) ? 1 : 0;
+  b = b < k.nb - 1 ? b : k.nb - 1;
   return (uint32_t)(cell * k.nb + b);
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t bin_key(const BinKeys& k, T x, T e, T w) {
+  return bin_key<T>(k, BinThrArgs<T>{k}, x, e, w);
 }
 
 // Stable compaction blocks: block b of the grid owns a contiguous run of
@@ -294,14 +366,22 @@ __device__ __forceinline__ double dep_scale_of(unsigned long long wmax_bits) {
   return ldexp(1.0, 48 - e);
 }
 
-// v * scale as a 64-bit word and the carries above it
+// v * scale truncated, as a 64-bit word and the carries above it.  Below
+// 2^64 the two 32-bit halves are converted separately: h = floor(t / 2^32)
+// and t - h * 2^32 are exact, and the low half truncates to the same word
+// as t itself would.
 __device__ __forceinline__ unsigned long long dep_fixed_split(double v, double scale,
                                                               uint32_t* qh) {
   const double t = v * scale;  // v >= 0; scale is a power of two
   if (t < 18446744073709551616.0) {
     *qh = 0;
-    return (unsigned long long)t;
+    const double h = floor(t * 2.3283064365386962890625e-10);
+    const double l = fma(-h, 4294967296.0, t);
+    return ((unsigned long long)(uint32_t)h << 32) | (unsigned long long)(uint32_t)l;
   }
+  // rare: an empty asm keeps the compiler from evaluating this branch for
+  // every contribution and selecting
+  asm volatile("" ::: "memory");
   const double h = floor(t * 5.421010862427522170e-20);  // beyond 64 bits
   *qh = (uint32_t)h;
   return (unsigned long long)(t - h * 18446744073709551616.0);

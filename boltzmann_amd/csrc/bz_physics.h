@@ -43,14 +43,87 @@ __device__ __forceinline__ double dlog10(double v) { return log10(v); }
 __device__ __forceinline__ float dlog10(float v) { return log10f(v); }
 __device__ __forceinline__ double dpow10(double v) { return pow(10.0, v); }
 __device__ __forceinline__ float dpow10(float v) { return powf(10.0f, v); }
-__device__ __forceinline__ double dexp(double v) { return exp(v); }
-__device__ __forceinline__ float dexp(float v) { return expf(v); }
+// e^v - 1 and e^v in plain IEEE operations, written the same way in the
+// oracle (orc_expm1), so the collision probabilities of the direct and the
+// fit modes agree bit for bit between the device and the CPU check:
+// v = k ln2 + r with k = floor(v / ln2 + 1/2) and r = (v - k ln2_hi) -
+// k ln2_lo (Cody-Waite split; k ln2_hi is exact), e^r - 1 = r + r^2 P(r)
+// with P the Taylor polynomial in Horner form (e^r - 1 to the r^15 term in
+// FP64, r^8 in FP32; |r| <= 0.35), then e^v - 1 = 2^k (e^r - 1) + (2^k - 1).
+// v is clamped to [-80, 80] (e^-80 is a normal FP32 number, so no
+// denormal is formed; below it e^v - 1 rounds to -1 in either precision).
+template <typename T>
+struct ExpSplit;
+template <>
+struct ExpSplit<double> {
+  static constexpr double inv_ln2 = 1.4426950408889634;
+  static constexpr double ln2_hi = 6.93147180369123816490e-01;
+  static constexpr double ln2_lo = 1.90821492927058770002e-10;
+  __device__ __forceinline__ static double poly(double r) {
+    double q = 7.647163731819816e-13;
+    q = q * r + 1.1470745597729725e-11;
+    q = q * r + 1.6059043836821613e-10;
+    q = q * r + 2.08767569878681e-09;
+    q = q * r + 2.505210838544172e-08;
+    q = q * r + 2.755731922398589e-07;
+    q = q * r + 2.755731922398589e-06;
+    q = q * r + 2.48015873015873e-05;
+    q = q * r + 0.0001984126984126984;
+    q = q * r + 0.001388888888888889;
+    q = q * r + 0.008333333333333333;
+    q = q * r + 0.041666666666666664;
+    q = q * r + 0.16666666666666666;
+    q = q * r + 0.5;
+    return q;
+  }
+};
+template <>
+struct ExpSplit<float> {
+  static constexpr float inv_ln2 = 1.44269504f;
+  static constexpr float ln2_hi = 0.693359375f;
+  static constexpr float ln2_lo = -2.12194440e-4f;
+  __device__ __forceinline__ static float poly(float r) {
+    float q = 2.48015873e-05f;
+    q = q * r + 1.98412698e-04f;
+    q = q * r + 1.38888889e-03f;
+    q = q * r + 8.33333333e-03f;
+    q = q * r + 4.16666667e-02f;
+    q = q * r + 1.66666667e-01f;
+    q = q * r + 0.5f;
+    return q;
+  }
+};
+
+// e^r - 1 of the reduced argument and k
+template <typename T>
+__device__ __forceinline__ T expm1_reduced(T v, int* k) {
+  using C = ExpSplit<T>;
+  v = v < (T)-80 ? (T)-80 : (v > (T)80 ? (T)80 : v);
+  const T kf = floor(v * C::inv_ln2 + (T)0.5);
+  const T r = (v - kf * C::ln2_hi) - kf * C::ln2_lo;
+  *k = (int)kf;
+  return r + (r * r) * C::poly(r);
+}
+
+template <typename T>
+__device__ __forceinline__ T dexpm1(T v) {
+  int k;
+  const T e = expm1_reduced<T>(v, &k);
+  if (k == 0) return e;
+  const T s = ldexp((T)1, k);
+  return s * e + (s - (T)1);
+}
+
+template <typename T>
+__device__ __forceinline__ T dexp(T v) {
+  int k;
+  const T e = expm1_reduced<T>(v, &k);
+  return ldexp((T)1 + e, k);
+}
 __device__ __forceinline__ double dlog(double v) { return log(v); }
 __device__ __forceinline__ float dlog(float v) { return logf(v); }
 __device__ __forceinline__ double dsqrt(double v) { return sqrt(v); }
 __device__ __forceinline__ float dsqrt(float v) { return sqrtf(v); }
-__device__ __forceinline__ double dexpm1(double v) { return expm1(v); }
-__device__ __forceinline__ float dexpm1(float v) { return expm1f(v); }
 __device__ __forceinline__ double dlog1p(double v) { return log1p(v); }
 __device__ __forceinline__ float dlog1p(float v) { return log1pf(v); }
 // sin(pi v), cos(pi v): the argument reduction is exact, so an angle 2*pi*u
@@ -282,18 +355,29 @@ struct StepOut {
   int n_el, n_exc, n_null;  // events of the step (several in free-flight mode)
 };
 
+// The field of the cells for the 1-D kick: a plain pointer, or FieldRef,
+// which reads the copy staged in LDS through an LDS-typed pointer (a ds_read
+// that waits for nothing else in flight) and global memory otherwise.
+template <typename T>
+struct FieldRef {
+  const T* g;
+  __attribute__((address_space(3))) const T* s;
+  bool lds;
+  __device__ __forceinline__ T operator[](int c) const { return lds ? s[c] : g[c]; }
+};
+
 // Push and walls after the collision (electron_kernel_numba.py:257-287):
 // velocity (vx, vy, vz) after the collision, field of the particle's cell
 // (1-D) or of its swarm, the configured update order; fills o.p, o.alive
 // and o.e_post.
-template <typename T, int DIM>
+template <typename T, int DIM, typename EF>
 __device__ __forceinline__ void push_tail(const P5<T>& in, T vx, T vy, T vz,
-                                          const bz_params_t& p, const T* __restrict__ E,
+                                          const bz_params_t& p, const EF& E,
                                           T e_local, StepOut<T>& o) {
   T x = in.x;
   T ef;
   if (DIM == 1) {
-    int c = (int)((double)in.x / p.dx);
+    int c = cell_of((double)in.x, p.dx, 1.0 / p.dx);
     c = c < 0 ? 0 : (c >= p.n_cells ? p.n_cells - 1 : c);
     ef = E[c];
   } else {
@@ -320,11 +404,14 @@ __device__ __forceinline__ void push_tail(const P5<T>& in, T vx, T vy, T vz,
 
 // One particle through collision + push + walls
 // (electron_kernel_numba.py:39-287; dead particles are dropped, not pushed).
-template <typename T, int XS, int DIM, int RNG>
+// r0p: the particle's decision uniform when the caller has drawn it (the
+// Philox stream; null: drawn here).
+template <typename T, int XS, int DIM, int RNG, typename EF>
 __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, uint64_t step,
                                                     const bz_params_t& p, const bz_xsec_t& xs,
-                                                    const T* __restrict__ E, const T* R,
-                                                    const T* G, T e_local) {
+                                                    const EF& E, const T* R,
+                                                    const T* G, T e_local,
+                                                    const T* r0p = nullptr) {
   StepOut<T> o;
   o.alive = 0;
   o.born = 0;
@@ -367,7 +454,7 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
     U.u[3] = R[3 * st + i];
     U.u[4] = R[5 * st + i];
   } else {
-    U.first(p.seed, i, step);
+    U.u[0] = r0p ? *r0p : Decision<T>{}.get(p.seed, i, step);
   }
 
   T r0, r1, r2;
@@ -385,13 +472,8 @@ __device__ __forceinline__ StepOut<T> step_particle(const P5<T> in, uint32_t i, 
     o.chan = kChanIon;
   if (o.chan != kChanNone) {
     // one scatter for every channel (a wave with mixed channels runs it
-    // once); FP32 takes the angles u[1], u[2] from the first Philox block,
-    // FP64 from the second; an ionization also draws the secondary's angles
-    constexpr bool kAnglesInSecond = sizeof(T) == 8;
-    if (RNG != BZ_RNG_STREAM) {
-      if (kAnglesInSecond || o.chan == kChanIon) U.second(p.seed, i, step);
-      if (o.chan == kChanIon) U.third(p.seed, i, step);
-    }
+    // once); an ionization also takes the secondary's angles
+    if (RNG != BZ_RNG_STREAM) U.scatter(p.seed, i, step, o.chan == kChanIon);
     T vnew;
     if (o.chan == kChanElastic) {
       vnew = vmag * (T)p.fel;
@@ -473,10 +555,10 @@ constexpr int kMaxFlights = 256;
 // the same step is treated as a null event.  The secondary starts at the
 // collision point and is not moved in its birth step.  Tabulated sigma and
 // the Philox stream only.
-template <typename T, int DIM>
+template <typename T, int DIM, typename EF>
 __device__ __forceinline__ StepOut<T> flight_particle(const P5<T> in, uint32_t i, uint64_t step,
                                                       const bz_params_t& p, const bz_xsec_t& xs,
-                                                      const T* __restrict__ E, T e_local) {
+                                                      const EF& E, T e_local) {
   StepOut<T> o;
   o.alive = 0;
   o.born = 0;
@@ -492,7 +574,7 @@ __device__ __forceinline__ StepOut<T> flight_particle(const P5<T> in, uint32_t i
   }
   T ef;
   if (DIM == 1) {
-    int c = (int)((double)in.x / p.dx);
+    int c = cell_of((double)in.x, p.dx, 1.0 / p.dx);
     c = c < 0 ? 0 : (c >= p.n_cells ? p.n_cells - 1 : c);
     ef = E[c];
   } else {
@@ -601,9 +683,14 @@ __device__ __forceinline__ int dep_node(double x, const bz_params_t& p) {
   // of it for |b| < 2^24 and one step corrects it; for b < 0 either rounding
   // ends at node 0 after the clamp
   const int bnf = p.bnf;
-  int k = (int)((float)b * (1.0f / (float)bnf));
-  if (k * bnf > b) --k;
-  else if ((k + 1) * bnf <= b) ++k;
+  int k;
+  if ((bnf & (bnf - 1)) == 0) {
+    k = b >> __builtin_ctz((unsigned)bnf);  // a power of two: floor(b / bnf)
+  } else {
+    k = (int)((float)b * (1.0f / (float)bnf));
+    if (k * bnf > b) --k;
+    else if ((k + 1) * bnf <= b) ++k;
+  }
   return k < 0 ? 0 : (k >= p.n_nodes ? p.n_nodes - 1 : k);
 }
 

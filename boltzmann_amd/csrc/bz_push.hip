@@ -506,9 +506,9 @@ struct QueuedPush {
 };
 
 // LDS bytes after the staged table: per-bucket bounds [n_hint] and the
-// per-wave candidate queues [NW][kQCap]
+// per-wave candidate queues [NW][kQCap]: slots, then their decision uniforms
 __host__ __device__ inline size_t queue_lds_bytes(int n_hint, int word, int nw) {
-  return (((size_t)n_hint * word + 15) & ~(size_t)15) + (size_t)nw * kQCap * 4;
+  return (((size_t)n_hint * word + 15) & ~(size_t)15) + (size_t)nw * kQCap * (4 + word);
 }
 
 // The kImN per-instance moments of one swarm particle's step (all zero for a
@@ -580,6 +580,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   extern __shared__ double s_dep[];
   __shared__ unsigned long long s_cnt[kCntRow];
   __shared__ uint32_t s_births;
+  __shared__ T s_thr[8];  // the binning's energy thresholds (keys written here)
 
   constexpr bool kQueue = QUEUE && QueuedPush<XS, DIM, RNG>::value;
   const bz_params_t& p = a.p;
@@ -591,6 +592,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   const int sums = DIM == 1 ? 3 * nn : n_inst * kImN;
   T* s_pup = nullptr;       // kQueue: collision-probability bound per hint bucket
   uint32_t* s_q = nullptr;  // kQueue: candidate slots, kQCap per wave
+  T* s_qr = nullptr;        // kQueue: their decision uniforms
   // The slot count first, then the first tile's particles: their round
   // trips overlap the staging of the sums, the table and the field below.
   const int64_t n = a.clk->n;
@@ -623,6 +625,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
     for (int j = tid; j < n_inst; j += BLOCK) s_ie[j] = ((const T*)a.inst_E)[j];
   if (tid < kCntRow) s_cnt[tid] = 0ull;
   if (tid == 0) s_births = 0;
+  if (tid < 8) s_thr[tid] = (T)a.bk.thr[tid];
+  const BinThrLds<T> thr{(__attribute__((address_space(3))) const T*)s_thr};
   bz_xsec_t xs = a.xs;
   if (XS != BZ_XS_FIT) {
     const int n = a.xs.n;
@@ -639,22 +643,22 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
     if (kQueue) {
       s_pup = (T*)((char*)s_tab + table_lds_bytes(n, a.xs.n_hint, (int)sizeof(T)));
       s_q = (uint32_t*)((char*)s_pup + (((size_t)a.xs.n_hint * sizeof(T) + 15) & ~(size_t)15));
+      s_qr = (T*)(s_q + NW * kQCap);
     }
   }
   // 1-D: the field of every cell in LDS (where it fits), so the kick reads
   // no global memory
-  const T* s_E = nullptr;
-  if (DIM == 1 && a.e_lds_off) {
-    T* e = (T*)((char*)s_dep + a.e_lds_off);
-    stage_lds<T, BLOCK>(e, (const T*)a.E, p.n_cells);
-    s_E = e;
-  }
+  FieldRef<T> E;
+  E.g = (const T*)a.E;
+  E.lds = DIM == 1 && a.e_lds_off != 0;
+  E.s = (__attribute__((address_space(3))) const T*)((char*)s_dep +
+                                                     (E.lds ? a.e_lds_off : 0));
+  if (E.lds) stage_lds<T, BLOCK>((T*)((char*)s_dep + a.e_lds_off), (const T*)a.E, p.n_cells);
 
   // the births stage reads the count from n_prev: the slot count itself is
   // committed while it runs
   if (blockIdx.x == 0 && tid == 0) a.clk->n_prev = n;
   T* st[5] = {(T*)a.in[0], (T*)a.in[1], (T*)a.in[2], (T*)a.in[3], (T*)a.in[4]};
-  const T* E = s_E ? s_E : (const T*)a.E;
   const T* R = (const T*)a.R;
   const T* G = (const T*)a.G;
   uint32_t* wave_births = a.block_births + kMaxPushBlocks;
@@ -693,6 +697,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   if constexpr (kQueue) {
     if (queued) {
       uint32_t* q = s_q + wid * kQCap;
+      T* qr = s_qr + wid * kQCap;
+      Decision<T> dec[ITEMS];  // the decision draw of the group each item is in
       const uint64_t lt = (1ull << lane) - 1ull;
       const T e_bot = ((const T*)xs.energy)[0], e_top = ((const T*)xs.energy)[xs.n - 1];
       int qn = 0;                 // queued slots (wave-uniform)
@@ -709,6 +715,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
       auto run_batch = [&](int nb) {
         const bool act = lane < nb;
         const uint32_t i = act ? q[lane] : 0u;
+        const T r0 = act ? qr[lane] : (T)1;
         P5<T> c;
         load5<T, true>(in, (int64_t)i, act ? n : 0, c);
         int inst = 0;
@@ -718,7 +725,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           inst = inst < 0 ? 0 : (inst >= n_inst ? n_inst - 1 : inst);
           e_local = s_ie[inst];
         }
-        const StepOut<T> o = step_particle<T, XS, PD, RNG>(c, i, step, p, xs, E, R, G, e_local);
+        const StepOut<T> o =
+            step_particle<T, XS, PD, RNG>(c, i, step, p, xs, E, R, G, e_local, &r0);
         if (act) {
           ++c_live;
           c_el += o.n_el;
@@ -753,7 +761,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
               dep_fixed_add_global(a.m1_lo, a.m1_hi, 0, 1.0, dscale);  // the reference's flag
           }
           if (DIM == 1 && a.keys)
-            a.keys[i] = o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post, o.p.w) : (uint32_t)a.bk.n_keys;
+            a.keys[i] = o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w) : (uint32_t)a.bk.n_keys;
         }
         // births: the batch's slots rise, so equal tiles are runs of lanes
         // (segments), taken in order
@@ -794,6 +802,9 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
         for (int k = 0; k < ITEMS; ++k) {
           const int64_t i = bw + k * kWave + lane;
           const bool live = i < n && cur[k].w > (T)0;
+          // every lane draws (the same group across the wave): a Philox
+          // block once per 4 tiles of 1024 slots (2 in FP64)
+          const T r0 = dec[k].get(p.seed, (uint32_t)i, step);
           bool cand = false;
           if (live) {
             const T s2 = cur[k].vx * cur[k].vx + cur[k].vy * cur[k].vy + cur[k].vz * cur[k].vz;
@@ -804,9 +815,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
               const T ec = e < e_bot ? e_bot : e;
               int b = xs_bucket<T>(ec, xs.hint_bits) - xs.hint_base;
               b = b < 0 ? 0 : (b >= xs.n_hint ? xs.n_hint - 1 : b);
-              Uniforms<T> U;
-              U.first(p.seed, (uint32_t)i, step);
-              cand = U.u[0] < s_pup[b];
+              cand = r0 < s_pup[b];
             }
           }
           const bool plain = live && !cand;
@@ -842,7 +851,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           }
           // keys of the slots settled here (candidates get theirs in their batch)
           if (DIM == 1 && a.keys && i < n && !(live && cand))
-            a.keys[i] = plain && o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post, o.p.w)
+            a.keys[i] = plain && o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w)
                                          : (uint32_t)a.bk.n_keys;
           if (DIM == 2) {
             // running moments of the wave's instance (the queued candidates
@@ -852,11 +861,18 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
             swarm_accumulate(mv, plain, inst, m_inst, macc, s_dep, lane);
           }
           const uint64_t cm = __ballot(live && cand);
-          if (live && cand) q[qn + (int)__popcll(cm & lt)] = (uint32_t)i;
+          if (live && cand) {
+            const int at = qn + (int)__popcll(cm & lt);
+            q[at] = (uint32_t)i;
+            qr[at] = r0;
+          }
           qn += (int)__popcll(cm);
           if (qn >= 64) {
             run_batch(64);
-            if (lane < qn - 64) q[lane] = q[64 + lane];
+            if (lane < qn - 64) {
+              q[lane] = q[64 + lane];
+              qr[lane] = qr[64 + lane];
+            }
             qn -= 64;
           }
         }
@@ -867,6 +883,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
       if (t_first < t_end) advance_to(t_end);  // the last tile's count, zeros after it
     }
   }
+  Decision<T> dec_nq[ITEMS];  // the decision draw of the group each item is in
   if (!queued)
   for (uint32_t t = t_first; t < t_end; ++t) {
     const int64_t bw = (int64_t)t * TILE + wid * WSPAN;
@@ -889,10 +906,15 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
       }
       constexpr int PD = DIM == 1 ? 1 : 0;  // physics dimension
       StepOut<T> o;
-      if (XS == BZ_XS_FLIGHT)
+      if (XS == BZ_XS_FLIGHT) {
         o = flight_particle<T, PD>(cur[k], (uint32_t)i, step, p, xs, E, e_local);
-      else
+      } else if (RNG == BZ_RNG_STREAM) {
         o = step_particle<T, XS, PD, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, e_local);
+      } else {
+        const T r0 = dec_nq[k].get(p.seed, (uint32_t)i, step);
+        o = step_particle<T, XS, PD, RNG>(cur[k], (uint32_t)i, step, p, xs, E, R, G, e_local,
+                                          &r0);
+      }
       if (live) {
         ++c_live;
         c_el += o.n_el;
@@ -922,7 +944,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
         }
       }
       if (DIM == 1 && a.keys && i < n)
-        a.keys[i] = live && o.alive ? bin_key<T>(a.bk, o.p.x, o.e_post, o.p.w)
+        a.keys[i] = live && o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w)
                                    : (uint32_t)a.bk.n_keys;
       if (DIM == 2) {
         // per-instance moments of this step; a wave whose lanes share one
@@ -1636,9 +1658,8 @@ __global__ void k_fill_uniforms(T* R, int64_t stride, int64_t n, uint64_t seed, 
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     Uniforms<T> U;
-    U.first(seed, (uint32_t)i, step);
-    U.second(seed, (uint32_t)i, step);
-    U.third(seed, (uint32_t)i, step);
+    U.u[0] = Decision<T>{}.get(seed, (uint32_t)i, step);
+    U.scatter(seed, (uint32_t)i, step, true);
     R[i] = U.u[0];
     R[stride + i] = U.u[1];
     R[2 * stride + i] = U.u[2];

@@ -562,7 +562,8 @@ BinKeys bin_keys(const bz_params_t* p, const double* thr, int nb) {
   BinKeys bk;
   const int n_nodes = p->dim == 1 ? p->n_nodes : 1;
   bk.dx = p->dx;
-  for (int k = 0; k < 8; ++k) bk.thr[k] = (thr && k < nb - 1) ? thr[k] : 0.0;
+  // unused thresholds are +inf: no energy reaches them
+  for (int k = 0; k < 8; ++k) bk.thr[k] = (thr && k < nb - 1) ? thr[k] : HUGE_VAL;
   bk.nb = nb;
   bk.n_nodes = n_nodes;
   bk.n_keys = n_nodes * nb;

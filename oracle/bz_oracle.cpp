@@ -70,31 +70,41 @@ static float orc_u24(uint32_t a) { return (float)(a >> 8) / 16777216.0f; }
 
 // The collision uniforms of step `step` for particles 0..n-1, laid out as the
 // reference's R matrix transposed: R[c*stride + i], columns 0,1,2,3,5 used.
+// Column 0 is the decision uniform (compared with the collision
+// probabilities): one Philox draw of purpose 5 serves the particles 1024
+// slots apart of a group, FP32: i = 4096a + 1024b + c takes word b of the
+// draw of group 1024a + c; FP64: i = 2048a + 1024b + c takes words 2b, 2b+1
+// of group 1024a + c.  Columns 1,2,3,5 are the scattering angles from the
+// particle's own draws (purpose 0): FP32 the four words of block 0, FP64
+// block 0 for columns 1, 2 and block 1 for columns 3, 5.  The device draws
+// the same numbers (the stream is this build's; the reference draws cuRAND).
 static void orc_uniforms_at(uint64_t seed, uint64_t step, int64_t n, int prec, void* R,
                             int64_t stride, uint32_t first) {
   for (int64_t j = 0; j < n; ++j) {
     const uint32_t i = first + (uint32_t)j;
-    uint32_t b0[4], b1[4], b2[4];
+    uint32_t d[4], b0[4], b1[4];
     orc_draw(seed, 0, i, 0, step, b0);
-    orc_draw(seed, 0, i, 1, step, b1);
     if (prec == 8) {
-      orc_draw(seed, 0, i, 2, step, b2);
+      orc_draw(seed, 5, ((i >> 11) << 10) | (i & 1023u), 0, step, d);
+      orc_draw(seed, 0, i, 1, step, b1);
+      const int h = (int)((i >> 10) & 1u);
       double* r = (double*)R;
-      r[0 * stride + j] = orc_u53(b0[0], b0[1]);
-      r[1 * stride + j] = orc_u53(b0[2], b0[3]);
-      r[2 * stride + j] = orc_u53(b1[0], b1[1]);
-      r[3 * stride + j] = orc_u53(b1[2], b1[3]);
+      r[0 * stride + j] = orc_u53(d[2 * h], d[2 * h + 1]);
+      r[1 * stride + j] = orc_u53(b0[0], b0[1]);
+      r[2 * stride + j] = orc_u53(b0[2], b0[3]);
+      r[3 * stride + j] = orc_u53(b1[0], b1[1]);
       r[4 * stride + j] = 0.0;
-      r[5 * stride + j] = orc_u53(b2[0], b2[1]);
+      r[5 * stride + j] = orc_u53(b1[2], b1[3]);
       r[6 * stride + j] = 0.0;
     } else {
+      orc_draw(seed, 5, ((i >> 12) << 10) | (i & 1023u), 0, step, d);
       float* r = (float*)R;
-      r[0 * stride + j] = orc_u24(b0[0]);
-      r[1 * stride + j] = orc_u24(b0[1]);
-      r[2 * stride + j] = orc_u24(b0[2]);
-      r[3 * stride + j] = orc_u24(b0[3]);
+      r[0 * stride + j] = orc_u24(d[(i >> 10) & 3u]);
+      r[1 * stride + j] = orc_u24(b0[0]);
+      r[2 * stride + j] = orc_u24(b0[1]);
+      r[3 * stride + j] = orc_u24(b0[2]);
       r[4 * stride + j] = 0.0f;
-      r[5 * stride + j] = orc_u24(b1[0]);
+      r[5 * stride + j] = orc_u24(b0[3]);
       r[6 * stride + j] = 0.0f;
     }
   }
@@ -174,10 +184,59 @@ inline double m_log10(double v) { return log10(v); }
 inline float m_log10(float v) { return log10f(v); }
 inline double m_p10(double v) { return pow(10.0, v); }
 inline float m_p10(float v) { return powf(10.0f, v); }
-inline double m_exp(double v) { return exp(v); }
-inline float m_exp(float v) { return expf(v); }
-inline double m_expm1(double v) { return expm1(v); }
-inline float m_expm1(float v) { return expm1f(v); }
+// e^v - 1 and e^v from plain IEEE operations (no libm), so that the
+// collision probabilities of the direct and fit modes are the same numbers
+// the device computes: v = k ln2 + r, k = floor(v / ln2 + 1/2), r = (v -
+// k ln2_hi) - k ln2_lo (Cody-Waite, k ln2_hi exact); e^r - 1 = r + r^2 P(r),
+// P the Taylor polynomial 1/2! + r/3! + ... in Horner form (to 1/15! in
+// FP64, 1/8! in FP32, |r| <= ln2/2); e^v - 1 = 2^k (e^r - 1) + (2^k - 1);
+// v clamped to [-80, 80].
+// Pinned against libm's expm1 / exp (within 2 ulp) in the oracle tests.
+static double orc_expm1_red(double v, int* k) {
+  v = v < -80.0 ? -80.0 : (v > 80.0 ? 80.0 : v);
+  const double kf = floor(v * 1.4426950408889634 + 0.5);
+  const double r = (v - kf * 6.93147180369123816490e-01) - kf * 1.90821492927058770002e-10;
+  static const double c[13] = {1.1470745597729725e-11, 1.6059043836821613e-10,
+                               2.08767569878681e-09,   2.505210838544172e-08,
+                               2.755731922398589e-07,  2.755731922398589e-06,
+                               2.48015873015873e-05,   0.0001984126984126984,
+                               0.001388888888888889,   0.008333333333333333,
+                               0.041666666666666664,   0.16666666666666666,
+                               0.5};
+  double q = 7.647163731819816e-13;
+  for (int j = 0; j < 13; ++j) q = q * r + c[j];
+  *k = (int)kf;
+  return r + (r * r) * q;
+}
+static float orc_expm1_red(float v, int* k) {
+  v = v < -80.0f ? -80.0f : (v > 80.0f ? 80.0f : v);
+  const float kf = floorf(v * 1.44269504f + 0.5f);
+  const float r = (v - kf * 0.693359375f) - kf * -2.12194440e-4f;
+  static const float c[6] = {1.98412698e-04f, 1.38888889e-03f, 8.33333333e-03f,
+                             4.16666667e-02f, 1.66666667e-01f, 0.5f};
+  float q = 2.48015873e-05f;
+  for (int j = 0; j < 6; ++j) q = q * r + c[j];
+  *k = (int)kf;
+  return r + (r * r) * q;
+}
+template <typename T>
+static T orc_expm1(T v) {
+  int k;
+  const T e = orc_expm1_red(v, &k);
+  if (k == 0) return e;
+  const T s = std::ldexp((T)1, k);
+  return s * e + (s - (T)1);
+}
+template <typename T>
+static T orc_exp(T v) {
+  int k;
+  const T e = orc_expm1_red(v, &k);
+  return std::ldexp((T)1 + e, k);
+}
+inline double m_exp(double v) { return orc_exp(v); }
+inline float m_exp(float v) { return orc_exp(v); }
+inline double m_expm1(double v) { return orc_expm1(v); }
+inline float m_expm1(float v) { return orc_expm1(v); }
 inline double m_sqrt(double v) { return sqrt(v); }
 inline float m_sqrt(float v) { return sqrtf(v); }
 // sin(pi v), cos(pi v) with an exact argument reduction: n = rint(2v) and
@@ -268,6 +327,20 @@ void fit_cross_sections(T e, const orc_params& P, T* s_el, T* s_ion) {
 }
 
 }  // namespace
+
+// the oracle's e^v - 1 and e^v (the collision probabilities' exponential)
+extern "C" void orc_expm1_f64(int64_t n, const double* v, double* em1, double* ex) {
+  for (int64_t i = 0; i < n; ++i) {
+    em1[i] = orc_expm1(v[i]);
+    ex[i] = orc_exp(v[i]);
+  }
+}
+extern "C" void orc_expm1_f32(int64_t n, const float* v, float* em1, float* ex) {
+  for (int64_t i = 0; i < n; ++i) {
+    em1[i] = orc_expm1(v[i]);
+    ex[i] = orc_exp(v[i]);
+  }
+}
 
 // sigma_el * nn and sigma_ion * nn of the reference fits at energies e[0..n)
 extern "C" void orc_fit_sigma_f64(const orc_params* P, int64_t n, const double* e, double* s_el,

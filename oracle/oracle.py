@@ -166,6 +166,17 @@ def fit_sigma(setup: "Setup", e: np.ndarray):
     return a, b
 
 
+def expm1_exp(v: np.ndarray):
+    """The oracle's (e^v - 1, e^v), the exponential of the collision
+    probabilities, in v's precision."""
+    lib = load()
+    v = np.ascontiguousarray(v)
+    a, b = np.zeros_like(v), np.zeros_like(v)
+    fn = lib.orc_expm1_f64 if v.dtype == np.float64 else lib.orc_expm1_f32
+    fn(C.c_int64(v.size), _ptr(v), _ptr(a), _ptr(b))
+    return a, b
+
+
 def resample_uniform(seed: int, step: int, key: int, m: int) -> float:
     return load().orc_resample_uniform(seed, step, key, m)
 

@@ -28,18 +28,31 @@ def _u53(a, b):
 
 
 def test_collision_uniform_layout():
-    seed, step, n = 0x1234_5678_9ABC, 7, 5
+    """Column 0 (the decision uniform) from the purpose-5 draw shared by the
+    particles 1024 slots apart (4 in FP32, 2 in FP64), columns 1,2,3,5 from
+    the particle's own purpose-0 draws; particles chosen across group and
+    word boundaries."""
+    seed, step = 0x1234_5678_9ABC, 7
+    idx = [0, 1, 1023, 1024, 2047, 2048, 3071, 3072, 4095, 4096, 5121, 9000]
+    n = max(idx) + 1
     R64 = O.collision_uniforms(seed, step, n, np.float64)
     R32 = O.collision_uniforms(seed, step, n, np.float32)
     key = (seed & 0xFFFFFFFF, seed >> 32)
-    for i in range(n):
-        b = [O.philox((i, blk, step, 0), key) for blk in range(3)]
-        want64 = [_u53(b[0][0], b[0][1]), _u53(b[0][2], b[0][3]), _u53(b[1][0], b[1][1]),
-                  _u53(b[1][2], b[1][3]), _u53(b[2][0], b[2][1])]
+    for i in idx:
+        b = [O.philox((i, blk, step, 0), key) for blk in range(2)]
+        g2, h = ((i >> 11) << 10) | (i & 1023), (i >> 10) & 1
+        d2 = O.philox((g2, 0, step, 5), key)
+        want64 = [_u53(d2[2 * h], d2[2 * h + 1]), _u53(b[0][0], b[0][1]), _u53(b[0][2], b[0][3]),
+                  _u53(b[1][0], b[1][1]), _u53(b[1][2], b[1][3])]
         assert list(R64[[0, 1, 2, 3, 5], i]) == want64
-        want32 = [(b[0][k] >> 8) / 2.0 ** 24 for k in range(4)] + [(b[1][0] >> 8) / 2.0 ** 24]
+        g4, q = ((i >> 12) << 10) | (i & 1023), (i >> 10) & 3
+        d4 = O.philox((g4, 0, step, 5), key)
+        want32 = [(d4[q] >> 8) / 2.0 ** 24] + [(b[0][k] >> 8) / 2.0 ** 24 for k in range(4)]
         assert list(R32[[0, 1, 2, 3, 5], i].astype(np.float64)) == want32
     assert np.all(R64 >= 0) and np.all(R64 < 1)
+    # the decision uniforms of a group's members are different words: no two
+    # of 4096 consecutive FP32 particles repeat one
+    assert np.unique(R32[0, :4096]).size > 4090
 
 
 def test_uniform_moments():
@@ -773,3 +786,22 @@ def test_energy_roundtrip_binning_moves_threshold_energies():
     assert flips > 10  # about 1 % of the weights
     assert c0[node * nb] == m and c0[node * nb + 1] == 0
     assert c2[node * nb] == m - flips and c2[node * nb + 1] == flips
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_exponential_within_two_ulp_of_libm(dtype):
+    """The oracle's own e^v - 1 and e^v (the device evaluates the same
+    operations, so the collision probabilities agree bit for bit) against
+    numpy's, over the probabilities' range v = -nu dt in [-60, 0] and a few
+    positive values: within 2 ulp, exact at 0."""
+    rng = np.random.default_rng(5)
+    v = np.concatenate([-np.logspace(-9, np.log10(60.0), 20000), rng.uniform(-1, 1, 5000),
+                        np.linspace(0, 20, 500), [0.0, -np.log(2.0), np.log(2.0) / 2]])
+    v = v.astype(dtype)
+    em1, ex = O.expm1_exp(v)
+    assert em1[v == 0].max() == 0.0 and ex[v == 0].min() == 1.0
+    v64 = v.astype(np.float64)
+    for got, want in ((em1, np.expm1(v64)), (ex, np.exp(v64))):
+        ulp = np.spacing(np.abs(want).astype(dtype)).astype(np.float64)
+        err = np.abs(got.astype(np.float64) - want) / ulp
+        assert err.max() <= 2.0, (dtype, err.max(), v[np.argmax(err)])

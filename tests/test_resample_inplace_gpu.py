@@ -22,10 +22,8 @@ pytestmark = pytest.mark.gpu
 KEV = 0.5 * O.M_E * O.J_EV
 
 
-def _state(counts_per_cell, N, L, dtype, seed=0, drift=0.0, e_ev=8.0, nb_energies=None):
-    """Particles per cell (at most one cell per key when nb_energies pins
-    every particle of a cell to one energy), weights uniform in [0.5, 2),
-    slots shuffled."""
+def _state(counts_per_cell, N, L, dtype, seed=0, drift=0.0, e_ev=8.0):
+    """Particles per cell, weights uniform in [0.5, 2), slots shuffled."""
     rng = np.random.default_rng(seed)
     dx = L / N
     xs, vs, ws = [], [], []
@@ -44,14 +42,15 @@ def _state(counts_per_cell, N, L, dtype, seed=0, drift=0.0, e_ev=8.0, nb_energie
     return tuple(np.ascontiguousarray(a[p], dtype=dtype) for a in (x, v[0], v[1], v[2], w))
 
 
-def _engine(N, prec, nb, target, n, capacity=None, **kw):
+def _engine(N, prec, nb, target, n, capacity=None, inplace_resample=True, **kw):
     from boltzmann_amd import physics as ph
     from boltzmann_amd.engine import KineticEngine
     cfg = ph.RunConfig(name="rsip", dim=1, n_particles=n, precision=prec, nb=nb, n_cells=N,
                        length_cm=2.5, dt=1e-11, target_ppc=target,
                        bin_fractions=ph.bin_fractions(nb), bin_thresholds=ph.bin_thresholds(nb),
                        **kw)
-    return cfg, KineticEngine(cfg, capacity=capacity or 4 * n + 4096)
+    return cfg, KineticEngine(cfg, capacity=capacity or 4 * n + 4096,
+                              inplace_resample=inplace_resample)
 
 
 def _policy(cfg, max_ppc=0):
@@ -130,12 +129,14 @@ def test_resample_inplace_parity(gpu, name, prec, nb, drift):
     eng.close()
 
 
-def test_resample_inplace_many_keys_and_large_segments(gpu):
+@pytest.mark.parametrize("inplace", [True, False], ids=["inplace", "key_order"])
+def test_resample_many_keys_and_large_segments(gpu, inplace):
     """More keys than the split / merge grids have blocks (300 cells x 5 bins
     = 1505 keys, each block walks several), a merge segment above the
     merge's LDS CDF (6144) clipped by max_ppc, and a split segment above the
-    split's LDS arrays (3968): the workspace paths, against the oracle, with
-    FP64 conservation to 1e-12."""
+    split's LDS arrays (3968): the workspace paths of both layouts (in place,
+    and the key-order output of bz_resample), against the oracle, with FP64
+    conservation to 1e-12."""
     from boltzmann_amd import physics as ph
     N, nb = 300, 5
     rng = np.random.default_rng(11)
@@ -143,7 +144,8 @@ def test_resample_inplace_many_keys_and_large_segments(gpu):
     cells[17] = 9000   # a large merge segment (all of cell 17 in one bin)
     cells[200] = 4400  # a large split segment
     n = int(sum(cells))
-    cfg, eng = _engine(N, 8, nb, 100, n, trigger_at=0.0, merge_at=2.0)
+    cfg, eng = _engine(N, 8, nb, 100, n, trigger_at=0.0, merge_at=2.0,
+                       inplace_resample=inplace)
     st = list(_state(cells, N, 2.5, np.float64, seed=12, e_ev=2.0))
     # pin cells 17 and 200 to one energy bin each: 2 eV, every bin-0 speed
     dx = 2.5 / N
@@ -164,17 +166,21 @@ def test_resample_inplace_many_keys_and_large_segments(gpu):
     eng.resample()
     got = eng.fetch_slots()
     s = O.Setup(dim=1, dt=cfg.dt, n_cells=N, length_cm=2.5)
-    want, wc = O.resample_inplace(s, _policy(cfg, max_ppc=7000), nb, ph.bin_thresholds(nb), tg,
-                                  cfg.seed, 9, st, capacity=eng.capacity)
+    oracle_resample = O.resample_inplace if inplace else O.resample
+    want, wc = oracle_resample(s, _policy(cfg, max_ppc=7000), nb, ph.bin_thresholds(nb), tg,
+                               cfg.seed, 9, st, capacity=eng.capacity)
     assert wc[8] > 0 and wc[9] > 0
+    c = eng.counters()
+    assert c["split"] == wc[8] and c["merge"] == wc[9]
     _compare_slots(got, want, 8)
     W0, P0, E0 = _moments(st)
     W1, P1, E1 = _moments(got)
     assert abs(W1 - W0) <= 1e-12 * W0
     assert abs(E1 - E0) <= 1e-12 * E0
     assert np.all(np.abs(P1 - P0) <= 1e-12 * math.sqrt(W0 * E0))
-    # the clipped merge leaves members 7000.. of cell 17's segment as they were
-    assert np.sum((got[4][:n] == 0) & (st[4] > 0)) >= 7000
+    if inplace:
+        # the clipped merge leaves members 7000.. of cell 17's segment as they were
+        assert np.sum((got[4][:n] == 0) & (st[4] > 0)) >= 7000
     eng.close()
 
 
