@@ -108,6 +108,7 @@ SIGNATURES = {
                               C.POINTER(XSec), _P, _P, _P]),
     "bz_deposit": (C.c_int, [_P, C.POINTER(State), C.POINTER(Params), _P]),
     "bz_compact": (C.c_int, [_P, C.POINTER(State), C.POINTER(State)]),
+    "bz_compact_inplace": (C.c_int, [_P, C.POINTER(State)]),
     "bz_reduction": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, C.c_int64]),
     "bz_reduction_static": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, C.c_int64,
                                       _P]),

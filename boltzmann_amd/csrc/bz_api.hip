@@ -443,6 +443,13 @@ int bz_compact(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out) {
   return launch_compact(c->env, in, out);
 }
 
+int bz_compact_inplace(bz_ctx_t c, const bz_state_t* s) {
+  BZ_CTX(c);
+  if (!s || !s->w) return fail("null state");
+  c->keys_state = nullptr;
+  return launch_fill_holes(c->env, s);
+}
+
 int bz_field_step(bz_ctx_t c, const bz_field_params_t* fp, const double* moments, double* ni,
                   double* ns, double* Te, double* g2, double* V, double* E, void* E_t,
                   const double* chol, int32_t advance_time) {

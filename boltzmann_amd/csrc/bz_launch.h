@@ -67,6 +67,8 @@ int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t*
 // the weight bound of a state into the clock (the deposit's fixed-point scale)
 int launch_dep_wmax(const LaunchEnv& env, const bz_state_t* s);
 int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out);
+// in place: the holes below the live count filled from the live slots above it
+int launch_fill_holes(const LaunchEnv& env, const bz_state_t* s);
 // sum the push's per-block partial rows (m doubles each) into out[m];
 // commit != 0 also sums the counter rows and commits the clock
 // sums the per-block partial rows into out (and adds them to acc when given)

@@ -1735,6 +1735,212 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
   return check_launch("resample in place");
 }
 
+// ---------------------------------------------------------------------------
+// In-place compaction by hole filling.  The reference compacts the live
+// particles after the push (post_processing_1D.py:26-38) with atomics, which
+// leave their order open; here L = the live count, the k-th hole below L (in
+// slot order) receives the k-th live particle at or above L (in slot order),
+// and the count becomes L.  Live particles below L keep their slots, so the
+// traffic is the weights read twice plus 40 B per moved particle, against a
+// full copy of the state for a stable filter.  Blocks own the contiguous
+// ranges of comp_range; block bb holds slot L.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kCompBlock) void k_fill_count(const T* __restrict__ w,
+                                                           const Clock* clk, uint32_t* cnt) {
+  __shared__ uint32_t s_red[kCompBlock / kWave];
+  int64_t lo, hi;
+  comp_range(clk->n, &lo, &hi);
+  uint32_t c = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kCompBlock) c += w[i] > (T)0 ? 1u : 0u;
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+// One block of 1024: L, the block bb holding slot L, the live slots of bb
+// below L, then the hole offsets of blocks 0..bb and the mover offsets of
+// blocks bb..grid-1.  off: [grid] hole offsets, [grid] mover offsets, H, bb,
+// the holes of block bb.  Sets n_next = L.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_fill_plan(const T* __restrict__ w, Clock* clk, int grid,
+                                                    const uint32_t* cnt, int64_t* off) {
+  __shared__ long long s_a[1024 / kWave], s_b[1024 / kWave];
+  __shared__ long long s_tot;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t n = clk->n;
+  const int64_t tiles = (n + kCompTile - 1) / kCompTile;
+  const int64_t per = (tiles + grid - 1) / grid;
+  const int64_t span = per * kCompTile;  // slots per block range
+  // L
+  long long c = 0;
+  for (int b = tid; b < grid; b += 1024) c += cnt[b];
+  c = wave_sum(c);
+  if (lane == 0) s_a[wid] = c;
+  __syncthreads();
+  if (tid == 0) {
+    long long t = 0;
+    for (int j = 0; j < 1024 / kWave; ++j) t += s_a[j];
+    s_tot = t;
+  }
+  __syncthreads();
+  const int64_t L = s_tot;
+  const int bb = (span > 0 && L < n) ? (int)(L / span) : grid;  // grid: no movers
+  // live slots of block bb below L
+  long long lb = 0;
+  if (bb < grid)
+    for (int64_t i = (int64_t)bb * span + tid; i < L; i += 1024) lb += w[i] > (T)0 ? 1 : 0;
+  lb = wave_sum(lb);
+  __syncthreads();
+  if (lane == 0) s_b[wid] = lb;
+  __syncthreads();
+  long long live_bb_below = 0;
+  for (int j = 0; j < 1024 / kWave; ++j) live_bb_below += s_b[j];
+  const long long holes_bb = bb < grid ? (L - (int64_t)bb * span) - live_bb_below : 0;
+  // offsets: one thread per 2 blocks, then a block scan (grid <= 2048)
+  long long h[2], m[2];
+  for (int q = 0; q < 2; ++q) {
+    const int b = 2 * tid + q;
+    const int64_t lo = b < grid ? min((int64_t)b * span, n) : n;
+    const int64_t hi = b < grid ? min(lo + span, n) : n;
+    h[q] = (b < bb) ? (hi - lo) - (long long)cnt[b] : (b == bb ? holes_bb : 0);
+    m[q] = (b > bb && b < grid) ? (long long)cnt[b]
+                                : (b == bb ? (long long)cnt[b] - live_bb_below : 0);
+  }
+  long long hs = h[0] + h[1], ms = m[0] + m[1];
+  const long long hi_incl = wave_incl_scan(hs), mi_incl = wave_incl_scan(ms);
+  __syncthreads();
+  if (lane == kWave - 1) {
+    s_a[wid] = hi_incl;
+    s_b[wid] = mi_incl;
+  }
+  __syncthreads();
+  long long hb = 0, mb = 0, htot = 0;
+  for (int j = 0; j < 1024 / kWave; ++j) {
+    if (j < wid) {
+      hb += s_a[j];
+      mb += s_b[j];
+    }
+    htot += s_a[j];
+  }
+  const long long h0 = hb + hi_incl - hs, m0 = mb + mi_incl - ms;
+  for (int q = 0; q < 2; ++q) {
+    const int b = 2 * tid + q;
+    if (b < grid) {
+      off[b] = q == 0 ? h0 : h0 + h[0];
+      off[grid + b] = q == 0 ? m0 : m0 + m[0];
+    }
+  }
+  if (tid == 0) {
+    off[2 * grid] = htot;
+    off[2 * grid + 1] = bb;
+    off[2 * grid + 2] = holes_bb;
+    clk->n_next = L;
+  }
+}
+
+// The holes below L and the movers at or above L of each block, ranked in
+// slot order: hole_pos[k] / mover_pos[k] = the slot of the k-th one.
+template <typename T>
+__global__ __launch_bounds__(kCompBlock) void k_fill_rank(const T* __restrict__ w,
+                                                          const Clock* clk, const int64_t* off,
+                                                          uint32_t* hole_pos,
+                                                          uint32_t* mover_pos) {
+  constexpr int NW = kCompBlock / kWave;
+  __shared__ uint64_t s_wsum[NW];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int grid = (int)gridDim.x;
+  const int b = (int)blockIdx.x;
+  const int bb = (int)off[2 * grid + 1];
+  const int64_t L = clk->n_next;
+  int64_t lo, hi;
+  comp_range(clk->n, &lo, &hi);
+  const int64_t hole0 = b <= bb ? off[b] : 0;
+  const int64_t mover0 = b >= bb && b < grid ? off[grid + b] : 0;
+  const int64_t holes_here = b == bb ? off[2 * grid + 2] : 0;
+  int64_t seen = 0;  // flagged slots of this block before the sub-tile
+  for (int64_t base = lo; base < hi; base += kCompTile) {
+    uint64_t packed = 0;
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      const int64_t i = base + k * kCompBlock + tid;
+      bool f = false;
+      if (i < hi) f = i < L ? !(w[i] > (T)0) : (w[i] > (T)0);
+      packed |= (uint64_t)(f ? 1u : 0u) << (16 * k);
+    }
+    const uint64_t incl = wave_incl_scan(packed);
+    if (lane == kWave - 1) s_wsum[wid] = incl;
+    __syncthreads();
+    uint64_t before = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      if (q < wid) before += s_wsum[q];
+      tot += s_wsum[q];
+    }
+    const uint64_t excl = before + incl - packed;
+    int64_t field_base = 0;
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      if ((packed >> (16 * k)) & 1u) {
+        const int64_t i = base + k * kCompBlock + tid;
+        const int64_t r = seen + field_base + (int64_t)((excl >> (16 * k)) & 0xFFFFu);
+        if (i < L) hole_pos[hole0 + r] = (uint32_t)i;
+        else mover_pos[mover0 + (r - holes_here)] = (uint32_t)i;
+      }
+      field_base += (int64_t)((tot >> (16 * k)) & 0xFFFFu);
+    }
+    seen += field_base;
+    __syncthreads();  // s_wsum is rewritten by the next sub-tile
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_move(bz_state_t s, const int64_t* off, int grid,
+                                                   const uint32_t* hole_pos,
+                                                   const uint32_t* mover_pos) {
+  const int64_t H = off[2 * grid];
+  T* f[5] = {(T*)s.x, (T*)s.vx, (T*)s.vy, (T*)s.vz, (T*)s.w};
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < H;
+       k += (int64_t)gridDim.x * 256) {
+    const uint32_t d = hole_pos[k], src = mover_pos[k];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) f[q][d] = f[q][src];
+  }
+}
+
+template <typename T>
+static int launch_fill_holes_t(const LaunchEnv& env, const bz_state_t* s) {
+  SortWs ws = carve(env.sort_ws, env.capacity, env.max_keys);
+  // the block counts [grid] and offsets [2*grid + 3] in the tile histogram
+  // buffer (4*256 B per tile of the capacity)
+  const int64_t bytes = (int64_t)4 * 256 * ws.max_tiles;
+  int64_t fit = (bytes - 64) / 20;
+  int grid = env.num_cu * 8;
+  if (grid > 2048) grid = 2048;
+  if (grid > fit) grid = (int)fit;
+  if (grid < 1) return fail("hole filling: workspace too small");
+  uint32_t* cnt = ws.hist;
+  int64_t* off = (int64_t*)(ws.hist + ((grid + 1) & ~1));
+  hipLaunchKernelGGL(k_fill_count<T>, dim3(grid), dim3(kCompBlock), 0, env.stream,
+                     (const T*)s->w, (const Clock*)env.clock, cnt);
+  hipLaunchKernelGGL(k_fill_plan<T>, dim3(1), dim3(1024), 0, env.stream, (const T*)s->w,
+                     env.clock, grid, (const uint32_t*)cnt, off);
+  hipLaunchKernelGGL(k_fill_rank<T>, dim3(grid), dim3(kCompBlock), 0, env.stream,
+                     (const T*)s->w, (const Clock*)env.clock, (const int64_t*)off, ws.mkeys,
+                     ws.mslot);
+  hipLaunchKernelGGL(k_fill_move<T>, dim3(env.num_cu * 4), dim3(256), 0, env.stream, *s,
+                     (const int64_t*)off, grid, (const uint32_t*)ws.mkeys,
+                     (const uint32_t*)ws.mslot);
+  hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, env.capacity);
+  return check_launch("hole filling");
+}
+
+int launch_fill_holes(const LaunchEnv& env, const bz_state_t* s) {
+  return env.precision == BZ_F64 ? launch_fill_holes_t<double>(env, s)
+                                 : launch_fill_holes_t<float>(env, s);
+}
+
 int launch_resample_inplace(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                             const uint32_t* keys, int nb, const int32_t* targets,
                             int32_t n_keys, const bz_resample_policy_t* pol) {

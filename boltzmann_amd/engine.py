@@ -51,7 +51,8 @@ class KineticEngine:
                  rng_mode: int = _lib.BZ_RNG_PHILOX, max_keys: Optional[int] = None,
                  compact_every: int = 64, push_keys: Optional[bool] = None,
                  inplace_resample: Optional[bool] = None,
-                 compact_resampled_every: Optional[int] = None):
+                 compact_resampled_every: Optional[int] = None,
+                 fill_holes: Optional[bool] = None):
         if not torch.cuda.is_available():
             raise _lib.BzError("no GPU visible: the electron step runs only in the HIP kernels")
         self.lib = _lib.load()
@@ -81,6 +82,11 @@ class KineticEngine:
             compact_resampled_every = 4 if cfg.resample_every == 1 else 1
         self.compact_resampled_every = int(compact_resampled_every)
         self._resampled_since_compact = 0
+        # compaction of the in-place runs: fill the holes below the live
+        # count from the live slots above it (bz_compact_inplace), or copy a
+        # stable filter to the other buffer (bz_compact)
+        self.fill_holes = self.inplace_resample if fill_holes is None else \
+            bool(fill_holes) and self.inplace
         # captured blocks of steps: (state buffer, steps) -> graph id
         self._graphs = {}
         self._cap_stream = None
@@ -213,11 +219,16 @@ class KineticEngine:
             self.cur = 1 - self.cur
 
     def compact(self):
-        """Remove the holes (w = 0), keeping slot order, into the other buffer."""
-        src, dst = self.bufs[self.cur], self.bufs[1 - self.cur]
-        _lib.check(self.lib.bz_compact(self.ctx, C.byref(src.struct), C.byref(dst.struct)),
-                   "bz_compact")
-        self.cur = 1 - self.cur
+        """Remove the holes (w = 0): in place by hole filling, or as a
+        stable filter into the other buffer."""
+        if self.fill_holes:
+            _lib.check(self.lib.bz_compact_inplace(self.ctx, C.byref(self.state.struct)),
+                       "bz_compact_inplace")
+        else:
+            src, dst = self.bufs[self.cur], self.bufs[1 - self.cur]
+            _lib.check(self.lib.bz_compact(self.ctx, C.byref(src.struct), C.byref(dst.struct)),
+                       "bz_compact")
+            self.cur = 1 - self.cur
         self._since_compact = 0
         self._resampled_since_compact = 0
 

@@ -282,6 +282,14 @@ int bz_push_mcc(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out,
  * reshuffle_data_atomics (kernels/post_processing_1D.py:26-38). */
 int bz_compact(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out);
 
+/* Compaction in place by hole filling: with L the live count (w > 0), the
+ * k-th slot below L that holds no particle (in slot order) receives the k-th
+ * particle at or above L (in slot order); live particles below L keep their
+ * slots and the count becomes L.  Also replaces reshuffle_data_atomics
+ * (kernels/post_processing_1D.py:26-38), whose atomics leave the order of
+ * the compacted particles open; this moves only the particles above L. */
+int bz_compact_inplace(bz_ctx_t ctx, const bz_state_t* s);
+
 /* The collision uniforms the Philox stream draws for particles 0..n-1 at
  * `step`, written as R[c*stride + i] (columns 0,1,2,3,5 used, 4 and 6 zero)
  * in the context precision.  The reference refreshes cp_R_vec with fresh

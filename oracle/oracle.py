@@ -166,6 +166,23 @@ def fit_sigma(setup: "Setup", e: np.ndarray):
     return a, b
 
 
+def fill_holes(state):
+    """Compaction in place by hole filling (the order this build fixes for
+    the reference's atomic compaction, kernels/post_processing_1D.py:26-38):
+    L = the number of slots with w > 0; the k-th slot below L with w <= 0 (in
+    slot order) receives the k-th slot at or above L with w > 0 (in slot
+    order).  Returns the first L slots of every array."""
+    arrs = [np.array(a, copy=True) for a in state]
+    w = arrs[4]
+    L = int(np.count_nonzero(w > 0))
+    holes = np.flatnonzero(~(w[:L] > 0))
+    movers = L + np.flatnonzero(w[L:] > 0)
+    assert holes.size == movers.size
+    for a in arrs:
+        a[holes] = a[movers]
+    return tuple(a[:L] for a in arrs)
+
+
 def expm1_exp(v: np.ndarray):
     """The oracle's (e^v - 1, e^v), the exponential of the collision
     probabilities, in v's precision."""

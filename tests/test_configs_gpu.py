@@ -4,7 +4,7 @@ C4 scaled to 256 cells and 300k electrons (5 bins, resampling every step,
 merge at 1.5x the 1200-per-cell target, binning keys written by the push,
 compaction every 4th pass): for 6 steps every stage is checked on the
 device's own input.  The in-place resampling equals the oracle's slot for
-slot, the compaction is a stable filter of the live slots, the push equals
+slot, the compaction fills the holes as the oracle does, the push equals
 the oracle's push of the compacted state in the field the device holds,
 and the field step equals the oracle's on the device's moments.
 
@@ -55,6 +55,7 @@ def test_c4_scaled_pipeline_stage_by_stage(gpu):
     # the holes of four passes and their appended particles fit
     assert eng.capacity >= 2.5 * cfg.n_particles
     assert eng.push_keys and eng.inplace_resample and eng.compact_resampled_every == 4
+    assert eng.fill_holes
     st0 = ph.initial_state(cfg)
     eng.load_state(st0)
     eng.initialize_fields()
@@ -81,12 +82,13 @@ def test_c4_scaled_pipeline_stage_by_stage(gpu):
         _close(after, want, 4, what=f"resample {i}")
         fired += int(wc[10])
         merges += int(wc[9])
-        # compaction when due: a stable filter of the live slots
+        # compaction when due: the holes below the live count filled from
+        # the live slots above it
         if eng._compact_due():
             eng.compact()
             compactions += 1
             comp = eng.fetch_slots()
-            for a, b in zip(comp, _live(after)):
+            for a, b in zip(comp, O.fill_holes(after)):
                 np.testing.assert_array_equal(a, b)
         # the push in the field the device holds
         src = eng.fetch_slots()

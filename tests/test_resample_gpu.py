@@ -172,8 +172,8 @@ def test_step_with_resampling_runs(gpu):
 def test_run_with_graphs_matches_steps(gpu):
     """run() replays captured blocks of pushes + field steps between the
     resampling checks and compactions; the run equals step() called step by
-    step, bit for bit (resampling fires and swaps the state buffers, and the
-    compactions fall inside the check interval)."""
+    step, bit for bit (resampling fires, and each pass is followed by a
+    compaction in place)."""
     import torch
     from boltzmann_amd import configs
     from boltzmann_amd import physics as ph
@@ -190,7 +190,7 @@ def test_run_with_graphs_matches_steps(gpu):
     for i in range(23):
         a.step(i)
     assert b.run(0, 23) == 23
-    assert len(b._graphs) >= 3
+    assert len(b._graphs) >= 2
     torch.cuda.synchronize()
     ca, cb = a.clock(), b.clock()
     assert ca[:3] == cb[:3] and ca[3] == cb[3]

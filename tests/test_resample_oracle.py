@@ -252,3 +252,29 @@ def test_inplace_layout_is_the_reference_layout():
     assert a.shape == b.shape
     # the same rows: lexicographic sort of both
     np.testing.assert_array_equal(a[np.lexsort(a.T[::-1])], b[np.lexsort(b.T[::-1])])
+
+
+def test_fill_holes_brute_force():
+    """Hole filling against a pair-by-pair construction on small inputs: the
+    live rows are kept as a multiset, the first L slots are all live, live
+    rows below L never move, and the k-th hole takes the k-th live row above
+    L; no holes and all holes included."""
+    rng = np.random.default_rng(4)
+    for n, p_hole in ((0, 0.3), (1, 1.0), (7, 0.0), (50, 0.3), (200, 0.9), (333, 0.05)):
+        w = np.where(rng.uniform(size=n) < p_hole, 0.0, rng.uniform(0.5, 2.0, n))
+        st = tuple(rng.normal(size=n) for _ in range(4)) + (w,)
+        out = O.fill_holes(st)
+        L = int((w > 0).sum())
+        assert out[4].size == L and np.all(out[4] > 0)
+        rows = list(zip(*st))
+        want = list(rows[:L])
+        tail = [r for r in rows[L:] if r[4] > 0]
+        k = 0
+        for j in range(L):
+            if not (rows[j][4] > 0):
+                want[j] = tail[k]
+                k += 1
+        assert k == len(tail)
+        got = list(zip(*out))
+        assert got == want
+        assert sorted(r[4] for r in got) == sorted(r[4] for r in rows if r[4] > 0)
