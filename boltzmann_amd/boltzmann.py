@@ -10,6 +10,22 @@ _check_for_resampling.  The step order is run()'s: resampling check every
 resample_freq steps, push + collisions + deposit + compaction, host moment
 update, heavies, Poisson, cycle averages and output every in_spc steps.
 Grid quantities stay on the device; host copies are made at output.
+
+restart_cyclenum > 0 starts from the dump files of that cycle
+(1bin/boltzmann.py:157-239, unreachable there because the prescribed
+initial condition is hard-wired, quirk Q15): the particles with w != 0 of
+e_x0/e_v0/e_v1/e_v2/e_wt, the ion density of ni, ns = 0, then the initial
+deposit and field solve as for the prescribed start.  The clock resumes at
+step restart_cyclenum * in_spc (time = that many steps of dt; the
+reference's reader keeps time 0).
+
+fix_quirks=True changes three diagnostics of the reference (left as they
+are by default): the collision counters are deposited every push, cycle
+averaged and written (Q10: collct is never reset and never written,
+1bin/boltzmann.py:310-312,618-630); the cycle averages include the output
+step (Q18: spc - 1 steps summed, divided by spc, :681,694-698); and the
+metastable average is written from ns (Q19: data_write is handed ni_CA for
+ns_CA, :705).
 """
 from __future__ import annotations
 
@@ -46,6 +62,23 @@ def reference_config(Nc: int, gridpts: int, bnf: int, in_dt: float, nb: int = 1,
         capacity_factor=4.0)
 
 
+def read_restart(name: str, cycle: int, rank: int = 0):
+    """The particles and ion density a run dumped at cycle `cycle` (the
+    files of data_write, 1bin/kernels/data_write_1D.py:9-75, read as
+    1bin/boltzmann.py:168-197 reads them): raw float64 e_x0, e_v0, e_v1,
+    e_v2, e_wt of the rank, the rows with w != 0 kept; ni of the cycle.
+    Returns ((x, vx, vy, vz, w), ni)."""
+    cyc = "_cycle" + str(int(cycle)) + ".dat"
+    cyc_rk = "_cycle" + str(int(cycle)) + "_mpi" + str(int(rank)) + ".dat"
+    ni = np.fromfile(name + "ni" + cyc, dtype=np.float64)
+    arrs = [np.fromfile(name + key + cyc_rk, dtype=np.float64)
+            for key in ("e_x0", "e_v0", "e_v1", "e_v2", "e_wt")]
+    if len({a.size for a in arrs}) != 1:
+        raise ValueError(f"restart files of cycle {cycle} differ in length")
+    keep = arrs[4] != 0
+    return tuple(np.ascontiguousarray(a[keep]) for a in arrs), ni
+
+
 def liu_initial_state(cfg: ph.RunConfig, seed: int, num_nodes: int = 1):
     """The reference's prescribed initial condition (IC_flag == 1,
     1bin/boltzmann.py:242-333): per node ii, ppc particles at the node
@@ -80,11 +113,10 @@ def liu_initial_state(cfg: ph.RunConfig, seed: int, num_nodes: int = 1):
 class BoltzmannDSMC:
     def __init__(self, num_steps, name, restart_cyclenum, Nc, gridpts, bnf, num_nodes, pyk,
                  resample_freq, in_dt, in_spc, nbins: int = 1, ref_rescale: bool = False,
-                 seed: int = 12345, device: int = 0):
-        if restart_cyclenum and restart_cyclenum > 0:
-            # the reference exits here as well (IC flag and restart together)
-            raise ValueError("restart is not combined with the prescribed initial condition")
+                 seed: int = 12345, device: int = 0, fix_quirks: bool = False, rank: int = 0):
         self.pyk = bool(pyk)
+        self.fix_quirks = bool(fix_quirks)
+        self.restart_cyclenum = int(restart_cyclenum or 0)
         self.cfg = reference_config(int(Nc), int(gridpts), int(bnf), float(in_dt), int(nbins),
                                     seed)
         self.cfg = self.cfg.with_(resample_every=int(resample_freq))
@@ -95,7 +127,16 @@ class BoltzmannDSMC:
         self.steps_per_cycle = int(in_spc)
         self.num_nodes = int(num_nodes)
         self.curr_t = 0.0
-        state, ne0 = liu_initial_state(self.cfg, seed, self.num_nodes)
+        step0 = 0
+        if self.restart_cyclenum > 0:
+            state, ni0 = read_restart(name, self.restart_cyclenum, rank)
+            if ni0.size != self.N + 1:
+                raise ValueError(f"ni of cycle {self.restart_cyclenum} has {ni0.size} nodes, "
+                                 f"the grid {self.N + 1}")
+            step0 = self.restart_cyclenum * self.steps_per_cycle
+            self.curr_t = step0 * self.dt_big
+        else:
+            state, ni0 = liu_initial_state(self.cfg, seed, self.num_nodes)
         self.eng = KineticEngine(self.cfg, device=device)
         tt = np.array([int(f * int(Nc) / self.N) for f in self.cfg.bin_fractions],
                       dtype=np.int32)
@@ -108,9 +149,11 @@ class BoltzmannDSMC:
         pol.ref_trigger = 1
         pol.max_ppc = int(10.0 * max(self.eng.targets_host.max(), 1))
         pol.ref_rescale = 1 if ref_rescale else 0
-        self.eng.ni.copy_(torch.from_numpy(ne0))
-        self.eng.load_state(state)
+        self.eng.ni.copy_(torch.from_numpy(ni0))
+        self.eng.load_state(state, step=step0, t=self.curr_t)
         self.eng.initialize_fields()
+        if self.fix_quirks:
+            self.eng.set_collision_counts(True)
         # the reference adds the (zero) initial ionization source and sets Te
         self.Nc_list = [self.eng.count()]
         nn = self.N + 1
@@ -118,6 +161,9 @@ class BoltzmannDSMC:
         self._ca = {k: torch.zeros(nn if k not in ("E", "V") else (self.N if k == "E" else
                                                                      self.N - 1), **f64)
                     for k in ("ne", "ni", "ns", "Te", "E", "V")}
+        if self.fix_quirks:
+            self._ca.update({k: torch.zeros(nn, **f64) for k in ("coll_el", "coll_exc",
+                                                                 "coll_ion")})
 
     # reference-named views (host copies)
     @property
@@ -148,14 +194,21 @@ class BoltzmannDSMC:
             e.step(bigstep)
             self.curr_t += self.dt_big
             cyclestep += 1
-            if (bigstep + 1) % self.steps_per_cycle != 0:
+            write = (bigstep + 1) % self.steps_per_cycle == 0
+            # the reference sums only the steps before the output step (Q18)
+            if not write or self.fix_quirks:
                 self._ca["ne"] += e.moments[:nn]
                 self._ca["ni"] += e.ni
                 self._ca["ns"] += e.ns
                 self._ca["Te"] += e.Te
                 self._ca["E"] += e.E
                 self._ca["V"] += e.V
-            else:
+                if self.fix_quirks:
+                    cc = e.collision_counts(reset=True)
+                    self._ca["coll_el"] += cc[0]
+                    self._ca["coll_exc"] += cc[1]
+                    self._ca["coll_ion"] += cc[2]
+            if write:
                 cyclestep = 0
                 for v in self._ca.values():
                     v /= self.steps_per_cycle
@@ -179,6 +232,10 @@ class BoltzmannDSMC:
                 arr.cpu().numpy().astype(np.float64).tofile(name + key + cyc)
             for key in ("ne", "ni", "Te", "E"):
                 self._ca[key].cpu().numpy().tofile(name + key + "_CA" + cyc)
+            if self.fix_quirks:
+                # the metastable average itself (Q19) and the collision counters (Q10)
+                for key in ("ns", "coll_el", "coll_exc", "coll_ion"):
+                    self._ca[key].cpu().numpy().tofile(name + key + "_CA" + cyc)
         st = e.fetch_state()
         for key, arr in (("e_x0", st[0]), ("e_v0", st[1]), ("e_v1", st[2]), ("e_v2", st[3]),
                          ("e_wt", st[4])):
@@ -187,4 +244,4 @@ class BoltzmannDSMC:
         sys.stdout.flush()
 
 
-__all__ = ["BoltzmannDSMC", "reference_config", "liu_initial_state"]
+__all__ = ["BoltzmannDSMC", "reference_config", "liu_initial_state", "read_restart"]

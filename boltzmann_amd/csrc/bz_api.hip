@@ -143,6 +143,16 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   if (st == hipSuccess)
     st = hipMemset(e.dep_m1_lo, 0, (size_t)max_nodes * sizeof(unsigned long long));
   if (st == hipSuccess) st = hipMemset(e.dep_m1_hi, 0, (size_t)max_nodes * sizeof(uint32_t));
+  // collision counters: 2 fixed-point rows, their carries, the counts, the accumulator
+  e.coll_on = 0;
+  const size_t coll_bytes = (size_t)max_nodes * (16 + 8 + 8 + 24);
+  if (st == hipSuccess) st = hipMalloc((void**)&e.coll_lo, coll_bytes);
+  if (st == hipSuccess) st = hipMemset(e.coll_lo, 0, coll_bytes);
+  if (st == hipSuccess) {
+    e.coll_cnt = e.coll_lo + 2 * (size_t)max_nodes;
+    e.coll_acc = (double*)(e.coll_cnt + max_nodes);
+    e.coll_hi = (uint32_t*)(e.coll_acc + 3 * (size_t)max_nodes);
+  }
   if (st == hipSuccess) st = hipHostMalloc((void**)&c->host_clock, sizeof(Clock));
   if (st == hipSuccess) st = hipMemset(e.clock, 0, sizeof(Clock));
   if (st == hipSuccess) st = hipMemset(e.status, 0, e.status_words * sizeof(uint64_t));
@@ -175,6 +185,7 @@ int bz_ctx_destroy(bz_ctx_t c) {
   if (c->env.dep_cols_done) (void)hipFree(c->env.dep_cols_done);
   if (c->env.dep_m1_lo) (void)hipFree(c->env.dep_m1_lo);
   if (c->env.dep_m1_hi) (void)hipFree(c->env.dep_m1_hi);
+  if (c->env.coll_lo) (void)hipFree(c->env.coll_lo);
   if (c->host_clock) (void)hipHostFree(c->host_clock);
   graph_release(c);
   if (Timing* t = c->env.timing) {
@@ -441,6 +452,19 @@ int bz_compact(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out) {
   if (in->w == out->w) return fail("bz_compact needs two distinct state buffers");
   c->keys_state = nullptr;
   return launch_compact(c->env, in, out);
+}
+
+int bz_set_collision_counts(bz_ctx_t c, int32_t on) {
+  BZ_CTX(c);
+  c->env.coll_on = on ? 1 : 0;
+  return 0;
+}
+
+int bz_collision_counts(bz_ctx_t c, double* out, int32_t n_nodes, int32_t reset) {
+  BZ_CTX(c);
+  if (!out) return fail("null output");
+  if (n_nodes < 1 || n_nodes > c->env.max_nodes) return fail("n_nodes outside [1, max_nodes]");
+  return launch_collision_read(c->env, out, n_nodes, reset);
 }
 
 int bz_compact_inplace(bz_ctx_t c, const bz_state_t* s) {

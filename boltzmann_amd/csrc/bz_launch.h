@@ -35,6 +35,14 @@ struct LaunchEnv {
   int push_queue;            // 1-D push candidate queue: 0 auto, 1 always, -1 never
   unsigned long long* dep_m1_lo;  // 1-D push: global fixed-point ionization sums [max_nodes]
   uint32_t* dep_m1_hi;            //   and their carry words (zero between pushes)
+  // 1-D push collision counters (bz_set_collision_counts): fixed-point sums
+  // [2][max_nodes] + carry words and the ionization counts [max_nodes],
+  // folded after every push into the FP64 accumulator coll_acc [3][max_nodes]
+  int coll_on;
+  unsigned long long* coll_lo;
+  uint32_t* coll_hi;
+  unsigned long long* coll_cnt;
+  double* coll_acc;
   int push_keys_nb;          // 1-D push writes binning keys for nb bins (0: off)
   double push_keys_thr[8];   // their energy thresholds
   // optional kernel timing: event pairs per timed kernel class
@@ -67,6 +75,8 @@ int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t*
 // the weight bound of a state into the clock (the deposit's fixed-point scale)
 int launch_dep_wmax(const LaunchEnv& env, const bz_state_t* s);
 int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out);
+// the 1-D collision counters accumulated since the last reset, to out[3][nn]
+int launch_collision_read(const LaunchEnv& env, double* out, int nn, int reset);
 // in place: the holes below the live count filled from the live slots above it
 int launch_fill_holes(const LaunchEnv& env, const bz_state_t* s);
 // sum the push's per-block partial rows (m doubles each) into out[m];

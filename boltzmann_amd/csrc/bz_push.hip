@@ -57,7 +57,31 @@ struct PushArgs {
   // reads and clears them)
   unsigned long long* m1_lo;
   uint32_t* m1_hi;
+  // 1-D collision counters (NULL: off): per node sum w of the elastic and of
+  // the excitation collisions (fixed point, carry words) and the number of
+  // ionizations; k_push_post folds them into coll_acc [3][n_nodes]
+  unsigned long long* coll_lo;
+  uint32_t* coll_hi;
+  unsigned long long* coll_cnt;
+  double* coll_acc;
 };
+
+// A collision of the 1-D push into the counters: at the primary's node after
+// the push, node 0 when it left the gap (the reference's collct rows,
+// electron_kernel_numba.py:199,232, are reduced at the post-push position,
+// x = 1e-8 for an absorbed particle, post_processing_1D.py:56); elastic and
+// excitation events add the weight each (free-flight mode: every event of
+// the step), an ionization counts 1.
+template <typename T>
+__device__ __forceinline__ void coll_add(const PushArgs& a, int nn, const StepOut<T>& o,
+                                         const bz_params_t& p, double w, double scale) {
+  if (!(o.n_el || o.n_exc || o.chan == kChanIon)) return;
+  const int nd = o.alive ? dep_node((double)o.p.x, p) : 0;
+  if (o.n_el) dep_fixed_add_global(a.coll_lo, a.coll_hi, nd, w * (double)o.n_el, scale);
+  if (o.n_exc)
+    dep_fixed_add_global(a.coll_lo + nn, a.coll_hi + nn, nd, w * (double)o.n_exc, scale);
+  if (o.chan == kChanIon) atomicAdd(&a.coll_cnt[nd], 1ull);
+}
 
 // per-block counter row of the push (u32): births, absorbed, elastic,
 // excitation, ionization, null, dropped, nu_max exceeded, live inputs
@@ -760,6 +784,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
             if (DIM == 1 && o.chan == kChanIon && (p.ref_flags & BZ_REF_DEAD_IONIZER))
               dep_fixed_add_global(a.m1_lo, a.m1_hi, 0, 1.0, dscale);  // the reference's flag
           }
+          if (DIM == 1 && a.coll_lo) coll_add<T>(a, nn, o, p, (double)c.w, dscale);
           if (DIM == 1 && a.keys)
             a.keys[i] = o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w) : (uint32_t)a.bk.n_keys;
         }
@@ -942,6 +967,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           if (DIM == 1 && o.chan == kChanIon && (p.ref_flags & BZ_REF_DEAD_IONIZER))
             dep_fixed_add_global(a.m1_lo, a.m1_hi, 0, 1.0, dscale);  // the reference's flag
         }
+        if (DIM == 1 && a.coll_lo) coll_add<T>(a, nn, o, p, (double)cur[k].w, dscale);
       }
       if (DIM == 1 && a.keys && i < n)
         a.keys[i] = live && o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w)
@@ -1113,10 +1139,35 @@ __global__ __launch_bounds__(kRedThreads) void k_push_post(PushArgs a, int tile,
     if (!a.m1_lo) f.lo = nullptr;
     dep_reduce_part(a.partials, nbirth, m, rpc, l2, col_done, out, acc, r % nx, r / nx, ny,
                     s_part, &s_last, f);
-  } else {
+  } else if (b == nbirth + nx * ny) {
     dep_commit(a.partials, nbirth, m, a.clk, a.capacity, false, advance_dt);
+  } else if (a.coll_lo) {
+    // the collision counters of this push into the accumulator, words cleared
+    const int nn = a.p.n_nodes;
+    const int k = (b - nbirth - nx * ny - 1) * kRedThreads + threadIdx.x;
+    if (k < nn) {
+      const double inv = 1.0 / dep_scale_of(a.clk->w_max_bits);
+      for (int r = 0; r < 2; ++r) {
+        const int j = r * nn + k;
+        a.coll_acc[j] += dep_fixed_value(a.coll_lo[j], a.coll_hi[j], inv);
+        a.coll_lo[j] = 0ull;
+        a.coll_hi[j] = 0u;
+      }
+      a.coll_acc[2 * nn + k] += (double)a.coll_cnt[k];
+      a.coll_cnt[k] = 0ull;
+    }
   }
 }
+
+// The collision counters accumulated since the last reset to out[3][nn]
+// (device memory); reset clears the accumulator.
+__global__ void k_coll_read(double* acc, double* out, int nn, int reset) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < 3 * nn; j += gridDim.x * blockDim.x) {
+    out[j] = acc[j];
+    if (reset) acc[j] = 0.0;
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // Stable compaction of the live slots (w > 0).  Block b owns a contiguous run
@@ -1533,7 +1584,9 @@ static int launch_push_post(const LaunchEnv& env, const PushArgs& a, const PushL
   int rpc = 0, nx = 0, ny = 0;
   const int rc = red_geometry(env, pl.grid, m, &rpc, &nx, &ny);
   if (rc) return rc;
-  const unsigned blocks = (unsigned)(pl.grid + nx * ny + 1);
+  const unsigned coll_blocks =
+      a.coll_lo ? (unsigned)((a.p.n_nodes + kRedThreads - 1) / kRedThreads) : 0u;
+  const unsigned blocks = (unsigned)(pl.grid + nx * ny + 1) + coll_blocks;
   if (env.precision == BZ_F64)
     hipLaunchKernelGGL(k_push_post<double>, dim3(blocks), dim3(kRedThreads), 0, env.stream, a,
                        pl.tile, pl.nw, pl.grid, m, rpc, nx, ny, env.dep_l2, env.dep_cols_done,
@@ -1572,6 +1625,11 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
   a.e_lds_off = 0;
   a.m1_lo = p->dim == 1 ? env.dep_m1_lo : nullptr;
   a.m1_hi = p->dim == 1 ? env.dep_m1_hi : nullptr;
+  const bool coll = p->dim == 1 && env.coll_on;
+  a.coll_lo = coll ? env.coll_lo : nullptr;
+  a.coll_hi = coll ? env.coll_hi : nullptr;
+  a.coll_cnt = coll ? env.coll_cnt : nullptr;
+  a.coll_acc = coll ? env.coll_acc : nullptr;
   if (p->dim == 1 && env.push_keys_nb > 0) {
     a.keys = (uint32_t*)sort_input_keys(env);
     a.bk = bin_keys(p, env.push_keys_thr, env.push_keys_nb);
@@ -1621,6 +1679,12 @@ static int launch_compact_t(const LaunchEnv& env, const bz_state_t* in, const bz
   if (rc) return rc;
   hipLaunchKernelGGL(k_clock_commit_next, dim3(1), dim3(1), 0, env.stream, env.clock);
   return check_launch("k_clock_commit_next");
+}
+
+int launch_collision_read(const LaunchEnv& env, double* out, int nn, int reset) {
+  hipLaunchKernelGGL(k_coll_read, dim3((3 * nn + 255) / 256), dim3(256), 0, env.stream,
+                     env.coll_acc, out, nn, reset);
+  return check_launch("k_coll_read");
 }
 
 int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out) {
@@ -1715,6 +1779,10 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
   a.e_lds_off = 0;
   a.m1_lo = nullptr;
   a.m1_hi = nullptr;
+  a.coll_lo = nullptr;
+  a.coll_hi = nullptr;
+  a.coll_cnt = nullptr;
+  a.coll_acc = nullptr;
   PushLaunch pl;
   const int rc = env.precision == BZ_F64 ? launch_push_prec<double>(env, a, shape, &pl)
                                          : launch_push_prec<float>(env, a, shape, &pl);

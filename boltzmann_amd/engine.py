@@ -316,10 +316,17 @@ class KineticEngine:
                     self._resampled_since_compact >= self.compact_resampled_every)
 
     def initialize_fields(self):
-        """Initial deposit and Poisson solve at t (boltzmann.py:363-399)."""
+        """Initial deposit, moment update and Poisson solve at t
+        (boltzmann.py:363-399): ni gains the (zero) ionization source and Te
+        and E follow from the deposit; the ions are not advanced."""
         self.deposit()
         if self.cfg.dim == 1:
-            self.field_step(advance=False)
+            fluid = self.fp.fluid_on
+            self.fp.fluid_on = 0
+            try:
+                self.field_step(advance=False)
+            finally:
+                self.fp.fluid_on = fluid
 
     def step(self, index: Optional[int] = None):
         """One kinetic step in the reference's order (boltzmann.py:570-678):
@@ -415,6 +422,23 @@ class KineticEngine:
         finally:
             self.cur = cur
         return len(self._graphs)
+
+    def set_collision_counts(self, on: bool) -> None:
+        """Turn the 1-D push's collision counters on or off (captured step
+        blocks are dropped: their pushes carry the setting)."""
+        _lib.check(self.lib.bz_set_collision_counts(self.ctx, 1 if on else 0),
+                   "bz_set_collision_counts")
+        self.drop_graphs()
+
+    def collision_counts(self, reset: bool = True) -> "torch.Tensor":
+        """Per node since the last reset: sum w of the elastic collisions, sum
+        w of the excitations, number of ionizations ([3, n_nodes], FP64 on
+        the device)."""
+        nn = self.cfg.n_nodes
+        out = torch.empty(3 * nn, dtype=torch.float64, device=self.dev)
+        _lib.check(self.lib.bz_collision_counts(self.ctx, _tptr(out), nn, 1 if reset else 0),
+                   "bz_collision_counts")
+        return out.view(3, nn)
 
     def drop_graphs(self):
         """Forget the captured blocks (after a change of the push knobs)."""

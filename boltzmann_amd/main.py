@@ -12,18 +12,18 @@ from .boltzmann import BoltzmannDSMC
 
 
 def main(restart_cyclenum, num_steps, name, num_particles, gridpts, bnf, num_nodes, pyk,
-         resample_freq, in_dt, in_spc, nbins=1, ref_rescale=False):
+         resample_freq, in_dt, in_spc, nbins=1, ref_rescale=False, fix_quirks=False):
     boltzmann = BoltzmannDSMC(num_steps, name, restart_cyclenum, num_particles, gridpts, bnf,
                               num_nodes, pyk, resample_freq, in_dt, in_spc, nbins=nbins,
-                              ref_rescale=ref_rescale)
+                              ref_rescale=ref_rescale, fix_quirks=fix_quirks)
     boltzmann.run(num_steps, name, restart_cyclenum)
     return boltzmann
 
 
 def run(restart_cyclenum, num_steps, name, num_particles, gridpts, bnf, num_nodes, pyk,
-        resample_freq, in_dt, in_spc, nbins=1, ref_rescale=False):
+        resample_freq, in_dt, in_spc, nbins=1, ref_rescale=False, fix_quirks=False):
     return main(restart_cyclenum, num_steps, name, num_particles, gridpts, bnf, num_nodes, pyk,
-                resample_freq, in_dt, in_spc, nbins, ref_rescale)
+                resample_freq, in_dt, in_spc, nbins, ref_rescale, fix_quirks)
 
 
 def parse(argv=None):
@@ -43,6 +43,9 @@ def parse(argv=None):
                         help="energy bins per cell (the reference's 1bin/3bin/5bin directories)")
     parser.add_argument("--ref-rescale", action="store_true",
                         help="merge rescale from total energy as the reference does")
+    parser.add_argument("--fix-quirks", action="store_true",
+                        help="write the collision counters and the ns average, and include "
+                             "the output step in the cycle averages")
     return parser.parse_args(argv)
 
 
@@ -50,4 +53,4 @@ if __name__ == "__main__":
     args = parse()
     run(args.restart_cyclenum, args.steps, args.data_name, args.num_particles, args.gridpts,
         args.bnf, args.num_nodes, args.pykokkos, args.resample_freq, args.in_dt, args.in_spc,
-        args.bins, args.ref_rescale)
+        args.bins, args.ref_rescale, args.fix_quirks)

@@ -290,6 +290,21 @@ int bz_compact(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out);
  * the compacted particles open; this moves only the particles above L. */
 int bz_compact_inplace(bz_ctx_t ctx, const bz_state_t* s);
 
+/* Collision counters of the 1-D push, the reference's collct deposit
+ * (kernels/electron_kernel_numba.py:199,232 set collct[i,0] = w for an
+ * elastic collision and collct[i,2] = 1 for an ionization;
+ * kernels/post_processing_1D.py:56 reduces them to counter_g0..g3 at the
+ * particles' post-push positions, an absorbed particle at x = 1e-8).  With
+ * the counters on, every 1-D push adds per node: [0] sum w of the elastic
+ * collisions, [1] sum w of the excitations (no column in the reference's
+ * 1bin kernel), [2] the number of ionizations; at the primary's node after
+ * the push, node 0 when it was absorbed.  The reference never resets its
+ * collct rows and never writes the counters (quirk Q10); here they cover
+ * the pushes since the last read with reset.  bz_collision_counts copies
+ * them to out[3][n_nodes] (device memory) and clears them when reset != 0. */
+int bz_set_collision_counts(bz_ctx_t ctx, int32_t on);
+int bz_collision_counts(bz_ctx_t ctx, double* out, int32_t n_nodes, int32_t reset);
+
 /* The collision uniforms the Philox stream draws for particles 0..n-1 at
  * `step`, written as R[c*stride + i] (columns 0,1,2,3,5 used, 4 and 6 zero)
  * in the context precision.  The reference refreshes cp_R_vec with fresh

@@ -691,7 +691,7 @@ template <typename T>
 int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const T* vy,
                   const T* vz, const T* w, const T* E, const T* R, int64_t rs, const T* G,
                   const orc_table* tab, T* ox, T* ovx, T* ovy, T* ovz, T* ow, int64_t cap,
-                  double* mom, uint64_t* cnt) {
+                  double* mom, uint64_t* cnt, double* coll) {
   const int nn = P->dim == 1 ? P->n_nodes : 1;
   int nt = P->nthreads > 0 ? P->nthreads : 1;
 #ifndef _OPENMP
@@ -700,6 +700,12 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
   // fixed chunking: chunk k handles [k*n/nt, (k+1)*n/nt)
   std::vector<int64_t> chunk_cnt(nt + 1, 0);
   std::vector<std::vector<double>> part(nt, std::vector<double>(3 * nn, 0.0));
+  // collision counters (coll != NULL): per node, sum w of the elastic and of
+  // the excitation collisions and the number of ionizations, at the
+  // primary's position after the push, node 0 when it left the gap (the
+  // reference's collct rows, electron_kernel_numba.py:199,232, deposited at
+  // x = 1e-8 for an absorbed particle, post_processing_1D.py:56)
+  std::vector<std::vector<double>> cpart(coll ? nt : 0, std::vector<double>(3 * nn, 0.0));
   std::vector<std::vector<uint64_t>> ccnt(nt, std::vector<uint64_t>(8, 0));
   std::vector<std::vector<Rec<T>>> buf(nt), sec(nt);
   const bool births_last = P->births_last != 0;
@@ -726,6 +732,13 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
           cc[0] += ev.ion;
           cc[5] += ev.null_;
           if (!ev.alive) cc[1] += 1;
+          if (coll && (ev.el || ev.exc || ev.ion) && P->dim == 1) {
+            const int nd = ev.alive ? node_of((double)o[0].x, *P) : 0;
+            double* cp = cpart[k].data();
+            cp[nd] += (double)w[i] * (double)ev.el;
+            cp[nn + nd] += (double)w[i] * (double)ev.exc;
+            cp[2 * nn + nd] += ev.ion ? 1.0 : 0.0;
+          }
         }
       } else {
         m = one_particle<T>(i, *P, x, vx, vy, vz, w, E, R, rs, G, tab, o, eo, &chan, &ion);
@@ -739,6 +752,13 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
           // ionizer that left keeps its unscaled flag 1 at x = 1e-8, node 0
           // (electron_kernel_numba.py:231,267-270; post_processing_1D.py:48-55)
           if (!alive_primary && chan == 3 && (P->ref_flags & 1) && P->dim == 1) pm[nn] += 1.0;
+          if (coll && chan != 0 && P->dim == 1) {
+            const int nd = alive_primary ? node_of((double)o[0].x, *P) : 0;
+            double* cp = cpart[k].data();
+            if (chan == 1) cp[nd] += (double)w[i];
+            else if (chan == 2) cp[nn + nd] += (double)w[i];
+            else cp[2 * nn + nd] += 1.0;
+          }
         }
       }
       // the secondary, when one was born, is the last record
@@ -783,6 +803,12 @@ int64_t push_step(const orc_params* P, int64_t n, const T* x, const T* vx, const
     for (int k = 0; k < nt; ++k) s += part[k][j];
     mom[j] = s;
   }
+  if (coll)
+    for (int j = 0; j < 3 * nn; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < nt; ++k) s += cpart[k][j];
+      coll[j] = s;
+    }
   for (int q = 0; q < 8; ++q) {
     uint64_t s = 0;
     for (int k = 0; k < nt; ++k) s += ccnt[k][q];
@@ -800,18 +826,18 @@ int64_t orc_push_step_f64(const orc_params* P, int64_t n, const double* x, const
                           const double* vy, const double* vz, const double* w, const double* E,
                           const double* R, int64_t rs, const double* G, const orc_table* tab,
                           double* ox, double* ovx, double* ovy, double* ovz, double* ow,
-                          int64_t cap, double* mom, uint64_t* cnt) {
+                          int64_t cap, double* mom, uint64_t* cnt, double* coll) {
   return push_step<double>(P, n, x, vx, vy, vz, w, E, R, rs, G, tab, ox, ovx, ovy, ovz, ow, cap,
-                           mom, cnt);
+                           mom, cnt, coll);
 }
 
 int64_t orc_push_step_f32(const orc_params* P, int64_t n, const float* x, const float* vx,
                           const float* vy, const float* vz, const float* w, const float* E,
                           const float* R, int64_t rs, const float* G, const orc_table* tab,
                           float* ox, float* ovx, float* ovy, float* ovz, float* ow, int64_t cap,
-                          double* mom, uint64_t* cnt) {
+                          double* mom, uint64_t* cnt, double* coll) {
   return push_step<float>(P, n, x, vx, vy, vz, w, E, R, rs, G, tab, ox, ovx, ovy, ovz, ow, cap,
-                          mom, cnt);
+                          mom, cnt, coll);
 }
 
 // Deposit of a state (no push): ne, ni_src (=0), nrg per node.

@@ -198,13 +198,16 @@ def resample_uniform(seed: int, step: int, key: int, m: int) -> float:
     return load().orc_resample_uniform(seed, step, key, m)
 
 
-def push_step(setup: Setup, state, E, R, G=None, table=None, capacity=None):
+def push_step(setup: Setup, state, E, R, G=None, table=None, capacity=None,
+              collisions: bool = False):
     """One collision + push + wall + deposit + compaction step.
 
     state: (x, vx, vy, vz, w) arrays of one dtype; R: [7, stride] uniforms
     (None in free-flight mode, which draws Philox(seed, step) itself);
     G: optional [3, stride] gas velocities; table: (energy, sig[3,n], slope[3,n]).
-    Returns (new_state, moments[3, n_nodes], counters[8])."""
+    Returns (new_state, moments[3, n_nodes], counters[8]), and with
+    collisions=True also the collision counters [3, n_nodes] (sum w of the
+    elastic and of the excitation collisions, number of ionizations)."""
     lib = load()
     x, vx, vy, vz, w = [np.ascontiguousarray(a) for a in state]
     dt = x.dtype
@@ -229,10 +232,13 @@ def push_step(setup: Setup, state, E, R, G=None, table=None, capacity=None):
     rs = Ra.shape[1] if Ra is not None else (Ga.shape[1] if Ga is not None else 1)
     if Ga is not None and Ga.shape[1] != rs:
         raise ValueError("R and G must share the column stride")
+    coll = np.zeros(3 * nn) if collisions else None
     fn = lib.orc_push_step_f64 if dt == np.float64 else lib.orc_push_step_f32
     m = fn(C.byref(p), C.c_int64(n), *[_ptr(a) for a in (x, vx, vy, vz, w)], _ptr(Ea), _ptr(Ra),
            C.c_int64(rs), _ptr(Ga), C.byref(tab) if tab is not None else None,
-           *[_ptr(a) for a in out], C.c_int64(cap), _ptr(mom), _ptr(cnt))
+           *[_ptr(a) for a in out], C.c_int64(cap), _ptr(mom), _ptr(cnt), _ptr(coll))
+    if collisions:
+        return tuple(a[:m] for a in out), mom.reshape(3, nn), cnt, coll.reshape(3, nn)
     return tuple(a[:m] for a in out), mom.reshape(3, nn), cnt
 
 

@@ -805,3 +805,29 @@ def test_exponential_within_two_ulp_of_libm(dtype):
         ulp = np.spacing(np.abs(want).astype(dtype)).astype(np.float64)
         err = np.abs(got.astype(np.float64) - want) / ulp
         assert err.max() <= 2.0, (dtype, err.max(), v[np.argmax(err)])
+
+
+@pytest.mark.parametrize("xs_mode", [1, 3, 2])
+def test_collision_counters_sum_to_the_channel_counts(xs_mode):
+    """The collision counters of the oracle's push (the reference's collct
+    deposit): with unit weights the elastic and excitation rows sum to the
+    step's elastic and excitation event counts and the ionization row to
+    the ionizations; with every particle at one node the counters sit at
+    that node, or at node 0 for the absorbed."""
+    n, N, L = 40_000, 32, 2.5
+    t, tab = _table(np.float64, gas=3.2e23)
+    for spread in (True, False):
+        rng = np.random.default_rng(7)
+        x = rng.uniform(0.05 * L, 0.95 * L, n) if spread else np.full(n, 10.3 * L / N)
+        sv = math.sqrt(20.0 / KEV / 3)
+        st = (x, *rng.normal(0, sv, size=(3, n)), np.ones(n))
+        s = O.Setup(dim=1, dt=2e-11, xs_mode=xs_mode, n_cells=N, length_cm=L, bnf=4,
+                    gas_density_m3=3.2e23, nu_max=t.nu_max)
+        R = None if xs_mode == 2 else O.collision_uniforms(3, 0, n, np.float64)
+        E = np.zeros(N)
+        _, _, cnt, coll = O.push_step(s, st, E, R, table=tab, collisions=True)
+        assert coll[0].sum() == cnt[2] and coll[1].sum() == cnt[3] and coll[2].sum() == cnt[4]
+        assert cnt[2] > 100 and cnt[4] > 10
+        if not spread:
+            # the electrons move less than a cell: node 10 (x = 10.3 dx)
+            assert set(np.flatnonzero(coll.sum(0))) <= {0, 10}

@@ -456,3 +456,55 @@ def test_push_dead_ionizer_flag(gpu):
         if flags:
             assert mom[1][0] >= 10  # dead ionizers happened and were counted
         eng.close()
+
+
+COLL_CASES = [("table-f64", 8, 1, False), ("direct-f32", 4, 3, False),
+              ("direct-f32-queued", 4, 3, True), ("flight-f64", 8, 2, False)]
+
+
+@pytest.mark.parametrize("name,prec,xs,queued", COLL_CASES, ids=[c[0] for c in COLL_CASES])
+def test_collision_counters_match_the_oracle(gpu, name, prec, xs, queued):
+    """bz_collision_counts after two 1-D pushes equals the oracle's counters
+    of the two steps (the reference's collct deposit: sum w of elastic and
+    of excitation collisions, the ionization count, at the primary's node
+    after the push, node 0 when absorbed): counts exact, weights within
+    1e-12 / 1e-6; a read with reset leaves zeros; counters off add nothing."""
+    import torch
+    from boltzmann_amd import _lib
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    n = 200_000 if queued else 30_000
+    gas = 3.2e24 if xs == 1 else 3.22e23
+    cfg = _cfg(ph, 1, prec, xs, n, gas=gas, push_order=1)
+    eng = KineticEngine(cfg, capacity=3 * n + 64)
+    if queued:
+        _lib.check(eng.lib.bz_set_push_queue(eng.ctx, 1))
+        _lib.check(eng.lib.bz_set_push_grid(eng.ctx, 1))
+        _lib.check(eng.lib.bz_set_push_shape(eng.ctx, 3))
+    dt_np = cfg.np_dtype
+    st = _state(n, dt_np, 1, seed=len(name))
+    eng.load_state(st)
+    E = np.random.default_rng(1).uniform(-400, 400, cfg.n_cells).astype(dt_np)
+    eng.E_t.copy_(torch.from_numpy(E))
+    eng.push()  # counters off: nothing is counted
+    assert not eng.collision_counts(reset=True).cpu().numpy().any()
+    eng.set_collision_counts(True)
+    s = _setup(cfg, eng.table)
+    s.seed = cfg.seed
+    want = np.zeros((3, cfg.n_nodes))
+    for _ in range(2):
+        src = eng.fetch_slots()
+        _, step, _, _ = eng.clock()
+        R = None if xs == 2 else O.collision_uniforms(cfg.seed, step, src[0].size, dt_np)
+        s.step = step
+        _, _, _, coll = O.push_step(s, src, E, R, table=_tab(eng.table),
+                                    capacity=3 * n + 64, collisions=True)
+        want += coll
+        eng.push()
+    got = eng.collision_counts(reset=True).cpu().numpy()
+    np.testing.assert_array_equal(got[2], want[2])
+    rt = 1e-12 if prec == 8 else 1e-6
+    np.testing.assert_allclose(got[:2], want[:2], rtol=rt, atol=rt * np.abs(want[:2]).max())
+    assert want[0].sum() > 0 and want[2].sum() > 0 and (xs == 1 or want[1].sum() > 0)
+    assert not eng.collision_counts(reset=False).cpu().numpy().any()
+    eng.close()

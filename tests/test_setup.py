@@ -123,3 +123,28 @@ def test_reference_initial_condition():
         assert abs(v.var() / var - 1) < 5 * math.sqrt(2 / n)
     # shuffled: nodes are not in blocks, and not in order
     assert np.mean(np.diff(node) != 0) > 0.9
+
+
+def test_read_restart_keeps_the_weighted_rows(tmp_path):
+    """Restart files (raw float64, the reference's names): rows with w = 0
+    are dropped, ni is read back, files of unequal length are an error."""
+    from boltzmann_amd.boltzmann import read_restart
+    rng = np.random.default_rng(1)
+    name = str(tmp_path / "run_")
+    n = 1000
+    cols = {k: rng.normal(size=n) for k in ("e_x0", "e_v0", "e_v1", "e_v2")}
+    w = rng.uniform(0.5, 2, n)
+    w[rng.uniform(size=n) < 0.3] = 0.0
+    cols["e_wt"] = w
+    for k, a in cols.items():
+        a.tofile(name + k + "_cycle3_mpi0.dat")
+    ni = rng.uniform(size=33)
+    ni.tofile(name + "ni_cycle3.dat")
+    st, ni2 = read_restart(name, 3, 0)
+    keep = w != 0
+    for a, k in zip(st, ("e_x0", "e_v0", "e_v1", "e_v2", "e_wt")):
+        np.testing.assert_array_equal(a, cols[k][keep])
+    np.testing.assert_array_equal(ni2, ni)
+    cols["e_v2"][:10].tofile(name + "e_v2_cycle3_mpi0.dat")
+    with pytest.raises(ValueError):
+        read_restart(name, 3, 0)

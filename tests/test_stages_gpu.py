@@ -180,3 +180,39 @@ def test_boltzmann_driver_runs(gpu, nbins, tmp_path):
     x = np.fromfile(name + "e_x0_cycle2_mpi0.dat")
     assert x.size > 0 and np.all((x > 0) & (x < 2.54))
     assert math.isfinite(float(np.fromfile(name + "Te_CA_cycle1.dat").sum()))
+
+
+def test_boltzmann_restart_and_quirk_fixes(gpu, tmp_path):
+    """A run writes cycle 2; a run restarted from cycle 2 holds exactly the
+    dumped particles (w != 0) and ion density, resumes the clock at step
+    2 * spc and writes cycle 3.  With fix_quirks the ns average and the
+    collision-counter averages are written, the counters are positive where
+    electrons collide and the ionization average is a count per step."""
+    import torch
+    from boltzmann_amd.boltzmann import BoltzmannDSMC
+    name = str(tmp_path / "run_")
+    spc = 8
+    a = BoltzmannDSMC(2 * spc, name, 0, 40_000, 65, 4, 1, False, 5, 1e4, spc,
+                      fix_quirks=True)
+    a.run(2 * spc, name, 0, verbose=False)
+    for key in ("ns", "coll_el", "coll_exc", "coll_ion"):
+        arr = np.fromfile(name + key + "_CA_cycle2.dat")
+        assert arr.size == 65 and np.all(np.isfinite(arr))
+    el = np.fromfile(name + "coll_el_CA_cycle2.dat")
+    assert el.sum() > 0
+    x = np.fromfile(name + "e_x0_cycle2_mpi0.dat")
+    w = np.fromfile(name + "e_wt_cycle2_mpi0.dat")
+    ni = np.fromfile(name + "ni_cycle2.dat")
+    b = BoltzmannDSMC(spc, name, 2, 40_000, 65, 4, 1, False, 5, 1e4, spc)
+    n, step, t, _ = b.eng.clock()
+    assert step == 2 * spc and abs(t - 2 * spc * b.dt_big) <= 1e-15 * t
+    st = b.eng.fetch_state()
+    keep = w != 0
+    assert n == int(keep.sum())
+    np.testing.assert_array_equal(st[0], x[keep])
+    np.testing.assert_array_equal(st[4], w[keep])
+    np.testing.assert_array_equal(b.eng.ni.cpu().numpy(), ni)
+    torch.cuda.synchronize()
+    b.run(spc, name, 2, verbose=False)
+    assert np.fromfile(name + "ne_cycle3.dat").size == 65
+    assert b.Nc_list[0] > 0
