@@ -110,6 +110,8 @@ SIGNATURES = {
     "bz_compact": (C.c_int, [_P, C.POINTER(State), C.POINTER(State)]),
     "bz_compact_inplace": (C.c_int, [_P, C.POINTER(State)]),
     "bz_set_collision_counts": (C.c_int, [_P, C.c_int32]),
+    "bz_bin_lists": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int32, C.c_int32, _P, _P,
+                               _P, _P]),
     "bz_collision_counts": (C.c_int, [_P, _P, C.c_int32, C.c_int32]),
     "bz_reduction": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, C.c_int64]),
     "bz_reduction_static": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, C.c_int64,

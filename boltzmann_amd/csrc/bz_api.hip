@@ -454,6 +454,27 @@ int bz_compact(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out) {
   return launch_compact(c->env, in, out);
 }
 
+int bz_bin_lists(bz_ctx_t c, const double* tosum, int64_t tosum_cols, const int64_t* curr_xbins,
+                 int64_t nc, int32_t n_cells, int32_t nb, const double* thresholds_host,
+                 int64_t* const* lists, int64_t* const* write_inds, const int64_t* max_ppc) {
+  BZ_CTX(c);
+  if (nb < 1 || nb > 8) return fail("nb must be in [1,8]");
+  if (nc < 0) return fail("negative row count");
+  if (n_cells < 1) return fail("n_cells must be >= 1");
+  if (tosum_cols < 4) return fail("tosum needs at least 4 columns (w ... energy)");
+  if (nc > 0 && (!tosum || !curr_xbins)) return fail("null tosum / curr_xbins");
+  if (!lists || !write_inds || !max_ppc || (nb > 1 && !thresholds_host))
+    return fail("null lists, write_inds, max_ppc or thresholds");
+  for (int b = 0; b < nb; ++b) {
+    if (!write_inds[b]) return fail("null write_inds[%d]", b);
+    if (max_ppc[b] < 0) return fail("negative max_ppc[%d]", b);
+    if (max_ppc[b] > 0 && !lists[b]) return fail("null list[%d]", b);
+  }
+  c->keys_state = nullptr;
+  return launch_bin_lists(c->env, tosum, tosum_cols, curr_xbins, nc, n_cells, nb,
+                          thresholds_host, lists, write_inds, max_ppc);
+}
+
 int bz_set_collision_counts(bz_ctx_t c, int32_t on) {
   BZ_CTX(c);
   c->env.coll_on = on ? 1 : 0;

@@ -75,6 +75,11 @@ int launch_deposit(const LaunchEnv& env, const bz_state_t* s, const bz_params_t*
 // the weight bound of a state into the clock (the deposit's fixed-point scale)
 int launch_dep_wmax(const LaunchEnv& env, const bz_state_t* s);
 int launch_compact(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* out);
+// the reference's bin_particles on its tosum rows and curr_xbins
+int launch_bin_lists(const LaunchEnv& env, const double* tosum, int64_t cols,
+                     const int64_t* curr_xbins, int64_t nc, int n_cells, int nb,
+                     const double* thr, int64_t* const* lists, int64_t* const* write_inds,
+                     const int64_t* max_ppc);
 // the 1-D collision counters accumulated since the last reset, to out[3][nn]
 int launch_collision_read(const LaunchEnv& env, double* out, int nn, int reset);
 // in place: the holes below the live count filled from the live slots above it

@@ -629,6 +629,100 @@ int launch_bin_sort(const LaunchEnv& env, const bz_state_t* s, const bz_params_t
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// The reference's bin_particles on its own arrays (1bin/kernels/
+// bin_particles.py:7-22, 3bin :7-37, 5bin :7-54): row j of tosum (w in
+// column 0, the energy in column 3) with w > 0 and 0 <= curr_xbins[j] <= N
+// goes to the list of its cell and energy bin, the bin being the number of
+// thresholds at or below the energy (the reference's chain of `<`
+// compares: a NaN energy matches none of them and is listed nowhere when
+// there are several bins).  Lists are filled in ascending row order (the
+// reference's atomic slots give one order of many) up to each bin's
+// max_ppc; write_inds get the full counts, as the reference's atomic
+// counters do.
+// ---------------------------------------------------------------------------
+struct RefBins {
+  double thr[8];
+  int64_t* list[8];
+  int64_t* inds[8];
+  int64_t max_ppc[8];
+};
+
+__global__ __launch_bounds__(256) void k_ref_keys(const double* __restrict__ tosum, int64_t cols,
+                                                  const int64_t* __restrict__ cx,
+                                                  const Clock* clk, int n_nodes, int nb,
+                                                  RefBins rb, uint32_t* keys) {
+  const int64_t n = clk->n;
+  const uint32_t dead = (uint32_t)(n_nodes * nb);
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * 256) {
+    const double w = tosum[j * cols];
+    const double e = tosum[j * cols + 3];
+    const int64_t cc = cx[j];
+    uint32_t key = dead;
+    if (w > 0.0 && cc >= 0 && cc < n_nodes && (nb == 1 || e == e)) {
+      int b = 0;
+      for (int q = 0; q < nb - 1; ++q) b += e >= rb.thr[q] ? 1 : 0;
+      key = (uint32_t)(cc * nb + b);
+    }
+    keys[j] = key;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ref_lists(const uint32_t* __restrict__ skeys,
+                                                   const uint32_t* __restrict__ perm,
+                                                   const int32_t* __restrict__ offsets,
+                                                   const int32_t* __restrict__ counts,
+                                                   const Clock* clk, int n_nodes, int nb,
+                                                   RefBins rb) {
+  const int64_t n = clk->n;
+  const int n_keys = n_nodes * nb;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n_keys; k += stride)
+    rb.inds[k % nb][k / nb] = counts[k];
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += stride) {
+    const uint32_t key = skeys[p];
+    if (key >= (uint32_t)n_keys) continue;
+    const int b = (int)(key % (uint32_t)nb), cell = (int)(key / (uint32_t)nb);
+    const int64_t j = p - offsets[key];
+    if (j < rb.max_ppc[b]) rb.list[b][j * n_nodes + cell] = (int64_t)perm[p];
+  }
+}
+
+__global__ void k_set_count(Clock* clk, int64_t n) { clk->n = n; }
+
+int launch_bin_lists(const LaunchEnv& env, const double* tosum, int64_t cols,
+                     const int64_t* curr_xbins, int64_t nc, int n_cells, int nb,
+                     const double* thr, int64_t* const* lists, int64_t* const* write_inds,
+                     const int64_t* max_ppc) {
+  const int n_nodes = n_cells + 1;
+  const int n_keys = n_nodes * nb;
+  if (n_keys > env.max_keys) return fail("bin lists: %d keys exceed the context's max_keys", n_keys);
+  if (nc > env.capacity) return fail("bin lists: %lld rows exceed the capacity", (long long)nc);
+  SortWs ws = carve(env.sort_ws, env.capacity, env.max_keys);
+  RefBins rb;
+  for (int q = 0; q < 8; ++q) {
+    rb.thr[q] = q < nb - 1 ? thr[q] : HUGE_VAL;
+    rb.list[q] = q < nb ? lists[q] : nullptr;
+    rb.inds[q] = q < nb ? write_inds[q] : nullptr;
+    rb.max_ppc[q] = q < nb ? max_ppc[q] : 0;
+  }
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, env.stream, env.clock, nc);
+  const int grid = env.num_cu * 4;
+  hipLaunchKernelGGL(k_ref_keys, dim3(grid), dim3(256), 0, env.stream, tosum, cols, curr_xbins,
+                     (const Clock*)env.clock, n_nodes, nb, rb, ws.keys_a);
+  int rc = check_launch("k_ref_keys");
+  if (rc) return rc;
+  const uint32_t* skeys = nullptr;
+  uint32_t* perm = (uint32_t*)ws.copies;
+  rc = sort_keys(env, ws, n_keys, ws.ip_counts, ws.ip_moff, perm, &skeys);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ref_lists, dim3(grid), dim3(256), 0, env.stream, skeys,
+                     (const uint32_t*)perm, (const int32_t*)ws.ip_moff,
+                     (const int32_t*)ws.ip_counts, (const Clock*)env.clock, n_nodes, nb, rb);
+  return check_launch("k_ref_lists");
+}
+
 // Batched swarms: key = instance (held in x), n_inst for a dead slot.
 template <typename T>
 __global__ __launch_bounds__(256) void k_inst_keys(const T* __restrict__ x,

@@ -69,6 +69,32 @@ def ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+def as_gpu(a, name: str = "array") -> torch.Tensor:
+    """A device array of the caller as a torch tensor over the same memory
+    (no copy): a torch tensor, any object exposing __cuda_array_interface__
+    (CuPy and numba device arrays), or one exposing DLPack."""
+    if isinstance(a, torch.Tensor):
+        t = a
+    elif hasattr(a, "__cuda_array_interface__"):
+        t = torch.as_tensor(a, device="cuda")
+    elif hasattr(a, "__dlpack__"):
+        t = torch.from_dlpack(a)
+    else:
+        raise TypeError(f"{name} must be a device array (torch tensor, __cuda_array_interface__ "
+                        f"or DLPack), got {type(a).__name__}")
+    if not t.is_cuda:
+        raise TypeError(f"{name} must live in device memory")
+    return t
+
+
+def field_array(a):
+    """A grid array of the caller: a device array (as_gpu) or host numpy."""
+    if isinstance(a, torch.Tensor) or hasattr(a, "__cuda_array_interface__") or \
+            hasattr(a, "__dlpack__") and not isinstance(a, np.ndarray):
+        return as_gpu(a)
+    return np.asarray(a)
+
+
 def as_device(a, dev, dtype=torch.float64) -> torch.Tensor:
     if isinstance(a, torch.Tensor):
         return a.to(device=dev, dtype=dtype)

@@ -1,20 +1,19 @@
 """bin_particles with the reference's 1/3/5-bin signatures
 (1bin/kernels/bin_particles.py:24-31, 3bin :71-78, 5bin :130-140).
 
-The reference appends particle indices to per-(bin, cell) lists with atomic
-slots.  Here the keys (cell from curr_xbins, energy bin from tosum[:,3]) are
-sorted by the stable HIP radix sort (bz_bin_sort), so every list holds its
-particles in ascending index order; write_inds receive the full counts and
-lists are filled up to max_ppc entries, as in the reference.
+The reference appends row indices to per-(bin, cell) lists with atomic
+slots, from tosum[:,3] (the energy post_processing left) and curr_xbins.
+Here libbz's bz_bin_lists reads the same arrays in place (torch tensors,
+__cuda_array_interface__ or DLPack objects, no copy): the keys come from
+tosum[:,3] with the reference's compares, a stable radix sort orders them,
+and a kernel fills every list in ascending row order up to its max_ppc and
+writes the full counts to write_inds, as the reference's atomics do.
 """
 from __future__ import annotations
 
 import ctypes as C
 
-import torch
-
 from .. import _lib
-from .. import physics as ph
 from . import _session as S
 
 _THRESHOLDS = {1: (), 3: (10.0, 15.76), 5: (5.0, 10.0, 15.76, 20.0)}
@@ -24,47 +23,28 @@ def _bin(d_tosum_ar, lists, inds, d_curr_xbins, Nc, N, maxes):
     """lists / inds / maxes ordered from the lowest energy bin upwards."""
     nb = len(lists)
     Nc, N = int(Nc), int(N)
-    cap = max(int(d_tosum_ar.shape[0]), 1)
-    dev = d_tosum_ar.device.index or 0
+    tosum = S.as_gpu(d_tosum_ar, "d_tosum_ar")
+    cx = S.as_gpu(d_curr_xbins, "d_curr_xbins")
+    lists = [S.as_gpu(a, "cell list") for a in lists]
+    inds = [S.as_gpu(a, "write_inds") for a in inds]
+    import torch
+    if tosum.dtype != torch.float64 or tosum.dim() != 2 or not tosum.is_contiguous():
+        raise TypeError("d_tosum_ar must be a contiguous float64 (Nmax, 4) device array")
+    if cx.dtype != torch.int64 or any(a.dtype != torch.int64 for a in lists + inds):
+        raise TypeError("curr_xbins, the cell lists and write_inds must be int64")
+    for a in lists:
+        if a.dim() != 2 or a.shape[1] != N + 1 or not a.is_contiguous():
+            raise TypeError("cell lists must be contiguous (max_ppc, N+1) arrays")
+    cap = max(int(tosum.shape[0]), 1)
+    dev = tosum.device.index or 0
     s = S.session(dev, _lib.BZ_F64, cap, N + 1, (N + 1) * nb)
-    a = s.soa[0]
-    kev = 0.5 * ph.M_E * ph.J_EV
-    # a state whose cell is exactly curr_xbins (dx = 1) and whose energy is tosum[:,3]
-    a[0][:Nc].copy_(d_curr_xbins[:Nc].to(torch.float64))
-    a[1][:Nc].copy_(torch.sqrt(torch.clamp(d_tosum_ar[:Nc, 3], min=0.0) / kev))
-    a[2][:Nc].zero_()
-    a[3][:Nc].zero_()
-    a[4][:Nc].copy_(d_tosum_ar[:Nc, 0])
-    p = _lib.Params()
-    p.dx = 1.0
-    p.L = float(N + 1)
-    p.kev = kev
-    p.dim = 1
-    p.xs_mode = _lib.BZ_XS_TABLE
-    p.n_cells = N
-    p.n_nodes = N + 1
-    p.bnf = 1
-    s.clock_set(Nc)
-    n_keys = (N + 1) * nb
-    counts = torch.zeros(n_keys + 1, dtype=torch.int32, device=s.dev)
-    offsets = torch.zeros(n_keys + 2, dtype=torch.int32, device=s.dev)
-    perm = torch.zeros(max(Nc, 1), dtype=torch.int32, device=s.dev)
     thr = (C.c_double * 8)(*_THRESHOLDS[nb])
-    _lib.check(s.lib.bz_bin_sort(s.ctx, C.byref(s.structs[0]), C.byref(p), thr, nb,
-                                 S.ptr(counts), S.ptr(offsets), S.ptr(perm)), "bz_bin_sort")
-    cnt = counts[:n_keys].view(N + 1, nb)
-    off = offsets[:n_keys].view(N + 1, nb)
-    for b in range(nb):
-        inds[b][:N + 1] = cnt[:, b].to(inds[b].dtype)
-        cap_b = min(int(maxes[b]), int(lists[b].shape[0]))
-        if cap_b <= 0:
-            continue
-        j = torch.arange(cap_b, device=s.dev).view(-1, 1)
-        valid = j < cnt[:, b].view(1, -1).to(torch.int64)
-        src = (off[:, b].view(1, -1).to(torch.int64) + j).clamp(max=max(Nc - 1, 0))
-        vals = perm.to(torch.int64)[src]
-        lists[b][:cap_b, :N + 1] = torch.where(valid, vals, lists[b][:cap_b, :N + 1].to(
-            torch.int64)).to(lists[b].dtype)
+    lp = (C.c_void_p * nb)(*[a.data_ptr() for a in lists])
+    ip = (C.c_void_p * nb)(*[a.data_ptr() for a in inds])
+    mp = (C.c_int64 * nb)(*[min(int(m), int(a.shape[0])) for m, a in zip(maxes, lists)])
+    _lib.check(s.lib.bz_bin_lists(s.ctx, C.c_void_p(tosum.data_ptr()), tosum.shape[1],
+                                  C.c_void_p(cx.data_ptr()), Nc, N, nb, thr, lp, ip, mp),
+               "bz_bin_lists")
 
 
 def bin_particles(*args):

@@ -2,13 +2,17 @@
 
 Returns (ne_ar, ni_src, ns_src, nrg_ar, counter_g0..g3, Nnew).  After
 electron_kernel the node moments were already summed by the fused HIP
-kernel, so they are returned as computed there and the rows are compacted
-(w > 0, stable: survivors, then secondaries) as reshuffle_data_atomics does
-(post_processing_1D.py:26-38); for a state that was not pushed (the set-up
-deposit, boltzmann.py:363-371) the deposit runs in bz_deposit.
-The collision-counter reduction (post_processing_1D.py:56) accumulates an
-array the reference never resets or writes out; its counters come back as
-zeros.
+kernel and are returned as computed there; the rows are compacted by libbz
+(bz_compact: w > 0, stable: survivors, then secondaries) as
+reshuffle_data_atomics does (post_processing_1D.py:26-38) and written back
+into the caller's arrays.  For rows that were not pushed (the set-up
+deposit, boltzmann.py:363-371) the compaction and the deposit (bz_deposit)
+run on a copy of the rows.  The collision counters (post_processing_1D.py:56)
+are the ones the push deposited (counter_g0: sum w of elastic collisions,
+g1: of excitations, g2: number of ionizations, g3: 0); the reference's
+collct rows are never reset (quirk Q10), these cover the last push.
+The reference's arrays may be torch tensors or expose
+__cuda_array_interface__ / DLPack; they are used in place.
 """
 from __future__ import annotations
 
@@ -19,77 +23,93 @@ import torch
 
 from .. import _lib
 from . import _session as S
-from .electron_kernel import _deposit_bins, _params
+from .electron_kernel import _deposit_bins, _params, _rows_from_state
 
 
 def post_processing(data_out, data_out_np, data_out2, data_out_np2, data_out3, data_out_np3,
                     big_data_ar, big_tosum_ar, big_collct_ar, need_to_reshuffle, Na, Nc, nni,
                     ne_ar, ni_src, ns_src, nrg_ar, counter_g0_ar, counter_g1_ar, counter_g2_ar,
                     counter_g3_ar, big_N, RN, L, dx, temp_x):
+    big_data_ar = S.as_gpu(big_data_ar, "big_data_ar")
+    big_tosum_ar = S.as_gpu(big_tosum_ar, "big_tosum_ar")
     Nc, nni, RN = int(Nc), int(nni), int(RN)
     cap = int(big_data_ar.shape[0])
     dev = big_data_ar.device.index or 0
     N = RN - 1
     s = S.session(dev, _lib.BZ_F64, cap, RN)
+    kev = 0.5 * 9.10938 * 10 ** (-31) * 6.241509 * (10 ** 18)
     key = (big_data_ar.data_ptr(), Nc + nni)
-    if key in s.cached:
-        mom, m = s.cached.pop(key)
-        alive = torch.nonzero(big_tosum_ar[:m, 0] > 0).flatten()
-        nnew = int(alive.numel())
-        if need_to_reshuffle and nnew < m:
-            big_data_ar[:nnew] = big_data_ar[alive]
-            big_tosum_ar[:nnew] = big_tosum_ar[alive]
-            big_data_ar[nnew:m].zero_()
-            big_tosum_ar[nnew:m].zero_()
+    pushed = key in s.cached
+    if pushed:
+        mom, m, cur = s.cached.pop(key)
+        counts = torch.empty(3 * RN, dtype=torch.float64, device=s.dev)
+        _lib.check(s.lib.bz_collision_counts(s.ctx, S.ptr(counts), RN, 1),
+                   "bz_collision_counts")
+        coll = counts.view(3, RN).cpu().numpy()
     else:
         bnf = int(big_N) // RN
         if bnf != _deposit_bins["bnf"]:
             _deposit_bins["bnf"] = bnf
         m = Nc + nni
-        alive = torch.nonzero(big_tosum_ar[:m, 0] > 0).flatten() if need_to_reshuffle else \
-            torch.arange(m, device=big_data_ar.device)
-        nnew = int(alive.numel())
-        if need_to_reshuffle:
-            big_data_ar[:nnew] = big_data_ar[alive]
-            big_tosum_ar[:nnew] = big_tosum_ar[alive]
-            if m > nnew:
-                big_data_ar[nnew:m].zero_()
-                big_tosum_ar[nnew:m].zero_()
+        cur = 0
         a = s.soa[0]
-        a[0][:nnew].copy_(big_data_ar[:nnew, 0])
+        a[0][:m].copy_(big_data_ar[:m, 0])
         for k, col in ((1, 3), (2, 4), (3, 5)):
-            a[k][:nnew].copy_(big_data_ar[:nnew, col])
-        a[4][:nnew].copy_(big_tosum_ar[:nnew, 0])
+            a[k][:m].copy_(big_data_ar[:m, col])
+        a[4][:m].copy_(big_tosum_ar[:m, 0])
+        coll = np.zeros((3, RN))
+    nnew = m
+    if need_to_reshuffle:
+        s.clock_set(m)
+        _lib.check(s.lib.bz_compact(s.ctx, C.byref(s.structs[cur]), C.byref(s.structs[1 - cur])),
+                   "bz_compact")
+        cur = 1 - cur
+        nnew = s.count()
+    if not pushed:
         p = _params(N, 0.0, 1e-12, s.capacity, 0)
         p.L = float(L)
         p.dx = float(dx)
         p.half_dx = 0.5 * p.dx
         p.L_dx = p.L + p.dx
         p.dep_M = float(big_N)
-        p.bnf = bnf
+        p.bnf = int(big_N) // RN
         s.clock_set(nnew)
-        _lib.check(s.lib.bz_deposit(s.ctx, C.byref(s.structs[0]), C.byref(p), S.ptr(s.moments)),
-                   "bz_deposit")
+        _lib.check(s.lib.bz_deposit(s.ctx, C.byref(s.structs[cur]), C.byref(p),
+                                    S.ptr(s.moments)), "bz_deposit")
         mom = s.moments.clone()
-        v2 = big_data_ar[:nnew, 3] ** 2 + big_data_ar[:nnew, 4] ** 2 + big_data_ar[:nnew, 5] ** 2
-        big_tosum_ar[:nnew, 3] = (0.5 * 9.10938 * 10 ** (-31) * 6.241509 * (10 ** 18)) * v2
+    # the compacted rows back into the caller's arrays, the vacated rows zeroed
+    if need_to_reshuffle or not pushed:
+        _rows_from_state(s, cur, nnew, big_data_ar, big_tosum_ar, kev)
+        if m > nnew:
+            big_data_ar[nnew:m].zero_()
+            big_tosum_ar[nnew:m].zero_()
     mom = mom[:3 * RN].view(3, RN)
     if data_out is not None:
+        data_out = S.as_gpu(data_out, "data_out")
         data_out.zero_()
         data_out[:RN, 0] = mom[0]
         data_out[:RN, 1] = mom[1]
         data_out[:RN, 3] = mom[2]
+    if data_out2 is not None:
+        data_out2 = S.as_gpu(data_out2, "data_out2")
+        data_out2.zero_()
+        data_out2[:RN, :3] = torch.from_numpy(coll.T.copy()).to(data_out2.device)
     host = mom.cpu().numpy()
     if data_out_np is not None:
         data_out_np[:RN, 0] = host[0]
         data_out_np[:RN, 1] = host[1]
         data_out_np[:RN, 2] = 0.0
         data_out_np[:RN, 3] = host[2]
+    if data_out_np2 is not None:
+        data_out_np2[:RN, :3] = coll.T
+        data_out_np2[:RN, 3] = 0.0
     ne_ar[0:RN] = host[0]
     ni_src = np.array(host[1])
     ns_src = np.zeros(RN)
     nrg_ar[0:RN] = host[2]
-    for c in (counter_g0_ar, counter_g1_ar, counter_g2_ar, counter_g3_ar):
-        c[0:RN] = 0.0
+    counter_g0_ar[0:RN] = coll[0]
+    counter_g1_ar[0:RN] = coll[1]
+    counter_g2_ar[0:RN] = coll[2]
+    counter_g3_ar[0:RN] = 0.0
     return (ne_ar, ni_src, ns_src, nrg_ar, counter_g0_ar, counter_g1_ar, counter_g2_ar,
             counter_g3_ar, int(nnew))

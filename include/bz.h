@@ -290,6 +290,24 @@ int bz_compact(bz_ctx_t ctx, const bz_state_t* in, const bz_state_t* out);
  * the compacted particles open; this moves only the particles above L. */
 int bz_compact_inplace(bz_ctx_t ctx, const bz_state_t* s);
 
+/* bin_particles on the reference's own arrays (1bin/kernels/
+ * bin_particles.py:7-31, 3bin :7-78, 5bin :7-140): tosum rows of
+ * tosum_cols doubles (w in column 0, the energy in eV in column 3, as
+ * post_processing leaves it), curr_xbins int64 cells, rows 0..nc-1.  A row
+ * with w > 0 and 0 <= cell <= n_cells is listed under its cell and energy
+ * bin, the number of thresholds_host[0..nb-2] at or below its energy (a NaN
+ * energy is listed nowhere when nb > 1, as the reference's `<` chain).
+ * lists[b] is the bin's int64 (max_ppc[b], n_cells+1) array, filled with
+ * row indices in ascending order up to max_ppc[b] per cell; write_inds[b]
+ * (int64, n_cells+1) receives the full counts.  Bins run from the lowest
+ * energy up (the reference names them LE, ME..., HE).  lists, write_inds
+ * and max_ppc are host arrays of nb entries; the arrays they point to are
+ * device memory.  Resets the context's slot count to nc. */
+int bz_bin_lists(bz_ctx_t ctx, const double* tosum, int64_t tosum_cols,
+                 const int64_t* curr_xbins, int64_t nc, int32_t n_cells, int32_t nb,
+                 const double* thresholds_host, int64_t* const* lists,
+                 int64_t* const* write_inds, const int64_t* max_ppc);
+
 /* Collision counters of the 1-D push, the reference's collct deposit
  * (kernels/electron_kernel_numba.py:199,232 set collct[i,0] = w for an
  * elastic collision and collct[i,2] = 1 for an ionization;

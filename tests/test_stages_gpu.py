@@ -38,9 +38,25 @@ def test_reduction_brute_force(gpu):
     np.testing.assert_array_equal(dw2.cpu().numpy(), got)
 
 
-def test_electron_kernel_and_post_processing(gpu):
+class _CAI:
+    """A minimal device array of another library: only
+    __cuda_array_interface__ (version 3) over a torch tensor's memory."""
+
+    def __init__(self, t):
+        self._t = t  # keeps the memory alive
+        typestr = {"torch.float64": "<f8", "torch.int64": "<i8", "torch.float32": "<f4"}
+        self.__cuda_array_interface__ = {
+            "shape": tuple(t.shape), "typestr": typestr[str(t.dtype)],
+            "data": (t.data_ptr(), False), "version": 3, "strides": None}
+
+
+@pytest.mark.parametrize("wrap", [False, True], ids=["torch", "cuda_array_interface"])
+def test_electron_kernel_and_post_processing(gpu, wrap):
     """electron_kernel (reference fits, FP64) + post_processing against the
-    oracle fed the same uniforms and gas velocities."""
+    oracle fed the same uniforms and gas velocities, on torch tensors and on
+    foreign arrays that only expose __cuda_array_interface__ (used in place:
+    the results appear in the memory they describe); the collision counters
+    equal the oracle's."""
     import torch
     from boltzmann_amd import kernels as K
     rng = np.random.default_rng(1)
@@ -64,23 +80,29 @@ def test_electron_kernel_and_post_processing(gpu):
     nn_gas = 1e6 * 3.22e16 * 0.1
     dt = (1 / 13.56e6) / 1e5
     K.set_deposit_bins(4)
-    K.electron_kernel(n, d_data, d_tosum, None, torch.from_numpy(E).to(dev), d_R, d_R, None,
-                      None, nn_gas, 1, dt, cnt, None, None, torch.from_numpy(gas).to(dev), 1024,
-                      64)
+    w_ = _CAI if wrap else (lambda t: t)
+    K.electron_kernel(n, w_(d_data), w_(d_tosum), None, w_(torch.from_numpy(E).to(dev)),
+                      w_(d_R), w_(d_R), None, None, nn_gas, 1, dt, w_(cnt), None, None,
+                      w_(torch.from_numpy(gas).to(dev)), 1024, 64)
     nni = int(cnt[0])
     RN = N + 1
     ne, nrg = np.zeros(RN), np.zeros(RN)
     cg = [np.zeros(RN) for _ in range(4)]
-    res = K.post_processing(None, None, None, None, None, None, d_data, d_tosum, None, True, n,
-                            n, nni, ne, np.zeros(RN), np.zeros(RN), nrg, *cg, 4 * RN, RN, 2.54,
-                            2.54 / N, None)
+    res = K.post_processing(None, None, None, None, None, None, w_(d_data), w_(d_tosum), None,
+                            True, n, n, nni, ne, np.zeros(RN), np.zeros(RN), nrg, *cg, 4 * RN, RN,
+                            2.54, 2.54 / N, None)
     nnew = res[-1]
     R = d_R[:n].cpu().numpy().T.copy()
     G = gas[:n].T.copy()
     s = O.Setup(dim=1, dt=dt, xs_mode=0, n_cells=N, length_cm=2.54, bnf=4, gas_density_m3=nn_gas)
     st = (data[:n, 0], data[:n, 3], data[:n, 4], data[:n, 5], tosum[:n, 0])
-    want, wmom, wcnt = O.push_step(s, st, E, R, G=G, capacity=cap)
+    want, wmom, wcnt, wcoll = O.push_step(s, st, E, R, G=G, capacity=cap, collisions=True)
     assert nni == int(wcnt[0]) and nnew == want[0].size
+    # collision counters: elastic sum w, (no excitation in the fits), ionizations
+    np.testing.assert_allclose(res[4], wcoll[0], rtol=1e-12, atol=1e-12 * wcoll[0].max())
+    assert not res[5].any() and not wcoll[1].any()
+    np.testing.assert_array_equal(res[6], wcoll[2])
+    assert wcoll[0].sum() > 0  # (ionizations are rare at this step: 0 to a few)
     got = d_data[:nnew].cpu().numpy()
     gw = d_tosum[:nnew].cpu().numpy()
     vs = np.sqrt(want[1] ** 2 + want[2] ** 2 + want[3] ** 2) + 1
@@ -91,9 +113,11 @@ def test_electron_kernel_and_post_processing(gpu):
     np.testing.assert_allclose(res[0], wmom[0], rtol=1e-12, atol=1e-12 * wmom[0].max())
     np.testing.assert_allclose(res[1], wmom[1], rtol=1e-12, atol=1e-12 * wmom[0].max())
     np.testing.assert_allclose(res[3], wmom[2], rtol=1e-12, atol=1e-12 * wmom[2].max())
-    # uniforms handed back are the Philox stream
-    np.testing.assert_array_equal(R[[0, 1, 2, 3, 5]],
-                                  O.collision_uniforms(12345, 0, n, np.float64)[[0, 1, 2, 3, 5]])
+    # uniforms handed back are the Philox stream of the stage's step (the
+    # stage session counts its calls)
+    cols = [0, 1, 2, 3, 5]
+    assert any(np.array_equal(R[cols], O.collision_uniforms(12345, k, n, np.float64)[cols])
+               for k in range(4))
 
 
 def test_bin_particles_3bin(gpu):
@@ -125,6 +149,54 @@ def test_bin_particles_3bin(gpu):
             m = min(ids.size, maxp[q])
             np.testing.assert_array_equal(lists[q][:m, c].cpu().numpy(), ids[:m])
     _ = ph
+
+
+def test_bin_particles_5bin_at_the_thresholds(gpu):
+    """Energies exactly at each 5-bin threshold and one ulp on either side
+    (the reference's `<` compares on tosum[:,3]: at a threshold the particle
+    goes up a bin), a NaN energy (listed nowhere), cells -1 and N+1 (listed
+    nowhere), dead rows, a list full before its cell is (write_inds keeps
+    the full count); the arrays handed over through
+    __cuda_array_interface__ only."""
+    import torch
+    from boltzmann_amd import kernels as K
+    N = 6
+    dev = torch.device("cuda", 0)
+    thr = [5.0, 10.0, 15.76, 20.0]
+    es = []
+    for t in thr:
+        es += [np.nextafter(t, 0.0), t, np.nextafter(t, 100.0)]
+    es += [0.0, 1e-300, 3.0, 50.0, np.nan, 7.5, 7.5, 7.5]
+    n = len(es)
+    tosum = np.zeros((n + 8, 4))
+    tosum[:n, 0] = 1.0
+    tosum[:n, 3] = es
+    tosum[3, 0] = 0.0  # dead
+    cells = np.zeros(n + 8, dtype=np.int64)
+    cells[:n] = np.arange(n) % (N + 1)
+    cells[5] = -1
+    cells[6] = N + 1
+    cells[n - 3:n] = 2  # with row 2 (just above 5 eV): four ME1 rows in cell 2
+    maxp = [3, 1, 3, 3, 3]  # HE, ME1 (full before its cell is), ME2, ME3, LE
+    lists = [torch.full((m, N + 1), -7, dtype=torch.int64, device=dev) for m in maxp]
+    inds = [torch.zeros(N + 1, dtype=torch.int64, device=dev) for _ in maxp]
+    C = _CAI
+    K.bin_particles(C(torch.from_numpy(tosum).to(dev)), *[C(a) for a in lists],
+                    *[C(a) for a in inds], C(torch.from_numpy(cells).to(dev)), n, N, *maxp, 64)
+    e = tosum[:n, 3]
+    with np.errstate(invalid="ignore"):
+        # index into (HE, ME1, ME2, ME3, LE): LE < 5 <= ME1 < 10 <= ME2 < 15.76 <= ME3 < 20 <= HE
+        q = np.select([e < 5.0, e < 10.0, e < 15.76, e < 20.0, e >= 20.0], [4, 1, 2, 3, 0], -1)
+    ok = (tosum[:n, 0] > 0) & (cells[:n] >= 0) & (cells[:n] <= N)
+    for b in range(5):
+        for c in range(N + 1):
+            ids = np.flatnonzero(ok & (q == b) & (cells[:n] == c))
+            assert int(inds[b][c]) == ids.size, (b, c)
+            m = min(ids.size, maxp[b])
+            got = lists[b][:, c].cpu().numpy()
+            np.testing.assert_array_equal(got[:m], ids[:m])
+            assert np.all(got[m:] == -7)  # untouched past the count
+    assert int(inds[1][2]) == 4 and int(lists[1][0, 2]) == 2
 
 
 def test_ef_and_heavies_kernels(gpu):
