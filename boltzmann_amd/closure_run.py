@@ -78,6 +78,8 @@ def swarm_training_set(n_samples: int = 1_000_000, per_instance: int = 4000,
             T.append(targets(k[:, 2], k[:, 1], r["mobility"] * N))
             if control_every and (w + 1) % control_every == 0:
                 b.control()
+            if log and (w + 1) % 200 == 0:
+                log(f"{te} eV: window {w + 1} of {per_temp}")
                 b.control()
         if log:
             log(f"swarm samples at {te} eV: {per_temp * b.n_inst}")
@@ -167,13 +169,15 @@ class ClosureDischarge:
         self.eng.load_state(state)
         self.eng.initialize_fields()
 
-    def macro_step(self):
+    def kinetic_update(self):
+        """The kinetic step of one macro step: the resampling check, then
+        n_kin pushes over n_sub * dt (in place, births appended) with the
+        field re-solved on the kinetic density between them; moments row 1
+        holds the ionization of the whole interval."""
         e = self.eng
         k = self.resample_every
         if k and self.cfg.target_ppc > 0 and self.macro % k == 0:
             e.resample()
-        # kinetic update over n_sub * dt: n_kin pushes (in place, births
-        # appended), the field re-solved on the kinetic density between them
         nn = self.cfg.n_nodes
         for kk in range(self.n_kin):
             if kk:
@@ -197,19 +201,32 @@ class ClosureDischarge:
             # the update's ionization source: every push of it
             e.moments[nn:2 * nn].copy_(self.ksrc)
 
+    def sub_step(self):
+        """One fluid / Poisson sub-step of dt driven by the learned
+        ionization source (self.src; self.acc accumulates it)."""
+        e = self.eng
         net = self.net.struct()
-        for _ in range(self.n_sub):
-            _lib.check(e.lib.bz_closure_source(e.ctx, C.byref(net), self.cfg.n_cells, _p(e.E),
-                                               _p(e.moments), self.cfg.gas_density_m3,
-                                               self.cfg.dt, _p(self.src), _p(self.acc)),
-                       "bz_closure_source")
-            _lib.check(e.lib.bz_field_substep(e.ctx, C.byref(e.fp), _p(e.moments), _p(self.src),
-                                              _p(e.ni), _p(e.ns), _p(e.Te), _p(e.g2), _p(e.V),
-                                              _p(e.E), _p(e.E_t), _p(e.chol)), "bz_field_substep")
+        _lib.check(e.lib.bz_closure_source(e.ctx, C.byref(net), self.cfg.n_cells, _p(e.E),
+                                           _p(e.moments), self.cfg.gas_density_m3, self.cfg.dt,
+                                           _p(self.src), _p(self.acc)), "bz_closure_source")
+        _lib.check(e.lib.bz_field_substep(e.ctx, C.byref(e.fp), _p(e.moments), _p(self.src),
+                                          _p(e.ni), _p(e.ns), _p(e.Te), _p(e.g2), _p(e.V),
+                                          _p(e.E), _p(e.E_t), _p(e.chol)), "bz_field_substep")
+
+    def commit(self):
+        """The kinetic ionization of the interval replaces the learned one;
+        the difference is tallied (the closure's rate error)."""
+        e = self.eng
         _lib.check(e.lib.bz_closure_commit(e.ctx, self.cfg.n_nodes, _p(e.moments), _p(self.acc),
                                            _p(e.ni), _p(e.g2), _p(e.Te), _p(self.tally)),
                    "bz_closure_commit")
         self.macro += 1
+
+    def macro_step(self):
+        self.kinetic_update()
+        for _ in range(self.n_sub):
+            self.sub_step()
+        self.commit()
 
     def report(self) -> dict:
         j = self.tally.cpu().numpy()

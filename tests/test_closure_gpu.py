@@ -148,3 +148,67 @@ def test_trained_closure_beats_untrained(gpu):
     assert b["k_ion_rel_err_median"] < 0.25 * a["k_ion_rel_err_median"], rep
     assert b["k_ion_rel_err_median"] < 0.2, rep
     net.close()
+
+
+def test_closure_discharge_two_macro_steps(gpu):
+    """The coupled loop of ClosureDischarge for two macro steps (a kinetic
+    update, then 8 sub-steps, then the commit), each stage checked on the
+    device's own input: every sub-step's learned source against the FP64
+    oracle network on that sub-step's field and kinetic moments, the
+    sub-step's ions and field against the FP64 field oracle, and the commit
+    against ni + (kinetic - learned) ionization, with the tally counting the
+    updates."""
+    import torch
+    from boltzmann_amd import _lib, configs
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.closure_run import ClosureDischarge
+    net = _net()
+    params = {"master": net.parameters(), "norm": dict(net.norm)}
+    cfg = configs.c3().with_(n_particles=100_000, n_cells=64, xs_mode=_lib.BZ_XS_FLIGHT,
+                             target_ppc=0)
+    run = ClosureDischarge(cfg, params, n_sub=8)
+    run.load(ph.initial_state(cfg))
+    e = run.eng
+    N, nn = cfg.n_cells, cfg.n_nodes
+    fp = e.fp
+    chol = e.chol.cpu().numpy()
+    for macro in range(2):
+        run.kinetic_update()
+        mom_kin = e.moments.cpu().numpy().copy()
+        ni_start = e.ni.cpu().numpy()
+        acc_host = np.zeros(nn)
+        for sub in range(run.n_sub):
+            E0 = e.E.cpu().numpy()
+            mom0 = e.moments.cpu().numpy()
+            ni0, ns0 = e.ni.cpu().numpy(), e.ns.cpu().numpy()
+            g20, V0 = e.g2.cpu().numpy(), e.V.cpu().numpy()
+            t0 = e.clock()[2]
+            run.sub_step()
+            src = run.src.cpu().numpy()
+            want = _want_src(run.net, E0, mom0, N, cfg.gas_density_m3, cfg.dt)
+            np.testing.assert_allclose(src, want, rtol=2e-4 * 2.0 * math.log(10), atol=0,
+                                       err_msg=f"source {macro}.{sub}")
+            acc_host += src
+            m2 = np.concatenate([mom0[:nn] + src, src, mom0[2 * nn:]])
+            w_ = O.field_step(N, fp.dx, fp.dt, fp.eps, fp.v0, fp.freq, fp.mu_i, fp.D_i, fp.D_s,
+                              fp.fluid_on, fp.poisson_on, t0 + cfg.dt, m2, ni0, ns0,
+                              np.zeros(nn), g20, V0, E0, chol, ion_scheme=fp.ion_scheme,
+                              max_substeps=fp.max_substeps)
+            for nm, w in zip(("ni", "V", "E"), (w_[0], w_[4], w_[5])):
+                g = getattr(e, nm).cpu().numpy()
+                np.testing.assert_allclose(g, w, rtol=1e-10, atol=1e-10 * np.abs(w).max(),
+                                           err_msg=f"{nm} {macro}.{sub}")
+            assert e.clock()[2] == pytest.approx(t0 + cfg.dt, rel=1e-14)
+        np.testing.assert_allclose(run.acc.cpu().numpy(), acc_host, rtol=1e-12, atol=0)
+        ni_pre = e.ni.cpu().numpy()
+        run.commit()
+        nsrc = mom_kin[nn:2 * nn]
+        np.testing.assert_allclose(e.ni.cpu().numpy(), ni_pre + nsrc - acc_host, rtol=1e-12,
+                                   atol=1e-12 * np.abs(ni_pre).max())
+        assert not run.acc.cpu().numpy().any()
+        assert run.tally.cpu().numpy()[3] == macro + 1
+        assert np.all(np.isfinite(ni_start))
+    rep = run.report()
+    assert rep["kinetic_updates"] == 2 and rep["ionization_kinetic"] >= 0.0
+    run.close()
+    net.close()

@@ -119,12 +119,21 @@ def test_resample_inplace_parity(gpu, name, prec, nb, drift):
         assert abs(W1 - W0) <= 1e-12 * W0
         assert abs(E1 - E0) <= 1e-12 * E0
         assert np.all(np.abs(P1 - P0) <= 1e-12 * math.sqrt(W0 * E0))
-    # live particles of the compacted state = the reference's compaction
+    else:
+        # FP32 states: each merged particle stores its FP64 weight and
+        # velocity rounded to FP32, so a merged segment keeps W, P and E to
+        # the FP32 rounding, 2^-24 relative at most (the FP64 sums of FP32
+        # values cannot in general equal the FP64 totals: 1e-12 is out of
+        # reach for FP32 storage); the whole state by the merged fraction
+        eps = 2.0 ** -24
+        assert abs(W1 - W0) <= eps * W0
+        assert abs(E1 - E0) <= 2 * eps * E0
+        assert np.all(np.abs(P1 - P0) <= 2 * eps * math.sqrt(W0 * E0))
+    # the compaction after the pass: the oracle's hole filling
     eng.compact()
     live = eng.fetch_slots()
-    keep = want[4] > 0
-    for k in range(5):
-        np.testing.assert_array_equal(live[k], got[k][keep])
+    for a, b in zip(live, O.fill_holes(got)):
+        np.testing.assert_array_equal(a, b)
     assert got[0].size >= n
     eng.close()
 
