@@ -3,9 +3,11 @@
 Each candidate shape (block size x particles per thread) is run for a few
 real steps of the workload and timed with HIP events; the fastest is kept on
 the context.  For the 1-D push the grid is tuned next, blocks per CU from 1
-up to what fits, timing whole steps (the grid also sets how many partial
-histograms the deposit reduction sums).  Steps taken here advance the
-simulation like warm-up steps.
+up to what fits, timing the push kernel: whole steps would mix in the
+resampling, whose cost drifts from trial to trial as the run evolves (a
+resampling discharge merges more as it goes), and the grid only changes the
+push and its short reduction.  Steps taken here advance the simulation like
+warm-up steps.
 """
 from __future__ import annotations
 
@@ -25,6 +27,13 @@ def pick_within(results: dict, order, tie: float = None):
     fastest = min(results.values())
     return min((k for k in results if results[k] <= fastest * (1.0 + tie)),
                key=lambda k: (order(k), results[k]))
+
+
+def grid_order(bpc: int) -> int:
+    """Preference among grid settings timed within a tie: the default (0,
+    as many blocks as fit: one per CU where the push's LDS allows one), then
+    the fewest blocks per CU."""
+    return -1 if bpc == 0 else bpc
 
 
 def push_time_ms(lib, ctx) -> tuple[float, int]:
@@ -87,10 +96,9 @@ def tune_shapes(lib, ctx, step: Callable[[], None], steps: int = 10,
         gres = {}
         for bpc in GRID_BPC:
             _lib.check(lib.bz_set_push_grid(ctx, bpc))
-            gres[bpc] = _time(lib, ctx, step, steps, whole=True)
+            gres[bpc] = _time(lib, ctx, step, steps)
             used += steps + 1
-        # within TIE, the fewest blocks (0, as many as fit, counts as the most)
-        gbest = pick_within(gres, lambda k: k if k > 0 else 99)
+        gbest = pick_within(gres, grid_order, TIE_SHORT)
         _lib.check(lib.bz_set_push_grid(ctx, gbest))
         out["blocks_per_cu"] = gbest
         out["grid_avg_ms"] = {str(k): v for k, v in gres.items()}

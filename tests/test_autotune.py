@@ -1,5 +1,5 @@
 """Autotuner choice rule (host logic, no GPU)."""
-from boltzmann_amd.autotune import PUSH_SHAPES, pick_within
+from boltzmann_amd.autotune import PUSH_SHAPES, TIE_SHORT, grid_order, pick_within
 
 
 def _tile(k):
@@ -22,3 +22,11 @@ def test_grid_prefers_fewest_blocks_within_tie():
     assert pick_within(g, lambda k: k if k > 0 else 99) == 1
     g = {0: 0.030, 1: 0.041, 2: 0.040, 4: 0.039}
     assert pick_within(g, lambda k: k if k > 0 else 99) == 0
+
+
+def test_grid_keeps_the_default_within_two_percent():
+    # push-kernel times of a C4 trial: 4 blocks per CU 1 % faster is noise
+    g = {0: 0.936, 1: 0.993, 2: 0.966, 4: 0.927}
+    assert pick_within(g, grid_order, TIE_SHORT) == 0
+    g = {0: 0.936, 1: 0.993, 2: 0.966, 4: 0.900}
+    assert pick_within(g, grid_order, TIE_SHORT) == 4
