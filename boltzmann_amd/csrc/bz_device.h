@@ -263,6 +263,16 @@ struct BinKeys {
   int ew;      // 1: bin the energy after (e*w)/w (BZ_REF_ENERGY_ROUNDTRIP)
 };
 
+// A wave-uniform FP64 constant held in a VGPR pair.  A kernel whose uniform
+// arguments outgrow the scalar register file has them spilled to VGPR lanes
+// and read back (v_readlane plus hazard wait states) inside its loops; the
+// constants the per-particle path reads most are kept in VGPRs instead.
+__device__ __forceinline__ double in_vgpr(double v) {
+  double r;
+  asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "s"(v));
+  return r;
+}
+
 // int(x / dx) for x >= 0, equal to the FP64 quotient's truncation, without
 // the divide: the product with the reciprocal is within a few ulps of the
 // quotient, so away from a cell edge (more than 1e-12 relative) it truncates

@@ -607,7 +607,19 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   __shared__ T s_thr[8];  // the binning's energy thresholds (keys written here)
 
   constexpr bool kQueue = QUEUE && QueuedPush<XS, DIM, RNG>::value;
-  const bz_params_t& p = a.p;
+  // the FP64 constants of every particle's kick, walls, deposit node and
+  // key in VGPRs (in_vgpr); the rest of the parameters stay uniform
+  bz_params_t p = a.p;
+  if (DIM == 1) {
+    p.dx = in_vgpr(p.dx);
+    p.half_dx = in_vgpr(p.half_dx);
+    p.L_dx = in_vgpr(p.L_dx);
+    p.dep_M = in_vgpr(p.dep_M);
+    p.L = in_vgpr(p.L);
+    p.afac = in_vgpr(p.afac);
+    p.xfac = in_vgpr(p.xfac);
+    p.kev = in_vgpr(p.kev);
+  }
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wid = tid / kWave;
