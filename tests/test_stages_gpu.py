@@ -288,3 +288,93 @@ def test_boltzmann_restart_and_quirk_fixes(gpu, tmp_path):
     b.run(spc, name, 2, verbose=False)
     assert np.fromfile(name + "ne_cycle3.dat").size == 65
     assert b.Nc_list[0] > 0
+
+
+def test_electron_kernel_sub_steps_and_setup_deposit(gpu):
+    """post_processing of rows that were not pushed (the set-up deposit,
+    boltzmann.py:363-371) against the oracle's compaction and deposit; then
+    electron_kernel with dt_ratio = 2 (two pushes of dt/2, the rows compacted
+    between them) against the oracle's push twice with its compaction in
+    between: rows, births, the second push's moments and the collision
+    counters summed over both pushes."""
+    import ctypes as C
+    import torch
+    from boltzmann_amd import _lib
+    from boltzmann_amd import kernels as K
+    from boltzmann_amd.kernels import _session as S
+    rng = np.random.default_rng(5)
+    N, n, cap = 63, 20_000, 80_000
+    RN = N + 1
+    dev = torch.device("cuda", 0)
+    e = np.exp(rng.uniform(np.log(0.5), np.log(60.0), n))
+    d = rng.normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0)
+    v = np.sqrt(e / KEV) * d
+    data = np.zeros((cap, 6))
+    tosum = np.zeros((cap, 4))
+    data[:n, 0] = rng.uniform(0.001, 2.539, n)
+    data[:n, 3:6] = v.T
+    tosum[:n, 0] = rng.uniform(1e3, 2e3, n)
+    tosum[:n:17, 0] = 0.0  # dead rows: compacted away
+    E = rng.uniform(-300, 300, N)
+    nn_gas = 1e6 * 3.22e16 * 0.1 * 50  # a dense gas: collisions in both sub-steps
+    dt = 2.0e-11
+    K.set_deposit_bins(4)
+    d_data = torch.from_numpy(data).to(dev)
+    d_tosum = torch.from_numpy(tosum).to(dev)
+
+    def post(Nc, nni):
+        return K.post_processing(None, None, None, None, None, None, d_data, d_tosum, None,
+                                 True, Nc, Nc, nni, np.zeros(RN), np.zeros(RN), np.zeros(RN),
+                                 np.zeros(RN), *[np.zeros(RN) for _ in range(4)], 4 * RN, RN,
+                                 2.54, 2.54 / N, None)
+
+    res = post(n, 0)
+    live = tosum[:n, 0] > 0
+    s = O.Setup(dim=1, dt=dt / 2, xs_mode=0, n_cells=N, length_cm=2.54, bnf=4,
+                gas_density_m3=nn_gas)
+    st = tuple(np.ascontiguousarray(a[live]) for a in
+               (data[:n, 0], data[:n, 3], data[:n, 4], data[:n, 5], tosum[:n, 0]))
+    wdep = O.deposit(s, st)
+    m = res[-1]
+    assert m == int(live.sum())
+    np.testing.assert_allclose(res[0], wdep[0], rtol=1e-12, atol=1e-12 * wdep[0].max())
+    np.testing.assert_allclose(res[1], wdep[1], rtol=1e-12, atol=1e-12 * wdep[0].max())
+    np.testing.assert_allclose(res[3], wdep[2], rtol=1e-12, atol=1e-12 * wdep[2].max())
+    np.testing.assert_array_equal(d_tosum[:m, 0].cpu().numpy(), st[4])
+    np.testing.assert_array_equal(d_data[:m, 0].cpu().numpy(), st[0])
+    assert not d_tosum[m:n].any()
+
+    # the stage session (shared by the stages of one device) names the
+    # Philox step of each push; its collision counters start from zero
+    sess = S.session(0, _lib.BZ_F64, cap, RN)
+    drained = torch.empty(3 * RN, dtype=torch.float64, device=dev)
+    _lib.check(sess.lib.bz_collision_counts(sess.ctx, C.c_void_p(drained.data_ptr()), RN, 1))
+    step0 = sess.step
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    K.electron_kernel(m, d_data, d_tosum, None, torch.from_numpy(E).to(dev), None, None, None,
+                      None, nn_gas, 2, dt, cnt, None, None, None)
+    nni = int(cnt[0])
+    cur = st
+    births = 0
+    coll = np.zeros((3, RN))
+    for k in range(2):
+        R = O.collision_uniforms(12345, step0 + k, cur[0].size, np.float64)
+        cur, wmom, wcnt, wc = O.push_step(s, cur, E, R, capacity=4 * cap, collisions=True)
+        births += int(wcnt[0])
+        coll += wc
+    assert nni == births > 0
+    res = post(m, nni)
+    want = cur
+    assert res[-1] == want[0].size
+    got = d_data[:res[-1]].cpu().numpy()
+    np.testing.assert_array_equal(d_tosum[:res[-1], 0].cpu().numpy(), want[4])
+    vs = np.sqrt(want[1] ** 2 + want[2] ** 2 + want[3] ** 2) + 1
+    np.testing.assert_allclose(got[:, 0], want[0], rtol=1e-12, atol=1e-12)
+    for kk, col in ((1, 3), (2, 4), (3, 5)):
+        assert np.all(np.abs(got[:, col] - want[kk]) <= 1e-12 * vs)
+    np.testing.assert_allclose(res[0], wmom[0], rtol=1e-12, atol=1e-12 * wmom[0].max())
+    np.testing.assert_allclose(res[3], wmom[2], rtol=1e-12, atol=1e-12 * wmom[2].max())
+    np.testing.assert_allclose(res[4], coll[0], rtol=1e-12, atol=1e-12 * coll[0].max())
+    np.testing.assert_array_equal(res[6], coll[2])
+    assert coll[0].sum() > 0
