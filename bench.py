@@ -263,8 +263,10 @@ def bench_engine(args):
         # sorted key 4 + permutation 4 + the five words gathered 20, written 20
         kernels["res_copy"] = _roofline(8 + 10 * word, dct["copied"] / nl[T_COPY], avg(T_COPY),
                                         "k_res_copy", cfg.name, "copied particle")
-    if nl[T_DOWN]:
-        passes = nl[T_DOWN] / max(nl[T_SORT], 1)
+    if nl[T_DOWN] and nl[T_SORT]:
+        # the full sort's passes (in-place resampling sorts only its candidate
+        # subsets: their time is in kernel_ms["downsweep"], with no slot count)
+        passes = nl[T_DOWN] / nl[T_SORT]
         # per slot and pass: key read 4, key + value written 8, value read 4
         # on every pass but the first (the first pass's values are the indices)
         per = (12 + 16 * (passes - 1)) / passes
