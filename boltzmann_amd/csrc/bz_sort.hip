@@ -1562,26 +1562,32 @@ __global__ __launch_bounds__(256) void k_ip_wscatter(const uint32_t* __restrict_
   }
 }
 
-// Heavy-particle counts (w > min_wt) of the split candidates, over their
-// grouped members; a grid sized to what is resident walks the keys.
+// Whether a split candidate has a heavy particle (w > min_wt) among its
+// grouped members (1) or none (0): the plan only asks whether the split
+// draws, so the walk over a segment stops at the first 256 members that hold
+// one.  A grid sized to what is resident walks the keys.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ip_hw(ResArgs a, const int32_t* cand,
                                                const int32_t* mcnt, const int32_t* moff,
                                                const uint32_t* mslot, int32_t* plan) {
-  __shared__ int s_sum;
+  __shared__ int s_any;
   for (int k = blockIdx.x; k < a.n_keys; k += gridDim.x) {
     if (cand[k] != 1) continue;  // block-uniform
     __syncthreads();
-    if (threadIdx.x == 0) s_sum = 0;
+    if (threadIdx.x == 0) s_any = 0;
     __syncthreads();
     const int c = mcnt[k];
-    int h = 0;
-    for (int j = threadIdx.x; j < c; j += 256)
-      h += wt<T>(a, mslot[moff[k] + j]) > a.pol.min_wt ? 1 : 0;
-    h = wave_sum(h);
-    if ((threadIdx.x & 63) == 0 && h) atomicAdd(&s_sum, h);
-    __syncthreads();
-    if (threadIdx.x == 0) plan[4 * k + 3] = s_sum;
+    for (int base = 0; base < c; base += 256) {
+      const int j = base + (int)threadIdx.x;
+      const bool h = j < c && wt<T>(a, mslot[moff[k] + j]) > a.pol.min_wt;
+      const bool wave_has = __any(h);  // every lane of the wave votes
+      if ((threadIdx.x & 63) == 0 && wave_has) s_any = 1;
+      __syncthreads();
+      const int any = s_any;
+      __syncthreads();  // every wave has read the flag before the next round writes it
+      if (any) break;   // block-uniform
+    }
+    if (threadIdx.x == 0) plan[4 * k + 3] = s_any;
   }
 }
 
@@ -1820,7 +1826,11 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
                      env.stream, a, (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
                      (const uint32_t*)ws.mslot, (const int32_t*)ws.plan, ws,
                      (const Clock*)env.clock, cap);
-  hipLaunchKernelGGL((k_res_merge<T, true>), dim3(sgrid), dim3(kSegBlock), (size_t)kMergeLds * 8,
+  // merges are gathers bound by memory latency: as many blocks as there are
+  // merge keys (up to 6 per CU; three fit in a CU's LDS at once) so that no
+  // block walks two segments while others idle
+  const int mgrid = n_keys < 6 * env.num_cu ? n_keys : 6 * env.num_cu;
+  hipLaunchKernelGGL((k_res_merge<T, true>), dim3(mgrid), dim3(kSegBlock), (size_t)kMergeLds * 8,
                      env.stream, a, (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
                      (const uint32_t*)ws.mslot, targets, (const int32_t*)ws.plan, ws,
                      env.clock, cap);
