@@ -153,6 +153,10 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
     e.coll_acc = (double*)(e.coll_cnt + max_nodes);
     e.coll_hi = (uint32_t*)(e.coll_acc + 3 * (size_t)max_nodes);
   }
+  // candidate rings of the queued push (per wave; read only after written)
+  e.push_q_waves = (int64_t)e.num_cu * kPushQueueWavesPerCu;
+  if (st == hipSuccess)
+    st = hipMalloc(&e.push_q, (size_t)e.push_q_waves * kPushQueueWords * (size_t)precision);
   if (st == hipSuccess) st = hipHostMalloc((void**)&c->host_clock, sizeof(Clock));
   if (st == hipSuccess) st = hipMemset(e.clock, 0, sizeof(Clock));
   if (st == hipSuccess) st = hipMemset(e.status, 0, e.status_words * sizeof(uint64_t));
@@ -186,6 +190,7 @@ int bz_ctx_destroy(bz_ctx_t c) {
   if (c->env.dep_m1_lo) (void)hipFree(c->env.dep_m1_lo);
   if (c->env.dep_m1_hi) (void)hipFree(c->env.dep_m1_hi);
   if (c->env.coll_lo) (void)hipFree(c->env.coll_lo);
+  if (c->env.push_q) (void)hipFree(c->env.push_q);
   if (c->host_clock) (void)hipHostFree(c->host_clock);
   graph_release(c);
   if (Timing* t = c->env.timing) {

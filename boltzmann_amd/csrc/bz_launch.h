@@ -11,6 +11,11 @@
 
 namespace bz {
 
+// candidate ring of one wave of the queued push: 5 state words x 128 entries
+constexpr int kPushQueueWords = 5 * 128;
+// waves per CU the queue buffer is sized for (at most 2048 threads per CU)
+constexpr int kPushQueueWavesPerCu = 32;
+
 struct LaunchEnv {
   hipStream_t stream;
   int precision;
@@ -43,6 +48,10 @@ struct LaunchEnv {
   uint32_t* coll_hi;
   unsigned long long* coll_cnt;
   double* coll_acc;
+  // queued push: per-wave rings of candidate states, push_q_waves rings of
+  // kPushQueueWords words (in the state type) each
+  void* push_q;
+  int64_t push_q_waves;
   int push_keys_nb;          // 1-D push writes binning keys for nb bins (0: off)
   double push_keys_thr[8];   // their energy thresholds
   // optional kernel timing: event pairs per timed kernel class

@@ -53,6 +53,10 @@ struct PushArgs {
   // of the next bin sort
   uint32_t* keys;
   BinKeys bk;
+  // queued 1-D push: per wave a ring of kQCap candidates' states [5][kQCap]
+  // (x, vx, vy, vz, w in the state type), so a batch reads its particles
+  // from a small buffer the wave keeps hot instead of the state
+  void* qbuf;
   // 1-D: the ionization moment's global fixed-point words (k_push_post
   // reads and clears them)
   unsigned long long* m1_lo;
@@ -732,8 +736,16 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
   constexpr int PD = DIM == 1 ? 1 : 0;  // physics dimension
   if constexpr (kQueue) {
     if (queued) {
+      // the wave's candidate ring, entries [qh, qh + qn) mod kQCap: slots and
+      // decision uniforms in LDS; 1-D: the particles' states in global memory
+      // too (a batch fills some ten tiles after its first candidate, when
+      // their lines have left L2; the swarms re-read theirs from the state,
+      // which measured faster there)
+      constexpr bool kRing = DIM == 1;
       uint32_t* q = s_q + wid * kQCap;
       T* qr = s_qr + wid * kQCap;
+      T* gq = (T*)a.qbuf + (size_t)(blockIdx.x * NW + wid) * 5 * kQCap;
+      int qh = 0;
       Decision<T> dec[ITEMS];  // the decision draw of the group each item is in
       const uint64_t lt = (1ull << lane) - 1ull;
       const T e_bot = ((const T*)xs.energy)[0], e_top = ((const T*)xs.energy)[xs.n - 1];
@@ -750,10 +762,26 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
       };
       auto run_batch = [&](int nb) {
         const bool act = lane < nb;
-        const uint32_t i = act ? q[lane] : 0u;
-        const T r0 = act ? qr[lane] : (T)1;
+        const int e = (qh + lane) & (kQCap - 1);
+        const uint32_t i = act ? q[e] : 0u;
+        const T r0 = act ? qr[e] : (T)1;
         P5<T> c;
-        load5<T, true>(in, (int64_t)i, act ? n : 0, c);
+        if (kRing) {
+          // the ring's global stores of this wave (other lanes' entries) are
+          // complete and visible before they are read
+          __threadfence_block();
+          if (act) {
+            c.x = gq[e];
+            c.vx = gq[kQCap + e];
+            c.vy = gq[2 * kQCap + e];
+            c.vz = gq[3 * kQCap + e];
+            c.w = gq[4 * kQCap + e];
+          } else {
+            c.x = c.vx = c.vy = c.vz = c.w = (T)0;
+          }
+        } else {
+          load5<T, true>(in, (int64_t)i, act ? n : 0, c);
+        }
         int inst = 0;
         T e_local = (T)0;
         if (DIM == 2) {
@@ -899,17 +927,21 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           }
           const uint64_t cm = __ballot(live && cand);
           if (live && cand) {
-            const int at = qn + (int)__popcll(cm & lt);
+            const int at = (qh + qn + (int)__popcll(cm & lt)) & (kQCap - 1);
             q[at] = (uint32_t)i;
             qr[at] = r0;
+            if (kRing) {
+              gq[at] = cur[k].x;
+              gq[kQCap + at] = cur[k].vx;
+              gq[2 * kQCap + at] = cur[k].vy;
+              gq[3 * kQCap + at] = cur[k].vz;
+              gq[4 * kQCap + at] = cur[k].w;
+            }
           }
           qn += (int)__popcll(cm);
           if (qn >= 64) {
             run_batch(64);
-            if (lane < qn - 64) {
-              q[lane] = q[64 + lane];
-              qr[lane] = qr[64 + lane];
-            }
+            qh = (qh + 64) & (kQCap - 1);
             qn -= 64;
           }
         }
@@ -1354,6 +1386,9 @@ __global__ __launch_bounds__(256) void k_dep_wmax(const T* __restrict__ w, Clock
 
 namespace bz {
 
+static_assert(5 * kQCap == kPushQueueWords && (kQCap & (kQCap - 1)) == 0,
+              "the candidate ring: a power of two, sized as the context's queue buffer");
+
 // Launch of k_dep_reduce: rows per chunk so that the level-2 rows fit.
 // Geometry of the two-level reduction of nblocks rows of m sums: 64 outputs
 // per block column (nx), row chunks of rpc rows (ny).
@@ -1459,7 +1494,13 @@ static int launch_push_inplace_q(const LaunchEnv& env, PushArgs& a, PushLaunch* 
   const int64_t cap_parts = env.partials_doubles / (part_doubles + kCntRow / 2);
   if (grid > cap_parts) grid = (int)cap_parts;
   if (grid > kMaxPushBlocks) grid = kMaxPushBlocks;
+  if (QUEUE && QueuedPush<XS, DIM, RNG>::value) {
+    // one candidate ring per wave
+    const int64_t gq = env.push_q_waves / (BLOCK / kWave);
+    if (grid > gq) grid = (int)gq;
+  }
   if (grid < 1) return fail("partial sum buffer too small");
+  a.qbuf = env.push_q;
   // per-(tile, wave) birth counts follow the per-block ones in the status words
   const int64_t max_tiles = env.capacity / (BLOCK * ITEMS) + 2;
   if (kMaxPushBlocks + max_tiles * (BLOCK / kWave) > 2 * env.status_words)
