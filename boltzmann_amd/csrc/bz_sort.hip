@@ -1056,8 +1056,8 @@ template <typename T, bool INPLACE>
 __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int32_t* counts,
                                               const int32_t* offsets, const uint32_t* perm,
                                               const int32_t* plan, const SortWs& ws,
-                                              const Clock* clk, int64_t cap) {
-  extern __shared__ int64_t s_split[];  // cdf[kSplitLds], copies/offs as int32 after
+                                              const Clock* clk, int64_t cap, int lds_cap) {
+  extern __shared__ int64_t s_split[];  // cdf[lds_cap], copies/offs as int32 after
   __shared__ double s_red[kSegWaves];
   __shared__ int64_t s_w64[kSegWaves];
   __shared__ int s_wi[kSegWaves];
@@ -1065,10 +1065,10 @@ __device__ __forceinline__ void res_split_key(int k, const ResArgs& a, const int
   const int64_t s0 = offsets[k];
   const uint64_t step = clk->step;
   const double minw = a.pol.min_wt;
-  const bool lds = c <= kSplitLds;
+  const bool lds = c <= lds_cap;
   int64_t* cdf = lds ? s_split : ws.cdf + s0;
-  int32_t* copies = lds ? (int32_t*)(s_split + kSplitLds) : ws.copies + s0;
-  int32_t* offs = lds ? (int32_t*)(s_split + kSplitLds) + kSplitLds : (int32_t*)ws.sel + s0;
+  int32_t* copies = lds ? (int32_t*)(s_split + lds_cap) : ws.copies + s0;
+  int32_t* offs = lds ? (int32_t*)(s_split + lds_cap) + lds_cap : (int32_t*)ws.sel + s0;
   const T* Wt = (const T*)a.in[4];
   // fixed-point scale from the largest heavy weight
   double wmax = 0.0;
@@ -1152,7 +1152,8 @@ template <typename T, bool INPLACE>
 __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int32_t* counts,
                                               const int32_t* offsets, const uint32_t* perm,
                                               const int32_t* targets, const int32_t* plan,
-                                              const SortWs& ws, Clock* clk, int64_t cap) {
+                                              const SortWs& ws, Clock* clk, int64_t cap,
+                                              int lds_cap) {
   __shared__ double s_red[5 * kSegWaves];
   __shared__ int64_t s_w64[kSegWaves];
   const int c = counts[k];
@@ -1200,7 +1201,7 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
   auto q = [&](int j) -> int64_t { return (int64_t)ldexp((double)Wt[perm[s0 + j]], S); };
   // the CDF in LDS when it fits (the selection's binary searches read it)
   extern __shared__ int64_t s_mcdf[];
-  const bool lds = ce <= kMergeLds;
+  const bool lds = ce <= lds_cap;
   int64_t* cdf = lds ? s_mcdf : ws.cdf + s0;
   const int64_t Wq = block_fixed_scan(0, ce, cdf, q, s_w64);
   const double wbar = W / (double)K;
@@ -1278,7 +1279,8 @@ __global__ __launch_bounds__(kSegBlock) void k_res_split(ResArgs a, const int32_
   const int n_list = ws.klist[0];
   for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
     __syncthreads();  // the previous key is done with the LDS
-    res_split_key<T, INPLACE>(ws.klist[2 + li], a, counts, offsets, perm, plan, ws, clk, cap);
+    res_split_key<T, INPLACE>(ws.klist[2 + li], a, counts, offsets, perm, plan, ws, clk, cap,
+                              kSplitLds);
   }
 }
 
@@ -1291,7 +1293,36 @@ __global__ __launch_bounds__(kSegBlock) void k_res_merge(ResArgs a, const int32_
   for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
     __syncthreads();
     res_merge_key<T, INPLACE>(ws.klist[2 + ws.kstride + li], a, counts, offsets, perm, targets,
-                              plan, ws, clk, cap);
+                              plan, ws, clk, cap, kMergeLds);
+  }
+}
+
+// The in-place pass runs its merges and splits in one launch, so a pass
+// takes about the longer of the two (the merges: gathers of their members)
+// rather than their sum.  Blocks of 39 KiB of LDS, four to a CU (the grid
+// holds every segment of a pass at once on C4); larger segments keep their
+// CDF in the workspace.  Merges first in the walk.
+constexpr int kSegSplitLds = 2432;  // x 16 B
+constexpr int kSegMergeLds = 4864;  // x 8 B
+constexpr size_t kSegLdsBytes = (size_t)kSegSplitLds * 16;
+static_assert((size_t)kSegMergeLds * 8 == kSegLdsBytes, "one dynamic LDS size for both");
+
+template <typename T>
+__global__ __launch_bounds__(kSegBlock) void k_res_segments(ResArgs a, const int32_t* counts,
+                                                            const int32_t* offsets,
+                                                            const uint32_t* perm,
+                                                            const int32_t* targets,
+                                                            const int32_t* plan, SortWs ws,
+                                                            Clock* clk, int64_t cap) {
+  const int n_split = ws.klist[0], n_merge = ws.klist[1];
+  for (int li = blockIdx.x; li < n_merge + n_split; li += gridDim.x) {
+    __syncthreads();  // the previous segment is done with the LDS
+    if (li < n_merge)
+      res_merge_key<T, true>(ws.klist[2 + ws.kstride + li], a, counts, offsets, perm, targets,
+                             plan, ws, clk, cap, kSegMergeLds);
+    else
+      res_split_key<T, true>(ws.klist[2 + (li - n_merge)], a, counts, offsets, perm, plan, ws, clk,
+                             cap, kSegSplitLds);
   }
 }
 
@@ -1321,6 +1352,7 @@ __global__ void k_res_commit(Clock* clk, int64_t cap) {
 // per block (lds_keys counters; 0: global atomics), added with integer
 // atomics: exact and order independent.
 constexpr int kIpHistBlock = 1024;  // one block per CU: the LDS histogram of every key
+constexpr int kIpHistItems = 8;     // keys per thread in flight
 
 __global__ __launch_bounds__(kIpHistBlock) void k_ip_hist(const uint32_t* __restrict__ keys,
                                                           const Clock* clk, int32_t* counts,
@@ -1329,17 +1361,17 @@ __global__ __launch_bounds__(kIpHistBlock) void k_ip_hist(const uint32_t* __rest
   const int64_t n = clk->n;
   for (int j = threadIdx.x; j < lds_keys; j += kIpHistBlock) s_c[j] = 0u;
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * kIpHistBlock * kKeyItems;
-  for (int64_t base = (int64_t)blockIdx.x * kIpHistBlock * kKeyItems + threadIdx.x; base < n;
+  const int64_t stride = (int64_t)gridDim.x * kIpHistBlock * kIpHistItems;
+  for (int64_t base = (int64_t)blockIdx.x * kIpHistBlock * kIpHistItems + threadIdx.x; base < n;
        base += stride) {
-    uint32_t kv[kKeyItems];
+    uint32_t kv[kIpHistItems];
 #pragma unroll
-    for (int k = 0; k < kKeyItems; ++k) {
+    for (int k = 0; k < kIpHistItems; ++k) {
       const int64_t i = base + k * kIpHistBlock;
       kv[k] = i < n ? keys[i] : 0xFFFFFFFFu;
     }
 #pragma unroll
-    for (int k = 0; k < kKeyItems; ++k) {
+    for (int k = 0; k < kIpHistItems; ++k) {
       if (kv[k] == 0xFFFFFFFFu) continue;
       if (lds_keys) atomicAdd(&s_c[kv[k]], 1u);
       else atomicAdd(&counts[kv[k]], 1);
@@ -1822,18 +1854,13 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
   hipLaunchKernelGGL(k_ip_plan_apply, dim3(nch), dim3(kKeyChunk), 0, env.stream, a, ws.plan,
                      (const int32_t*)ws.ip_tmp, (const long long*)ws.ip_chunk,
                      (const Clock*)env.clock, ws.klist, ws.kstride);
-  hipLaunchKernelGGL((k_res_split<T, true>), dim3(sgrid), dim3(kSegBlock), (size_t)kSplitLds * 16,
-                     env.stream, a, (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
-                     (const uint32_t*)ws.mslot, (const int32_t*)ws.plan, ws,
-                     (const Clock*)env.clock, cap);
-  // merges are gathers bound by memory latency: as many blocks as there are
-  // merge keys (up to 6 per CU; three fit in a CU's LDS at once) so that no
-  // block walks two segments while others idle
-  const int mgrid = n_keys < 6 * env.num_cu ? n_keys : 6 * env.num_cu;
-  hipLaunchKernelGGL((k_res_merge<T, true>), dim3(mgrid), dim3(kSegBlock), (size_t)kMergeLds * 8,
-                     env.stream, a, (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
-                     (const uint32_t*)ws.mslot, targets, (const int32_t*)ws.plan, ws,
-                     env.clock, cap);
+  // merges and splits together: every segment of a pass resident at once
+  // where the list allows (at most 6 blocks per CU; 4 fit)
+  const int ggrid = n_keys < 6 * env.num_cu ? n_keys : 6 * env.num_cu;
+  hipLaunchKernelGGL(k_res_segments<T>, dim3(ggrid), dim3(kSegBlock), kSegLdsBytes, env.stream, a,
+                     (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
+                     (const uint32_t*)ws.mslot, targets, (const int32_t*)ws.plan, ws, env.clock,
+                     cap);
   hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, cap);
   timing_end(env, 2);
   return check_launch("resample in place");
