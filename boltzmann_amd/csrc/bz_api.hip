@@ -134,6 +134,9 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   if (st == hipSuccess) st = hipMalloc((void**)&e.status, e.status_words * sizeof(uint64_t));
   if (st == hipSuccess) st = hipMalloc((void**)&e.partials, e.partials_doubles * sizeof(double));
   if (st == hipSuccess) st = hipMalloc(&e.sort_ws, e.sort_ws_bytes);
+  // the in-place resampling's fire word starts at zero (its candidate kernels
+  // clear it after every pass)
+  if (st == hipSuccess) st = hipMemset(e.sort_ws, 0, e.sort_ws_bytes);
   if (st == hipSuccess) st = hipMalloc((void**)&e.dep_l2, e.dep_l2_doubles * sizeof(double));
   if (st == hipSuccess) st = hipMalloc((void**)&e.dep_cols_done, e.dep_cols * sizeof(uint32_t));
   if (st == hipSuccess) st = hipMemset(e.dep_cols_done, 0, e.dep_cols * sizeof(uint32_t));

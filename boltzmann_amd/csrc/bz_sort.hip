@@ -1812,9 +1812,9 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
                           lds_keys * 4) != hipSuccess)
     return fail("hipFuncSetAttribute(key histogram LDS) failed");
   hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, ws.ip_counts, (int64_t)m);
-  // the fire word the candidate kernel raises
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, env.stream,
-                     (int32_t*)&ws.ip_chunk[8 * kMaxKeyChunks], (int64_t)2);
+  // the fire word the candidate kernel raises is zero: the workspace is
+  // cleared when the context is made, and k_ip_cand_carry clears the word
+  // after every pass
   hipLaunchKernelGGL(k_ip_hist, dim3(env.num_cu), dim3(kIpHistBlock), (size_t)lds_keys * 4,
                      env.stream, keys, (const Clock*)env.clock, ws.ip_counts, lds_keys);
   // candidates, trigger and member offsets (one key per thread, 1024-key chunks)

@@ -90,6 +90,11 @@ class KineticEngine:
         # captured blocks of steps: (state buffer, steps) -> graph id
         self._graphs = {}
         self._cap_stream = None
+        # directly launched 1-D steps (a resampling check every step): the
+        # field step of step i runs on a side stream beside the resampling
+        # of step i + 1, which does not read it (run() only; see _run_overlapped)
+        self.overlap_field = cfg.dim == 1
+        self._field_stream = None
         # collision model
         self.table = None
         self.xs = None
@@ -352,6 +357,8 @@ class KineticEngine:
         # a check every step leaves one push per block: launched directly,
         # which also keeps the keys that push writes for the next sort
         if not graphs or not self.inplace or k == 1:
+            if self.overlap_field and self.inplace and self.inplace_resample and k == 1:
+                return self._run_overlapped(index, n_steps)
             for _ in range(n_steps):
                 self.step(index)
                 index += 1
@@ -369,6 +376,49 @@ class KineticEngine:
                 m = max(1, min(m, self.compact_every - self._since_compact))
             self._replay(m)
             index += m
+        return index
+
+    def _run_overlapped(self, index: int, n_steps: int) -> int:
+        """step(index), step(index + 1), ... with the grid step of each on a
+        side stream: after the push the clock advances on the engine's
+        stream (the time and step the field step would advance), the field
+        step (fluid ions, Poisson: no time advance of its own) follows on the
+        side stream and the next step's resampling check and compaction run
+        beside it (they read the keys and the state, not the fields, and
+        key their draws on the advanced step); the next push waits for the
+        field.  Every kernel sees the inputs of step(); the engine's stream
+        waits for the last field step before this returns."""
+        if self._field_stream is None:
+            self._field_stream = torch.cuda.Stream(self.dev)
+        fs = self._field_stream
+        a_ptr = C.c_void_p(self.stream.cuda_stream)
+        f_ptr = C.c_void_p(fs.cuda_stream)
+        pushed = torch.cuda.Event()
+        fielded = None
+        k = self.cfg.resample_every
+        try:
+            for _ in range(n_steps):
+                if k and self.cfg.target_ppc > 0 and index % k == 0:
+                    self.resample()
+                if self._compact_due():
+                    self.compact()
+                if fielded is not None:
+                    self.stream.wait_event(fielded)
+                self.push()
+                _lib.check(self.lib.bz_clock_advance(self.ctx, self.cfg.dt), "bz_clock_advance")
+                pushed.record(self.stream)
+                fs.wait_event(pushed)
+                _lib.check(self.lib.bz_set_stream(self.ctx, f_ptr))
+                try:
+                    self.field_step(advance=False)
+                finally:
+                    _lib.check(self.lib.bz_set_stream(self.ctx, a_ptr))
+                fielded = torch.cuda.Event()
+                fielded.record(fs)
+                index += 1
+        finally:
+            if fielded is not None:
+                self.stream.wait_event(fielded)
         return index
 
     def _replay(self, m: int):
