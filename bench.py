@@ -252,7 +252,8 @@ def bench_engine(args):
     # push: read + write of the carried components (0-D runs carry no position)
     bytes_push = 2 * (5 if cfg.dim == 1 else 4) * word
     if eng.push_keys:
-        bytes_push += 4  # the binning key of the next sort, written by the push
+        # the binning key of the next sort (or in-place pass), written by the push
+        bytes_push += 2 if eng.push_keys16 else 4
     dct = {k: ct1[k] - ct0[k] for k in ct1}
 
     def avg(kind):

@@ -95,6 +95,7 @@ SIGNATURES = {
     "bz_set_push_grid": (C.c_int, [_P, C.c_int]),
     "bz_set_push_queue": (C.c_int, [_P, C.c_int]),
     "bz_set_push_keys": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    "bz_set_push_keys16": (C.c_int, [_P, C.c_int32]),
     "bz_synchronize": (C.c_int, [_P]),
     "bz_clock_set": (C.c_int, [_P, C.c_int64, C.c_uint64, C.c_double]),
     "bz_clock_get": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),

@@ -53,6 +53,7 @@ struct bz_ctx {
   const void* keys_state;
   BinKeys keys_bk;
   double keys_kev;
+  int keys16;  // those keys are 16-bit words
   // the clock's weight bound was taken from the state loaded by the last
   // clock set (merges raise it on the device afterwards)
   bool wmax_valid;
@@ -121,6 +122,7 @@ int bz_ctx_create(bz_ctx_t* out, int device, int precision, int64_t capacity, in
   e.push_bpc = 0;
   e.push_queue = 0;
   e.push_keys_nb = 0;
+  e.push_keys16 = 0;
   // smallest push tile is 256 particles
   // look-back tile words, or the birth counts of the in-place 1-D push (u32:
   // 4096 per block, then one per tile and wave: at most capacity/64 + 64)
@@ -238,6 +240,13 @@ int bz_set_push_keys(bz_ctx_t c, int32_t nb, const double* thresholds_host) {
   c->env.push_keys_nb = nb;
   for (int k = 0; k < 8; ++k)
     c->env.push_keys_thr[k] = (thresholds_host && k < nb - 1) ? thresholds_host[k] : 0.0;
+  c->keys_state = nullptr;
+  return 0;
+}
+
+int bz_set_push_keys16(bz_ctx_t c, int32_t on) {
+  if (!c) return fail("null context");
+  c->env.push_keys16 = on ? 1 : 0;
   c->keys_state = nullptr;
   return 0;
 }
@@ -433,6 +442,9 @@ int bz_push_mcc(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const v
     if (p->r_stride < 1) return fail("stream mode needs r_stride >= 1");
   }
   c->keys_state = nullptr;
+  if (p->dim == 1 && c->env.push_keys_nb > 0 && c->env.push_keys16 &&
+      (int64_t)p->n_nodes * c->env.push_keys_nb >= 65535)
+    return fail("16-bit push keys: n_nodes * nb must be below 65535");
   rc = ensure_wmax(c, in);
   if (rc) return rc;
   rc = launch_push(c->env, in, out, E, p, xs, R, gas_v, moments, c->env.push_shape);
@@ -440,6 +452,7 @@ int bz_push_mcc(bz_ctx_t c, const bz_state_t* in, const bz_state_t* out, const v
     c->keys_state = in->x;
     c->keys_bk = bin_keys(p, c->env.push_keys_thr, c->env.push_keys_nb);
     c->keys_kev = p->kev;
+    c->keys16 = c->env.push_keys16;
   }
   return rc;
 }
@@ -576,7 +589,7 @@ int bz_bin_sort(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
   // the keys the last push wrote serve when they are this state's, made
   // with this binning
   const bool ready = c->keys_state && c->keys_state == s->x && c->keys_kev == p->kev &&
-                     same_bins(c->keys_bk, bin_keys(p, thresholds_host, nb));
+                     !c->keys16 && same_bins(c->keys_bk, bin_keys(p, thresholds_host, nb));
   c->keys_state = nullptr;  // the sort overwrites its input keys
   return launch_bin_sort(c->env, s, p, thresholds_host, nb, counts, offsets, perm, ready);
 }
@@ -610,13 +623,14 @@ int bz_resample_inplace(bz_ctx_t c, const bz_state_t* s, const bz_params_t* p,
   if (pol->uniforms && pol->u_stride < 1) return fail("resample uniforms need u_stride >= 1");
   const bool ready = c->keys_state && c->keys_state == s->x && c->keys_kev == p->kev &&
                      same_bins(c->keys_bk, bin_keys(p, thresholds_host, nb));
+  const int keys16 = ready ? c->keys16 : 0;  // launch_slot_keys writes 32-bit keys
   c->keys_state = nullptr;  // the state changes
   if (!ready) {
     rc = launch_slot_keys(c->env, s, p, thresholds_host, nb);
     if (rc) return rc;
   }
-  return launch_resample_inplace(c->env, s, p, (const uint32_t*)sort_input_keys(c->env), nb,
-                                 targets, n_keys, pol);
+  return launch_resample_inplace(c->env, s, p, sort_input_keys(c->env), keys16, nb, targets,
+                                 n_keys, pol);
 }
 
 }  // extern "C"

@@ -53,6 +53,7 @@ struct LaunchEnv {
   void* push_q;
   int64_t push_q_waves;
   int push_keys_nb;          // 1-D push writes binning keys for nb bins (0: off)
+  int push_keys16;           // ... as 16-bit words (bz_set_push_keys16)
   double push_keys_thr[8];   // their energy thresholds
   // optional kernel timing: event pairs per timed kernel class
   struct Timing* timing;
@@ -124,9 +125,10 @@ int launch_resample(const LaunchEnv& env, const bz_state_t* in, const bz_state_t
                     const bz_resample_policy_t* pol);
 size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys);
 // in-place resampling (reference layout) of a state whose slot keys are in
-// the sort input (written by the push, or by launch_slot_keys)
+// the sort input (written by the push, or by launch_slot_keys), as 32-bit or
+// (keys16) 16-bit words
 int launch_resample_inplace(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
-                            const uint32_t* keys, int nb, const int32_t* targets,
+                            const void* keys, int keys16, int nb, const int32_t* targets,
                             int32_t n_keys, const bz_resample_policy_t* pol);
 int launch_slot_keys(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
                      const double* thr, int nb);

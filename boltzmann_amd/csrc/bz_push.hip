@@ -52,6 +52,7 @@ struct PushArgs {
   // 1-D push: binning key of every slot it leaves (NULL: none), the input
   // of the next bin sort
   uint32_t* keys;
+  int keys16;  // keys written as 16-bit words
   BinKeys bk;
   // queued 1-D push: per wave a ring of kQCap candidates' states [5][kQCap]
   // (x, vx, vy, vz, w in the state type), so a batch reads its particles
@@ -69,6 +70,12 @@ struct PushArgs {
   unsigned long long* coll_cnt;
   double* coll_acc;
 };
+
+// The binning key of slot i into the sort input (32- or 16-bit words)
+__device__ __forceinline__ void put_key(const PushArgs& a, int64_t i, uint32_t k) {
+  if (a.keys16) ((uint16_t*)a.keys)[i] = (uint16_t)k;
+  else a.keys[i] = k;
+}
 
 // A collision of the 1-D push into the counters: at the primary's node after
 // the push, node 0 when it left the gap (the reference's collct rows,
@@ -826,7 +833,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           }
           if (DIM == 1 && a.coll_lo) coll_add<T>(a, nn, o, p, (double)c.w, dscale);
           if (DIM == 1 && a.keys)
-            a.keys[i] = o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w) : (uint32_t)a.bk.n_keys;
+            put_key(a, i, o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w)
+                                  : (uint32_t)a.bk.n_keys);
         }
         // births: the batch's slots rise, so equal tiles are runs of lanes
         // (segments), taken in order
@@ -916,8 +924,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
           }
           // keys of the slots settled here (candidates get theirs in their batch)
           if (DIM == 1 && a.keys && i < n && !(live && cand))
-            a.keys[i] = plain && o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w)
-                                         : (uint32_t)a.bk.n_keys;
+            put_key(a, i, plain && o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w)
+                                           : (uint32_t)a.bk.n_keys);
           if (DIM == 2) {
             // running moments of the wave's instance (the queued candidates
             // add theirs in their batch)
@@ -1014,8 +1022,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_inplace(PushArgs a) {
         if (DIM == 1 && a.coll_lo) coll_add<T>(a, nn, o, p, (double)cur[k].w, dscale);
       }
       if (DIM == 1 && a.keys && i < n)
-        a.keys[i] = live && o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w)
-                                   : (uint32_t)a.bk.n_keys;
+        put_key(a, i, live && o.alive ? bin_key<T>(a.bk, thr, o.p.x, o.e_post, o.p.w)
+                                     : (uint32_t)a.bk.n_keys);
       if (DIM == 2) {
         // per-instance moments of this step; a wave whose lanes share one
         // instance (the common case: instances are contiguous between
@@ -1146,7 +1154,7 @@ __device__ void births_part(const PushArgs& a, int b, int nblk, int tile, int nw
         }
         if (a.keys) {
           const T e = (T)a.p.kev * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
-          a.keys[dst] = bin_key<T>(a.bk, v[0], e, v[4]);
+          put_key(a, dst, bin_key<T>(a.bk, v[0], e, v[4]));
         }
       }
       run += tot;
@@ -1675,6 +1683,7 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
   a.inst_E = nullptr;
   a.n_inst = 0;
   a.keys = nullptr;
+  a.keys16 = 0;
   a.e_lds_off = 0;
   a.m1_lo = p->dim == 1 ? env.dep_m1_lo : nullptr;
   a.m1_hi = p->dim == 1 ? env.dep_m1_hi : nullptr;
@@ -1685,6 +1694,7 @@ int launch_push(const LaunchEnv& env, const bz_state_t* in, const bz_state_t* ou
   a.coll_acc = coll ? env.coll_acc : nullptr;
   if (p->dim == 1 && env.push_keys_nb > 0) {
     a.keys = (uint32_t*)sort_input_keys(env);
+    a.keys16 = env.push_keys16;
     a.bk = bin_keys(p, env.push_keys_thr, env.push_keys_nb);
   }
   PushLaunch pl;
@@ -1829,6 +1839,7 @@ int launch_push_swarm(const LaunchEnv& env, const bz_state_t* in, const bz_state
   a.inst_E = inst_E;
   a.n_inst = n_inst;
   a.keys = nullptr;
+  a.keys16 = 0;
   a.e_lds_off = 0;
   a.m1_lo = nullptr;
   a.m1_hi = nullptr;

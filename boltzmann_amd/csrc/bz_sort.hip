@@ -1354,7 +1354,8 @@ __global__ void k_res_commit(Clock* clk, int64_t cap) {
 constexpr int kIpHistBlock = 1024;  // one block per CU: the LDS histogram of every key
 constexpr int kIpHistItems = 8;     // keys per thread in flight
 
-__global__ __launch_bounds__(kIpHistBlock) void k_ip_hist(const uint32_t* __restrict__ keys,
+template <typename K>
+__global__ __launch_bounds__(kIpHistBlock) void k_ip_hist(const K* __restrict__ keys,
                                                           const Clock* clk, int32_t* counts,
                                                           int lds_keys) {
   extern __shared__ uint32_t s_c[];
@@ -1368,7 +1369,7 @@ __global__ __launch_bounds__(kIpHistBlock) void k_ip_hist(const uint32_t* __rest
 #pragma unroll
     for (int k = 0; k < kIpHistItems; ++k) {
       const int64_t i = base + k * kIpHistBlock;
-      kv[k] = i < n ? keys[i] : 0xFFFFFFFFu;
+      kv[k] = i < n ? (uint32_t)keys[i] : 0xFFFFFFFFu;
     }
 #pragma unroll
     for (int k = 0; k < kIpHistItems; ++k) {
@@ -1519,7 +1520,8 @@ __device__ __forceinline__ void ip_wave_range(int64_t n, int64_t* lo, int64_t* h
   *hi = b < n ? b : n;
 }
 
-__global__ __launch_bounds__(256) void k_ip_wcount(const uint32_t* __restrict__ keys,
+template <typename K>
+__global__ __launch_bounds__(256) void k_ip_wcount(const K* __restrict__ keys,
                                                    const Clock* clk,
                                                    const int32_t* __restrict__ cand, int n_keys,
                                                    const uint32_t* __restrict__ bits,
@@ -1537,7 +1539,7 @@ __global__ __launch_bounds__(256) void k_ip_wcount(const uint32_t* __restrict__ 
 #pragma unroll
       for (int u = 0; u < kIpRow; ++u) {
         const int64_t i = b + u * 64 + lane;
-        kv[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
+        kv[u] = i < hi ? (uint32_t)keys[i] : 0xFFFFFFFFu;
       }
 #pragma unroll
       for (int u = 0; u < kIpRow; ++u) c += ip_listed(s_bits, kv[u], n_keys) ? 1u : 0u;
@@ -1559,7 +1561,8 @@ __global__ __launch_bounds__(1024) void k_ip_wscan(uint32_t* wcnt) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_ip_wscatter(const uint32_t* __restrict__ keys,
+template <typename K>
+__global__ __launch_bounds__(256) void k_ip_wscatter(const K* __restrict__ keys,
                                                      const Clock* clk,
                                                      const int32_t* __restrict__ cand,
                                                      int n_keys, const uint32_t* __restrict__ bits,
@@ -1578,7 +1581,7 @@ __global__ __launch_bounds__(256) void k_ip_wscatter(const uint32_t* __restrict_
 #pragma unroll
     for (int u = 0; u < kIpRow; ++u) {
       const int64_t i = b + u * 64 + lane;
-      kv[u] = i < hi ? keys[i] : 0xFFFFFFFFu;
+      kv[u] = i < hi ? (uint32_t)keys[i] : 0xFFFFFFFFu;
     }
 #pragma unroll
     for (int u = 0; u < kIpRow; ++u) {
@@ -1782,9 +1785,9 @@ static int sort_members(const LaunchEnv& env, const SortWs& ws, int n_keys) {
   return 0;
 }
 
-template <typename T>
+template <typename T, typename K>
 static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
-                                     const bz_params_t* p, const uint32_t* keys, int nb,
+                                     const bz_params_t* p, const K* keys, int nb,
                                      const int32_t* targets, int32_t n_keys,
                                      const bz_resample_policy_t* pol) {
   SortWs ws = carve(env.sort_ws, env.capacity, env.max_keys);
@@ -1808,14 +1811,14 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
   const int m = n_keys + 1;
   const int lds_keys = m <= 40960 ? m : 0;
   if ((size_t)lds_keys * 4 > 64 * 1024 &&
-      hipFuncSetAttribute((const void*)k_ip_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
+      hipFuncSetAttribute((const void*)k_ip_hist<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           lds_keys * 4) != hipSuccess)
     return fail("hipFuncSetAttribute(key histogram LDS) failed");
   hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, ws.ip_counts, (int64_t)m);
   // the fire word the candidate kernel raises is zero: the workspace is
   // cleared when the context is made, and k_ip_cand_carry clears the word
   // after every pass
-  hipLaunchKernelGGL(k_ip_hist, dim3(env.num_cu), dim3(kIpHistBlock), (size_t)lds_keys * 4,
+  hipLaunchKernelGGL(k_ip_hist<K>, dim3(env.num_cu), dim3(kIpHistBlock), (size_t)lds_keys * 4,
                      env.stream, keys, (const Clock*)env.clock, ws.ip_counts, lds_keys);
   // candidates, trigger and member offsets (one key per thread, 1024-key chunks)
   const int nch = (n_keys + kKeyChunk - 1) / kKeyChunk;
@@ -1830,11 +1833,11 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
                      (const long long*)ws.ip_chunk, ws.ip_moff, ws.ip_mcnt);
   // members of the candidate segments, in slot order, then grouped by key
   const size_t bits_lds = 4 * ((size_t)n_keys / 32 + 1);
-  hipLaunchKernelGGL(k_ip_wcount, dim3(kIpBlocks), dim3(256), bits_lds, env.stream, keys,
+  hipLaunchKernelGGL(k_ip_wcount<K>, dim3(kIpBlocks), dim3(256), bits_lds, env.stream, keys,
                      (const Clock*)env.clock, (const int32_t*)ws.ip_cand, n_keys,
                      (const uint32_t*)ws.ip_bits, ws.ip_bcnt);
   hipLaunchKernelGGL(k_ip_wscan, dim3(1), dim3(1024), 0, env.stream, ws.ip_bcnt);
-  hipLaunchKernelGGL(k_ip_wscatter, dim3(kIpBlocks), dim3(256), bits_lds, env.stream, keys,
+  hipLaunchKernelGGL(k_ip_wscatter<K>, dim3(kIpBlocks), dim3(256), bits_lds, env.stream, keys,
                      (const Clock*)env.clock, (const int32_t*)ws.ip_cand, n_keys,
                      (const uint32_t*)ws.ip_bits, (const uint32_t*)ws.ip_bcnt, ws.mkeys,
                      ws.mslot);
@@ -2073,12 +2076,16 @@ int launch_fill_holes(const LaunchEnv& env, const bz_state_t* s) {
 }
 
 int launch_resample_inplace(const LaunchEnv& env, const bz_state_t* s, const bz_params_t* p,
-                            const uint32_t* keys, int nb, const int32_t* targets,
+                            const void* keys, int keys16, int nb, const int32_t* targets,
                             int32_t n_keys, const bz_resample_policy_t* pol) {
   if (n_keys + 1 > 65535) return fail("resample: at most 65534 keys");
+  const uint16_t* k16 = (const uint16_t*)keys;
+  const uint32_t* k32 = (const uint32_t*)keys;
   if (env.precision == BZ_F64)
-    return launch_resample_inplace_t<double>(env, s, p, keys, nb, targets, n_keys, pol);
-  return launch_resample_inplace_t<float>(env, s, p, keys, nb, targets, n_keys, pol);
+    return keys16 ? launch_resample_inplace_t<double>(env, s, p, k16, nb, targets, n_keys, pol)
+                  : launch_resample_inplace_t<double>(env, s, p, k32, nb, targets, n_keys, pol);
+  return keys16 ? launch_resample_inplace_t<float>(env, s, p, k16, nb, targets, n_keys, pol)
+                : launch_resample_inplace_t<float>(env, s, p, k32, nb, targets, n_keys, pol);
 }
 
 // the slot keys of a state (k_keys) into the sort input, for a caller whose

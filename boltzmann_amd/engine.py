@@ -52,7 +52,7 @@ class KineticEngine:
                  compact_every: int = 64, push_keys: Optional[bool] = None,
                  inplace_resample: Optional[bool] = None,
                  compact_resampled_every: Optional[int] = None,
-                 fill_holes: Optional[bool] = None):
+                 fill_holes: Optional[bool] = None, push_keys16: Optional[bool] = None):
         if not torch.cuda.is_available():
             raise _lib.BzError("no GPU visible: the electron step runs only in the HIP kernels")
         self.lib = _lib.load()
@@ -150,6 +150,13 @@ class KineticEngine:
         self.push_keys = bool(push_keys) and cfg.dim == 1
         _lib.check(self.lib.bz_set_push_keys(self.ctx, self.nb if self.push_keys else 0,
                                              self.thr), "bz_set_push_keys")
+        # the in-place resampling reads the keys three times: 16-bit words
+        # where they fit (bz_bin_sort, the key-order path, takes 32-bit ones)
+        if push_keys16 is None:
+            push_keys16 = self.inplace_resample
+        self.push_keys16 = bool(push_keys16) and self.push_keys and nn * max(cfg.nb, 1) < 65535
+        _lib.check(self.lib.bz_set_push_keys16(self.ctx, 1 if self.push_keys16 else 0),
+                   "bz_set_push_keys16")
 
     # ------------------------------------------------------------------ state
     @property

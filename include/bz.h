@@ -229,6 +229,15 @@ int bz_set_push_queue(bz_ctx_t ctx, int mode);
  * this function again first.  nb = 0 turns it off (default).  No
  * counterpart in the reference. */
 int bz_set_push_keys(bz_ctx_t ctx, int32_t nb, const double* thresholds);
+/* Width of the keys the 1-D push writes (bz_set_push_keys): on = 1 writes
+ * them as 16-bit words, half the bytes the push writes and the in-place
+ * resampling (bz_resample_inplace) reads three times (key histogram, member
+ * count, member extraction); the next bz_resample_inplace of that state
+ * reads them as such.  bz_bin_sort, which sorts 32-bit keys, recomputes its
+ * keys from the state after a push that wrote 16-bit ones.  Keys above 65535
+ * (n_nodes * nb >= 65535) fail the push.  on = 0 (default): 32-bit keys.
+ * No counterpart in the reference. */
+int bz_set_push_keys16(bz_ctx_t ctx, int32_t on);
 /* Tile schedule of the fused push: 0 static tile order under a cooperative
  * launch (default), 1 tiles claimed from a ticket counter by running blocks.
  * No counterpart in the reference. */

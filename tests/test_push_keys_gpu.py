@@ -38,7 +38,8 @@ def _engine(prec, xs, nb, n, push_keys, queue):
     cfg = ph.RunConfig(name="keys", dim=1, n_particles=n, precision=prec, xs_mode=xs, nb=nb,
                        n_cells=48, length_cm=2.5, dt=2e-11, gas_density_m3=gas, target_ppc=100,
                        bin_fractions=ph.bin_fractions(nb), bin_thresholds=ph.bin_thresholds(nb))
-    eng = KineticEngine(cfg, capacity=3 * n + 64, push_keys=push_keys)
+    # 32-bit keys: the bin sort (the key-order path) takes the push's keys
+    eng = KineticEngine(cfg, capacity=3 * n + 64, push_keys=push_keys, push_keys16=False)
     _lib.check(eng.lib.bz_set_push_queue(eng.ctx, queue))
     if queue == 1:
         # one block per CU: every wave walks several tiles and its batches cross them
@@ -131,7 +132,7 @@ def test_push_keys_with_births_past_capacity(gpu):
                        gas_density_m3=3.2e24, target_ppc=100, bin_fractions=ph.bin_fractions(3),
                        bin_thresholds=ph.bin_thresholds(3))
     cap = n + 64  # far fewer free slots than births
-    a = KineticEngine(cfg, capacity=cap, push_keys=True)
+    a = KineticEngine(cfg, capacity=cap, push_keys=True, push_keys16=False)
     b = KineticEngine(cfg, capacity=cap, push_keys=False)
     st = _state(n, cfg.np_dtype, seed=9, dead_frac=0.0)
     for e in (a, b):
@@ -156,3 +157,56 @@ def test_push_keys_empty_state(gpu):
     assert int(a.counts.cpu().numpy().sum()) == 0
     a.close()
     _ = ph
+
+
+def test_push_keys16_inplace_resampling_equals_32bit(gpu):
+    """16-bit push keys (bz_set_push_keys16, the engine's choice for in-place
+    resampling): a run that resamples every step in place, births appended
+    with their keys by k_push_post, equals the run on 32-bit keys slot for
+    slot (C4's layout scaled: 5 bins, 256 cells, compaction every 4th pass)."""
+    import torch
+    from boltzmann_amd import configs
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    cfg = configs.c4().with_(n_particles=300_000, n_cells=256, target_ppc=1200)
+    st = ph.initial_state(cfg)
+    runs = []
+    for k16 in (True, False):
+        e = KineticEngine(cfg, push_keys16=k16)
+        assert e.push_keys and e.inplace_resample and e.push_keys16 == k16
+        e.load_state(st)
+        e.initialize_fields()
+        for i in range(9):
+            e.step(i)
+        torch.cuda.synchronize()
+        runs.append(e)
+    a, b = runs
+    ca, cb = a.counters(), b.counters()
+    assert ca == cb and ca["merge"] > 0 and ca["split"] > 0 and ca["births"] > 0
+    for u, v in zip(a.fetch_slots(), b.fetch_slots()):
+        np.testing.assert_array_equal(u, v)
+    np.testing.assert_array_equal(a.moments.cpu().numpy(), b.moments.cpu().numpy())
+    a.close()
+    b.close()
+
+
+def test_push_keys16_rejects_keys_beyond_16_bits(gpu):
+    """n_nodes * nb >= 65535 does not fit 16-bit keys: the push fails with
+    an error and leaves the state as it was."""
+    from boltzmann_amd import _lib
+    from boltzmann_amd import physics as ph
+    from boltzmann_amd.engine import KineticEngine
+    cfg = ph.RunConfig(name="keys16-wide", dim=1, n_particles=10_000, precision=4,
+                       xs_mode=_lib.BZ_XS_TABLE, nb=5, n_cells=13106, length_cm=2.5, dt=2e-11,
+                       gas_density_m3=3.2e24, target_ppc=10, bin_fractions=ph.bin_fractions(5),
+                       bin_thresholds=ph.bin_thresholds(5))
+    assert cfg.n_nodes * 5 == 65535
+    e = KineticEngine(cfg, push_keys=True, push_keys16=False)
+    st = _state(10_000, cfg.np_dtype, seed=3, dead_frac=0.0)
+    e.load_state(st)
+    _lib.check(e.lib.bz_set_push_keys16(e.ctx, 1))
+    with pytest.raises(_lib.BzError, match="16-bit push keys"):
+        e.push()
+    for u, v in zip(e.fetch_slots(), st):
+        np.testing.assert_array_equal(u, v)
+    e.close()
