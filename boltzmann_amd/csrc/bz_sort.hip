@@ -1350,34 +1350,63 @@ __global__ void k_res_commit(Clock* clk, int64_t cap) {
 
 // Slots per key from the keys the push (or k_keys) wrote.  An LDS histogram
 // per block (lds_keys counters; 0: global atomics), added with integer
-// atomics: exact and order independent.
+// atomics: exact and order independent.  A thread reads 16-byte vectors of
+// consecutive keys and adds each run of equal keys in a vector with one
+// atomic: neighbouring slots often share a key (a segment's appended merges
+// and clones, a cell's particles loaded in order), and lanes adding to one
+// LDS word serialise.
 constexpr int kIpHistBlock = 1024;  // one block per CU: the LDS histogram of every key
-constexpr int kIpHistItems = 8;     // keys per thread in flight
+constexpr int kIpHistVecs = 4;      // 16-byte vectors per thread in flight
+
+__device__ __forceinline__ void ip_hist_add(uint32_t key, uint32_t c, uint32_t* s_c,
+                                            int32_t* counts, int lds_keys) {
+  if (lds_keys) atomicAdd(&s_c[key], c);
+  else atomicAdd(&counts[key], (int32_t)c);
+}
 
 template <typename K>
 __global__ __launch_bounds__(kIpHistBlock) void k_ip_hist(const K* __restrict__ keys,
                                                           const Clock* clk, int32_t* counts,
                                                           int lds_keys) {
+  static_assert(sizeof(K) == 2 || sizeof(K) == 4, "16- or 32-bit keys");
+  constexpr int KPV = 16 / (int)sizeof(K);  // keys per vector
   extern __shared__ uint32_t s_c[];
   const int64_t n = clk->n;
   for (int j = threadIdx.x; j < lds_keys; j += kIpHistBlock) s_c[j] = 0u;
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * kIpHistBlock * kIpHistItems;
-  for (int64_t base = (int64_t)blockIdx.x * kIpHistBlock * kIpHistItems + threadIdx.x; base < n;
+  // whole vectors (the key buffer is 16-byte aligned), then the tail
+  const int64_t nv = n / KPV;
+  const uint4* kv = (const uint4*)keys;
+  const int64_t stride = (int64_t)gridDim.x * kIpHistBlock * kIpHistVecs;
+  for (int64_t base = (int64_t)blockIdx.x * kIpHistBlock * kIpHistVecs + threadIdx.x; base < nv;
        base += stride) {
-    uint32_t kv[kIpHistItems];
+    uint4 v[kIpHistVecs];
 #pragma unroll
-    for (int k = 0; k < kIpHistItems; ++k) {
+    for (int k = 0; k < kIpHistVecs; ++k) {
       const int64_t i = base + k * kIpHistBlock;
-      kv[k] = i < n ? (uint32_t)keys[i] : 0xFFFFFFFFu;
+      v[k] = i < nv ? kv[i] : make_uint4(0u, 0u, 0u, 0u);
     }
 #pragma unroll
-    for (int k = 0; k < kIpHistItems; ++k) {
-      if (kv[k] == 0xFFFFFFFFu) continue;
-      if (lds_keys) atomicAdd(&s_c[kv[k]], 1u);
-      else atomicAdd(&counts[kv[k]], 1);
+    for (int k = 0; k < kIpHistVecs; ++k) {
+      if (base + k * kIpHistBlock >= nv) continue;
+      const uint32_t w[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+      uint32_t run = KPV == 8 ? (w[0] & 0xFFFFu) : w[0], c = 0;
+#pragma unroll
+      for (int q = 0; q < KPV; ++q) {
+        const uint32_t key = KPV == 8 ? ((q & 1) ? w[q >> 1] >> 16 : w[q >> 1] & 0xFFFFu) : w[q];
+        if (key != run) {
+          ip_hist_add(run, c, s_c, counts, lds_keys);
+          run = key;
+          c = 0;
+        }
+        ++c;
+      }
+      ip_hist_add(run, c, s_c, counts, lds_keys);
     }
   }
+  if (blockIdx.x == gridDim.x - 1)
+    for (int64_t i = nv * KPV + threadIdx.x; i < n; i += kIpHistBlock)
+      ip_hist_add((uint32_t)keys[i], 1u, s_c, counts, lds_keys);
   __syncthreads();
   for (int j = threadIdx.x; j < lds_keys; j += kIpHistBlock)
     if (s_c[j]) atomicAdd(&counts[j], (int32_t)s_c[j]);
