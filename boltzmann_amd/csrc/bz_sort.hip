@@ -42,7 +42,8 @@ struct SortWs {
   uint32_t* mslot;
   uint32_t* mkeys2;
   uint32_t* mslot2;
-  int32_t* ip_counts;   // [max_keys+1] slots per key
+  int32_t* ip_counts;   // [max_keys+1] slots per key (key-order lists)
+  int32_t* ip_hcnt;     // [max_keys+1] slots per key of an in-place pass; zero between passes
   int32_t* ip_moff;     // [max_keys+2] member offsets per key
   int32_t* ip_mcnt;     // [max_keys+1] members per key
   int32_t* ip_cand;     // [max_keys+1] 0 untouched, 1 split candidate, 2 merge; [max_keys]: fire
@@ -57,7 +58,8 @@ struct SortWs {
 constexpr int kIpBlocks = 2048;
 constexpr int kIpWaves = kIpBlocks * 4;
 // plan scans: 1024 keys per block (n_keys <= 65534: at most 64 chunks); per
-// chunk 8 words: member total, appended, splits, merges and their prefixes
+// chunk 8 words: member total, appended, splits, merges and their prefixes;
+// then the fire word and the slot count the pass's appends start from
 constexpr int kKeyChunk = 1024;
 constexpr int kMaxKeyChunks = 64;
 constexpr int kIpChunkWords = 8 * kMaxKeyChunks + 8;
@@ -90,6 +92,7 @@ static SortWs carve(void* base, int64_t cap, int32_t max_keys) {
   w.mkeys2 = (uint32_t*)take(4 * cap);
   w.mslot2 = (uint32_t*)take(4 * cap);
   w.ip_counts = (int32_t*)take(4 * ((size_t)max_keys + 1));
+  w.ip_hcnt = (int32_t*)take(4 * ((size_t)max_keys + 1));
   w.ip_moff = (int32_t*)take(4 * ((size_t)max_keys + 2));
   w.ip_mcnt = (int32_t*)take(4 * ((size_t)max_keys + 1));
   w.ip_cand = (int32_t*)take(4 * ((size_t)max_keys + 1));
@@ -118,6 +121,7 @@ size_t sort_workspace_bytes(int64_t capacity, int32_t max_keys) {
   add(4 * cap);
   add(4 * (2 + 2 * ((size_t)max_keys + 1)));
   for (int k = 0; k < 4; ++k) add(4 * cap);
+  add(4 * ((size_t)max_keys + 1));
   add(4 * ((size_t)max_keys + 1));
   add(4 * ((size_t)max_keys + 2));
   add(4 * ((size_t)max_keys + 1));
@@ -1695,11 +1699,14 @@ __global__ __launch_bounds__(kKeyChunk) void k_ip_plan_local(ResArgs a, const in
   }
 }
 
-// One block: chunk prefixes of appended / split / merge, the new slot count,
-// the list lengths and the counters.
+// One block: chunk prefixes of appended / split / merge, the list lengths and
+// the counters; the new slot count is committed here (clipped to the
+// capacity, the excess counted as overflow), the count the appends start
+// from kept in the chunk words for k_ip_plan_apply.  No kernel of the pass
+// after this one reads the slot count.
 __global__ __launch_bounds__(64) void k_ip_plan_carry(ResArgs a, int nch, const int32_t* cand,
                                                       long long* chunk, Clock* clk,
-                                                      int32_t* klist) {
+                                                      int32_t* klist, int64_t cap) {
   if (threadIdx.x != 0) return;
   long long app = 0, ns = 0, nm = 0;
   for (int b = 0; b < nch; ++b) {
@@ -1713,23 +1720,35 @@ __global__ __launch_bounds__(64) void k_ip_plan_carry(ResArgs a, int nch, const 
   }
   klist[0] = (int32_t)ns;
   klist[1] = (int32_t)nm;
-  clk->n_next = clk->n + app;
+  const int64_t n0 = clk->n;
+  chunk[8 * kMaxKeyChunks + 1] = n0;
+  int64_t n1 = n0 + app;
+  if (n1 > cap) {
+    clk->cnt[BZ_CNT_OVERFLOW] += (unsigned long long)(n1 - cap);
+    n1 = cap;
+  }
+  clk->n = n1;
+  clk->n_next = n1;
   if (ns) clk->cnt[BZ_CNT_SPLIT] += (unsigned long long)ns;
   if (nm) clk->cnt[BZ_CNT_MERGE] += (unsigned long long)nm;
   if (cand[a.n_keys]) clk->cnt[BZ_CNT_RESAMPLE] += 1ull;
 }
 
-// The first appended slot of every key (after the n slots in use, in key
-// order) and the key lists in key order.
+// The first appended slot of every key (after the slots in use before the
+// pass, in key order) and the key lists in key order.  The pass's key
+// histogram is read for the last time before this kernel: it is cleared
+// here for the next pass.
 __global__ __launch_bounds__(kKeyChunk) void k_ip_plan_apply(ResArgs a, int32_t* plan,
                                                              const int32_t* tmp,
                                                              const long long* chunk,
-                                                             const Clock* clk, int32_t* klist,
-                                                             int kstride) {
+                                                             int32_t* klist, int kstride,
+                                                             int32_t* hcnt) {
   const int k = blockIdx.x * kKeyChunk + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) hcnt[a.n_keys] = 0;  // the holes' key
   if (k >= a.n_keys) return;
+  hcnt[k] = 0;
   const long long* c = chunk + 8 * blockIdx.x;
-  plan[4 * k + 2] = (int32_t)(clk->n + c[5] + plan[4 * k + 2]);
+  plan[4 * k + 2] = (int32_t)(chunk[8 * kMaxKeyChunks + 1] + c[5] + plan[4 * k + 2]);
   const int kind = plan[4 * k + 0];
   if (kind == kSplit) klist[2 + c[6] + tmp[k]] = k;
   else if (kind == kMerge) klist[2 + kstride + c[7] + tmp[k]] = k;
@@ -1843,22 +1862,26 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
       hipFuncSetAttribute((const void*)k_ip_hist<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           lds_keys * 4) != hipSuccess)
     return fail("hipFuncSetAttribute(key histogram LDS) failed");
-  hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, ws.ip_counts, (int64_t)m);
-  // the fire word the candidate kernel raises is zero: the workspace is
-  // cleared when the context is made, and k_ip_cand_carry clears the word
-  // after every pass
-  hipLaunchKernelGGL(k_ip_hist<K>, dim3(env.num_cu), dim3(kIpHistBlock), (size_t)lds_keys * 4,
-                     env.stream, keys, (const Clock*)env.clock, ws.ip_counts, lds_keys);
-  // candidates, trigger and member offsets (one key per thread, 1024-key chunks)
   const int nch = (n_keys + kKeyChunk - 1) / kKeyChunk;
   if (nch > kMaxKeyChunks) return fail("resample: at most %d keys", kMaxKeyChunks * kKeyChunk);
+  // The histogram and the fire word the candidate kernel raises start at
+  // zero: the workspace is cleared when the context is made, and every pass
+  // clears them again (k_ip_plan_apply, k_ip_cand_carry), also one that
+  // stops on a launch error (bail)
+  auto bail = [&](int rc) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, ws.ip_hcnt, (int64_t)m);
+    return rc;
+  };
+  hipLaunchKernelGGL(k_ip_hist<K>, dim3(env.num_cu), dim3(kIpHistBlock), (size_t)lds_keys * 4,
+                     env.stream, keys, (const Clock*)env.clock, ws.ip_hcnt, lds_keys);
+  // candidates, trigger and member offsets (one key per thread, 1024-key chunks)
   hipLaunchKernelGGL(k_ip_cand, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
-                     (const int32_t*)ws.ip_counts, targets, nb, ws.ip_cand, ws.ip_moff,
+                     (const int32_t*)ws.ip_hcnt, targets, nb, ws.ip_cand, ws.ip_moff,
                      ws.ip_chunk, ws.ip_bits);
   hipLaunchKernelGGL(k_ip_cand_carry, dim3(1), dim3(64), 0, env.stream, a, nch, ws.ip_cand,
                      ws.ip_chunk, ws.ip_mclk);
   hipLaunchKernelGGL(k_ip_cand_apply, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
-                     (const int32_t*)ws.ip_counts, (const int32_t*)ws.ip_cand,
+                     (const int32_t*)ws.ip_hcnt, (const int32_t*)ws.ip_cand,
                      (const long long*)ws.ip_chunk, ws.ip_moff, ws.ip_mcnt);
   // members of the candidate segments, in slot order, then grouped by key
   const size_t bits_lds = 4 * ((size_t)n_keys / 32 + 1);
@@ -1871,21 +1894,21 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
                      (const uint32_t*)ws.ip_bits, (const uint32_t*)ws.ip_bcnt, ws.mkeys,
                      ws.mslot);
   int rc = check_launch("member extraction");
-  if (rc) return rc;
+  if (rc) return bail(rc);
   rc = sort_members(env, ws, n_keys);
-  if (rc) return rc;
+  if (rc) return bail(rc);
   const int sgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
   hipLaunchKernelGGL(k_ip_hw<T>, dim3(sgrid), dim3(256), 0, env.stream, a,
                      (const int32_t*)ws.ip_cand, (const int32_t*)ws.ip_mcnt,
                      (const int32_t*)ws.ip_moff, (const uint32_t*)ws.mslot, ws.plan);
   hipLaunchKernelGGL(k_ip_plan_local, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
-                     (const int32_t*)ws.ip_counts, targets, (const int32_t*)ws.ip_cand, ws.plan,
+                     (const int32_t*)ws.ip_hcnt, targets, (const int32_t*)ws.ip_cand, ws.plan,
                      ws.ip_tmp, ws.ip_chunk, ws.kstride);
   hipLaunchKernelGGL(k_ip_plan_carry, dim3(1), dim3(64), 0, env.stream, a, nch,
-                     (const int32_t*)ws.ip_cand, ws.ip_chunk, env.clock, ws.klist);
+                     (const int32_t*)ws.ip_cand, ws.ip_chunk, env.clock, ws.klist, cap);
   hipLaunchKernelGGL(k_ip_plan_apply, dim3(nch), dim3(kKeyChunk), 0, env.stream, a, ws.plan,
-                     (const int32_t*)ws.ip_tmp, (const long long*)ws.ip_chunk,
-                     (const Clock*)env.clock, ws.klist, ws.kstride);
+                     (const int32_t*)ws.ip_tmp, (const long long*)ws.ip_chunk, ws.klist,
+                     ws.kstride, ws.ip_hcnt);
   // merges and splits together: every segment of a pass resident at once
   // where the list allows (at most 6 blocks per CU; 4 fit)
   const int ggrid = n_keys < 6 * env.num_cu ? n_keys : 6 * env.num_cu;
@@ -1893,7 +1916,6 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
                      (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
                      (const uint32_t*)ws.mslot, targets, (const int32_t*)ws.plan, ws, env.clock,
                      cap);
-  hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, cap);
   timing_end(env, 2);
   return check_launch("resample in place");
 }
