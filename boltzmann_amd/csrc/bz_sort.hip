@@ -1582,16 +1582,27 @@ __global__ __launch_bounds__(256) void k_ip_wcount(const K* __restrict__ keys,
   if (lane == 0) wcnt[(int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)] = c;
 }
 
-// exclusive scan of the kIpWaves wave counts in place (one block)
+// exclusive scan of the kIpWaves wave counts in place (one block, eight
+// consecutive counts per thread)
+static_assert(kIpWaves == 8 * 1024, "k_ip_wscan: eight wave counts per thread");
 __global__ __launch_bounds__(1024) void k_ip_wscan(uint32_t* wcnt) {
   __shared__ long long s[1][16];
-  long long carry = 0;
-  for (int base = 0; base < kIpWaves; base += 1024) {
-    long long v[1] = {(long long)wcnt[base + threadIdx.x]}, tot[1];
-    block_scan1024<1>(v, tot, s);
-    wcnt[base + threadIdx.x] = (uint32_t)(carry + v[0]);
-    carry += tot[0];
+  uint4* p = (uint4*)wcnt + 2 * threadIdx.x;
+  const uint4 lo = p[0], hi = p[1];
+  uint32_t v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  long long x[1] = {0}, tot[1];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) x[0] += v[q];
+  block_scan1024<1>(x, tot, s);
+  uint32_t run = (uint32_t)x[0];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const uint32_t t = v[q];
+    v[q] = run;
+    run += t;
   }
+  p[0] = make_uint4(v[0], v[1], v[2], v[3]);
+  p[1] = make_uint4(v[4], v[5], v[6], v[7]);
 }
 
 template <typename K>
@@ -1632,31 +1643,23 @@ __global__ __launch_bounds__(256) void k_ip_wscatter(const K* __restrict__ keys,
 
 // Whether a split candidate has a heavy particle (w > min_wt) among its
 // grouped members (1) or none (0): the plan only asks whether the split
-// draws, so the walk over a segment stops at the first 256 members that hold
-// one.  A grid sized to what is resident walks the keys.
+// draws, so the walk over a segment stops at the first 64 members that hold
+// one.  One wave per key; the waves of other keys leave at once.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ip_hw(ResArgs a, const int32_t* cand,
                                                const int32_t* mcnt, const int32_t* moff,
                                                const uint32_t* mslot, int32_t* plan) {
-  __shared__ int s_any;
-  for (int k = blockIdx.x; k < a.n_keys; k += gridDim.x) {
-    if (cand[k] != 1) continue;  // block-uniform
-    __syncthreads();
-    if (threadIdx.x == 0) s_any = 0;
-    __syncthreads();
-    const int c = mcnt[k];
-    for (int base = 0; base < c; base += 256) {
-      const int j = base + (int)threadIdx.x;
-      const bool h = j < c && wt<T>(a, mslot[moff[k] + j]) > a.pol.min_wt;
-      const bool wave_has = __any(h);  // every lane of the wave votes
-      if ((threadIdx.x & 63) == 0 && wave_has) s_any = 1;
-      __syncthreads();
-      const int any = s_any;
-      __syncthreads();  // every wave has read the flag before the next round writes it
-      if (any) break;   // block-uniform
-    }
-    if (threadIdx.x == 0) plan[4 * k + 3] = s_any;
+  const int k = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= a.n_keys || cand[k] != 1) return;  // wave-uniform
+  const int c = mcnt[k];
+  const int64_t off = moff[k];
+  bool any = false;
+  for (int base = 0; base < c && !any; base += 64) {  // wave-uniform
+    const int j = base + lane;
+    any = __any(j < c && wt<T>(a, mslot[off + j]) > a.pol.min_wt);
   }
+  if (lane == 0) plan[4 * k + 3] = any ? 1 : 0;
 }
 
 // Final kinds and appended counts, one key per thread, with chunk-local
@@ -1897,8 +1900,7 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
   if (rc) return bail(rc);
   rc = sort_members(env, ws, n_keys);
   if (rc) return bail(rc);
-  const int sgrid = n_keys < 2 * env.num_cu ? n_keys : 2 * env.num_cu;
-  hipLaunchKernelGGL(k_ip_hw<T>, dim3(sgrid), dim3(256), 0, env.stream, a,
+  hipLaunchKernelGGL(k_ip_hw<T>, dim3((n_keys + 3) / 4), dim3(256), 0, env.stream, a,
                      (const int32_t*)ws.ip_cand, (const int32_t*)ws.ip_mcnt,
                      (const int32_t*)ws.ip_moff, (const uint32_t*)ws.mslot, ws.plan);
   hipLaunchKernelGGL(k_ip_plan_local, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
