@@ -1512,19 +1512,6 @@ __global__ __launch_bounds__(64) void k_ip_cand_carry(ResArgs a, int nch, int32_
   mclk->n_next = mclk->n;
 }
 
-// Member offsets and counts of every key (zero for keys that are not
-// candidates, or when the pass does not fire).
-__global__ __launch_bounds__(kKeyChunk) void k_ip_cand_apply(ResArgs a, const int32_t* counts,
-                                                             const int32_t* cand,
-                                                             const long long* chunk,
-                                                             int32_t* moff, int32_t* mcnt) {
-  const int k = blockIdx.x * kKeyChunk + threadIdx.x;
-  if (k >= a.n_keys) return;
-  const int fire = cand[a.n_keys];
-  moff[k] = (int32_t)(chunk[8 * blockIdx.x + 4] + moff[k]);
-  mcnt[k] = fire && cand[k] ? counts[k] : 0;
-}
-
 // Extraction of the members of candidate segments, stable: global wave g
 // owns a contiguous run of slots (whole 512-slot rows of 8 per lane);
 // k_ip_wcount counts its members, k_ip_wscan turns the counts into
@@ -1641,19 +1628,27 @@ __global__ __launch_bounds__(256) void k_ip_wscatter(const K* __restrict__ keys,
   }
 }
 
-// Whether a split candidate has a heavy particle (w > min_wt) among its
-// grouped members (1) or none (0): the plan only asks whether the split
-// draws, so the walk over a segment stops at the first 64 members that hold
-// one.  One wave per key; the waves of other keys leave at once.
+// Member offsets and counts of every key (zero for keys that are not
+// candidates, or when the pass does not fire), then whether a split
+// candidate has a heavy particle (w > min_wt) among its grouped members (1)
+// or none (0): the plan only asks whether the split draws, so the walk over
+// a segment stops at the first 64 members that hold one.  One wave per key.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ip_hw(ResArgs a, const int32_t* cand,
-                                               const int32_t* mcnt, const int32_t* moff,
+__global__ __launch_bounds__(256) void k_ip_hw(ResArgs a, const int32_t* counts,
+                                               const int32_t* cand, const long long* chunk,
+                                               int32_t* moff, int32_t* mcnt,
                                                const uint32_t* mslot, int32_t* plan) {
   const int k = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (k >= a.n_keys || cand[k] != 1) return;  // wave-uniform
-  const int c = mcnt[k];
-  const int64_t off = moff[k];
+  if (k >= a.n_keys) return;  // wave-uniform
+  const int kc = cand[k];
+  const int c = cand[a.n_keys] && kc ? counts[k] : 0;
+  const int64_t off = chunk[8 * (k / kKeyChunk) + 4] + moff[k];
+  if (lane == 0) {
+    moff[k] = (int32_t)off;
+    mcnt[k] = c;
+  }
+  if (kc != 1) return;  // wave-uniform
   bool any = false;
   for (int base = 0; base < c && !any; base += 64) {  // wave-uniform
     const int j = base + lane;
@@ -1883,9 +1878,6 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
                      ws.ip_chunk, ws.ip_bits);
   hipLaunchKernelGGL(k_ip_cand_carry, dim3(1), dim3(64), 0, env.stream, a, nch, ws.ip_cand,
                      ws.ip_chunk, ws.ip_mclk);
-  hipLaunchKernelGGL(k_ip_cand_apply, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
-                     (const int32_t*)ws.ip_hcnt, (const int32_t*)ws.ip_cand,
-                     (const long long*)ws.ip_chunk, ws.ip_moff, ws.ip_mcnt);
   // members of the candidate segments, in slot order, then grouped by key
   const size_t bits_lds = 4 * ((size_t)n_keys / 32 + 1);
   hipLaunchKernelGGL(k_ip_wcount<K>, dim3(kIpBlocks), dim3(256), bits_lds, env.stream, keys,
@@ -1901,8 +1893,9 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
   rc = sort_members(env, ws, n_keys);
   if (rc) return bail(rc);
   hipLaunchKernelGGL(k_ip_hw<T>, dim3((n_keys + 3) / 4), dim3(256), 0, env.stream, a,
-                     (const int32_t*)ws.ip_cand, (const int32_t*)ws.ip_mcnt,
-                     (const int32_t*)ws.ip_moff, (const uint32_t*)ws.mslot, ws.plan);
+                     (const int32_t*)ws.ip_hcnt, (const int32_t*)ws.ip_cand,
+                     (const long long*)ws.ip_chunk, ws.ip_moff, ws.ip_mcnt,
+                     (const uint32_t*)ws.mslot, ws.plan);
   hipLaunchKernelGGL(k_ip_plan_local, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
                      (const int32_t*)ws.ip_hcnt, targets, (const int32_t*)ws.ip_cand, ws.plan,
                      ws.ip_tmp, ws.ip_chunk, ws.kstride);
