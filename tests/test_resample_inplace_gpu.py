@@ -193,6 +193,40 @@ def test_resample_many_keys_and_large_segments(gpu, inplace):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", [8, 4], ids=["f64", "f32"])
+def test_resample_inplace_keys_beyond_the_lds_histogram(gpu, prec):
+    """8200 cells x 5 bins = 41005 keys: more than the key histogram holds in
+    LDS (40960 counters), so the slots are counted with global atomics; 41
+    key chunks in the plan scans.  Slot count not a multiple of the 16-byte
+    key vectors, holes in the input.  Counts, selections and every slot
+    against the oracle."""
+    from boltzmann_amd import physics as ph
+    N, nb, target = 8200, 5, 40
+    rng = np.random.default_rng(41)
+    cells = [int(v) for v in rng.choice([0, 5, 15, 40, 120], N + 1)]
+    n = int(sum(cells))
+    if n % 8 == 0:
+        cells[3] += 1
+        n += 1
+    cfg, eng = _engine(N, prec, nb, target, n, trigger_at=0.0, merge_at=2.0)
+    assert (N + 1) * nb > 40960
+    st = _state(cells, N, 2.5, cfg.np_dtype, seed=42, e_ev=9.0)
+    w = st[4].copy()
+    w[::29] = 0
+    st = st[:4] + (w,)
+    eng.load_state(st, step=7)
+    eng.resample()
+    got = eng.fetch_slots()
+    cnt = eng.counters()
+    s = O.Setup(dim=1, dt=cfg.dt, n_cells=N, length_cm=2.5)
+    want, wc = O.resample_inplace(s, _policy(cfg), nb, ph.bin_thresholds(nb), eng.targets_host,
+                                  cfg.seed, 7, st, capacity=eng.capacity)
+    assert wc[8] > 0 and wc[9] > 0
+    assert cnt["split"] == wc[8] and cnt["merge"] == wc[9] and cnt["resample"] == 1
+    _compare_slots(got, want, prec)
+    eng.close()
+
+
 def _members(st, dx, nb, thr, key, n_nodes):
     x, vx, vy, vz, w = [a.astype(np.float64) for a in st]
     cell = np.clip(((x + 0.5 * dx) / dx).astype(int), 0, n_nodes - 1)
