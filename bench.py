@@ -177,6 +177,17 @@ def measured_traffic(workload: str, kernel: str, psteps_per_launch: float):
     return per * psteps_per_launch
 
 
+def measured_launch_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of a kernel whose work has no per-unit model
+    (profiles/traffic.json, fetch and write per launch), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)[workload][kernel]
+        return rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
 def copy_probe(mib: int) -> None:
     """Three device-to-device copies of `mib` MiB: a kernel of known traffic."""
     import torch
@@ -199,7 +210,8 @@ def _roofline(bytes_per_unit, units_per_launch, avg_s, kernel, workload=None, un
 
 
 # timed kinds of the library's event timing (bz_timing_get_n)
-T_PUSH, T_SORT, T_RESAMPLE, T_COPY, T_DOWN = range(5)
+T_PUSH, T_SORT, T_RESAMPLE, T_COPY, T_DOWN, T_SEG, T_HIST, T_FILL = range(8)
+N_TIMED_KINDS = 8
 
 
 def bench_engine(args):
@@ -230,11 +242,15 @@ def bench_engine(args):
     n_timed = max(10, min(50, args.steps // 4))
     ct0 = eng.counters()
     _lib.check(eng.lib.bz_timing_enable(eng.ctx, 1))
-    index = eng.run(index, n_timed, graphs=False)
+    # step by step: the slots each step starts from (the key histogram's units)
+    slots = 0
+    for _ in range(n_timed):
+        slots += eng.count()
+        index = eng.run(index, 1, graphs=False)
     torch.cuda.synchronize()
-    ms = (C.c_double * 5)()
-    nl = (C.c_int64 * 5)()
-    _lib.check(eng.lib.bz_timing_get_n(eng.ctx, 5, ms, nl))
+    ms = (C.c_double * N_TIMED_KINDS)()
+    nl = (C.c_int64 * N_TIMED_KINDS)()
+    _lib.check(eng.lib.bz_timing_get_n(eng.ctx, N_TIMED_KINDS, ms, nl))
     _lib.check(eng.lib.bz_timing_enable(eng.ctx, 0))
     ct1 = eng.counters()
     # every block the timed run replays is captured before the clock starts
@@ -277,9 +293,30 @@ def bench_engine(args):
         per = (12 + 16 * (passes - 1)) / passes
         kernels["downsweep"] = _roofline(per, dct["sorted"] / max(nl[T_SORT], 1), avg(T_DOWN),
                                          "k_downsweep", cfg.name, "sorted slot per pass")
-    # the roofline line names the kernel with the most time per step
-    share = {"push": ms[T_PUSH], "res_copy": ms[T_COPY], "downsweep": ms[T_DOWN]}
-    dom = max((k for k in kernels), key=lambda k: share[k])
+    if nl[T_HIST]:
+        # in-place resampling's key histogram: each slot's key read once
+        kernels["key_histogram"] = _roofline(2 if eng.push_keys16 else 4,
+                                             slots / n_timed, avg(T_HIST), "k_ip_hist",
+                                             None, "slot")
+        kernels["key_histogram"]["traffic"] = measured_launch_traffic(cfg.name, "k_ip_hist")
+    # merges and splits (member gathers) and the hole filling's moves (scattered
+    # writes) have no per-unit byte model: their time per launch and the HBM
+    # bytes rocprofv3 counted for a launch of the same workload
+    for key, kind, kname, bound in (("segments", T_SEG, "k_res_segments", "gather"),
+                                    ("hole_fill_moves", T_FILL, "k_fill_move", "scatter")):
+        if not nl[kind]:
+            continue
+        tr = measured_launch_traffic(cfg.name, kname)
+        kernels[key] = {"bound": bound, "kernel": kname, "kernel_avg_us": avg(kind) * 1e6,
+                        "launches": int(nl[kind]), "traffic": tr,
+                        "achieved_measured": tr / avg(kind) / 1e9 if tr else None,
+                        "frac_measured": tr / avg(kind) / 1e9 / HBM_PEAK_GBS if tr else None,
+                        "unit": "GB/s", "peak": HBM_PEAK_GBS}
+    # the roofline line names the kernel with the most time per step among
+    # those with an algorithmic byte count
+    share = {"push": ms[T_PUSH], "res_copy": ms[T_COPY], "downsweep": ms[T_DOWN],
+             "key_histogram": ms[T_HIST]}
+    dom = max((k for k in kernels if k in share), key=lambda k: share[k])
     out = {
         "metric": "particle-steps/s on 1 MI355X",
         "value": psteps / elapsed,
@@ -305,6 +342,8 @@ def bench_engine(args):
                                                 "sorted")},
         "kernel_ms": {"push": ms[T_PUSH], "sort": ms[T_SORT], "resample": ms[T_RESAMPLE],
                       "res_copy": ms[T_COPY], "downsweep": ms[T_DOWN],
+                      "segments": ms[T_SEG], "key_histogram": ms[T_HIST],
+                      "hole_fill_moves": ms[T_FILL],
                       "launches": [int(v) for v in nl], "over_steps": n_timed},
         "autotune": tuned,
     }

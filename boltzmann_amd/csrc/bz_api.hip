@@ -271,7 +271,7 @@ int bz_timing_enable(bz_ctx_t c, int on) {
     Timing* t = new (std::nothrow) Timing();
     if (!t) return fail("out of host memory");
     memset(t, 0, sizeof(*t));
-    t->cap = 4096;
+    t->cap = 2048;
     for (int k = 0; k < kTimedKinds; ++k) {
       t->start[k] = new hipEvent_t[t->cap];
       t->stop[k] = new hipEvent_t[t->cap];

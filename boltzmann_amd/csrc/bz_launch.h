@@ -60,8 +60,10 @@ struct LaunchEnv {
 };
 
 // 0 push, 1 sort, 2 resample (whole stages); 3 the resampling copy and
-// 4 each radix downsweep (single kernels inside stages 2 and 1)
-constexpr int kTimedKinds = 5;
+// 4 each radix downsweep (single kernels inside stages 2 and 1); single
+// kernels of the in-place resampling: 5 merges and splits (k_res_segments),
+// 6 the key histogram (k_ip_hist); 7 the hole filling's moves (k_fill_move)
+constexpr int kTimedKinds = 8;
 struct Timing {
   int on;
   int cap;

@@ -1870,8 +1870,10 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
     hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, env.stream, ws.ip_hcnt, (int64_t)m);
     return rc;
   };
+  timing_begin(env, 6);
   hipLaunchKernelGGL(k_ip_hist<K>, dim3(env.num_cu), dim3(kIpHistBlock), (size_t)lds_keys * 4,
                      env.stream, keys, (const Clock*)env.clock, ws.ip_hcnt, lds_keys);
+  timing_end(env, 6);
   // candidates, trigger and member offsets (one key per thread, 1024-key chunks)
   hipLaunchKernelGGL(k_ip_cand, dim3(nch), dim3(kKeyChunk), 0, env.stream, a,
                      (const int32_t*)ws.ip_hcnt, targets, nb, ws.ip_cand, ws.ip_moff,
@@ -1907,10 +1909,12 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
   // merges and splits together: every segment of a pass resident at once
   // where the list allows (at most 6 blocks per CU; 4 fit)
   const int ggrid = n_keys < 6 * env.num_cu ? n_keys : 6 * env.num_cu;
+  timing_begin(env, 5);
   hipLaunchKernelGGL(k_res_segments<T>, dim3(ggrid), dim3(kSegBlock), kSegLdsBytes, env.stream, a,
                      (const int32_t*)ws.ip_mcnt, (const int32_t*)ws.ip_moff,
                      (const uint32_t*)ws.mslot, targets, (const int32_t*)ws.plan, ws, env.clock,
                      cap);
+  timing_end(env, 5);
   timing_end(env, 2);
   return check_launch("resample in place");
 }
@@ -2109,9 +2113,11 @@ static int launch_fill_holes_t(const LaunchEnv& env, const bz_state_t* s) {
   hipLaunchKernelGGL(k_fill_rank<T>, dim3(grid), dim3(kCompBlock), 0, env.stream,
                      (const T*)s->w, (const Clock*)env.clock, (const int64_t*)off, ws.mkeys,
                      ws.mslot);
+  timing_begin(env, 7);
   hipLaunchKernelGGL(k_fill_move<T>, dim3(env.num_cu * 4), dim3(256), 0, env.stream, *s,
                      (const int64_t*)off, grid, (const uint32_t*)ws.mkeys,
                      (const uint32_t*)ws.mslot);
+  timing_end(env, 7);
   hipLaunchKernelGGL(k_res_commit, dim3(1), dim3(1), 0, env.stream, env.clock, env.capacity);
   return check_launch("hole filling");
 }

@@ -259,9 +259,12 @@ int bz_clock_advance(bz_ctx_t ctx, double dt);
  * [0] push, [1] sort, [2] resample.  No counterpart in the reference. */
 int bz_timing_enable(bz_ctx_t ctx, int on);
 int bz_timing_get(bz_ctx_t ctx, double* ms3, int64_t* launches3);
-/* The same for the first n (<= 5) timed kinds: [0] push, [1] sort,
+/* The same for the first n (<= 8) timed kinds: [0] push, [1] sort,
  * [2] resample, [3] the resampling copy kernel alone (k_res_copy), [4] each
- * radix downsweep pass (k_downsweep).  No counterpart in the reference. */
+ * radix downsweep pass (k_downsweep), and single kernels of the in-place
+ * resampling: [5] its merges and splits (k_res_segments), [6] its key
+ * histogram (k_ip_hist), [7] the moves of the hole filling (k_fill_move).
+ * No counterpart in the reference. */
 int bz_timing_get_n(bz_ctx_t ctx, int n, double* ms, int64_t* launches);
 
 /* Fused electron step: MCC collision (elastic / excitation / ionization with
