@@ -1003,8 +1003,11 @@ __device__ __forceinline__ double block_max_d(double v, double* s_red) {
 
 // Inclusive fixed-point prefix of q over sorted positions [s0, s0+c) into cdf.
 // Returns the total.  q(j) is supplied by the functor; s_w64 holds kSegWaves.
+// With samp, the last entry of every group of S positions also goes to
+// samp[group] (the LDS index of a CDF kept in global memory).
 template <typename Q>
-__device__ int64_t block_fixed_scan(int64_t s0, int c, int64_t* cdf, Q q, int64_t* s_w64) {
+__device__ int64_t block_fixed_scan(int64_t s0, int c, int64_t* cdf, Q q, int64_t* s_w64,
+                                   int64_t* samp = nullptr, int S = 1) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int64_t carry = 0;
   for (int base = 0; base < c; base += kSegBlock) {
@@ -1016,7 +1019,10 @@ __device__ int64_t block_fixed_scan(int64_t s0, int c, int64_t* cdf, Q q, int64_
     __syncthreads();
     int64_t before = carry;
     for (int w = 0; w < wid; ++w) before += s_w64[w];
-    if (j < c) cdf[s0 + j] = before + inc;
+    if (j < c) {
+      cdf[s0 + j] = before + inc;
+      if (samp && ((j + 1) % S == 0 || j == c - 1)) samp[j / S] = before + inc;
+    }
     for (int w = 0; w < kSegWaves; ++w) carry += s_w64[w];
   }
   __syncthreads();
@@ -1035,6 +1041,25 @@ __device__ __forceinline__ int search_cdf(const int64_t* cdf, int64_t s0, int c,
     else lo = mid + 1;
   }
   return lo < c ? lo : c - 1;
+}
+
+// The same search (left) over a CDF in global memory with the last entry of
+// every group of S in LDS (samp): the group from LDS, then at most S - 1
+// global reads instead of a binary search's log2(c) dependent ones.
+__device__ __forceinline__ int search_cdf_sampled(const int64_t* cdf, const int64_t* samp, int c,
+                                                  int S, int64_t thr) {
+  const int ng = (c + S - 1) / S;
+  int lo = 0, hi = ng;  // first group whose last entry is >= thr
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (samp[mid] >= thr) hi = mid;
+    else lo = mid + 1;
+  }
+  if (lo >= ng) return c - 1;
+  const int j0 = lo * S, j1 = min(j0 + S, c);
+  for (int j = j0; j < j1 - 1; ++j)
+    if (cdf[j] >= thr) return j;
+  return j1 - 1;
 }
 
 __device__ __forceinline__ double res_uniform(uint64_t seed, uint64_t step, uint32_t key,
@@ -1203,11 +1228,13 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
   frexp(wmax, &ew);
   const int S = 61 - ew - ceil_log2_i64(ce);
   auto q = [&](int j) -> int64_t { return (int64_t)ldexp((double)Wt[perm[s0 + j]], S); };
-  // the CDF in LDS when it fits (the selection's binary searches read it)
+  // the CDF in LDS when it fits (the selection's binary searches read it);
+  // else in the workspace, with every S-th entry in LDS
   extern __shared__ int64_t s_mcdf[];
   const bool lds = ce <= lds_cap;
   int64_t* cdf = lds ? s_mcdf : ws.cdf + s0;
-  const int64_t Wq = block_fixed_scan(0, ce, cdf, q, s_w64);
+  const int G = lds ? 1 : (ce + lds_cap - 1) / lds_cap;  // entries per LDS sample
+  const int64_t Wq = block_fixed_scan(0, ce, cdf, q, s_w64, lds ? nullptr : s_mcdf, G);
   const double wbar = W / (double)K;
   // the merged weight may exceed every weight so far: raise the bound the
   // deposit scales its fixed-point sums by
@@ -1219,7 +1246,8 @@ __device__ __forceinline__ void res_merge_key(int k, const ResArgs& a, const int
     const double u = seg_uniform(a, step, (uint32_t)k, (uint32_t)m);
     const double r = (1.0 / (double)K) * (u + (double)m);
     const int64_t thr = (int64_t)(r * (double)Wq);
-    const int j = search_cdf(cdf, 0, ce, thr, false);
+    const int j = lds ? search_cdf(cdf, 0, ce, thr, false)
+                      : search_cdf_sampled(cdf, s_mcdf, ce, G, thr);
     if (out0 + m >= cap) continue;  // overflow: counted at commit
     ws.sel[out0 + m] = (uint32_t)(s0 + j);
     const uint32_t i = perm[s0 + j];
