@@ -88,6 +88,64 @@ def _one_core(setup, state, E, table, seed, seconds):
     return {"value": rate, "cores": 1, "sample": f"{m} particles x {done} steps, 1 thread"}
 
 
+def _full_step(cfg, seconds, threads, E):
+    """The oracle's particle stages of a whole 1-D step on a bounded sample
+    with the workload's particles per cell (up to 1M particles, the cells cut
+    in proportion): in-place resampling (key histogram, trigger, splits and
+    merges) and push + collisions + deposit + compaction, for about
+    `seconds`.  The field step, O(cells), is left out."""
+    import dataclasses
+
+    import numpy as np
+
+    from boltzmann_amd import _lib
+    from boltzmann_amd import physics as ph
+    from oracle import oracle as O
+
+    n = min(cfg.n_particles, 1_000_000)
+    cells = max(8, int(round(cfg.n_cells * n / cfg.n_particles)))
+    sub = cfg.with_(n_particles=n, n_cells=cells, precision=_lib.BZ_F64)
+    cur = ph.initial_state(sub)
+    table = ph.build_table(cfg.xsec, cfg.gas_density_m3, sub.np_dtype) \
+        if cfg.xs_mode != _lib.BZ_XS_FIT else None
+    tab = None if table is None else (table.energy, table.sig, table.slope)
+    setup = O.Setup(dim=1, dt=cfg.dt, xs_mode=cfg.xs_mode, n_cells=cells,
+                    length_cm=cfg.length_cm, bnf=cfg.bnf, gas_density_m3=cfg.gas_density_m3,
+                    nu_max=table.nu_max if table is not None else 0.0, nthreads=threads,
+                    push_order=cfg.push_order)
+    nb = max(cfg.nb, 1)
+    fr = cfg.bin_fractions if len(cfg.bin_fractions) == nb else ph.bin_fractions(nb)
+    thr = cfg.bin_thresholds if len(cfg.bin_thresholds) == nb - 1 else ph.bin_thresholds(nb)
+    targets = np.array([int(f * cfg.target_ppc) for f in fr] * (cells + 1), dtype=np.int32)
+    pol = O.policy(split_below=cfg.split_below, merge_at=cfg.merge_at,
+                   merge_keep=cfg.merge_keep, trigger_at=cfg.trigger_at, min_wt=cfg.min_wt,
+                   split_to=cfg.split_to)
+    # the GPU run's field averaged onto the sample's cells (zero without one)
+    Es = np.zeros(cells)
+    if E is not None and np.any(E):
+        xc = (np.arange(cells) + 0.5) / cells
+        xg = (np.arange(len(E)) + 0.5) / len(E)
+        Es = np.interp(xc, xg, np.asarray(E, dtype=np.float64))
+    done = psteps = 0
+    t0 = time.perf_counter()
+    while True:
+        s = dataclasses.replace(setup, seed=cfg.seed, step=done)
+        if cfg.resample_every and done % cfg.resample_every == 0:
+            cur, _ = O.resample_inplace(s, pol, nb, thr, targets, cfg.seed, done, cur,
+                                        capacity=3 * cur[0].size + 1024)
+        psteps += int(np.count_nonzero(cur[4] > 0))
+        cur, _, _ = O.push_step(s, cur, Es, None, table=tab, capacity=2 * cur[0].size + 1024)
+        done += 1
+        if time.perf_counter() - t0 > seconds or done >= 1000:
+            break
+    rate = psteps / (time.perf_counter() - t0)
+    return {"value": rate, "unit": "particle-steps/s", "cores": threads,
+            "sample": f"{n} particles on {cells} cells ({cfg.n_particles // cfg.n_cells} per "
+                      f"cell as the workload) x {done} steps of the oracle's in-place "
+                      f"resampling (one thread) then push+MCC+deposit+compaction (OpenMP "
+                      f"{threads} threads), FP64; the O(cells) field step left out"}
+
+
 def cpu_baseline(cfg, seconds: float = 10.0, E=None):
     """Oracle (C++ restatement of the reference step, OpenMP over particles)
     on a bounded sample of the same workload: push + collisions + deposit +
@@ -120,12 +178,15 @@ def cpu_baseline(cfg, seconds: float = 10.0, E=None):
     else:
         E, field = None, "the uniform 0-D field"
     rate, done = _time_oracle(setup, st, E, n, tab, cfg.seed, seconds, 2 * n)
-    return {"value": rate, "unit": "particle-steps/s", "cores": cores["threads"], "kind": "port",
-            "host": cores,
-            "sample": f"{n} particles x {done} steps of the oracle push+MCC+deposit+compaction "
-                      f"in {field} (FP64 like the reference, Philox uniforms drawn in the "
-                      f"loop, OpenMP {cores['threads']} threads)",
-            "one_core": _one_core(setup, st, E, tab, cfg.seed, seconds / 4)}
+    out = {"value": rate, "unit": "particle-steps/s", "cores": cores["threads"], "kind": "port",
+           "host": cores,
+           "sample": f"{n} particles x {done} steps of the oracle push+MCC+deposit+compaction "
+                     f"in {field} (FP64 like the reference, Philox uniforms drawn in the "
+                     f"loop, OpenMP {cores['threads']} threads)",
+           "one_core": _one_core(setup, st, E, tab, cfg.seed, seconds / 4)}
+    if cfg.dim == 1 and cfg.target_ppc > 0:
+        out["full_step"] = _full_step(cfg, seconds, cores["threads"], E)
+    return out
 
 
 def cpu_baseline_swarms(scfg, seconds: float = 10.0):
