@@ -1331,9 +1331,10 @@ __global__ __launch_bounds__(kSegBlock) void k_res_merge(ResArgs a, const int32_
 
 // The in-place pass runs its merges and splits in one launch, so a pass
 // takes about the longer of the two (the merges: gathers of their members)
-// rather than their sum.  Blocks of 39 KiB of LDS, four to a CU (the grid
-// holds every segment of a pass at once on C4); larger segments keep their
-// CDF in the workspace.  Merges first in the walk.
+// rather than their sum.  Blocks of 39 KiB of LDS (four would fit a CU;
+// the merge's 117 VGPRs leave room for two 512-thread blocks, and forcing
+// more spills registers); larger segments keep their CDF in the workspace.
+// Merges first in the walk.
 constexpr int kSegSplitLds = 2432;  // x 16 B
 constexpr int kSegMergeLds = 4864;  // x 8 B
 constexpr size_t kSegLdsBytes = (size_t)kSegSplitLds * 16;
@@ -1935,7 +1936,7 @@ static int launch_resample_inplace_t(const LaunchEnv& env, const bz_state_t* s,
                      (const int32_t*)ws.ip_tmp, (const long long*)ws.ip_chunk, ws.klist,
                      ws.kstride, ws.ip_hcnt);
   // merges and splits together: every segment of a pass resident at once
-  // where the list allows (at most 6 blocks per CU; 4 fit)
+  // where the list allows (at most 6 blocks per CU; 2 are resident at once)
   const int ggrid = n_keys < 6 * env.num_cu ? n_keys : 6 * env.num_cu;
   timing_begin(env, 5);
   hipLaunchKernelGGL(k_res_segments<T>, dim3(ggrid), dim3(kSegBlock), kSegLdsBytes, env.stream, a,
